@@ -1,0 +1,200 @@
+/*
+ * CPU oracle for the leave-out (KSS) CMG-PCG path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * Rust engine (johannes-schmieder/fevc, rust/crates/vckss-core and the
+ * vendored CMG crate rust/vendor/cmg).  It is used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker
+ * and as the reported CPU baseline.  The product library never links it.
+ *
+ * Every function names the reference file:line it restates.  Parity pins:
+ *   - RNG: permanent vectors of rng.rs:366-419 (bit-exact).
+ *   - CMG graph, hierarchy, LDL, cycle, PCG: FNV-1a digests of
+ *     rust/vendor/cmg/tests/fixtures/frozen_bits.txt [default] (bit-exact).
+ *   - Engine end to end: rust_mata_shared_atoms.rs known answers (3e-12).
+ *   - Hybrid graph: cmg_impl.rs.in:749-835 weights and the dense Schur
+ *     closed form.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error transport (error.rs:8-47 codes) ---------------------------- */
+enum {
+    ORC_OK = 0,
+    ORC_USER_BREAK = 1,
+    ORC_UNSUPPORTED = 11,
+    ORC_INVALID_INPUT = 20,
+    ORC_INVALID_IDENTIFIER = 21,
+    ORC_INVALID_WEIGHT = 22,
+    ORC_INVALID_TARGET_WEIGHT = 23,
+    ORC_GRAPH_EMPTY = 30,
+    ORC_GRAPH_UNIDENTIFIED = 31,
+    ORC_GRAPH_CERTIFICATE = 32,
+    ORC_RESOURCE_LIMIT = 40,
+    ORC_ALLOCATION = 41,
+    ORC_CMG_SETUP = 50,
+    ORC_CMG_APPLY = 51,
+    ORC_PCG_CURVATURE = 60,
+    ORC_PCG_PRECONDITIONER = 61,
+    ORC_PCG_MAX_ITERATIONS = 63,
+    ORC_FULL_RESIDUAL = 64,
+    ORC_RNG_CONTRACT = 70,
+    ORC_CORRECTION_NONFINITE = 71,
+    ORC_ACCOUNTING = 72,
+    ORC_JLA_MOMENT = 73,
+    ORC_JLA_CONSTRAINT = 74,
+    ORC_NONESTIMABLE = 75,
+    ORC_BLOCK_INVERSE = 76,
+    ORC_TARGET_CENTERING = 77,
+    ORC_TARGET_IDENTITY = 78,
+    ORC_NONFINITE_CORRECTED = 79,
+    ORC_INVARIANT = 90
+};
+
+const char *orc_last_error(void);
+
+/* ---- Counter-V1 RNG (rng.rs) ------------------------------------------ */
+#define ORC_TAG_LEVERAGE 0x4c45564552414745ULL
+#define ORC_TAG_TARGET 0x0054415247455401ULL
+
+void orc_philox_block(uint64_t seed, uint64_t tag, uint64_t probe_block,
+                      uint64_t entity, uint64_t word, uint64_t out[4]);
+uint64_t orc_rng_word(uint64_t seed, uint64_t tag, uint64_t probe,
+                      uint64_t entity, uint64_t word);
+int orc_rademacher_sum(uint64_t seed, uint64_t tag, uint64_t probe,
+                       uint64_t entity, uint64_t trials, int64_t *out);
+int orc_rademacher_fill(uint64_t seed, uint64_t tag, uint64_t first_probe,
+                        uint64_t columns, const uint64_t *entity,
+                        const uint64_t *trials, uint64_t n, int64_t *out);
+
+/* ---- sample pipeline (types.rs, problem.rs, graph.rs, jla.rs) ---------- */
+typedef struct orc_problem orc_problem;
+
+/* Canonicalize, select the match-deletion leave-out graph, compress and
+ * build the no-control JLA plan.  Returns a code; *out on success. */
+int orc_prepare(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                const uint64_t *deletion, const double *outcome,
+                const uint64_t *frequency, const double *target,
+                orc_problem **out);
+void orc_problem_free(orc_problem *p);
+/* dims[0..]: input_rows, retained_rows, retained_physical, workers, firms,
+ * cells, deletion_units, target_strata_exact, plan_target_strata */
+void orc_problem_dims(const orc_problem *p, uint64_t dims[9]);
+double orc_problem_target_total(const orc_problem *p);
+uint64_t orc_problem_topology_checksum(const orc_problem *p);
+void orc_problem_mask(const orc_problem *p, uint8_t *mask);
+/* which: 0 cell_worker(u32) 1 cell_firm(u32) 2 cell_weight(f64)
+ *        3 unit_cell(u32) 4 unit_count(u64) 5 unit_rank(u64)
+ *        6 stratum_cell(u32) 7 stratum_count(u64) 8 stratum_rank(u64)
+ *        9 stratum_per_copy(f64) 10 cell_outcome_sum(f64)
+ *        11 cell_target_sum(f64) 12 unit_outcome_sum(f64) */
+int orc_problem_array(const orc_problem *p, int which, void *dst);
+
+/* ---- CMG (rust/vendor/cmg) --------------------------------------------- */
+typedef struct orc_graph orc_graph;
+typedef struct orc_cmg orc_cmg;
+
+orc_graph *orc_graph_from_edges(uint64_t n, uint64_t m, const uint32_t *u,
+                                const uint32_t *v, const double *w);
+void orc_graph_free(orc_graph *g);
+uint64_t orc_graph_vertices(const orc_graph *g);
+uint64_t orc_graph_edges(const orc_graph *g);
+void orc_graph_export(const orc_graph *g, uint32_t *u, uint32_t *v, double *w,
+                      double *diag);
+void orc_graph_matvec(const orc_graph *g, const double *x, double *y);
+
+typedef struct {
+    uint64_t direct_threshold;      /* 700 */
+    uint64_t max_levels;            /* 32 (128 on the full-CMG route) */
+    double max_hierarchy_nnz_factor; /* 5.0 */
+    double low_degree_threshold;    /* 0.125 */
+} orc_cmg_options;
+
+int orc_cmg_build(const orc_graph *g, orc_cmg_options opt, orc_cmg **out);
+void orc_cmg_free(orc_cmg *c);
+uint64_t orc_cmg_levels(const orc_cmg *c);
+/* info: n, edges, nnz, repeat(effective), terminal_reason(+1, 0 none), coarse */
+void orc_cmg_level_info(const orc_cmg *c, uint64_t level, uint64_t info[6]);
+int orc_cmg_level_labels(const orc_cmg *c, uint64_t level, uint32_t *labels);
+const orc_graph *orc_cmg_level_graph(const orc_cmg *c, uint64_t level);
+int orc_cmg_apply(const orc_cmg *c, const double *rhs, double *out, int project);
+
+typedef struct {
+    double relative_tolerance;
+    double absolute_tolerance;
+    uint64_t max_iterations;
+    uint64_t recompute_interval;
+} orc_pcg_options;
+
+typedef struct {
+    uint64_t iterations, restarts;
+    double initial_residual_norm, residual_norm, relative_residual,
+        backward_error, tolerance, rhs_projection_norm;
+} orc_pcg_diag;
+
+int orc_pcg_solve(const orc_graph *g, const orc_cmg *c, const double *rhs,
+                  const double *guess, double *solution, orc_pcg_options opt,
+                  orc_pcg_diag *diag);
+
+/* frozen_bits.rs digests: family "chain", "chain_direct", "random", "grid";
+ * kind "graph_from_edges", "build", "pcg", "pcg_guess". */
+int orc_frozen_digest(const char *family, const char *kind, uint64_t *out);
+/* sign of Rust's empty f64 sum (-0.0 since Rust 1.83); 1 = negative zero */
+void orc_set_empty_sum_negative(int negative);
+
+/* ---- hybrid graph + engine (cmg_impl.rs.in, operator.rs, engine.rs) ---- */
+typedef struct {
+    uint64_t seed;
+    uint32_t probes;
+    uint32_t leverage_width;
+    uint32_t target_width;
+    uint32_t threads;
+    double fit_tolerance;      /* full-fit effective tolerance */
+    double probe_tolerance;    /* probe effective tolerance */
+    uint32_t max_iterations;
+    uint32_t recompute_interval;
+    double rank_tolerance;
+    double block_tolerance;
+} orc_engine_options;
+
+typedef struct {
+    double plugin[4], correction[4], corrected[4], mcse[4];
+    double weighted_rss;
+    uint64_t leverage_iterations, target_iterations, fit_iterations;
+    double max_complete_residual;
+    double solve_seconds;
+} orc_engine_summary;
+
+typedef struct orc_engine_result orc_engine_result;
+
+int orc_hybrid_graph(const orc_problem *p, orc_graph **out,
+                     uint64_t *auxiliary_count);
+int orc_engine_run(const orc_problem *p, orc_engine_options opt,
+                   orc_engine_result **out);
+void orc_engine_result_free(orc_engine_result *r);
+void orc_engine_result_summary(const orc_engine_result *r,
+                               orc_engine_summary *s);
+/* which: 0 fitted_cell 1 projection_share 2 residual_share 3 finite_bias
+ *        4 finite_variance 5 residual_mass 6 deleted_mass
+ *        7 cell_correction_weight 8 target_draws (4 per probe)
+ *        9 leverage_iterations(per probe, as double) */
+int orc_engine_result_array(const orc_engine_result *r, int which, double *dst,
+                            uint64_t capacity);
+
+/* Timed probe solves on an existing problem: solves `count` leverage probes
+ * (first probe `first`) with `threads` host threads; returns seconds. */
+int orc_time_probe_solves(const orc_problem *p, orc_engine_options opt,
+                          uint32_t first, uint32_t count, double *seconds,
+                          uint64_t *iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
