@@ -1,0 +1,352 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+The oracle is a plain-C restatement of the reference engine; see oracle/orc.h
+for the reference file:line each function follows.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB = None
+LEVERAGE_TAG = 0x4C45564552414745
+TARGET_TAG = 0x0054415247455401
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"oracle error {code}: {message}")
+        self.code = code
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _f64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class EngineOptions(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("probes", C.c_uint32),
+                ("leverage_width", C.c_uint32), ("target_width", C.c_uint32),
+                ("threads", C.c_uint32), ("fit_tolerance", C.c_double),
+                ("probe_tolerance", C.c_double), ("max_iterations", C.c_uint32),
+                ("recompute_interval", C.c_uint32), ("rank_tolerance", C.c_double),
+                ("block_tolerance", C.c_double)]
+
+
+class EngineSummary(C.Structure):
+    _fields_ = [("plugin", C.c_double * 4), ("correction", C.c_double * 4),
+                ("corrected", C.c_double * 4), ("mcse", C.c_double * 4),
+                ("weighted_rss", C.c_double), ("leverage_iterations", C.c_uint64),
+                ("target_iterations", C.c_uint64), ("fit_iterations", C.c_uint64),
+                ("max_complete_residual", C.c_double), ("solve_seconds", C.c_double)]
+
+
+class PcgOptions(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_double), ("absolute_tolerance", C.c_double),
+                ("max_iterations", C.c_uint64), ("recompute_interval", C.c_uint64)]
+
+
+class PcgDiag(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("restarts", C.c_uint64),
+                ("initial_residual_norm", C.c_double), ("residual_norm", C.c_double),
+                ("relative_residual", C.c_double), ("backward_error", C.c_double),
+                ("tolerance", C.c_double), ("rhs_projection_norm", C.c_double)]
+
+
+class CmgOptions(C.Structure):
+    _fields_ = [("direct_threshold", C.c_uint64), ("max_levels", C.c_uint64),
+                ("max_hierarchy_nnz_factor", C.c_double), ("low_degree_threshold", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liborc.so")
+        if not os.path.exists(path):
+            from . import build as _b  # type: ignore
+            _b.build()
+        L = C.CDLL(path)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_rng_word.restype = C.c_uint64
+        L.orc_graph_from_edges.restype = C.c_void_p
+        L.orc_graph_from_edges.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_graph_vertices.restype = C.c_uint64
+        L.orc_graph_vertices.argtypes = [C.c_void_p]
+        L.orc_graph_edges.restype = C.c_uint64
+        L.orc_graph_edges.argtypes = [C.c_void_p]
+        L.orc_graph_export.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.orc_graph_free.argtypes = [C.c_void_p]
+        L.orc_graph_matvec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_cmg_build.argtypes = [C.c_void_p, CmgOptions, C.POINTER(C.c_void_p)]
+        L.orc_cmg_free.argtypes = [C.c_void_p]
+        L.orc_cmg_levels.restype = C.c_uint64
+        L.orc_cmg_levels.argtypes = [C.c_void_p]
+        L.orc_cmg_level_info.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.orc_cmg_level_labels.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.orc_cmg_level_graph.restype = C.c_void_p
+        L.orc_cmg_level_graph.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_cmg_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_pcg_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, PcgOptions, C.POINTER(PcgDiag)]
+        L.orc_prepare.argtypes = [C.c_uint64] + [C.c_void_p] * 6 + [C.POINTER(C.c_void_p)]
+        L.orc_prepare_masked.argtypes = [C.c_uint64] + [C.c_void_p] * 7 + [C.POINTER(C.c_void_p)]
+        L.orc_problem_free.argtypes = [C.c_void_p]
+        L.orc_problem_dims.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_problem_target_total.restype = C.c_double
+        L.orc_problem_target_total.argtypes = [C.c_void_p]
+        L.orc_problem_topology_checksum.restype = C.c_uint64
+        L.orc_problem_topology_checksum.argtypes = [C.c_void_p]
+        L.orc_problem_mask.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_problem_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_hybrid_graph.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+        L.orc_engine_run.argtypes = [C.c_void_p, EngineOptions, C.POINTER(C.c_void_p)]
+        L.orc_engine_result_free.argtypes = [C.c_void_p]
+        L.orc_engine_result_summary.argtypes = [C.c_void_p, C.POINTER(EngineSummary)]
+        L.orc_engine_result_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]
+        L.orc_time_probe_solves.argtypes = [C.c_void_p, EngineOptions, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+        L.orc_frozen_digest.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)]
+        L.orc_philox_block.argtypes = [C.c_uint64] * 5 + [C.c_void_p]
+        L.orc_rng_word.argtypes = [C.c_uint64] * 5
+        L.orc_rademacher_sum.argtypes = [C.c_uint64] * 5 + [C.POINTER(C.c_int64)]
+        L.orc_rademacher_fill.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.orc_set_empty_sum_negative.argtypes = [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise OracleError(rc, lib().orc_last_error().decode())
+
+
+# ---------------- RNG ----------------
+
+def philox_block(seed, tag, block, entity, word):
+    out = np.zeros(4, dtype=np.uint64)
+    lib().orc_philox_block(seed, tag, block, entity, word, out.ctypes.data)
+    return [int(x) for x in out]
+
+
+def rng_word(seed, tag, probe, entity, word):
+    return int(lib().orc_rng_word(seed, tag, probe, entity, word))
+
+
+def rademacher_sum(seed, tag, probe, entity, trials):
+    out = C.c_int64()
+    _check(lib().orc_rademacher_sum(seed, tag, probe, entity, trials, C.byref(out)))
+    return out.value
+
+
+def rademacher_fill(seed, tag, first_probe, columns, entity, trials):
+    entity = np.ascontiguousarray(entity, dtype=np.uint64)
+    trials = np.ascontiguousarray(trials, dtype=np.uint64)
+    out = np.zeros(columns * len(entity), dtype=np.int64)
+    _check(lib().orc_rademacher_fill(seed, tag, first_probe, columns, entity.ctypes.data,
+                                     trials.ctypes.data, len(entity), out.ctypes.data))
+    return out.reshape(columns, len(entity))
+
+
+# ---------------- CMG ----------------
+
+def frozen_digest(family: str, kind: str) -> int:
+    out = C.c_uint64()
+    _check(lib().orc_frozen_digest(family.encode(), kind.encode(), C.byref(out)))
+    return out.value
+
+
+class Graph:
+    def __init__(self, n, u, v, w):
+        u = np.ascontiguousarray(u, dtype=np.uint32)
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._h = lib().orc_graph_from_edges(n, len(u), u.ctypes.data, v.ctypes.data, w.ctypes.data)
+        if not self._h:
+            raise OracleError(50, lib().orc_last_error().decode())
+        self._owned = True
+
+    @classmethod
+    def _borrow(cls, handle):
+        g = cls.__new__(cls)
+        g._h = handle
+        g._owned = False
+        return g
+
+    @property
+    def n(self):
+        return int(lib().orc_graph_vertices(self._h))
+
+    @property
+    def m(self):
+        return int(lib().orc_graph_edges(self._h))
+
+    def export(self):
+        m, n = self.m, self.n
+        u = np.zeros(m, np.uint32); v = np.zeros(m, np.uint32)
+        w = np.zeros(m, np.float64); d = np.zeros(n, np.float64)
+        lib().orc_graph_export(self._h, u.ctypes.data, v.ctypes.data, w.ctypes.data, d.ctypes.data)
+        return u, v, w, d
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        lib().orc_graph_matvec(self._h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            lib().orc_graph_free(self._h)
+            self._h = None
+
+
+class Cmg:
+    def __init__(self, graph: Graph, direct_threshold=700, max_levels=32):
+        self.graph = graph
+        h = C.c_void_p()
+        _check(lib().orc_cmg_build(graph._h, CmgOptions(direct_threshold, max_levels, 5.0, 0.125),
+                                   C.byref(h)))
+        self._h = h.value
+
+    def levels(self):
+        out = []
+        for i in range(int(lib().orc_cmg_levels(self._h))):
+            info = np.zeros(6, np.uint64)
+            lib().orc_cmg_level_info(self._h, i, info.ctypes.data)
+            g = Graph._borrow(lib().orc_cmg_level_graph(self._h, i))
+            labels = None
+            if info[5] or i + 1 < int(lib().orc_cmg_levels(self._h)):
+                labels = np.zeros(int(info[0]), np.uint32)
+                if lib().orc_cmg_level_labels(self._h, i, labels.ctypes.data) != 0:
+                    labels = None
+            u, v, w, d = g.export()
+            out.append(dict(n=int(info[0]), m=int(info[1]), nnz=int(info[2]),
+                            repeat=int(info[3]), terminal=int(info[4]), coarse=int(info[5]),
+                            u=u, v=v, w=w, diag=d, labels=labels))
+        return out
+
+    def apply(self, rhs, project=False):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.zeros(self.graph.n)
+        _check(lib().orc_cmg_apply(self._h, rhs.ctypes.data, out.ctypes.data, int(project)))
+        return out
+
+    def solve(self, rhs, rel=1e-8, abs_tol=0.0, max_iterations=1000, recompute=25, guess=None):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.graph.n)
+        d = PcgDiag()
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        _check(lib().orc_pcg_solve(self.graph._h, self._h, rhs.ctypes.data,
+                                   None if g is None else g.ctypes.data, x.ctypes.data,
+                                   PcgOptions(rel, abs_tol, max_iterations, recompute), C.byref(d)))
+        return x, d
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_cmg_free(self._h)
+            self._h = None
+
+
+# ---------------- sample + engine ----------------
+
+class Problem:
+    ARRAYS = {"cell_worker": (0, np.uint32, 5), "cell_firm": (1, np.uint32, 5),
+              "cell_weight": (2, np.float64, 5), "unit_cell": (3, np.uint32, 6),
+              "unit_count": (4, np.uint64, 6), "unit_rank": (5, np.uint64, 6),
+              "stratum_cell": (6, np.uint32, 8), "stratum_count": (7, np.uint64, 8),
+              "stratum_rank": (8, np.uint64, 8), "stratum_per_copy": (9, np.float64, 8),
+              "cell_outcome_sum": (10, np.float64, 5), "cell_target_sum": (11, np.float64, 5),
+              "unit_outcome_sum": (12, np.float64, 6)}
+
+    def __init__(self, worker, firm, deletion, outcome, frequency, target, mask=None):
+        cols = [np.ascontiguousarray(worker, np.uint64), np.ascontiguousarray(firm, np.uint64),
+                np.ascontiguousarray(deletion, np.uint64), np.ascontiguousarray(outcome, np.float64),
+                np.ascontiguousarray(frequency, np.uint64), np.ascontiguousarray(target, np.float64)]
+        h = C.c_void_p()
+        if mask is None:
+            _check(lib().orc_prepare(len(cols[0]), *[c.ctypes.data for c in cols], C.byref(h)))
+        else:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            _check(lib().orc_prepare_masked(len(cols[0]), *[c.ctypes.data for c in cols],
+                                            m.ctypes.data, C.byref(h)))
+        self._h = h.value
+        d = np.zeros(9, np.uint64)
+        lib().orc_problem_dims(self._h, d.ctypes.data)
+        (self.input_rows, self.rows, self.physical, self.workers, self.firms, self.cells,
+         self.units, self.exact_strata, self.strata) = [int(x) for x in d]
+        self.target_total = float(lib().orc_problem_target_total(self._h))
+        self.topology_checksum = int(lib().orc_problem_topology_checksum(self._h))
+
+    def mask(self):
+        m = np.zeros(self.input_rows, np.uint8)
+        lib().orc_problem_mask(self._h, m.ctypes.data)
+        return m.astype(bool)
+
+    def array(self, name):
+        which, dtype, dim = self.ARRAYS[name]
+        n = int([self.input_rows, self.rows, self.physical, self.workers, self.firms, self.cells,
+                 self.units, self.exact_strata, self.strata][dim])
+        out = np.zeros(n, dtype)
+        _check(lib().orc_problem_array(self._h, which, out.ctypes.data))
+        return out
+
+    def hybrid_graph(self):
+        h = C.c_void_p()
+        aux = C.c_uint64()
+        _check(lib().orc_hybrid_graph(self._h, C.byref(h), C.byref(aux)))
+        g = Graph._borrow(h.value)
+        g._owned = True
+        return g, int(aux.value)
+
+    def run(self, seed=8675309, probes=200, leverage_width=8, target_width=8, threads=1,
+            fit_tolerance=1e-10, probe_tolerance=1e-6, max_iterations=10000,
+            recompute_interval=100, rank_tolerance=1e-10, block_tolerance=1e-10):
+        opt = EngineOptions(seed, probes, leverage_width, target_width, threads, fit_tolerance,
+                            probe_tolerance, max_iterations, recompute_interval, rank_tolerance,
+                            block_tolerance)
+        h = C.c_void_p()
+        _check(lib().orc_engine_run(self._h, opt, C.byref(h)))
+        try:
+            s = EngineSummary()
+            lib().orc_engine_result_summary(h.value, C.byref(s))
+            res = dict(plugin=list(s.plugin), correction=list(s.correction),
+                       corrected=list(s.corrected), mcse=list(s.mcse),
+                       weighted_rss=s.weighted_rss, leverage_iterations=s.leverage_iterations,
+                       target_iterations=s.target_iterations, fit_iterations=s.fit_iterations,
+                       max_complete_residual=s.max_complete_residual,
+                       solve_seconds=s.solve_seconds)
+            names = ["fitted_cell", "projection_share", "residual_share", "finite_bias",
+                     "finite_variance", "residual_mass", "deleted_mass", "cell_correction_weight",
+                     "target_draws", "leverage_iterations_per_probe"]
+            sizes = [self.cells] + [self.units] * 6 + [self.cells, 4 * probes, probes]
+            for i, (name, n) in enumerate(zip(names, sizes)):
+                a = np.zeros(n)
+                _check(lib().orc_engine_result_array(h.value, i, a.ctypes.data, n))
+                res[name] = a.reshape(probes, 4) if name == "target_draws" else a
+            return res
+        finally:
+            lib().orc_engine_result_free(h.value)
+
+    def time_probe_solves(self, count, first=0, seed=8675309, threads=1, probe_tolerance=1e-8,
+                          max_iterations=10000, recompute_interval=100):
+        opt = EngineOptions(seed, 2, 1, 1, threads, probe_tolerance, probe_tolerance,
+                            max_iterations, recompute_interval, 1e-10, 1e-10)
+        sec = C.c_double()
+        it = C.c_uint64()
+        _check(lib().orc_time_probe_solves(self._h, opt, first, count, C.byref(sec), C.byref(it)))
+        return sec.value, int(it.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_problem_free(self._h)
+            self._h = None

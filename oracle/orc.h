@@ -83,6 +83,10 @@ int orc_prepare(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
                 const uint64_t *deletion, const double *outcome,
                 const uint64_t *frequency, const double *target,
                 orc_problem **out);
+int orc_prepare_masked(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                       const uint64_t *deletion, const double *outcome,
+                       const uint64_t *frequency, const double *target, const uint8_t *mask,
+                       orc_problem **out);
 void orc_problem_free(orc_problem *p);
 /* dims[0..]: input_rows, retained_rows, retained_physical, workers, firms,
  * cells, deletion_units, target_strata_exact, plan_target_strata */
