@@ -1,0 +1,367 @@
+"""MI355X leave-out (KSS) engine: Python host mirror of the C ABI.
+
+The product is libvckss_mi355x.so (fevc_amd/csrc, built by fevc_amd/build.py).
+This module binds its vckss_rust_* entry points the way the reference Stata
+shim binds the Rust plugin (rust/stata_backend/cshim/stata_entry.c:1836-1852
+prepare, 2530-2588 solvefull, result, release) and exposes the extension
+entry points used by the benchmark and the parity tests.  It never computes
+anything itself: a missing library or device raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["EngineError", "Engine", "library", "build_hierarchy"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libvckss_mi355x.so")
+_LIB = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"vckss status {code}: {message}")
+        self.code = code
+
+
+Poll = C.CFUNCTYPE(C.c_int32, C.c_void_p)
+
+
+class PrepareRequestV4(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("struct_size", C.c_uint32), ("rows", C.c_uint64),
+                ("cleanup_abandoned", C.c_uint32), ("reserved", C.c_uint32),
+                ("memory_limit_bytes", C.c_uint64), ("caller_copy_bytes", C.c_uint64),
+                ("deletion_mode", C.c_uint32), ("controls_count", C.c_uint32),
+                ("reserved_2", C.c_uint64), ("implicit_match", C.c_uint32), ("reserved_3", C.c_uint32)]
+
+
+class PrepareRequestInterruptV3(C.Structure):
+    _fields_ = [("options", PrepareRequestV4), ("interrupt_poll", C.c_void_p),
+                ("interrupt_context", C.c_void_p), ("checkpoint_interval", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class ColumnsV3(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("reserved", C.c_uint32), ("rows", C.c_uint64),
+                ("worker", C.c_void_p), ("firm", C.c_void_p), ("deletion", C.c_void_p),
+                ("outcome", C.c_void_p), ("frequency", C.c_void_p), ("target_weight", C.c_void_p),
+                ("controls", C.c_void_p), ("controls_count", C.c_uint32), ("reserved_2", C.c_uint32),
+                ("probe_order", C.c_void_p), ("probeorder_supplied", C.c_uint32),
+                ("reserved_3", C.c_uint32)]
+
+
+class SolveRequestV5(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("struct_size", C.c_uint32), ("seed", C.c_uint64),
+        ("probes", C.c_uint32), ("leverage_batch_width", C.c_uint32),
+        ("target_batch_width", C.c_uint32), ("deletion_mode", C.c_uint32),
+        ("rng_contract", C.c_uint32), ("solver_route", C.c_uint32),
+        ("allow_automatic_cmg_setup_fallback", C.c_uint32), ("exact_dimension_limit", C.c_uint64),
+        ("cmg_minimum_dimension", C.c_uint64), ("pcg_tolerance", C.c_double),
+        ("maximum_iterations", C.c_uint32), ("residual_replacement_interval", C.c_uint32),
+        ("rank_tolerance", C.c_double), ("block_tolerance", C.c_double),
+        ("cmg_terminal_vertices", C.c_uint64), ("cmg_dense_vertex_cap", C.c_uint64),
+        ("cmg_maximum_levels", C.c_uint64), ("cmg_aggregate_cap", C.c_uint64),
+        ("cmg_minimum_reduction", C.c_double), ("cmg_jacobi_weight", C.c_double),
+        ("cmg_pre_sweeps", C.c_uint32), ("cmg_post_sweeps", C.c_uint32),
+        ("cmg_maximum_edge_complexity", C.c_double), ("cmg_maximum_vertex_complexity", C.c_double),
+        ("cmg_memory_limit_bytes", C.c_uint64),
+        ("algorithm", C.c_uint32), ("nuisance_mode", C.c_uint32),
+        ("exact_estimator_limit", C.c_uint64), ("blocksize_limit", C.c_uint64),
+        ("engine", C.c_uint32), ("batch_mode", C.c_uint32), ("stayers_mode", C.c_uint32),
+        ("target_weight_mode", C.c_uint32), ("deletion_unit_source", C.c_uint32),
+        ("probeorder_supplied", C.c_uint32), ("wallseconds_supplied", C.c_uint32),
+        ("capability_schema", C.c_uint32), ("capability_profile", C.c_uint32),
+        ("frequency_use", C.c_uint32), ("physical_limit", C.c_uint64),
+        ("request_signature", C.c_uint64), ("reserved_3", C.c_uint64),
+        ("leverage_batch_mode", C.c_uint32), ("target_batch_mode", C.c_uint32),
+        ("wallseconds", C.c_double), ("reserved_4", C.c_uint64),
+        ("threads", C.c_uint32), ("tolerance_supplied", C.c_uint32), ("full_cmg_v2", C.c_uint32),
+        ("reserved_5", C.c_uint32)]
+
+
+class SolveRequestInterruptV5(C.Structure):
+    _fields_ = [("options", SolveRequestV5), ("interrupt_poll", C.c_void_p),
+                ("interrupt_context", C.c_void_p), ("checkpoint_interval", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class ComponentVector(C.Structure):
+    _fields_ = [("worker", C.c_double), ("firm", C.c_double), ("covariance", C.c_double),
+                ("total", C.c_double)]
+
+    def as_list(self):
+        return [self.worker, self.firm, self.covariance, self.total]
+
+
+class ResultV1(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("reserved", C.c_uint32), ("generation", C.c_uint64),
+                ("plugin", ComponentVector), ("correction", ComponentVector),
+                ("corrected", ComponentVector), ("numerical_mcse", ComponentVector)]
+
+
+class SnapshotV1(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("state", C.c_uint32), ("generation", C.c_uint64),
+                ("last_released_generation", C.c_uint64)]
+
+
+# (name, restype, argtypes) of every exported symbol declared in include/*.h
+SIGNATURES = [
+    ("vckss_rust_abi_version", C.c_uint32, []),
+    ("vckss_rust_backend_version", C.c_char_p, []),
+    ("vckss_rust_capabilities_json", C.c_char_p, []),
+    ("vckss_rust_last_error", C.c_char_p, []),
+    ("vckss_rust_selftest", C.c_int32, []),
+    ("vckss_rust_engine_default_prepare_request_interrupt_v3", C.c_int32,
+     [C.POINTER(PrepareRequestInterruptV3), C.c_uint32]),
+    ("vckss_rust_engine_default_solve_request_interrupt_v5", C.c_int32,
+     [C.POINTER(SolveRequestInterruptV5), C.c_uint32]),
+    ("vckss_rust_engine_prepare_interrupt_v4", C.c_int32,
+     [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64), C.c_uint32]),
+    ("vckss_rust_engine_retained_mask_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint64]),
+    ("vckss_rust_engine_solve_v5", C.c_int32, [C.c_uint64, C.POINTER(SolveRequestV5)]),
+    ("vckss_rust_engine_solve_interrupt_v5", C.c_int32,
+     [C.c_uint64, C.POINTER(SolveRequestInterruptV5)]),
+    ("vckss_rust_engine_result_v1", C.c_int32, [C.c_uint64, C.POINTER(ResultV1), C.c_uint32]),
+    ("vckss_rust_engine_release_v1", C.c_int32, [C.c_uint64]),
+    ("vckss_rust_engine_clear_abandoned_v1", C.c_int32, []),
+    ("vckss_rust_engine_snapshot_v1", C.c_int32, [C.POINTER(SnapshotV1), C.c_uint32]),
+    ("vckss_rust_engine_last_error", C.c_char_p, []),
+    ("vckss_mi355x_configure_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("vckss_mi355x_dims_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_array_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint64]),
+    ("vckss_mi355x_atoms_v1", C.c_int32,
+     [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint64]),
+    ("vckss_mi355x_spmv_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_vcycle_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_pcg_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_time_leverage_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_double),
+      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double),
+      C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_hierarchy_build_v1", C.c_int32,
+     [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
+      C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_hierarchy_level_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_hierarchy_levels_v1", C.c_int32,
+     [C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_hierarchy_free_v1", C.c_int32, [C.c_uint64]),
+    ("vckss_mi355x_prepare_host_v1", C.c_int32,
+     [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64)]),
+]
+
+
+def library():
+    """Load the built engine library; raises if it was never built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIB_PATH):
+            raise EngineError(11, f"{_LIB_PATH} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(_LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _check(code: int):
+    if code != 0:
+        raise EngineError(code, library().vckss_rust_engine_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_prepare_request(rows: int) -> PrepareRequestInterruptV3:
+    req = PrepareRequestInterruptV3()
+    _check(library().vckss_rust_engine_default_prepare_request_interrupt_v3(C.byref(req), C.sizeof(req)))
+    req.options.rows = rows
+    return req
+
+
+def default_solve_request() -> SolveRequestInterruptV5:
+    req = SolveRequestInterruptV5()
+    _check(library().vckss_rust_engine_default_solve_request_interrupt_v5(C.byref(req), C.sizeof(req)))
+    return req
+
+
+class Engine:
+    """One prepared context (generation) of the engine.
+
+    Mirrors the reference call sequence prepare -> solve -> result -> release.
+    """
+
+    def __init__(self, worker, firm, deletion, outcome, frequency, target, *, upload=True,
+                 poll=None):
+        lib = library()
+        cols = [_f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency), _f64(target)]
+        self._cols = cols
+        rows = len(cols[0])
+        req = default_prepare_request(rows)
+        req.options.cleanup_abandoned = 1
+        self._poll = Poll(poll) if poll else None
+        if self._poll:
+            req.interrupt_poll = C.cast(self._poll, C.c_void_p)
+        c = ColumnsV3()
+        c.struct_size = C.sizeof(c)
+        c.rows = rows
+        c.worker, c.firm, c.deletion, c.outcome, c.frequency, c.target_weight = [a.ctypes.data for a in cols]
+        h = C.c_uint64()
+        if upload:
+            _check(lib.vckss_rust_engine_prepare_interrupt_v4(C.byref(req), C.byref(c), C.byref(h), 8))
+        else:
+            _check(lib.vckss_mi355x_prepare_host_v1(C.byref(req), C.byref(c), C.byref(h)))
+        self.generation = h.value
+        d = np.zeros(15, np.uint64)
+        _check(lib.vckss_mi355x_dims_v1(self.generation, d.ctypes.data, 15))
+        keys = ["input_rows", "rows", "physical", "workers", "firms", "cells", "units",
+                "exact_strata", "strata", "hybrid_vertices", "hybrid_edges", "auxiliary", "levels",
+                "terminal_dimension", "topology_checksum"]
+        self.dims = {k: int(v) for k, v in zip(keys, d)}
+
+    def mask(self):
+        m = np.zeros(self.dims["input_rows"], np.uint8)
+        _check(library().vckss_rust_engine_retained_mask_v1(self.generation, m.ctypes.data, m.size))
+        return m.astype(bool)
+
+    def configure(self, precision=64, leverage_width=32, target_width=16):
+        _check(library().vckss_mi355x_configure_v1(self.generation, precision, leverage_width,
+                                                   target_width))
+
+    def solve(self, seed=8675309, probes=200, tolerance=None, fit_tolerance=1e-10,
+              maximum_iterations=10000, residual_replacement_interval=100, poll=None, **overrides):
+        req = default_solve_request()
+        o = req.options
+        o.algorithm = 2  # JLA
+        o.full_cmg_v2 = 1
+        o.threads = 1
+        o.seed = seed
+        o.probes = probes
+        o.pcg_tolerance = fit_tolerance
+        o.maximum_iterations = maximum_iterations
+        o.residual_replacement_interval = residual_replacement_interval
+        if tolerance is not None:
+            o.pcg_tolerance = tolerance
+            o.tolerance_supplied = 1
+        for k, v in overrides.items():
+            setattr(o, k, v)
+        cb = Poll(poll) if poll else None
+        if cb:
+            req.interrupt_poll = C.cast(cb, C.c_void_p)
+        _check(library().vckss_rust_engine_solve_interrupt_v5(self.generation, C.byref(req)))
+        return self.result()
+
+    def result(self):
+        r = ResultV1()
+        _check(library().vckss_rust_engine_result_v1(self.generation, C.byref(r), C.sizeof(r)))
+        return dict(plugin=r.plugin.as_list(), correction=r.correction.as_list(),
+                    corrected=r.corrected.as_list(), numerical_mcse=r.numerical_mcse.as_list())
+
+    def array(self, which: int, size: int):
+        out = np.zeros(size, np.float64)
+        _check(library().vckss_mi355x_array_v1(self.generation, which, out.ctypes.data, size))
+        return out
+
+    def arrays(self, probes):
+        d = self.dims
+        names = [("fitted_cell", 0, d["cells"]), ("projection_share", 1, d["units"]),
+                 ("residual_share", 2, d["units"]), ("finite_bias", 3, d["units"]),
+                 ("finite_variance", 4, d["units"]), ("residual_mass", 5, d["units"]),
+                 ("deleted_mass", 6, d["units"]), ("cell_correction_weight", 7, d["cells"]),
+                 ("target_draws", 8, 4 * probes), ("leverage_iterations", 9, probes),
+                 ("target_iterations", 10, 2 * probes), ("leverage_complete_residual", 11, probes),
+                 ("timing", 12, 4)]
+        out = {n: self.array(w, s) for n, w, s in names}
+        out["target_draws"] = out["target_draws"].reshape(probes, 4)
+        return out
+
+    def atoms(self, seed, domain, first_probe, columns):
+        n = self.dims["units"] if domain == 0 else self.dims["strata"]
+        out = np.zeros(n * columns, np.int64)
+        _check(library().vckss_mi355x_atoms_v1(self.generation, seed, domain, first_probe, columns,
+                                               out.ctypes.data, out.size))
+        return out.reshape(columns, n)
+
+    def spmv(self, level, x, columns):
+        x = _f64(x)
+        y = np.zeros_like(x)
+        _check(library().vckss_mi355x_spmv_v1(self.generation, level, columns, x.ctypes.data,
+                                              y.ctypes.data))
+        return y
+
+    def vcycle(self, r, columns, precision=64):
+        r = _f64(r)
+        z = np.zeros_like(r)
+        _check(library().vckss_mi355x_vcycle_v1(self.generation, columns, precision, r.ctypes.data,
+                                                z.ctypes.data))
+        return z
+
+    def pcg(self, rhs, columns, tolerance=1e-8, precision=64):
+        rhs = _f64(rhs)
+        x = np.zeros_like(rhs)
+        it = np.zeros(columns, np.uint32)
+        _check(library().vckss_mi355x_pcg_v1(self.generation, columns, tolerance, precision,
+                                             rhs.ctypes.data, x.ctypes.data, it.ctypes.data))
+        return x, it
+
+    def time_leverage(self, width, precision, tolerance, batches):
+        sec, ms, by = C.c_double(), C.c_double(), C.c_double()
+        solves, iters, launches = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(library().vckss_mi355x_time_leverage_v1(
+            self.generation, width, precision, tolerance, batches, C.byref(sec), C.byref(solves),
+            C.byref(iters), C.byref(ms), C.byref(by), C.byref(launches)))
+        return dict(seconds=sec.value, solves=solves.value, iterations=iters.value,
+                    kernel_ms=ms.value, kernel_bytes=by.value, kernel_launches=launches.value)
+
+    def release(self):
+        if self.generation:
+            _check(library().vckss_rust_engine_release_v1(self.generation))
+            self.generation = 0
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def build_hierarchy(n, u, v, w, direct_threshold=700, max_levels=128):
+    """Host CMG hierarchy of an arbitrary graph as a list of level dicts."""
+    lib = library()
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    w = np.ascontiguousarray(w, np.float64)
+    h = C.c_uint64()
+    _check(lib.vckss_mi355x_hierarchy_build_v1(n, len(u), u.ctypes.data, v.ctypes.data,
+                                               w.ctypes.data, direct_threshold, max_levels, C.byref(h)))
+    try:
+        nl, fnnz = C.c_uint32(), C.c_uint64()
+        _check(lib.vckss_mi355x_hierarchy_levels_v1(h.value, C.byref(nl), C.byref(fnnz)))
+        levels = []
+        for i in range(nl.value):
+            info = np.zeros(6, np.uint64)
+            _check(lib.vckss_mi355x_hierarchy_level_v1(h.value, i, info.ctypes.data, None, None, None,
+                                                       None, None))
+            n_i, m_i = int(info[0]), int(info[1])
+            lab = np.zeros(n_i, np.uint32)
+            eu = np.zeros(m_i, np.uint32)
+            ev = np.zeros(m_i, np.uint32)
+            ew = np.zeros(m_i, np.float64)
+            dg = np.zeros(n_i, np.float64)
+            _check(lib.vckss_mi355x_hierarchy_level_v1(h.value, i, info.ctypes.data, lab.ctypes.data,
+                                                       eu.ctypes.data, ev.ctypes.data, ew.ctypes.data,
+                                                       dg.ctypes.data))
+            levels.append(dict(n=n_i, m=m_i, nnz=int(info[2]), repeat=int(info[3]),
+                               terminal=int(info[4]), coarse=int(info[5]),
+                               labels=lab if int(info[5]) else None, u=eu, v=ev, w=ew, diag=dg))
+        return levels, int(fnnz.value)
+    finally:
+        lib.vckss_mi355x_hierarchy_free_v1(h.value)

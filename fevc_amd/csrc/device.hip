@@ -1,0 +1,851 @@
+// Device engine: uploads the prepared problem and CMG hierarchy to HBM and
+// runs the per-probe path of the leave-out estimator on the GPU:
+//   Counter-V1 atoms -> exact right-hand sides -> Schur right-hand side ->
+//   lockstep certified PCG with the CMG V-cycle -> extraction and the
+//   complete-residual gate -> moments / target contraction.
+// Reference flow: engine.rs:1060-1563, full_cmg.rs:809-1340, pcg.rs:1537-1909,
+// preconditioner/cycle.rs:136-478.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <type_traits>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels_engine.hpp"
+
+#define VK_HIP(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      ::vk::fail(::vk::kResourceLimit, "hip", std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace vk {
+namespace {
+
+using dev::kBlock;
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept { std::swap(p, o.p); std::swap(n, o.n); return *this; }
+  ~DBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t count) {
+    if (p && count <= n) return;
+    if (p) VK_HIP(hipFree(p));
+    p = nullptr;
+    n = count;
+    VK_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  }
+  template <typename V>
+  void upload(const std::vector<V> &v) {
+    alloc(v.size());
+    if (v.empty()) return;
+    if constexpr (std::is_same_v<T, V>) {
+      VK_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    } else {
+      std::vector<T> c(v.begin(), v.end());
+      VK_HIP(hipMemcpy(p, c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+  }
+  // Ordered after the work already queued on `stream`; returns once the copy
+  // has landed.
+  void download(std::vector<T> &v, size_t count, hipStream_t stream) const {
+    v.resize(count);
+    if (count) VK_HIP(hipMemcpyAsync(v.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+    VK_HIP(hipStreamSynchronize(stream));
+  }
+};
+
+inline unsigned grid_for(uint64_t threads, unsigned cap = 8192) {
+  uint64_t g = (threads + kBlock - 1) / kBlock;
+  return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+}
+inline unsigned reduce_grid(uint64_t threads) { return grid_for(threads, dev::kMaxReduceBlocks); }
+
+template <typename F>
+void with_k(uint32_t k, F &&f) {
+  switch (k) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    default: fail(kInvalidInput, "device", "batch lane count must be a power of two in [1, 64]");
+  }
+}
+
+uint32_t lanes_for(uint32_t columns) {
+  uint32_t k = 1;
+  while (k < columns) k <<= 1;
+  if (k > 64) fail(kInvalidInput, "device", "at most 64 right-hand sides per batch");
+  return k;
+}
+
+struct DevLevel {
+  uint32_t n = 0, coarse = 0, repeat = 0;
+  uint64_t nnz = 0;
+  int terminal = 0;
+  DBuf<uint32_t> rp, col, lab, aptr, amem;
+  DBuf<double> val, inv;
+  DBuf<float> val32, inv32;
+};
+
+template <typename T>
+struct LevelVecs {
+  DBuf<T> b, x, y;
+  DBuf<double> mean;  // coarse mean of b, filled by the parent's restriction
+};
+
+constexpr uint64_t kTagLeverage = 0x4c45564552414745ULL;
+constexpr uint64_t kTagTarget = 0x0054415247455401ULL;
+
+}  // namespace
+
+struct DeviceEngine::Impl {
+  hipStream_t stream = nullptr;
+  const Prepared *P = nullptr;
+  const HybridSystem *HS = nullptr;
+  uint64_t W = 0, F = 0, C = 0, G = 0, S = 0, n0 = 0, aux = 0;
+  double target_total = 0, bound = 0;
+  DBuf<uint64_t> worker_ptr, firm_ptr, cell_unit_ptr, cell_stratum_ptr, aux_ptr;
+  DBuf<uint32_t> firm_cells, cell_worker, cell_firm, cell_units, unit_cell, aux_nbr;
+  DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
+  DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
+  std::vector<DevLevel> levels;
+  DBuf<double> ginv;
+  DBuf<double> partial;
+  DBuf<unsigned> counters;
+  unsigned counter_next = 0;
+  uint32_t cap = 0;
+  std::vector<LevelVecs<double>> lv64;
+  std::vector<LevelVecs<float>> lv32;
+  DBuf<double> b0, proj, x, r, z, p, ap, guess_rhs;
+  DBuf<dev::ColState> st;
+  DBuf<uint8_t> use;
+  DBuf<double> wrhs, frhs, worker, red, wsum, sout;
+  DBuf<double> lane_a, lane_b, lane_c, reduced_res, complete_res, rhs_norm, sums;
+  DBuf<double> atoms, moments, adjust, cellw, dir, totals, draws, fitted;
+  DBuf<int> status;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double timed_bytes = 0;
+  uint64_t vcycles = 0;
+
+  unsigned *counter() { return counters.p + (counter_next++ % 64); }
+
+  void ensure(uint32_t lanes) {
+    if (lanes <= cap) return;
+    cap = lanes;
+    lv64.clear();
+    lv32.clear();
+    lv64.resize(levels.size());
+    lv32.resize(levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+      size_t m = (size_t)levels[l].n * lanes;
+      lv64[l].b.alloc(m); lv64[l].x.alloc(m); lv64[l].y.alloc(m); lv64[l].mean.alloc(64);
+      lv32[l].b.alloc(m); lv32[l].x.alloc(m); lv32[l].y.alloc(m); lv32[l].mean.alloc(64);
+    }
+    size_t m0 = (size_t)n0 * lanes;
+    b0.alloc(m0); proj.alloc(m0); x.alloc(m0); r.alloc(m0); z.alloc(m0); p.alloc(m0); ap.alloc(m0);
+    guess_rhs.alloc(m0);
+    wrhs.alloc(W * lanes); frhs.alloc(F * lanes); worker.alloc(W * lanes); wsum.alloc(W * lanes);
+    red.alloc(F * lanes); sout.alloc(F * lanes);
+  }
+
+  void check_status(const char *phase) {
+    int h = 0;
+    VK_HIP(hipMemcpyAsync(&h, status.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    VK_HIP(hipStreamSynchronize(stream));
+    if (h) fail(h, phase, "device gate failed with status " + std::to_string(h));
+  }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ V-cycle
+template <typename T, int K>
+struct Cycle {
+  DeviceEngine::Impl &E;
+  std::vector<LevelVecs<T>> &V;
+
+  const T *val(const DevLevel &L) const {
+    if constexpr (sizeof(T) == 4) return L.val32.p; else return L.val.p;
+  }
+  const T *inv(const DevLevel &L) const {
+    if constexpr (sizeof(T) == 4) return L.inv32.p; else return L.inv.p;
+  }
+
+  // cycle.rs:136-311 with the coarse centring of cycle.rs:459-478 folded into
+  // the first kernel of the child level.
+  void level(size_t l, uint32_t iterations, const double *mean) {
+    DevLevel &L = E.levels[l];
+    LevelVecs<T> &v = V[l];
+    const uint64_t work = (uint64_t)L.n * K;
+    if (L.terminal) {
+      if (L.terminal == kDirect)
+        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(work), kBlock, 0, E.stream, L.n,
+                           E.ginv.p, v.b.p, mean, v.x.p);
+      else
+        hipLaunchKernelGGL((dev::k_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L),
+                           v.b.p, v.x.p, mean);
+      return;
+    }
+    DevLevel &Lc = E.levels[l + 1];
+    LevelVecs<T> &c = V[l + 1];
+    for (uint32_t it = 0; it < iterations; ++it) {
+      if (it == 0) {
+        hipLaunchKernelGGL((dev::k_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L),
+                           v.b.p, v.x.p, mean);
+      } else {
+        hipLaunchKernelGGL((dev::k_smooth<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, L.rp.p,
+                           L.col.p, val(L), inv(L), v.b.p, v.x.p, v.y.p);
+        std::swap(v.x, v.y);
+      }
+      hipLaunchKernelGGL((dev::k_restrict<T, K>), reduce_grid((uint64_t)Lc.n * K), kBlock, 0, E.stream,
+                         Lc.n, L.aptr.p, L.amem.p, L.rp.p, L.col.p, val(L), v.b.p, v.x.p, c.b.p,
+                         E.partial.p, E.counter(), c.mean.p);
+      level(l + 1, L.repeat, c.mean.p);
+      const bool timed = E.timing && l == 0;
+      if (timed) {
+        hipEvent_t a, b;
+        VK_HIP(hipEventCreate(&a));
+        VK_HIP(hipEventCreate(&b));
+        VK_HIP(hipEventRecord(a, E.stream));
+        E.events.push_back({a, b});
+      }
+      hipLaunchKernelGGL((dev::k_prolong_smooth<T, K>), grid_for(work), kBlock, 0, E.stream, L.n,
+                         L.rp.p, L.col.p, val(L), inv(L), L.lab.p, c.x.p, v.b.p, v.x.p, v.y.p);
+      if (timed) {
+        VK_HIP(hipEventRecord(E.events.back().second, E.stream));
+        // algorithmic bytes: CSR once, inv + labels, b/x read, y write, e read
+        const double sv = sizeof(T), sx = sizeof(T);
+        E.timed_bytes = (double)L.nnz * (sv + 4) + 4.0 * (L.n + 1) + (sv + 4) * L.n +
+                        K * sx * (3.0 * L.n + Lc.n);
+      }
+      std::swap(v.x, v.y);
+    }
+  }
+};
+
+// z = M r for K lanes.  With `mean_r` non-null, r is centred in place first.
+template <int K>
+void apply_cycle(DeviceEngine::Impl &E, double *r, const double *mean_r, double *z, bool fp32) {
+  const uint64_t m0 = (uint64_t)E.n0 * K;
+  E.vcycles++;
+  if (!fp32) {
+    auto &V = E.lv64;
+    double *saved = V[0].b.p;
+    V[0].b.p = r;
+    Cycle<double, K> cyc{E, V};
+    cyc.level(0, 1, mean_r);
+    VK_HIP(hipMemcpyAsync(z, V[0].x.p, m0 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
+    V[0].b.p = saved;
+  } else {
+    auto &V = E.lv32;
+    if (mean_r)
+      hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)E.n0, r, mean_r);
+    hipLaunchKernelGGL((dev::k_to_float<K>), grid_for(m0), kBlock, 0, E.stream, m0, r, V[0].b.p);
+    Cycle<float, K> cyc{E, V};
+    cyc.level(0, 1, nullptr);
+    hipLaunchKernelGGL((dev::k_to_double<K>), grid_for(m0), kBlock, 0, E.stream, m0, V[0].x.p, z);
+  }
+}
+
+// Lockstep certified PCG (pcg.rs:1537-1909) on the finest hybrid Laplacian.
+// b: n0 x K right-hand sides.  With `warm`, E.x holds the initial guesses.
+template <int K>
+void pcg_lanes(DeviceEngine::Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it,
+               uint32_t interval, bool warm, bool fp32, const std::vector<uint8_t> &use,
+               std::vector<dev::ColState> &out) {
+  const uint32_t n = E.n0;
+  const uint64_t m0 = (uint64_t)n * K;
+  const DevLevel &L0 = E.levels[0];
+  dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval};
+  VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
+  hipLaunchKernelGGL((dev::k_pcg_stats<K>), reduce_grid(m0), kBlock, 0, E.stream, n, b, E.partial.p,
+                     E.counter(), E.st.p, prm, E.use.p);
+  if (!warm) {
+    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p,
+                       E.r.p, E.x.p, E.st.p);
+    hipLaunchKernelGGL((dev::k_pcg_start<K>), 1, 64, 0, E.stream, E.st.p, prm);
+  } else {
+    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p,
+                       E.r.p, E.ap.p, E.st.p);
+    hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
+                       E.partial.p, E.counter(), E.lane_a.p);
+    hipLaunchKernelGGL((dev::k_mask_means<K>), 1, 64, 0, E.stream, E.lane_a.p, E.use.p);
+    hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
+                       E.lane_a.p);
+    hipLaunchKernelGGL((dev::k_pcg_warm<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p, L0.col.p,
+                       L0.val.p, b, E.proj.p, E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
+  }
+  std::vector<dev::ColState> host(K);
+  auto any_active = [&]() {
+    VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
+    VK_HIP(hipStreamSynchronize(E.stream));
+    for (int j = 0; j < K; ++j)
+      if (host[j].active) return true;
+    return false;
+  };
+  auto precondition = [&]() {
+    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p);
+    apply_cycle<K>(E, E.r.p, E.lane_b.p, E.z.p, fp32);
+    hipLaunchKernelGGL((dev::k_pcg_rho<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.r.p, E.z.p,
+                       E.partial.p, E.counter(), E.st.p);
+    hipLaunchKernelGGL((dev::k_pcg_direction<K>), grid_for(m0), kBlock, 0, E.stream, n, E.z.p, E.p.p, E.st.p);
+  };
+  if (any_active()) {
+    precondition();
+    for (uint32_t it = 1; it <= max_it; ++it) {
+      hipLaunchKernelGGL((dev::k_pcg_curvature<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p,
+                         L0.col.p, L0.val.p, E.p.p, E.ap.p, E.partial.p, E.counter(), E.st.p);
+      hipLaunchKernelGGL((dev::k_pcg_update<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.p.p, E.ap.p,
+                         E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
+      hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p);
+      hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
+                         E.lane_a.p);
+      hipLaunchKernelGGL((dev::k_pcg_fresh<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p, L0.col.p,
+                         L0.val.p, b, E.proj.p, E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
+      hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm);
+      if (!any_active()) break;
+      precondition();
+    }
+  }
+  VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
+  VK_HIP(hipStreamSynchronize(E.stream));
+  out = host;
+}
+
+// full_cmg.rs:1158-1340 extraction for K lanes: normalise the firm mean, the
+// reduced Schur residual, worker reconstruction and the complete residual.
+template <int K>
+void extract(DeviceEngine::Impl &E) {
+  const uint64_t F = E.F, W = E.W;
+  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.x.p, E.partial.p,
+                     E.counter(), E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p);
+  VK_HIP(hipMemcpyAsync(E.red.p, E.x.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
+  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.red.p, E.partial.p,
+                     E.counter(), E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
+                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
+  hipLaunchKernelGGL((dev::k_schur_apply<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
+                     E.firm_ptr.p, E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.fdiag.p, E.red.p,
+                     E.wsum.p, E.sout.p, E.partial.p, E.counter(), E.lane_b.p);
+  hipLaunchKernelGGL((dev::k_reduced_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
+                     E.b0.p, E.sout.p, E.lane_b.p, E.partial.p, E.counter(), E.reduced_res.p);
+  hipLaunchKernelGGL((dev::k_worker_solution<K>), reduce_grid(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
+                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.x.p, E.worker.p, E.partial.p,
+                     E.counter(), E.sums.p);
+  hipLaunchKernelGGL((dev::k_firm_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
+                     E.firm_ptr.p, E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.fdiag.p, E.frhs.p,
+                     E.x.p, E.worker.p, E.sums.p, E.partial.p, E.counter(), E.complete_res.p, E.rhs_norm.p);
+}
+
+double euclid(const std::vector<double> &v, uint64_t n, uint32_t K, uint32_t lane) {
+  double scale = 0.0;
+  for (uint64_t i = 0; i < n; ++i) scale = std::max(scale, std::fabs(v[i * K + lane]));
+  if (scale == 0.0 || !std::isfinite(scale)) return scale;
+  double s = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    double q = v[i * K + lane] / scale;
+    s += q * q;
+  }
+  return scale * std::sqrt(s);
+}
+
+// One batch of right-hand sides already in E.wrhs / E.frhs (K lanes, the first
+// `columns` in use): the CMG_FULL_V2 batch solve with its complete-residual
+// gate and warm-start refinement ladder (full_cmg.rs:809-1111).
+template <int K>
+void solve_columns(DeviceEngine::Impl &E, uint32_t columns, double effective_tol, double inner_ratio,
+                   const SolveOptions &opt, std::vector<ColumnReceipt> &receipts) {
+  const uint64_t F = E.F;
+  const uint64_t m0 = E.n0 * K;
+  const double gate = std::max(10.0 * effective_tol, 1e-11);
+  VK_HIP(hipMemsetAsync(E.b0.p, 0, m0 * sizeof(double), E.stream));
+  hipLaunchKernelGGL((dev::k_schur_rhs<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F, E.firm_ptr.p,
+                     E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.frhs.p, E.b0.p,
+                     E.partial.p, E.counter(), E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.b0.p, E.lane_a.p);
+  std::vector<uint8_t> use(K, 0);
+  for (uint32_t j = 0; j < columns; ++j) use[j] = 1;
+  std::vector<dev::ColState> st;
+  pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
+               opt.precision == 32, use, st);
+  receipts.assign(columns, ColumnReceipt{});
+  for (uint32_t j = 0; j < columns; ++j) {
+    if (st[j].status) fail(st[j].status, "cmg_full_v2", "lane " + std::to_string(j) + " PCG failed");
+    receipts[j].iterations = st[j].iters;
+    receipts[j].restarts = st[j].restarts;
+  }
+  extract<K>(E);
+  std::vector<double> red, comp, rn;
+  auto read_gate = [&]() {
+    E.reduced_res.download(red, K, E.stream);
+    E.complete_res.download(comp, K, E.stream);
+    E.rhs_norm.download(rn, K, E.stream);
+  };
+  read_gate();
+  static const double factors[3] = {0.1, 0.01, 0.001};
+  for (int round = 0; round < 3; ++round) {
+    std::vector<uint8_t> fail_lane(K, 0);
+    bool any = false;
+    for (uint32_t j = 0; j < columns; ++j)
+      if (!std::isfinite(comp[j]) || comp[j] > gate || !std::isfinite(red[j]) || red[j] > gate) {
+        fail_lane[j] = 1;
+        any = true;
+      }
+    if (!any) break;
+    // warm start from the lifted firm solution (full_cmg.rs:970-1088)
+    if (E.aux)
+      hipLaunchKernelGGL((dev::k_lift<K>), grid_for(E.aux * K), kBlock, 0, E.stream, (uint32_t)F,
+                         (uint32_t)E.aux, E.aux_ptr.p, E.aux_nbr.p, E.aux_w.p, E.x.p);
+    std::vector<double> hb;
+    E.b0.download(hb, m0, E.stream);
+    double smallest = INFINITY;
+    for (uint32_t j = 0; j < columns; ++j)
+      if (fail_lane[j]) smallest = std::min(smallest, euclid(hb, E.n0, K, j));
+    double target = factors[round] * gate * smallest, rel = 0.0, abs_tol = target;
+    if (!(std::isfinite(target) && target > 0.0)) {
+      rel = effective_tol * inner_ratio * factors[round];
+      abs_tol = 0.0;
+    }
+    pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true,
+                 opt.precision == 32, fail_lane, st);
+    for (uint32_t j = 0; j < columns; ++j)
+      if (fail_lane[j]) {
+        if (st[j].status) fail(st[j].status, "cmg_full_v2_refinement", "refinement PCG failed");
+        receipts[j].iterations += st[j].iters;
+        receipts[j].restarts += st[j].restarts;
+      }
+    extract<K>(E);
+    read_gate();
+  }
+  for (uint32_t j = 0; j < columns; ++j) {
+    if (!std::isfinite(comp[j]) || comp[j] > gate || !std::isfinite(red[j]) || red[j] > gate)
+      fail(kFullResidual, "cmg_full_v2_refinement",
+           "zero-based RHS column " + std::to_string(j) + ": reduced residual " + std::to_string(red[j]) +
+               " or complete residual " + std::to_string(comp[j]) + " exceeds tolerance " +
+               std::to_string(gate));
+    receipts[j].reduced = red[j];
+    receipts[j].complete = comp[j];
+    receipts[j].rhs_norm = rn[j];
+  }
+}
+
+// jla.rs:286-364 on the host (one pass over the rows after the full fit).
+void plugin_components(const Prepared &P, const std::vector<double> &alpha, const std::vector<double> &psi,
+                       double out[4]) {
+  Neumaier wm, fm;
+  for (uint64_t r = 0; r < P.rows; ++r) {
+    wm.add(P.target[r] * alpha[P.row_worker[r]]);
+    fm.add(P.target[r] * psi[P.row_firm[r]]);
+  }
+  const double wmean = wm.value() / P.target_total, fmean = fm.value() / P.target_total;
+  Neumaier a, b, c, d;
+  for (uint64_t r = 0; r < P.rows; ++r) {
+    const double wt = P.target[r];
+    const double cw = alpha[P.row_worker[r]] - wmean, cf = psi[P.row_firm[r]] - fmean;
+    a.add(wt * cw * cw);
+    b.add(wt * cf * cf);
+    c.add(wt * cw * cf);
+    const double ct = cw + cf;
+    d.add(wt * ct * ct);
+  }
+  out[0] = a.value() / P.target_total;
+  out[1] = b.value() / P.target_total;
+  out[2] = c.value() / P.target_total;
+  out[3] = d.value() / P.target_total;
+  const double implied = out[0] + out[1] + 2.0 * out[2];
+  const double scale = std::max({std::fabs(out[0]), std::fabs(out[1]), std::fabs(2.0 * out[2]), std::fabs(out[3]), 1.0});
+  if (std::fabs(out[3] - implied) > 1e-11 * scale)
+    fail(kAccounting, "jla_components", "plug-in accounting identity failed");
+}
+
+double component_identity(const double c[4]) {
+  const double implied = c[0] + c[1] + 2.0 * c[2];
+  const double scale = std::max({std::fabs(c[0]), std::fabs(c[1]), std::fabs(2.0 * c[2]), std::fabs(c[3]), 1.0});
+  return std::fabs(c[3] - implied) / scale;
+}
+
+}  // namespace
+
+bool device_available() {
+  int devices = 0;
+  return hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
+}
+
+DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h)
+    : impl_(new Impl) {
+  Impl &E = *impl_;
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
+    fail(kUnsupported, "device", "no HIP device is available for the MI355X engine");
+  VK_HIP(hipStreamCreate(&E.stream));
+  E.P = &p;
+  E.HS = &hs;
+  E.W = p.workers; E.F = p.firms; E.C = p.cells; E.G = p.units; E.S = p.strata;
+  E.target_total = p.target_total;
+  E.aux = hs.aux;
+  E.n0 = h.levels[0].graph.n;
+  E.bound = h.levels[0].graph.norm_bound;
+  if (h.finest_components != 1)
+    fail(kUnsupported, "device", "the hybrid Laplacian must be connected");
+  std::vector<double> wd(E.W, 0.0), fd(E.F, 0.0);
+  for (uint64_t c = 0; c < E.C; ++c) {
+    wd[p.cell_worker[c]] += p.cell_weight[c];
+    fd[p.cell_firm[c]] += p.cell_weight[c];
+  }
+  E.worker_ptr.upload(p.worker_ptr); E.firm_ptr.upload(p.firm_ptr); E.firm_cells.upload(p.firm_cells);
+  E.cell_worker.upload(p.cell_worker); E.cell_firm.upload(p.cell_firm); E.cell_weight.upload(p.cell_weight);
+  E.wdiag.upload(wd); E.fdiag.upload(fd);
+  E.cell_unit_ptr.upload(p.cell_unit_ptr); E.cell_units.upload(p.cell_units);
+  E.unit_cell.upload(p.unit_cell); E.unit_count.upload(p.unit_count); E.unit_rank.upload(p.unit_rank);
+  E.unit_outcome.upload(p.unit_outcome);
+  E.stratum_per_copy.upload(p.stratum_per_copy); E.stratum_rank.upload(p.stratum_rank);
+  E.stratum_count.upload(p.stratum_count); E.cell_stratum_ptr.upload(p.cell_stratum_ptr);
+  E.cell_target.upload(p.cell_target);
+  E.aux_ptr.upload(hs.aux_ptr); E.aux_nbr.upload(hs.aux_nbr); E.aux_w.upload(hs.aux_w);
+  E.levels.resize(h.levels.size());
+  for (size_t l = 0; l < h.levels.size(); ++l) {
+    const Level &L = h.levels[l];
+    DevLevel &D = E.levels[l];
+    D.n = (uint32_t)L.graph.n;
+    D.nnz = L.col.size();
+    D.coarse = (uint32_t)L.coarse;
+    D.repeat = (uint32_t)std::min<uint64_t>(L.repeat, 1u << 30);
+    D.terminal = L.terminal;
+    D.rp.upload(L.row_ptr); D.col.upload(L.col); D.val.upload(L.val); D.val32.upload(L.val);
+    D.inv.upload(L.inv_diag); D.inv32.upload(L.inv_diag);
+    if (!L.labels.empty()) {
+      D.lab.upload(L.labels); D.aptr.upload(L.agg_ptr); D.amem.upload(L.agg_members);
+    }
+  }
+  if (h.terminal_reason == kDirect) {
+    E.ginv.upload(h.terminal_inverse);
+  }
+  E.partial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
+  E.counters.alloc(64);
+  VK_HIP(hipMemset(E.counters.p, 0, 64 * sizeof(unsigned)));
+  E.st.alloc(64); E.use.alloc(64);
+  E.lane_a.alloc(64); E.lane_b.alloc(64); E.lane_c.alloc(64);
+  E.reduced_res.alloc(64); E.complete_res.alloc(64); E.rhs_norm.alloc(64); E.sums.alloc(128);
+  E.status.alloc(1);
+  VK_HIP(hipMemset(E.status.p, 0, sizeof(int)));
+  E.totals.alloc(128); E.draws.alloc(4 * 64);
+}
+
+DeviceEngine::~DeviceEngine() {
+  if (impl_ && impl_->stream) {
+    (void)hipStreamSynchronize(impl_->stream);
+    for (auto &e : impl_->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    (void)hipStreamDestroy(impl_->stream);
+  }
+}
+
+SolveResult DeviceEngine::run(const SolveOptions &opt) {
+  Impl &E = *impl_;
+  const Prepared &P = *E.P;
+  if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+  const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
+  const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
+  const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
+  E.ensure(std::max({KL, K2, 1u}));
+  SolveResult res;
+  auto t0 = std::chrono::steady_clock::now();
+  const uint64_t R = opt.probes;
+
+  // ---- full fit (engine.rs:1108-1134)
+  {
+    std::vector<double> wr(E.W, 0.0), fr(E.F, 0.0);
+    for (uint64_t c = 0; c < E.C; ++c) {
+      wr[P.cell_worker[c]] += P.cell_outcome[c];
+      fr[P.cell_firm[c]] += P.cell_outcome[c];
+    }
+    VK_HIP(hipMemcpy(E.wrhs.p, wr.data(), E.W * sizeof(double), hipMemcpyHostToDevice));
+    VK_HIP(hipMemcpy(E.frhs.p, fr.data(), E.F * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<ColumnReceipt> rc;
+    solve_columns<1>(E, 1, opt.fit_tolerance, 0.01, opt, rc);
+    res.fit = rc[0];
+    std::vector<double> alpha, psi;
+    E.worker.download(alpha, E.W, E.stream);
+    E.x.download(psi, E.F, E.stream);
+    res.fitted_cell.resize(E.C);
+    for (uint64_t c = 0; c < E.C; ++c) res.fitted_cell[c] = alpha[P.cell_worker[c]] + psi[P.cell_firm[c]];
+    Neumaier rss;
+    for (uint64_t r = 0; r < P.rows; ++r) {
+      const double e = P.outcome[r] - res.fitted_cell[P.row_cell[r]];
+      rss.add(static_cast<double>(P.frequency[r]) * e * e);
+    }
+    res.weighted_rss = rss.value();
+    plugin_components(P, alpha, psi, res.plugin);
+    res.max_complete_residual = res.fit.complete;
+    E.fitted.upload(res.fitted_cell);
+  }
+
+  // ---- leverage probes (engine.rs:1150-1246)
+  auto tl = std::chrono::steady_clock::now();
+  E.moments.alloc(10 * E.G);
+  VK_HIP(hipMemsetAsync(E.moments.p, 0, 10 * E.G * sizeof(double), E.stream));
+  E.atoms.alloc(std::max<uint64_t>((uint64_t)KL * E.G, (uint64_t)KT * E.S));
+  for (uint64_t first = 0; first < R; first += lw) {
+    const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
+    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
+                       E.unit_count.p, opt.seed, kTagLeverage, first, k, E.atoms.p);
+    std::vector<ColumnReceipt> rc;
+    with_k(KL, [&](auto kk) {
+      constexpr int K = decltype(kk)::value;
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
+                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p);
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.F * K), kBlock, 0, E.stream, E.F, E.firm_ptr.p,
+                         E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.frhs.p);
+      solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.x.p, E.atoms.p,
+                         E.moments.p, E.status.p);
+    });
+    for (auto &c : rc) {
+      res.leverage.push_back(c);
+      res.max_complete_residual = std::max(res.max_complete_residual, c.complete);
+    }
+  }
+  E.check_status("jla_leverage");
+  res.seconds_leverage = std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
+
+  // ---- leverage adjustment (engine.rs:2269-2410)
+  E.adjust.alloc(7 * E.G);
+  E.cellw.alloc(E.C);
+  hipLaunchKernelGGL(dev::k_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.unit_cell.p,
+                     E.unit_count.p, E.unit_outcome.p, E.fitted.p,
+                     dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
+  E.check_status("jla_leverage_adjustment");
+  hipLaunchKernelGGL(dev::k_cell_weight, grid_for(E.C), kBlock, 0, E.stream, E.C, E.cell_unit_ptr.p,
+                     E.cell_units.p, E.adjust.p + 6 * E.G, E.cellw.p);
+  {
+    std::vector<double> adj;
+    E.adjust.download(adj, 6 * E.G, E.stream);
+    auto slice = [&](int q, std::vector<double> &out) { out.assign(adj.begin() + q * E.G, adj.begin() + (q + 1) * E.G); };
+    slice(0, res.projection_share); slice(1, res.residual_share); slice(2, res.finite_bias);
+    slice(3, res.finite_variance); slice(4, res.residual_mass); slice(5, res.deleted_mass);
+    E.cellw.download(res.cell_weight, E.C, E.stream);
+  }
+
+  // ---- target probes (engine.rs:1373-1484)
+  auto tt = std::chrono::steady_clock::now();
+  E.dir.alloc((uint64_t)KT * E.C);
+  res.draws.assign(4 * R, 0.0);
+  for (uint64_t first = 0; first < R; first += tw) {
+    const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
+    VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(double), E.stream));
+    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
+                       E.stratum_rank.p, E.stratum_count.p, opt.seed, kTagTarget, first, k, E.atoms.p);
+    std::vector<ColumnReceipt> rc;
+    with_k(KT, [&](auto kt) {
+      constexpr int KTc = decltype(kt)::value;
+      constexpr int K2c = 2 * KTc;
+      hipLaunchKernelGGL((dev::k_target_direction<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
+                         E.cell_stratum_ptr.p, E.stratum_per_copy.p, E.target_total, E.atoms.p, E.S, E.dir.p,
+                         E.partial.p, E.counter(), E.totals.p);
+      hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.W * K2c), kBlock, 0, E.stream, E.W,
+                         E.worker_ptr.p, (const uint32_t *)nullptr, E.dir.p, E.cell_target.p, E.target_total,
+                         E.totals.p, 0, E.wrhs.p);
+      hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.F * K2c), kBlock, 0, E.stream, E.F,
+                         E.firm_ptr.p, E.firm_cells.p, E.dir.p, E.cell_target.p, E.target_total, E.totals.p, 1,
+                         E.frhs.p);
+      hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.W * K2c), kBlock, 0, E.stream, E.W, E.wrhs.p, 0,
+                         E.totals.p, E.partial.p, E.counter(), E.status.p);
+      hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.F * K2c), kBlock, 0, E.stream, E.F, E.frhs.p, 1,
+                         E.totals.p, E.partial.p, E.counter(), E.status.p);
+      solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
+      hipLaunchKernelGGL((dev::k_contract<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
+                         E.cell_worker.p, E.cell_firm.p, E.cellw.p, E.worker.p, E.x.p, E.partial.p, E.counter(),
+                         E.draws.p);
+    });
+    std::vector<double> d;
+    E.draws.download(d, 4 * k, E.stream);
+    std::copy(d.begin(), d.end(), res.draws.begin() + 4 * first);
+    for (auto &c : rc) {
+      res.target.push_back(c);
+      res.max_complete_residual = std::max(res.max_complete_residual, c.complete);
+    }
+  }
+  E.check_status("jla_target");
+  res.seconds_target = std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
+
+  // ---- finalize (engine.rs:3108-3272)
+  Neumaier m[4];
+  for (uint64_t t = 0; t < R; ++t)
+    for (int j = 0; j < 4; ++j) {
+      if (!std::isfinite(res.draws[4 * t + j])) fail(kCorrectionNonFinite, "jla_target", "nonfinite target draw");
+      m[j].add(res.draws[4 * t + j]);
+    }
+  for (int j = 0; j < 4; ++j) res.correction[j] = m[j].value() / (double)R;
+  for (int j = 0; j < 4; ++j) res.corrected[j] = res.plugin[j] - res.correction[j];
+  Neumaier v[4];
+  for (uint64_t t = 0; t < R; ++t)
+    for (int j = 0; j < 4; ++j) {
+      const double e = res.draws[4 * t + j] - res.correction[j];
+      v[j].add(e * e);
+    }
+  const double denom = (double)(R * (R - 1));
+  for (int j = 0; j < 4; ++j) res.mcse[j] = std::sqrt(v[j].value() / denom);
+  double acc = std::max({component_identity(res.plugin), component_identity(res.correction),
+                         component_identity(res.corrected)});
+  for (uint64_t t = 0; t < R; ++t) acc = std::max(acc, component_identity(&res.draws[4 * t]));
+  res.accounting_residual = acc;
+  if (acc > 4096.0 * 2.220446049250313e-16)
+    fail(kTargetIdentity, "jla_accounting", "component accounting residual exceeds the roundoff gate");
+  res.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  res.vcycles = E.vcycles;
+  return res;
+}
+
+void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint32_t columns, bool target,
+                         std::vector<int64_t> &out) {
+  Impl &E = *impl_;
+  const uint64_t n = target ? E.S : E.G;
+  DBuf<double> buf;
+  buf.alloc(n * columns);
+  hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
+                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p,
+                     seed, tag, first_probe, columns, buf.p);
+  VK_HIP(hipStreamSynchronize(E.stream));
+  std::vector<double> h;
+  buf.download(h, n * columns, E.stream);
+  out.resize(h.size());
+  for (size_t i = 0; i < h.size(); ++i) out[i] = (int64_t)h[i];
+}
+
+void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t columns,
+                        std::vector<double> &y) {
+  Impl &E = *impl_;
+  if (level >= E.levels.size()) fail(kInvalidInput, "device", "level out of range");
+  const uint32_t K = lanes_for(columns);
+  if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
+  DevLevel &L = E.levels[level];
+  const uint64_t m = (uint64_t)L.n * K;
+  if (xin.size() != m) fail(kInvalidInput, "device", "input has the wrong size");
+  DBuf<double> a, b;
+  a.upload(xin);
+  b.alloc(m);
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    hipLaunchKernelGGL((dev::k_spmv<double, KK>), grid_for(m), kBlock, 0, E.stream, L.n, L.rp.p, L.col.p,
+                       L.val.p, a.p, b.p);
+  });
+  VK_HIP(hipStreamSynchronize(E.stream));
+  b.download(y, m, E.stream);
+}
+
+void DeviceEngine::vcycle(const std::vector<double> &rhs, uint32_t columns, int precision,
+                          std::vector<double> &out) {
+  Impl &E = *impl_;
+  const uint32_t K = lanes_for(columns);
+  if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
+  E.ensure(K);
+  const uint64_t m0 = E.n0 * K;
+  if (rhs.size() != m0) fail(kInvalidInput, "device", "rhs has the wrong size");
+  VK_HIP(hipMemcpy(E.guess_rhs.p, rhs.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    apply_cycle<KK>(E, E.guess_rhs.p, nullptr, E.z.p, precision == 32);
+  });
+  VK_HIP(hipStreamSynchronize(E.stream));
+  E.z.download(out, m0, E.stream);
+}
+
+void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
+                       std::vector<double> &xout, std::vector<ColumnReceipt> &receipts) {
+  Impl &E = *impl_;
+  const uint32_t K = lanes_for(columns);
+  if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
+  E.ensure(K);
+  const uint64_t m0 = E.n0 * K;
+  if (rhs.size() != m0) fail(kInvalidInput, "device", "rhs has the wrong size");
+  VK_HIP(hipMemcpy(E.guess_rhs.p, rhs.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<uint8_t> use(K, 1);
+  std::vector<dev::ColState> st;
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    pcg_lanes<KK>(E, E.guess_rhs.p, tolerance, 0.0, 1000, 25, false, precision == 32, use, st);
+  });
+  receipts.assign(K, ColumnReceipt{});
+  for (uint32_t j = 0; j < K; ++j) {
+    if (st[j].status) fail(st[j].status, "pcg", "lane " + std::to_string(j) + " failed");
+    receipts[j].iterations = st[j].iters;
+    receipts[j].restarts = st[j].restarts;
+    receipts[j].rhs_norm = st[j].rhs_norm;
+  }
+  E.x.download(xout, m0, E.stream);
+}
+
+double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
+                                           uint64_t &iterations) {
+  Impl &E = *impl_;
+  const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
+  const uint32_t KL = lanes_for(lw);
+  E.ensure(KL);
+  E.atoms.alloc((uint64_t)KL * E.G);
+  E.moments.alloc(10 * E.G);
+  E.fitted.alloc(E.C);
+  VK_HIP(hipMemsetAsync(E.moments.p, 0, 10 * E.G * sizeof(double), E.stream));
+  VK_HIP(hipStreamSynchronize(E.stream));
+  for (auto &e : E.events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  E.events.clear();
+  E.timing = true;
+  solves = 0;
+  iterations = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t b = 0; b < batches; ++b) {
+    const uint64_t first = (uint64_t)b * lw;
+    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((lw + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
+                       E.unit_count.p, opt.seed, kTagLeverage, first, lw, E.atoms.p);
+    std::vector<ColumnReceipt> rc;
+    with_k(KL, [&](auto kk) {
+      constexpr int K = decltype(kk)::value;
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
+                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p);
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.F * K), kBlock, 0, E.stream, E.F, E.firm_ptr.p,
+                         E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.frhs.p);
+      solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
+                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.x.p, E.atoms.p,
+                         E.moments.p, E.status.p);
+    });
+    for (auto &c : rc) iterations += c.iterations;
+    solves += lw;
+  }
+  VK_HIP(hipStreamSynchronize(E.stream));
+  E.timing = false;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+DeviceEngine::KernelTiming DeviceEngine::dominant_kernel_timing() const {
+  const Impl &E = *impl_;
+  KernelTiming t;
+  double total = 0.0;
+  for (auto &e : E.events) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) total += ms;
+  }
+  t.launches = E.events.size();
+  t.ms_per_launch = t.launches ? total / (double)t.launches : 0.0;
+  t.bytes_per_launch = E.timed_bytes;
+  return t;
+}
+
+}  // namespace vk

@@ -1,0 +1,74 @@
+// Device engine interface: the per-probe hot path of the leave-out estimator
+// on one MI355X (gfx950).  Host code prepares the problem and the CMG
+// hierarchy; everything per probe runs in HIP kernels.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "host_cmg.hpp"
+#include "host_prepare.hpp"
+
+namespace vk {
+
+struct SolveOptions {
+  uint64_t seed = 8675309;
+  uint32_t probes = 200;
+  uint32_t leverage_width = 32;   // probes per leverage batch (<= 64)
+  uint32_t target_width = 32;     // probes per target batch (2 columns each, <= 32)
+  double fit_tolerance = 1e-10;   // effective full-fit tolerance
+  double probe_tolerance = 1e-6;  // effective probe tolerance
+  uint32_t max_iterations = 10000;
+  uint32_t recompute_interval = 100;
+  double rank_tolerance = 1e-10;
+  double block_tolerance = 1e-10;
+  int precision = 64;             // 64: FP64 V-cycle; 32: FP32 V-cycle, FP64 Krylov
+};
+
+struct ColumnReceipt {
+  uint32_t iterations = 0, restarts = 0;
+  double reduced = 0.0, complete = 0.0, rhs_norm = 0.0;
+};
+
+struct SolveResult {
+  double plugin[4] = {0, 0, 0, 0}, correction[4] = {0, 0, 0, 0}, corrected[4] = {0, 0, 0, 0},
+         mcse[4] = {0, 0, 0, 0};
+  double weighted_rss = 0.0, accounting_residual = 0.0, max_complete_residual = 0.0;
+  std::vector<double> fitted_cell, projection_share, residual_share, finite_bias, finite_variance,
+      residual_mass, deleted_mass, cell_weight, draws;
+  std::vector<ColumnReceipt> leverage, target;
+  ColumnReceipt fit;
+  double seconds_total = 0.0, seconds_leverage = 0.0, seconds_target = 0.0;
+  uint64_t vcycles = 0;
+};
+
+bool device_available();
+
+class DeviceEngine {
+ public:
+  DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h);
+  ~DeviceEngine();
+  SolveResult run(const SolveOptions &opt);
+
+  // Diagnostics used by the parity tests (no counterpart in the reference).
+  void atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint32_t columns, bool target,
+             std::vector<int64_t> &out);
+  void vcycle(const std::vector<double> &rhs, uint32_t columns, int precision, std::vector<double> &out);
+  void spmv(uint32_t level, const std::vector<double> &x, uint32_t columns, std::vector<double> &y);
+  // Solve hybrid systems (n x columns, vertex-major) to a relative tolerance.
+  void pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
+           std::vector<double> &x, std::vector<ColumnReceipt> &receipts);
+  double time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
+                               uint64_t &iterations);
+  // Average duration of the dominant kernel over a timed run.
+  struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
+  KernelTiming dominant_kernel_timing() const;
+
+  struct Impl;
+
+ private:
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace vk

@@ -1,0 +1,69 @@
+// Host-side preparation: canonical identifiers, leave-out connected set,
+// compressed problem and the no-control JLA plan.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace vk {
+
+struct Columns {
+  uint64_t rows = 0;
+  const uint64_t *worker = nullptr, *firm = nullptr, *deletion = nullptr;
+  const double *outcome = nullptr;
+  const uint64_t *frequency = nullptr;
+  const double *target = nullptr;
+};
+
+struct SelectionReceipt {
+  uint64_t input_rows = 0, retained_rows = 0, input_physical_mass = 0, retained_physical_mass = 0;
+  uint64_t initial_components = 0, maximum_components = 0, initial_component_rows = 0;
+  uint64_t mover_input_rows = 0, initial_deletion_edges = 0, retained_deletion_edges = 0;
+  uint64_t insufficient_workers_removed = 0, articulation_workers_removed = 0;
+  uint64_t bridge_units_removed = 0, bridge_rows_removed = 0;
+  uint64_t degree_iterations = 0, articulation_iterations = 0, bridge_iterations = 0;
+  uint64_t fixed_point_iterations = 0;
+};
+
+// The compressed problem (problem.rs:647-674) with the JLA plan (jla.rs:20-46)
+// and the gather indices the device kernels read.
+struct Prepared {
+  uint64_t input_rows = 0, input_physical = 0;
+  double input_target_total = 0.0;
+  uint64_t input_topology_checksum = 0;
+  std::vector<uint8_t> mask;
+  SelectionReceipt selection;
+
+  uint64_t rows = 0, workers = 0, firms = 0, cells = 0, units = 0, exact_strata = 0, strata = 0;
+  std::vector<uint32_t> row_worker, row_firm, row_deletion, row_cell;
+  std::vector<double> outcome, target;
+  std::vector<uint64_t> frequency;
+  std::vector<uint32_t> cell_worker, cell_firm;
+  std::vector<double> cell_weight, cell_outcome, cell_target;
+  std::vector<uint64_t> worker_ptr;  // cells of worker w are [ptr[w], ptr[w+1])
+  std::vector<uint64_t> firm_ptr;    // firm_cells[ptr[f]..] in (worker, cell) order
+  std::vector<uint32_t> firm_cells;
+  std::vector<uint64_t> unit_ptr;    // rows of unit g, ascending
+  std::vector<uint32_t> unit_rows;
+  uint64_t physical_total = 0;
+  double target_total = 0.0;
+  uint64_t topology_checksum = 0;
+
+  std::vector<double> per_copy;
+  std::vector<uint64_t> row_rank;
+  std::vector<uint32_t> unit_cell;
+  std::vector<uint64_t> unit_count, unit_rank;
+  std::vector<double> unit_outcome, unit_target;
+  std::vector<uint32_t> stratum_cell;
+  std::vector<double> stratum_per_copy, stratum_target;
+  std::vector<uint64_t> stratum_count, stratum_rank;
+  std::vector<uint64_t> cell_unit_ptr;  // units of a cell, ascending
+  std::vector<uint32_t> cell_units;
+  std::vector<uint64_t> cell_stratum_ptr;  // strata of a cell are contiguous
+};
+
+// Validate, canonicalize, select (unless `mask` is given), compress, plan.
+// Throws vk::Error with the reference status code.
+Prepared prepare(const Columns &cols, const uint8_t *mask);
+
+}  // namespace vk

@@ -1,0 +1,705 @@
+// HIP kernels of the certified lockstep PCG and of the estimator passes
+// around it (right-hand sides, extraction, moments, contraction).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace vk {
+namespace dev {
+
+#define VK_GRID_LOOP(t, count)                                                         \
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (count);      \
+       t += (uint64_t)gridDim.x * blockDim.x)
+
+// Per-column state of the lockstep certified PCG (pcg.rs:1537-1909).
+struct ColState {
+  double rho, alpha, beta, rhs_norm, init_norm, tol, mean_x, mean_r, mean_z, xnorm, rnorm, fresh,
+      orig, proj_norm;
+  int32_t active, status, iters, restarts, need_fresh, restarted, first, pad;
+};
+
+struct PcgParams {
+  uint32_t n;
+  double rel, abs, bound;
+  int32_t max_iterations, interval;
+};
+
+// ---- setup: compatibility check, norms (components.rs:669-744, pcg.rs:1596-1660)
+template <int K>
+__global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__restrict__ partial,
+                            unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm,
+                            const uint8_t *__restrict__ use) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const double v = b[t];
+    acc[0] += v;
+    acc[1] += fabs(v);
+    acc[2] += v * v;
+  }
+  double tot[3];
+  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    const double mean = tot[0] / (double)n;
+    s = ColState{};
+    s.mean_r = mean;
+    s.rhs_norm = sqrt(tot[2]);
+    s.active = use[threadIdx.x] ? 1 : 0;
+    s.first = 1;
+    if (s.active && fabs(tot[0]) > 1e-7 * fmax(tot[1], 1.0)) {
+      s.status = 51;  // incompatible Laplacian right-hand side
+      s.active = 0;
+    }
+  }
+}
+
+// proj = b - mean, x = 0 or centred guess handled on the host, r = proj.
+template <int K>
+__global__ void k_pcg_project(uint32_t n, const double *__restrict__ b, double *__restrict__ proj,
+                              double *__restrict__ r, double *__restrict__ x,
+                              const ColState *__restrict__ st) {
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const int j = t % K;
+    const double p = b[t] - st[j].mean_r;
+    proj[t] = p;
+    r[t] = p;
+    x[t] = 0.0;
+  }
+}
+
+// Initial tolerance test with a zero start: initial residual = ||b||.
+template <int K>
+__global__ void k_pcg_start(ColState *__restrict__ st, PcgParams prm) {
+  const int j = threadIdx.x;
+  if (j >= K) return;
+  ColState &s = st[j];
+  s.init_norm = s.rhs_norm;
+  s.tol = prm.abs + prm.rel * s.rhs_norm;
+  s.mean_r = 0.0;
+  if (s.active && s.init_norm <= s.tol) {
+    s.active = 0;
+    s.iters = 0;
+  }
+}
+
+// Warm start: r = proj - A x (x centred), initial residual of the original rhs.
+template <int K>
+__global__ void k_pcg_warm(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
+                           const double *__restrict__ val, const double *__restrict__ b,
+                           const double *__restrict__ proj, const double *__restrict__ x,
+                           double *__restrict__ r, double *__restrict__ partial,
+                           unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const uint32_t i = t / K;
+    const int j = t % K;
+    const double xi = x[t];
+    const double rr = proj[t] - row_laplacian<double, K>(rp, col, val, x, i, j, xi);
+    r[t] = rr;
+    const double o = rr + (b[t] - proj[t]);
+    acc[0] += o * o;
+    acc[1] += xi * xi;
+    acc[2] += rr;
+    acc[3] += rr * rr;
+  }
+  double tot[4];
+  if (reduce_lanes<K, 4>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    s.init_norm = sqrt(tot[0]);
+    s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * sqrt(tot[1]));
+    s.mean_r = tot[2] / (double)n;
+    if (s.active && s.init_norm <= s.tol) {
+      s.active = 0;
+    } else if (s.active && tot[3] == 0.0) {
+      s.status = 64;
+      s.active = 0;
+    }
+  }
+}
+
+template <int K>
+__global__ void k_center_cols(uint64_t rows, double *__restrict__ v, const double *__restrict__ mean) {
+  VK_GRID_LOOP(t, rows * K) v[t] -= mean[t % K];
+}
+
+// Lane means of the active lanes of the PCG state (zero for finished lanes,
+// whose iterates stay frozen).
+template <int K>
+__global__ void k_state_means(const ColState *__restrict__ st, double *__restrict__ mean_x,
+                              double *__restrict__ mean_r) {
+  const int j = threadIdx.x;
+  if (j >= K) return;
+  mean_x[j] = st[j].active ? st[j].mean_x : 0.0;
+  mean_r[j] = st[j].active ? st[j].mean_r : 0.0;
+}
+
+// Zero the means of lanes that are not part of the current solve.
+template <int K>
+__global__ void k_mask_means(double *__restrict__ mean, const uint8_t *__restrict__ use) {
+  const int j = threadIdx.x;
+  if (j < K && !use[j]) mean[j] = 0.0;
+}
+
+// z centred implicitly: rho = r.z - mean_z * sum r (pcg.rs:1912-1947).
+template <int K>
+__global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double *__restrict__ z,
+                          double *__restrict__ partial, unsigned *__restrict__ counter,
+                          ColState *__restrict__ st) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const double zz = z[t], rr = r[t];
+    acc[0] += zz;
+    acc[1] += rr * zz;
+    acc[2] += rr;
+  }
+  double tot[3];
+  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    if (!s.active) return;
+    s.mean_z = tot[0] / (double)n;
+    const double rho = tot[1] - s.mean_z * tot[2];
+    if (!(rho > 0.0) || !isfinite(rho)) {
+      s.status = 61;  // preconditioner breakdown: r'Mr not positive
+      s.active = 0;
+      return;
+    }
+    if (s.first || s.restarted) {
+      s.beta = 0.0;
+    } else {
+      s.beta = rho / s.rho;
+    }
+    s.rho = rho;
+  }
+}
+
+// p = z_c (first or after a residual replacement) or z_c + beta p.
+template <int K>
+__global__ void k_pcg_direction(uint32_t n, const double *__restrict__ z, double *__restrict__ p,
+                                ColState *__restrict__ st) {
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const int j = t % K;
+    const ColState &s = st[j];
+    if (!s.active) continue;
+    const double zc = z[t] - s.mean_z;
+    p[t] = (s.first || s.restarted) ? zc : zc + s.beta * p[t];
+  }
+}
+
+// Ap = A p and the curvature p'Ap; alpha = rho / p'Ap (pcg.rs:1665-1690).
+template <int K>
+__global__ void k_pcg_curvature(uint32_t n, const uint32_t *__restrict__ rp,
+                                const uint32_t *__restrict__ col, const double *__restrict__ val,
+                                const double *__restrict__ p, double *__restrict__ ap,
+                                double *__restrict__ partial, unsigned *__restrict__ counter,
+                                ColState *__restrict__ st) {
+  double acc[1] = {0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const uint32_t i = t / K;
+    const int j = t % K;
+    const double pi = p[t];
+    const double a = row_laplacian<double, K>(rp, col, val, p, i, j, pi);
+    ap[t] = a;
+    acc[0] += pi * a;
+  }
+  double tot[1];
+  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    s.first = 0;
+    if (!s.active) return;
+    const double curv = tot[0];
+    if (!(curv > 0.0) || !isfinite(curv)) {
+      s.status = 60;
+      s.active = 0;
+      return;
+    }
+    s.alpha = s.rho / curv;
+  }
+}
+
+// x += alpha p, r -= alpha Ap; norms of the centred x and of r; candidate and
+// scheduled residual replacement flags (pcg.rs:1692-1760).
+template <int K>
+__global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const double *__restrict__ ap,
+                             double *__restrict__ x, double *__restrict__ r,
+                             double *__restrict__ partial, unsigned *__restrict__ counter,
+                             ColState *__restrict__ st, PcgParams prm) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const int j = t % K;
+    const ColState &s = st[j];
+    double xv = x[t], rv = r[t];
+    if (s.active) {
+      xv += s.alpha * p[t];
+      rv -= s.alpha * ap[t];
+      x[t] = xv;
+      r[t] = rv;
+    }
+    acc[0] += xv;
+    acc[1] += xv * xv;
+    acc[2] += rv;
+    acc[3] += rv * rv;
+  }
+  double tot[4];
+  if (reduce_lanes<K, 4>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    s.restarted = 0;
+    s.need_fresh = 0;
+    if (!s.active) return;
+    s.iters += 1;
+    s.mean_x = tot[0] / (double)n;
+    s.xnorm = sqrt(fmax(tot[1] - (double)n * s.mean_x * s.mean_x, 0.0));
+    s.mean_r = tot[2] / (double)n;
+    s.rnorm = sqrt(tot[3]);
+    s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * s.xnorm);
+    s.need_fresh = (s.rnorm <= s.tol) || (s.iters % prm.interval == 0);
+  }
+}
+
+// Fresh residual for flagged lanes: r = proj - A x (x already centred), the
+// certificate against the unprojected rhs (pcg.rs:1757-1817, 2209-2236).
+template <int K>
+__global__ void k_pcg_fresh(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
+                            const double *__restrict__ val, const double *__restrict__ b,
+                            const double *__restrict__ proj, const double *__restrict__ x,
+                            double *__restrict__ r, double *__restrict__ partial,
+                            unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)n * K) {
+    const uint32_t i = t / K;
+    const int j = t % K;
+    if (!(st[j].active && st[j].need_fresh)) continue;
+    const double xi = x[t];
+    const double rr = proj[t] - row_laplacian<double, K>(rp, col, val, x, i, j, xi);
+    r[t] = rr;
+    const double o = rr + (b[t] - proj[t]);
+    acc[0] += rr * rr;
+    acc[1] += o * o;
+    acc[2] += rr;
+  }
+  double tot[3];
+  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
+    ColState &s = st[threadIdx.x];
+    if (!(s.active && s.need_fresh)) return;
+    s.restarted = 1;
+    s.restarts += 1;
+    s.fresh = sqrt(tot[0]);
+    s.orig = sqrt(tot[1]);
+    s.mean_r = tot[2] / (double)n;
+    if (s.fresh <= s.tol) {
+      if (s.orig <= s.tol) {
+        s.active = 0;
+        return;
+      }
+      if (s.iters >= prm.max_iterations) {
+        s.status = 64;
+        s.active = 0;
+        return;
+      }
+    }
+    if (s.iters >= prm.max_iterations) {
+      s.status = 63;
+      s.active = 0;
+    }
+  }
+}
+
+template <int K>
+__global__ void k_pcg_limit(ColState *__restrict__ st, PcgParams prm) {
+  const int j = threadIdx.x;
+  if (j >= K) return;
+  ColState &s = st[j];
+  if (s.active && !s.need_fresh && s.iters >= prm.max_iterations) {
+    s.status = 63;
+    s.active = 0;
+  }
+}
+
+// ---------------------------------------------------------------- estimator passes
+
+// Column sums over `rows` rows of a K-lane matrix into out[K] (last block).
+template <int K>
+__global__ void k_lane_mean(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial,
+                            unsigned *__restrict__ counter, double *__restrict__ mean) {
+  double acc[1] = {0.0};
+  VK_GRID_LOOP(t, rows * K) acc[0] += v[t];
+  double tot[1];
+  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
+    mean[threadIdx.x] = tot[0] / (double)rows;
+}
+
+// Exact-integer leverage right-hand sides (engine.rs:2562-2619): per worker
+// and per firm, the sum of the atoms of the units of its cells.
+template <int K>
+__global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
+                           const uint32_t *__restrict__ cells, const uint64_t *__restrict__ cu_ptr,
+                           const uint32_t *__restrict__ cu, const double *__restrict__ atoms,
+                           uint64_t units, uint32_t kcols, double *__restrict__ out) {
+  VK_GRID_LOOP(t, rows * K) {
+    const uint64_t i = t / K;
+    const int j = t % K;
+    double s = 0.0;
+    if (j < (int)kcols) {
+      for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const uint32_t c = cells ? cells[k] : (uint32_t)k;
+        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += atoms[(uint64_t)j * units + cu[q]];
+      }
+    }
+    out[t] = s;
+  }
+}
+
+// Schur right-hand side b_F = firm_rhs - sum_c w_c worker_rhs[w_c] / d_w
+// (operator.rs:269-304), uncentred, plus its column mean.
+template <int K>
+__global__ void k_schur_rhs(uint32_t firms, const uint64_t *__restrict__ fptr,
+                            const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
+                            const double *__restrict__ cell_weight, const double *__restrict__ wdiag,
+                            const double *__restrict__ wrhs, const double *__restrict__ frhs,
+                            double *__restrict__ out, double *__restrict__ partial,
+                            unsigned *__restrict__ counter, double *__restrict__ mean) {
+  double acc[1] = {0.0};
+  VK_GRID_LOOP(t, (uint64_t)firms * K) {
+    const uint32_t f = t / K;
+    const int j = t % K;
+    double v = frhs[t];
+    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
+      const uint32_t c = fcells[k], w = cell_worker[c];
+      v -= cell_weight[c] * (wrhs[(uint64_t)w * K + j] / wdiag[w]);
+    }
+    out[t] = v;
+    acc[0] += v;
+  }
+  double tot[1];
+  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
+    mean[threadIdx.x] = tot[0] / (double)firms;
+}
+
+// wsum[w] = sum_c w_c v[f_c] / d_w  (operator.rs:210-262 first half)
+template <int K>
+__global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ wptr,
+                                const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
+                                const double *__restrict__ wdiag, const double *__restrict__ v,
+                                double *__restrict__ out) {
+  VK_GRID_LOOP(t, workers * K) {
+    const uint64_t w = t / K;
+    const int j = t % K;
+    double s = 0.0;
+    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) s += cell_weight[c] * v[(uint64_t)cell_firm[c] * K + j];
+    out[t] = s / wdiag[w];
+  }
+}
+
+// out[f] = d_f red[f] - sum_c w_c wsum[w_c]  and its column mean.
+template <int K>
+__global__ void k_schur_apply(uint32_t firms, const uint64_t *__restrict__ fptr,
+                              const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
+                              const double *__restrict__ cell_weight, const double *__restrict__ fdiag,
+                              const double *__restrict__ red, const double *__restrict__ wsum,
+                              double *__restrict__ out, double *__restrict__ partial,
+                              unsigned *__restrict__ counter, double *__restrict__ mean) {
+  double acc[1] = {0.0};
+  VK_GRID_LOOP(t, (uint64_t)firms * K) {
+    const uint32_t f = t / K;
+    const int j = t % K;
+    double v = fdiag[f] * red[t];
+    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
+      const uint32_t c = fcells[k];
+      v -= cell_weight[c] * wsum[(uint64_t)cell_worker[c] * K + j];
+    }
+    out[t] = v;
+    acc[0] += v;
+  }
+  double tot[1];
+  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
+    mean[threadIdx.x] = tot[0] / (double)firms;
+}
+
+// reduced relative residual ||rhs_F - (out - mean)|| / ||rhs_F||  (full_cmg.rs:1425-1443)
+template <int K>
+__global__ void k_reduced_residual(uint32_t firms, const double *__restrict__ rhs,
+                                   const double *__restrict__ out, const double *__restrict__ mean,
+                                   double *__restrict__ partial, unsigned *__restrict__ counter,
+                                   double *__restrict__ result) {
+  double acc[2] = {0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)firms * K) {
+    const int j = t % K;
+    const double a = rhs[t] - (out[t] - mean[j]);
+    acc[0] += a * a;
+    acc[1] += rhs[t] * rhs[t];
+  }
+  double tot[2];
+  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K)
+    result[threadIdx.x] = tot[1] == 0.0 ? sqrt(tot[0]) : sqrt(tot[0]) / sqrt(tot[1]);
+}
+
+// worker = (wrhs - sum_c w_c firm[f_c]) / d_w (operator.rs:311-343) and the
+// worker block of the complete residual (operator.rs:386-458).
+template <int K>
+__global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__ wptr,
+                                  const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
+                                  const double *__restrict__ wdiag, const double *__restrict__ wrhs,
+                                  const double *__restrict__ firm, double *__restrict__ worker,
+                                  double *__restrict__ partial, unsigned *__restrict__ counter,
+                                  double *__restrict__ sums) {
+  double acc[2] = {0.0, 0.0};
+  VK_GRID_LOOP(t, workers * K) {
+    const uint64_t w = t / K;
+    const int j = t % K;
+    const double b = wrhs[t];
+    double s = b;
+    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) s -= cell_weight[c] * firm[(uint64_t)cell_firm[c] * K + j];
+    const double x = s / wdiag[w];
+    worker[t] = x;
+    double rw = b - wdiag[w] * x;
+    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) rw -= cell_weight[c] * firm[(uint64_t)cell_firm[c] * K + j];
+    acc[0] += rw * rw;
+    acc[1] += b * b;
+  }
+  double tot[2];
+  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K) {
+    sums[threadIdx.x] = tot[0];
+    sums[K + threadIdx.x] = tot[1];
+  }
+}
+
+// firm block of the complete residual; result = complete relative residual.
+template <int K>
+__global__ void k_firm_residual(uint32_t firms, const uint64_t *__restrict__ fptr,
+                                const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
+                                const double *__restrict__ cell_weight, const double *__restrict__ fdiag,
+                                const double *__restrict__ frhs, const double *__restrict__ firm,
+                                const double *__restrict__ worker, const double *__restrict__ wsums,
+                                double *__restrict__ partial, unsigned *__restrict__ counter,
+                                double *__restrict__ result, double *__restrict__ rhs_norm) {
+  double acc[2] = {0.0, 0.0};
+  VK_GRID_LOOP(t, (uint64_t)firms * K) {
+    const uint32_t f = t / K;
+    const int j = t % K;
+    const double b = frhs[t];
+    double rf = b - fdiag[f] * firm[t];
+    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
+      const uint32_t c = fcells[k];
+      rf -= cell_weight[c] * worker[(uint64_t)cell_worker[c] * K + j];
+    }
+    acc[0] += rf * rf;
+    acc[1] += b * b;
+  }
+  double tot[2];
+  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K) {
+    const int j = threadIdx.x;
+    const double absn = sqrt(fmax(wsums[j] + tot[0], 0.0));
+    const double rhsn = sqrt(fmax(wsums[K + j] + tot[1], 0.0));
+    result[j] = rhsn == 0.0 ? absn : absn / rhsn;
+    rhs_norm[j] = rhsn;
+  }
+}
+
+// FiveMoments accumulation per deletion unit in probe order (engine.rs:1190-1228,
+// 354-374): SoA state m[10][units] = 5 x (sum, correction).
+__device__ inline void neumaier_add(double &s, double &c, double v) {
+  const double next = s + v;
+  c += (fabs(s) >= fabs(v)) ? ((s - next) + v) : ((v - next) + s);
+  s = next;
+}
+
+template <int K>
+__global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__restrict__ unit_cell,
+                          const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ cell_worker,
+                          const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
+                          const double *__restrict__ firm, const double *__restrict__ atoms,
+                          double *__restrict__ m, int *__restrict__ status) {
+  VK_GRID_LOOP(g, units) {
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double root = sqrt((double)unit_count[g]);
+    double s[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) s[q] = m[(uint64_t)q * units + g];
+    for (uint32_t j = 0; j < kcols; ++j) {
+      const double pred = worker[w * K + j] + firm[f * K + j];
+      const double proj = root * pred;
+      const double res = atoms[(uint64_t)j * units + g] / root - proj;
+      if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
+      const double ps = proj * proj, rs = res * res;
+      neumaier_add(s[0], s[1], ps);
+      neumaier_add(s[2], s[3], rs);
+      neumaier_add(s[4], s[5], ps * ps);
+      neumaier_add(s[6], s[7], rs * rs);
+      neumaier_add(s[8], s[9], ps * rs);
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) m[(uint64_t)q * units + g] = s[q];
+  }
+}
+
+// Leverage adjustment per unit (engine.rs:2269-2410), outcome mean 0.
+struct AdjustParams {
+  double probes, block_tolerance, rank_tolerance;
+};
+__global__ void k_adjust(uint64_t units, const double *__restrict__ m, const uint32_t *__restrict__ unit_cell,
+                         const uint64_t *__restrict__ unit_count, const double *__restrict__ unit_outcome,
+                         const double *__restrict__ fitted_cell, AdjustParams prm,
+                         double *__restrict__ out /* 7 x units */, int *__restrict__ status) {
+  VK_GRID_LOOP(g, units) {
+    const double pf = m[0 * units + g] + m[1 * units + g];
+    const double mf = m[2 * units + g] + m[3 * units + g];
+    const double total = (pf + mf) / prm.probes;
+    if (!isfinite(total) || total <= prm.block_tolerance) { atomicCAS(status, 0, 74); continue; }
+    const double proj = (pf / prm.probes) / total;
+    const double res = (mf / prm.probes) / total;
+    const double p2 = (m[4 * units + g] + m[5 * units + g]) / prm.probes;
+    const double m2 = (m[6 * units + g] + m[7 * units + g]) / prm.probes;
+    const double mixed = (m[8 * units + g] + m[9 * units + g]) / prm.probes;
+    const double bias = (res * p2 - proj * m2 + (res - proj) * mixed) / prm.probes;
+    double var = (res * res * p2 + proj * proj * m2 - 2.0 * proj * res * mixed) / prm.probes;
+    if (!isfinite(proj) || !isfinite(res) || !isfinite(bias) || !isfinite(var)) { atomicCAS(status, 0, 73); continue; }
+    if (var < -100.0 * prm.rank_tolerance) { atomicCAS(status, 0, 73); continue; }
+    var = fmax(var, 0.0);
+    if (1.0 - proj <= prm.block_tolerance) { atomicCAS(status, 0, 75); continue; }
+    const double recip = 1.0 / res;
+    const double rr = fabs(res * recip - 1.0);
+    if (!isfinite(recip) || !isfinite(rr) || rr > fmax(100.0 * prm.rank_tolerance, 1e-10)) { atomicCAS(status, 0, 76); continue; }
+    const uint32_t c = unit_cell[g];
+    const double resid = unit_outcome[g] - (double)unit_count[g] * fitted_cell[c];
+    const double mult = recip + bias * (recip * recip) - var * (recip * recip * recip);
+    const double deleted = resid * mult;
+    if (!isfinite(resid) || !isfinite(mult) || !isfinite(deleted)) { atomicCAS(status, 0, 71); continue; }
+    out[0 * units + g] = proj;
+    out[1 * units + g] = res;
+    out[2 * units + g] = bias;
+    out[3 * units + g] = var;
+    out[4 * units + g] = resid;
+    out[5 * units + g] = deleted;
+    out[6 * units + g] = unit_outcome[g] * deleted;
+  }
+}
+
+// cell_correction_weight[c] = compensated sum over the cell's units ascending.
+__global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_ptr, const uint32_t *__restrict__ cu,
+                              const double *__restrict__ contrib, double *__restrict__ out) {
+  VK_GRID_LOOP(c, cells) {
+    double s = 0.0, k = 0.0;
+    for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) neumaier_add(s, k, contrib[cu[q]]);
+    out[c] = s + k;
+  }
+}
+
+// Target direction per cell: compensated sum over its strata of
+// sqrt(per_copy / T) atom (engine.rs:2621-2703), totals for centring.
+template <int KT>
+__global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ csp,
+                                   const double *__restrict__ per_copy, double target_total,
+                                   const double *__restrict__ atoms, uint64_t strata,
+                                   double *__restrict__ dir, double *__restrict__ partial,
+                                   unsigned *__restrict__ counter, double *__restrict__ totals) {
+  double acc[2] = {0.0, 0.0};
+  VK_GRID_LOOP(t, cells * KT) {
+    const uint64_t c = t / KT;
+    const int j = t % KT;
+    double s = 0.0, k = 0.0;
+    for (uint64_t q = csp[c]; q < csp[c + 1]; ++q)
+      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * atoms[(uint64_t)j * strata + q]);
+    const double d = s + k;
+    dir[t] = d;
+    acc[0] += d;
+    acc[1] += fabs(d);
+  }
+  double tot[2];
+  if (reduce_lanes<KT, 2>(acc, partial, counter, tot) && threadIdx.x < KT) {
+    totals[threadIdx.x] = tot[0];
+    totals[KT + threadIdx.x] = tot[1] + fabs(tot[0]);  // reference scale
+  }
+}
+
+// Centre, then transpose to worker/firm scores in the 2-column layout:
+// column 2j = (score_w, 0), column 2j+1 = (0, score_f)  (engine.rs:2705-2971).
+template <int KT>
+__global__ void k_target_scores(uint64_t rows, const uint64_t *__restrict__ ptr,
+                                const uint32_t *__restrict__ cells, const double *__restrict__ dir,
+                                const double *__restrict__ cell_target, double target_total,
+                                const double *__restrict__ totals, int parity, double *__restrict__ out) {
+  constexpr int K2 = 2 * KT;
+  VK_GRID_LOOP(t, rows * K2) {
+    const uint64_t i = t / K2;
+    const int l = t % K2;
+    const int j = l / 2;
+    double s = 0.0;
+    if ((l & 1) == parity) {
+      for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const uint64_t c = cells ? cells[k] : k;
+        s += dir[c * KT + j] - cell_target[c] / target_total * totals[j];
+      }
+    }
+    out[t] = s;
+  }
+}
+
+// Compatibility repair of the last score entry (engine.rs:2842-2881).
+template <int K2>
+__global__ void k_balance(uint64_t rows, double *__restrict__ v, int parity,
+                          const double *__restrict__ totals, double *__restrict__ partial,
+                          unsigned *__restrict__ counter, int *__restrict__ status) {
+  double acc[1] = {0.0};
+  VK_GRID_LOOP(t, (rows - 1) * K2) acc[0] += v[t];
+  double tot[1];
+  if (reduce_lanes<K2, 1>(acc, partial, counter, tot) && threadIdx.x < K2) {
+    const int l = threadIdx.x;
+    if ((l & 1) != parity) return;
+    const double ref = totals[K2 / 2 + l / 2];
+    const double last = -tot[0];
+    const double original = v[(rows - 1) * K2 + l];
+    const double gate = 4096.0 * 2.220446049250313e-16;
+    if ((ref == 0.0 && last != original) || (ref > 0.0 && fabs(last - original) > gate * ref))
+      atomicCAS(status, 0, 77);
+    v[(rows - 1) * K2 + l] = last;
+  }
+}
+
+// Target contraction per probe (engine.rs:3003-3100).
+template <int KT>
+__global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_worker,
+                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ weight,
+                           const double *__restrict__ worker, const double *__restrict__ firm,
+                           double *__restrict__ partial, unsigned *__restrict__ counter,
+                           double *__restrict__ draws) {
+  constexpr int K2 = 2 * KT;
+  double acc[3] = {0.0, 0.0, 0.0};
+  VK_GRID_LOOP(t, cells * KT) {
+    const uint64_t c = t / KT;
+    const int j = t % KT;
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double pw = worker[w * K2 + 2 * j] + firm[f * K2 + 2 * j];
+    const double pf = worker[w * K2 + 2 * j + 1] + firm[f * K2 + 2 * j + 1];
+    const double wt = weight[c];
+    acc[0] += wt * pw * pw;
+    acc[1] += wt * pf * pf;
+    acc[2] += wt * pw * pf;
+  }
+  double tot[3];
+  if (reduce_lanes<KT, 3>(acc, partial, counter, tot) && threadIdx.x < KT) {
+    double *d = draws + 4 * threadIdx.x;
+    d[0] = tot[0];
+    d[1] = tot[1];
+    d[2] = tot[2];
+    d[3] = tot[0] + tot[1] + 2.0 * tot[2];
+  }
+}
+
+// Harmonic lift of auxiliary vertices for a warm start (cmg_impl.rs.in:459-500).
+template <int K>
+__global__ void k_lift(uint32_t firms, uint32_t aux, const uint64_t *__restrict__ aptr,
+                       const uint32_t *__restrict__ anbr, const double *__restrict__ aw,
+                       double *__restrict__ x) {
+  VK_GRID_LOOP(t, (uint64_t)aux * K) {
+    const uint32_t a = t / K;
+    const int j = t % K;
+    double num = 0.0, den = 0.0;
+    for (uint64_t k = aptr[a]; k < aptr[a + 1]; ++k) {
+      num += aw[k] * x[(uint64_t)anbr[k] * K + j];
+      den += aw[k];
+    }
+    x[(uint64_t)(firms + a) * K + j] = num / den;
+  }
+}
+
+}  // namespace dev
+}  // namespace vk

@@ -1,0 +1,649 @@
+// C ABI of the MI355X leave-out engine: the vckss_rust_* entry points of the
+// reference Stata backend for the CMG_FULL_V2 compressed JLA path, plus the
+// vckss_mi355x_* extension and diagnostic entry points.
+//
+// Conventions follow rust/crates/vckss-plugin/src/ffi_engine.rs:
+//   * every call returns an int32 status (error.rs:8-47) and leaves a message
+//     readable through vckss_rust_last_error / vckss_rust_engine_last_error;
+//   * C++ exceptions never cross the boundary (panic containment -> 99);
+//   * one live context per process, identified by a nonzero generation;
+//     release is idempotent (ffi_engine.rs:7146, context.rs:19-120);
+//   * requests are copied and size-checked before use; outputs carry a
+//     capacity that is checked before any write.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/vckss_mi355x.h"
+#include "../../include/vckss_mi355x_ext.h"
+#include "common.hpp"
+#include "device.hpp"
+#include "host_cmg.hpp"
+#include "host_prepare.hpp"
+
+namespace {
+
+using namespace vk;
+
+enum State : uint32_t { kNone = 0, kPrepared = 1, kSolving = 2, kSolved = 3, kFailed = 4, kPoisoned = 5 };
+
+struct Context {
+  uint64_t generation = 0;
+  State state = kNone;
+  Prepared P;
+  HybridSystem HS;
+  Hierarchy H;
+  std::unique_ptr<DeviceEngine> dev;
+  bool has_result = false;
+  SolveResult result;
+  uint32_t precision = 64, leverage_width = 32, target_width = 16;
+};
+
+std::mutex g_mutex;
+std::unique_ptr<Context> g_ctx;
+uint64_t g_next_generation = 1, g_last_released = 0;
+std::map<uint64_t, std::unique_ptr<Hierarchy>> g_hierarchies;
+uint64_t g_next_hierarchy = 1;
+thread_local std::string g_last_error;
+
+template <typename F>
+int32_t guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return kOk;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "allocation failed";
+    return kAllocationFailed;
+  } catch (const std::exception &e) {
+    g_last_error = std::string("internal error: ") + e.what();
+    return kPanic;
+  } catch (...) {
+    g_last_error = "internal error";
+    return kPanic;
+  }
+}
+
+Context &live(uint64_t generation) {
+  if (!g_ctx || generation == 0 || g_ctx->generation != generation)
+    fail(kStaleContext, "engine_context", "generation " + std::to_string(generation) + " is not live");
+  if (g_ctx->state == kPoisoned) fail(kContextPoisoned, "engine_context", "context is poisoned");
+  return *g_ctx;
+}
+
+template <typename T>
+void require_capacity(const void *out, uint64_t capacity, const char *what) {
+  if (!out) fail(kInvalidInput, "engine_abi", std::string(what) + " output pointer is null");
+  if (capacity < sizeof(T)) fail(kAbiMismatch, "engine_abi", std::string(what) + " output capacity is too small");
+}
+
+// ffi_engine.rs:10717-10752: exact binary64 integers, signed ids order-encoded.
+std::vector<uint64_t> integer_column(const double *p, uint64_t rows, const char *label, int32_t code,
+                                     bool allow_signed) {
+  if (!p) fail(kInvalidInput, "engine_ingest", std::string(label) + " column is null");
+  const double lim = 9007199254740992.0;
+  std::vector<uint64_t> out(rows);
+  for (uint64_t r = 0; r < rows; ++r) {
+    const double v = p[r];
+    const bool in_range = allow_signed ? (v >= -lim && v <= lim) : (v > 0.0 && v <= lim);
+    if (!std::isfinite(v) || !in_range || std::trunc(v) != v)
+      fail(code, "engine_ingest", std::string(label) + " must be a " + (allow_signed ? "signed" : "positive") +
+                                      " exact binary64 integer at zero-based row " + std::to_string(r));
+    out[r] = allow_signed ? (static_cast<uint64_t>(static_cast<int64_t>(v)) ^ (1ULL << 63))
+                          : static_cast<uint64_t>(v);
+  }
+  return out;
+}
+
+void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *request,
+                  const VckssEngineColumnsV3 *columns) {
+  if (!request || !columns) fail(kInvalidInput, "engine_prepare", "request or columns pointer is null");
+  VckssEnginePrepareRequestInterruptV3 req;
+  std::memcpy(&req, request, sizeof req);
+  const auto &o = req.options;
+  if (o.v3.v2.abi_version != VCKSS_RUST_ABI_VERSION_V1)
+    fail(kAbiMismatch, "engine_prepare", "unsupported ABI version");
+  if (o.v3.v2.struct_size < sizeof(VckssEnginePrepareRequestV4))
+    fail(kAbiMismatch, "engine_prepare", "prepare request reports a short structure size");
+  if (o.v3.v2.reserved || o.v3.reserved_2 || o.reserved_3 || req.reserved)
+    fail(kAbiMismatch, "engine_prepare", "reserved fields must be zero");
+  if (o.v3.deletion_mode != VCKSS_DELETION_MATCH)
+    fail(kUnsupported, "engine_prepare", "the MI355X engine supports match deletion only");
+  if (o.v3.controls_count != 0 || columns->v2.controls_count != 0)
+    fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
+  if (columns->probeorder_supplied)
+    fail(kUnsupported, "engine_prepare", "probe-order keys are not supported by the MI355X engine");
+  if (o.implicit_match > 1) fail(kInvalidInput, "engine_prepare", "implicit_match must be 0 or 1");
+  if (o.implicit_match)
+    fail(kUnsupported, "engine_prepare", "the certified implicit-match route is not supported by the MI355X engine");
+  const VckssEngineColumnsV1 &c = columns->v2.v1;
+  if (c.rows != o.v3.v2.rows) fail(kInvalidInput, "engine_prepare", "column rows differ from the request");
+  const uint64_t rows = c.rows;
+  if (req.interrupt_poll && req.interrupt_poll(req.interrupt_context) == VCKSS_INTERRUPT_USER_BREAK)
+    fail(kUserBreak, "engine_prepare", "user break");
+  auto worker = integer_column(c.worker, rows, "worker identifier", kInvalidIdentifier, true);
+  auto firm = integer_column(c.firm, rows, "firm identifier", kInvalidIdentifier, true);
+  auto deletion = integer_column(c.deletion, rows, "deletion identifier", kInvalidIdentifier, true);
+  auto frequency = integer_column(c.frequency, rows, "frequency weight", kInvalidWeight, false);
+  if (!c.outcome || !c.target_weight) fail(kInvalidInput, "engine_ingest", "outcome or target column is null");
+  Columns cols{rows, worker.data(), firm.data(), deletion.data(), c.outcome, frequency.data(), c.target_weight};
+  ctx.P = prepare(cols, nullptr);
+  if (req.interrupt_poll && req.interrupt_poll(req.interrupt_context) == VCKSS_INTERRUPT_USER_BREAK)
+    fail(kUserBreak, "engine_prepare", "user break");
+  ctx.HS = build_hybrid(ctx.P);
+  CmgOptions copt;  // full_cmg.rs:467-476: max_levels 128
+  ctx.H = build_hierarchy(ctx.HS.graph, copt);
+}
+
+SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ctx) {
+  const auto &v4 = q.v4;
+  const auto &v3 = v4.v3;
+  const auto &v2 = v3.v2;
+  const auto &v1 = v2.v1;
+  if (v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_solve", "unsupported ABI version");
+  if (v1.struct_size < sizeof(VckssEngineSolveRequestV5))
+    fail(kAbiMismatch, "engine_solve", "V5 solve request reports a short structure size");
+  if (q.reserved_5 != 0 || q.tolerance_supplied > 1 || q.full_cmg_v2 > 1)
+    fail(kAbiMismatch, "engine_solve", "invalid V5 solve request flags or reserved field");
+  if (q.full_cmg_v2 != 1)
+    fail(kUnsupported, "engine_solve", "the MI355X engine implements the CMG_FULL_V2 route only");
+  if (q.threads == 0) fail(kInvalidInput, "cmg_full_v2", "CMG_FULL_V2 requires a positive caller-declared thread count");
+  // ffi_engine.rs:10335-10425 eligibility, restricted to the compressed path
+  const bool eligible =
+      v2.algorithm == VCKSS_ALGORITHM_JLA &&
+      (v3.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED || v3.engine == VCKSS_ENGINE_GENERIC) &&
+      v1.deletion_mode == VCKSS_DELETION_MATCH && v2.nuisance_mode == VCKSS_NUISANCE_JOINT &&
+      v1.rng_contract == VCKSS_RNG_COUNTER_V1 &&
+      (v1.solver_route == VCKSS_ROUTE_AUTO || v1.solver_route == VCKSS_ROUTE_CMG_PCG) &&
+      v3.batch_mode == VCKSS_BATCH_MODE_AUTO && v4.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO &&
+      v4.target_batch_mode == VCKSS_BATCH_MODE_AUTO && v1.leverage_batch_width == 0 &&
+      v1.target_batch_width == 0 && v3.stayers_mode == VCKSS_STAYERS_MOVERS &&
+      (v3.target_weight_mode == VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT ||
+       v3.target_weight_mode == VCKSS_TARGET_WEIGHT_STORED_ROW_EXPLICIT) &&
+      (v3.deletion_unit_source == VCKSS_DELETION_SOURCE_CELL_DEFAULT ||
+       v3.deletion_unit_source == VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT) &&
+      v3.probeorder_supplied == 0 &&
+      (v3.frequency_use == VCKSS_REQUEST_FREQUENCY_UNIT || v3.frequency_use == VCKSS_REQUEST_FREQUENCY_LITERAL);
+  if (!eligible)
+    fail(kUnsupported, "cmg_full_v2",
+         "the MI355X engine requires JLA, auto/generic engine, auto/CMG solver, match deletion, joint nuisance, "
+         "movers, automatic batches and no probe order");
+  if (v1.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+  SolveOptions o;
+  o.seed = v1.seed;
+  o.probes = v1.probes;
+  o.fit_tolerance = v1.pcg_tolerance;
+  o.probe_tolerance = q.tolerance_supplied ? v1.pcg_tolerance : 1.0e-6;  // full_cmg.rs:29
+  for (double t : {o.fit_tolerance, o.probe_tolerance})
+    if (!std::isfinite(t) || t < 1e-15 || t > 1e-4) fail(kInvalidInput, "cmg_full_v2", "tolerance must lie in [1e-15, 1e-4]");
+  o.max_iterations = v1.maximum_iterations;
+  o.recompute_interval = v1.residual_replacement_interval;
+  if (o.max_iterations == 0 || o.recompute_interval == 0)
+    fail(kInvalidInput, "pcg_options", "iteration limit and replacement interval must be positive");
+  o.rank_tolerance = v1.rank_tolerance;
+  o.block_tolerance = v1.block_tolerance;
+  if (!std::isfinite(o.rank_tolerance) || o.rank_tolerance < 1e-14 || o.rank_tolerance >= 0.1)
+    fail(kInvalidInput, "jla_validate", "rank tolerance must be finite and lie in [1e-14, 0.1)");
+  if (!std::isfinite(o.block_tolerance) || o.block_tolerance < 1e-14 || o.block_tolerance >= 1.0)
+    fail(kInvalidInput, "jla_validate", "block tolerance must be finite and lie in [1e-14, 1)");
+  o.precision = static_cast<int>(ctx.precision);
+  o.leverage_width = ctx.leverage_width;
+  o.target_width = ctx.target_width;
+  return o;
+}
+
+void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, VckssInterruptPollV1 poll, void *poll_ctx) {
+  SolveOptions o = solve_options(q, ctx);
+  if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK) fail(kUserBreak, "engine_solve", "user break");
+  if (!ctx.dev) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
+  ctx.state = kSolving;
+  ctx.has_result = false;
+  try {
+    ctx.result = ctx.dev->run(o);
+  } catch (...) {
+    ctx.state = kFailed;
+    throw;
+  }
+  ctx.has_result = true;
+  ctx.state = kSolved;
+}
+
+VckssEngineSolveRequestV5 default_solve_v5() {
+  VckssEngineSolveRequestV5 q;
+  std::memset(&q, 0, sizeof q);
+  auto &v1 = q.v4.v3.v2.v1;
+  v1.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+  v1.struct_size = sizeof(VckssEngineSolveRequestV5);
+  v1.seed = 8675309;  // engine.rs:83-103
+  v1.probes = 200;
+  v1.deletion_mode = VCKSS_DELETION_MATCH;
+  v1.rng_contract = VCKSS_RNG_COUNTER_V1;
+  v1.solver_route = VCKSS_ROUTE_AUTO;
+  v1.exact_dimension_limit = 500;  // solver.rs:42-58
+  v1.cmg_minimum_dimension = 2000;
+  v1.pcg_tolerance = 1e-10;
+  v1.maximum_iterations = 10000;
+  v1.residual_replacement_interval = 100;
+  v1.rank_tolerance = 1e-10;
+  v1.block_tolerance = 1e-10;
+  v1.cmg_terminal_vertices = 128;  // vckss-core/src/cmg/hierarchy.rs:38-55
+  v1.cmg_dense_vertex_cap = 6144;
+  v1.cmg_maximum_levels = 96;
+  v1.cmg_aggregate_cap = 8;
+  v1.cmg_minimum_reduction = 0.20;
+  v1.cmg_jacobi_weight = 2.0 / 3.0;
+  v1.cmg_pre_sweeps = 1;
+  v1.cmg_post_sweeps = 1;
+  v1.cmg_maximum_edge_complexity = 12.0;
+  v1.cmg_maximum_vertex_complexity = 5.0;
+  v1.cmg_memory_limit_bytes = 2ULL << 30;
+  auto &v2 = q.v4.v3.v2;
+  v2.algorithm = VCKSS_ALGORITHM_AUTO;
+  v2.nuisance_mode = VCKSS_NUISANCE_JOINT;
+  v2.exact_estimator_limit = 500;
+  v2.blocksize_limit = 5000;
+  auto &v3 = q.v4.v3;
+  v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
+  v3.batch_mode = VCKSS_BATCH_MODE_AUTO;
+  v3.stayers_mode = VCKSS_STAYERS_MOVERS;
+  v3.target_weight_mode = VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT;
+  v3.deletion_unit_source = VCKSS_DELETION_SOURCE_CELL_DEFAULT;
+  v3.capability_schema = VCKSS_REQUEST_CAPABILITY_SCHEMA_V3;
+  v3.capability_profile = VCKSS_REQUEST_PROFILE_PLANNED_V1;
+  v3.frequency_use = VCKSS_REQUEST_FREQUENCY_UNIT;
+  v3.physical_limit = 50000000;
+  q.v4.leverage_batch_mode = VCKSS_BATCH_MODE_AUTO;
+  q.v4.target_batch_mode = VCKSS_BATCH_MODE_AUTO;
+  q.threads = 1;
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t vckss_rust_abi_version(void) { return VCKSS_RUST_ABI_VERSION_V1; }
+const char *vckss_rust_backend_version(void) { return "vckss-mi355x 0.1.0 (gfx950, CMG_FULL_V2)"; }
+const char *vckss_rust_capabilities_json(void) {
+  return "{\"backend\":\"vckss-mi355x\",\"device\":\"gfx950\",\"routes\":[\"CMG_FULL_V2\"],"
+         "\"deletion\":[\"match\"],\"rng\":[\"VCKSS-COUNTER-V1\"],\"controls\":0,"
+         "\"precision\":[64,32],\"max_batch_rhs\":64}";
+}
+const char *vckss_rust_last_error(void) { return g_last_error.c_str(); }
+const char *vckss_rust_engine_last_error(void) { return g_last_error.c_str(); }
+
+int32_t vckss_rust_selftest(void) {
+  return guarded([] {
+    // A two-vertex hierarchy exercises the host setup; a visible HIP device
+    // is required for every solve.
+    const uint32_t u[1] = {0}, v[1] = {1};
+    const double w[1] = {1.0};
+    Hierarchy h = build_hierarchy(laplacian_from_edges(2, 1, u, v, w), CmgOptions{});
+    if (h.levels.size() != 1 || h.ldl_dim != 1) fail(kInvariant, "selftest", "host hierarchy self-test failed");
+    if (!device_available()) fail(kUnsupported, "selftest", "no HIP device is visible");
+  });
+}
+
+int32_t vckss_rust_engine_default_prepare_request_interrupt_v3(VckssEnginePrepareRequestInterruptV3 *output,
+                                                               uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEnginePrepareRequestInterruptV3>(output, output_capacity_bytes, "prepare request");
+    VckssEnginePrepareRequestInterruptV3 d;
+    std::memset(&d, 0, sizeof d);
+    d.options.v3.v2.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+    d.options.v3.v2.struct_size = sizeof(VckssEnginePrepareRequestV4);
+    d.options.v3.v2.memory_limit_bytes = 4ULL << 30;
+    d.options.v3.deletion_mode = VCKSS_DELETION_MATCH;
+    d.checkpoint_interval = 4096;
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+int32_t vckss_rust_engine_default_solve_request_interrupt_v5(VckssEngineSolveRequestInterruptV5 *output,
+                                                             uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEngineSolveRequestInterruptV5>(output, output_capacity_bytes, "solve request");
+    VckssEngineSolveRequestInterruptV5 d;
+    std::memset(&d, 0, sizeof d);
+    d.options = default_solve_v5();
+    d.checkpoint_interval = 4096;
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+static int32_t prepare_common(const VckssEnginePrepareRequestInterruptV3 *request,
+                              const VckssEngineColumnsV3 *columns, uint64_t *output_handle, bool upload) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!output_handle) fail(kInvalidInput, "engine_prepare", "output handle pointer is null");
+    *output_handle = 0;
+    if (g_ctx) {
+      if (request && request->options.v3.v2.cleanup_abandoned) {
+        g_last_released = g_ctx->generation;
+        g_ctx.reset();
+      } else {
+        fail(kStaleContext, "engine_prepare", "a prepared context is still live; release it first");
+      }
+    }
+    std::unique_ptr<Context> ctx(new Context);
+    prepare_into(*ctx, request, columns);
+    if (upload) ctx->dev.reset(new DeviceEngine(ctx->P, ctx->HS, ctx->H));
+    ctx->generation = g_next_generation++;
+    ctx->state = kPrepared;
+    *output_handle = ctx->generation;
+    g_ctx = std::move(ctx);
+  });
+}
+
+int32_t vckss_rust_engine_prepare_interrupt_v4(const VckssEnginePrepareRequestInterruptV3 *request,
+                                               const VckssEngineColumnsV3 *columns, uint64_t *output_handle,
+                                               uint32_t output_capacity_bytes) {
+  if (output_capacity_bytes < sizeof(uint64_t)) {
+    g_last_error = "engine_prepare: output handle capacity is too small";
+    return kAbiMismatch;
+  }
+  return prepare_common(request, columns, output_handle, true);
+}
+
+int32_t vckss_mi355x_prepare_host_v1(const VckssEnginePrepareRequestInterruptV3 *request,
+                                     const VckssEngineColumnsV3 *columns, uint64_t *output_handle) {
+  return prepare_common(request, columns, output_handle, false);
+}
+
+int32_t vckss_rust_engine_retained_mask_v1(uint64_t generation, uint8_t *output, uint64_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!output) fail(kInvalidInput, "engine_mask", "output pointer is null");
+    if (output_capacity_bytes < ctx.P.input_rows) fail(kAbiMismatch, "engine_mask", "output capacity is too small");
+    std::memcpy(output, ctx.P.mask.data(), ctx.P.input_rows);
+  });
+}
+
+int32_t vckss_rust_engine_solve_v5(uint64_t generation, const VckssEngineSolveRequestV5 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestV5 q;
+    std::memcpy(&q, request, sizeof q);
+    run_solve(live(generation), q, nullptr, nullptr);
+  });
+}
+
+int32_t vckss_rust_engine_solve_interrupt_v5(uint64_t generation, const VckssEngineSolveRequestInterruptV5 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestInterruptV5 q;
+    std::memcpy(&q, request, sizeof q);
+    if (q.reserved) fail(kAbiMismatch, "engine_solve", "reserved fields must be zero");
+    run_solve(live(generation), q.options, q.interrupt_poll, q.interrupt_context);
+  });
+}
+
+int32_t vckss_rust_engine_result_v1(uint64_t generation, VckssEngineResultV1 *output, uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    require_capacity<VckssEngineResultV1>(output, output_capacity_bytes, "result");
+    if (!ctx.has_result) fail(kStaleContext, "engine_result", "no solved result is available");
+    const SolveResult &r = ctx.result;
+    // ffi_engine.rs:6009-6011: accounting re-verified before export
+    if (r.accounting_residual > 1e-10) fail(kAccounting, "engine_result", "accounting identity failed");
+    VckssEngineResultV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.generation = generation;
+    auto put = [](VckssComponentVectorV1 &v, const double *c) {
+      v.worker = c[0]; v.firm = c[1]; v.covariance = c[2]; v.total = c[3];
+    };
+    put(o.plugin, r.plugin);
+    put(o.correction, r.correction);
+    put(o.corrected, r.corrected);
+    put(o.numerical_mcse, r.mcse);
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+int32_t vckss_rust_engine_release_v1(uint64_t generation) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (generation == 0) fail(kStaleContext, "engine_release", "generation 0 is never live");
+    if (g_ctx && g_ctx->generation == generation) {
+      g_last_released = generation;
+      g_ctx.reset();
+      return;
+    }
+    if (generation <= g_last_released || generation < g_next_generation) return;  // idempotent
+    fail(kStaleContext, "engine_release", "unknown generation");
+  });
+}
+
+int32_t vckss_rust_engine_clear_abandoned_v1(void) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (g_ctx) {
+      g_last_released = g_ctx->generation;
+      g_ctx.reset();
+    }
+  });
+}
+
+int32_t vckss_rust_engine_snapshot_v1(VckssEngineSnapshotV1 *output, uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssEngineSnapshotV1>(output, output_capacity_bytes, "snapshot");
+    VckssEngineSnapshotV1 s;
+    s.struct_size = sizeof s;
+    s.state = g_ctx ? g_ctx->state : kNone;
+    s.generation = g_ctx ? g_ctx->generation : 0;
+    s.last_released_generation = g_last_released;
+    std::memcpy(output, &s, sizeof s);
+  });
+}
+
+// ------------------------------------------------------------------ extensions
+
+int32_t vckss_mi355x_configure_v1(uint64_t generation, uint32_t precision, uint32_t leverage_width,
+                                  uint32_t target_width) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (precision != 64 && precision != 32) fail(kInvalidInput, "configure", "precision must be 64 or 32");
+    if (leverage_width < 1 || leverage_width > 64) fail(kInvalidInput, "configure", "leverage width must lie in [1, 64]");
+    if (target_width < 1 || target_width > 32) fail(kInvalidInput, "configure", "target width must lie in [1, 32]");
+    ctx.precision = precision;
+    ctx.leverage_width = leverage_width;
+    ctx.target_width = target_width;
+  });
+}
+
+int32_t vckss_mi355x_dims_v1(uint64_t generation, uint64_t *dims, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!dims || capacity < 15) fail(kAbiMismatch, "dims", "capacity must be at least 15");
+    const Prepared &P = ctx.P;
+    uint64_t v[15] = {P.input_rows, P.rows, P.physical_total, P.workers, P.firms, P.cells, P.units,
+                      P.exact_strata, P.strata, ctx.HS.graph.n, ctx.HS.graph.u.size(), ctx.HS.aux,
+                      ctx.H.levels.size(), ctx.H.ldl_dim, P.topology_checksum};
+    std::memcpy(dims, v, sizeof v);
+  });
+}
+
+int32_t vckss_mi355x_array_v1(uint64_t generation, uint32_t which, double *out, uint64_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    std::vector<double> tmp;
+    const std::vector<double> *src = &tmp;
+    const Prepared &P = ctx.P;
+    if (which < 20) {
+      if (!ctx.has_result) fail(kStaleContext, "array", "no solved result is available");
+      const SolveResult &r = ctx.result;
+      switch (which) {
+        case 0: src = &r.fitted_cell; break;
+        case 1: src = &r.projection_share; break;
+        case 2: src = &r.residual_share; break;
+        case 3: src = &r.finite_bias; break;
+        case 4: src = &r.finite_variance; break;
+        case 5: src = &r.residual_mass; break;
+        case 6: src = &r.deleted_mass; break;
+        case 7: src = &r.cell_weight; break;
+        case 8: src = &r.draws; break;
+        case 9: for (auto &c : r.leverage) tmp.push_back(c.iterations); break;
+        case 10: for (auto &c : r.target) tmp.push_back(c.iterations); break;
+        case 11: for (auto &c : r.leverage) tmp.push_back(c.complete); break;
+        case 12: tmp = {r.seconds_total, r.seconds_leverage, r.seconds_target, (double)r.vcycles}; break;
+        default: fail(kInvalidInput, "array", "unknown array");
+      }
+    } else {
+      switch (which) {
+        case 20: tmp.assign(P.unit_rank.begin(), P.unit_rank.end()); break;
+        case 21: tmp.assign(P.unit_count.begin(), P.unit_count.end()); break;
+        case 22: tmp.assign(P.stratum_rank.begin(), P.stratum_rank.end()); break;
+        case 23: tmp.assign(P.stratum_count.begin(), P.stratum_count.end()); break;
+        case 24: src = &P.cell_weight; break;
+        case 25: tmp.assign(P.unit_cell.begin(), P.unit_cell.end()); break;
+        default: fail(kInvalidInput, "array", "unknown array");
+      }
+    }
+    if (!out || capacity < src->size()) fail(kAbiMismatch, "array", "capacity " + std::to_string(capacity) +
+                                                                        " < " + std::to_string(src->size()));
+    std::memcpy(out, src->data(), src->size() * sizeof(double));
+  });
+}
+
+static DeviceEngine &device_of(Context &ctx) {
+  if (!ctx.dev) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
+  return *ctx.dev;
+}
+
+int32_t vckss_mi355x_atoms_v1(uint64_t generation, uint64_t seed, uint32_t domain, uint64_t first_probe,
+                              uint32_t columns, int64_t *out, uint64_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (domain > 1 || columns == 0) fail(kInvalidInput, "atoms", "invalid domain or column count");
+    std::vector<int64_t> a;
+    device_of(ctx).atoms(seed, domain ? 0x0054415247455401ULL : 0x4c45564552414745ULL, first_probe, columns,
+                         domain == 1, a);
+    if (!out || capacity < a.size()) fail(kAbiMismatch, "atoms", "capacity is too small");
+    std::memcpy(out, a.data(), a.size() * sizeof(int64_t));
+  });
+}
+
+int32_t vckss_mi355x_spmv_v1(uint64_t generation, uint32_t level, uint32_t columns, const double *x, double *y) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (level >= ctx.H.levels.size()) fail(kInvalidInput, "spmv", "level out of range");
+    const uint64_t m = ctx.H.levels[level].graph.n * (uint64_t)columns;
+    std::vector<double> in(x, x + m), outv;
+    device_of(ctx).spmv(level, in, columns, outv);
+    std::memcpy(y, outv.data(), m * sizeof(double));
+  });
+}
+
+int32_t vckss_mi355x_vcycle_v1(uint64_t generation, uint32_t columns, uint32_t precision, const double *r, double *z) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    const uint64_t m = ctx.H.levels[0].graph.n * (uint64_t)columns;
+    std::vector<double> in(r, r + m), outv;
+    device_of(ctx).vcycle(in, columns, (int)precision, outv);
+    std::memcpy(z, outv.data(), m * sizeof(double));
+  });
+}
+
+int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolerance, uint32_t precision,
+                            const double *rhs, double *x, uint32_t *iterations) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    const uint64_t m = ctx.H.levels[0].graph.n * (uint64_t)columns;
+    std::vector<double> in(rhs, rhs + m), outv;
+    std::vector<ColumnReceipt> rc;
+    device_of(ctx).pcg(in, columns, tolerance, (int)precision, outv, rc);
+    std::memcpy(x, outv.data(), m * sizeof(double));
+    for (uint32_t j = 0; j < columns; ++j) iterations[j] = rc[j].iterations;
+  });
+}
+
+int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint32_t precision, double tolerance,
+                                      uint32_t batches, double *seconds, uint64_t *solves, uint64_t *iterations,
+                                      double *kernel_ms, double *kernel_bytes, uint64_t *kernel_launches) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    SolveOptions o;
+    o.leverage_width = width;
+    o.precision = (int)precision;
+    o.probe_tolerance = tolerance;
+    DeviceEngine &d = device_of(ctx);
+    *seconds = d.time_leverage_batches(o, batches, *solves, *iterations);
+    auto t = d.dominant_kernel_timing();
+    *kernel_ms = t.ms_per_launch;
+    *kernel_bytes = t.bytes_per_launch;
+    *kernel_launches = t.launches;
+  });
+}
+
+int32_t vckss_mi355x_hierarchy_build_v1(uint64_t vertices, uint64_t edges, const uint32_t *u, const uint32_t *v,
+                                        const double *w, uint64_t direct_threshold, uint64_t max_levels,
+                                        uint64_t *handle) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!handle || !u || !v || !w) fail(kInvalidInput, "hierarchy", "null pointer");
+    CmgOptions o;
+    o.direct_threshold = direct_threshold;
+    o.max_levels = max_levels;
+    if (!o.direct_threshold || !o.max_levels) fail(kInvalidInput, "hierarchy", "invalid options");
+    Laplacian g = laplacian_from_edges(vertices, edges, u, v, w);
+    std::unique_ptr<Hierarchy> h(new Hierarchy(build_hierarchy(g, o)));
+    *handle = g_next_hierarchy++;
+    g_hierarchies[*handle] = std::move(h);
+  });
+}
+
+int32_t vckss_mi355x_hierarchy_levels_v1(uint64_t handle, uint32_t *levels, uint64_t *factor_nnz) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    auto it = g_hierarchies.find(handle);
+    if (it == g_hierarchies.end()) fail(kStaleContext, "hierarchy", "unknown handle");
+    *levels = (uint32_t)it->second->levels.size();
+    *factor_nnz = it->second->factor_nnz;
+  });
+}
+
+int32_t vckss_mi355x_hierarchy_level_v1(uint64_t handle, uint32_t level, uint64_t *info, uint32_t *labels,
+                                        uint32_t *eu, uint32_t *ev, double *ew, double *diag) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    auto it = g_hierarchies.find(handle);
+    if (it == g_hierarchies.end()) fail(kStaleContext, "hierarchy", "unknown handle");
+    const Hierarchy &H = *it->second;
+    if (level >= H.levels.size()) fail(kInvalidInput, "hierarchy", "level out of range");
+    const Level &L = H.levels[level];
+    uint64_t v6[6] = {L.graph.n, L.graph.u.size(), L.graph.nnz, L.repeat, (uint64_t)L.terminal, L.coarse};
+    std::memcpy(info, v6, sizeof v6);
+    if (labels && !L.labels.empty()) std::memcpy(labels, L.labels.data(), L.labels.size() * 4);
+    if (eu) std::memcpy(eu, L.graph.u.data(), L.graph.u.size() * 4);
+    if (ev) std::memcpy(ev, L.graph.v.data(), L.graph.v.size() * 4);
+    if (ew) std::memcpy(ew, L.graph.w.data(), L.graph.w.size() * 8);
+    if (diag) std::memcpy(diag, L.graph.diag.data(), L.graph.diag.size() * 8);
+  });
+}
+
+int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] { g_hierarchies.erase(handle); });
+}
+
+}  // extern "C"

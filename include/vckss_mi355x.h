@@ -1,0 +1,279 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * C ABI of the MI355X leave-out engine (libvckss_mi355x.so).
+ *
+ * The engine is a drop-in for the compressed no-control match-deletion
+ * improved-JLA path of the fevc Rust backend (route CMG_FULL_V2).  Every
+ * declaration below keeps the name, argument meaning, struct layout and status
+ * codes of the reference interface it replaces, cited as
+ * rust/stata_backend/include/vckss_rust.h:<line>; the implementation lives in
+ * fevc_amd/csrc/vckss_abi.cpp.  Requests outside this path are refused with
+ * VCKSS_STATUS_UNSUPPORTED (11), as the reference refuses unsupported features.
+ */
+#ifndef VCKSS_MI355X_H
+#define VCKSS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants: vckss_rust.h:16-146 */
+#define VCKSS_RUST_ABI_VERSION_V1 1u
+#define VCKSS_DELETION_MATCH 1u
+#define VCKSS_DELETION_OBSERVATION 2u
+#define VCKSS_RNG_NONE 0u
+#define VCKSS_RNG_COUNTER_V1 1u
+#define VCKSS_ROUTE_AUTO 0u
+#define VCKSS_ROUTE_EXACT 1u
+#define VCKSS_ROUTE_DIAGONAL_PCG 2u
+#define VCKSS_ROUTE_CMG_PCG 3u
+#define VCKSS_INTERRUPT_CONTINUE 0
+#define VCKSS_INTERRUPT_USER_BREAK 1
+#define VCKSS_ALGORITHM_AUTO 0u
+#define VCKSS_ALGORITHM_EXACT 1u
+#define VCKSS_ALGORITHM_JLA 2u
+#define VCKSS_NUISANCE_JOINT 1u
+#define VCKSS_NUISANCE_FIXED_OFFSET 2u
+#define VCKSS_ENGINE_AUTO_OR_UNSPECIFIED 0u
+#define VCKSS_ENGINE_COMPRESSED 1u
+#define VCKSS_ENGINE_GENERIC 2u
+#define VCKSS_BATCH_MODE_AUTO 0u
+#define VCKSS_BATCH_MODE_EXPLICIT 1u
+#define VCKSS_STAYERS_MOVERS 1u
+#define VCKSS_STAYERS_ALL 2u
+#define VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT 0u
+#define VCKSS_TARGET_WEIGHT_STORED_ROW_EXPLICIT 1u
+#define VCKSS_DELETION_SOURCE_CELL_DEFAULT 1u
+#define VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT 2u
+#define VCKSS_DELETION_SOURCE_OBSERVATION_ROW 3u
+#define VCKSS_REQUEST_CAPABILITY_SCHEMA_V3 3u
+#define VCKSS_REQUEST_PROFILE_PLANNED_V1 4u
+#define VCKSS_REQUEST_FREQUENCY_UNIT 0u
+#define VCKSS_REQUEST_FREQUENCY_LITERAL 1u
+#define VCKSS_CORE_MATCH_GRAPH_READY (UINT64_C(1) << 0)
+#define VCKSS_CORE_CMG_GRAPH_READY (UINT64_C(1) << 4)
+#define VCKSS_CORE_COUNTER_RNG_READY (UINT64_C(1) << 6)
+#define VCKSS_CORE_JLA_PLAN_READY (UINT64_C(1) << 7)
+#define VCKSS_CORE_FULL_CMG_V2_READY (UINT64_C(1) << 8)
+
+/* Status codes: rust/crates/vckss-core/src/error.rs:8-47 */
+#define VCKSS_STATUS_OK 0
+#define VCKSS_STATUS_USER_BREAK 1
+#define VCKSS_STATUS_ABI_MISMATCH 10
+#define VCKSS_STATUS_UNSUPPORTED 11
+#define VCKSS_STATUS_INVALID_INPUT 20
+#define VCKSS_STATUS_STALE_CONTEXT 80
+#define VCKSS_STATUS_PANIC 99
+
+typedef int32_t (*VckssInterruptPollV1)(void *context); /* vckss_rust.h:292 */
+
+/* vckss_rust.h:269-296 */
+typedef struct VckssEnginePrepareRequestV2 {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    uint64_t rows;
+    uint32_t cleanup_abandoned;
+    uint32_t reserved;
+    uint64_t memory_limit_bytes;
+    uint64_t caller_copy_bytes;
+} VckssEnginePrepareRequestV2;
+
+typedef struct VckssEnginePrepareRequestV3 {
+    VckssEnginePrepareRequestV2 v2;
+    uint32_t deletion_mode;
+    uint32_t controls_count;
+    uint64_t reserved_2;
+} VckssEnginePrepareRequestV3;
+
+typedef struct VckssEnginePrepareRequestV4 {
+    VckssEnginePrepareRequestV3 v3;
+    uint32_t implicit_match;
+    uint32_t reserved_3;
+} VckssEnginePrepareRequestV4;
+
+/* vckss_rust.h:335-341 */
+typedef struct VckssEnginePrepareRequestInterruptV3 {
+    VckssEnginePrepareRequestV4 options;
+    VckssInterruptPollV1 interrupt_poll;
+    void *interrupt_context;
+    uint32_t checkpoint_interval;
+    uint32_t reserved;
+} VckssEnginePrepareRequestInterruptV3;
+
+/* vckss_rust.h:343-367: identifiers are exact binary64 integers */
+typedef struct VckssEngineColumnsV1 {
+    uint32_t struct_size;
+    uint32_t reserved;
+    uint64_t rows;
+    const double *worker;
+    const double *firm;
+    const double *deletion;
+    const double *outcome;
+    const double *frequency;
+    const double *target_weight;
+} VckssEngineColumnsV1;
+
+typedef struct VckssEngineColumnsV2 {
+    VckssEngineColumnsV1 v1;
+    const double *const *controls;
+    uint32_t controls_count;
+    uint32_t reserved_2;
+} VckssEngineColumnsV2;
+
+typedef struct VckssEngineColumnsV3 {
+    VckssEngineColumnsV2 v2;
+    const double *probe_order;
+    uint32_t probeorder_supplied;
+    uint32_t reserved_3;
+} VckssEngineColumnsV3;
+
+/* vckss_rust.h:501-575 */
+typedef struct VckssEngineSolveRequestV1 {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    uint64_t seed;
+    uint32_t probes;
+    uint32_t leverage_batch_width;
+    uint32_t target_batch_width;
+    uint32_t deletion_mode;
+    uint32_t rng_contract;
+    uint32_t solver_route;
+    uint32_t allow_automatic_cmg_setup_fallback;
+    uint64_t exact_dimension_limit;
+    uint64_t cmg_minimum_dimension;
+    double pcg_tolerance;
+    uint32_t maximum_iterations;
+    uint32_t residual_replacement_interval;
+    double rank_tolerance;
+    double block_tolerance;
+    uint64_t cmg_terminal_vertices;
+    uint64_t cmg_dense_vertex_cap;
+    uint64_t cmg_maximum_levels;
+    uint64_t cmg_aggregate_cap;
+    double cmg_minimum_reduction;
+    double cmg_jacobi_weight;
+    uint32_t cmg_pre_sweeps;
+    uint32_t cmg_post_sweeps;
+    double cmg_maximum_edge_complexity;
+    double cmg_maximum_vertex_complexity;
+    uint64_t cmg_memory_limit_bytes;
+} VckssEngineSolveRequestV1;
+
+typedef struct VckssEngineSolveRequestV2 {
+    VckssEngineSolveRequestV1 v1;
+    uint32_t algorithm;
+    uint32_t nuisance_mode;
+    uint64_t exact_estimator_limit;
+    uint64_t blocksize_limit;
+} VckssEngineSolveRequestV2;
+
+typedef struct VckssEngineSolveRequestV3 {
+    VckssEngineSolveRequestV2 v2;
+    uint32_t engine;
+    uint32_t batch_mode;
+    uint32_t stayers_mode;
+    uint32_t target_weight_mode;
+    uint32_t deletion_unit_source;
+    uint32_t probeorder_supplied;
+    uint32_t wallseconds_supplied;
+    uint32_t capability_schema;
+    uint32_t capability_profile;
+    uint32_t frequency_use;
+    uint64_t physical_limit;
+    uint64_t request_signature;
+    uint64_t reserved_3;
+} VckssEngineSolveRequestV3;
+
+typedef struct VckssEngineSolveRequestV4 {
+    VckssEngineSolveRequestV3 v3;
+    uint32_t leverage_batch_mode;
+    uint32_t target_batch_mode;
+    double wallseconds;
+    uint64_t reserved_4;
+} VckssEngineSolveRequestV4;
+
+typedef struct VckssEngineSolveRequestV5 {
+    VckssEngineSolveRequestV4 v4;
+    uint32_t threads;
+    uint32_t tolerance_supplied;
+    uint32_t full_cmg_v2;
+    uint32_t reserved_5;
+} VckssEngineSolveRequestV5;
+
+/* vckss_rust.h:630-636 */
+typedef struct VckssEngineSolveRequestInterruptV5 {
+    VckssEngineSolveRequestV5 options;
+    VckssInterruptPollV1 interrupt_poll;
+    void *interrupt_context;
+    uint32_t checkpoint_interval;
+    uint32_t reserved;
+} VckssEngineSolveRequestInterruptV5;
+
+/* vckss_rust.h:1037-1052 */
+typedef struct VckssComponentVectorV1 {
+    double worker;
+    double firm;
+    double covariance;
+    double total;
+} VckssComponentVectorV1;
+
+typedef struct VckssEngineResultV1 {
+    uint32_t struct_size;
+    uint32_t reserved;
+    uint64_t generation;
+    VckssComponentVectorV1 plugin;
+    VckssComponentVectorV1 correction;
+    VckssComponentVectorV1 corrected;
+    VckssComponentVectorV1 numerical_mcse;
+} VckssEngineResultV1;
+
+/* vckss_rust.h:1635-1640 */
+typedef struct VckssEngineSnapshotV1 {
+    uint32_t struct_size;
+    uint32_t state;
+    uint64_t generation;
+    uint64_t last_released_generation;
+} VckssEngineSnapshotV1;
+
+/* vckss_rust.h:1642-1646 */
+uint32_t vckss_rust_abi_version(void);
+const char *vckss_rust_backend_version(void);
+const char *vckss_rust_capabilities_json(void);
+const char *vckss_rust_last_error(void);
+int32_t vckss_rust_selftest(void);
+
+/* vckss_rust.h:1713-1716 */
+int32_t vckss_rust_engine_default_prepare_request_interrupt_v3(
+    VckssEnginePrepareRequestInterruptV3 *output, uint32_t output_capacity_bytes);
+/* vckss_rust.h:1745-1748 */
+int32_t vckss_rust_engine_default_solve_request_interrupt_v5(
+    VckssEngineSolveRequestInterruptV5 *output, uint32_t output_capacity_bytes);
+/* vckss_rust.h:1799-1804: canonicalise, select the leave-out connected set,
+ * compress, build the JLA plan, the hybrid Laplacian and its CMG hierarchy,
+ * upload to the GPU.  *output_handle receives a nonzero generation. */
+int32_t vckss_rust_engine_prepare_interrupt_v4(
+    const VckssEnginePrepareRequestInterruptV3 *request, const VckssEngineColumnsV3 *columns,
+    uint64_t *output_handle, uint32_t output_capacity_bytes);
+/* vckss_rust.h:1928-1932 */
+int32_t vckss_rust_engine_retained_mask_v1(uint64_t generation, uint8_t *output,
+                                           uint64_t output_capacity_bytes);
+/* vckss_rust.h:1949-1952 and 1969-1972: the per-probe path on the GPU */
+int32_t vckss_rust_engine_solve_v5(uint64_t generation, const VckssEngineSolveRequestV5 *request);
+int32_t vckss_rust_engine_solve_interrupt_v5(uint64_t generation,
+                                             const VckssEngineSolveRequestInterruptV5 *request);
+/* vckss_rust.h:1973-1977 */
+int32_t vckss_rust_engine_result_v1(uint64_t generation, VckssEngineResultV1 *output,
+                                    uint32_t output_capacity_bytes);
+/* vckss_rust.h:2136-2142 */
+int32_t vckss_rust_engine_release_v1(uint64_t generation);
+int32_t vckss_rust_engine_clear_abandoned_v1(void);
+int32_t vckss_rust_engine_snapshot_v1(VckssEngineSnapshotV1 *output, uint32_t output_capacity_bytes);
+const char *vckss_rust_engine_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

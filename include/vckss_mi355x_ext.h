@@ -1,0 +1,91 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * Extension and diagnostic entry points of libvckss_mi355x.so.
+ *
+ * These have no counterpart in the reference ABI.  They exist for callers
+ * that tune the GPU batch shape and precision, and for the parity tests that
+ * compare individual device stages (Counter-V1 atoms, level products, V-cycle,
+ * PCG) and the host CMG hierarchy with the CPU oracle.  Status codes and the
+ * thread-local error string follow the vckss_rust_* conventions.
+ */
+#ifndef VCKSS_MI355X_EXT_H
+#define VCKSS_MI355X_EXT_H
+
+#include <stdint.h>
+
+#include "vckss_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GPU execution shape for the next solve of `generation`:
+ * precision 64 (FP64 V-cycle) or 32 (FP32 V-cycle, FP64 Krylov);
+ * probes per leverage batch (1..64) and per target batch (1..32). */
+int32_t vckss_mi355x_configure_v1(uint64_t generation, uint32_t precision,
+                                  uint32_t leverage_width, uint32_t target_width);
+
+/* dims: input_rows, retained_rows, physical, workers, firms, cells, units,
+ * exact_strata, plan_strata, hybrid_vertices, hybrid_edges, auxiliary,
+ * levels, terminal_dimension, topology_checksum */
+int32_t vckss_mi355x_dims_v1(uint64_t generation, uint64_t *dims, uint32_t capacity);
+
+/* Solved arrays of the last solve: 0 fitted_cell 1 projection_share
+ * 2 residual_share 3 finite_bias 4 finite_variance 5 residual_mass
+ * 6 deleted_mass 7 cell_correction_weight 8 target_draws (4 per probe)
+ * 9 leverage iterations per probe 10 target iterations per column
+ * 11 leverage complete residual per probe; 12 solve seconds (total,
+ * leverage, target) and V-cycles applied. */
+int32_t vckss_mi355x_array_v1(uint64_t generation, uint32_t which, double *out,
+                              uint64_t capacity);
+
+/* Counter-V1 atoms on the device: domain 0 leverage (deletion units),
+ * 1 target (strata); out[col * entities + row]. */
+int32_t vckss_mi355x_atoms_v1(uint64_t generation, uint64_t seed, uint32_t domain,
+                              uint64_t first_probe, uint32_t columns, int64_t *out,
+                              uint64_t capacity);
+
+/* y = A_level x for `columns` interleaved lanes (FP64). */
+int32_t vckss_mi355x_spmv_v1(uint64_t generation, uint32_t level, uint32_t columns,
+                             const double *x, double *y);
+
+/* One V-cycle z = M r on the finest hybrid level. */
+int32_t vckss_mi355x_vcycle_v1(uint64_t generation, uint32_t columns, uint32_t precision,
+                               const double *r, double *z);
+
+/* Certified lockstep PCG on the finest hybrid level; iterations[columns]. */
+int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolerance,
+                            uint32_t precision, const double *rhs, double *x,
+                            uint32_t *iterations);
+
+/* Timed leverage batches (atoms, rhs, solve, extraction, moments) with the
+ * dominant kernel's average duration measured by HIP events. */
+int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint32_t precision,
+                                      double tolerance, uint32_t batches, double *seconds,
+                                      uint64_t *solves, uint64_t *iterations,
+                                      double *kernel_ms, double *kernel_bytes,
+                                      uint64_t *kernel_launches);
+
+/* Host CMG hierarchy of an arbitrary graph (no GPU needed). */
+int32_t vckss_mi355x_hierarchy_build_v1(uint64_t vertices, uint64_t edges, const uint32_t *u,
+                                        const uint32_t *v, const double *w,
+                                        uint64_t direct_threshold, uint64_t max_levels,
+                                        uint64_t *handle);
+/* info: n, edges, nnz, repeat, terminal (0 none, 1 direct, 2.. iterative), coarse */
+int32_t vckss_mi355x_hierarchy_level_v1(uint64_t handle, uint32_t level, uint64_t *info,
+                                        uint32_t *labels, uint32_t *eu, uint32_t *ev,
+                                        double *ew, double *diag);
+int32_t vckss_mi355x_hierarchy_levels_v1(uint64_t handle, uint32_t *levels, uint64_t *factor_nnz);
+int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle);
+
+/* Host preparation only (no GPU): same as the prepare entry point but stops
+ * before device upload; the generation can be queried with dims/arrays of
+ * the prepared problem (which 20..): 20 unit_rank 21 unit_count
+ * 22 stratum_rank 23 stratum_count 24 cell_weight 25 unit_cell. */
+int32_t vckss_mi355x_prepare_host_v1(const VckssEnginePrepareRequestInterruptV3 *request,
+                                     const VckssEngineColumnsV3 *columns, uint64_t *output_handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,180 @@
+"""Host side of the product against the oracle (no GPU needed).
+
+* every entry point declared in include/*.h is exported by the library;
+* preparation (selection, compression, JLA plan) reproduces the oracle's
+  dimensions, retained mask and plan arrays exactly, on the reference KAT
+  (rust/crates/vckss-core/tests/rust_mata_shared_atoms.rs:39-294) and on
+  synthetic panels;
+* the host CMG hierarchy is bit-identical to the oracle's, level by level
+  (graph, diagonal, aggregation labels, repeat counts, terminal reason), on
+  chains, grids, random graphs and the degree-4 hybrid graphs of panels.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_golden
+
+F = pytest.importorskip("fevc_amd")
+
+
+def _declared():
+    names = []
+    for h in ("vckss_mi355x.h", "vckss_mi355x_ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(v\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = F.library()
+    declared = _declared()
+    assert len(declared) >= 29
+    bound = {name for name, _, _ in F.SIGNATURES}
+    for name in declared:
+        assert hasattr(lib, name), name
+        assert name in bound, f"{name} has no Python binding"
+
+
+def test_abi_identity():
+    lib = F.library()
+    assert lib.vckss_rust_abi_version() == 1
+    assert b"CMG_FULL_V2" in lib.vckss_rust_capabilities_json()
+    req = F.default_solve_request()
+    assert req.options.probes == 200 and req.options.seed == 8675309
+    assert req.options.pcg_tolerance == 1e-10
+    assert req.options.maximum_iterations == 10000
+    assert req.options.residual_replacement_interval == 100
+
+
+def kat_columns():
+    K = load_golden("shared_atoms_kat.json")
+    inp = K["input"]
+    return K, [np.array([inp[k][r] for r in inp["order"]]) for k in
+               ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+
+
+def _compare_prepared(oracle, cols):
+    p = oracle.Problem(*cols)
+    e = F.Engine(*cols, upload=False)
+    try:
+        d = e.dims
+        assert (d["input_rows"], d["rows"], d["physical"], d["workers"], d["firms"], d["cells"],
+                d["units"], d["exact_strata"], d["strata"]) == (
+            p.input_rows, p.rows, p.physical, p.workers, p.firms, p.cells, p.units,
+            p.exact_strata, p.strata)
+        assert d["topology_checksum"] == p.topology_checksum
+        np.testing.assert_array_equal(e.mask(), p.mask())
+        for which, name, n in [(20, "unit_rank", p.units), (21, "unit_count", p.units),
+                               (22, "stratum_rank", p.strata), (23, "stratum_count", p.strata),
+                               (24, "cell_weight", p.cells), (25, "unit_cell", p.units)]:
+            got = e.array(which, n)
+            want = p.array(name).astype(np.float64)
+            assert np.array_equal(got, want), name
+        g, aux = p.hybrid_graph()
+        assert (d["hybrid_vertices"], d["hybrid_edges"], d["auxiliary"]) == (g.n, g.m, aux)
+        return p, e
+    except Exception:
+        e.release()
+        raise
+
+
+def test_prepare_kat(oracle):
+    K, cols = kat_columns()
+    p, e = _compare_prepared(oracle, cols)
+    e.release()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_prepare_synthetic_panels(oracle, seed):
+    from fevc_amd import synth
+    spec = synth.PanelSpec(workers=3000, firms=150, periods=4, mobility=0.15, seed=seed,
+                           name=f"p{seed}")
+    data = synth.generate(spec)
+    cols = [data[k] for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    p, e = _compare_prepared(oracle, cols)
+    assert e.dims["cells"] > 1000
+    e.release()
+
+
+def test_prepare_refuses_bad_input():
+    K, cols = kat_columns()
+    bad = [c.astype(np.float64) for c in cols]
+    bad[0][3] = 0.5  # non-integer worker id
+    with pytest.raises(F.EngineError) as ei:
+        F.Engine(*bad, upload=False)
+    assert ei.value.code == 21  # InvalidIdentifier, rust/crates/vckss-core/src/error.rs:14
+
+
+# ------------------------------------------------------------------ hierarchy
+
+def _assert_same_hierarchy(oracle, n, u, v, w, max_levels=128):
+    g = oracle.Graph(n, u, v, w)
+    ref = oracle.Cmg(g, direct_threshold=700, max_levels=max_levels).levels()
+    got, _ = F.build_hierarchy(n, u, v, w, direct_threshold=700, max_levels=max_levels)
+    assert len(got) == len(ref)
+    for i, (a, b) in enumerate(zip(got, ref)):
+        for key in ("n", "m", "nnz", "repeat", "terminal", "coarse"):
+            assert a[key] == b[key], (i, key, a[key], b[key])
+        for key in ("u", "v"):
+            assert np.array_equal(a[key], b[key]), (i, key)
+        for key in ("w", "diag"):
+            assert np.array_equal(a[key].view(np.uint64), b[key].view(np.uint64)), (i, key)
+        if a["coarse"]:
+            assert np.array_equal(a["labels"], b["labels"]), (i, "labels")
+    return got
+
+
+def test_hierarchy_chain(oracle):
+    n = 5000
+    u = np.arange(n - 1)
+    _assert_same_hierarchy(oracle, n, u, u + 1, 1.0 + (u % 7) * 0.25)
+
+
+def test_hierarchy_grid(oracle):
+    s = 120
+    idx = np.arange(s * s).reshape(s, s)
+    u = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    v = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    w = 1.0 + ((u * 7 + v * 3) % 11) / 10.0
+    levels = _assert_same_hierarchy(oracle, s * s, u, v, w)
+    assert len(levels) >= 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_hierarchy_random(oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = 4000
+    u = rng.integers(0, n, 3 * n)
+    v = rng.integers(0, n, 3 * n)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    # two components: a path over the first half, random edges stay as drawn
+    path = np.arange(n // 2 - 1)
+    u = np.concatenate([u, path])
+    v = np.concatenate([v, path + 1])
+    w = rng.uniform(0.1, 3.0, len(u))
+    _assert_same_hierarchy(oracle, n, u, v, w)
+
+
+def test_hierarchy_max_levels(oracle):
+    n = 3000
+    u = np.arange(n - 1)
+    _assert_same_hierarchy(oracle, n, u, u + 1, np.ones(n - 1), max_levels=2)
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_hierarchy_hybrid_graph(oracle, seed):
+    from fevc_amd import synth
+    spec = synth.PanelSpec(workers=20000, firms=1500, periods=5, mobility=0.2, seed=seed,
+                           name=f"h{seed}")
+    data = synth.generate(spec)
+    cols = [data[k] for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    p = oracle.Problem(*cols)
+    g, aux = p.hybrid_graph()
+    u, v, w, _ = g.export()
+    levels = _assert_same_hierarchy(oracle, g.n, u, v, w)
+    assert len(levels) >= 2
