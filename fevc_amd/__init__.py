@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-__all__ = ["EngineError", "Engine", "library", "build_hierarchy"]
+__all__ = ["EngineError", "Engine", "library", "build_hierarchy", "hierarchy_layout"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libvckss_mi355x.so")
@@ -151,6 +151,8 @@ SIGNATURES = [
     ("vckss_mi355x_hierarchy_levels_v1", C.c_int32,
      [C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_hierarchy_free_v1", C.c_int32, [C.c_uint64]),
+    ("vckss_mi355x_layout_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p] + [C.c_void_p] * 12),
     ("vckss_mi355x_prepare_host_v1", C.c_int32,
      [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64)]),
 ]
@@ -331,6 +333,46 @@ class Engine:
             self.release()
         except Exception:
             pass
+
+
+def hierarchy_layout(n, u, v, w, chunk=64, direct_threshold=700, max_levels=128):
+    """Device layout of every level of the host CMG hierarchy of a graph."""
+    lib = library()
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    w = np.ascontiguousarray(w, np.float64)
+    h = C.c_uint64()
+    _check(lib.vckss_mi355x_hierarchy_build_v1(n, len(u), u.ctypes.data, v.ctypes.data,
+                                               w.ctypes.data, direct_threshold, max_levels, C.byref(h)))
+    try:
+        nl, fnnz = C.c_uint32(), C.c_uint64()
+        _check(lib.vckss_mi355x_hierarchy_levels_v1(h.value, C.byref(nl), C.byref(fnnz)))
+        out = []
+        for lv in range(nl.value):
+            s = np.zeros(8, np.uint64)
+            _check(lib.vckss_mi355x_layout_v1(h.value, lv, chunk, s.ctypes.data, *([None] * 12)))
+            n_, nnz, nc, rch, rmu, ach, amu, nt = [int(x) for x in s]
+            a = dict(perm=np.zeros(n_, np.uint32), row_ptr=np.zeros(n_ + 1, np.uint32),
+                     col=np.zeros(nnz, np.uint32), val=np.zeros(nnz), inv=np.zeros(n_),
+                     lab=np.zeros(n_ if nc else 0, np.uint32),
+                     agg_ptr=np.zeros(nc + 1 if nc else 0, np.uint32),
+                     row_chunks=np.zeros(4 * rch, np.uint32), row_multi=np.zeros(2 * rmu, np.uint32),
+                     agg_chunks=np.zeros(4 * ach, np.uint32), agg_multi=np.zeros(2 * amu, np.uint32),
+                     terminal_inverse=np.zeros(nt * nt))
+            order = ["perm", "row_ptr", "col", "val", "inv", "lab", "agg_ptr", "row_chunks",
+                     "row_multi", "agg_chunks", "agg_multi", "terminal_inverse"]
+            _check(lib.vckss_mi355x_layout_v1(h.value, lv, chunk, s.ctypes.data,
+                                              *[a[k].ctypes.data if a[k].size else None for k in order]))
+            a["row_chunks"] = a["row_chunks"].reshape(-1, 4)
+            a["agg_chunks"] = a["agg_chunks"].reshape(-1, 4)
+            a["row_multi"] = a["row_multi"].reshape(-1, 2)
+            a["agg_multi"] = a["agg_multi"].reshape(-1, 2)
+            a["terminal_inverse"] = a["terminal_inverse"].reshape(nt, nt)
+            a["n"], a["coarse"] = n_, nc
+            out.append(a)
+        return out
+    finally:
+        lib.vckss_mi355x_hierarchy_free_v1(h.value)
 
 
 def build_hierarchy(n, u, v, w, direct_threshold=700, max_levels=128):

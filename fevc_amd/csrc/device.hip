@@ -5,6 +5,10 @@
 //   complete-residual gate -> moments / target contraction.
 // Reference flow: engine.rs:1060-1563, full_cmg.rs:809-1340, pcg.rs:1537-1909,
 // preconditioner/cycle.rs:136-478.
+//
+// Every level is renumbered so that aggregates are contiguous (restriction is
+// a contiguous segmented sum, prolongation labels are monotone); the level-0
+// renumbering is undone at the firm/auxiliary boundary of the hybrid system.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +32,10 @@ namespace vk {
 namespace {
 
 using dev::kBlock;
+
+constexpr uint32_t kChunkEntries = 64;  // CSR entries or aggregate members per chunk
+constexpr int kPollSlots = 8;
+constexpr int kPollLag = 2;
 
 template <typename T>
 struct DBuf {
@@ -93,19 +101,33 @@ uint32_t lanes_for(uint32_t columns) {
   return k;
 }
 
+struct DevChunks {
+  DBuf<uint32_t> desc, mrange;
+  uint32_t count = 0, n_multi = 0;
+  void upload(const ChunkTable &t) {
+    desc.upload(t.desc);
+    mrange.upload(t.mrange);
+    count = static_cast<uint32_t>(t.chunks());
+    n_multi = t.multi();
+  }
+  dev::Chunks view() const { return {reinterpret_cast<const uint4 *>(desc.p), mrange.p, count, n_multi}; }
+};
+
 struct DevLevel {
   uint32_t n = 0, coarse = 0, repeat = 0;
   uint64_t nnz = 0;
   int terminal = 0;
-  DBuf<uint32_t> rp, col, lab, aptr, amem;
+  DBuf<uint32_t> col, lab;
   DBuf<double> val, inv;
   DBuf<float> val32, inv32;
+  DevChunks rows, aggs;
+  std::vector<uint32_t> perm;  // host copy: renumbered -> original vertex
 };
 
 template <typename T>
 struct LevelVecs {
-  DBuf<T> b, x, y;
-  DBuf<double> mean;  // coarse mean of b, filled by the parent's restriction
+  DBuf<T> braw, b, x, y, r;
+  DBuf<double> mean;  // coarse mean of braw, filled by the parent's restriction
 };
 
 constexpr uint64_t kTagLeverage = 0x4c45564552414745ULL;
@@ -124,17 +146,26 @@ struct DeviceEngine::Impl {
   DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
   DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
   std::vector<DevLevel> levels;
+  DBuf<uint32_t> perm0, pos0;  // level-0 renumbering
+  DevChunks firm_chunks;       // firms over their cells (firm_cells order)
   DBuf<double> ginv;
-  DBuf<double> partial;
+  // reductions and segment scratch
+  DBuf<double> rpartial, bpartial, seg_partial;
   DBuf<unsigned> counters;
   unsigned counter_next = 0;
+  uint64_t max_chunks = 0;
+  uint32_t max_multi = 0;
+  DBuf<int> flags;
+  const int *gate = nullptr;
+  int *host_flags = nullptr;
+  hipEvent_t poll_events[kPollSlots] = {};
   uint32_t cap = 0;
   std::vector<LevelVecs<double>> lv64;
   std::vector<LevelVecs<float>> lv32;
-  DBuf<double> b0, proj, x, r, z, p, ap, guess_rhs;
+  DBuf<double> b0, proj, x, r, rc, z, p, ap, guess_rhs;
   DBuf<dev::ColState> st;
   DBuf<uint8_t> use;
-  DBuf<double> wrhs, frhs, worker, red, wsum, sout;
+  DBuf<double> wrhs, frhs, worker, red, fsol, sch, wsum, sout;
   DBuf<double> lane_a, lane_b, lane_c, reduced_res, complete_res, rhs_norm, sums;
   DBuf<double> atoms, moments, adjust, cellw, dir, totals, draws, fitted;
   DBuf<int> status;
@@ -144,6 +175,11 @@ struct DeviceEngine::Impl {
   uint64_t vcycles = 0;
 
   unsigned *counter() { return counters.p + (counter_next++ % 64); }
+  const int *always() const { return flags.p + dev::kFlagOne; }
+
+  dev::SegScratch scratch(unsigned *rcounter) const {
+    return {seg_partial.p, bpartial.p, rpartial.p, rcounter, gate};
+  }
 
   void ensure(uint32_t lanes) {
     if (lanes <= cap) return;
@@ -153,15 +189,18 @@ struct DeviceEngine::Impl {
     lv64.resize(levels.size());
     lv32.resize(levels.size());
     for (size_t l = 0; l < levels.size(); ++l) {
-      size_t m = (size_t)levels[l].n * lanes;
-      lv64[l].b.alloc(m); lv64[l].x.alloc(m); lv64[l].y.alloc(m); lv64[l].mean.alloc(64);
-      lv32[l].b.alloc(m); lv32[l].x.alloc(m); lv32[l].y.alloc(m); lv32[l].mean.alloc(64);
+      const size_t m = (size_t)levels[l].n * lanes;
+      for (auto *v : {&lv64[l].x, &lv64[l].y, &lv64[l].r}) v->alloc(m);
+      for (auto *v : {&lv32[l].braw, &lv32[l].b, &lv32[l].x, &lv32[l].y, &lv32[l].r}) v->alloc(m);
+      if (l > 0) { lv64[l].braw.alloc(m); lv64[l].b.alloc(m); }
+      lv64[l].mean.alloc(64);
+      lv32[l].mean.alloc(64);
     }
-    size_t m0 = (size_t)n0 * lanes;
-    b0.alloc(m0); proj.alloc(m0); x.alloc(m0); r.alloc(m0); z.alloc(m0); p.alloc(m0); ap.alloc(m0);
-    guess_rhs.alloc(m0);
+    const size_t m0 = (size_t)n0 * lanes;
+    for (auto *v : {&b0, &proj, &x, &r, &rc, &z, &p, &ap, &guess_rhs}) v->alloc(m0);
     wrhs.alloc(W * lanes); frhs.alloc(F * lanes); worker.alloc(W * lanes); wsum.alloc(W * lanes);
-    red.alloc(F * lanes); sout.alloc(F * lanes);
+    red.alloc(F * lanes); fsol.alloc(F * lanes); sch.alloc(F * lanes); sout.alloc(F * lanes);
+    seg_partial.alloc(std::max<uint64_t>(max_chunks, 1) * lanes);
   }
 
   void check_status(const char *phase) {
@@ -174,11 +213,29 @@ struct DeviceEngine::Impl {
 
 namespace {
 
+using Impl = DeviceEngine::Impl;
+
+// Launch a segment operation over `ch` with operator `op`: pass 1 over all
+// chunks, pass 2 for multi-chunk segments and the final reduction.
+template <typename T, int K, int NV, class Op>
+void segments(Impl &E, const DevChunks &ch, const Op &op) {
+  constexpr int G = dev::Shape<K>::G;
+  const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
+  const dev::SegScratch sc = E.scratch(NV > 0 ? E.counter() : nullptr);
+  hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
+  if (ch.n_multi == 0 && NV == 0) return;
+  const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
+  hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
+                     NV > 0 ? grid1 : 0u);
+}
+
 // ------------------------------------------------------------------ V-cycle
 template <typename T, int K>
 struct Cycle {
-  DeviceEngine::Impl &E;
+  Impl &E;
   std::vector<LevelVecs<T>> &V;
+  const T *braw0;  // level-0 right-hand side
+  T *b0;           // level-0 centred copy
 
   const T *val(const DevLevel &L) const {
     if constexpr (sizeof(T) == 4) return L.val32.p; else return L.val.p;
@@ -192,137 +249,136 @@ struct Cycle {
   void level(size_t l, uint32_t iterations, const double *mean) {
     DevLevel &L = E.levels[l];
     LevelVecs<T> &v = V[l];
+    const T *braw = l ? v.braw.p : braw0;
+    T *b = l ? v.b.p : b0;
     const uint64_t work = (uint64_t)L.n * K;
     if (L.terminal) {
       if (L.terminal == kDirect)
-        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(work), kBlock, 0, E.stream, L.n,
-                           E.ginv.p, v.b.p, mean, v.x.p);
+        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, E.ginv.p,
+                           braw, mean, b, v.x.p, E.gate);
       else
-        hipLaunchKernelGGL((dev::k_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L),
-                           v.b.p, v.x.p, mean);
+        hipLaunchKernelGGL((dev::k_terminal_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L), braw,
+                           mean, b, v.x.p, E.gate);
       return;
     }
     DevLevel &Lc = E.levels[l + 1];
     LevelVecs<T> &c = V[l + 1];
     for (uint32_t it = 0; it < iterations; ++it) {
       if (it == 0) {
-        hipLaunchKernelGGL((dev::k_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L),
-                           v.b.p, v.x.p, mean);
+        segments<T, K, 0>(E, L.rows,
+                          dev::OpScaleResidual<T, K>{{L.col.p, val(L)}, braw, mean, inv(L), b, v.x.p, v.r.p});
       } else {
-        hipLaunchKernelGGL((dev::k_smooth<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, L.rp.p,
-                           L.col.p, val(L), inv(L), v.b.p, v.x.p, v.y.p);
+        segments<T, K, 0>(E, L.rows, dev::OpSmooth<T, K>{{L.col.p, val(L)}, b, v.x.p, inv(L), v.y.p});
         std::swap(v.x, v.y);
+        segments<T, K, 0>(E, L.rows, dev::OpResidual<T, K>{{L.col.p, val(L)}, b, v.x.p, v.r.p});
       }
-      hipLaunchKernelGGL((dev::k_restrict<T, K>), reduce_grid((uint64_t)Lc.n * K), kBlock, 0, E.stream,
-                         Lc.n, L.aptr.p, L.amem.p, L.rp.p, L.col.p, val(L), v.b.p, v.x.p, c.b.p,
-                         E.partial.p, E.counter(), c.mean.p);
+      segments<T, K, 1>(E, L.aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, c.mean.p, Lc.n});
       level(l + 1, L.repeat, c.mean.p);
       const bool timed = E.timing && l == 0;
       if (timed) {
-        hipEvent_t a, b;
+        hipEvent_t a, e;
         VK_HIP(hipEventCreate(&a));
-        VK_HIP(hipEventCreate(&b));
+        VK_HIP(hipEventCreate(&e));
         VK_HIP(hipEventRecord(a, E.stream));
-        E.events.push_back({a, b});
+        E.events.push_back({a, e});
       }
-      hipLaunchKernelGGL((dev::k_prolong_smooth<T, K>), grid_for(work), kBlock, 0, E.stream, L.n,
-                         L.rp.p, L.col.p, val(L), inv(L), L.lab.p, c.x.p, v.b.p, v.x.p, v.y.p);
+      segments<T, K, 0>(E, L.rows,
+                        dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, b, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p});
       if (timed) {
         VK_HIP(hipEventRecord(E.events.back().second, E.stream));
-        // algorithmic bytes: CSR once, inv + labels, b/x read, y write, e read
-        const double sv = sizeof(T), sx = sizeof(T);
-        E.timed_bytes = (double)L.nnz * (sv + 4) + 4.0 * (L.n + 1) + (sv + 4) * L.n +
-                        K * sx * (3.0 * L.n + Lc.n);
+        // algorithmic bytes: CSR once (values, columns, chunk descriptors),
+        // inverse diagonal and labels, b and x read, y written, e read
+        const double s = sizeof(T);
+        E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * L.rows.count + (s + 4) * L.n +
+                        K * s * (3.0 * L.n + Lc.n);
       }
       std::swap(v.x, v.y);
     }
   }
 };
 
-// z = M r for K lanes.  With `mean_r` non-null, r is centred in place first.
+// z = M r for K lanes; r is centred by `mean_r` (lane means) into `rc`.
+// Returns the device address of z.
 template <int K>
-void apply_cycle(DeviceEngine::Impl &E, double *r, const double *mean_r, double *z, bool fp32) {
-  const uint64_t m0 = (uint64_t)E.n0 * K;
+const double *apply_cycle(Impl &E, const double *r, const double *mean_r, double *rc, bool fp32) {
   E.vcycles++;
   if (!fp32) {
-    auto &V = E.lv64;
-    double *saved = V[0].b.p;
-    V[0].b.p = r;
-    Cycle<double, K> cyc{E, V};
+    Cycle<double, K> cyc{E, E.lv64, r, rc};
     cyc.level(0, 1, mean_r);
-    VK_HIP(hipMemcpyAsync(z, V[0].x.p, m0 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
-    V[0].b.p = saved;
-  } else {
-    auto &V = E.lv32;
-    if (mean_r)
-      hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)E.n0, r, mean_r);
-    hipLaunchKernelGGL((dev::k_to_float<K>), grid_for(m0), kBlock, 0, E.stream, m0, r, V[0].b.p);
-    Cycle<float, K> cyc{E, V};
-    cyc.level(0, 1, nullptr);
-    hipLaunchKernelGGL((dev::k_to_double<K>), grid_for(m0), kBlock, 0, E.stream, m0, V[0].x.p, z);
+    return E.lv64[0].x.p;
   }
+  const uint64_t m0 = E.n0 * K;
+  hipLaunchKernelGGL((dev::k_centre_split<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, r, mean_r, rc,
+                     E.lv32[0].braw.p, E.gate);
+  Cycle<float, K> cyc{E, E.lv32, E.lv32[0].braw.p, E.lv32[0].b.p};
+  cyc.level(0, 1, nullptr);
+  hipLaunchKernelGGL((dev::k_to_double<K>), grid_for(m0), kBlock, 0, E.stream, m0, E.lv32[0].x.p, E.z.p, E.gate);
+  return E.z.p;
 }
 
-// Lockstep certified PCG (pcg.rs:1537-1909) on the finest hybrid Laplacian.
-// b: n0 x K right-hand sides.  With `warm`, E.x holds the initial guesses.
+// Lockstep certified PCG (pcg.rs:1537-1909) on the finest hybrid Laplacian
+// (renumbered).  b: n0 x K right-hand sides.  With `warm`, E.x holds the
+// initial guesses.  The host polls the any-lane-active flag with a lag of
+// kPollLag iterations; iterations queued after convergence return at once.
 template <int K>
-void pcg_lanes(DeviceEngine::Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it,
-               uint32_t interval, bool warm, bool fp32, const std::vector<uint8_t> &use,
-               std::vector<dev::ColState> &out) {
+void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
+               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out) {
   const uint32_t n = E.n0;
   const uint64_t m0 = (uint64_t)n * K;
   const DevLevel &L0 = E.levels[0];
   dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval};
+  int *flags = E.flags.p;
+  const int *active = flags + dev::kFlagActive;
   VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
-  hipLaunchKernelGGL((dev::k_pcg_stats<K>), reduce_grid(m0), kBlock, 0, E.stream, n, b, E.partial.p,
+  hipLaunchKernelGGL((dev::k_pcg_stats<K>), reduce_grid(m0), kBlock, 0, E.stream, n, b, E.rpartial.p,
                      E.counter(), E.st.p, prm, E.use.p);
+  E.gate = E.always();
   if (!warm) {
-    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p,
-                       E.r.p, E.x.p, E.st.p);
+    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.x.p,
+                       E.st.p);
     hipLaunchKernelGGL((dev::k_pcg_start<K>), 1, 64, 0, E.stream, E.st.p, prm);
   } else {
-    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p,
-                       E.r.p, E.ap.p, E.st.p);
+    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.ap.p,
+                       E.st.p);
     hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
-                       E.partial.p, E.counter(), E.lane_a.p);
+                       E.rpartial.p, E.counter(), E.lane_a.p);
     hipLaunchKernelGGL((dev::k_mask_means<K>), 1, 64, 0, E.stream, E.lane_a.p, E.use.p);
     hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
-                       E.lane_a.p);
-    hipLaunchKernelGGL((dev::k_pcg_warm<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p, L0.col.p,
-                       L0.val.p, b, E.proj.p, E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
+                       E.lane_a.p, nullptr);
+    segments<double, K, 4>(E, L0.rows, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
   }
-  std::vector<dev::ColState> host(K);
-  auto any_active = [&]() {
-    VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
-    VK_HIP(hipStreamSynchronize(E.stream));
-    for (int j = 0; j < K; ++j)
-      if (host[j].active) return true;
-    return false;
-  };
+  E.gate = active;
   auto precondition = [&]() {
-    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p);
-    apply_cycle<K>(E, E.r.p, E.lane_b.p, E.z.p, fp32);
-    hipLaunchKernelGGL((dev::k_pcg_rho<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.r.p, E.z.p,
-                       E.partial.p, E.counter(), E.st.p);
-    hipLaunchKernelGGL((dev::k_pcg_direction<K>), grid_for(m0), kBlock, 0, E.stream, n, E.z.p, E.p.p, E.st.p);
+    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags,
+                       (const int *)nullptr);
+    const double *zp = apply_cycle<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
+    hipLaunchKernelGGL((dev::k_pcg_rho<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.rc.p, zp, E.rpartial.p,
+                       E.counter(), E.st.p, active);
+    hipLaunchKernelGGL((dev::k_pcg_direction<K>), grid_for(m0), kBlock, 0, E.stream, n, zp, E.p.p, E.st.p, active);
   };
-  if (any_active()) {
-    precondition();
-    for (uint32_t it = 1; it <= max_it; ++it) {
-      hipLaunchKernelGGL((dev::k_pcg_curvature<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p,
-                         L0.col.p, L0.val.p, E.p.p, E.ap.p, E.partial.p, E.counter(), E.st.p);
-      hipLaunchKernelGGL((dev::k_pcg_update<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.p.p, E.ap.p,
-                         E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
-      hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p);
-      hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
-                         E.lane_a.p);
-      hipLaunchKernelGGL((dev::k_pcg_fresh<K>), reduce_grid(m0), kBlock, 0, E.stream, n, L0.rp.p, L0.col.p,
-                         L0.val.p, b, E.proj.p, E.x.p, E.r.p, E.partial.p, E.counter(), E.st.p, prm);
-      hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm);
-      if (!any_active()) break;
-      precondition();
+  precondition();
+  for (uint32_t it = 1; it <= max_it; ++it) {
+    segments<double, K, 1>(E, L0.rows, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
+    hipLaunchKernelGGL((dev::k_pcg_update<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p,
+                       E.rc.p, E.r.p, E.rpartial.p, E.counter(), E.st.p, prm, active);
+    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags, active);
+    hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.lane_a.p,
+                       active);
+    segments<double, K, 3>(E, L0.rows,
+                           dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags});
+    hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm, flags);
+    const int slot = it % kPollSlots;
+    VK_HIP(hipMemcpyAsync(E.host_flags + slot, active, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    VK_HIP(hipEventRecord(E.poll_events[slot], E.stream));
+    if (it > (uint32_t)kPollLag) {
+      const int seen = (it - kPollLag) % kPollSlots;
+      VK_HIP(hipEventSynchronize(E.poll_events[seen]));
+      if (E.host_flags[seen] == 0) break;
     }
+    precondition();
   }
+  E.gate = E.always();
+  std::vector<dev::ColState> host(K);
   VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
   VK_HIP(hipStreamSynchronize(E.stream));
   out = host;
@@ -330,29 +386,35 @@ void pcg_lanes(DeviceEngine::Impl &E, const double *b, double rel, double abs_to
 
 // full_cmg.rs:1158-1340 extraction for K lanes: normalise the firm mean, the
 // reduced Schur residual, worker reconstruction and the complete residual.
+// fsol: the normalised firm solution in firm order; red: its recentred copy.
 template <int K>
-void extract(DeviceEngine::Impl &E) {
+void extract(Impl &E) {
   const uint64_t F = E.F, W = E.W;
-  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.x.p, E.partial.p,
+  hipLaunchKernelGGL((dev::k_gather_rows<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.pos0.p, E.x.p, E.fsol.p);
+  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.fsol.p, E.rpartial.p,
                      E.counter(), E.lane_a.p);
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p);
-  VK_HIP(hipMemcpyAsync(E.red.p, E.x.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
-  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.red.p, E.partial.p,
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p,
+                     nullptr);
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.fsol.p, E.lane_a.p,
+                     nullptr);
+  VK_HIP(hipMemcpyAsync(E.red.p, E.fsol.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
+  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.red.p, E.rpartial.p,
                      E.counter(), E.lane_a.p);
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p,
+                     nullptr);
   hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
                      E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
-  hipLaunchKernelGGL((dev::k_schur_apply<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
-                     E.firm_ptr.p, E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.fdiag.p, E.red.p,
-                     E.wsum.p, E.sout.p, E.partial.p, E.counter(), E.lane_b.p);
-  hipLaunchKernelGGL((dev::k_reduced_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
-                     E.b0.p, E.sout.p, E.lane_b.p, E.partial.p, E.counter(), E.reduced_res.p);
+  segments<double, K, 1>(E, E.firm_chunks,
+                         dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.red.p,
+                                              E.wsum.p, E.sout.p, E.lane_b.p, (uint32_t)F});
+  hipLaunchKernelGGL((dev::k_reduced_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F, E.sch.p,
+                     E.sout.p, E.lane_b.p, E.rpartial.p, E.counter(), E.reduced_res.p);
   hipLaunchKernelGGL((dev::k_worker_solution<K>), reduce_grid(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
-                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.x.p, E.worker.p, E.partial.p,
+                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p, E.worker.p, E.rpartial.p,
                      E.counter(), E.sums.p);
-  hipLaunchKernelGGL((dev::k_firm_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F,
-                     E.firm_ptr.p, E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.fdiag.p, E.frhs.p,
-                     E.x.p, E.worker.p, E.sums.p, E.partial.p, E.counter(), E.complete_res.p, E.rhs_norm.p);
+  segments<double, K, 2>(E, E.firm_chunks,
+                         dev::OpFirmResidual<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.frhs.p,
+                                                E.fsol.p, E.worker.p, E.sums.p, E.complete_res.p, E.rhs_norm.p});
 }
 
 double euclid(const std::vector<double> &v, uint64_t n, uint32_t K, uint32_t lane) {
@@ -371,16 +433,17 @@ double euclid(const std::vector<double> &v, uint64_t n, uint32_t K, uint32_t lan
 // `columns` in use): the CMG_FULL_V2 batch solve with its complete-residual
 // gate and warm-start refinement ladder (full_cmg.rs:809-1111).
 template <int K>
-void solve_columns(DeviceEngine::Impl &E, uint32_t columns, double effective_tol, double inner_ratio,
-                   const SolveOptions &opt, std::vector<ColumnReceipt> &receipts) {
+void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner_ratio, const SolveOptions &opt,
+                   std::vector<ColumnReceipt> &receipts) {
   const uint64_t F = E.F;
   const uint64_t m0 = E.n0 * K;
   const double gate = std::max(10.0 * effective_tol, 1e-11);
-  VK_HIP(hipMemsetAsync(E.b0.p, 0, m0 * sizeof(double), E.stream));
-  hipLaunchKernelGGL((dev::k_schur_rhs<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F, E.firm_ptr.p,
-                     E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.frhs.p, E.b0.p,
-                     E.partial.p, E.counter(), E.lane_a.p);
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.b0.p, E.lane_a.p);
+  segments<double, K, 1>(E, E.firm_chunks,
+                         dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.wdiag.p, E.wrhs.p,
+                                            E.frhs.p, E.sch.p, E.lane_a.p, (uint32_t)F});
+  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.sch.p, E.lane_a.p, nullptr);
+  hipLaunchKernelGGL((dev::k_hybrid_rhs<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, (uint32_t)F, E.perm0.p,
+                     E.sch.p, E.b0.p);
   std::vector<uint8_t> use(K, 0);
   for (uint32_t j = 0; j < columns; ++j) use[j] = 1;
   std::vector<dev::ColState> st;
@@ -412,8 +475,8 @@ void solve_columns(DeviceEngine::Impl &E, uint32_t columns, double effective_tol
     if (!any) break;
     // warm start from the lifted firm solution (full_cmg.rs:970-1088)
     if (E.aux)
-      hipLaunchKernelGGL((dev::k_lift<K>), grid_for(E.aux * K), kBlock, 0, E.stream, (uint32_t)F,
-                         (uint32_t)E.aux, E.aux_ptr.p, E.aux_nbr.p, E.aux_w.p, E.x.p);
+      hipLaunchKernelGGL((dev::k_lift<K>), grid_for(E.aux * K), kBlock, 0, E.stream, (uint32_t)F, (uint32_t)E.aux,
+                         E.aux_ptr.p, E.aux_nbr.p, E.aux_w.p, E.pos0.p, E.fsol.p, E.x.p);
     std::vector<double> hb;
     E.b0.download(hb, m0, E.stream);
     double smallest = INFINITY;
@@ -424,8 +487,8 @@ void solve_columns(DeviceEngine::Impl &E, uint32_t columns, double effective_tol
       rel = effective_tol * inner_ratio * factors[round];
       abs_tol = 0.0;
     }
-    pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true,
-                 opt.precision == 32, fail_lane, st);
+    pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true, opt.precision == 32,
+                 fail_lane, st);
     for (uint32_t j = 0; j < columns; ++j)
       if (fail_lane[j]) {
         if (st[j].status) fail(st[j].status, "cmg_full_v2_refinement", "refinement PCG failed");
@@ -439,8 +502,7 @@ void solve_columns(DeviceEngine::Impl &E, uint32_t columns, double effective_tol
     if (!std::isfinite(comp[j]) || comp[j] > gate || !std::isfinite(red[j]) || red[j] > gate)
       fail(kFullResidual, "cmg_full_v2_refinement",
            "zero-based RHS column " + std::to_string(j) + ": reduced residual " + std::to_string(red[j]) +
-               " or complete residual " + std::to_string(comp[j]) + " exceeds tolerance " +
-               std::to_string(gate));
+               " or complete residual " + std::to_string(comp[j]) + " exceeds tolerance " + std::to_string(gate));
     receipts[j].reduced = red[j];
     receipts[j].complete = comp[j];
     receipts[j].rhs_norm = rn[j];
@@ -482,6 +544,20 @@ double component_identity(const double c[4]) {
   return std::fabs(c[3] - implied) / scale;
 }
 
+// Host permutation of K-lane vectors between original and renumbered order.
+std::vector<double> to_renumbered(const std::vector<double> &in, const std::vector<uint32_t> &perm, uint32_t K) {
+  std::vector<double> out(in.size());
+  for (size_t p = 0; p < perm.size(); ++p)
+    for (uint32_t j = 0; j < K; ++j) out[p * K + j] = in[(size_t)perm[p] * K + j];
+  return out;
+}
+std::vector<double> from_renumbered(const std::vector<double> &in, const std::vector<uint32_t> &perm, uint32_t K) {
+  std::vector<double> out(in.size());
+  for (size_t p = 0; p < perm.size(); ++p)
+    for (uint32_t j = 0; j < K; ++j) out[(size_t)perm[p] * K + j] = in[p * K + j];
+  return out;
+}
+
 }  // namespace
 
 bool device_available() {
@@ -489,8 +565,7 @@ bool device_available() {
   return hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
 }
 
-DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h)
-    : impl_(new Impl) {
+DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h) : impl_(new Impl) {
   Impl &E = *impl_;
   int devices = 0;
   if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
@@ -503,8 +578,7 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.aux = hs.aux;
   E.n0 = h.levels[0].graph.n;
   E.bound = h.levels[0].graph.norm_bound;
-  if (h.finest_components != 1)
-    fail(kUnsupported, "device", "the hybrid Laplacian must be connected");
+  if (h.finest_components != 1) fail(kUnsupported, "device", "the hybrid Laplacian must be connected");
   std::vector<double> wd(E.W, 0.0), fd(E.F, 0.0);
   for (uint64_t c = 0; c < E.C; ++c) {
     wd[p.cell_worker[c]] += p.cell_weight[c];
@@ -520,27 +594,47 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.stratum_count.upload(p.stratum_count); E.cell_stratum_ptr.upload(p.cell_stratum_ptr);
   E.cell_target.upload(p.cell_target);
   E.aux_ptr.upload(hs.aux_ptr); E.aux_nbr.upload(hs.aux_nbr); E.aux_w.upload(hs.aux_w);
+
+  {
+    std::vector<uint32_t> fptr(p.firm_ptr.begin(), p.firm_ptr.end());
+    E.firm_chunks.upload(chunk_segments(fptr, kChunkEntries));
+  }
+  std::vector<LevelLayout> layout = device_layout(h, kChunkEntries);
   E.levels.resize(h.levels.size());
   for (size_t l = 0; l < h.levels.size(); ++l) {
     const Level &L = h.levels[l];
+    LevelLayout &Y = layout[l];
     DevLevel &D = E.levels[l];
     D.n = (uint32_t)L.graph.n;
-    D.nnz = L.col.size();
+    D.nnz = Y.col.size();
     D.coarse = (uint32_t)L.coarse;
     D.repeat = (uint32_t)std::min<uint64_t>(L.repeat, 1u << 30);
     D.terminal = L.terminal;
-    D.rp.upload(L.row_ptr); D.col.upload(L.col); D.val.upload(L.val); D.val32.upload(L.val);
-    D.inv.upload(L.inv_diag); D.inv32.upload(L.inv_diag);
-    if (!L.labels.empty()) {
-      D.lab.upload(L.labels); D.aptr.upload(L.agg_ptr); D.amem.upload(L.agg_members);
+    D.col.upload(Y.col); D.val.upload(Y.val); D.val32.upload(Y.val);
+    D.inv.upload(Y.inv); D.inv32.upload(Y.inv);
+    D.rows.upload(Y.rows);
+    if (!Y.lab.empty()) {
+      D.lab.upload(Y.lab);
+      D.aggs.upload(Y.aggs);
     }
+    D.perm = Y.perm;
+    E.max_chunks = std::max<uint64_t>({E.max_chunks, Y.rows.chunks(), Y.aggs.chunks()});
+    E.max_multi = std::max({E.max_multi, Y.rows.multi(), Y.aggs.multi()});
   }
-  if (h.terminal_reason == kDirect) {
-    E.ginv.upload(h.terminal_inverse);
-  }
-  E.partial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
+  E.max_chunks = std::max<uint64_t>(E.max_chunks, E.firm_chunks.count);
+  E.perm0.upload(layout[0].perm);
+  E.pos0.upload(layout[0].pos);
+  if (h.terminal_reason == kDirect) E.ginv.upload(h.terminal_inverse);
+  E.rpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
   E.counters.alloc(64);
   VK_HIP(hipMemset(E.counters.p, 0, 64 * sizeof(unsigned)));
+  E.bpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
+  E.flags.alloc(4);
+  const int init_flags[4] = {1, 0, 1, 0};
+  VK_HIP(hipMemcpy(E.flags.p, init_flags, sizeof init_flags, hipMemcpyHostToDevice));
+  E.gate = E.always();
+  VK_HIP(hipHostMalloc((void **)&E.host_flags, kPollSlots * sizeof(int), hipHostMallocDefault));
+  for (auto &ev : E.poll_events) VK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   E.st.alloc(64); E.use.alloc(64);
   E.lane_a.alloc(64); E.lane_b.alloc(64); E.lane_c.alloc(64);
   E.reduced_res.alloc(64); E.complete_res.alloc(64); E.rhs_norm.alloc(64); E.sums.alloc(128);
@@ -553,6 +647,9 @@ DeviceEngine::~DeviceEngine() {
   if (impl_ && impl_->stream) {
     (void)hipStreamSynchronize(impl_->stream);
     for (auto &e : impl_->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto &ev : impl_->poll_events)
+      if (ev) (void)hipEventDestroy(ev);
+    if (impl_->host_flags) (void)hipHostFree(impl_->host_flags);
     (void)hipStreamDestroy(impl_->stream);
   }
 }
@@ -583,7 +680,7 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
     res.fit = rc[0];
     std::vector<double> alpha, psi;
     E.worker.download(alpha, E.W, E.stream);
-    E.x.download(psi, E.F, E.stream);
+    E.fsol.download(psi, E.F, E.stream);
     res.fitted_cell.resize(E.C);
     for (uint64_t c = 0; c < E.C; ++c) res.fitted_cell[c] = alpha[P.cell_worker[c]] + psi[P.cell_firm[c]];
     Neumaier rss;
@@ -611,11 +708,12 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
       constexpr int K = decltype(kk)::value;
       hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
                          (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p);
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.F * K), kBlock, 0, E.stream, E.F, E.firm_ptr.p,
-                         E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.frhs.p);
+      segments<double, K, 0>(E, E.firm_chunks,
+                             dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
+                                                 E.frhs.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
       hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
-                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.x.p, E.atoms.p,
+                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });
     for (auto &c : rc) {
@@ -638,7 +736,9 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
   {
     std::vector<double> adj;
     E.adjust.download(adj, 6 * E.G, E.stream);
-    auto slice = [&](int q, std::vector<double> &out) { out.assign(adj.begin() + q * E.G, adj.begin() + (q + 1) * E.G); };
+    auto slice = [&](int q, std::vector<double> &out) {
+      out.assign(adj.begin() + q * E.G, adj.begin() + (q + 1) * E.G);
+    };
     slice(0, res.projection_share); slice(1, res.residual_share); slice(2, res.finite_bias);
     slice(3, res.finite_variance); slice(4, res.residual_mass); slice(5, res.deleted_mass);
     E.cellw.download(res.cell_weight, E.C, E.stream);
@@ -659,20 +759,20 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
       constexpr int K2c = 2 * KTc;
       hipLaunchKernelGGL((dev::k_target_direction<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
                          E.cell_stratum_ptr.p, E.stratum_per_copy.p, E.target_total, E.atoms.p, E.S, E.dir.p,
-                         E.partial.p, E.counter(), E.totals.p);
+                         E.rpartial.p, E.counter(), E.totals.p);
       hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.W * K2c), kBlock, 0, E.stream, E.W,
                          E.worker_ptr.p, (const uint32_t *)nullptr, E.dir.p, E.cell_target.p, E.target_total,
                          E.totals.p, 0, E.wrhs.p);
-      hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.F * K2c), kBlock, 0, E.stream, E.F,
-                         E.firm_ptr.p, E.firm_cells.p, E.dir.p, E.cell_target.p, E.target_total, E.totals.p, 1,
-                         E.frhs.p);
+      segments<double, K2c, 0>(E, E.firm_chunks,
+                               dev::OpFirmTargetScores<KTc>{E.firm_cells.p, E.dir.p, E.cell_target.p, E.totals.p,
+                                                            E.target_total, E.frhs.p});
       hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.W * K2c), kBlock, 0, E.stream, E.W, E.wrhs.p, 0,
-                         E.totals.p, E.partial.p, E.counter(), E.status.p);
+                         E.totals.p, E.rpartial.p, E.counter(), E.status.p);
       hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.F * K2c), kBlock, 0, E.stream, E.F, E.frhs.p, 1,
-                         E.totals.p, E.partial.p, E.counter(), E.status.p);
+                         E.totals.p, E.rpartial.p, E.counter(), E.status.p);
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
       hipLaunchKernelGGL((dev::k_contract<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
-                         E.cell_worker.p, E.cell_firm.p, E.cellw.p, E.worker.p, E.x.p, E.partial.p, E.counter(),
+                         E.cell_worker.p, E.cell_firm.p, E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p, E.counter(),
                          E.draws.p);
     });
     std::vector<double> d;
@@ -721,17 +821,15 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
   DBuf<double> buf;
   buf.alloc(n * columns);
   hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
-                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p,
-                     seed, tag, first_probe, columns, buf.p);
-  VK_HIP(hipStreamSynchronize(E.stream));
+                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p, seed,
+                     tag, first_probe, columns, buf.p);
   std::vector<double> h;
   buf.download(h, n * columns, E.stream);
   out.resize(h.size());
   for (size_t i = 0; i < h.size(); ++i) out[i] = (int64_t)h[i];
 }
 
-void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t columns,
-                        std::vector<double> &y) {
+void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t columns, std::vector<double> &y) {
   Impl &E = *impl_;
   if (level >= E.levels.size()) fail(kInvalidInput, "device", "level out of range");
   const uint32_t K = lanes_for(columns);
@@ -739,33 +837,39 @@ void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t
   DevLevel &L = E.levels[level];
   const uint64_t m = (uint64_t)L.n * K;
   if (xin.size() != m) fail(kInvalidInput, "device", "input has the wrong size");
+  E.ensure(K);
   DBuf<double> a, b;
-  a.upload(xin);
+  a.upload(to_renumbered(xin, L.perm, K));
   b.alloc(m);
+  E.gate = E.always();
   with_k(K, [&](auto kk) {
     constexpr int KK = decltype(kk)::value;
-    hipLaunchKernelGGL((dev::k_spmv<double, KK>), grid_for(m), kBlock, 0, E.stream, L.n, L.rp.p, L.col.p,
-                       L.val.p, a.p, b.p);
+    segments<double, KK, 0>(E, L.rows, dev::OpProduct<double, KK>{{L.col.p, L.val.p}, a.p, b.p});
   });
-  VK_HIP(hipStreamSynchronize(E.stream));
-  b.download(y, m, E.stream);
+  std::vector<double> yn;
+  b.download(yn, m, E.stream);
+  y = from_renumbered(yn, L.perm, K);
 }
 
-void DeviceEngine::vcycle(const std::vector<double> &rhs, uint32_t columns, int precision,
-                          std::vector<double> &out) {
+void DeviceEngine::vcycle(const std::vector<double> &rhs, uint32_t columns, int precision, std::vector<double> &out) {
   Impl &E = *impl_;
   const uint32_t K = lanes_for(columns);
   if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
   E.ensure(K);
   const uint64_t m0 = E.n0 * K;
   if (rhs.size() != m0) fail(kInvalidInput, "device", "rhs has the wrong size");
-  VK_HIP(hipMemcpy(E.guess_rhs.p, rhs.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
+  const std::vector<double> rn = to_renumbered(rhs, E.levels[0].perm, K);
+  VK_HIP(hipMemcpy(E.guess_rhs.p, rn.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
+  E.gate = E.always();
+  const double *zp = nullptr;
   with_k(K, [&](auto kk) {
     constexpr int KK = decltype(kk)::value;
-    apply_cycle<KK>(E, E.guess_rhs.p, nullptr, E.z.p, precision == 32);
+    zp = apply_cycle<KK>(E, E.guess_rhs.p, nullptr, E.rc.p, precision == 32);
   });
+  std::vector<double> zn(m0);
+  VK_HIP(hipMemcpyAsync(zn.data(), zp, m0 * sizeof(double), hipMemcpyDeviceToHost, E.stream));
   VK_HIP(hipStreamSynchronize(E.stream));
-  E.z.download(out, m0, E.stream);
+  out = from_renumbered(zn, E.levels[0].perm, K);
 }
 
 void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
@@ -776,7 +880,8 @@ void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double 
   E.ensure(K);
   const uint64_t m0 = E.n0 * K;
   if (rhs.size() != m0) fail(kInvalidInput, "device", "rhs has the wrong size");
-  VK_HIP(hipMemcpy(E.guess_rhs.p, rhs.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
+  const std::vector<double> rn = to_renumbered(rhs, E.levels[0].perm, K);
+  VK_HIP(hipMemcpy(E.guess_rhs.p, rn.data(), m0 * sizeof(double), hipMemcpyHostToDevice));
   std::vector<uint8_t> use(K, 1);
   std::vector<dev::ColState> st;
   with_k(K, [&](auto kk) {
@@ -790,7 +895,9 @@ void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double 
     receipts[j].restarts = st[j].restarts;
     receipts[j].rhs_norm = st[j].rhs_norm;
   }
-  E.x.download(xout, m0, E.stream);
+  std::vector<double> xn;
+  E.x.download(xn, m0, E.stream);
+  xout = from_renumbered(xn, E.levels[0].perm, K);
 }
 
 double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
@@ -819,11 +926,12 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
       constexpr int K = decltype(kk)::value;
       hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
                          (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p);
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.F * K), kBlock, 0, E.stream, E.F, E.firm_ptr.p,
-                         E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.frhs.p);
+      segments<double, K, 0>(E, E.firm_chunks,
+                             dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
+                                                 E.frhs.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
       hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
-                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.x.p, E.atoms.p,
+                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });
     for (auto &c : rc) iterations += c.iterations;

@@ -509,4 +509,79 @@ Hierarchy build_hierarchy(const Laplacian &graph, const CmgOptions &opt) {
   return H;
 }
 
+ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk) {
+  if (chunk == 0) fail(kInvalidInput, "device_layout", "chunk size must be positive");
+  ChunkTable t;
+  const uint64_t segments = ptr.empty() ? 0 : ptr.size() - 1;
+  for (uint64_t s = 0; s < segments; ++s) {
+    const uint64_t begin = ptr[s], len = ptr[s + 1] - begin;
+    const uint64_t parts = std::max<uint64_t>(1, (len + chunk - 1) / chunk);
+    const uint32_t first = static_cast<uint32_t>(t.chunks());
+    const uint32_t multi = parts > 1 ? t.multi() : kSingleChunk;
+    for (uint64_t q = 0; q < parts; ++q) {
+      t.desc.push_back(static_cast<uint32_t>(s));
+      t.desc.push_back(static_cast<uint32_t>(begin + len * q / parts));
+      t.desc.push_back(static_cast<uint32_t>(begin + len * (q + 1) / parts));
+      t.desc.push_back(multi);
+    }
+    if (parts > 1) {
+      t.mrange.push_back(first);
+      t.mrange.push_back(static_cast<uint32_t>(t.chunks()));
+    }
+  }
+  return t;
+}
+
+std::vector<LevelLayout> device_layout(const Hierarchy &H, uint32_t chunk) {
+  const size_t L = H.levels.size();
+  std::vector<LevelLayout> out(L);
+  for (size_t li = L; li-- > 0;) {
+    const Level &lv = H.levels[li];
+    LevelLayout &d = out[li];
+    const uint32_t n = static_cast<uint32_t>(lv.graph.n);
+    d.perm.resize(n);
+    d.pos.resize(n);
+    const bool has_child = li + 1 < L;
+    if (has_child) {
+      // stable counting sort of the vertices by the renumbered coarse index
+      const std::vector<uint32_t> &cpos = out[li + 1].pos;
+      const uint32_t nc = static_cast<uint32_t>(cpos.size());
+      if (lv.labels.size() != n || lv.coarse != nc) fail(kInvariant, "device_layout", "labels do not match the coarse level");
+      d.agg_ptr.assign(nc + 1, 0);
+      for (uint32_t v = 0; v < n; ++v) d.agg_ptr[cpos[lv.labels[v]] + 1]++;
+      for (uint32_t c = 0; c < nc; ++c) d.agg_ptr[c + 1] += d.agg_ptr[c];
+      std::vector<uint32_t> cur(d.agg_ptr.begin(), d.agg_ptr.end() - 1);
+      for (uint32_t v = 0; v < n; ++v) d.perm[cur[cpos[lv.labels[v]]]++] = v;
+    } else {
+      std::iota(d.perm.begin(), d.perm.end(), 0u);
+    }
+    for (uint32_t p = 0; p < n; ++p) d.pos[d.perm[p]] = p;
+    d.row_ptr.assign(n + 1, 0);
+    for (uint32_t p = 0; p < n; ++p) {
+      const uint32_t v = d.perm[p];
+      d.row_ptr[p + 1] = d.row_ptr[p] + (lv.row_ptr[v + 1] - lv.row_ptr[v]);
+    }
+    d.col.resize(lv.col.size());
+    d.val.resize(lv.val.size());
+    d.inv.resize(n);
+    for (uint32_t p = 0; p < n; ++p) {
+      const uint32_t v = d.perm[p];
+      uint32_t k = d.row_ptr[p];
+      for (uint32_t e = lv.row_ptr[v]; e < lv.row_ptr[v + 1]; ++e, ++k) {
+        d.col[k] = d.pos[lv.col[e]];
+        d.val[k] = lv.val[e];
+      }
+      d.inv[p] = lv.inv_diag[v];
+    }
+    if (has_child) {
+      const std::vector<uint32_t> &cpos = out[li + 1].pos;
+      d.lab.resize(n);
+      for (uint32_t p = 0; p < n; ++p) d.lab[p] = cpos[lv.labels[d.perm[p]]];
+      d.aggs = chunk_segments(d.agg_ptr, chunk);
+    }
+    d.rows = chunk_segments(d.row_ptr, chunk);
+  }
+  return out;
+}
+
 }  // namespace vk

@@ -69,4 +69,34 @@ HybridSystem build_hybrid(const Prepared &p);
 Laplacian laplacian_from_edges(uint64_t n, uint64_t m, const uint32_t *u, const uint32_t *v, const double *w);
 Hierarchy build_hierarchy(const Laplacian &g, const CmgOptions &opt);
 
+// ------------------------------------------------------------------ device layout
+// Segments (CSR rows, or aggregates over their contiguous members) are cut
+// into chunks of at most `chunk` entries so that no GPU thread group walks a
+// long row or a large aggregate alone.  A segment with several chunks gets a
+// "multi" index: its chunk partials are combined by whichever chunk group
+// finishes last.
+constexpr uint32_t kSingleChunk = 0xffffffffu;
+
+struct ChunkTable {
+  std::vector<uint32_t> desc;    // 4 per chunk: segment, begin, end, multi index or kSingleChunk
+  std::vector<uint32_t> mrange;  // 2 per multi-chunk segment: first chunk, end chunk
+  uint64_t chunks() const { return desc.size() / 4; }
+  uint32_t multi() const { return static_cast<uint32_t>(mrange.size() / 2); }
+};
+ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk);
+
+// One level renumbered so that every aggregate is a contiguous run of
+// vertices, runs ordered like the coarse level's own renumbering; the last
+// level keeps its numbering.  Row entries keep the reference order (ascending
+// original neighbour index).
+struct LevelLayout {
+  std::vector<uint32_t> perm, pos;  // new -> old, old -> new
+  std::vector<uint32_t> row_ptr, col;
+  std::vector<double> val, inv;
+  std::vector<uint32_t> lab;        // coarse vertex (renumbered) of every vertex; monotone
+  std::vector<uint32_t> agg_ptr;    // member range of every coarse vertex
+  ChunkTable rows, aggs;
+};
+std::vector<LevelLayout> device_layout(const Hierarchy &H, uint32_t chunk);
+
 }  // namespace vk

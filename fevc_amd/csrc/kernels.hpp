@@ -1,11 +1,21 @@
-// HIP kernels of the leave-out engine (gfx950).
+// HIP kernels of the leave-out engine (gfx950): RNG, deterministic
+// reductions, and the chunked segment kernel behind every sparse pass of the
+// CMG V-cycle and of PCG.
 //
 // Vectors of a batch of K right-hand sides are stored vertex-major and
-// interleaved, v[row * K + lane], so one wavefront reads K contiguous lanes of
-// a row and the CSR structure is read once per batch.  K is a compile-time
-// power of two in [1, 64].  Reductions are deterministic: every block writes a
-// per-lane partial and the last block to finish (atomic ticket) sums the
-// partials in block order and runs the per-lane scalar logic.
+// interleaved, v[row * K + lane], so K neighbouring threads read one row's K
+// lanes in one contiguous access and the CSR structure is read once per
+// batch.  K is a compile-time power of two in [1, 64].
+//
+// Sparse passes walk "segments": CSR rows (y_i = sum_j w_ij (x_i - x_j)) or
+// aggregates (coarse_c = sum of the contiguous members' residuals).  The host
+// cuts segments into chunks of at most 64 entries.  A chunk is processed by a
+// group of G = K * V threads, V slices per lane striding over its entries and
+// combined by a fixed butterfly.  A segment with several chunks stores chunk
+// partials that a second, small kernel sums in chunk order; the kernel
+// boundary publishes them, so the hot pass needs no device-scope fence (on
+// gfx950 such a fence writes back and invalidates the L2 of the XCD).
+// Every order is fixed, so results do not depend on scheduling.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,8 +26,16 @@ namespace vk {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr int kMaxReduceBlocks = 1024;
+constexpr int kMaxReduceBlocks = 256;
 constexpr int kMaxReduceValues = 4;
+constexpr uint32_t kSingle = 0xffffffffu;
+
+// Threads per chunk: V slices of K lanes, G = K * V <= 64 (one wavefront).
+template <int K>
+struct Shape {
+  static constexpr int V = K >= 64 ? 1 : (K >= 4 ? 64 / K : 16);
+  static constexpr int G = K * V;
+};
 
 // ---------------------------------------------------------------- RNG
 // Counter-V1 Philox4x64-10 with SplitMix64 keys (rng.rs:14-99, 309-327).
@@ -75,16 +93,18 @@ __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
 // ---------------------------------------------------------------- reductions
 
 // Block-reduce NV values per lane (K lanes, kBlock/K rows per block), write
-// block partials and let the last block sum all partials in block order.
-// Returns true in the last block; totals[v * K + lane] then holds the sums for
+// block partials and let the last block sum all partials in block order,
+// followed by `extra` per-lane contributions (multi-chunk segments) in index
+// order.  Returns true in the last block; total[v] then holds the sums for
 // threads with threadIdx.x < K.
 template <int K, int NV>
 __device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ partial,
-                                    unsigned *__restrict__ counter, double (&total)[NV]) {
+                                    unsigned *__restrict__ counter, const double *__restrict__ extra,
+                                    uint32_t n_extra, double (&total)[NV]) {
   __shared__ double lds[NV][kBlock];
   __shared__ bool last;
   constexpr int R = kBlock / K;
-  const int tid = threadIdx.x, row = tid / K;
+  const int tid = threadIdx.x, row = tid / K, lane = tid % K;
 #pragma unroll
   for (int v = 0; v < NV; ++v) lds[v][tid] = val[v];
   __syncthreads();
@@ -105,11 +125,24 @@ __device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ part
   __syncthreads();
   if (!last) return false;
   __threadfence();
-  const volatile double *vp = partial;
+  // sum items row, row + R, ... in order, four loads in flight
+  auto item = [&](unsigned b, int v) -> double {
+    return b < gridDim.x ? partial[((uint64_t)b * NV + v) * K + lane]
+                         : extra[((uint64_t)(b - gridDim.x) * NV + v) * K + lane];
+  };
+  const unsigned items = gridDim.x + n_extra;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
-    for (unsigned b = row; b < gridDim.x; b += R) s += vp[((uint64_t)b * NV + v) * K + (tid % K)];
+    unsigned b = row;
+    for (; b + 3 * R < items; b += 4 * R) {
+      const double a0 = item(b, v), a1 = item(b + R, v), a2 = item(b + 2 * R, v), a3 = item(b + 3 * R, v);
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; b < items; b += R) s += item(b, v);
     lds[v][tid] = s;
   }
   __syncthreads();
@@ -126,154 +159,331 @@ __device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ part
   return true;
 }
 
-// ---------------------------------------------------------------- CSR rows
-// y_i = sum_j w_ij (x_i - x_j), neighbours ascending: the same sum order as
-// the reference's canonical edge scatter (vendor/cmg/src/graph.rs:664-705).
-template <typename T, int K>
-__device__ inline T row_laplacian(const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                                  const T *__restrict__ val, const T *__restrict__ x, uint32_t i,
-                                  int j, T xi) {
-  T acc = T(0);
-  const uint32_t e = rp[i + 1];
-  for (uint32_t k = rp[i]; k < e; ++k) acc += val[k] * (xi - x[(uint64_t)col[k] * K + j]);
-  return acc;
+// Plain block + last-block reduction with no extra contributions.
+template <int K, int NV>
+__device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ partial,
+                                    unsigned *__restrict__ counter, double (&total)[NV]) {
+  return reduce_lanes<K, NV>(val, partial, counter, nullptr, 0u, total);
 }
 
-// Same row with x replaced by x + P e (prolongation folded into the read).
+// ---------------------------------------------------------------- segments
+
+struct Chunks {
+  const uint4 *__restrict__ desc;     // segment, begin, end, multi index or kSingle
+  const uint32_t *__restrict__ mrange;  // first and end chunk of every multi segment
+  uint32_t count, n_multi;
+};
+
+// Shared scratch of the segment kernels (one stream, kernels in sequence).
+struct SegScratch {
+  double *partial;      // chunk partials of multi-chunk segments, chunks x K (as T)
+  double *bpartial;     // pass-1 block partials of the reduction, NV x K per block
+  double *rpartial;     // pass-2 block partials of the reduction
+  unsigned *rcounter;   // pass-2 last-block ticket
+  const int *gate;      // kernels return at once when *gate == 0
+};
+
 template <typename T, int K>
-__device__ inline T row_laplacian_prolonged(const uint32_t *__restrict__ rp,
-                                            const uint32_t *__restrict__ col,
-                                            const T *__restrict__ val, const T *__restrict__ x,
-                                            const uint32_t *__restrict__ lab,
-                                            const T *__restrict__ e, uint32_t i, int j, T xi) {
-  T acc = T(0);
-  const uint32_t end = rp[i + 1];
-  for (uint32_t k = rp[i]; k < end; ++k) {
-    const uint32_t c = col[k];
-    acc += val[k] * (xi - (x[(uint64_t)c * K + j] + e[(uint64_t)lab[c] * K + j]));
+__device__ inline T slice_sum(T v) {
+#pragma unroll
+  for (int off = K; off < Shape<K>::G; off <<= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Pass 1 of a segment operation.  Op supplies
+//   T partial(const uint4 &d, int j, int s)  sum over the chunk's entries
+//                                            d.y + s, d.y + s + V, ... < d.z
+//   void finish(uint32_t seg, int j, T total, double (&acc)[NA])
+//   void reduced(int j, const double (&tot)[NV])   (NV > 0 only)
+//   bool enabled()                           uniform early exit
+// Single-chunk segments are finished here; multi-chunk segments leave their
+// chunk partials for pass 2 (the kernel boundary publishes them, so no
+// device-scope fence is needed).  With NV > 0 every block stores its
+// reduction partial (no ticket).
+template <typename T, int K, int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratch sc) {
+  if (sc.gate && *sc.gate == 0) return;
+  if (!op.enabled()) return;
+  constexpr int G = Shape<K>::G, NA = NV > 0 ? NV : 1;
+  const int j = threadIdx.x % K, s = (threadIdx.x % G) / K;
+  double acc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
+  T *partial = reinterpret_cast<T *>(sc.partial);
+  const uint64_t total = (uint64_t)ch.count * G;
+  for (uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t u = (uint32_t)(t / G);
+    const uint4 d = ch.desc[u];
+    const T part = slice_sum<T, K>(op.partial(d, j, s));
+    if (s != 0) continue;
+    if (d.w == kSingle)
+      op.finish(d.x, j, part, acc);
+    else
+      partial[(uint64_t)u * K + j] = part;
   }
-  return acc;
-}
-
-// ---------------------------------------------------------------- V-cycle
-// cycle.rs:342-437: output = D^-1 rhs; a non-null mean first centres rhs in
-// place (coarse component centring, cycle.rs:459-478).
-template <typename T, int K>
-__global__ void k_scale(uint32_t n, const T *__restrict__ inv, T *__restrict__ b,
-                        T *__restrict__ x, const double *__restrict__ mean) {
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    T bi = b[t];
-    if (mean) {
-      bi = bi - (T)mean[j];
-      b[t] = bi;
+  if constexpr (NV > 0) {
+    __shared__ double lds[NA][kBlock];
+    constexpr int R = kBlock / K;
+    const int tid = threadIdx.x, row = tid / K;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lds[v][tid] = acc[v];
+    __syncthreads();
+    for (int w = R / 2; w > 0; w >>= 1) {
+      if (row < w) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w * K];
+      }
+      __syncthreads();
     }
-    x[t] = inv[i] * bi;
-  }
-}
-
-// y = x + D^-1 (b - A x)   (cycle.rs:380-437 after the level matvec)
-template <typename T, int K>
-__global__ void k_smooth(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                         const T *__restrict__ val, const T *__restrict__ inv, const T *__restrict__ b,
-                         const T *__restrict__ x, T *__restrict__ y) {
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    const T xi = x[t];
-    const T ax = row_laplacian<T, K>(rp, col, val, x, i, j, xi);
-    y[t] = xi + inv[i] * (b[t] - ax);
-  }
-}
-
-// y = x' + D^-1 (b - A x') with x' = x + P e  (prolong-add then smoothing)
-template <typename T, int K>
-__global__ void k_prolong_smooth(uint32_t n, const uint32_t *__restrict__ rp,
-                                 const uint32_t *__restrict__ col, const T *__restrict__ val,
-                                 const T *__restrict__ inv, const uint32_t *__restrict__ lab,
-                                 const T *__restrict__ e, const T *__restrict__ b,
-                                 const T *__restrict__ x, T *__restrict__ y) {
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    const T xi = x[t] + e[(uint64_t)lab[i] * K + j];
-    const T ax = row_laplacian_prolonged<T, K>(rp, col, val, x, lab, e, i, j, xi);
-    y[t] = xi + inv[i] * (b[t] - ax);
-  }
-}
-
-// coarse[c] = sum over members v ascending of (b_v - (A x)_v)  (coarsen.rs:476-503)
-// plus the coarse mean per lane for the centring of the next level.
-template <typename T, int K>
-__global__ void k_restrict(uint32_t nc, const uint32_t *__restrict__ aptr,
-                           const uint32_t *__restrict__ amem, const uint32_t *__restrict__ rp,
-                           const uint32_t *__restrict__ col, const T *__restrict__ val,
-                           const T *__restrict__ b, const T *__restrict__ x, T *__restrict__ bc,
-                           double *__restrict__ partial, unsigned *__restrict__ counter,
-                           double *__restrict__ mean) {
-  double acc[1] = {0.0};
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)nc * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t c = t / K;
-    const int j = t % K;
-    T s = T(0);
-    const uint32_t end = aptr[c + 1];
-    for (uint32_t m = aptr[c]; m < end; ++m) {
-      const uint32_t v = amem[m];
-      const T xv = x[(uint64_t)v * K + j];
-      s += b[(uint64_t)v * K + j] - row_laplacian<T, K>(rp, col, val, x, v, j, xv);
+    if (tid < K) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sc.bpartial[((uint64_t)blockIdx.x * NV + v) * K + tid] = lds[v][tid];
     }
-    bc[t] = s;
-    acc[0] += (double)s;
   }
-  double tot[1];
-  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
-    mean[threadIdx.x] = tot[0] / (double)nc;
 }
 
-// Direct terminal: x = G (b - mean), G the grounded inverse (ldl.rs:892-997).
+// Pass 2: every multi-chunk segment is summed by one block (kBlock / K
+// slices per lane, a fixed LDS tree) in chunk order and finished; with NV > 0
+// the last block adds the pass-2 block partials and then the `blocks1`
+// pass-1 block partials, in order, and runs Op::reduced.  Launched whenever
+// there are multi-chunk segments or NV > 0.
+template <typename T, int K, int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, SegScratch sc, uint32_t blocks1) {
+  if (sc.gate && *sc.gate == 0) return;
+  if (!op.enabled()) return;
+  constexpr int S = kBlock / K, NA = NV > 0 ? NV : 1;
+  __shared__ T tree[kBlock];
+  const int j = threadIdx.x % K, s = threadIdx.x / K;
+  double acc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
+  const T *partial = reinterpret_cast<const T *>(sc.partial);
+  for (uint32_t m = blockIdx.x; m < ch.n_multi; m += gridDim.x) {
+    const uint32_t first = ch.mrange[2 * m], end = ch.mrange[2 * m + 1];
+    T sum = T(0);
+    uint32_t q = first + s;
+    for (; q + 3 * S < end; q += 4 * S) {
+      const T a0 = partial[(uint64_t)q * K + j], a1 = partial[(uint64_t)(q + S) * K + j];
+      const T a2 = partial[(uint64_t)(q + 2 * S) * K + j], a3 = partial[(uint64_t)(q + 3 * S) * K + j];
+      sum += a0;
+      sum += a1;
+      sum += a2;
+      sum += a3;
+    }
+    for (; q < end; q += S) sum += partial[(uint64_t)q * K + j];
+    tree[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = S / 2; w > 0; w >>= 1) {
+      if (s < w) tree[threadIdx.x] += tree[threadIdx.x + w * K];
+      __syncthreads();
+    }
+    if (s == 0) op.finish(ch.desc[first].x, j, tree[j], acc);
+    __syncthreads();
+  }
+  if constexpr (NV > 0) {
+    double tot[NA];
+    if (reduce_lanes<K, NA>(acc, sc.rpartial, sc.rcounter, sc.bpartial, blocks1, tot) && threadIdx.x < K)
+      op.reduced(threadIdx.x, tot);
+  }
+}
+
+// Row chunk of y = A x with the neighbour values supplied by D:
+// sum over entries of w_ik (x_i - x_k), the reference's per-edge form
+// (vendor/cmg/src/graph.rs:664-705).
+template <typename T, int K, class D>
+struct CsrOp {
+  const uint32_t *__restrict__ col;
+  const T *__restrict__ val;
+  __device__ T partial(const uint4 &d, int j, int s) const {
+    const D &op = static_cast<const D &>(*this);
+    const T xi = op.self(d.x, j);
+    T acc = T(0);
+#pragma unroll 4
+    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) acc += val[k] * (xi - op.nbr(col[k], j));
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NV>
+  __device__ void reduced(int, const double (&)[NV]) const {}
+};
+
+// ---------------------------------------------------------------- V-cycle ops
+// cycle.rs:136-437.  Level vectors: braw (the parent's restricted residual),
+// b (centred), x, y, r.
+
+// First visit: b = braw - mean, x = D^-1 b, r = b - A x  (it == 0 scale,
+// matvec; coarse centring cycle.rs:459-478 folded in).
 template <typename T, int K>
-__global__ void k_terminal_direct(uint32_t n, const double *__restrict__ g, const T *__restrict__ b,
-                                  const double *__restrict__ mean, T *__restrict__ x) {
+struct OpScaleResidual : CsrOp<T, K, OpScaleResidual<T, K>> {
+  const T *__restrict__ braw;
+  const double *__restrict__ mean;
+  const T *__restrict__ inv;
+  T *__restrict__ b, *__restrict__ x, *__restrict__ r;
+  __device__ T centred(uint32_t i, int j) const {
+    const T v = braw[(uint64_t)i * K + j];
+    return mean ? v - (T)mean[j] : v;
+  }
+  __device__ T self(uint32_t i, int j) const { return inv[i] * centred(i, j); }
+  __device__ T nbr(uint32_t c, int j) const { return inv[c] * centred(c, j); }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    const T bi = centred(i, j);
+    b[q] = bi;
+    x[q] = inv[i] * bi;
+    r[q] = bi - ax;
+  }
+};
+
+// Later visits: y = x + D^-1 (b - A x)
+template <typename T, int K>
+struct OpSmooth : CsrOp<T, K, OpSmooth<T, K>> {
+  const T *__restrict__ b, *__restrict__ x, *__restrict__ inv;
+  T *__restrict__ y;
+  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    y[q] = x[q] + inv[i] * (b[q] - ax);
+  }
+};
+
+// r = b - A x
+template <typename T, int K>
+struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
+  const T *__restrict__ b, *__restrict__ x;
+  T *__restrict__ r;
+  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    r[q] = b[q] - ax;
+  }
+};
+
+// y = x' + D^-1 (b - A x'), x' = x + P e  (prolong-add, matvec, Jacobi)
+template <typename T, int K>
+struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K>> {
+  const T *__restrict__ b, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
+  const uint32_t *__restrict__ lab;
+  T *__restrict__ y;
+  __device__ T self(uint32_t i, int j) const {
+    return x[(uint64_t)i * K + j] + e[(uint64_t)lab[i] * K + j];
+  }
+  __device__ T nbr(uint32_t c, int j) const { return self(c, j); }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    y[q] = self(i, j) + inv[i] * (b[q] - ax);
+  }
+};
+
+// y = A x (diagnostics)
+template <typename T, int K>
+struct OpProduct : CsrOp<T, K, OpProduct<T, K>> {
+  const T *__restrict__ x;
+  T *__restrict__ y;
+  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    y[(uint64_t)i * K + j] = ax;
+  }
+};
+
+// Restriction: coarse[c] = sum over the aggregate's contiguous members of r
+// (coarsen.rs:476-503), plus the per-lane coarse mean for the centring.
+template <typename T, int K>
+struct OpAggregate {
+  const T *__restrict__ r;
+  T *__restrict__ out;
+  double *__restrict__ mean;
+  uint32_t coarse;
+  __device__ T partial(const uint4 &d, int j, int s) const {
+    T acc = T(0);
+    for (uint32_t m = d.y + s; m < d.z; m += Shape<K>::V) acc += r[(uint64_t)m * K + j];
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NA>
+  __device__ void finish(uint32_t c, int j, T sum, double (&acc)[NA]) const {
+    out[(uint64_t)c * K + j] = sum;
+    acc[0] += (double)sum;
+  }
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const {
+    mean[j] = tot[0] / (double)coarse;
+  }
+};
+
+// Direct terminal: b = braw - mean, x = G b with G the grounded inverse
+// (ldl.rs:892-997).
+template <typename T, int K>
+__global__ void k_terminal_direct(uint32_t n, const double *__restrict__ g, const T *__restrict__ braw,
+                                  const double *__restrict__ mean, T *__restrict__ b, T *__restrict__ x,
+                                  const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t i = t / K;
     const int j = t % K;
     const double mu = mean ? mean[j] : 0.0;
     double s = 0.0;
-    for (uint32_t k = 0; k < n; ++k) s += g[(uint64_t)i * n + k] * ((double)b[(uint64_t)k * K + j] - mu);
+    for (uint32_t k = 0; k < n; ++k) s += g[(uint64_t)i * n + k] * ((double)braw[(uint64_t)k * K + j] - mu);
     x[t] = (T)s;
+    b[t] = (T)((double)braw[t] - mu);
   }
 }
 
-// Plain level product y = A x (diagnostics and tests).
+// Iterative terminal: b = braw - mean, x = D^-1 b  (cycle.rs:342-379)
 template <typename T, int K>
-__global__ void k_spmv(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                       const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y) {
+__global__ void k_terminal_scale(uint32_t n, const T *__restrict__ inv, const T *__restrict__ braw,
+                                 const double *__restrict__ mean, T *__restrict__ b, T *__restrict__ x,
+                                 const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t i = t / K;
     const int j = t % K;
-    y[t] = row_laplacian<T, K>(rp, col, val, x, i, j, x[t]);
+    const T bi = mean ? braw[t] - (T)mean[j] : braw[t];
+    b[t] = bi;
+    x[t] = inv[i] * bi;
   }
 }
 
 // ---------------------------------------------------------------- precision casts
+// rc = r - mean (FP64, kept for the Krylov recurrences) and its FP32 copy for
+// the V-cycle.
 template <int K>
-__global__ void k_to_float(uint64_t count, const double *__restrict__ a, float *__restrict__ b) {
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
-       t += (uint64_t)gridDim.x * blockDim.x)
-    b[t] = (float)a[t];
+__global__ void k_centre_split(uint64_t rows, const double *__restrict__ r, const double *__restrict__ mean,
+                               double *__restrict__ rc, float *__restrict__ rf, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < rows * K;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const double v = mean ? r[t] - mean[t % K] : r[t];
+    rc[t] = v;
+    rf[t] = (float)v;
+  }
 }
 template <int K>
-__global__ void k_to_double(uint64_t count, const float *__restrict__ a, double *__restrict__ b) {
+__global__ void k_to_double(uint64_t count, const float *__restrict__ a, double *__restrict__ b,
+                            const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
        t += (uint64_t)gridDim.x * blockDim.x)
     b[t] = (double)a[t];
+}
+
+// Row permutation of K-lane vectors: out[p] = in[perm[p]] (gather).
+template <int K>
+__global__ void k_gather_rows(uint64_t rows, const uint32_t *__restrict__ idx, const double *__restrict__ in,
+                              double *__restrict__ out) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < rows * K;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    out[t] = in[(uint64_t)idx[t / K] * K + t % K];
 }
 
 }  // namespace dev

@@ -24,6 +24,14 @@ struct PcgParams {
   int32_t max_iterations, interval;
 };
 
+// Device flags of the lockstep PCG: [0] some lane is active (the gate of
+// every iteration kernel), [1] some active lane needs a fresh residual,
+// [2] constant 1 (always-open gate).
+enum { kFlagActive = 0, kFlagFresh = 1, kFlagOne = 2 };
+
+#define VK_GATE(gate) \
+  if ((gate) && *(gate) == 0) return
+
 // ---- setup: compatibility check, norms (components.rs:669-744, pcg.rs:1596-1660)
 template <int K>
 __global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__restrict__ partial,
@@ -52,7 +60,8 @@ __global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__
   }
 }
 
-// proj = b - mean, x = 0 or centred guess handled on the host, r = proj.
+// proj = b - mean, r = proj, x = 0 (cold start; a warm start passes a
+// scratch buffer as `x`).
 template <int K>
 __global__ void k_pcg_project(uint32_t n, const double *__restrict__ b, double *__restrict__ proj,
                               double *__restrict__ r, double *__restrict__ x,
@@ -83,30 +92,30 @@ __global__ void k_pcg_start(ColState *__restrict__ st, PcgParams prm) {
 
 // Warm start: r = proj - A x (x centred), initial residual of the original rhs.
 template <int K>
-__global__ void k_pcg_warm(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                           const double *__restrict__ val, const double *__restrict__ b,
-                           const double *__restrict__ proj, const double *__restrict__ x,
-                           double *__restrict__ r, double *__restrict__ partial,
-                           unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm) {
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    const double xi = x[t];
-    const double rr = proj[t] - row_laplacian<double, K>(rp, col, val, x, i, j, xi);
-    r[t] = rr;
-    const double o = rr + (b[t] - proj[t]);
+struct OpWarm : CsrOp<double, K, OpWarm<K>> {
+  const double *__restrict__ b, *__restrict__ proj, *__restrict__ x;
+  double *__restrict__ r;
+  ColState *__restrict__ st;
+  PcgParams prm;
+  __device__ double self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    const double xi = x[q], rr = proj[q] - ax;
+    r[q] = rr;
+    const double o = rr + (b[q] - proj[q]);
     acc[0] += o * o;
     acc[1] += xi * xi;
     acc[2] += rr;
     acc[3] += rr * rr;
   }
-  double tot[4];
-  if (reduce_lanes<K, 4>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const {
+    ColState &s = st[j];
     s.init_norm = sqrt(tot[0]);
     s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * sqrt(tot[1]));
-    s.mean_r = tot[2] / (double)n;
+    s.mean_r = tot[2] / (double)prm.n;
     if (s.active && s.init_norm <= s.tol) {
       s.active = 0;
     } else if (s.active && tot[3] == 0.0) {
@@ -114,22 +123,35 @@ __global__ void k_pcg_warm(uint32_t n, const uint32_t *__restrict__ rp, const ui
       s.active = 0;
     }
   }
-}
+};
 
 template <int K>
-__global__ void k_center_cols(uint64_t rows, double *__restrict__ v, const double *__restrict__ mean) {
+__global__ void k_center_cols(uint64_t rows, double *__restrict__ v, const double *__restrict__ mean,
+                              const int *__restrict__ gate = nullptr) {
+  VK_GATE(gate);
   VK_GRID_LOOP(t, rows * K) v[t] -= mean[t % K];
 }
 
-// Lane means of the active lanes of the PCG state (zero for finished lanes,
-// whose iterates stay frozen).
+// Lane means of the active lanes (zero for finished lanes, whose iterates stay
+// frozen) and the any-lane flags.
 template <int K>
 __global__ void k_state_means(const ColState *__restrict__ st, double *__restrict__ mean_x,
-                              double *__restrict__ mean_r) {
+                              double *__restrict__ mean_r, int *__restrict__ flags,
+                              const int *__restrict__ gate) {
+  VK_GATE(gate);
   const int j = threadIdx.x;
-  if (j >= K) return;
-  mean_x[j] = st[j].active ? st[j].mean_x : 0.0;
-  mean_r[j] = st[j].active ? st[j].mean_r : 0.0;
+  const bool lane = j < K;
+  const bool active = lane && st[j].active;
+  const bool fresh = active && st[j].need_fresh;
+  if (lane) {
+    mean_x[j] = active ? st[j].mean_x : 0.0;
+    mean_r[j] = active ? st[j].mean_r : 0.0;
+  }
+  const unsigned long long any_active = __ballot(active), any_fresh = __ballot(fresh);
+  if (j == 0) {
+    flags[kFlagActive] = any_active != 0ULL;
+    flags[kFlagFresh] = any_fresh != 0ULL;
+  }
 }
 
 // Zero the means of lanes that are not part of the current solve.
@@ -139,11 +161,12 @@ __global__ void k_mask_means(double *__restrict__ mean, const uint8_t *__restric
   if (j < K && !use[j]) mean[j] = 0.0;
 }
 
-// z centred implicitly: rho = r.z - mean_z * sum r (pcg.rs:1912-1947).
+// rho = r.z - mean_z * sum r with r already centred (pcg.rs:1912-1947).
 template <int K>
 __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double *__restrict__ z,
                           double *__restrict__ partial, unsigned *__restrict__ counter,
-                          ColState *__restrict__ st) {
+                          ColState *__restrict__ st, const int *__restrict__ gate) {
+  VK_GATE(gate);
   double acc[3] = {0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
     const double zz = z[t], rr = r[t];
@@ -174,7 +197,8 @@ __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double
 // p = z_c (first or after a residual replacement) or z_c + beta p.
 template <int K>
 __global__ void k_pcg_direction(uint32_t n, const double *__restrict__ z, double *__restrict__ p,
-                                ColState *__restrict__ st) {
+                                const ColState *__restrict__ st, const int *__restrict__ gate) {
+  VK_GATE(gate);
   VK_GRID_LOOP(t, (uint64_t)n * K) {
     const int j = t % K;
     const ColState &s = st[j];
@@ -186,23 +210,21 @@ __global__ void k_pcg_direction(uint32_t n, const double *__restrict__ z, double
 
 // Ap = A p and the curvature p'Ap; alpha = rho / p'Ap (pcg.rs:1665-1690).
 template <int K>
-__global__ void k_pcg_curvature(uint32_t n, const uint32_t *__restrict__ rp,
-                                const uint32_t *__restrict__ col, const double *__restrict__ val,
-                                const double *__restrict__ p, double *__restrict__ ap,
-                                double *__restrict__ partial, unsigned *__restrict__ counter,
-                                ColState *__restrict__ st) {
-  double acc[1] = {0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    const double pi = p[t];
-    const double a = row_laplacian<double, K>(rp, col, val, p, i, j, pi);
-    ap[t] = a;
-    acc[0] += pi * a;
+struct OpCurvature : CsrOp<double, K, OpCurvature<K>> {
+  const double *__restrict__ p;
+  double *__restrict__ ap;
+  ColState *__restrict__ st;
+  __device__ double self(uint32_t i, int j) const { return p[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j) const { return p[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    ap[q] = ax;
+    acc[0] += p[q] * ax;
   }
-  double tot[1];
-  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const {
+    ColState &s = st[j];
     s.first = 0;
     if (!s.active) return;
     const double curv = tot[0];
@@ -213,26 +235,28 @@ __global__ void k_pcg_curvature(uint32_t n, const uint32_t *__restrict__ rp,
     }
     s.alpha = s.rho / curv;
   }
-}
+};
 
-// x += alpha p, r -= alpha Ap; norms of the centred x and of r; candidate and
-// scheduled residual replacement flags (pcg.rs:1692-1760).
+// x += alpha p, r = rc - alpha Ap (rc: the centred residual the V-cycle saw);
+// norms of the centred x and of r; candidate and scheduled residual
+// replacement flags (pcg.rs:1692-1760).
 template <int K>
 __global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const double *__restrict__ ap,
-                             double *__restrict__ x, double *__restrict__ r,
+                             double *__restrict__ x, const double *__restrict__ rc, double *__restrict__ r,
                              double *__restrict__ partial, unsigned *__restrict__ counter,
-                             ColState *__restrict__ st, PcgParams prm) {
+                             ColState *__restrict__ st, PcgParams prm, const int *__restrict__ gate) {
+  VK_GATE(gate);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
     const int j = t % K;
     const ColState &s = st[j];
-    double xv = x[t], rv = r[t];
+    double xv = x[t], rv = rc[t];
     if (s.active) {
       xv += s.alpha * p[t];
       rv -= s.alpha * ap[t];
       x[t] = xv;
-      r[t] = rv;
     }
+    r[t] = rv;
     acc[0] += xv;
     acc[1] += xv * xv;
     acc[2] += rv;
@@ -257,33 +281,35 @@ __global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const dou
 // Fresh residual for flagged lanes: r = proj - A x (x already centred), the
 // certificate against the unprojected rhs (pcg.rs:1757-1817, 2209-2236).
 template <int K>
-__global__ void k_pcg_fresh(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
-                            const double *__restrict__ val, const double *__restrict__ b,
-                            const double *__restrict__ proj, const double *__restrict__ x,
-                            double *__restrict__ r, double *__restrict__ partial,
-                            unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm) {
-  double acc[3] = {0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    if (!(st[j].active && st[j].need_fresh)) continue;
-    const double xi = x[t];
-    const double rr = proj[t] - row_laplacian<double, K>(rp, col, val, x, i, j, xi);
-    r[t] = rr;
-    const double o = rr + (b[t] - proj[t]);
+struct OpFresh : CsrOp<double, K, OpFresh<K>> {
+  const double *__restrict__ b, *__restrict__ proj, *__restrict__ x;
+  double *__restrict__ r;
+  ColState *__restrict__ st;
+  PcgParams prm;
+  const int *__restrict__ flags;
+  __device__ bool enabled() const { return flags[kFlagFresh] != 0; }
+  __device__ double self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
+    if (!(st[j].active && st[j].need_fresh)) return;
+    const uint64_t q = (uint64_t)i * K + j;
+    const double rr = proj[q] - ax;
+    r[q] = rr;
+    const double o = rr + (b[q] - proj[q]);
     acc[0] += rr * rr;
     acc[1] += o * o;
     acc[2] += rr;
   }
-  double tot[3];
-  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const {
+    ColState &s = st[j];
     if (!(s.active && s.need_fresh)) return;
     s.restarted = 1;
     s.restarts += 1;
     s.fresh = sqrt(tot[0]);
     s.orig = sqrt(tot[1]);
-    s.mean_r = tot[2] / (double)n;
+    s.mean_r = tot[2] / (double)prm.n;
     if (s.fresh <= s.tol) {
       if (s.orig <= s.tol) {
         s.active = 0;
@@ -300,17 +326,23 @@ __global__ void k_pcg_fresh(uint32_t n, const uint32_t *__restrict__ rp, const u
       s.active = 0;
     }
   }
-}
+};
 
+// Iteration limit and the active flag for the next iteration.
 template <int K>
-__global__ void k_pcg_limit(ColState *__restrict__ st, PcgParams prm) {
+__global__ void k_pcg_limit(ColState *__restrict__ st, PcgParams prm, int *__restrict__ flags) {
   const int j = threadIdx.x;
-  if (j >= K) return;
-  ColState &s = st[j];
-  if (s.active && !s.need_fresh && s.iters >= prm.max_iterations) {
-    s.status = 63;
-    s.active = 0;
+  bool active = false;
+  if (j < K) {
+    ColState &s = st[j];
+    if (s.active && !s.need_fresh && s.iters >= prm.max_iterations) {
+      s.status = 63;
+      s.active = 0;
+    }
+    active = s.active != 0;
   }
+  const unsigned long long any = __ballot(active);
+  if (j == 0) flags[kFlagActive] = any != 0ULL;
 }
 
 // ---------------------------------------------------------------- estimator passes
@@ -347,32 +379,6 @@ __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
   }
 }
 
-// Schur right-hand side b_F = firm_rhs - sum_c w_c worker_rhs[w_c] / d_w
-// (operator.rs:269-304), uncentred, plus its column mean.
-template <int K>
-__global__ void k_schur_rhs(uint32_t firms, const uint64_t *__restrict__ fptr,
-                            const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
-                            const double *__restrict__ cell_weight, const double *__restrict__ wdiag,
-                            const double *__restrict__ wrhs, const double *__restrict__ frhs,
-                            double *__restrict__ out, double *__restrict__ partial,
-                            unsigned *__restrict__ counter, double *__restrict__ mean) {
-  double acc[1] = {0.0};
-  VK_GRID_LOOP(t, (uint64_t)firms * K) {
-    const uint32_t f = t / K;
-    const int j = t % K;
-    double v = frhs[t];
-    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
-      const uint32_t c = fcells[k], w = cell_worker[c];
-      v -= cell_weight[c] * (wrhs[(uint64_t)w * K + j] / wdiag[w]);
-    }
-    out[t] = v;
-    acc[0] += v;
-  }
-  double tot[1];
-  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
-    mean[threadIdx.x] = tot[0] / (double)firms;
-}
-
 // wsum[w] = sum_c w_c v[f_c] / d_w  (operator.rs:210-262 first half)
 template <int K>
 __global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ wptr,
@@ -386,31 +392,6 @@ __global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ w
     for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) s += cell_weight[c] * v[(uint64_t)cell_firm[c] * K + j];
     out[t] = s / wdiag[w];
   }
-}
-
-// out[f] = d_f red[f] - sum_c w_c wsum[w_c]  and its column mean.
-template <int K>
-__global__ void k_schur_apply(uint32_t firms, const uint64_t *__restrict__ fptr,
-                              const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
-                              const double *__restrict__ cell_weight, const double *__restrict__ fdiag,
-                              const double *__restrict__ red, const double *__restrict__ wsum,
-                              double *__restrict__ out, double *__restrict__ partial,
-                              unsigned *__restrict__ counter, double *__restrict__ mean) {
-  double acc[1] = {0.0};
-  VK_GRID_LOOP(t, (uint64_t)firms * K) {
-    const uint32_t f = t / K;
-    const int j = t % K;
-    double v = fdiag[f] * red[t];
-    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
-      const uint32_t c = fcells[k];
-      v -= cell_weight[c] * wsum[(uint64_t)cell_worker[c] * K + j];
-    }
-    out[t] = v;
-    acc[0] += v;
-  }
-  double tot[1];
-  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
-    mean[threadIdx.x] = tot[0] / (double)firms;
 }
 
 // reduced relative residual ||rhs_F - (out - mean)|| / ||rhs_F||  (full_cmg.rs:1425-1443)
@@ -461,37 +442,117 @@ __global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__
   }
 }
 
-// firm block of the complete residual; result = complete relative residual.
-template <int K>
-__global__ void k_firm_residual(uint32_t firms, const uint64_t *__restrict__ fptr,
-                                const uint32_t *__restrict__ fcells, const uint32_t *__restrict__ cell_worker,
-                                const double *__restrict__ cell_weight, const double *__restrict__ fdiag,
-                                const double *__restrict__ frhs, const double *__restrict__ firm,
-                                const double *__restrict__ worker, const double *__restrict__ wsums,
-                                double *__restrict__ partial, unsigned *__restrict__ counter,
-                                double *__restrict__ result, double *__restrict__ rhs_norm) {
-  double acc[2] = {0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)firms * K) {
-    const uint32_t f = t / K;
-    const int j = t % K;
-    const double b = frhs[t];
-    double rf = b - fdiag[f] * firm[t];
-    for (uint64_t k = fptr[f]; k < fptr[f + 1]; ++k) {
+// ---------------------------------------------------------------- firm passes
+// Firm rows walk their cells (firm_cells order); Pareto-sized firms own
+// hundreds of thousands of cells, so they run as chunked segments.
+
+// Chunk of a firm's cells: sum over cells c of w_c * term(worker of c).
+template <int K, class D>
+struct FirmCellOp {
+  const uint32_t *__restrict__ fcells, *__restrict__ cell_worker;
+  const double *__restrict__ cell_weight;
+  __device__ double partial(const uint4 &d, int j, int s) const {
+    const D &op = static_cast<const D &>(*this);
+    double acc = 0.0;
+#pragma unroll 4
+    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
       const uint32_t c = fcells[k];
-      rf -= cell_weight[c] * worker[(uint64_t)cell_worker[c] * K + j];
+      acc += cell_weight[c] * op.term(cell_worker[c], j);
     }
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NV>
+  __device__ void reduced(int, const double (&)[NV]) const {}
+};
+
+// Exact-integer leverage right-hand side of a firm (engine.rs:2562-2619):
+// sum of the atoms of the units of its cells.
+template <int K>
+struct OpFirmUnits {
+  const uint32_t *__restrict__ fcells;
+  const uint64_t *__restrict__ cu_ptr;
+  const uint32_t *__restrict__ cu;
+  const double *__restrict__ atoms;
+  uint64_t units;
+  uint32_t kcols;
+  double *__restrict__ out;
+  __device__ double partial(const uint4 &d, int j, int s) const {
+    double acc = 0.0;
+    if (j >= (int)kcols) return acc;
+    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
+      const uint32_t c = fcells[k];
+      for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += atoms[(uint64_t)j * units + cu[q]];
+    }
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NA>
+  __device__ void finish(uint32_t f, int j, double v, double (&)[NA]) const { out[(uint64_t)f * K + j] = v; }
+  template <int NV>
+  __device__ void reduced(int, const double (&)[NV]) const {}
+};
+
+// Schur right-hand side b_F = firm_rhs - sum_c w_c worker_rhs[w_c] / d_w
+// (operator.rs:269-304), uncentred, plus its column mean.
+template <int K>
+struct OpSchurRhs : FirmCellOp<K, OpSchurRhs<K>> {
+  const double *__restrict__ wdiag, *__restrict__ wrhs, *__restrict__ frhs;
+  double *__restrict__ out, *__restrict__ mean;
+  uint32_t firms;
+  __device__ double term(uint32_t w, int j) const { return wrhs[(uint64_t)w * K + j] / wdiag[w]; }
+  template <int NA>
+  __device__ void finish(uint32_t f, int j, double sum, double (&acc)[NA]) const {
+    const double v = frhs[(uint64_t)f * K + j] - sum;
+    out[(uint64_t)f * K + j] = v;
+    acc[0] += v;
+  }
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const { mean[j] = tot[0] / (double)firms; }
+};
+
+// out[f] = d_f red[f] - sum_c w_c wsum[w_c]  and its column mean.
+template <int K>
+struct OpSchurApply : FirmCellOp<K, OpSchurApply<K>> {
+  const double *__restrict__ fdiag, *__restrict__ red, *__restrict__ wsum;
+  double *__restrict__ out, *__restrict__ mean;
+  uint32_t firms;
+  __device__ double term(uint32_t w, int j) const { return wsum[(uint64_t)w * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t f, int j, double sum, double (&acc)[NA]) const {
+    const uint64_t q = (uint64_t)f * K + j;
+    const double v = fdiag[f] * red[q] - sum;
+    out[q] = v;
+    acc[0] += v;
+  }
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const { mean[j] = tot[0] / (double)firms; }
+};
+
+// Firm block of the complete two-way residual (operator.rs:386-458); the
+// result is the complete relative residual with the worker block sums.
+template <int K>
+struct OpFirmResidual : FirmCellOp<K, OpFirmResidual<K>> {
+  const double *__restrict__ fdiag, *__restrict__ frhs, *__restrict__ firm, *__restrict__ worker,
+      *__restrict__ wsums;
+  double *__restrict__ result, *__restrict__ rhs_norm;
+  __device__ double term(uint32_t w, int j) const { return worker[(uint64_t)w * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t f, int j, double sum, double (&acc)[NA]) const {
+    const uint64_t q = (uint64_t)f * K + j;
+    const double b = frhs[q];
+    const double rf = b - fdiag[f] * firm[q] - sum;
     acc[0] += rf * rf;
     acc[1] += b * b;
   }
-  double tot[2];
-  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K) {
-    const int j = threadIdx.x;
+  template <int NV>
+  __device__ void reduced(int j, const double (&tot)[NV]) const {
     const double absn = sqrt(fmax(wsums[j] + tot[0], 0.0));
     const double rhsn = sqrt(fmax(wsums[K + j] + tot[1], 0.0));
     result[j] = rhsn == 0.0 ? absn : absn / rhsn;
     rhs_norm[j] = rhsn;
   }
-}
+};
 
 // FiveMoments accumulation per deletion unit in probe order (engine.rs:1190-1228,
 // 354-374): SoA state m[10][units] = 5 x (sum, correction).
@@ -633,6 +694,32 @@ __global__ void k_target_scores(uint64_t rows, const uint64_t *__restrict__ ptr,
   }
 }
 
+// Firm scores of the target directions in the 2-column layout: column
+// 2j + 1 = sum over the firm's cells of the centred direction, column 2j = 0
+// (engine.rs:2705-2971).  Chunked like the other firm passes.
+template <int KT>
+struct OpFirmTargetScores {
+  const uint32_t *__restrict__ fcells;
+  const double *__restrict__ dir, *__restrict__ cell_target, *__restrict__ totals;
+  double target_total;
+  double *__restrict__ out;
+  __device__ double partial(const uint4 &d, int l, int s) const {
+    double acc = 0.0;
+    if ((l & 1) == 0) return acc;
+    const int j = l / 2;
+    for (uint32_t k = d.y + s; k < d.z; k += Shape<2 * KT>::V) {
+      const uint32_t c = fcells[k];
+      acc += dir[(uint64_t)c * KT + j] - cell_target[c] / target_total * totals[j];
+    }
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NA>
+  __device__ void finish(uint32_t f, int l, double v, double (&)[NA]) const { out[(uint64_t)f * 2 * KT + l] = v; }
+  template <int NV>
+  __device__ void reduced(int, const double (&)[NV]) const {}
+};
+
 // Compatibility repair of the last score entry (engine.rs:2842-2881).
 template <int K2>
 __global__ void k_balance(uint64_t rows, double *__restrict__ v, int parity,
@@ -684,20 +771,34 @@ __global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_wor
   }
 }
 
-// Harmonic lift of auxiliary vertices for a warm start (cmg_impl.rs.in:459-500).
+// Harmonic lift of auxiliary vertices for a warm start (cmg_impl.rs.in:459-500):
+// x at the renumbered position of aux vertex a = weighted mean of its firms'
+// normalised values `firm`.
 template <int K>
 __global__ void k_lift(uint32_t firms, uint32_t aux, const uint64_t *__restrict__ aptr,
                        const uint32_t *__restrict__ anbr, const double *__restrict__ aw,
+                       const uint32_t *__restrict__ pos, const double *__restrict__ firm,
                        double *__restrict__ x) {
   VK_GRID_LOOP(t, (uint64_t)aux * K) {
     const uint32_t a = t / K;
     const int j = t % K;
     double num = 0.0, den = 0.0;
     for (uint64_t k = aptr[a]; k < aptr[a + 1]; ++k) {
-      num += aw[k] * x[(uint64_t)anbr[k] * K + j];
+      num += aw[k] * firm[(uint64_t)anbr[k] * K + j];
       den += aw[k];
     }
-    x[(uint64_t)(firms + a) * K + j] = num / den;
+    x[(uint64_t)pos[firms + a] * K + j] = num / den;
+  }
+}
+
+// Hybrid right-hand side in the renumbered order: firm entries from the
+// centred Schur right-hand side, auxiliary entries zero (full_cmg.rs:842-895).
+template <int K>
+__global__ void k_hybrid_rhs(uint64_t rows, uint32_t firms, const uint32_t *__restrict__ perm,
+                             const double *__restrict__ sch, double *__restrict__ b) {
+  VK_GRID_LOOP(t, rows * K) {
+    const uint32_t v = perm[t / K];
+    b[t] = v < firms ? sch[(uint64_t)v * K + t % K] : 0.0;
   }
 }
 

@@ -641,6 +641,34 @@ int32_t vckss_mi355x_hierarchy_level_v1(uint64_t handle, uint32_t level, uint64_
   });
 }
 
+int32_t vckss_mi355x_layout_v1(uint64_t handle, uint32_t level, uint32_t chunk, uint64_t *sizes, uint32_t *perm,
+                               uint32_t *row_ptr, uint32_t *col, double *val, double *inv, uint32_t *lab,
+                               uint32_t *agg_ptr, uint32_t *row_chunks, uint32_t *row_multi, uint32_t *agg_chunks,
+                               uint32_t *agg_multi, double *terminal_inverse) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    auto it = g_hierarchies.find(handle);
+    if (it == g_hierarchies.end()) fail(kStaleContext, "layout", "unknown handle");
+    const Hierarchy &H = *it->second;
+    if (level >= H.levels.size()) fail(kInvalidInput, "layout", "level out of range");
+    if (!sizes) fail(kInvalidInput, "layout", "sizes pointer is null");
+    const std::vector<LevelLayout> all = device_layout(H, chunk);
+    const LevelLayout &L = all[level];
+    const uint64_t s[8] = {L.perm.size(), L.col.size(), L.agg_ptr.empty() ? 0 : L.agg_ptr.size() - 1,
+                           L.rows.chunks(), L.rows.multi(), L.aggs.chunks(), L.aggs.multi(),
+                           level + 1 == H.levels.size() && H.terminal_reason == kDirect ? L.perm.size() : 0};
+    std::memcpy(sizes, s, sizeof s);
+    auto put = [](auto *dst, const auto &src) {
+      if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    put(perm, L.perm); put(row_ptr, L.row_ptr); put(col, L.col); put(val, L.val); put(inv, L.inv);
+    put(lab, L.lab); put(agg_ptr, L.agg_ptr); put(row_chunks, L.rows.desc); put(row_multi, L.rows.mrange);
+    put(agg_chunks, L.aggs.desc); put(agg_multi, L.aggs.mrange);
+    if (s[7] && H.terminal_inverse.size() != s[7] * s[7]) fail(kInvariant, "layout", "terminal inverse size");
+    if (s[7]) put(terminal_inverse, H.terminal_inverse);
+  });
+}
+
 int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] { g_hierarchies.erase(handle); });

@@ -78,6 +78,20 @@ int32_t vckss_mi355x_hierarchy_level_v1(uint64_t handle, uint32_t level, uint64_
 int32_t vckss_mi355x_hierarchy_levels_v1(uint64_t handle, uint32_t *levels, uint64_t *factor_nnz);
 int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle);
 
+/* Device layout of one hierarchy level, as uploaded: the renumbering
+ * (perm: renumbered -> original), the renumbered CSR, inverse diagonal,
+ * monotone coarse labels with contiguous aggregate ranges, and the chunk
+ * tables (4 words per chunk: segment, begin, end, multi index or
+ * 0xffffffff; 2 words per multi-chunk segment: first and end chunk).
+ * sizes: n, nnz, coarse, row chunks, row multi, aggregate chunks, aggregate
+ * multi, terminal dimension (direct terminal only, else 0).  Call with null
+ * arrays to read the sizes first. */
+int32_t vckss_mi355x_layout_v1(uint64_t handle, uint32_t level, uint32_t chunk, uint64_t *sizes,
+                               uint32_t *perm, uint32_t *row_ptr, uint32_t *col, double *val,
+                               double *inv, uint32_t *lab, uint32_t *agg_ptr, uint32_t *row_chunks,
+                               uint32_t *row_multi, uint32_t *agg_chunks, uint32_t *agg_multi,
+                               double *terminal_inverse);
+
 /* Host preparation only (no GPU): same as the prepare entry point but stops
  * before device upload; the generation can be queried with dims/arrays of
  * the prepared problem (which 20..): 20 unit_rank 21 unit_count

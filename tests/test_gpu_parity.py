@@ -2,7 +2,9 @@
 
 Bars (stated per test):
   * Counter-V1 atoms: bit-exact (integer work);
-  * level products A x: bit-exact (CSR rows keep the reference's scatter order);
+  * level products A x: bit-exact where a row is summed serially (K = 64,
+    at most 64 entries; renumbered rows keep the reference's neighbour
+    order), 1e-13 of the row scale where rows are split over threads;
   * one V-cycle: 1e-10 relative (FP64; the coarse means are tree sums on the
     device and the terminal solve is a dense grounded inverse instead of LDL);
   * PCG: certified residual below the requested tolerance, iteration counts
@@ -72,8 +74,11 @@ def _hybrid(oracle, cols):
     return p, g
 
 
-def test_level_products_bit_exact(oracle):
-    cols = panel()
+def test_level_products(oracle):
+    # Rows of at most 64 entries at K = 64 are summed serially in the
+    # reference's neighbour order: bit-exact.  Other rows are split over
+    # slices/chunks: rounding-level agreement, 1e-13 of the row's scale.
+    cols = panel(heavy_frequency=True)
     p, g = _hybrid(oracle, cols)
     e = F.Engine(*cols)
     try:
@@ -82,12 +87,17 @@ def test_level_products_bit_exact(oracle):
         rng = np.random.default_rng(3)
         for li, lv in enumerate(levels):
             gl = oracle.Graph(lv["n"], lv["u"], lv["v"], lv["w"])
-            for K in (1, 4, 32):
+            deg = np.bincount(lv["u"], minlength=lv["n"]) + np.bincount(lv["v"], minlength=lv["n"])
+            for K in (1, 4, 32, 64):
                 X = rng.standard_normal((lv["n"], K))
                 Y = e.spmv(li, X.ravel(), K).reshape(lv["n"], K)
                 for j in range(K):
                     want = gl.matvec(X[:, j])
-                    assert np.array_equal(Y[:, j].view(np.uint64), want.view(np.uint64)), (li, K, j)
+                    scale = lv["diag"] * (np.abs(X[:, j]) + np.abs(X[:, j]).max())
+                    assert np.all(np.abs(Y[:, j] - want) <= 1e-13 * scale + 1e-300), (li, K, j)
+                    if K == 64:
+                        short = deg <= 64
+                        assert np.array_equal(Y[short, j].view(np.uint64), want[short].view(np.uint64)), (li, j)
     finally:
         e.release()
 
@@ -186,7 +196,8 @@ def _compare_engine(oracle, cols, probes, tolerance, rel_components, abs_arrays,
             g, w = np.asarray(got[key]), np.asarray(want[okey])
             assert np.all(np.abs(g - w) <= rel_components * np.maximum(np.abs(w), 1e-3)), (key, g, w)
         for name in ("fitted_cell", "projection_share", "residual_share", "cell_correction_weight"):
-            assert np.max(np.abs(arrays[name] - want[name])) <= abs_arrays, name
+            scale = max(np.max(np.abs(want[name])), 1.0)
+            assert np.max(np.abs(arrays[name] - want[name])) <= abs_arrays * scale, name
         assert np.max(arrays["leverage_complete_residual"]) <= max(10 * (tolerance or 1e-6), 1e-11)
         return got, want, arrays
     finally:
