@@ -151,8 +151,6 @@ struct DeviceEngine::Impl {
   DBuf<double> ginv;
   // reductions and segment scratch
   DBuf<double> rpartial, bpartial, seg_partial;
-  DBuf<unsigned> counters;
-  unsigned counter_next = 0;
   uint64_t max_chunks = 0;
   uint32_t max_multi = 0;
   DBuf<int> flags;
@@ -174,12 +172,9 @@ struct DeviceEngine::Impl {
   double timed_bytes = 0;
   uint64_t vcycles = 0;
 
-  unsigned *counter() { return counters.p + (counter_next++ % 64); }
   const int *always() const { return flags.p + dev::kFlagOne; }
 
-  dev::SegScratch scratch(unsigned *rcounter) const {
-    return {seg_partial.p, bpartial.p, rpartial.p, rcounter, gate};
-  }
+  dev::SegScratch scratch() const { return {seg_partial.p, bpartial.p, gate}; }
 
   void ensure(uint32_t lanes) {
     if (lanes <= cap) return;
@@ -215,18 +210,25 @@ namespace {
 
 using Impl = DeviceEngine::Impl;
 
+// Final per-lane reduction of `items` block partials (one block per lane).
+template <int K, int NV, class Fin>
+void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, const int *gate = nullptr) {
+  hipLaunchKernelGGL((dev::k_finish<K, NV, Fin>), K, kBlock, 0, E.stream, partial, items, fin, gate);
+}
+
 // Launch a segment operation over `ch` with operator `op`: pass 1 over all
-// chunks, pass 2 for multi-chunk segments and the final reduction.
+// chunks, pass 2 for multi-chunk segments, then the final reduction.
 template <typename T, int K, int NV, class Op>
 void segments(Impl &E, const DevChunks &ch, const Op &op) {
   constexpr int G = dev::Shape<K>::G;
   const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
-  const dev::SegScratch sc = E.scratch(NV > 0 ? E.counter() : nullptr);
-  hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
-  if (ch.n_multi == 0 && NV == 0) return;
   const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
-  hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
-                     NV > 0 ? grid1 : 0u);
+  const dev::SegScratch sc = E.scratch();
+  hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
+  if (ch.n_multi)
+    hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
+                       NV > 0 ? grid1 : 0u);
+  if constexpr (NV > 0) finish<K, NV>(E, E.bpartial.p, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
 }
 
 // ------------------------------------------------------------------ V-cycle
@@ -330,8 +332,9 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   int *flags = E.flags.p;
   const int *active = flags + dev::kFlagActive;
   VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
-  hipLaunchKernelGGL((dev::k_pcg_stats<K>), reduce_grid(m0), kBlock, 0, E.stream, n, b, E.rpartial.p,
-                     E.counter(), E.st.p, prm, E.use.p);
+  const unsigned rg = reduce_grid(m0);
+  hipLaunchKernelGGL((dev::k_pcg_stats<K>), rg, kBlock, 0, E.stream, n, b, E.rpartial.p);
+  finish<K, 3>(E, E.rpartial.p, rg, dev::FinStats{E.st.p, E.use.p, n});
   E.gate = E.always();
   if (!warm) {
     hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.x.p,
@@ -340,9 +343,8 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   } else {
     hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.ap.p,
                        E.st.p);
-    hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
-                       E.rpartial.p, E.counter(), E.lane_a.p);
-    hipLaunchKernelGGL((dev::k_mask_means<K>), 1, 64, 0, E.stream, E.lane_a.p, E.use.p);
+    hipLaunchKernelGGL((dev::k_lane_sum<K>), rg, kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.rpartial.p);
+    finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
     hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
                        E.lane_a.p, nullptr);
     segments<double, K, 4>(E, L0.rows, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
@@ -352,15 +354,16 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags,
                        (const int *)nullptr);
     const double *zp = apply_cycle<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
-    hipLaunchKernelGGL((dev::k_pcg_rho<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.rc.p, zp, E.rpartial.p,
-                       E.counter(), E.st.p, active);
+    hipLaunchKernelGGL((dev::k_pcg_rho<K>), rg, kBlock, 0, E.stream, n, E.rc.p, zp, E.rpartial.p, active);
+    finish<K, 3>(E, E.rpartial.p, rg, dev::FinRho{E.st.p, n}, active);
     hipLaunchKernelGGL((dev::k_pcg_direction<K>), grid_for(m0), kBlock, 0, E.stream, n, zp, E.p.p, E.st.p, active);
   };
   precondition();
   for (uint32_t it = 1; it <= max_it; ++it) {
     segments<double, K, 1>(E, L0.rows, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
-    hipLaunchKernelGGL((dev::k_pcg_update<K>), reduce_grid(m0), kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p,
-                       E.rc.p, E.r.p, E.rpartial.p, E.counter(), E.st.p, prm, active);
+    hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
+                       E.rpartial.p, E.st.p, active);
+    finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
     hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags, active);
     hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.lane_a.p,
                        active);
@@ -391,15 +394,16 @@ template <int K>
 void extract(Impl &E) {
   const uint64_t F = E.F, W = E.W;
   hipLaunchKernelGGL((dev::k_gather_rows<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.pos0.p, E.x.p, E.fsol.p);
-  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.fsol.p, E.rpartial.p,
-                     E.counter(), E.lane_a.p);
+  const unsigned fg = reduce_grid(F * K), wg = reduce_grid(W * K);
+  hipLaunchKernelGGL((dev::k_lane_sum<K>), fg, kBlock, 0, E.stream, F, E.fsol.p, E.rpartial.p);
+  finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p,
                      nullptr);
   hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.fsol.p, E.lane_a.p,
                      nullptr);
   VK_HIP(hipMemcpyAsync(E.red.p, E.fsol.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
-  hipLaunchKernelGGL((dev::k_lane_mean<K>), reduce_grid(F * K), kBlock, 0, E.stream, F, E.red.p, E.rpartial.p,
-                     E.counter(), E.lane_a.p);
+  hipLaunchKernelGGL((dev::k_lane_sum<K>), fg, kBlock, 0, E.stream, F, E.red.p, E.rpartial.p);
+  finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p,
                      nullptr);
   hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
@@ -407,26 +411,15 @@ void extract(Impl &E) {
   segments<double, K, 1>(E, E.firm_chunks,
                          dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.red.p,
                                               E.wsum.p, E.sout.p, E.lane_b.p, (uint32_t)F});
-  hipLaunchKernelGGL((dev::k_reduced_residual<K>), reduce_grid(F * K), kBlock, 0, E.stream, (uint32_t)F, E.sch.p,
-                     E.sout.p, E.lane_b.p, E.rpartial.p, E.counter(), E.reduced_res.p);
-  hipLaunchKernelGGL((dev::k_worker_solution<K>), reduce_grid(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
-                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p, E.worker.p, E.rpartial.p,
-                     E.counter(), E.sums.p);
+  hipLaunchKernelGGL((dev::k_reduced_residual<K>), fg, kBlock, 0, E.stream, (uint32_t)F, E.sch.p, E.sout.p,
+                     E.lane_b.p, E.rpartial.p);
+  finish<K, 2>(E, E.rpartial.p, fg, dev::FinRelative{E.reduced_res.p});
+  hipLaunchKernelGGL((dev::k_worker_solution<K>), wg, kBlock, 0, E.stream, W, E.worker_ptr.p, E.cell_firm.p,
+                     E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p, E.worker.p, E.rpartial.p);
+  finish<K, 2>(E, E.rpartial.p, wg, dev::FinSums<K>{E.sums.p});
   segments<double, K, 2>(E, E.firm_chunks,
                          dev::OpFirmResidual<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.frhs.p,
                                                 E.fsol.p, E.worker.p, E.sums.p, E.complete_res.p, E.rhs_norm.p});
-}
-
-double euclid(const std::vector<double> &v, uint64_t n, uint32_t K, uint32_t lane) {
-  double scale = 0.0;
-  for (uint64_t i = 0; i < n; ++i) scale = std::max(scale, std::fabs(v[i * K + lane]));
-  if (scale == 0.0 || !std::isfinite(scale)) return scale;
-  double s = 0.0;
-  for (uint64_t i = 0; i < n; ++i) {
-    double q = v[i * K + lane] / scale;
-    s += q * q;
-  }
-  return scale * std::sqrt(s);
 }
 
 // One batch of right-hand sides already in E.wrhs / E.frhs (K lanes, the first
@@ -450,7 +443,9 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
                opt.precision == 32, use, st);
   receipts.assign(columns, ColumnReceipt{});
+  std::vector<double> first_norm(K, 0.0);
   for (uint32_t j = 0; j < columns; ++j) {
+    first_norm[j] = st[j].rhs_norm;
     if (st[j].status) fail(st[j].status, "cmg_full_v2", "lane " + std::to_string(j) + " PCG failed");
     receipts[j].iterations = st[j].iters;
     receipts[j].restarts = st[j].restarts;
@@ -477,11 +472,11 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
     if (E.aux)
       hipLaunchKernelGGL((dev::k_lift<K>), grid_for(E.aux * K), kBlock, 0, E.stream, (uint32_t)F, (uint32_t)E.aux,
                          E.aux_ptr.p, E.aux_nbr.p, E.aux_w.p, E.pos0.p, E.fsol.p, E.x.p);
-    std::vector<double> hb;
-    E.b0.download(hb, m0, E.stream);
+    // smallest Euclidean norm of a failing hybrid right-hand side (the
+    // lane norms of the setup pass of the first solve)
     double smallest = INFINITY;
     for (uint32_t j = 0; j < columns; ++j)
-      if (fail_lane[j]) smallest = std::min(smallest, euclid(hb, E.n0, K, j));
+      if (fail_lane[j]) smallest = std::min(smallest, first_norm[j]);
     double target = factors[round] * gate * smallest, rel = 0.0, abs_tol = target;
     if (!(std::isfinite(target) && target > 0.0)) {
       rel = effective_tol * inner_ratio * factors[round];
@@ -626,9 +621,7 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.pos0.upload(layout[0].pos);
   if (h.terminal_reason == kDirect) E.ginv.upload(h.terminal_inverse);
   E.rpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
-  E.counters.alloc(64);
-  VK_HIP(hipMemset(E.counters.p, 0, 64 * sizeof(unsigned)));
-  E.bpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
+  E.bpartial.alloc((size_t)2 * dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
   E.flags.alloc(4);
   const int init_flags[4] = {1, 0, 1, 0};
   VK_HIP(hipMemcpy(E.flags.p, init_flags, sizeof init_flags, hipMemcpyHostToDevice));
@@ -702,7 +695,7 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
   for (uint64_t first = 0; first < R; first += lw) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, opt.seed, kTagLeverage, first, k, E.atoms.p);
+                       E.unit_count.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
@@ -752,28 +745,30 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
     VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(double), E.stream));
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
-                       E.stratum_rank.p, E.stratum_count.p, opt.seed, kTagTarget, first, k, E.atoms.p);
+                       E.stratum_rank.p, E.stratum_count.p, opt.seed, kTagTarget, first, k, KT, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KT, [&](auto kt) {
       constexpr int KTc = decltype(kt)::value;
       constexpr int K2c = 2 * KTc;
-      hipLaunchKernelGGL((dev::k_target_direction<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
-                         E.cell_stratum_ptr.p, E.stratum_per_copy.p, E.target_total, E.atoms.p, E.S, E.dir.p,
-                         E.rpartial.p, E.counter(), E.totals.p);
+      const unsigned cg = reduce_grid(E.C * KTc);
+      hipLaunchKernelGGL((dev::k_target_direction<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_stratum_ptr.p,
+                         E.stratum_per_copy.p, E.target_total, E.atoms.p, E.dir.p, E.rpartial.p);
+      finish<KTc, 2>(E, E.rpartial.p, cg, dev::FinTotals<KTc>{E.totals.p});
       hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.W * K2c), kBlock, 0, E.stream, E.W,
                          E.worker_ptr.p, (const uint32_t *)nullptr, E.dir.p, E.cell_target.p, E.target_total,
                          E.totals.p, 0, E.wrhs.p);
       segments<double, K2c, 0>(E, E.firm_chunks,
                                dev::OpFirmTargetScores<KTc>{E.firm_cells.p, E.dir.p, E.cell_target.p, E.totals.p,
                                                             E.target_total, E.frhs.p});
-      hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.W * K2c), kBlock, 0, E.stream, E.W, E.wrhs.p, 0,
-                         E.totals.p, E.rpartial.p, E.counter(), E.status.p);
-      hipLaunchKernelGGL((dev::k_balance<K2c>), reduce_grid(E.F * K2c), kBlock, 0, E.stream, E.F, E.frhs.p, 1,
-                         E.totals.p, E.rpartial.p, E.counter(), E.status.p);
+      const unsigned bw = reduce_grid(E.W * K2c), bf = reduce_grid(E.F * K2c);
+      hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bw, kBlock, 0, E.stream, E.W, E.wrhs.p, E.rpartial.p);
+      finish<K2c, 1>(E, E.rpartial.p, bw, dev::FinBalance<K2c>{E.wrhs.p, E.W, 0, E.totals.p, E.status.p});
+      hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bf, kBlock, 0, E.stream, E.F, E.frhs.p, E.rpartial.p);
+      finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.status.p});
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_contract<KTc>), reduce_grid(E.C * KTc), kBlock, 0, E.stream, E.C,
-                         E.cell_worker.p, E.cell_firm.p, E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p, E.counter(),
-                         E.draws.p);
+      hipLaunchKernelGGL((dev::k_contract<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_worker.p, E.cell_firm.p,
+                         E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p);
+      finish<KTc, 3>(E, E.rpartial.p, cg, dev::FinDraws{E.draws.p});
     });
     std::vector<double> d;
     E.draws.download(d, 4 * k, E.stream);
@@ -822,11 +817,12 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
   buf.alloc(n * columns);
   hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
                      target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p, seed,
-                     tag, first_probe, columns, buf.p);
+                     tag, first_probe, columns, columns, buf.p);
   std::vector<double> h;
   buf.download(h, n * columns, E.stream);
   out.resize(h.size());
-  for (size_t i = 0; i < h.size(); ++i) out[i] = (int64_t)h[i];
+  for (uint64_t row = 0; row < n; ++row)
+    for (uint32_t col = 0; col < columns; ++col) out[col * n + row] = (int64_t)h[row * columns + col];
 }
 
 void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t columns, std::vector<double> &y) {
@@ -920,7 +916,7 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   for (uint32_t b = 0; b < batches; ++b) {
     const uint64_t first = (uint64_t)b * lw;
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((lw + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, opt.seed, kTagLeverage, first, lw, E.atoms.p);
+                       E.unit_count.p, opt.seed, kTagLeverage, first, lw, KL, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;

@@ -26,7 +26,7 @@ namespace vk {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr int kMaxReduceBlocks = 256;
+constexpr int kMaxReduceBlocks = 1024;
 constexpr int kMaxReduceValues = 4;
 constexpr uint32_t kSingle = 0xffffffffu;
 
@@ -61,10 +61,10 @@ __device__ inline void philox(uint64_t k0, uint64_t k1, uint64_t c[4]) {
 }
 
 // Rademacher atoms 2*popcount - n for `columns` probes starting at
-// `first_probe`; out[col * n + row] as exact binary64 integers.
+// `first_probe`, as exact binary64 integers, interleaved: out[row * ld + col].
 __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
                         const uint64_t *__restrict__ trials, uint64_t seed, uint64_t tag,
-                        uint64_t first_probe, uint32_t columns, double *__restrict__ out) {
+                        uint64_t first_probe, uint32_t columns, uint32_t ld, double *__restrict__ out) {
   const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
   const uint64_t blocks = last_block - first_block + 1;
   const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
@@ -85,85 +85,67 @@ __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
     for (int l = 0; l < 4; ++l) {
       const uint64_t probe = pb * 4 + l;
       if (probe < first_probe || probe >= first_probe + columns) continue;
-      out[(probe - first_probe) * n + row] = (double)((int64_t)(2 * pos[l]) - (int64_t)tr);
+      out[row * ld + (probe - first_probe)] = (double)((int64_t)(2 * pos[l]) - (int64_t)tr);
     }
   }
 }
 
 // ---------------------------------------------------------------- reductions
+// Two-kernel deterministic reductions without device-scope fences (on
+// gfx950 a device-scope fence writes back and invalidates the L2 of the
+// XCD): every block of a pass stores one partial per lane (block_partial),
+// then k_finish runs one block per lane that sums the partials of all blocks
+// in a fixed order and hands the totals to Fin::reduced(lane, totals).
 
-// Block-reduce NV values per lane (K lanes, kBlock/K rows per block), write
-// block partials and let the last block sum all partials in block order,
-// followed by `extra` per-lane contributions (multi-chunk segments) in index
-// order.  Returns true in the last block; total[v] then holds the sums for
-// threads with threadIdx.x < K.
 template <int K, int NV>
-__device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ partial,
-                                    unsigned *__restrict__ counter, const double *__restrict__ extra,
-                                    uint32_t n_extra, double (&total)[NV]) {
+__device__ inline void block_partial(const double (&val)[NV], double *__restrict__ out) {
   __shared__ double lds[NV][kBlock];
-  __shared__ bool last;
   constexpr int R = kBlock / K;
-  const int tid = threadIdx.x, row = tid / K, lane = tid % K;
+  const int tid = threadIdx.x, row = tid / K;
 #pragma unroll
   for (int v = 0; v < NV; ++v) lds[v][tid] = val[v];
   __syncthreads();
-  for (int s = R / 2; s > 0; s >>= 1) {
-    if (row < s) {
+  for (int w = R / 2; w > 0; w >>= 1) {
+    if (row < w) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + s * K];
+      for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w * K];
     }
     __syncthreads();
   }
   if (tid < K) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) partial[((uint64_t)blockIdx.x * NV + v) * K + tid] = lds[v][tid];
+    for (int v = 0; v < NV; ++v) out[((uint64_t)blockIdx.x * NV + v) * K + tid] = lds[v][tid];
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // sum items row, row + R, ... in order, four loads in flight
-  auto item = [&](unsigned b, int v) -> double {
-    return b < gridDim.x ? partial[((uint64_t)b * NV + v) * K + lane]
-                         : extra[((uint64_t)(b - gridDim.x) * NV + v) * K + lane];
-  };
-  const unsigned items = gridDim.x + n_extra;
+}
+
+// Lane blockIdx.x of `items` block partials: thread t adds items t, t + kBlock,
+// ... in order, then a fixed LDS tree; thread 0 calls fin.reduced.
+template <int K, int NV, class Fin>
+__global__ void __launch_bounds__(kBlock) k_finish(const double *__restrict__ partial, uint32_t items, Fin fin,
+                                                   const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double lds[NV][kBlock];
+  const int j = blockIdx.x, tid = threadIdx.x;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
-    unsigned b = row;
-    for (; b + 3 * R < items; b += 4 * R) {
-      const double a0 = item(b, v), a1 = item(b + R, v), a2 = item(b + 2 * R, v), a3 = item(b + 3 * R, v);
-      s += a0;
-      s += a1;
-      s += a2;
-      s += a3;
-    }
-    for (; b < items; b += R) s += item(b, v);
+    for (uint32_t b = tid; b < items; b += kBlock) s += partial[((uint64_t)b * NV + v) * K + j];
     lds[v][tid] = s;
   }
   __syncthreads();
-  for (int s = R / 2; s > 0; s >>= 1) {
-    if (row < s) {
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + s * K];
+      for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w];
     }
     __syncthreads();
   }
+  if (tid == 0) {
+    double tot[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) total[v] = lds[v][tid];
-  if (tid == 0) atomicExch(counter, 0u);  // device-scope, visible to the next launch
-  return true;
-}
-
-// Plain block + last-block reduction with no extra contributions.
-template <int K, int NV>
-__device__ inline bool reduce_lanes(double (&val)[NV], double *__restrict__ partial,
-                                    unsigned *__restrict__ counter, double (&total)[NV]) {
-  return reduce_lanes<K, NV>(val, partial, counter, nullptr, 0u, total);
+    for (int v = 0; v < NV; ++v) tot[v] = lds[v][0];
+    fin.reduced(j, tot);
+  }
 }
 
 // ---------------------------------------------------------------- segments
@@ -177,9 +159,7 @@ struct Chunks {
 // Shared scratch of the segment kernels (one stream, kernels in sequence).
 struct SegScratch {
   double *partial;      // chunk partials of multi-chunk segments, chunks x K (as T)
-  double *bpartial;     // pass-1 block partials of the reduction, NV x K per block
-  double *rpartial;     // pass-2 block partials of the reduction
-  unsigned *rcounter;   // pass-2 last-block ticket
+  double *bpartial;     // block partials of the reduction: pass 1, then pass 2 after `blocks1`
   const int *gate;      // kernels return at once when *gate == 0
 };
 
@@ -222,32 +202,13 @@ __global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratc
     else
       partial[(uint64_t)u * K + j] = part;
   }
-  if constexpr (NV > 0) {
-    __shared__ double lds[NA][kBlock];
-    constexpr int R = kBlock / K;
-    const int tid = threadIdx.x, row = tid / K;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) lds[v][tid] = acc[v];
-    __syncthreads();
-    for (int w = R / 2; w > 0; w >>= 1) {
-      if (row < w) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w * K];
-      }
-      __syncthreads();
-    }
-    if (tid < K) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) sc.bpartial[((uint64_t)blockIdx.x * NV + v) * K + tid] = lds[v][tid];
-    }
-  }
+  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial);
 }
 
 // Pass 2: every multi-chunk segment is summed by one block (kBlock / K
 // slices per lane, a fixed LDS tree) in chunk order and finished; with NV > 0
-// the last block adds the pass-2 block partials and then the `blocks1`
-// pass-1 block partials, in order, and runs Op::reduced.  Launched whenever
-// there are multi-chunk segments or NV > 0.
+// each block stores its reduction partial after the `blocks1` pass-1
+// partials, and k_finish over both passes runs Op::reduced.
 template <typename T, int K, int NV, class Op>
 __global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, SegScratch sc, uint32_t blocks1) {
   if (sc.gate && *sc.gate == 0) return;
@@ -281,11 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, S
     if (s == 0) op.finish(ch.desc[first].x, j, tree[j], acc);
     __syncthreads();
   }
-  if constexpr (NV > 0) {
-    double tot[NA];
-    if (reduce_lanes<K, NA>(acc, sc.rpartial, sc.rcounter, sc.bpartial, blocks1, tot) && threadIdx.x < K)
-      op.reduced(threadIdx.x, tot);
-  }
+  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial + (uint64_t)blocks1 * NA * K);
 }
 
 // Row chunk of y = A x with the neighbour values supplied by D:

@@ -34,9 +34,7 @@ enum { kFlagActive = 0, kFlagFresh = 1, kFlagOne = 2 };
 
 // ---- setup: compatibility check, norms (components.rs:669-744, pcg.rs:1596-1660)
 template <int K>
-__global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__restrict__ partial,
-                            unsigned *__restrict__ counter, ColState *__restrict__ st, PcgParams prm,
-                            const uint8_t *__restrict__ use) {
+__global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__restrict__ partial) {
   double acc[3] = {0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
     const double v = b[t];
@@ -44,21 +42,25 @@ __global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__
     acc[1] += fabs(v);
     acc[2] += v * v;
   }
-  double tot[3];
-  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
-    const double mean = tot[0] / (double)n;
+  block_partial<K, 3>(acc, partial);
+}
+struct FinStats {
+  ColState *__restrict__ st;
+  const uint8_t *__restrict__ use;
+  uint32_t n;
+  __device__ void reduced(int j, const double (&tot)[3]) const {
+    ColState &s = st[j];
     s = ColState{};
-    s.mean_r = mean;
+    s.mean_r = tot[0] / (double)n;
     s.rhs_norm = sqrt(tot[2]);
-    s.active = use[threadIdx.x] ? 1 : 0;
+    s.active = use[j] ? 1 : 0;
     s.first = 1;
     if (s.active && fabs(tot[0]) > 1e-7 * fmax(tot[1], 1.0)) {
       s.status = 51;  // incompatible Laplacian right-hand side
       s.active = 0;
     }
   }
-}
+};
 
 // proj = b - mean, r = proj, x = 0 (cold start; a warm start passes a
 // scratch buffer as `x`).
@@ -164,8 +166,7 @@ __global__ void k_mask_means(double *__restrict__ mean, const uint8_t *__restric
 // rho = r.z - mean_z * sum r with r already centred (pcg.rs:1912-1947).
 template <int K>
 __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double *__restrict__ z,
-                          double *__restrict__ partial, unsigned *__restrict__ counter,
-                          ColState *__restrict__ st, const int *__restrict__ gate) {
+                          double *__restrict__ partial, const int *__restrict__ gate) {
   VK_GATE(gate);
   double acc[3] = {0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
@@ -174,9 +175,13 @@ __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double
     acc[1] += rr * zz;
     acc[2] += rr;
   }
-  double tot[3];
-  if (reduce_lanes<K, 3>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
+  block_partial<K, 3>(acc, partial);
+}
+struct FinRho {
+  ColState *__restrict__ st;
+  uint32_t n;
+  __device__ void reduced(int j, const double (&tot)[3]) const {
+    ColState &s = st[j];
     if (!s.active) return;
     s.mean_z = tot[0] / (double)n;
     const double rho = tot[1] - s.mean_z * tot[2];
@@ -185,14 +190,10 @@ __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double
       s.active = 0;
       return;
     }
-    if (s.first || s.restarted) {
-      s.beta = 0.0;
-    } else {
-      s.beta = rho / s.rho;
-    }
+    s.beta = (s.first || s.restarted) ? 0.0 : rho / s.rho;
     s.rho = rho;
   }
-}
+};
 
 // p = z_c (first or after a residual replacement) or z_c + beta p.
 template <int K>
@@ -243,8 +244,8 @@ struct OpCurvature : CsrOp<double, K, OpCurvature<K>> {
 template <int K>
 __global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const double *__restrict__ ap,
                              double *__restrict__ x, const double *__restrict__ rc, double *__restrict__ r,
-                             double *__restrict__ partial, unsigned *__restrict__ counter,
-                             ColState *__restrict__ st, PcgParams prm, const int *__restrict__ gate) {
+                             double *__restrict__ partial, const ColState *__restrict__ st,
+                             const int *__restrict__ gate) {
   VK_GATE(gate);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
@@ -262,21 +263,26 @@ __global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const dou
     acc[2] += rv;
     acc[3] += rv * rv;
   }
-  double tot[4];
-  if (reduce_lanes<K, 4>(acc, partial, counter, tot) && threadIdx.x < K) {
-    ColState &s = st[threadIdx.x];
+  block_partial<K, 4>(acc, partial);
+}
+struct FinUpdate {
+  ColState *__restrict__ st;
+  PcgParams prm;
+  __device__ void reduced(int j, const double (&tot)[4]) const {
+    ColState &s = st[j];
     s.restarted = 0;
     s.need_fresh = 0;
     if (!s.active) return;
+    const double n = (double)prm.n;
     s.iters += 1;
-    s.mean_x = tot[0] / (double)n;
-    s.xnorm = sqrt(fmax(tot[1] - (double)n * s.mean_x * s.mean_x, 0.0));
-    s.mean_r = tot[2] / (double)n;
+    s.mean_x = tot[0] / n;
+    s.xnorm = sqrt(fmax(tot[1] - n * s.mean_x * s.mean_x, 0.0));
+    s.mean_r = tot[2] / n;
     s.rnorm = sqrt(tot[3]);
     s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * s.xnorm);
     s.need_fresh = (s.rnorm <= s.tol) || (s.iters % prm.interval == 0);
   }
-}
+};
 
 // Fresh residual for flagged lanes: r = proj - A x (x already centred), the
 // certificate against the unprojected rhs (pcg.rs:1757-1817, 2209-2236).
@@ -347,16 +353,21 @@ __global__ void k_pcg_limit(ColState *__restrict__ st, PcgParams prm, int *__res
 
 // ---------------------------------------------------------------- estimator passes
 
-// Column sums over `rows` rows of a K-lane matrix into out[K] (last block).
+// Column means over `rows` rows of a K-lane matrix.
 template <int K>
-__global__ void k_lane_mean(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial,
-                            unsigned *__restrict__ counter, double *__restrict__ mean) {
+__global__ void k_lane_sum(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial) {
   double acc[1] = {0.0};
   VK_GRID_LOOP(t, rows * K) acc[0] += v[t];
-  double tot[1];
-  if (reduce_lanes<K, 1>(acc, partial, counter, tot) && threadIdx.x < K)
-    mean[threadIdx.x] = tot[0] / (double)rows;
+  block_partial<K, 1>(acc, partial);
 }
+struct FinMean {
+  double *__restrict__ mean;
+  const uint8_t *__restrict__ use;  // optional: zero the means of unused lanes
+  uint64_t rows;
+  __device__ void reduced(int j, const double (&tot)[1]) const {
+    mean[j] = (use && !use[j]) ? 0.0 : tot[0] / (double)rows;
+  }
+};
 
 // Exact-integer leverage right-hand sides (engine.rs:2562-2619): per worker
 // and per firm, the sum of the atoms of the units of its cells.
@@ -372,7 +383,7 @@ __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
     if (j < (int)kcols) {
       for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
         const uint32_t c = cells ? cells[k] : (uint32_t)k;
-        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += atoms[(uint64_t)j * units + cu[q]];
+        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += atoms[(uint64_t)cu[q] * K + j];
       }
     }
     out[t] = s;
@@ -398,8 +409,7 @@ __global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ w
 template <int K>
 __global__ void k_reduced_residual(uint32_t firms, const double *__restrict__ rhs,
                                    const double *__restrict__ out, const double *__restrict__ mean,
-                                   double *__restrict__ partial, unsigned *__restrict__ counter,
-                                   double *__restrict__ result) {
+                                   double *__restrict__ partial) {
   double acc[2] = {0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)firms * K) {
     const int j = t % K;
@@ -407,10 +417,14 @@ __global__ void k_reduced_residual(uint32_t firms, const double *__restrict__ rh
     acc[0] += a * a;
     acc[1] += rhs[t] * rhs[t];
   }
-  double tot[2];
-  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K)
-    result[threadIdx.x] = tot[1] == 0.0 ? sqrt(tot[0]) : sqrt(tot[0]) / sqrt(tot[1]);
+  block_partial<K, 2>(acc, partial);
 }
+struct FinRelative {
+  double *__restrict__ result;
+  __device__ void reduced(int j, const double (&tot)[2]) const {
+    result[j] = tot[1] == 0.0 ? sqrt(tot[0]) : sqrt(tot[0]) / sqrt(tot[1]);
+  }
+};
 
 // worker = (wrhs - sum_c w_c firm[f_c]) / d_w (operator.rs:311-343) and the
 // worker block of the complete residual (operator.rs:386-458).
@@ -419,8 +433,7 @@ __global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__
                                   const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
                                   const double *__restrict__ wdiag, const double *__restrict__ wrhs,
                                   const double *__restrict__ firm, double *__restrict__ worker,
-                                  double *__restrict__ partial, unsigned *__restrict__ counter,
-                                  double *__restrict__ sums) {
+                                  double *__restrict__ partial) {
   double acc[2] = {0.0, 0.0};
   VK_GRID_LOOP(t, workers * K) {
     const uint64_t w = t / K;
@@ -435,12 +448,16 @@ __global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__
     acc[0] += rw * rw;
     acc[1] += b * b;
   }
-  double tot[2];
-  if (reduce_lanes<K, 2>(acc, partial, counter, tot) && threadIdx.x < K) {
-    sums[threadIdx.x] = tot[0];
-    sums[K + threadIdx.x] = tot[1];
-  }
+  block_partial<K, 2>(acc, partial);
 }
+template <int K>
+struct FinSums {
+  double *__restrict__ sums;
+  __device__ void reduced(int j, const double (&tot)[2]) const {
+    sums[j] = tot[0];
+    sums[K + j] = tot[1];
+  }
+};
 
 // ---------------------------------------------------------------- firm passes
 // Firm rows walk their cells (firm_cells order); Pareto-sized firms own
@@ -482,7 +499,7 @@ struct OpFirmUnits {
     if (j >= (int)kcols) return acc;
     for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
       const uint32_t c = fcells[k];
-      for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += atoms[(uint64_t)j * units + cu[q]];
+      for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += atoms[(uint64_t)cu[q] * K + j];
     }
     return acc;
   }
@@ -578,7 +595,7 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
     for (uint32_t j = 0; j < kcols; ++j) {
       const double pred = worker[w * K + j] + firm[f * K + j];
       const double proj = root * pred;
-      const double res = atoms[(uint64_t)j * units + g] / root - proj;
+      const double res = atoms[g * K + j] / root - proj;
       if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
       const double ps = proj * proj, rs = res * res;
       neumaier_add(s[0], s[1], ps);
@@ -646,30 +663,34 @@ __global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_pt
 
 // Target direction per cell: compensated sum over its strata of
 // sqrt(per_copy / T) atom (engine.rs:2621-2703), totals for centring.
+// atoms: strata x KT, interleaved.
 template <int KT>
 __global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ csp,
                                    const double *__restrict__ per_copy, double target_total,
-                                   const double *__restrict__ atoms, uint64_t strata,
-                                   double *__restrict__ dir, double *__restrict__ partial,
-                                   unsigned *__restrict__ counter, double *__restrict__ totals) {
+                                   const double *__restrict__ atoms, double *__restrict__ dir,
+                                   double *__restrict__ partial) {
   double acc[2] = {0.0, 0.0};
   VK_GRID_LOOP(t, cells * KT) {
     const uint64_t c = t / KT;
     const int j = t % KT;
     double s = 0.0, k = 0.0;
     for (uint64_t q = csp[c]; q < csp[c + 1]; ++q)
-      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * atoms[(uint64_t)j * strata + q]);
+      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * atoms[q * KT + j]);
     const double d = s + k;
     dir[t] = d;
     acc[0] += d;
     acc[1] += fabs(d);
   }
-  double tot[2];
-  if (reduce_lanes<KT, 2>(acc, partial, counter, tot) && threadIdx.x < KT) {
-    totals[threadIdx.x] = tot[0];
-    totals[KT + threadIdx.x] = tot[1] + fabs(tot[0]);  // reference scale
-  }
+  block_partial<KT, 2>(acc, partial);
 }
+template <int KT>
+struct FinTotals {
+  double *__restrict__ totals;
+  __device__ void reduced(int j, const double (&tot)[2]) const {
+    totals[j] = tot[0];
+    totals[KT + j] = tot[1] + fabs(tot[0]);  // reference scale
+  }
+};
 
 // Centre, then transpose to worker/firm scores in the 2-column layout:
 // column 2j = (score_w, 0), column 2j+1 = (0, score_f)  (engine.rs:2705-2971).
@@ -722,14 +743,19 @@ struct OpFirmTargetScores {
 
 // Compatibility repair of the last score entry (engine.rs:2842-2881).
 template <int K2>
-__global__ void k_balance(uint64_t rows, double *__restrict__ v, int parity,
-                          const double *__restrict__ totals, double *__restrict__ partial,
-                          unsigned *__restrict__ counter, int *__restrict__ status) {
+__global__ void k_balance_sum(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial) {
   double acc[1] = {0.0};
   VK_GRID_LOOP(t, (rows - 1) * K2) acc[0] += v[t];
-  double tot[1];
-  if (reduce_lanes<K2, 1>(acc, partial, counter, tot) && threadIdx.x < K2) {
-    const int l = threadIdx.x;
+  block_partial<K2, 1>(acc, partial);
+}
+template <int K2>
+struct FinBalance {
+  double *__restrict__ v;
+  uint64_t rows;
+  int parity;
+  const double *__restrict__ totals;
+  int *__restrict__ status;
+  __device__ void reduced(int l, const double (&tot)[1]) const {
     if ((l & 1) != parity) return;
     const double ref = totals[K2 / 2 + l / 2];
     const double last = -tot[0];
@@ -739,15 +765,14 @@ __global__ void k_balance(uint64_t rows, double *__restrict__ v, int parity,
       atomicCAS(status, 0, 77);
     v[(rows - 1) * K2 + l] = last;
   }
-}
+};
 
 // Target contraction per probe (engine.rs:3003-3100).
 template <int KT>
 __global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_worker,
                            const uint32_t *__restrict__ cell_firm, const double *__restrict__ weight,
                            const double *__restrict__ worker, const double *__restrict__ firm,
-                           double *__restrict__ partial, unsigned *__restrict__ counter,
-                           double *__restrict__ draws) {
+                           double *__restrict__ partial) {
   constexpr int K2 = 2 * KT;
   double acc[3] = {0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, cells * KT) {
@@ -761,15 +786,18 @@ __global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_wor
     acc[1] += wt * pf * pf;
     acc[2] += wt * pw * pf;
   }
-  double tot[3];
-  if (reduce_lanes<KT, 3>(acc, partial, counter, tot) && threadIdx.x < KT) {
-    double *d = draws + 4 * threadIdx.x;
+  block_partial<KT, 3>(acc, partial);
+}
+struct FinDraws {
+  double *__restrict__ draws;
+  __device__ void reduced(int j, const double (&tot)[3]) const {
+    double *d = draws + 4 * j;
     d[0] = tot[0];
     d[1] = tot[1];
     d[2] = tot[2];
     d[3] = tot[0] + tot[1] + 2.0 * tot[2];
   }
-}
+};
 
 // Harmonic lift of auxiliary vertices for a warm start (cmg_impl.rs.in:459-500):
 // x at the renumbered position of aux vertex a = weighted mean of its firms'
