@@ -9,9 +9,15 @@ A step is one batch of `--width` probes through the whole per-probe path on
 the GPU: atoms -> exact right-hand sides -> Schur right-hand side ->
 lockstep certified PCG with the CMG V-cycle -> extraction and the complete
 residual gate -> leverage moments.  Inputs (panel, hierarchy) are resident in
-HBM before the timed region.  The plain single-RHS FP64 path is timed beside
-it, and the oracle (the CPU restatement of the reference engine) is timed on
-the host's cores on a bounded sample as the CPU baseline.
+HBM before the timed region.  Reported beside it: the plain single-RHS FP64
+path on the same GPU, the full KSS estimate (fit, 256 leverage and 256
+target probes) wall time, and the oracle (the CPU restatement of the
+reference engine) on the host's cores on a bounded sample.
+
+The roofline entry is the finest-level prolong + smooth pass of the V-cycle
+(k_segments<..., OpProlongSmooth<..., true>>), timed with HIP events on the
+engine stream; `traffic` comes from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary of that kernel for this configuration when present.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -34,17 +40,30 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(width, precision):
+    """HBM bytes per launch of the roofline kernel from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "finest_prolong_smooth_traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    entry = data.get(f"w{width}_fp{precision}")
+    return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--width", type=int, default=32, help="probes per batch (1..64)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=64, help="probes per batch (1..64)")
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
                     help="V-cycle precision (Krylov recurrences are always FP64)")
     ap.add_argument("--tolerance", type=float, default=1e-8)
-    ap.add_argument("--plain-steps", type=int, default=8,
+    ap.add_argument("--plain-steps", type=int, default=16,
                     help="single-RHS FP64 solves timed for the comparison (0 skips)")
+    ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
     ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--scale", type=float, default=1.0, help="panel size factor (1.0 = full size)")
@@ -53,8 +72,6 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the engine runs one process on one MI355X).", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
 
     import fevc_amd as F
     from fevc_amd import synth
@@ -70,6 +87,10 @@ def main():
     d = eng.dims
     log(f"prepared in {prep_s:.1f}s: {d}")
 
+    copy_bytes = 1 << 29
+    copy_ms = eng.stream_copy(copy_bytes)
+    log(f"stream copy {copy_bytes >> 20} MiB: {2 * copy_bytes / copy_ms / 1e6:.0f} GB/s")
+
     # warmup also builds the device copy of the problem and hierarchy
     log(f"warmup: {args.warmup} batch(es) of {args.width}, fp{args.precision} V-cycle")
     eng.time_leverage(args.width, args.precision, args.tolerance, max(args.warmup, 1))
@@ -82,7 +103,7 @@ def main():
     iters = tm["iterations"] / max(solves, 1)
     kernel_s = tm["kernel_ms"] / 1e3
     achieved = tm["kernel_bytes"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
-    log(f"batched: {value:.1f} solves/s, {iters:.1f} it/solve, dominant kernel "
+    log(f"batched: {value:.1f} solves/s, {iters:.1f} it/solve, finest prolong-smooth "
         f"{tm['kernel_ms'] * 1e3:.1f} us x {tm['kernel_launches']} -> {achieved:.0f} GB/s")
 
     plain = None
@@ -91,9 +112,19 @@ def main():
         t0 = time.perf_counter()
         tp = eng.time_leverage(1, 64, args.tolerance, args.plain_steps)
         pw = time.perf_counter() - t0
-        plain = {"solves_per_s": tp["solves"] / pw, "solves": tp["solves"],
-                 "iterations_per_solve": tp["iterations"] / max(tp["solves"], 1)}
+        plain = {"solves_per_s": round(tp["solves"] / pw, 3), "solves": tp["solves"],
+                 "iterations_per_solve": round(tp["iterations"] / max(tp["solves"], 1), 2)}
         log(f"plain single-RHS FP64: {plain['solves_per_s']:.1f} solves/s")
+
+    kss = None
+    if args.kss_probes > 0:
+        eng.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
+        t0 = time.perf_counter()
+        res = eng.solve(probes=args.kss_probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
+        kw = time.perf_counter() - t0
+        kss = {"wall_seconds": round(kw, 3), "probes": args.kss_probes,
+               "corrected": res["corrected"], "numerical_mcse": res["numerical_mcse"]}
+        log(f"KSS estimate ({args.kss_probes} probes): {kw:.2f}s, corrected {res['corrected']}")
     eng.release()
 
     cpu = None
@@ -106,13 +137,14 @@ def main():
         log(f"cpu baseline: {args.cpu_solves} probe solves on {threads} threads")
         sec, it = p.time_probe_solves(args.cpu_solves, first=0, threads=threads,
                                       probe_tolerance=args.tolerance)
-        cpu = {"value": args.cpu_solves / sec, "unit": "probe solves/s", "cores": threads,
+        cpu = {"value": round(args.cpu_solves / sec, 3), "unit": "probe solves/s", "cores": threads,
                "kind": "port",
                "sample": f"{args.cpu_solves} leverage probe solves (atoms, rhs, CMG-PCG to "
                          f"{args.tolerance:g}, extraction) of the same panel, {sec:.1f}s wall, "
                          f"{it / args.cpu_solves:.1f} it/solve"}
         log(f"cpu baseline: {cpu['value']:.2f} solves/s")
 
+    traffic = measured_traffic(args.width, args.precision)
     line = {
         "metric": "probe solves/s at rel.res 1e-8 (leverage probes, CMG-PCG)",
         "value": round(value, 3),
@@ -133,13 +165,15 @@ def main():
                    "vcycle_precision": args.precision,
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_prolong_smooth level 0",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_segments<OpProlongSmooth, finest level> pass 1",
                      "kernel_us": round(tm["kernel_ms"] * 1e3, 2),
                      "algorithmic_bytes": tm["kernel_bytes"]},
         "cpu_baseline": cpu,
         "plain_fp64_single_rhs": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
+        "kss_estimate": kss,
+        "stream_copy_GBs": round(2 * copy_bytes / copy_ms / 1e6, 1),
         "prepare_seconds": round(prep_s, 2),
     }
     print(json.dumps(line), flush=True)

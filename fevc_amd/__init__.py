@@ -143,6 +143,7 @@ SIGNATURES = [
      [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("vckss_mi355x_hierarchy_build_v1", C.c_int32,
      [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
       C.POINTER(C.c_uint64)]),
@@ -322,6 +323,11 @@ class Engine:
             C.byref(iters), C.byref(ms), C.byref(by), C.byref(launches)))
         return dict(seconds=sec.value, solves=solves.value, iterations=iters.value,
                     kernel_ms=ms.value, kernel_bytes=by.value, kernel_launches=launches.value)
+
+    def stream_copy(self, nbytes):
+        ms = C.c_double()
+        _check(library().vckss_mi355x_stream_copy_v1(self.generation, nbytes, C.byref(ms)))
+        return ms.value
 
     def release(self):
         if self.generation:

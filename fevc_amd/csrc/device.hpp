@@ -61,6 +61,9 @@ class DeviceEngine {
            std::vector<double> &x, std::vector<ColumnReceipt> &receipts);
   double time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
                                uint64_t &iterations);
+  // Plain device-to-device stream copy of `bytes` (counter calibration);
+  // returns milliseconds.
+  double stream_copy(uint64_t bytes);
   // Average duration of the dominant kernel over a timed run.
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
   KernelTiming dominant_kernel_timing() const;

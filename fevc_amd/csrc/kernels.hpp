@@ -321,9 +321,11 @@ struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
   }
 };
 
-// y = x' + D^-1 (b - A x'), x' = x + P e  (prolong-add, matvec, Jacobi)
-template <typename T, int K>
-struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K>> {
+// y = x' + D^-1 (b - A x'), x' = x + P e  (prolong-add, matvec, Jacobi).
+// `Finest` only names the finest-level instance (its own kernel symbol for
+// profiling); the arithmetic is the same.
+template <typename T, int K, bool Finest = false>
+struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
   const T *__restrict__ b, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
   const uint32_t *__restrict__ lab;
   T *__restrict__ y;
@@ -409,6 +411,13 @@ __global__ void k_terminal_scale(uint32_t n, const T *__restrict__ inv, const T 
     b[t] = bi;
     x[t] = inv[i] * bi;
   }
+}
+
+// Plain stream copy (HBM traffic calibration of the profiler counters).
+__global__ void k_stream_copy(uint64_t count, const double4 *__restrict__ a, double4 *__restrict__ b) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
 }
 
 // ---------------------------------------------------------------- precision casts

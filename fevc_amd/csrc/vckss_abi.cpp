@@ -595,6 +595,15 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
   });
 }
 
+int32_t vckss_mi355x_stream_copy_v1(uint64_t generation, uint64_t bytes, double *milliseconds) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!milliseconds || bytes == 0) fail(kInvalidInput, "stream_copy", "invalid arguments");
+    *milliseconds = device_of(ctx).stream_copy(bytes);
+  });
+}
+
 int32_t vckss_mi355x_hierarchy_build_v1(uint64_t vertices, uint64_t edges, const uint32_t *u, const uint32_t *v,
                                         const double *w, uint64_t direct_threshold, uint64_t max_levels,
                                         uint64_t *handle) {

@@ -66,6 +66,11 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
                                       double *kernel_ms, double *kernel_bytes,
                                       uint64_t *kernel_launches);
 
+/* Plain device-to-device copy of `bytes` through a streaming kernel
+ * (k_stream_copy), for calibrating hardware traffic counters; returns the
+ * kernel time in milliseconds. */
+int32_t vckss_mi355x_stream_copy_v1(uint64_t generation, uint64_t bytes, double *milliseconds);
+
 /* Host CMG hierarchy of an arbitrary graph (no GPU needed). */
 int32_t vckss_mi355x_hierarchy_build_v1(uint64_t vertices, uint64_t edges, const uint32_t *u,
                                         const uint32_t *v, const double *w,

@@ -14,6 +14,13 @@
  *   - Engine end to end: rust_mata_shared_atoms.rs known answers (3e-12).
  *   - Hybrid graph: cmg_impl.rs.in:749-835 weights and the dense Schur
  *     closed form.
+ *   - Leave-out selection: brute force (articulation and bridge deletion
+ *     plus BFS) on small panels.
+ * Parity unpinned: the CMG_FULL_V2 refinement ladder and the harmonic lift
+ * of auxiliary vertices (full_cmg.rs:970-1088, cmg_impl.rs.in:459-500).  No
+ * reference vector makes a complete-residual gate fail, so these are
+ * restated from the source and checked only through the invariant that
+ * every returned column meets the gate.
  */
 #ifndef ORC_H
 #define ORC_H
