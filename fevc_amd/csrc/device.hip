@@ -74,9 +74,12 @@ struct DBuf {
   }
 };
 
+// Grid for `threads` threads, capped, rounded up to a multiple of the 8 XCDs
+// (the segment kernels map block ranges per XCD).
 inline unsigned grid_for(uint64_t threads, unsigned cap = 8192) {
   uint64_t g = (threads + kBlock - 1) / kBlock;
-  return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+  g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+  return static_cast<unsigned>((g + 7) / 8 * 8);
 }
 inline unsigned reduce_grid(uint64_t threads) { return grid_for(threads, dev::kMaxReduceBlocks); }
 
@@ -165,7 +168,8 @@ struct DeviceEngine::Impl {
   DBuf<uint8_t> use;
   DBuf<double> wrhs, frhs, worker, red, fsol, sch, wsum, sout;
   DBuf<double> lane_a, lane_b, lane_c, reduced_res, complete_res, rhs_norm, sums;
-  DBuf<double> atoms, moments, adjust, cellw, dir, totals, draws, fitted;
+  DBuf<int32_t> atoms;
+  DBuf<double> moments, adjust, cellw, dir, totals, draws, fitted;
   DBuf<int> status;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -579,6 +583,10 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.n0 = h.levels[0].graph.n;
   E.bound = h.levels[0].graph.norm_bound;
   if (h.finest_components != 1) fail(kUnsupported, "device", "the hybrid Laplacian must be connected");
+  // device atoms are 32-bit integers: 2 popcount - n needs n < 2^31
+  for (const auto *counts : {&p.unit_count, &p.stratum_count})
+    for (uint64_t c : *counts)
+      if (c >= (1ULL << 31)) fail(kUnsupported, "device", "a physical count of 2^31 or more is not supported");
   std::vector<double> wd(E.W, 0.0), fd(E.F, 0.0);
   for (uint64_t c = 0; c < E.C; ++c) {
     wd[p.cell_worker[c]] += p.cell_weight[c];
@@ -748,12 +756,13 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
   res.draws.assign(4 * R, 0.0);
   for (uint64_t first = 0; first < R; first += tw) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
-    VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(double), E.stream));
+    VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(int32_t), E.stream));
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
                        E.stratum_rank.p, E.stratum_count.p, opt.seed, kTagTarget, first, k, KT, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KT, [&](auto kt) {
-      constexpr int KTc = decltype(kt)::value;
+      // target batches hold at most 32 probes (64 columns)
+      constexpr int KTc = decltype(kt)::value > 32 ? 32 : decltype(kt)::value;
       constexpr int K2c = 2 * KTc;
       const unsigned cg = reduce_grid(E.C * KTc);
       hipLaunchKernelGGL((dev::k_target_direction<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_stratum_ptr.p,
@@ -818,12 +827,12 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
                          std::vector<int64_t> &out) {
   Impl &E = *impl_;
   const uint64_t n = target ? E.S : E.G;
-  DBuf<double> buf;
+  DBuf<int32_t> buf;
   buf.alloc(n * columns);
   hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
                      target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p, seed,
                      tag, first_probe, columns, columns, buf.p);
-  std::vector<double> h;
+  std::vector<int32_t> h;
   buf.download(h, n * columns, E.stream);
   out.resize(h.size());
   for (uint64_t row = 0; row < n; ++row)

@@ -61,16 +61,18 @@ __device__ inline void philox(uint64_t k0, uint64_t k1, uint64_t c[4]) {
 }
 
 // Rademacher atoms 2*popcount - n for `columns` probes starting at
-// `first_probe`, as exact binary64 integers, interleaved: out[row * ld + col].
+// `first_probe`, as exact integers, interleaved: out[row * ld + col].  The
+// probe block varies fastest across threads so that stores are contiguous.
+// Callers guarantee trials < 2^31.
 __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
                         const uint64_t *__restrict__ trials, uint64_t seed, uint64_t tag,
-                        uint64_t first_probe, uint32_t columns, uint32_t ld, double *__restrict__ out) {
+                        uint64_t first_probe, uint32_t columns, uint32_t ld, int32_t *__restrict__ out) {
   const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
   const uint64_t blocks = last_block - first_block + 1;
   const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n * blocks;
        t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t row = t % n, pb = first_block + t / n;
+    const uint64_t row = t / blocks, pb = first_block + t % blocks;
     const uint64_t tr = trials[row], complete = tr / 64, rem = tr % 64;
     uint64_t pos[4] = {0, 0, 0, 0};
     for (uint64_t w = 0; w <= complete; ++w) {
@@ -85,7 +87,7 @@ __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
     for (int l = 0; l < 4; ++l) {
       const uint64_t probe = pb * 4 + l;
       if (probe < first_probe || probe >= first_probe + columns) continue;
-      out[row * ld + (probe - first_probe)] = (double)((int64_t)(2 * pos[l]) - (int64_t)tr);
+      out[row * ld + (probe - first_probe)] = (int32_t)((int64_t)(2 * pos[l]) - (int64_t)tr);
     }
   }
 }
@@ -193,8 +195,16 @@ __global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratc
   const uint64_t total = (uint64_t)ch.count * G;
   for (uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x; t < total;
        t += (uint64_t)gridDim.x * kBlock) {
-    const uint32_t u = (uint32_t)(t / G);
-    const uint4 d = ch.desc[u];
+    uint32_t u = (uint32_t)(t / G);
+    if constexpr (G == 64) u = __builtin_amdgcn_readfirstlane(u);  // one chunk per wavefront
+    uint4 d = ch.desc[u];
+    if constexpr (G == 64) {
+      // wave-uniform chunk: the CSR walk below then uses scalar loads
+      d.x = __builtin_amdgcn_readfirstlane(d.x);
+      d.y = __builtin_amdgcn_readfirstlane(d.y);
+      d.z = __builtin_amdgcn_readfirstlane(d.z);
+      d.w = __builtin_amdgcn_readfirstlane(d.w);
+    }
     const T part = slice_sum<T, K>(op.partial(d, j, s));
     if (s != 0) continue;
     if (d.w == kSingle)

@@ -374,7 +374,7 @@ struct FinMean {
 template <int K>
 __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
                            const uint32_t *__restrict__ cells, const uint64_t *__restrict__ cu_ptr,
-                           const uint32_t *__restrict__ cu, const double *__restrict__ atoms,
+                           const uint32_t *__restrict__ cu, const int32_t *__restrict__ atoms,
                            uint64_t units, uint32_t kcols, double *__restrict__ out) {
   VK_GRID_LOOP(t, rows * K) {
     const uint64_t i = t / K;
@@ -383,7 +383,7 @@ __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
     if (j < (int)kcols) {
       for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
         const uint32_t c = cells ? cells[k] : (uint32_t)k;
-        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += atoms[(uint64_t)cu[q] * K + j];
+        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += (double)atoms[(uint64_t)cu[q] * K + j];
       }
     }
     out[t] = s;
@@ -490,7 +490,7 @@ struct OpFirmUnits {
   const uint32_t *__restrict__ fcells;
   const uint64_t *__restrict__ cu_ptr;
   const uint32_t *__restrict__ cu;
-  const double *__restrict__ atoms;
+  const int32_t *__restrict__ atoms;
   uint64_t units;
   uint32_t kcols;
   double *__restrict__ out;
@@ -499,7 +499,7 @@ struct OpFirmUnits {
     if (j >= (int)kcols) return acc;
     for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
       const uint32_t c = fcells[k];
-      for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += atoms[(uint64_t)cu[q] * K + j];
+      for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += (double)atoms[(uint64_t)cu[q] * K + j];
     }
     return acc;
   }
@@ -583,8 +583,11 @@ template <int K>
 __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__restrict__ unit_cell,
                           const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ cell_worker,
                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
-                          const double *__restrict__ firm, const double *__restrict__ atoms,
+                          const double *__restrict__ firm, const int32_t *__restrict__ atoms,
                           double *__restrict__ m, int *__restrict__ status) {
+  // One thread per unit, probes in order; the unit's K lanes are read in
+  // 16-byte pieces (2 lanes) so each thread walks whole cache lines.
+  constexpr int P = K >= 2 ? 2 : 1;
   VK_GRID_LOOP(g, units) {
     const uint32_t c = unit_cell[g];
     const uint64_t w = cell_worker[c], f = cell_firm[c];
@@ -592,18 +595,32 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
     double s[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) s[q] = m[(uint64_t)q * units + g];
-    for (uint32_t j = 0; j < kcols; ++j) {
-      const double pred = worker[w * K + j] + firm[f * K + j];
-      const double proj = root * pred;
-      const double res = atoms[g * K + j] / root - proj;
-      if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
-      const double ps = proj * proj, rs = res * res;
-      neumaier_add(s[0], s[1], ps);
-      neumaier_add(s[2], s[3], rs);
-      neumaier_add(s[4], s[5], ps * ps);
-      neumaier_add(s[6], s[7], rs * rs);
-      neumaier_add(s[8], s[9], ps * rs);
+    bool bad = false;
+    for (uint32_t j0 = 0; j0 < kcols; j0 += P) {
+      double wv[P], fv[P], av[P];
+      if constexpr (P == 2) {
+        const double2 a2 = *reinterpret_cast<const double2 *>(worker + w * K + j0);
+        const double2 b2 = *reinterpret_cast<const double2 *>(firm + f * K + j0);
+        const int2 c2 = *reinterpret_cast<const int2 *>(atoms + g * K + j0);
+        wv[0] = a2.x; wv[1] = a2.y; fv[0] = b2.x; fv[1] = b2.y; av[0] = c2.x; av[1] = c2.y;
+      } else {
+        wv[0] = worker[w * K + j0]; fv[0] = firm[f * K + j0]; av[0] = atoms[g * K + j0];
+      }
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (j0 + q >= kcols) break;
+        const double proj = root * (wv[q] + fv[q]);
+        const double res = av[q] / root - proj;
+        bad |= !isfinite(proj) || !isfinite(res);
+        const double ps = proj * proj, rs = res * res;
+        neumaier_add(s[0], s[1], ps);
+        neumaier_add(s[2], s[3], rs);
+        neumaier_add(s[4], s[5], ps * ps);
+        neumaier_add(s[6], s[7], rs * rs);
+        neumaier_add(s[8], s[9], ps * rs);
+      }
     }
+    if (bad) atomicCAS(status, 0, 73);
 #pragma unroll
     for (int q = 0; q < 10; ++q) m[(uint64_t)q * units + g] = s[q];
   }
@@ -667,7 +684,7 @@ __global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_pt
 template <int KT>
 __global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ csp,
                                    const double *__restrict__ per_copy, double target_total,
-                                   const double *__restrict__ atoms, double *__restrict__ dir,
+                                   const int32_t *__restrict__ atoms, double *__restrict__ dir,
                                    double *__restrict__ partial) {
   double acc[2] = {0.0, 0.0};
   VK_GRID_LOOP(t, cells * KT) {
@@ -675,7 +692,7 @@ __global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ 
     const int j = t % KT;
     double s = 0.0, k = 0.0;
     for (uint64_t q = csp[c]; q < csp[c + 1]; ++q)
-      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * atoms[q * KT + j]);
+      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * (double)atoms[q * KT + j]);
     const double d = s + k;
     dir[t] = d;
     acc[0] += d;
