@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=16,
                     help="single-RHS FP64 solves timed for the comparison (0 skips)")
     ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
+    ap.add_argument("--compare-plain", type=int, default=1,
+                    help="also run the KSS estimate on the plain single-RHS FP64 path and report the error")
     ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--scale", type=float, default=1.0, help="panel size factor (1.0 = full size)")
@@ -118,13 +120,32 @@ def main():
 
     kss = None
     if args.kss_probes > 0:
+        import numpy as np
+
         eng.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
         t0 = time.perf_counter()
         res = eng.solve(probes=args.kss_probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
         kw = time.perf_counter() - t0
+        arr = eng.arrays(args.kss_probes)
         kss = {"wall_seconds": round(kw, 3), "probes": args.kss_probes,
                "corrected": res["corrected"], "numerical_mcse": res["numerical_mcse"]}
         log(f"KSS estimate ({args.kss_probes} probes): {kw:.2f}s, corrected {res['corrected']}")
+        if args.compare_plain:
+            # the same estimate on the plain single-RHS FP64 path, identical draws
+            eng.configure(64, 1, 1)
+            t0 = time.perf_counter()
+            ref = eng.solve(probes=args.kss_probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
+            pw = time.perf_counter() - t0
+            ref_arr = eng.arrays(args.kss_probes)
+            rel = {k: float(np.max(np.abs(np.asarray(res[k]) - np.asarray(ref[k])) /
+                                   np.maximum(np.abs(np.asarray(ref[k])), 1e-300)))
+                   for k in ("plugin", "correction", "corrected")}
+            lev = {k: float(np.max(np.abs(arr[k] - ref_arr[k])))
+                   for k in ("projection_share", "residual_share")}
+            kss["plain_fp64_wall_seconds"] = round(pw, 3)
+            kss["max_rel_error_vs_plain_fp64"] = rel
+            kss["max_abs_leverage_error_vs_plain_fp64"] = lev
+            log(f"plain FP64 KSS: {pw:.2f}s; component rel. error {rel}; leverage abs. error {lev}")
     eng.release()
 
     cpu = None

@@ -144,6 +144,7 @@ SIGNATURES = [
       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
+    ("vckss_mi355x_counters_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_hierarchy_build_v1", C.c_int32,
      [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
       C.POINTER(C.c_uint64)]),
@@ -323,6 +324,12 @@ class Engine:
             C.byref(iters), C.byref(ms), C.byref(by), C.byref(launches)))
         return dict(seconds=sec.value, solves=solves.value, iterations=iters.value,
                     kernel_ms=ms.value, kernel_bytes=by.value, kernel_launches=launches.value)
+
+    def counters(self):
+        out = np.zeros(6, np.uint64)
+        _check(library().vckss_mi355x_counters_v1(self.generation, out.ctypes.data, 6))
+        keys = ["vcycles", "batches", "refine_rounds", "refined_columns", "iterations_launched"]
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def stream_copy(self, nbytes):
         ms = C.c_double()

@@ -175,6 +175,8 @@ struct DeviceEngine::Impl {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   double timed_bytes = 0;
   uint64_t vcycles = 0;
+  // diagnostics: solve batches, refinement rounds, refined columns, PCG iterations launched
+  uint64_t batches = 0, refine_rounds = 0, refined_columns = 0, iterations_launched = 0;
 
   const int *always() const { return flags.p + dev::kFlagOne; }
 
@@ -369,6 +371,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   };
   precondition();
   for (uint32_t it = 1; it <= max_it; ++it) {
+    E.iterations_launched++;
     segments<double, K, 1>(E, L0.rows, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
     hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
                        E.rpartial.p, E.st.p, active);
@@ -449,6 +452,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   std::vector<uint8_t> use(K, 0);
   for (uint32_t j = 0; j < columns; ++j) use[j] = 1;
   std::vector<dev::ColState> st;
+  E.batches++;
   pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
                opt.precision == 32, use, st);
   receipts.assign(columns, ColumnReceipt{});
@@ -491,6 +495,8 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
       rel = effective_tol * inner_ratio * factors[round];
       abs_tol = 0.0;
     }
+    E.refine_rounds++;
+    for (uint32_t j = 0; j < columns; ++j) E.refined_columns += fail_lane[j];
     pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true, opt.precision == 32,
                  fail_lane, st);
     for (uint32_t j = 0; j < columns; ++j)
@@ -718,7 +724,7 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
                                                  E.frhs.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * K), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });
@@ -940,7 +946,7 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
                                                  E.frhs.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * K), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });
@@ -950,6 +956,12 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   VK_HIP(hipStreamSynchronize(E.stream));
   E.timing = false;
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void DeviceEngine::counters(uint64_t out[6]) const {
+  const Impl &E = *impl_;
+  const uint64_t v[6] = {E.vcycles, E.batches, E.refine_rounds, E.refined_columns, E.iterations_launched, 0};
+  for (int i = 0; i < 6; ++i) out[i] = v[i];
 }
 
 double DeviceEngine::stream_copy(uint64_t bytes) {

@@ -64,6 +64,9 @@ class DeviceEngine {
   // Plain device-to-device stream copy of `bytes` (counter calibration);
   // returns milliseconds.
   double stream_copy(uint64_t bytes);
+  // V-cycles applied, solve batches, refinement rounds, refined columns,
+  // PCG iterations launched (including ones queued after convergence).
+  void counters(uint64_t out[6]) const;
   // Average duration of the dominant kernel over a timed run.
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
   KernelTiming dominant_kernel_timing() const;

@@ -26,7 +26,7 @@ namespace vk {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr int kMaxReduceBlocks = 1024;
+constexpr int kMaxReduceBlocks = 4096;
 constexpr int kMaxReduceValues = 4;
 constexpr uint32_t kSingle = 0xffffffffu;
 

@@ -571,8 +571,9 @@ struct OpFirmResidual : FirmCellOp<K, OpFirmResidual<K>> {
   }
 };
 
-// FiveMoments accumulation per deletion unit in probe order (engine.rs:1190-1228,
-// 354-374): SoA state m[10][units] = 5 x (sum, correction).
+// FiveMoments accumulation per deletion unit (engine.rs:1190-1228, 354-374):
+// SoA state m[10][units] = 5 x (sum, correction), one compensated addition
+// per batch of probes.
 __device__ inline void neumaier_add(double &s, double &c, double v) {
   const double next = s + v;
   c += (fabs(s) >= fabs(v)) ? ((s - next) + v) : ((v - next) + s);
@@ -585,44 +586,41 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
                           const double *__restrict__ firm, const int32_t *__restrict__ atoms,
                           double *__restrict__ m, int *__restrict__ status) {
-  // One thread per unit, probes in order; the unit's K lanes are read in
-  // 16-byte pieces (2 lanes) so each thread walks whole cache lines.
-  constexpr int P = K >= 2 ? 2 : 1;
-  VK_GRID_LOOP(g, units) {
-    const uint32_t c = unit_cell[g];
-    const uint64_t w = cell_worker[c], f = cell_firm[c];
-    const double root = sqrt((double)unit_count[g]);
-    double s[10];
+  // K consecutive threads per unit (one lane each, coalesced loads); the
+  // batch's five sums are combined by a fixed xor butterfly and added to the
+  // unit's running compensated sums by lane 0, batches in probe order.
+  VK_GRID_LOOP(t, units * K) {
+    const uint64_t g = t / K;
+    const int j = t % K;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j < (int)kcols) {
+      const uint32_t c = unit_cell[g];
+      const uint64_t w = cell_worker[c], f = cell_firm[c];
+      const double root = sqrt((double)unit_count[g]);
+      const double proj = root * (worker[w * K + j] + firm[f * K + j]);
+      const double res = (double)atoms[g * K + j] / root - proj;
+      if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
+      const double ps = proj * proj, rs = res * res;
+      v[0] = ps;
+      v[1] = rs;
+      v[2] = ps * ps;
+      v[3] = rs * rs;
+      v[4] = ps * rs;
+    }
 #pragma unroll
-    for (int q = 0; q < 10; ++q) s[q] = m[(uint64_t)q * units + g];
-    bool bad = false;
-    for (uint32_t j0 = 0; j0 < kcols; j0 += P) {
-      double wv[P], fv[P], av[P];
-      if constexpr (P == 2) {
-        const double2 a2 = *reinterpret_cast<const double2 *>(worker + w * K + j0);
-        const double2 b2 = *reinterpret_cast<const double2 *>(firm + f * K + j0);
-        const int2 c2 = *reinterpret_cast<const int2 *>(atoms + g * K + j0);
-        wv[0] = a2.x; wv[1] = a2.y; fv[0] = b2.x; fv[1] = b2.y; av[0] = c2.x; av[1] = c2.y;
-      } else {
-        wv[0] = worker[w * K + j0]; fv[0] = firm[f * K + j0]; av[0] = atoms[g * K + j0];
-      }
+    for (int off = 1; off < K; off <<= 1) {
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        if (j0 + q >= kcols) break;
-        const double proj = root * (wv[q] + fv[q]);
-        const double res = av[q] / root - proj;
-        bad |= !isfinite(proj) || !isfinite(res);
-        const double ps = proj * proj, rs = res * res;
-        neumaier_add(s[0], s[1], ps);
-        neumaier_add(s[2], s[3], rs);
-        neumaier_add(s[4], s[5], ps * ps);
-        neumaier_add(s[6], s[7], rs * rs);
-        neumaier_add(s[8], s[9], ps * rs);
+      for (int q = 0; q < 5; ++q) v[q] += __shfl_xor(v[q], off);
+    }
+    if (j == 0) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double s = m[(uint64_t)(2 * q) * units + g], e = m[(uint64_t)(2 * q + 1) * units + g];
+        neumaier_add(s, e, v[q]);
+        m[(uint64_t)(2 * q) * units + g] = s;
+        m[(uint64_t)(2 * q + 1) * units + g] = e;
       }
     }
-    if (bad) atomicCAS(status, 0, 73);
-#pragma unroll
-    for (int q = 0; q < 10; ++q) m[(uint64_t)q * units + g] = s[q];
   }
 }
 

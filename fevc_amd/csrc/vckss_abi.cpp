@@ -595,6 +595,15 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
   });
 }
 
+int32_t vckss_mi355x_counters_v1(uint64_t generation, uint64_t *out, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!out || capacity < 6) fail(kAbiMismatch, "counters", "capacity must be at least 6");
+    device_of(ctx).counters(out);
+  });
+}
+
 int32_t vckss_mi355x_stream_copy_v1(uint64_t generation, uint64_t bytes, double *milliseconds) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {

@@ -66,6 +66,12 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
                                       double *kernel_ms, double *kernel_bytes,
                                       uint64_t *kernel_launches);
 
+/* Device counters of the generation since upload: V-cycles applied, solve
+ * batches, refinement rounds, refined columns, PCG iterations launched
+ * (including iterations queued after convergence, which return at once),
+ * reserved. */
+int32_t vckss_mi355x_counters_v1(uint64_t generation, uint64_t *out, uint32_t capacity);
+
 /* Plain device-to-device copy of `bytes` through a streaming kernel
  * (k_stream_copy), for calibrating hardware traffic counters; returns the
  * kernel time in milliseconds. */
