@@ -126,12 +126,24 @@ template <int K, int NV, class Fin>
 __global__ void __launch_bounds__(kBlock) k_finish(const double *__restrict__ partial, uint32_t items, Fin fin,
                                                    const int *__restrict__ gate) {
   if (gate && *gate == 0) return;
+  if (!fin.enabled()) return;
   __shared__ double lds[NV][kBlock];
   const int j = blockIdx.x, tid = threadIdx.x;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
-    for (uint32_t b = tid; b < items; b += kBlock) s += partial[((uint64_t)b * NV + v) * K + j];
+    uint32_t b = tid;
+    for (; b + 3 * kBlock < items; b += 4 * kBlock) {
+      const double a0 = partial[((uint64_t)b * NV + v) * K + j];
+      const double a1 = partial[((uint64_t)(b + kBlock) * NV + v) * K + j];
+      const double a2 = partial[((uint64_t)(b + 2 * kBlock) * NV + v) * K + j];
+      const double a3 = partial[((uint64_t)(b + 3 * kBlock) * NV + v) * K + j];
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; b < items; b += kBlock) s += partial[((uint64_t)b * NV + v) * K + j];
     lds[v][tid] = s;
   }
   __syncthreads();

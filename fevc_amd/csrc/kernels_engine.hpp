@@ -45,6 +45,7 @@ __global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__
   block_partial<K, 3>(acc, partial);
 }
 struct FinStats {
+  __device__ bool enabled() const { return true; }
   ColState *__restrict__ st;
   const uint8_t *__restrict__ use;
   uint32_t n;
@@ -178,6 +179,7 @@ __global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double
   block_partial<K, 3>(acc, partial);
 }
 struct FinRho {
+  __device__ bool enabled() const { return true; }
   ColState *__restrict__ st;
   uint32_t n;
   __device__ void reduced(int j, const double (&tot)[3]) const {
@@ -266,6 +268,7 @@ __global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const dou
   block_partial<K, 4>(acc, partial);
 }
 struct FinUpdate {
+  __device__ bool enabled() const { return true; }
   ColState *__restrict__ st;
   PcgParams prm;
   __device__ void reduced(int j, const double (&tot)[4]) const {
@@ -361,6 +364,7 @@ __global__ void k_lane_sum(uint64_t rows, const double *__restrict__ v, double *
   block_partial<K, 1>(acc, partial);
 }
 struct FinMean {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ mean;
   const uint8_t *__restrict__ use;  // optional: zero the means of unused lanes
   uint64_t rows;
@@ -420,6 +424,7 @@ __global__ void k_reduced_residual(uint32_t firms, const double *__restrict__ rh
   block_partial<K, 2>(acc, partial);
 }
 struct FinRelative {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ result;
   __device__ void reduced(int j, const double (&tot)[2]) const {
     result[j] = tot[1] == 0.0 ? sqrt(tot[0]) : sqrt(tot[0]) / sqrt(tot[1]);
@@ -452,6 +457,7 @@ __global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__
 }
 template <int K>
 struct FinSums {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ sums;
   __device__ void reduced(int j, const double (&tot)[2]) const {
     sums[j] = tot[0];
@@ -700,6 +706,7 @@ __global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ 
 }
 template <int KT>
 struct FinTotals {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ totals;
   __device__ void reduced(int j, const double (&tot)[2]) const {
     totals[j] = tot[0];
@@ -765,6 +772,7 @@ __global__ void k_balance_sum(uint64_t rows, const double *__restrict__ v, doubl
 }
 template <int K2>
 struct FinBalance {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ v;
   uint64_t rows;
   int parity;
@@ -804,6 +812,7 @@ __global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_wor
   block_partial<KT, 3>(acc, partial);
 }
 struct FinDraws {
+  __device__ bool enabled() const { return true; }
   double *__restrict__ draws;
   __device__ void reduced(int j, const double (&tot)[3]) const {
     double *d = draws + 4 * j;
