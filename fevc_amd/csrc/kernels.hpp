@@ -270,16 +270,22 @@ __global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, S
 // Row chunk of y = A x with the neighbour values supplied by D:
 // sum over entries of w_ik (x_i - x_k), the reference's per-edge form
 // (vendor/cmg/src/graph.rs:664-705).
+// An operator may hoist per-lane values out of the walk with lane(j); the
+// result is handed to self/nbr (NoLane when nothing is hoisted).
+struct NoLane {};
+
 template <typename T, int K, class D>
 struct CsrOp {
   const uint32_t *__restrict__ col;
   const T *__restrict__ val;
+  __device__ NoLane lane(int) const { return {}; }
   __device__ T partial(const uint4 &d, int j, int s) const {
     const D &op = static_cast<const D &>(*this);
-    const T xi = op.self(d.x, j);
+    const auto ln = op.lane(j);
+    const T xi = op.self(d.x, j, ln);
     T acc = T(0);
-#pragma unroll 4
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) acc += val[k] * (xi - op.nbr(col[k], j));
+#pragma unroll 8
+    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) acc += val[k] * (xi - op.nbr(col[k], j, ln));
     return acc;
   }
   __device__ bool enabled() const { return true; }
@@ -299,16 +305,13 @@ struct OpScaleResidual : CsrOp<T, K, OpScaleResidual<T, K>> {
   const double *__restrict__ mean;
   const T *__restrict__ inv;
   T *__restrict__ b, *__restrict__ x, *__restrict__ r;
-  __device__ T centred(uint32_t i, int j) const {
-    const T v = braw[(uint64_t)i * K + j];
-    return mean ? v - (T)mean[j] : v;
-  }
-  __device__ T self(uint32_t i, int j) const { return inv[i] * centred(i, j); }
-  __device__ T nbr(uint32_t c, int j) const { return inv[c] * centred(c, j); }
+  __device__ T lane(int j) const { return mean ? (T)mean[j] : T(0); }  // the lane's coarse mean
+  __device__ T self(uint32_t i, int j, T mu) const { return inv[i] * (braw[(uint64_t)i * K + j] - mu); }
+  __device__ T nbr(uint32_t c, int j, T mu) const { return inv[c] * (braw[(uint64_t)c * K + j] - mu); }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
-    const T bi = centred(i, j);
+    const T bi = braw[q] - lane(j);
     b[q] = bi;
     x[q] = inv[i] * bi;
     r[q] = bi - ax;
@@ -320,8 +323,8 @@ template <typename T, int K>
 struct OpSmooth : CsrOp<T, K, OpSmooth<T, K>> {
   const T *__restrict__ b, *__restrict__ x, *__restrict__ inv;
   T *__restrict__ y;
-  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
-  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  __device__ T self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -334,8 +337,8 @@ template <typename T, int K>
 struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
   const T *__restrict__ b, *__restrict__ x;
   T *__restrict__ r;
-  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
-  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  __device__ T self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -351,10 +354,10 @@ struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
   const T *__restrict__ b, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
   const uint32_t *__restrict__ lab;
   T *__restrict__ y;
-  __device__ T self(uint32_t i, int j) const {
+  __device__ T self(uint32_t i, int j, NoLane = {}) const {
     return x[(uint64_t)i * K + j] + e[(uint64_t)lab[i] * K + j];
   }
-  __device__ T nbr(uint32_t c, int j) const { return self(c, j); }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return self(c, j); }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -367,8 +370,8 @@ template <typename T, int K>
 struct OpProduct : CsrOp<T, K, OpProduct<T, K>> {
   const T *__restrict__ x;
   T *__restrict__ y;
-  __device__ T self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
-  __device__ T nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  __device__ T self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     y[(uint64_t)i * K + j] = ax;

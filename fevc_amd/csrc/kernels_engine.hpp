@@ -100,8 +100,8 @@ struct OpWarm : CsrOp<double, K, OpWarm<K>> {
   double *__restrict__ r;
   ColState *__restrict__ st;
   PcgParams prm;
-  __device__ double self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
-  __device__ double nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  __device__ double self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -217,8 +217,8 @@ struct OpCurvature : CsrOp<double, K, OpCurvature<K>> {
   const double *__restrict__ p;
   double *__restrict__ ap;
   ColState *__restrict__ st;
-  __device__ double self(uint32_t i, int j) const { return p[(uint64_t)i * K + j]; }
-  __device__ double nbr(uint32_t c, int j) const { return p[(uint64_t)c * K + j]; }
+  __device__ double self(uint32_t i, int j, NoLane = {}) const { return p[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return p[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -297,8 +297,8 @@ struct OpFresh : CsrOp<double, K, OpFresh<K>> {
   PcgParams prm;
   const int *__restrict__ flags;
   __device__ bool enabled() const { return flags[kFlagFresh] != 0; }
-  __device__ double self(uint32_t i, int j) const { return x[(uint64_t)i * K + j]; }
-  __device__ double nbr(uint32_t c, int j) const { return x[(uint64_t)c * K + j]; }
+  __device__ double self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
     if (!(st[j].active && st[j].need_fresh)) return;
