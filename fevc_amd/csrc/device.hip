@@ -293,17 +293,17 @@ struct Cycle {
           VK_HIP(hipEventCreate(&e));
           E.events.push_back({a, e});
           // algorithmic bytes: CSR once (values, columns, chunk descriptors),
-          // inverse diagonal and labels, b and x read, y written, e read
+          // inverse diagonal and labels, r and x read, y written, e read
           const double s = sizeof(T);
           E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * L.rows.count + (s + 4) * L.n +
                           K * s * (3.0 * L.n + Lc.n);
         }
         segments<T, K, 0>(
             E, L.rows,
-            dev::OpProlongSmooth<T, K, true>{{L.col.p, val(L)}, b, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p}, a, e);
+            dev::OpProlongSmooth<T, K, true>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p}, a, e);
       } else {
         segments<T, K, 0>(E, L.rows,
-                          dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, b, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p});
+                          dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p});
       }
       std::swap(v.x, v.y);
     }

@@ -347,21 +347,21 @@ struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
 };
 
 // y = x' + D^-1 (b - A x'), x' = x + P e  (prolong-add, matvec, Jacobi).
-// `Finest` only names the finest-level instance (its own kernel symbol for
-// profiling); the arithmetic is the same.
+// With r = b - A x kept from the restriction, A x' = (b - r) + A P e, so
+// y = x' + D^-1 (r - A P e): the walk gathers only the coarse correction e
+// (small, cache resident), never the fine iterate.  `Finest` only names the
+// finest-level instance (its own kernel symbol for profiling).
 template <typename T, int K, bool Finest = false>
 struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
-  const T *__restrict__ b, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
+  const T *__restrict__ r, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
   const uint32_t *__restrict__ lab;
   T *__restrict__ y;
-  __device__ T self(uint32_t i, int j, NoLane = {}) const {
-    return x[(uint64_t)i * K + j] + e[(uint64_t)lab[i] * K + j];
-  }
-  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return self(c, j); }
+  __device__ T self(uint32_t i, int j, NoLane = {}) const { return e[(uint64_t)lab[i] * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return e[(uint64_t)lab[c] * K + j]; }
   template <int NA>
-  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+  __device__ void finish(uint32_t i, int j, T ape, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
-    y[q] = self(i, j) + inv[i] * (b[q] - ax);
+    y[q] = x[q] + self(i, j) + inv[i] * (r[q] - ape);
   }
 };
 

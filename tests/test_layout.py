@@ -141,8 +141,9 @@ def _vcycle(lays, levels, r0):
             r = b - matvec(lay, x)
             coarse = np.add.reduceat(r, agg[:-1]) if len(r) else r
             e = level(l + 1, coarse, coarse.mean(), lv["repeat"])
-            xp = x + e[lab]
-            x = xp + lay["inv"] * (b - matvec(lay, xp))
+            # prolong + smooth from the kept residual: A x' = (b - r) + A P e
+            pe = e[lab]
+            x = x + pe + lay["inv"] * (r - matvec(lay, pe))
         return x
 
     return level(0, r0, 0.0, 1)
