@@ -14,7 +14,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["device.hip", "host_prepare.cpp", "host_cmg.cpp", "vckss_abi.cpp"]
+SOURCES = ["device.hip", "host_prepare.cpp", "host_cmg.cpp", "vckss_abi.cpp", "vckss_abi_refusals.cpp"]
 OUTPUT = os.path.join(HERE, "libvckss_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

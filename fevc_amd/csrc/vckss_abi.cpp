@@ -266,6 +266,15 @@ VckssEngineSolveRequestV5 default_solve_v5() {
 
 }  // namespace
 
+namespace vk {
+void record_unsupported(const char *name) {
+  g_last_error = std::string(name) +
+                 ": not supported by the MI355X engine (it implements the compressed, no-control, "
+                 "match-deletion JLA path on route CMG_FULL_V2)";
+}
+const char *thread_last_error() { return g_last_error.c_str(); }
+}  // namespace vk
+
 extern "C" {
 
 uint32_t vckss_rust_abi_version(void) { return VCKSS_RUST_ABI_VERSION_V1; }

@@ -178,3 +178,25 @@ def test_hierarchy_hybrid_graph(oracle, seed):
     u, v, w, _ = g.export()
     levels = _assert_same_hierarchy(oracle, g.n, u, v, w)
     assert len(levels) >= 2
+
+
+def test_full_reference_abi_is_exported():
+    """Every function of the reference header links; the ones outside the
+    path are typed refusals (status 11 with a message, schema queries 0)."""
+    import ctypes as C
+    lib = F.library()
+    names = load_golden("abi_symbols.json")["functions"]
+    assert len(names) == 147
+    for f in names:
+        assert hasattr(lib, f["name"]), f["name"]
+    implemented = {name for name, _, _ in F.SIGNATURES}
+    refused = [f for f in names if f["name"] not in implemented and f["returns"] == "int32_t"]
+    assert len(refused) > 100
+    fn = getattr(lib, refused[0]["name"])
+    fn.restype = C.c_int32
+    assert fn() == 11
+    assert refused[0]["name"].encode() in lib.vckss_rust_last_error()
+    schema = [f for f in names if f["returns"] == "uint32_t" and f["name"] != "vckss_rust_abi_version"]
+    g = getattr(lib, schema[0]["name"])
+    g.restype = C.c_uint32
+    assert g() == 0
