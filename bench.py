@@ -61,6 +61,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
                     help="V-cycle precision (Krylov recurrences are always FP64)")
     ap.add_argument("--tolerance", type=float, default=1e-8)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick the batch width and segment chunk size by a timed search first "
+                         "(untimed; --width is then ignored)")
     ap.add_argument("--plain-steps", type=int, default=16,
                     help="single-RHS FP64 solves timed for the comparison (0 skips)")
     ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
@@ -90,9 +93,20 @@ def main():
     log(f"prepared in {prep_s:.1f}s: {d}")
 
     copy_bytes = 1 << 29
+    eng.stream_copy(copy_bytes)  # first call allocates and faults the pages in
     copy_ms = eng.stream_copy(copy_bytes)
     log(f"stream copy {copy_bytes >> 20} MiB: {2 * copy_bytes / copy_ms / 1e6:.0f} GB/s")
 
+    tuning = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuning = eng.autotune(args.precision, args.tolerance, batches=1)
+        args.width = tuning["width"]
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): width {tuning['width']}, chunk "
+            f"{tuning['chunk']}; trials " + ", ".join(f"{int(w)}/{int(c)}: {r:.0f}"
+                                                      for w, c, r in tuning["trials"]))
+        tuning = {"width": tuning["width"], "chunk": tuning["chunk"],
+                  "trials_solves_per_s": [[int(w), int(c), round(r, 1)] for w, c, r in tuning["trials"]]}
     # warmup also builds the device copy of the problem and hierarchy
     log(f"warmup: {args.warmup} batch(es) of {args.width}, fp{args.precision} V-cycle")
     eng.time_leverage(args.width, args.precision, args.tolerance, max(args.warmup, 1))
@@ -184,6 +198,7 @@ def main():
                    "deletion_units": d["units"], "hybrid_vertices": d["hybrid_vertices"],
                    "cmg_levels": d["levels"], "rhs_per_batch": args.width,
                    "vcycle_precision": args.precision,
+                   "segment_chunk": tuning["chunk"] if tuning else 64,
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -191,6 +206,7 @@ def main():
                      "kernel_us": round(tm["kernel_ms"] * 1e3, 2),
                      "algorithmic_bytes": tm["kernel_bytes"]},
         "cpu_baseline": cpu,
+        "autotune": tuning,
         "plain_fp64_single_rhs": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,

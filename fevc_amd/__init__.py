@@ -145,6 +145,10 @@ SIGNATURES = [
       C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("vckss_mi355x_counters_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_set_chunk_v1", C.c_int32, [C.c_uint64, C.c_uint32]),
+    ("vckss_mi355x_autotune_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+      C.POINTER(C.c_double), C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_hierarchy_build_v1", C.c_int32,
      [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
       C.POINTER(C.c_uint64)]),
@@ -324,6 +328,18 @@ class Engine:
             C.byref(iters), C.byref(ms), C.byref(by), C.byref(launches)))
         return dict(seconds=sec.value, solves=solves.value, iterations=iters.value,
                     kernel_ms=ms.value, kernel_bytes=by.value, kernel_launches=launches.value)
+
+    def set_chunk(self, chunk):
+        _check(library().vckss_mi355x_set_chunk_v1(self.generation, chunk))
+
+    def autotune(self, precision=32, tolerance=1e-8, batches=1):
+        w, c, r = C.c_uint32(), C.c_uint32(), C.c_double()
+        trials = np.zeros(3 * 9)
+        _check(library().vckss_mi355x_autotune_v1(self.generation, precision, tolerance, batches,
+                                                  C.byref(w), C.byref(c), C.byref(r),
+                                                  trials.ctypes.data, 9))
+        return dict(width=w.value, chunk=c.value, rate=r.value,
+                    trials=[tuple(x) for x in trials.reshape(9, 3)])
 
     def counters(self):
         out = np.zeros(6, np.uint64)

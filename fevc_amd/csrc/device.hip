@@ -33,7 +33,7 @@ namespace {
 
 using dev::kBlock;
 
-constexpr uint32_t kChunkEntries = 64;  // CSR entries or aggregate members per chunk
+constexpr uint32_t kChunkEntries = 64;  // default CSR entries or aggregate members per chunk
 constexpr int kPollSlots = 8;
 constexpr int kPollLag = 2;
 
@@ -142,6 +142,8 @@ struct DeviceEngine::Impl {
   hipStream_t stream = nullptr;
   const Prepared *P = nullptr;
   const HybridSystem *HS = nullptr;
+  const Hierarchy *H = nullptr;
+  uint32_t chunk = kChunkEntries;
   uint64_t W = 0, F = 0, C = 0, G = 0, S = 0, n0 = 0, aux = 0;
   double target_total = 0, bound = 0;
   DBuf<uint64_t> worker_ptr, firm_ptr, cell_unit_ptr, cell_stratum_ptr, aux_ptr;
@@ -568,6 +570,46 @@ std::vector<double> from_renumbered(const std::vector<double> &in, const std::ve
   return out;
 }
 
+// Device layout of the hierarchy and the firm passes for chunks of at most
+// `chunk` entries (renumbering, CSR, labels, chunk tables).
+void relayout(Impl &E, uint32_t chunk) {
+  const Hierarchy &h = *E.H;
+  const Prepared &p = *E.P;
+  E.chunk = chunk;
+  E.max_chunks = 0;
+  E.max_multi = 0;
+  {
+    std::vector<uint32_t> fptr(p.firm_ptr.begin(), p.firm_ptr.end());
+    E.firm_chunks.upload(chunk_segments(fptr, chunk));
+  }
+  std::vector<LevelLayout> layout = device_layout(h, chunk);
+  E.levels.resize(h.levels.size());
+  for (size_t l = 0; l < h.levels.size(); ++l) {
+    const Level &L = h.levels[l];
+    LevelLayout &Y = layout[l];
+    DevLevel &D = E.levels[l];
+    D.n = (uint32_t)L.graph.n;
+    D.nnz = Y.col.size();
+    D.coarse = (uint32_t)L.coarse;
+    D.repeat = (uint32_t)std::min<uint64_t>(L.repeat, 1u << 30);
+    D.terminal = L.terminal;
+    D.col.upload(Y.col); D.val.upload(Y.val); D.val32.upload(Y.val);
+    D.inv.upload(Y.inv); D.inv32.upload(Y.inv);
+    D.rows.upload(Y.rows);
+    if (!Y.lab.empty()) {
+      D.lab.upload(Y.lab);
+      D.aggs.upload(Y.aggs);
+    }
+    D.perm = Y.perm;
+    E.max_chunks = std::max<uint64_t>({E.max_chunks, Y.rows.chunks(), Y.aggs.chunks()});
+    E.max_multi = std::max({E.max_multi, Y.rows.multi(), Y.aggs.multi()});
+  }
+  E.max_chunks = std::max<uint64_t>(E.max_chunks, E.firm_chunks.count);
+  E.perm0.upload(layout[0].perm);
+  E.pos0.upload(layout[0].pos);
+  if (E.cap) E.seg_partial.alloc(std::max<uint64_t>(E.max_chunks, 1) * E.cap);
+}
+
 }  // namespace
 
 bool device_available() {
@@ -609,35 +651,8 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.cell_target.upload(p.cell_target);
   E.aux_ptr.upload(hs.aux_ptr); E.aux_nbr.upload(hs.aux_nbr); E.aux_w.upload(hs.aux_w);
 
-  {
-    std::vector<uint32_t> fptr(p.firm_ptr.begin(), p.firm_ptr.end());
-    E.firm_chunks.upload(chunk_segments(fptr, kChunkEntries));
-  }
-  std::vector<LevelLayout> layout = device_layout(h, kChunkEntries);
-  E.levels.resize(h.levels.size());
-  for (size_t l = 0; l < h.levels.size(); ++l) {
-    const Level &L = h.levels[l];
-    LevelLayout &Y = layout[l];
-    DevLevel &D = E.levels[l];
-    D.n = (uint32_t)L.graph.n;
-    D.nnz = Y.col.size();
-    D.coarse = (uint32_t)L.coarse;
-    D.repeat = (uint32_t)std::min<uint64_t>(L.repeat, 1u << 30);
-    D.terminal = L.terminal;
-    D.col.upload(Y.col); D.val.upload(Y.val); D.val32.upload(Y.val);
-    D.inv.upload(Y.inv); D.inv32.upload(Y.inv);
-    D.rows.upload(Y.rows);
-    if (!Y.lab.empty()) {
-      D.lab.upload(Y.lab);
-      D.aggs.upload(Y.aggs);
-    }
-    D.perm = Y.perm;
-    E.max_chunks = std::max<uint64_t>({E.max_chunks, Y.rows.chunks(), Y.aggs.chunks()});
-    E.max_multi = std::max({E.max_multi, Y.rows.multi(), Y.aggs.multi()});
-  }
-  E.max_chunks = std::max<uint64_t>(E.max_chunks, E.firm_chunks.count);
-  E.perm0.upload(layout[0].perm);
-  E.pos0.upload(layout[0].pos);
+  E.H = &h;
+  relayout(E, kChunkEntries);
   if (h.terminal_reason == kDirect) E.ginv.upload(h.terminal_inverse);
   E.rpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
   E.bpartial.alloc((size_t)2 * dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
@@ -957,6 +972,41 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   E.timing = false;
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+DeviceEngine::Tuning DeviceEngine::autotune(const SolveOptions &base, uint32_t batches) {
+  Impl &E = *impl_;
+  Tuning best;
+  double default_rate = 0.0;
+  for (uint32_t chunk : {32u, 64u, 128u}) {
+    relayout(E, chunk);
+    for (uint32_t width : {16u, 32u, 64u}) {
+      SolveOptions o = base;
+      o.leverage_width = width;
+      uint64_t solves = 0, iterations = 0;
+      (void)time_leverage_batches(o, 1, solves, iterations);  // warm
+      const double sec = time_leverage_batches(o, batches, solves, iterations);
+      const double rate = sec > 0 ? (double)solves / sec : 0.0;
+      best.trials.push_back({width, chunk, rate});
+      if (width == 64 && chunk == kChunkEntries) default_rate = rate;
+      if (rate > best.rate) {
+        best.rate = rate;
+        best.width = width;
+        best.chunk = chunk;
+      }
+    }
+  }
+  // the upload shape (64 probes, 64-entry chunks) stays unless another
+  // candidate is clearly faster, so timing noise does not flip the choice
+  if (best.rate < 1.03 * default_rate) {
+    best.rate = default_rate;
+    best.width = 64;
+    best.chunk = kChunkEntries;
+  }
+  relayout(E, best.chunk);
+  return best;
+}
+
+void DeviceEngine::set_chunk(uint32_t chunk) { relayout(*impl_, chunk); }
 
 void DeviceEngine::counters(uint64_t out[6]) const {
   const Impl &E = *impl_;

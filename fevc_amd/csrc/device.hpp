@@ -61,6 +61,17 @@ class DeviceEngine {
            std::vector<double> &x, std::vector<ColumnReceipt> &receipts);
   double time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
                                uint64_t &iterations);
+  // Timed search over probes per batch {16, 32, 64} and segment chunk sizes
+  // {32, 64, 128} on leverage batches of this problem; keeps the fastest
+  // layout and returns the choice (the caller keeps the width).
+  struct Tuning {
+    uint32_t width = 64, chunk = 64;
+    double rate = 0.0;
+    struct Trial { uint32_t width, chunk; double rate; };
+    std::vector<Trial> trials;
+  };
+  Tuning autotune(const SolveOptions &base, uint32_t batches);
+  void set_chunk(uint32_t chunk);
   // Plain device-to-device stream copy of `bytes` (counter calibration);
   // returns milliseconds.
   double stream_copy(uint64_t bytes);

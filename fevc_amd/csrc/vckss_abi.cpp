@@ -604,6 +604,42 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
   });
 }
 
+int32_t vckss_mi355x_autotune_v1(uint64_t generation, uint32_t precision, double tolerance, uint32_t batches,
+                                 uint32_t *width, uint32_t *chunk, double *rate, double *trials,
+                                 uint32_t trial_capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (precision != 64 && precision != 32) fail(kInvalidInput, "autotune", "precision must be 64 or 32");
+    if (!width || !chunk || !rate || batches == 0) fail(kInvalidInput, "autotune", "invalid arguments");
+    SolveOptions o;
+    o.precision = (int)precision;
+    o.probe_tolerance = tolerance;
+    auto t = device_of(ctx).autotune(o, batches);
+    *width = t.width;
+    *chunk = t.chunk;
+    *rate = t.rate;
+    if (trials)
+      for (size_t i = 0; i < t.trials.size() && i < trial_capacity; ++i) {
+        trials[3 * i] = t.trials[i].width;
+        trials[3 * i + 1] = t.trials[i].chunk;
+        trials[3 * i + 2] = t.trials[i].rate;
+      }
+    ctx.precision = precision;
+    ctx.leverage_width = t.width;
+    ctx.target_width = std::max<uint32_t>(1, std::min<uint32_t>(t.width / 2, 32));
+  });
+}
+
+int32_t vckss_mi355x_set_chunk_v1(uint64_t generation, uint32_t chunk) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (chunk < 16 || chunk > 1024) fail(kInvalidInput, "set_chunk", "chunk must lie in [16, 1024]");
+    device_of(ctx).set_chunk(chunk);
+  });
+}
+
 int32_t vckss_mi355x_counters_v1(uint64_t generation, uint64_t *out, uint32_t capacity) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {

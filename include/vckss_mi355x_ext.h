@@ -66,6 +66,20 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
                                       double *kernel_ms, double *kernel_bytes,
                                       uint64_t *kernel_launches);
 
+/* Autotuner: times `batches` leverage batches (after one warm batch) for every
+ * combination of probes per batch {16, 32, 64} and segment chunk size
+ * {32, 64, 128} on this problem at `precision`, keeps the fastest device
+ * layout (the upload shape, 64 and 64, unless another is 3% faster), and
+ * configures the generation's batch widths (leverage = width,
+ * target = width / 2).  trials (optional): (width, chunk, solves/s) triples. */
+int32_t vckss_mi355x_autotune_v1(uint64_t generation, uint32_t precision, double tolerance, uint32_t batches,
+                                 uint32_t *width, uint32_t *chunk, double *rate, double *trials,
+                                 uint32_t trial_capacity);
+
+/* Re-lays out the device hierarchy with segment chunks of at most `chunk`
+ * entries (16..1024; 64 at upload).  Results change only by summation order. */
+int32_t vckss_mi355x_set_chunk_v1(uint64_t generation, uint32_t chunk);
+
 /* Device counters of the generation since upload: V-cycles applied, solve
  * batches, refinement rounds, refined columns, PCG iterations launched
  * (including iterations queued after convergence, which return at once),

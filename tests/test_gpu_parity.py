@@ -183,13 +183,15 @@ def test_kat_end_to_end(widths):
 
 
 def _compare_engine(oracle, cols, probes, tolerance, rel_components, abs_arrays, precision=64,
-                    widths=(32, 16)):
+                    widths=(32, 16), chunk=None):
     p = oracle.Problem(*cols)
     want = p.run(probes=probes, fit_tolerance=1e-10,
                  probe_tolerance=tolerance if tolerance is not None else 1e-6)
     e = F.Engine(*cols)
     try:
         e.configure(precision, *widths)
+        if chunk is not None:
+            e.set_chunk(chunk)
         got = e.solve(probes=probes, tolerance=tolerance)
         arrays = e.arrays(probes)
         for key, okey in (("plugin", "plugin"), ("correction", "correction"), ("corrected", "corrected")):
@@ -233,6 +235,32 @@ def test_panel_width_invariance(oracle):
 def test_heavy_frequency_panel(oracle):
     _compare_engine(oracle, list(panel(True)), probes=8, tolerance=1e-11, rel_components=1e-8,
                     abs_arrays=1e-8)
+
+
+@pytest.mark.parametrize("chunk", [16, 128])
+def test_panel_chunk_sizes(oracle, chunk):
+    # the segment chunk size only reorders sums: same answers at the same tolerance
+    _compare_engine(oracle, list(panel(True)), probes=8, tolerance=1e-11, rel_components=1e-8,
+                    abs_arrays=1e-8, precision=32, widths=(64, 32), chunk=chunk)
+
+
+def test_autotune_then_solve(oracle):
+    cols = list(panel())
+    e = F.Engine(*cols)
+    try:
+        t = e.autotune(precision=32, tolerance=1e-8, batches=1)
+        assert t["width"] in (16, 32, 64) and t["chunk"] in (32, 64, 128)
+        assert len(t["trials"]) == 9 and all(r > 0 for _, _, r in t["trials"])
+        rates = {(int(w), int(c)): r for w, c, r in t["trials"]}
+        assert t["rate"] == rates[(t["width"], t["chunk"])]
+        assert t["rate"] == max(rates.values()) or (t["width"], t["chunk"]) == (64, 64)
+        got = e.solve(probes=16, tolerance=1e-11)
+    finally:
+        e.release()
+    want = oracle.Problem(*cols).run(probes=16, fit_tolerance=1e-10, probe_tolerance=1e-11)
+    for key in ("plugin", "correction", "corrected"):
+        g, w = np.asarray(got[key]), np.asarray(want[key])
+        assert np.all(np.abs(g - w) <= 1e-8 * np.maximum(np.abs(w), 1e-3)), (key, g, w)
 
 
 def test_error_paths():
