@@ -8,11 +8,16 @@ PCG on the degree-4 hybrid Laplacian to relative residual 1e-8.
 A step is one batch of `--width` probes through the whole per-probe path on
 the GPU: atoms -> exact right-hand sides -> Schur right-hand side ->
 lockstep certified PCG with the CMG V-cycle -> extraction and the complete
-residual gate -> leverage moments.  Inputs (panel, hierarchy) are resident in
-HBM before the timed region.  Reported beside it: the plain single-RHS FP64
-path on the same GPU, the full KSS estimate (fit, 256 leverage and 256
-target probes) wall time, and the oracle (the CPU restatement of the
-reference engine) on the host's cores on a bounded sample.
+residual gate -> leverage moments.  The headline runs the learned smoother
+weights (the library's built-in trained network, kept only when its
+calibration batch needs fewer iterations) with batched probes and the FP32
+V-cycle.  Inputs (panel, hierarchy) are resident in HBM before the timed
+region.  Reported beside it: the same batches with the plain CMG cycle, the
+plain single-RHS FP64 path on the same GPU, the full KSS estimate (fit, 256
+leverage and 256 target probes) wall time learned+batched against plain
+single-RHS FP64 with the errors between them, and the oracle (the CPU
+restatement of the reference engine) on the host's cores on a bounded
+sample.
 
 The roofline entry is the finest-level prolong + smooth pass of the V-cycle
 (k_segments<..., OpProlongSmooth<..., true>>), timed with HIP events on the
@@ -61,6 +66,8 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
                     help="V-cycle precision (Krylov recurrences are always FP64)")
     ap.add_argument("--tolerance", type=float, default=1e-8)
+    ap.add_argument("--learned", type=int, default=1,
+                    help="use the built-in learned smoother weights for the headline (0: plain CMG)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the batch width and segment chunk size by a timed search first "
                          "(untimed; --width is then ignored)")
@@ -97,6 +104,15 @@ def main():
     copy_ms = eng.stream_copy(copy_bytes)
     log(f"stream copy {copy_bytes >> 20} MiB: {2 * copy_bytes / copy_ms / 1e6:.0f} GB/s")
 
+    smoother = None
+    eng.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
+    if args.learned:
+        t0 = time.perf_counter()
+        eng.smoother(1)
+        smoother = eng.smoother_status()
+        smoother["setup_seconds"] = round(time.perf_counter() - t0, 3)
+        log(f"learned smoother: calibration {smoother['plain_iterations']} plain -> "
+            f"{smoother['learned_iterations']} learned iterations; in use: {smoother['active']}")
     tuning = None
     if args.autotune:
         t0 = time.perf_counter()
@@ -122,6 +138,18 @@ def main():
     log(f"batched: {value:.1f} solves/s, {iters:.1f} it/solve, finest prolong-smooth "
         f"{tm['kernel_ms'] * 1e3:.1f} us x {tm['kernel_launches']} -> {achieved:.0f} GB/s")
 
+    plain_batched = None
+    if args.learned and args.plain_steps > 0:
+        eng.smoother(0)
+        eng.time_leverage(args.width, args.precision, args.tolerance, 1)
+        t0 = time.perf_counter()
+        tb = eng.time_leverage(args.width, args.precision, args.tolerance, args.steps)
+        bw = time.perf_counter() - t0
+        plain_batched = {"solves_per_s": round(tb["solves"] / bw, 3),
+                         "iterations_per_solve": round(tb["iterations"] / max(tb["solves"], 1), 2)}
+        log(f"plain CMG, same batches: {plain_batched['solves_per_s']:.1f} solves/s, "
+            f"{plain_batched['iterations_per_solve']:.1f} it/solve")
+    eng.smoother(0)
     plain = None
     if args.plain_steps > 0:
         eng.time_leverage(1, 64, args.tolerance, 1)
@@ -137,6 +165,7 @@ def main():
         import numpy as np
 
         eng.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
+        eng.smoother(1 if args.learned else 0)
         t0 = time.perf_counter()
         res = eng.solve(probes=args.kss_probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
         kw = time.perf_counter() - t0
@@ -147,6 +176,7 @@ def main():
         if args.compare_plain:
             # the same estimate on the plain single-RHS FP64 path, identical draws
             eng.configure(64, 1, 1)
+            eng.smoother(0)
             t0 = time.perf_counter()
             ref = eng.solve(probes=args.kss_probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
             pw = time.perf_counter() - t0
@@ -157,6 +187,7 @@ def main():
             lev = {k: float(np.max(np.abs(arr[k] - ref_arr[k])))
                    for k in ("projection_share", "residual_share")}
             kss["plain_fp64_wall_seconds"] = round(pw, 3)
+            kss["wall_speedup_vs_plain_fp64"] = round(pw / kw, 2)
             kss["max_rel_error_vs_plain_fp64"] = rel
             kss["max_abs_leverage_error_vs_plain_fp64"] = lev
             log(f"plain FP64 KSS: {pw:.2f}s; component rel. error {rel}; leverage abs. error {lev}")
@@ -181,7 +212,7 @@ def main():
 
     traffic = measured_traffic(args.width, args.precision)
     line = {
-        "metric": "probe solves/s at rel.res 1e-8 (leverage probes, CMG-PCG)",
+        "metric": "probe solves/s at rel.res 1e-8 (leverage probes, learned+batched CMG-PCG)",
         "value": round(value, 3),
         "unit": "probe solves/s",
         "n_gpus": 1,
@@ -198,6 +229,7 @@ def main():
                    "deletion_units": d["units"], "hybrid_vertices": d["hybrid_vertices"],
                    "cmg_levels": d["levels"], "rhs_per_batch": args.width,
                    "vcycle_precision": args.precision,
+                   "smoother": "learned" if smoother and smoother["active"] else "plain",
                    "segment_chunk": tuning["chunk"] if tuning else 64,
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -207,6 +239,10 @@ def main():
                      "algorithmic_bytes": tm["kernel_bytes"]},
         "cpu_baseline": cpu,
         "autotune": tuning,
+        "learned_smoother": smoother,
+        "plain_cmg_batched": plain_batched,
+        "speedup_vs_plain_cmg_batched": (round(value / plain_batched["solves_per_s"], 3)
+                                         if plain_batched else None),
         "plain_fp64_single_rhs": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,

@@ -146,6 +146,16 @@ SIGNATURES = [
     ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("vckss_mi355x_counters_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_set_chunk_v1", C.c_int32, [C.c_uint64, C.c_uint32]),
+    ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_smoother_params_v1", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("vckss_mi355x_smoother_omega_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_smoother_gradient_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.c_void_p]),
+    ("vckss_mi355x_smoother_train_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_double, C.c_uint64, C.c_uint32,
+      C.c_void_p]),
     ("vckss_mi355x_autotune_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
       C.POINTER(C.c_double), C.c_void_p, C.c_uint32]),
@@ -329,6 +339,48 @@ class Engine:
         return dict(seconds=sec.value, solves=solves.value, iterations=iters.value,
                     kernel_ms=ms.value, kernel_bytes=by.value, kernel_launches=launches.value)
 
+    # ---- learned smoother weights (no counterpart in the reference)
+    def smoother(self, mode, params=None):
+        """mode 0: the reference cycle; 1: learned weights (built-in when params is None)."""
+        if params is None:
+            _check(library().vckss_mi355x_smoother_v1(self.generation, mode, None, 0))
+        else:
+            p = np.ascontiguousarray(params, np.float64)
+            _check(library().vckss_mi355x_smoother_v1(self.generation, mode, p.ctypes.data, len(p)))
+
+    def smoother_status(self):
+        out = np.zeros(5, np.uint32)
+        _check(library().vckss_mi355x_smoother_status_v1(self.generation, out.ctypes.data, 5))
+        return dict(mode=int(out[0]), active=bool(out[1]), breakdown=bool(out[2]),
+                    plain_iterations=int(out[3]), learned_iterations=int(out[4]))
+
+    def smoother_omega(self, params):
+        p = np.ascontiguousarray(params, np.float64)
+        written = C.c_uint64()
+        _check(library().vckss_mi355x_smoother_omega_v1(self.generation, p.ctypes.data, len(p), None, 0,
+                                                        C.byref(written)))
+        out = np.zeros(written.value)
+        _check(library().vckss_mi355x_smoother_omega_v1(self.generation, p.ctypes.data, len(p), out.ctypes.data,
+                                                        len(out), C.byref(written)))
+        return out
+
+    def smoother_gradient(self, params, errors):
+        p = np.ascontiguousarray(params, np.float64)
+        e = np.ascontiguousarray(errors, np.float64)
+        loss = C.c_double()
+        grad = np.zeros(len(p))
+        _check(library().vckss_mi355x_smoother_gradient_v1(self.generation, p.ctypes.data, len(p), e.ctypes.data,
+                                                           e.shape[0], C.byref(loss), grad.ctypes.data))
+        return loss.value, grad
+
+    def smoother_train(self, state, step0, steps, lr, seed, passes=2):
+        """Adam steps; `state` (3 x params) is updated in place; returns the losses."""
+        assert state.dtype == np.float64 and state.flags.c_contiguous
+        losses = np.zeros(max(steps, 1))
+        _check(library().vckss_mi355x_smoother_train_v1(self.generation, state.ctypes.data, len(state), step0,
+                                                        steps, lr, seed, passes, losses.ctypes.data))
+        return losses[:steps]
+
     def set_chunk(self, chunk):
         _check(library().vckss_mi355x_set_chunk_v1(self.generation, chunk))
 
@@ -402,6 +454,17 @@ def hierarchy_layout(n, u, v, w, chunk=64, direct_threshold=700, max_levels=128)
         return out
     finally:
         lib.vckss_mi355x_hierarchy_free_v1(h.value)
+
+
+def builtin_smoother():
+    """The library's built-in trained smoother parameters (None if it carries none)."""
+    n = C.c_uint32()
+    _check(library().vckss_mi355x_smoother_params_v1(None, 0, C.byref(n)))
+    if not n.value:
+        return None
+    out = np.zeros(n.value)
+    _check(library().vckss_mi355x_smoother_params_v1(out.ctypes.data, n.value, C.byref(n)))
+    return out
 
 
 def build_hierarchy(n, u, v, w, direct_threshold=700, max_levels=128):

@@ -9,237 +9,10 @@
 // Every level is renumbered so that aggregates are contiguous (restriction is
 // a contiguous segmented sum, prolongation labels are monotone); the level-0
 // renumbering is undone at the firm/auxiliary boundary of the hybrid system.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cstdio>
-#include <string>
-#include <type_traits>
-
-#include "common.hpp"
-#include "device.hpp"
-#include "kernels_engine.hpp"
-
-#define VK_HIP(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      ::vk::fail(::vk::kResourceLimit, "hip", std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+#include "device_impl.hpp"
 
 namespace vk {
 namespace {
-
-using dev::kBlock;
-
-constexpr uint32_t kChunkEntries = 64;  // default CSR entries or aggregate members per chunk
-constexpr int kPollSlots = 8;
-constexpr int kPollLag = 2;
-
-template <typename T>
-struct DBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf &) = delete;
-  DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-  DBuf &operator=(DBuf &&o) noexcept { std::swap(p, o.p); std::swap(n, o.n); return *this; }
-  ~DBuf() { if (p) (void)hipFree(p); }
-  void alloc(size_t count) {
-    if (p && count <= n) return;
-    if (p) VK_HIP(hipFree(p));
-    p = nullptr;
-    n = count;
-    VK_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-  }
-  template <typename V>
-  void upload(const std::vector<V> &v) {
-    alloc(v.size());
-    if (v.empty()) return;
-    if constexpr (std::is_same_v<T, V>) {
-      VK_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    } else {
-      std::vector<T> c(v.begin(), v.end());
-      VK_HIP(hipMemcpy(p, c.data(), c.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
-  }
-  // Ordered after the work already queued on `stream`; returns once the copy
-  // has landed.
-  void download(std::vector<T> &v, size_t count, hipStream_t stream) const {
-    v.resize(count);
-    if (count) VK_HIP(hipMemcpyAsync(v.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, stream));
-    VK_HIP(hipStreamSynchronize(stream));
-  }
-};
-
-// Grid for `threads` threads, capped, rounded up to a multiple of the 8 XCDs
-// (the segment kernels map block ranges per XCD).
-inline unsigned grid_for(uint64_t threads, unsigned cap = 8192) {
-  uint64_t g = (threads + kBlock - 1) / kBlock;
-  g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
-  return static_cast<unsigned>((g + 7) / 8 * 8);
-}
-inline unsigned reduce_grid(uint64_t threads) { return grid_for(threads, dev::kMaxReduceBlocks); }
-
-template <typename F>
-void with_k(uint32_t k, F &&f) {
-  switch (k) {
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 8: f(std::integral_constant<int, 8>{}); break;
-    case 16: f(std::integral_constant<int, 16>{}); break;
-    case 32: f(std::integral_constant<int, 32>{}); break;
-    case 64: f(std::integral_constant<int, 64>{}); break;
-    default: fail(kInvalidInput, "device", "batch lane count must be a power of two in [1, 64]");
-  }
-}
-
-uint32_t lanes_for(uint32_t columns) {
-  uint32_t k = 1;
-  while (k < columns) k <<= 1;
-  if (k > 64) fail(kInvalidInput, "device", "at most 64 right-hand sides per batch");
-  return k;
-}
-
-struct DevChunks {
-  DBuf<uint32_t> desc, mrange;
-  uint32_t count = 0, n_multi = 0;
-  void upload(const ChunkTable &t) {
-    desc.upload(t.desc);
-    mrange.upload(t.mrange);
-    count = static_cast<uint32_t>(t.chunks());
-    n_multi = t.multi();
-  }
-  dev::Chunks view() const { return {reinterpret_cast<const uint4 *>(desc.p), mrange.p, count, n_multi}; }
-};
-
-struct DevLevel {
-  uint32_t n = 0, coarse = 0, repeat = 0;
-  uint64_t nnz = 0;
-  int terminal = 0;
-  DBuf<uint32_t> col, lab;
-  DBuf<double> val, inv;
-  DBuf<float> val32, inv32;
-  DevChunks rows, aggs;
-  std::vector<uint32_t> perm;  // host copy: renumbered -> original vertex
-};
-
-template <typename T>
-struct LevelVecs {
-  DBuf<T> braw, b, x, y, r;
-  DBuf<double> mean;  // coarse mean of braw, filled by the parent's restriction
-};
-
-constexpr uint64_t kTagLeverage = 0x4c45564552414745ULL;
-constexpr uint64_t kTagTarget = 0x0054415247455401ULL;
-
-}  // namespace
-
-struct DeviceEngine::Impl {
-  hipStream_t stream = nullptr;
-  const Prepared *P = nullptr;
-  const HybridSystem *HS = nullptr;
-  const Hierarchy *H = nullptr;
-  uint32_t chunk = kChunkEntries;
-  uint64_t W = 0, F = 0, C = 0, G = 0, S = 0, n0 = 0, aux = 0;
-  double target_total = 0, bound = 0;
-  DBuf<uint64_t> worker_ptr, firm_ptr, cell_unit_ptr, cell_stratum_ptr, aux_ptr;
-  DBuf<uint32_t> firm_cells, cell_worker, cell_firm, cell_units, unit_cell, aux_nbr;
-  DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
-  DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
-  std::vector<DevLevel> levels;
-  DBuf<uint32_t> perm0, pos0;  // level-0 renumbering
-  DevChunks firm_chunks;       // firms over their cells (firm_cells order)
-  DBuf<double> ginv;
-  // reductions and segment scratch
-  DBuf<double> rpartial, bpartial, seg_partial;
-  uint64_t max_chunks = 0;
-  uint32_t max_multi = 0;
-  DBuf<int> flags;
-  const int *gate = nullptr;
-  int *host_flags = nullptr;
-  hipEvent_t poll_events[kPollSlots] = {};
-  uint32_t cap = 0;
-  std::vector<LevelVecs<double>> lv64;
-  std::vector<LevelVecs<float>> lv32;
-  DBuf<double> b0, proj, x, r, rc, z, p, ap, guess_rhs;
-  DBuf<dev::ColState> st;
-  DBuf<uint8_t> use;
-  DBuf<double> wrhs, frhs, worker, red, fsol, sch, wsum, sout;
-  DBuf<double> lane_a, lane_b, lane_c, reduced_res, complete_res, rhs_norm, sums;
-  DBuf<int32_t> atoms;
-  DBuf<double> moments, adjust, cellw, dir, totals, draws, fitted;
-  DBuf<int> status;
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-  double timed_bytes = 0;
-  uint64_t vcycles = 0;
-  // diagnostics: solve batches, refinement rounds, refined columns, PCG iterations launched
-  uint64_t batches = 0, refine_rounds = 0, refined_columns = 0, iterations_launched = 0;
-
-  const int *always() const { return flags.p + dev::kFlagOne; }
-
-  dev::SegScratch scratch() const { return {seg_partial.p, bpartial.p, gate}; }
-
-  void ensure(uint32_t lanes) {
-    if (lanes <= cap) return;
-    cap = lanes;
-    lv64.clear();
-    lv32.clear();
-    lv64.resize(levels.size());
-    lv32.resize(levels.size());
-    for (size_t l = 0; l < levels.size(); ++l) {
-      const size_t m = (size_t)levels[l].n * lanes;
-      for (auto *v : {&lv64[l].x, &lv64[l].y, &lv64[l].r}) v->alloc(m);
-      for (auto *v : {&lv32[l].braw, &lv32[l].b, &lv32[l].x, &lv32[l].y, &lv32[l].r}) v->alloc(m);
-      if (l > 0) { lv64[l].braw.alloc(m); lv64[l].b.alloc(m); }
-      lv64[l].mean.alloc(64);
-      lv32[l].mean.alloc(64);
-    }
-    const size_t m0 = (size_t)n0 * lanes;
-    for (auto *v : {&b0, &proj, &x, &r, &rc, &z, &p, &ap, &guess_rhs}) v->alloc(m0);
-    wrhs.alloc(W * lanes); frhs.alloc(F * lanes); worker.alloc(W * lanes); wsum.alloc(W * lanes);
-    red.alloc(F * lanes); fsol.alloc(F * lanes); sch.alloc(F * lanes); sout.alloc(F * lanes);
-    seg_partial.alloc(std::max<uint64_t>(max_chunks, 1) * lanes);
-  }
-
-  void check_status(const char *phase) {
-    int h = 0;
-    VK_HIP(hipMemcpyAsync(&h, status.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-    VK_HIP(hipStreamSynchronize(stream));
-    if (h) fail(h, phase, "device gate failed with status " + std::to_string(h));
-  }
-};
-
-namespace {
-
-using Impl = DeviceEngine::Impl;
-
-// Final per-lane reduction of `items` block partials (one block per lane).
-template <int K, int NV, class Fin>
-void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, const int *gate = nullptr) {
-  hipLaunchKernelGGL((dev::k_finish<K, NV, Fin>), K, kBlock, 0, E.stream, partial, items, fin, gate);
-}
-
-// Launch a segment operation over `ch` with operator `op`: pass 1 over all
-// chunks, pass 2 for multi-chunk segments, then the final reduction.
-template <typename T, int K, int NV, class Op>
-void segments(Impl &E, const DevChunks &ch, const Op &op, hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
-  constexpr int G = dev::Shape<K>::G;
-  const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
-  const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
-  const dev::SegScratch sc = E.scratch();
-  if (before) VK_HIP(hipEventRecord(before, E.stream));
-  hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
-  if (after) VK_HIP(hipEventRecord(after, E.stream));
-  if (ch.n_multi)
-    hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
-                       NV > 0 ? grid1 : 0u);
-  if constexpr (NV > 0) finish<K, NV>(E, E.bpartial.p, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
-}
 
 // ------------------------------------------------------------------ V-cycle
 template <typename T, int K>
@@ -252,8 +25,12 @@ struct Cycle {
   const T *val(const DevLevel &L) const {
     if constexpr (sizeof(T) == 4) return L.val32.p; else return L.val.p;
   }
+  // smoother inverse diagonal: 0.5 / d, or the learned omega / d on smoothed
+  // levels when the learned weights are in use (terminals stay plain)
   const T *inv(const DevLevel &L) const {
-    if constexpr (sizeof(T) == 4) return L.inv32.p; else return L.inv.p;
+    const bool learned = E.sm.active && !L.terminal;
+    if constexpr (sizeof(T) == 4) return learned ? L.linv32.p : L.inv32.p;
+    else return learned ? L.linv.p : L.inv.p;
   }
 
   // cycle.rs:136-311 with the coarse centring of cycle.rs:459-478 folded into
@@ -557,19 +334,6 @@ double component_identity(const double c[4]) {
 }
 
 // Host permutation of K-lane vectors between original and renumbered order.
-std::vector<double> to_renumbered(const std::vector<double> &in, const std::vector<uint32_t> &perm, uint32_t K) {
-  std::vector<double> out(in.size());
-  for (size_t p = 0; p < perm.size(); ++p)
-    for (uint32_t j = 0; j < K; ++j) out[p * K + j] = in[(size_t)perm[p] * K + j];
-  return out;
-}
-std::vector<double> from_renumbered(const std::vector<double> &in, const std::vector<uint32_t> &perm, uint32_t K) {
-  std::vector<double> out(in.size());
-  for (size_t p = 0; p < perm.size(); ++p)
-    for (uint32_t j = 0; j < K; ++j) out[(size_t)perm[p] * K + j] = in[p * K + j];
-  return out;
-}
-
 // Device layout of the hierarchy and the firm passes for chunks of at most
 // `chunk` entries (renumbering, CSR, labels, chunk tables).
 void relayout(Impl &E, uint32_t chunk) {
@@ -681,7 +445,7 @@ DeviceEngine::~DeviceEngine() {
   }
 }
 
-SolveResult DeviceEngine::run(const SolveOptions &opt) {
+SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
@@ -862,6 +626,7 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
 
 void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t columns, std::vector<double> &y) {
   Impl &E = *impl_;
+  PlainCycle plain(E);  // diagnostics use the reference cycle
   if (level >= E.levels.size()) fail(kInvalidInput, "device", "level out of range");
   const uint32_t K = lanes_for(columns);
   if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
@@ -884,6 +649,7 @@ void DeviceEngine::spmv(uint32_t level, const std::vector<double> &xin, uint32_t
 
 void DeviceEngine::vcycle(const std::vector<double> &rhs, uint32_t columns, int precision, std::vector<double> &out) {
   Impl &E = *impl_;
+  PlainCycle plain(E);  // diagnostics use the reference cycle
   const uint32_t K = lanes_for(columns);
   if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
   E.ensure(K);
@@ -906,6 +672,7 @@ void DeviceEngine::vcycle(const std::vector<double> &rhs, uint32_t columns, int 
 void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
                        std::vector<double> &xout, std::vector<ColumnReceipt> &receipts) {
   Impl &E = *impl_;
+  PlainCycle plain(E);  // diagnostics use the reference cycle
   const uint32_t K = lanes_for(columns);
   if (K != columns) fail(kInvalidInput, "device", "columns must be a power of two");
   E.ensure(K);

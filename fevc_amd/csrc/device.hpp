@@ -82,9 +82,35 @@ class DeviceEngine {
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
   KernelTiming dominant_kernel_timing() const;
 
+  // Learned smoother weights (no counterpart in the reference).  mode 0: the
+  // reference cycle; mode 1: the network's per-node weights (theta), used
+  // only if a calibration batch needs fewer PCG iterations than the plain
+  // cycle, and dropped for the plain cycle if a solve breaks down.
+  void smoother_mode(int mode, const std::vector<double> &theta, int precision);
+  // Learned weights chosen for this engine, calibrated on the first solve.
+  void smoother_default(const std::vector<double> &theta);
+  struct SmootherStatus {
+    int mode = 0;
+    bool active = false, breakdown = false;
+    uint32_t plain_iterations = 0, learned_iterations = 0;
+  };
+  SmootherStatus smoother_status() const;
+  // omega of every node of every smoothed level (levels in order, original
+  // vertex order within a level).
+  void smoother_omega(const std::vector<double> &theta, std::vector<double> &omega);
+  // Training loss and gradient for 32 given error columns (n x 32, original
+  // vertex order).
+  double smoother_gradient(const std::vector<double> &theta, const std::vector<double> &errors,
+                           std::vector<double> &grad);
+  // Adam steps on this hierarchy; state = parameters, first and second
+  // moments (3 x count); returns the loss of every step.
+  void smoother_train(std::vector<double> &state, uint64_t step0, uint32_t steps, double lr, uint64_t seed,
+                      uint32_t passes, std::vector<double> &losses);
+
   struct Impl;
 
  private:
+  SolveResult run_once(const SolveOptions &opt);
   std::unique_ptr<Impl> impl_;
 };
 

@@ -64,7 +64,7 @@ __device__ inline void philox(uint64_t k0, uint64_t k1, uint64_t c[4]) {
 // `first_probe`, as exact integers, interleaved: out[row * ld + col].  The
 // probe block varies fastest across threads so that stores are contiguous.
 // Callers guarantee trials < 2^31.
-__global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
+static __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
                         const uint64_t *__restrict__ trials, uint64_t seed, uint64_t tag,
                         uint64_t first_probe, uint32_t columns, uint32_t ld, int32_t *__restrict__ out) {
   const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
@@ -439,7 +439,7 @@ __global__ void k_terminal_scale(uint32_t n, const T *__restrict__ inv, const T 
 }
 
 // Plain stream copy (HBM traffic calibration of the profiler counters).
-__global__ void k_stream_copy(uint64_t count, const double4 *__restrict__ a, double4 *__restrict__ b) {
+static __global__ void k_stream_copy(uint64_t count, const double4 *__restrict__ a, double4 *__restrict__ b) {
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
        t += (uint64_t)gridDim.x * blockDim.x)
     b[t] = a[t];

@@ -634,7 +634,7 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
 struct AdjustParams {
   double probes, block_tolerance, rank_tolerance;
 };
-__global__ void k_adjust(uint64_t units, const double *__restrict__ m, const uint32_t *__restrict__ unit_cell,
+static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, const uint32_t *__restrict__ unit_cell,
                          const uint64_t *__restrict__ unit_count, const double *__restrict__ unit_outcome,
                          const double *__restrict__ fitted_cell, AdjustParams prm,
                          double *__restrict__ out /* 7 x units */, int *__restrict__ status) {
@@ -673,7 +673,7 @@ __global__ void k_adjust(uint64_t units, const double *__restrict__ m, const uin
 }
 
 // cell_correction_weight[c] = compensated sum over the cell's units ascending.
-__global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_ptr, const uint32_t *__restrict__ cu,
+static __global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_ptr, const uint32_t *__restrict__ cu,
                               const double *__restrict__ contrib, double *__restrict__ out) {
   VK_GRID_LOOP(c, cells) {
     double s = 0.0, k = 0.0;

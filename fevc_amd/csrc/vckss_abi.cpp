@@ -23,6 +23,7 @@
 #include "device.hpp"
 #include "host_cmg.hpp"
 #include "host_prepare.hpp"
+#include "smoother_params.inc"  // generated by fevc_amd/build.py from fevc_amd/smoother_params.json
 
 namespace {
 
@@ -198,14 +199,27 @@ SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ct
   return o;
 }
 
+std::vector<double> builtin_smoother() {
+  return std::vector<double>(kBuiltinSmoother, kBuiltinSmoother + kBuiltinSmootherCount);
+}
+
+DeviceEngine &device_of(Context &ctx) {
+  if (!ctx.dev) {
+    ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
+    // the built-in trained smoother weights, used where they help
+    if (kBuiltinSmootherCount) ctx.dev->smoother_default(builtin_smoother());
+  }
+  return *ctx.dev;
+}
+
 void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, VckssInterruptPollV1 poll, void *poll_ctx) {
   SolveOptions o = solve_options(q, ctx);
   if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK) fail(kUserBreak, "engine_solve", "user break");
-  if (!ctx.dev) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
+  DeviceEngine &dev = device_of(ctx);
   ctx.state = kSolving;
   ctx.has_result = false;
   try {
-    ctx.result = ctx.dev->run(o);
+    ctx.result = dev.run(o);
   } catch (...) {
     ctx.state = kFailed;
     throw;
@@ -529,11 +543,6 @@ int32_t vckss_mi355x_array_v1(uint64_t generation, uint32_t which, double *out, 
   });
 }
 
-static DeviceEngine &device_of(Context &ctx) {
-  if (!ctx.dev) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
-  return *ctx.dev;
-}
-
 int32_t vckss_mi355x_atoms_v1(uint64_t generation, uint64_t seed, uint32_t domain, uint64_t first_probe,
                               uint32_t columns, int64_t *out, uint64_t capacity) {
   std::lock_guard<std::mutex> lock(g_mutex);
@@ -637,6 +646,91 @@ int32_t vckss_mi355x_set_chunk_v1(uint64_t generation, uint32_t chunk) {
     Context &ctx = live(generation);
     if (chunk < 16 || chunk > 1024) fail(kInvalidInput, "set_chunk", "chunk must lie in [16, 1024]");
     device_of(ctx).set_chunk(chunk);
+  });
+}
+
+int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const double *params, uint32_t count) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (mode > 1) fail(kInvalidInput, "smoother", "mode must be 0 (plain) or 1 (learned)");
+    std::vector<double> theta;
+    if (mode == 1) {
+      if (params) {
+        theta.assign(params, params + count);
+      } else {
+        if (!kBuiltinSmootherCount) fail(kUnsupported, "smoother", "the library carries no trained smoother weights");
+        theta = builtin_smoother();
+      }
+    }
+    device_of(ctx).smoother_mode((int)mode, theta, (int)ctx.precision);
+  });
+}
+
+int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!out || capacity < 5) fail(kAbiMismatch, "smoother_status", "capacity must be at least 5");
+    const auto s = device_of(ctx).smoother_status();
+    const uint32_t v[5] = {(uint32_t)s.mode, s.active ? 1u : 0u, s.breakdown ? 1u : 0u, s.plain_iterations,
+                           s.learned_iterations};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+int32_t vckss_mi355x_smoother_params_v1(double *out, uint32_t capacity, uint32_t *count) {
+  return guarded([&] {
+    if (!count) fail(kInvalidInput, "smoother_params", "count is required");
+    *count = (uint32_t)kBuiltinSmootherCount;
+    if (out) {
+      if (capacity < kBuiltinSmootherCount) fail(kAbiMismatch, "smoother_params", "capacity too small");
+      std::memcpy(out, kBuiltinSmoother, kBuiltinSmootherCount * sizeof(double));
+    }
+  });
+}
+
+int32_t vckss_mi355x_smoother_omega_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                       uint64_t capacity, uint64_t *written) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!params || !written) fail(kInvalidInput, "smoother_omega", "params and written are required");
+    std::vector<double> w;
+    device_of(ctx).smoother_omega(std::vector<double>(params, params + count), w);
+    *written = w.size();
+    if (omega) {
+      if (capacity < w.size()) fail(kAbiMismatch, "smoother_omega", "capacity too small");
+      std::memcpy(omega, w.data(), w.size() * sizeof(double));
+    }
+  });
+}
+
+int32_t vckss_mi355x_smoother_gradient_v1(uint64_t generation, const double *params, uint32_t count,
+                                          const double *errors, uint64_t rows, double *loss, double *grad) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!params || !errors || !loss || !grad) fail(kInvalidInput, "smoother_gradient", "null argument");
+    if (rows != ctx.HS.graph.n) fail(kInvalidInput, "smoother_gradient", "errors must have one row per vertex");
+    std::vector<double> g;
+    *loss = device_of(ctx).smoother_gradient(std::vector<double>(params, params + count),
+                                             std::vector<double>(errors, errors + rows * 32), g);
+    std::memcpy(grad, g.data(), g.size() * sizeof(double));
+  });
+}
+
+int32_t vckss_mi355x_smoother_train_v1(uint64_t generation, double *state, uint32_t count, uint64_t step0,
+                                       uint32_t steps, double learning_rate, uint64_t seed, uint32_t passes,
+                                       double *losses) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!state || (steps && !losses)) fail(kInvalidInput, "smoother_train", "null argument");
+    std::vector<double> s(state, state + count), l;
+    device_of(ctx).smoother_train(s, step0, steps, learning_rate, seed, passes, l);
+    std::memcpy(state, s.data(), s.size() * sizeof(double));
+    if (steps) std::memcpy(losses, l.data(), l.size() * sizeof(double));
   });
 }
 

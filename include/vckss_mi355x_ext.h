@@ -80,6 +80,41 @@ int32_t vckss_mi355x_autotune_v1(uint64_t generation, uint32_t precision, double
  * entries (16..1024; 64 at upload).  Results change only by summation order. */
 int32_t vckss_mi355x_set_chunk_v1(uint64_t generation, uint32_t chunk);
 
+/* ---- Learned smoother weights (no counterpart in the reference) ----------
+ * A message-passing network (8 node features, 2 layers of 32 channels)
+ * predicts a Jacobi weight omega_i in (0, 0.95) for every vertex of every
+ * smoothed CMG level; the V-cycle then smooths with omega_i / d_i where the
+ * reference uses 0.5 / d_i.  Parameters: 4481 doubles.
+ *
+ * mode 0: the reference cycle.  mode 1: learned weights (params, or the
+ * library's built-in trained weights when params is NULL), kept only if a
+ * calibration batch of 16 leverage probes to rel.res 1e-8 needs fewer PCG
+ * iterations than with the plain cycle; a solve that breaks down with them
+ * (status 60 or 61) is redone with the plain cycle, which then stays.  A new
+ * generation starts in mode 1 with the built-in weights when the library
+ * carries them (calibrated on its first solve), else in mode 0. */
+int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const double *params, uint32_t count);
+/* out[5]: mode, learned weights in use, breakdown fallback taken, calibration
+ * iterations with the plain cycle, with the learned weights. */
+int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity);
+/* Built-in trained parameters (count = 0 when the library carries none). */
+int32_t vckss_mi355x_smoother_params_v1(double *out, uint32_t capacity, uint32_t *count);
+/* omega of every vertex of every smoothed level of the generation's hierarchy,
+ * levels in order, original vertex order within a level. */
+int32_t vckss_mi355x_smoother_omega_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                       uint64_t capacity, uint64_t *written);
+/* Training loss L = mean_j |e_j - M A e_j|_A^2 / |e_j|_A^2 over 32 error
+ * columns (rows x 32, vertex-major, original hybrid-vertex order) and dL/dparams. */
+int32_t vckss_mi355x_smoother_gradient_v1(uint64_t generation, const double *params, uint32_t count,
+                                          const double *errors, uint64_t rows, double *loss, double *grad);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on this generation's hierarchy.
+ * state: parameters, first moments, second moments (count = 3 x 4481),
+ * updated in place; step0 = steps already taken.  Error columns come from
+ * `seed` and `passes` plain-cycle error steps.  losses[steps]. */
+int32_t vckss_mi355x_smoother_train_v1(uint64_t generation, double *state, uint32_t count, uint64_t step0,
+                                       uint32_t steps, double learning_rate, uint64_t seed, uint32_t passes,
+                                       double *losses);
+
 /* Device counters of the generation since upload: V-cycles applied, solve
  * batches, refinement rounds, refined columns, PCG iterations launched
  * (including iterations queued after convergence, which return at once),
