@@ -61,8 +61,12 @@ struct Cycle {
         std::swap(v.x, v.y);
         segments<T, K, 0>(E, L.rows, dev::OpResidual<T, K>{{L.col.p, val(L)}, b, v.x.p, v.r.p});
       }
-      segments<T, K, 1>(E, L.aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, c.mean.p, Lc.n});
-      level(l + 1, L.repeat, c.mean.p);
+      // The coarse right-hand side sums residuals of a Laplacian (zero row
+      // sums) of a centred right-hand side, so its lane means vanish up to
+      // rounding: the reference's coarse centring (cycle.rs:459-478) is a
+      // rounding-level projection here and is not recomputed.
+      segments<T, K, 0>(E, L.aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, nullptr, Lc.n});
+      level(l + 1, L.repeat, nullptr);
       if (l == 0) {
         // the finest prolong-smooth pass is the kernel the roofline reports:
         // HIP events around its pass-1 launch when timing
@@ -90,21 +94,34 @@ struct Cycle {
 };
 
 // z = M r for K lanes; r is centred by `mean_r` (lane means) into `rc`.
-// Returns the device address of z.
+// z is the FP64 level-0 output, or with the FP32 cycle its FP32 output (the
+// Krylov kernels read it directly).
+struct CycleOut {
+  const double *z64 = nullptr;
+  const float *z32 = nullptr;
+};
 template <int K>
-const double *apply_cycle(Impl &E, const double *r, const double *mean_r, double *rc, bool fp32) {
+CycleOut apply_cycle_raw(Impl &E, const double *r, const double *mean_r, double *rc, bool fp32) {
   E.vcycles++;
   if (!fp32) {
     Cycle<double, K> cyc{E, E.lv64, r, rc};
     cyc.level(0, 1, mean_r);
-    return E.lv64[0].x.p;
+    return {E.lv64[0].x.p, nullptr};
   }
   const uint64_t m0 = E.n0 * K;
   hipLaunchKernelGGL((dev::k_centre_split<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, r, mean_r, rc,
                      E.lv32[0].braw.p, E.gate);
   Cycle<float, K> cyc{E, E.lv32, E.lv32[0].braw.p, E.lv32[0].b.p};
   cyc.level(0, 1, nullptr);
-  hipLaunchKernelGGL((dev::k_to_double<K>), grid_for(m0), kBlock, 0, E.stream, m0, E.lv32[0].x.p, E.z.p, E.gate);
+  return {nullptr, E.lv32[0].x.p};
+}
+// As above with z always in FP64 (returns its device address).
+template <int K>
+const double *apply_cycle(Impl &E, const double *r, const double *mean_r, double *rc, bool fp32) {
+  const CycleOut o = apply_cycle_raw<K>(E, r, mean_r, rc, fp32);
+  if (o.z64) return o.z64;
+  const uint64_t m0 = E.n0 * K;
+  hipLaunchKernelGGL((dev::k_to_double<K>), grid_for(m0), kBlock, 0, E.stream, m0, o.z32, E.z.p, E.gate);
   return E.z.p;
 }
 
@@ -143,10 +160,21 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   auto precondition = [&]() {
     hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags,
                        (const int *)nullptr);
-    const double *zp = apply_cycle<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
-    hipLaunchKernelGGL((dev::k_pcg_rho<K>), rg, kBlock, 0, E.stream, n, E.rc.p, zp, E.rpartial.p, active);
+    const CycleOut z = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
+    if (z.z32) {
+      hipLaunchKernelGGL((dev::k_pcg_rho<K, float>), rg, kBlock, 0, E.stream, n, E.rc.p, z.z32, E.rpartial.p, active);
+    } else {
+      hipLaunchKernelGGL((dev::k_pcg_rho<K, double>), rg, kBlock, 0, E.stream, n, E.rc.p, z.z64, E.rpartial.p,
+                         active);
+    }
     finish<K, 3>(E, E.rpartial.p, rg, dev::FinRho{E.st.p, n}, active);
-    hipLaunchKernelGGL((dev::k_pcg_direction<K>), grid_for(m0), kBlock, 0, E.stream, n, zp, E.p.p, E.st.p, active);
+    if (z.z32) {
+      hipLaunchKernelGGL((dev::k_pcg_direction<K, float>), grid_for(m0), kBlock, 0, E.stream, n, z.z32, E.p.p, E.st.p,
+                         active);
+    } else {
+      hipLaunchKernelGGL((dev::k_pcg_direction<K, double>), grid_for(m0), kBlock, 0, E.stream, n, z.z64, E.p.p,
+                         E.st.p, active);
+    }
   };
   precondition();
   for (uint32_t it = 1; it <= max_it; ++it) {
@@ -418,8 +446,8 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.H = &h;
   relayout(E, kChunkEntries);
   if (h.terminal_reason == kDirect) E.ginv.upload(h.terminal_inverse);
-  E.rpartial.alloc((size_t)dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
-  E.bpartial.alloc((size_t)2 * dev::kMaxReduceBlocks * dev::kMaxReduceValues * 64);
+  E.rpartial.alloc((size_t)dev::kPartialStride * dev::kMaxReduceValues * 64);
+  E.bpartial.alloc((size_t)dev::kPartialStride * dev::kMaxReduceValues * 64);
   E.flags.alloc(4);
   const int init_flags[4] = {1, 0, 1, 0};
   VK_HIP(hipMemcpy(E.flags.p, init_flags, sizeof init_flags, hipMemcpyHostToDevice));

@@ -28,6 +28,10 @@ namespace dev {
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 4096;
 constexpr int kMaxReduceValues = 4;
+// Block partials: value v of lane j from block slot b sits at
+// [(v * K + j) * kPartialStride + b], so the final per-lane sum reads
+// consecutive slots (pass-1 blocks, then combine blocks).
+constexpr uint32_t kPartialStride = 2 * kMaxReduceBlocks;
 constexpr uint32_t kSingle = 0xffffffffu;
 
 // Threads per chunk: V slices of K lanes, G = K * V <= 64 (one wavefront).
@@ -100,7 +104,7 @@ static __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
 // in a fixed order and hands the totals to Fin::reduced(lane, totals).
 
 template <int K, int NV>
-__device__ inline void block_partial(const double (&val)[NV], double *__restrict__ out) {
+__device__ inline void block_partial(const double (&val)[NV], double *__restrict__ out, uint32_t slot) {
   __shared__ double lds[NV][kBlock];
   constexpr int R = kBlock / K;
   const int tid = threadIdx.x, row = tid / K;
@@ -116,8 +120,12 @@ __device__ inline void block_partial(const double (&val)[NV], double *__restrict
   }
   if (tid < K) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[((uint64_t)blockIdx.x * NV + v) * K + tid] = lds[v][tid];
+    for (int v = 0; v < NV; ++v) out[((uint64_t)v * K + tid) * kPartialStride + slot] = lds[v][tid];
   }
+}
+template <int K, int NV>
+__device__ inline void block_partial(const double (&val)[NV], double *__restrict__ out) {
+  block_partial<K, NV>(val, out, blockIdx.x);
 }
 
 // Lane blockIdx.x of `items` block partials: thread t adds items t, t + kBlock,
@@ -132,18 +140,19 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double *__restrict__ pa
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
+    const double *row = partial + ((uint64_t)v * K + j) * kPartialStride;
     uint32_t b = tid;
     for (; b + 3 * kBlock < items; b += 4 * kBlock) {
-      const double a0 = partial[((uint64_t)b * NV + v) * K + j];
-      const double a1 = partial[((uint64_t)(b + kBlock) * NV + v) * K + j];
-      const double a2 = partial[((uint64_t)(b + 2 * kBlock) * NV + v) * K + j];
-      const double a3 = partial[((uint64_t)(b + 3 * kBlock) * NV + v) * K + j];
+      const double a0 = row[b];
+      const double a1 = row[b + kBlock];
+      const double a2 = row[b + 2 * kBlock];
+      const double a3 = row[b + 3 * kBlock];
       s += a0;
       s += a1;
       s += a2;
       s += a3;
     }
-    for (; b < items; b += kBlock) s += partial[((uint64_t)b * NV + v) * K + j];
+    for (; b < items; b += kBlock) s += row[b];
     lds[v][tid] = s;
   }
   __syncthreads();
@@ -264,7 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, S
     if (s == 0) op.finish(ch.desc[first].x, j, tree[j], acc);
     __syncthreads();
   }
-  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial + (uint64_t)blocks1 * NA * K);
+  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial, blocks1 + blockIdx.x);
 }
 
 // Row chunk of y = A x with the neighbour values supplied by D:

@@ -165,13 +165,13 @@ __global__ void k_mask_means(double *__restrict__ mean, const uint8_t *__restric
 }
 
 // rho = r.z - mean_z * sum r with r already centred (pcg.rs:1912-1947).
-template <int K>
-__global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const double *__restrict__ z,
+template <int K, typename Z>
+__global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const Z *__restrict__ z,
                           double *__restrict__ partial, const int *__restrict__ gate) {
   VK_GATE(gate);
   double acc[3] = {0.0, 0.0, 0.0};
   VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const double zz = z[t], rr = r[t];
+    const double zz = (double)z[t], rr = r[t];
     acc[0] += zz;
     acc[1] += rr * zz;
     acc[2] += rr;
@@ -198,15 +198,15 @@ struct FinRho {
 };
 
 // p = z_c (first or after a residual replacement) or z_c + beta p.
-template <int K>
-__global__ void k_pcg_direction(uint32_t n, const double *__restrict__ z, double *__restrict__ p,
+template <int K, typename Z>
+__global__ void k_pcg_direction(uint32_t n, const Z *__restrict__ z, double *__restrict__ p,
                                 const ColState *__restrict__ st, const int *__restrict__ gate) {
   VK_GATE(gate);
   VK_GRID_LOOP(t, (uint64_t)n * K) {
     const int j = t % K;
     const ColState &s = st[j];
     if (!s.active) continue;
-    const double zc = z[t] - s.mean_z;
+    const double zc = (double)z[t] - s.mean_z;
     p[t] = (s.first || s.restarted) ? zc : zc + s.beta * p[t];
   }
 }
