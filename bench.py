@@ -111,8 +111,8 @@ def main():
         eng.smoother(1)
         smoother = eng.smoother_status()
         smoother["setup_seconds"] = round(time.perf_counter() - t0, 3)
-        log(f"learned smoother: calibration {smoother['plain_iterations']} plain -> "
-            f"{smoother['learned_iterations']} learned iterations; in use: {smoother['active']}")
+        log(f"tuned preconditioner: calibration iterations (reference, learned, shaped, both) "
+            f"{smoother['calibration']}; learned weights {smoother['active']}, cycle levels {smoother['shape']}")
     tuning = None
     if args.autotune:
         t0 = time.perf_counter()
@@ -230,6 +230,7 @@ def main():
                    "cmg_levels": d["levels"], "rhs_per_batch": args.width,
                    "vcycle_precision": args.precision,
                    "smoother": "learned" if smoother and smoother["active"] else "plain",
+                   "cycle_levels": smoother["shape"] if smoother else None,
                    "segment_chunk": tuning["chunk"] if tuning else 64,
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

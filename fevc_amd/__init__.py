@@ -148,6 +148,7 @@ SIGNATURES = [
     ("vckss_mi355x_set_chunk_v1", C.c_int32, [C.c_uint64, C.c_uint32]),
     ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_params_v1", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_omega_v1", C.c_int32,
      [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -349,10 +350,19 @@ class Engine:
             _check(library().vckss_mi355x_smoother_v1(self.generation, mode, p.ctypes.data, len(p)))
 
     def smoother_status(self):
-        out = np.zeros(5, np.uint32)
-        _check(library().vckss_mi355x_smoother_status_v1(self.generation, out.ctypes.data, 5))
+        """mode, learned weights in use (active), breakdown fallback, calibration
+        iterations of the reference cycle and of the chosen configuration,
+        tuned shape in use, and the visited levels."""
+        out = np.zeros(10, np.uint32)
+        _check(library().vckss_mi355x_smoother_status_v1(self.generation, out.ctypes.data, 10))
+        n = C.c_uint32()
+        _check(library().vckss_mi355x_cycle_shape_v1(self.generation, None, 0, C.byref(n)))
+        shape = np.zeros(n.value, np.uint32)
+        _check(library().vckss_mi355x_cycle_shape_v1(self.generation, shape.ctypes.data, n.value, C.byref(n)))
         return dict(mode=int(out[0]), active=bool(out[1]), breakdown=bool(out[2]),
-                    plain_iterations=int(out[3]), learned_iterations=int(out[4]))
+                    plain_iterations=int(out[3]), learned_iterations=int(out[4]), shaped=bool(out[5]),
+                    shape=[int(x) for x in shape],
+                    calibration=[None if x == 0xFFFFFFFF else int(x) for x in out[6:10]])
 
     def smoother_omega(self, params):
         p = np.ascontiguousarray(params, np.float64)

@@ -50,8 +50,14 @@ struct Cycle {
                            mean, b, v.x.p, E.gate);
       return;
     }
-    DevLevel &Lc = E.levels[l + 1];
-    LevelVecs<T> &c = V[l + 1];
+    // next level visited: l + 1, or with the tuned shape a deeper level
+    // reached through the composite aggregation
+    const uint32_t nx = (E.sm.shaped && L.next) ? L.next : (uint32_t)l + 1;
+    const bool composite = nx != l + 1;
+    const uint32_t *lab = composite ? L.clab.p : L.lab.p;
+    const DevChunks &aggs = composite ? L.caggs : L.aggs;
+    DevLevel &Lc = E.levels[nx];
+    LevelVecs<T> &c = V[nx];
     for (uint32_t it = 0; it < iterations; ++it) {
       if (it == 0) {
         segments<T, K, 0>(E, L.rows,
@@ -65,8 +71,8 @@ struct Cycle {
       // sums) of a centred right-hand side, so its lane means vanish up to
       // rounding: the reference's coarse centring (cycle.rs:459-478) is a
       // rounding-level projection here and is not recomputed.
-      segments<T, K, 0>(E, L.aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, nullptr, Lc.n});
-      level(l + 1, L.repeat, nullptr);
+      segments<T, K, 0>(E, aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, nullptr, Lc.n});
+      level(nx, E.levels[nx - 1].repeat, nullptr);
       if (l == 0) {
         // the finest prolong-smooth pass is the kernel the roofline reports:
         // HIP events around its pass-1 launch when timing
@@ -83,10 +89,10 @@ struct Cycle {
         }
         segments<T, K, 0>(
             E, L.rows,
-            dev::OpProlongSmooth<T, K, true>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p}, a, e);
+            dev::OpProlongSmooth<T, K, true>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), lab, v.y.p}, a, e);
       } else {
         segments<T, K, 0>(E, L.rows,
-                          dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), L.lab.p, v.y.p});
+                          dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), lab, v.y.p});
       }
       std::swap(v.x, v.y);
     }
@@ -393,10 +399,12 @@ void relayout(Impl &E, uint32_t chunk) {
       D.aggs.upload(Y.aggs);
     }
     D.perm = Y.perm;
+    D.lab_host = Y.lab;
     E.max_chunks = std::max<uint64_t>({E.max_chunks, Y.rows.chunks(), Y.aggs.chunks()});
     E.max_multi = std::max({E.max_multi, Y.rows.multi(), Y.aggs.multi()});
   }
   E.max_chunks = std::max<uint64_t>(E.max_chunks, E.firm_chunks.count);
+  if (!E.sm.shape.empty()) set_cycle_shape(E, E.sm.shape);
   E.perm0.upload(layout[0].perm);
   E.pos0.upload(layout[0].pos);
   if (E.cap) E.seg_partial.alloc(std::max<uint64_t>(E.max_chunks, 1) * E.cap);

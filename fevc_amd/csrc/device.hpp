@@ -82,17 +82,21 @@ class DeviceEngine {
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
   KernelTiming dominant_kernel_timing() const;
 
-  // Learned smoother weights (no counterpart in the reference).  mode 0: the
-  // reference cycle; mode 1: the network's per-node weights (theta), used
-  // only if a calibration batch needs fewer PCG iterations than the plain
-  // cycle, and dropped for the plain cycle if a solve breaks down.
+  // Tuned preconditioner (no counterpart in the reference).  mode 0: the
+  // reference cycle; mode 1: the network's per-node smoother weights (theta)
+  // and/or a cycle shape that skips levels which coarsen little, chosen by a
+  // calibration batch (least work among configurations that need no more PCG
+  // iterations than the reference cycle), and dropped for the reference
+  // cycle if a solve breaks down.
   void smoother_mode(int mode, const std::vector<double> &theta, int precision);
   // Learned weights chosen for this engine, calibrated on the first solve.
   void smoother_default(const std::vector<double> &theta);
   struct SmootherStatus {
     int mode = 0;
-    bool active = false, breakdown = false;
-    uint32_t plain_iterations = 0, learned_iterations = 0;
+    bool active = false, breakdown = false, shaped = false;
+    uint32_t plain_iterations = 0, learned_iterations = 0;  // calibration: reference cycle, chosen one
+    std::vector<uint32_t> shape;  // visited levels when shaped
+    std::vector<uint32_t> calibration;  // iterations: reference, learned, shaped, both
   };
   SmootherStatus smoother_status() const;
   // omega of every node of every smoothed level (levels in order, original

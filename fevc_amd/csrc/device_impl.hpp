@@ -121,6 +121,12 @@ struct DevLevel {
   DBuf<float> linv32;
   DevChunks rows, aggs;
   std::vector<uint32_t> perm;  // host copy: renumbered -> original vertex
+  std::vector<uint32_t> lab_host;  // host copy of lab (renumbered coarse vertex)
+  // Cycle shape: the next level the tuned cycle visits after this one; when it
+  // skips levels, the composite aggregation to it (clab, caggs).
+  uint32_t next = 0;
+  DBuf<uint32_t> clab;
+  DevChunks caggs;
 };
 
 template <typename T>
@@ -165,6 +171,11 @@ struct SmootherState {
   bool calibrated = false;  // learned-vs-plain iteration check done
   bool active = false;      // learned inverse diagonals used by the solve path
   bool breakdown = false;   // a learned solve broke down and fell back
+  bool shaped = false;      // the tuned cycle shape (levels skipped) is in use
+  std::vector<uint32_t> shape;  // visited levels of the tuned shape
+  // calibration iterations: reference cycle, learned weights, tuned shape,
+  // both (UINT32_MAX where not tried)
+  std::vector<uint32_t> calibration;
   uint32_t plain_iterations = 0, learned_iterations = 0;
   std::vector<double> params;  // parameters of the learned weights in use
 };
@@ -253,12 +264,48 @@ struct DeviceEngine::Impl {
 
 namespace gpu {
 
+// Tuned cycle shape: the cycle visits the levels of `shape` (ascending, from 0
+// to the terminal); between visited levels a and b > a + 1 it restricts and
+// prolongs through the composite aggregation lab_{b-1}(... lab_a(i)), which
+// is monotone in the renumbered order, so its member ranges are contiguous.
+inline void set_cycle_shape(DeviceEngine::Impl &E, const std::vector<uint32_t> &shape) {
+  for (DevLevel &D : E.levels) D.next = 0;
+  for (size_t k = 0; k + 1 < shape.size(); ++k) {
+    const uint32_t a = shape[k], b = shape[k + 1];
+    if (b <= a || b >= E.levels.size()) fail(kInvariant, "cycle_shape", "levels must ascend within the hierarchy");
+    DevLevel &D = E.levels[a];
+    D.next = b;
+    if (b == a + 1) continue;
+    std::vector<uint32_t> lab = D.lab_host;
+    for (uint32_t l = a + 1; l < b; ++l)
+      for (uint32_t &x : lab) x = E.levels[l].lab_host[x];
+    const uint32_t nb = E.levels[b].n;
+    std::vector<uint32_t> ptr(nb + 1, 0);
+    for (uint32_t x : lab) ptr[x + 1]++;
+    for (uint32_t c = 0; c < nb; ++c) ptr[c + 1] += ptr[c];
+    D.clab.upload(lab);
+    const ChunkTable table = chunk_segments(ptr, E.chunk);
+    D.caggs.upload(table);
+    E.max_chunks = std::max<uint64_t>(E.max_chunks, table.chunks());
+    E.max_multi = std::max(E.max_multi, table.multi());
+  }
+  E.sm.shape = shape;
+  if (E.cap) E.seg_partial.alloc(std::max<uint64_t>(E.max_chunks, 1) * E.cap);
+}
+
 // Scope in which the V-cycle uses the reference smoother (0.5 / d).
 struct PlainCycle {
   DeviceEngine::Impl &E;
   bool saved;
-  explicit PlainCycle(DeviceEngine::Impl &e) : E(e), saved(e.sm.active) { E.sm.active = false; }
-  ~PlainCycle() { E.sm.active = saved; }
+  bool saved_shape;
+  explicit PlainCycle(DeviceEngine::Impl &e) : E(e), saved(e.sm.active), saved_shape(e.sm.shaped) {
+    E.sm.active = false;
+    E.sm.shaped = false;
+  }
+  ~PlainCycle() {
+    E.sm.active = saved;
+    E.sm.shaped = saved_shape;
+  }
 };
 
 using Impl = DeviceEngine::Impl;

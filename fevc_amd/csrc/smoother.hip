@@ -345,6 +345,10 @@ bool has_smoothed_levels(const Impl &E) {
   return false;
 }
 
+std::vector<uint32_t> tuned_shape(const Impl &E);
+std::vector<uint32_t> full_shape(const Impl &E);
+double iteration_work(const Impl &E, const std::vector<uint32_t> &shape);
+
 }  // namespace
 
 void DeviceEngine::smoother_omega(const std::vector<double> &theta, std::vector<double> &omega) {
@@ -432,20 +436,54 @@ void DeviceEngine::smoother_train(std::vector<double> &state, uint64_t step0, ui
   }
 }
 
+namespace {
+
+// Levels the tuned cycle visits: level 0, every level that at least halves
+// the last visited one, and the terminal.  Levels that coarsen less add
+// smoothing work but (measured) no convergence.
+std::vector<uint32_t> tuned_shape(const Impl &E) {
+  const uint32_t T = (uint32_t)E.levels.size() - 1;
+  std::vector<uint32_t> s{0};
+  for (uint32_t l = 1; l < T; ++l)
+    if (2ull * E.levels[l].n <= E.levels[s.back()].n) s.push_back(l);
+  if (T > 0) s.push_back(T);
+  return s;
+}
+
+std::vector<uint32_t> full_shape(const Impl &E) {
+  std::vector<uint32_t> s(E.levels.size());
+  for (uint32_t l = 0; l < s.size(); ++l) s[l] = l;
+  return s;
+}
+
+// Work of one PCG iteration with a cycle shape, in CSR-entry units: the
+// Krylov SpMV and vector passes at level 0 plus two level products and the
+// vector passes of every visited smoothed level.
+double iteration_work(const Impl &E, const std::vector<uint32_t> &shape) {
+  double w = 3.0 * (double)E.levels[0].nnz + 8.0 * E.levels[0].n;
+  for (uint32_t l : shape)
+    if (!E.levels[l].terminal) w += 2.0 * (double)E.levels[l].nnz + 4.0 * E.levels[l].n;
+  return w;
+}
+
+}  // namespace
+
 void DeviceEngine::smoother_mode(int mode, const std::vector<double> &theta, int precision) {
   Impl &E = *impl_;
   if (mode != 0 && mode != 1) fail(kInvalidInput, "smoother", "mode must be 0 (plain) or 1 (learned)");
   SmootherState &S = E.sm;
   S.active = false;
+  S.shaped = false;
   S.calibrated = false;
   S.breakdown = false;
   S.plain_iterations = S.learned_iterations = 0;
+  S.calibration.clear();
   S.mode = mode;
   E.precision_hint = precision;
   if (mode == 0) return;
   check_params(theta);
   S.params = theta;
-  if (!has_smoothed_levels(E)) {  // a direct terminal alone: nothing to smooth
+  if (!has_smoothed_levels(E)) {  // a direct terminal alone: nothing to tune
     S.calibrated = true;
     return;
   }
@@ -455,27 +493,54 @@ void DeviceEngine::smoother_mode(int mode, const std::vector<double> &theta, int
   upload_theta(E, theta);
   mpnn_forward(E);
   VK_HIP(hipStreamSynchronize(E.stream));
-  // calibration: one leverage batch of 16 probes to rel.res 1e-8, plain then
-  // learned; the learned weights stay only if they need fewer iterations
+  // Calibration: one leverage batch of 16 probes to rel.res 1e-8 with the
+  // reference cycle, then with the learned weights and/or the tuned shape.
+  // The configuration of least work (iterations x iteration work) among
+  // those needing no more iterations than the reference cycle is kept.
   SolveOptions o;
   o.leverage_width = 16;
   o.precision = precision;
   o.probe_tolerance = 1e-8;
-  uint64_t solves = 0, it_plain = 0, it_learned = 0;
-  S.active = false;
-  (void)time_leverage_batches(o, 1, solves, it_plain);
-  S.active = true;
-  try {
-    (void)time_leverage_batches(o, 1, solves, it_learned);
-  } catch (const Error &e) {
-    if (e.code != kPcgCurvature && e.code != kPcgPreconditioner) throw;
-    S.active = false;
-    S.breakdown = true;
-    it_learned = UINT32_MAX;
+  const std::vector<uint32_t> full = full_shape(E), tuned = tuned_shape(E);
+  const bool try_shape = tuned.size() < full.size();
+  set_cycle_shape(E, tuned);
+  auto iterations = [&](bool learned, bool shaped) -> uint64_t {
+    S.active = learned;
+    S.shaped = shaped;
+    uint64_t solves = 0, it = 0;
+    try {
+      (void)time_leverage_batches(o, 1, solves, it);
+    } catch (const Error &e) {
+      if (!learned || (e.code != kPcgCurvature && e.code != kPcgPreconditioner)) throw;
+      S.breakdown = true;
+      it = UINT64_MAX;
+    }
+    return it;
+  };
+  struct Option { bool learned, shaped; uint64_t it; double work; };
+  std::vector<Option> opts;
+  opts.push_back({false, false, iterations(false, false), iteration_work(E, full)});
+  opts.push_back({true, false, iterations(true, false), iteration_work(E, full)});
+  if (try_shape) {
+    opts.push_back({false, true, iterations(false, true), iteration_work(E, tuned)});
+    opts.push_back({true, true, iterations(true, true), iteration_work(E, tuned)});
   }
-  S.plain_iterations = (uint32_t)it_plain;
-  S.learned_iterations = (uint32_t)std::min<uint64_t>(it_learned, UINT32_MAX);
-  S.active = it_learned < it_plain;
+  // learned weights only where they cut iterations against the plain weights
+  // on the same shape; a shape only within 10% of the reference iterations
+  const uint64_t base = opts[0].it;
+  size_t best = 0;
+  for (size_t k = 1; k < opts.size(); ++k) {
+    if (opts[k].learned && opts[k].it >= opts[k - 1].it) continue;
+    if ((double)opts[k].it > 1.1 * (double)base) continue;
+    const double a = (double)opts[k].it * opts[k].work, b = (double)opts[best].it * opts[best].work;
+    if (a < b || (a == b && opts[k].it < opts[best].it)) best = k;
+  }
+  S.calibration.assign(4, UINT32_MAX);
+  for (size_t k = 0; k < opts.size(); ++k) S.calibration[k] = (uint32_t)std::min<uint64_t>(opts[k].it, UINT32_MAX);
+  S.active = opts[best].learned;
+  S.shaped = opts[best].shaped;
+  S.plain_iterations = (uint32_t)std::min<uint64_t>(base, UINT32_MAX);
+  S.learned_iterations = (uint32_t)std::min<uint64_t>(opts[best].it, UINT32_MAX);
   S.calibrated = true;
 }
 
@@ -496,6 +561,9 @@ DeviceEngine::SmootherStatus DeviceEngine::smoother_status() const {
   s.breakdown = S.breakdown;
   s.plain_iterations = S.plain_iterations;
   s.learned_iterations = S.learned_iterations;
+  s.shaped = S.shaped;
+  if (S.shaped) s.shape = S.shape;
+  s.calibration = S.calibration;
   return s;
 }
 
@@ -507,12 +575,13 @@ SolveResult DeviceEngine::run(const SolveOptions &opt) {
   Impl &E = *impl_;
   SmootherState &S = E.sm;
   if (S.mode == 1 && !S.calibrated) smoother_mode(1, S.params, opt.precision);
-  if (!S.active) return run_once(opt);
+  if (!S.active && !S.shaped) return run_once(opt);
   try {
     return run_once(opt);
   } catch (const Error &e) {
     if (e.code != kPcgCurvature && e.code != kPcgPreconditioner) throw;
     S.active = false;
+    S.shaped = false;
     S.breakdown = true;
     return run_once(opt);
   }

@@ -671,11 +671,31 @@ int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {
     Context &ctx = live(generation);
-    if (!out || capacity < 5) fail(kAbiMismatch, "smoother_status", "capacity must be at least 5");
+    if (!out || capacity < 10) fail(kAbiMismatch, "smoother_status", "capacity must be at least 10");
     const auto s = device_of(ctx).smoother_status();
-    const uint32_t v[5] = {(uint32_t)s.mode, s.active ? 1u : 0u, s.breakdown ? 1u : 0u, s.plain_iterations,
-                           s.learned_iterations};
+    uint32_t v[10] = {(uint32_t)s.mode, s.active ? 1u : 0u, s.breakdown ? 1u : 0u, s.plain_iterations,
+                      s.learned_iterations, s.shaped ? 1u : 0u, UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
+    for (size_t k = 0; k < s.calibration.size() && k < 4; ++k) v[6 + k] = s.calibration[k];
     std::memcpy(out, v, sizeof v);
+  });
+}
+
+int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint32_t capacity, uint32_t *count) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!count) fail(kInvalidInput, "cycle_shape", "count is required");
+    const auto s = device_of(ctx).smoother_status();
+    std::vector<uint32_t> shape = s.shape;
+    if (!s.shaped) {
+      shape.resize(ctx.H.levels.size());
+      for (uint32_t l = 0; l < shape.size(); ++l) shape[l] = l;
+    }
+    *count = (uint32_t)shape.size();
+    if (levels) {
+      if (capacity < shape.size()) fail(kAbiMismatch, "cycle_shape", "capacity too small");
+      std::memcpy(levels, shape.data(), shape.size() * sizeof(uint32_t));
+    }
   });
 }
 

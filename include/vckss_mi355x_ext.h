@@ -80,23 +80,33 @@ int32_t vckss_mi355x_autotune_v1(uint64_t generation, uint32_t precision, double
  * entries (16..1024; 64 at upload).  Results change only by summation order. */
 int32_t vckss_mi355x_set_chunk_v1(uint64_t generation, uint32_t chunk);
 
-/* ---- Learned smoother weights (no counterpart in the reference) ----------
- * A message-passing network (8 node features, 2 layers of 32 channels)
- * predicts a Jacobi weight omega_i in (0, 0.95) for every vertex of every
- * smoothed CMG level; the V-cycle then smooths with omega_i / d_i where the
- * reference uses 0.5 / d_i.  Parameters: 4481 doubles.
+/* ---- Tuned preconditioner (no counterpart in the reference) -------------
+ * Learned smoother weights: a message-passing network (8 node features, 2
+ * layers of 32 channels) predicts a Jacobi weight omega_i in (0, 0.95) for
+ * every vertex of every smoothed CMG level; the V-cycle then smooths with
+ * omega_i / d_i where the reference uses 0.5 / d_i.  Parameters: 4481
+ * doubles.  Cycle shape: the cycle may visit only level 0, the levels that at
+ * least halve the last visited one, and the terminal, restricting through the
+ * composite aggregation of the same hierarchy.
  *
- * mode 0: the reference cycle.  mode 1: learned weights (params, or the
- * library's built-in trained weights when params is NULL), kept only if a
- * calibration batch of 16 leverage probes to rel.res 1e-8 needs fewer PCG
- * iterations than with the plain cycle; a solve that breaks down with them
- * (status 60 or 61) is redone with the plain cycle, which then stays.  A new
- * generation starts in mode 1 with the built-in weights when the library
- * carries them (calibrated on its first solve), else in mode 0. */
+ * mode 0: the reference cycle.  mode 1: a calibration batch of 16 leverage
+ * probes to rel.res 1e-8 is solved with the reference cycle, the learned
+ * weights (params, or the library's built-in trained weights when params is
+ * NULL), the tuned shape and both; the configuration of least work
+ * (iterations x level products per iteration) is kept, where learned weights
+ * count only if they cut iterations against the plain weights on the same
+ * shape and a shape only within 10% of the reference cycle's iterations.  A solve that breaks down with
+ * it (status 60 or 61) is redone with the reference cycle, which then stays.
+ * A new generation starts in mode 1 with the built-in weights when the
+ * library carries them (calibrated on its first solve), else in mode 0. */
 int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const double *params, uint32_t count);
-/* out[5]: mode, learned weights in use, breakdown fallback taken, calibration
- * iterations with the plain cycle, with the learned weights. */
+/* out[10]: mode, learned weights in use, breakdown fallback taken, calibration
+ * iterations with the reference cycle, with the chosen configuration, tuned
+ * cycle shape in use, then the calibration iterations of the reference cycle,
+ * the learned weights, the tuned shape and both (UINT32_MAX if not tried). */
 int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity);
+/* Levels the cycle visits (all levels unless the tuned shape is in use). */
+int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint32_t capacity, uint32_t *count);
 /* Built-in trained parameters (count = 0 when the library carries none). */
 int32_t vckss_mi355x_smoother_params_v1(double *out, uint32_t capacity, uint32_t *count);
 /* omega of every vertex of every smoothed level of the generation's hierarchy,

@@ -22,6 +22,8 @@ ONE_LEVEL = synth.PanelSpec(workers=60000, firms=3000, periods=5, mobility=0.15,
                             name="smoother-one-level")
 DEEP = synth.PanelSpec(workers=300000, firms=15000, periods=5, mobility=0.1, pareto_shape=1.1, seed=24,
                        name="smoother-deep")
+DEEP_SKIP = synth.PanelSpec(workers=600000, firms=30000, periods=5, mobility=0.2, pareto_shape=1.2, seed=25,
+                            name="smoother-deep-skip")
 HELD_OUT = synth.PanelSpec(workers=400000, firms=20000, periods=5, mobility=0.12, pareto_shape=1.15, seed=9001,
                            name="smoother-held-out")
 
@@ -128,16 +130,41 @@ def test_builtin_weights_cut_iterations_on_held_out_panel():
 
 def test_plain_cycle_takes_over_when_learned_weights_do_not_help():
     # a0 far negative: omega ~ 0, a useless smoother; the calibration keeps
-    # the plain cycle and the solve equals the plain solve bit for bit
+    # the plain weights and, on a one-level hierarchy (no shape to tune),
+    # the solve equals the plain solve bit for bit
     theta = S.initial_params(seed=2)
     theta[S.OFF["a0"][0]] = -12.0
     cols = columns(ONE_LEVEL)
     plain, _, _ = _solve(cols, 0, probes=8)
     got, st, _ = _solve(cols, 1, params=theta, probes=8)
-    assert st["mode"] == 1 and not st["active"]
-    assert st["learned_iterations"] >= st["plain_iterations"]
+    assert st["mode"] == 1 and not st["active"] and not st["shaped"]
+    assert st["learned_iterations"] == st["plain_iterations"]
     for key in ("plugin", "correction", "corrected"):
         assert np.array_equal(np.asarray(got[key]), np.asarray(plain[key])), key
+
+
+def test_tuned_shape_skips_levels_without_losing_iterations():
+    """The tuned cycle shape on a deep hierarchy: fewer levels visited,
+    within 10% of the reference cycle's PCG iterations, same components."""
+    cols = columns(DEEP_SKIP)
+    plain, sp, ap = _solve(cols, 0)
+    tuned, st, at = _solve(cols, 1, params=_useless_weights())
+    assert not st["active"]
+    assert st["shaped"], st
+    assert len(st["shape"]) < len(sp["shape"]) and st["shape"][0] == 0 and st["shape"][-1] == sp["shape"][-1]
+    assert st["learned_iterations"] <= 1.1 * st["plain_iterations"]
+    it_plain, it_tuned = np.mean(ap["leverage_iterations"]), np.mean(at["leverage_iterations"])
+    print(f"tuned shape {st['shape']} (of {sp['shape']}): {it_plain:.2f} -> {it_tuned:.2f} iterations per solve")
+    assert it_tuned <= 1.1 * it_plain
+    for key in ("plugin", "correction", "corrected"):
+        a, b = np.asarray(tuned[key]), np.asarray(plain[key])
+        assert np.all(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1e-3)), key
+
+
+def _useless_weights():
+    theta = S.initial_params(seed=2)
+    theta[S.OFF["a0"][0]] = -12.0
+    return theta
 
 
 def test_learned_solve_matches_oracle(oracle):
