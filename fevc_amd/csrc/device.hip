@@ -219,7 +219,7 @@ template <int K>
 void extract(Impl &E) {
   const uint64_t F = E.F, W = E.W;
   hipLaunchKernelGGL((dev::k_gather_rows<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.pos0.p, E.x.p, E.fsol.p);
-  const unsigned fg = reduce_grid(F * K), wg = reduce_grid(W * K);
+  const unsigned fg = reduce_grid(F * K), wg = reduce_grid(W * dev::Pack<K>::TPR);
   hipLaunchKernelGGL((dev::k_lane_sum<K>), fg, kBlock, 0, E.stream, F, E.fsol.p, E.rpartial.p);
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p,
@@ -231,7 +231,7 @@ void extract(Impl &E) {
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p,
                      nullptr);
-  hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * K), kBlock, 0, E.stream, W, E.worker_ptr.p,
+  hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * dev::Pack<K>::TPR), kBlock, 0, E.stream, W, E.worker_ptr.p,
                      E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
   segments<double, K, 1>(E, E.firm_chunks,
                          dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.red.p,
@@ -533,13 +533,13 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
                          (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p);
       segments<double, K, 0>(E, E.firm_chunks,
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
                                                  E.frhs.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * K), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });
@@ -758,13 +758,13 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * K), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
+      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
                          (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p);
       segments<double, K, 0>(E, E.firm_chunks,
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
                                                  E.frhs.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * K), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                          E.moments.p, E.status.p);
     });

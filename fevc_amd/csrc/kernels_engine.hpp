@@ -373,6 +373,38 @@ struct FinMean {
   }
 };
 
+// Lane packing for the per-row gather passes: LP consecutive lanes per
+// thread (16- and 32-byte loads) and TPR = K / LP threads per row, so a
+// wavefront walks 64 / TPR rows at once (more loads in flight).
+template <int K>
+struct Pack {
+  static constexpr int LP = K >= 4 ? 4 : K;
+  static constexpr int TPR = K / LP;
+};
+
+// block_partial for packed threads: thread t holds lanes (t % TPR) * LP + q.
+template <int K, int NV>
+__device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV], double *__restrict__ out) {
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR, R = kBlock / TPR;
+  __shared__ double lds[NV][R * K];
+  const int tid = threadIdx.x, row = tid / TPR, j0 = (tid % TPR) * LP;
+#pragma unroll
+  for (int q = 0; q < LP; ++q)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lds[v][row * K + j0 + q] = val[q][v];
+  __syncthreads();
+  for (int w = R / 2; w > 0; w >>= 1) {
+    for (int e = tid; e < w * K; e += kBlock)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lds[v][e] += lds[v][e + w * K];
+    __syncthreads();
+  }
+  if (tid < K) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[((uint64_t)v * K + tid) * kPartialStride + blockIdx.x] = lds[v][tid];
+  }
+}
+
 // Exact-integer leverage right-hand sides (engine.rs:2562-2619): per worker
 // and per firm, the sum of the atoms of the units of its cells.
 template <int K>
@@ -380,17 +412,23 @@ __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
                            const uint32_t *__restrict__ cells, const uint64_t *__restrict__ cu_ptr,
                            const uint32_t *__restrict__ cu, const int32_t *__restrict__ atoms,
                            uint64_t units, uint32_t kcols, double *__restrict__ out) {
-  VK_GRID_LOOP(t, rows * K) {
-    const uint64_t i = t / K;
-    const int j = t % K;
-    double s = 0.0;
-    if (j < (int)kcols) {
-      for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-        const uint32_t c = cells ? cells[k] : (uint32_t)k;
-        for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) s += (double)atoms[(uint64_t)cu[q] * K + j];
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+  VK_GRID_LOOP(t, rows * TPR) {
+    const uint64_t i = t / TPR;
+    const int j0 = (int)(t % TPR) * LP;
+    double s[LP];
+#pragma unroll
+    for (int q = 0; q < LP; ++q) s[q] = 0.0;
+    for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const uint32_t c = cells ? cells[k] : (uint32_t)k;
+      for (uint64_t u = cu_ptr[c]; u < cu_ptr[c + 1]; ++u) {
+        const int32_t *a = atoms + (uint64_t)cu[u] * K + j0;
+#pragma unroll
+        for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
       }
     }
-    out[t] = s;
+#pragma unroll
+    for (int q = 0; q < LP; ++q) out[i * K + j0 + q] = j0 + q < (int)kcols ? s[q] : 0.0;
   }
 }
 
@@ -400,12 +438,22 @@ __global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ w
                                 const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
                                 const double *__restrict__ wdiag, const double *__restrict__ v,
                                 double *__restrict__ out) {
-  VK_GRID_LOOP(t, workers * K) {
-    const uint64_t w = t / K;
-    const int j = t % K;
-    double s = 0.0;
-    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) s += cell_weight[c] * v[(uint64_t)cell_firm[c] * K + j];
-    out[t] = s / wdiag[w];
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+  VK_GRID_LOOP(t, workers * TPR) {
+    const uint64_t w = t / TPR;
+    const int j0 = (int)(t % TPR) * LP;
+    double s[LP];
+#pragma unroll
+    for (int q = 0; q < LP; ++q) s[q] = 0.0;
+    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
+      const double cw = cell_weight[c];
+      const double *f = v + (uint64_t)cell_firm[c] * K + j0;
+#pragma unroll
+      for (int q = 0; q < LP; ++q) s[q] += cw * f[q];
+    }
+    const double d = wdiag[w];
+#pragma unroll
+    for (int q = 0; q < LP; ++q) out[w * K + j0 + q] = s[q] / d;
   }
 }
 
@@ -432,28 +480,41 @@ struct FinRelative {
 };
 
 // worker = (wrhs - sum_c w_c firm[f_c]) / d_w (operator.rs:311-343) and the
-// worker block of the complete residual (operator.rs:386-458).
+// worker block of the complete residual (operator.rs:386-458),
+// r_w = (wrhs - sum_c w_c firm[f_c]) - d_w worker with the computed worker.
 template <int K>
 __global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__ wptr,
                                   const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
                                   const double *__restrict__ wdiag, const double *__restrict__ wrhs,
                                   const double *__restrict__ firm, double *__restrict__ worker,
                                   double *__restrict__ partial) {
-  double acc[2] = {0.0, 0.0};
-  VK_GRID_LOOP(t, workers * K) {
-    const uint64_t w = t / K;
-    const int j = t % K;
-    const double b = wrhs[t];
-    double s = b;
-    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) s -= cell_weight[c] * firm[(uint64_t)cell_firm[c] * K + j];
-    const double x = s / wdiag[w];
-    worker[t] = x;
-    double rw = b - wdiag[w] * x;
-    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) rw -= cell_weight[c] * firm[(uint64_t)cell_firm[c] * K + j];
-    acc[0] += rw * rw;
-    acc[1] += b * b;
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+  double acc[LP][2];
+#pragma unroll
+  for (int q = 0; q < LP; ++q) acc[q][0] = acc[q][1] = 0.0;
+  VK_GRID_LOOP(t, workers * TPR) {
+    const uint64_t w = t / TPR;
+    const int j0 = (int)(t % TPR) * LP;
+    double b[LP], s[LP];
+#pragma unroll
+    for (int q = 0; q < LP; ++q) s[q] = b[q] = wrhs[w * K + j0 + q];
+    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
+      const double cw = cell_weight[c];
+      const double *f = firm + (uint64_t)cell_firm[c] * K + j0;
+#pragma unroll
+      for (int q = 0; q < LP; ++q) s[q] -= cw * f[q];
+    }
+    const double d = wdiag[w];
+#pragma unroll
+    for (int q = 0; q < LP; ++q) {
+      const double x = s[q] / d;
+      worker[w * K + j0 + q] = x;
+      const double rw = s[q] - d * x;
+      acc[q][0] += rw * rw;
+      acc[q][1] += b[q] * b[q];
+    }
   }
-  block_partial<K, 2>(acc, partial);
+  block_partial_packed<K, 2>(acc, partial);
 }
 template <int K>
 struct FinSums {
@@ -592,33 +653,39 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
                           const double *__restrict__ firm, const int32_t *__restrict__ atoms,
                           double *__restrict__ m, int *__restrict__ status) {
-  // K consecutive threads per unit (one lane each, coalesced loads); the
-  // batch's five sums are combined by a fixed xor butterfly and added to the
-  // unit's running compensated sums by lane 0, batches in probe order.
-  VK_GRID_LOOP(t, units * K) {
-    const uint64_t g = t / K;
-    const int j = t % K;
+  // TPR threads per unit, LP lanes each (packed loads); the batch's five sums
+  // are combined in-thread, then by a fixed xor butterfly over the unit's
+  // threads, and added to the unit's running compensated sums by its first
+  // thread, batches in probe order.
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+  VK_GRID_LOOP(t, units * TPR) {
+    const uint64_t g = t / TPR;
+    const int j0 = (int)(t % TPR) * LP;
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double root = sqrt((double)unit_count[g]);
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j < (int)kcols) {
-      const uint32_t c = unit_cell[g];
-      const uint64_t w = cell_worker[c], f = cell_firm[c];
-      const double root = sqrt((double)unit_count[g]);
-      const double proj = root * (worker[w * K + j] + firm[f * K + j]);
-      const double res = (double)atoms[g * K + j] / root - proj;
-      if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
-      const double ps = proj * proj, rs = res * res;
-      v[0] = ps;
-      v[1] = rs;
-      v[2] = ps * ps;
-      v[3] = rs * rs;
-      v[4] = ps * rs;
+#pragma unroll
+    for (int q = 0; q < LP; ++q) {
+      const int j = j0 + q;
+      if (j < (int)kcols) {
+        const double proj = root * (worker[w * K + j] + firm[f * K + j]);
+        const double res = (double)atoms[g * K + j] / root - proj;
+        if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
+        const double ps = proj * proj, rs = res * res;
+        v[0] += ps;
+        v[1] += rs;
+        v[2] += ps * ps;
+        v[3] += rs * rs;
+        v[4] += ps * rs;
+      }
     }
 #pragma unroll
-    for (int off = 1; off < K; off <<= 1) {
+    for (int off = 1; off < TPR; off <<= 1) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) v[q] += __shfl_xor(v[q], off);
     }
-    if (j == 0) {
+    if (j0 == 0) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         double s = m[(uint64_t)(2 * q) * units + g], e = m[(uint64_t)(2 * q + 1) * units + g];
