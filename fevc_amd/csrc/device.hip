@@ -842,12 +842,21 @@ double DeviceEngine::stream_copy(uint64_t bytes) {
 DeviceEngine::KernelTiming DeviceEngine::dominant_kernel_timing() const {
   const Impl &E = *impl_;
   KernelTiming t;
-  double total = 0.0;
-  for (auto &e : E.events) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) total += ms;
+  // Launches queued after every lane converged return at the device gate
+  // without touching the level (a few microseconds); only launches that did
+  // the pass (longer than a quarter of the longest) are averaged.
+  std::vector<float> ms(E.events.size(), 0.f);
+  float longest = 0.f;
+  for (size_t k = 0; k < E.events.size(); ++k) {
+    if (hipEventElapsedTime(&ms[k], E.events[k].first, E.events[k].second) != hipSuccess) ms[k] = 0.f;
+    longest = std::max(longest, ms[k]);
   }
-  t.launches = E.events.size();
+  double total = 0.0;
+  for (float v : ms)
+    if (v > 0.25f * longest) {
+      total += v;
+      t.launches++;
+    }
   t.ms_per_launch = t.launches ? total / (double)t.launches : 0.0;
   t.bytes_per_launch = E.timed_bytes;
   return t;

@@ -78,7 +78,8 @@ class DeviceEngine {
   // V-cycles applied, solve batches, refinement rounds, refined columns,
   // PCG iterations launched (including ones queued after convergence).
   void counters(uint64_t out[6]) const;
-  // Average duration of the dominant kernel over a timed run.
+  // Average duration of the dominant kernel over the launches of a timed run
+  // that did work (launches returning at the convergence gate excluded).
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
   KernelTiming dominant_kernel_timing() const;
 
