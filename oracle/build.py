@@ -12,7 +12,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["rng.c", "sample.c", "cmg.c", "engine.c"]
+SOURCES = ["rng.c", "sample.c", "cmg.c", "engine.c", "numerical.c"]
 OUTPUT = os.path.join(HERE, "_build", "liborc.so")
 
 

@@ -652,9 +652,128 @@ static int target_columns(const orc_problem *p, const int64_t *atoms, uint64_t p
     return rc;
 }
 
+/* numerical attachment state (engine/numerical.rs:49-122) */
+typedef struct {
+    double *grad;  /* 5 per unit */
+    double *offs;  /* per unit */
+    ssum *folds;   /* 6 per unit: [fold][sw^2, sf^2, sw sf] */
+    uint32_t status;
+} num_state;
+
+static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
+                      orc_numerical_result *num);
+
 int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out) {
+    return engine_run(p, opt, out, NULL);
+}
+
+int orc_engine_run_numerical(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
+                             orc_numerical_result *num) {
+    memset(num, 0, sizeof *num);
+    return engine_run(p, opt, out, num);
+}
+
+/* engine/numerical.rs:160-330 (replay): the leverage probes are solved again
+ * in batches of at most 4; every probe's fold scores are
+ * -sum_g folds[g][fold][j] * (gradient_g . u_g - offset_g). */
+static int numerical_replay(const orc_problem *p, const orc_engine_options *opt, const solver_t *s,
+                            num_state *ns, const double *draws, orc_numerical_result *num) {
+    uint64_t G = p->units, W = p->workers, F = p->firms, R = opt->probes;
+    double *prim = orc_alloc(3 * R, 8);
+    for (uint64_t t = 0; t < R; ++t)
+        for (int j = 0; j < 3; ++j) prim[3 * t + j] = draws[4 * t + j];
+    double cond[9];
+    int rc = orc_cross_covariance(prim, prim, R, cond);
+    free(prim);
+    if (rc) return rc;
+    uint32_t status = ns->status;
+    double *scores = orc_alloc(6 * R, 8); /* [fold][probe][3] */
+    uint64_t batch = opt->leverage_width < 1 ? 1 : (opt->leverage_width > 4 ? 4 : opt->leverage_width);
+    int64_t *atoms = orc_alloc(G * batch, 8);
+    double *wr = orc_alloc(W * batch, 8), *fr = orc_alloc(F * batch, 8);
+    column_t *cols = alloc_cols(p, batch);
+    num->failed_replay_probe = UINT32_MAX;
+    if (status == 2) {
+        for (uint64_t first = 0; first < R; first += batch) {
+            uint64_t width = R - first < batch ? R - first : batch;
+            if ((rc = orc_rademacher_fill(opt->seed, ORC_TAG_LEVERAGE, first, width, p->unit_rank,
+                                          p->unit_count, G, atoms)))
+                goto out;
+            for (uint64_t g = 0; g < G; ++g)
+                num->replay_generator_words += (p->unit_count[g] + 63) / 64 * width;
+            for (uint64_t c = 0; c < width; ++c) leverage_rhs(p, atoms + c * G, wr + c * W, fr + c * F);
+            num->attempted_replay_rhs += width;
+            rc = solve_batch(s, wr, fr, width, opt->probe_tolerance, 1.0, opt, cols);
+            if (rc == ORC_PCG_MAX_ITERATIONS || rc == 62 || rc == ORC_PCG_CURVATURE ||
+                rc == ORC_PCG_PRECONDITIONER || rc == ORC_FULL_RESIDUAL) {
+                status = 7;
+                num->failed_replay_probe = (uint32_t)first;
+                num->replay_error_code = (uint32_t)rc;
+                rc = ORC_OK;
+                break;
+            }
+            if (rc) goto out;
+            for (uint64_t c = 0; c < width; ++c) {
+                uint64_t probe = first + c;
+                num->certified_replay_rhs++;
+                num->max_replay_complete_residual = fmax(num->max_replay_complete_residual, cols[c].complete);
+                ssum sc[2][3] = {{{0, 0}, {0, 0}, {0, 0}}, {{0, 0}, {0, 0}, {0, 0}}};
+                for (uint64_t g = 0; g < G; ++g) {
+                    uint32_t cell = p->unit_cell[g];
+                    double root = sqrt((double)p->unit_count[g]);
+                    double pp = root * (cols[c].worker[p->cell_worker[cell]] + cols[c].firm[p->cell_firm[cell]]);
+                    double m = (double)atoms[c * G + g] / root - pp;
+                    double p2 = pp * pp, m2 = m * m;
+                    double u[5] = {p2, m2, p2 * p2, m2 * m2, p2 * m2};
+                    ssum alpha = {0, 0};
+                    ss_add(&alpha, -ns->offs[g]);
+                    for (int j = 0; j < 5; ++j) ss_add(&alpha, ns->grad[5 * g + j] * u[j]);
+                    double al = ss_fin(alpha);
+                    for (int f = 0; f < 2; ++f)
+                        for (int j = 0; j < 3; ++j) ss_add(&sc[f][j], -ss_fin(ns->folds[6 * g + 3 * f + j]) * al);
+                }
+                for (int f = 0; f < 2; ++f)
+                    for (int j = 0; j < 3; ++j) scores[(f * R + probe) * 3 + j] = ss_fin(sc[f][j]);
+            }
+        }
+    }
+    for (int f = 0; f < 2; ++f) {
+        ssum m[3] = {{0, 0}, {0, 0}, {0, 0}};
+        for (uint64_t t = 0; t < R; ++t)
+            for (int j = 0; j < 3; ++j) ss_add(&m[j], scores[(f * R + t) * 3 + j] / (double)R);
+        for (int j = 0; j < 3; ++j) num->score_means[3 * f + j] = ss_fin(m[j]);
+    }
+    double lev[9];
+    if (status == 2) {
+        if (orc_cross_covariance(scores, scores + 3 * R, R, lev) != ORC_OK) {
+            status = 6;
+            for (int k = 0; k < 9; ++k) lev[k] = NAN;
+        }
+    } else {
+        for (int k = 0; k < 9; ++k) lev[k] = NAN;
+    }
+    uint32_t fin_status;
+    orc_numerical_finalize(cond, lev, num->raw, num->usable, num->mcse, &fin_status, &num->psd_adjustment);
+    memcpy(num->conditional, cond, sizeof cond);
+    memcpy(num->leverage, lev, sizeof lev);
+    num->status = status != 2 ? status : fin_status;
+    num->leverage_probes = num->target_probes = (uint32_t)R;
+    num->fold_a = (uint32_t)((R + 1) / 2);
+    num->fold_b = (uint32_t)(R / 2);
+out:
+    free(scores);
+    free(atoms);
+    free(wr);
+    free(fr);
+    free_cols(cols, batch);
+    return rc;
+}
+
+static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
+                      orc_numerical_result *num) {
     if (opt.probes < 2)
         return orc_fail(ORC_INVALID_INPUT, "jla_validate", "JLA probe count must be at least two");
+    num_state ns = {NULL, NULL, NULL, 0};
     solver_t s;
     int rc = solver_init(&s, p);
     if (rc) {
@@ -732,6 +851,7 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
         }
     }
     /* leverage adjustment: engine.rs:2269-2410 (outcome mean 0, centering None) */
+    int clipped = 0;
     {
         double probes = (double)R;
         res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
@@ -758,6 +878,7 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
                 rc = orc_fail(ORC_JLA_MOMENT, "jla_leverage", "finite-projection variance negative");
                 goto fail_mom;
             }
+            if (variance < 0.0) clipped = 1;
             variance = fmax(variance, 0.0);
             if (1.0 - projection <= opt.block_tolerance) {
                 free(cw);
@@ -786,6 +907,43 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
         res->cell_weight = orc_alloc(C, 8);
         for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);
         free(cw);
+    }
+    /* numerical state: engine/numerical.rs:62-122 (after the adjustment) */
+    if (num) {
+        ns.grad = orc_alloc(5 * G, 8);
+        ns.offs = orc_alloc(G, 8);
+        ns.folds = orc_alloc(6 * G, sizeof(ssum));
+        ns.status = clipped ? 5 : 2;
+        num->minimum_constrained = INFINITY;
+        num->minimum_residual_margin = INFINITY;
+        num->maximum_sensitivity_ratio = 0.0;
+        double r = (double)R;
+        for (uint64_t g = 0; g < G; ++g) {
+            double u[5] = {ss_fin(mom[g].proj) / r, ss_fin(mom[g].res) / r, ss_fin(mom[g].proj4) / r,
+                           ss_fin(mom[g].res4) / r, ss_fin(mom[g].mixed) / r};
+            double fd[20], gr[5];
+            if (!orc_finite_derivative(u, (uint32_t)R, fd)) {
+                ns.status = 6;
+                continue;
+            }
+            num->minimum_constrained = fmin(num->minimum_constrained, u[0] + u[1]);
+            num->minimum_residual_margin = fmin(num->minimum_residual_margin, fd[1]);
+            num->maximum_sensitivity_ratio = fmax(num->maximum_sensitivity_ratio, sqrt(fmax(fd[3], 0.0)) / fd[1]);
+            if (fd[19] != 0.0) {
+                ns.status = 5;
+                continue;
+            }
+            if (!orc_fd_observation(fd, 0.0, gr)) {
+                ns.status = 6;
+                continue;
+            }
+            ssum off = {0, 0};
+            for (int j = 0; j < 5; ++j) {
+                ns.grad[5 * g + j] = gr[j];
+                ss_add(&off, gr[j] * u[j]);
+            }
+            ns.offs[g] = ss_fin(off);
+        }
     }
     free(mom);
     mom = NULL;
@@ -821,6 +979,20 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
             d[1] = ss_fin(fs);
             d[2] = ss_fin(cv);
             d[3] = d[0] + d[1] + 2.0 * d[2];
+            /* engine/numerical.rs:124-157 (target folds) */
+            if (num) {
+                uint64_t probe = first + c;
+                int fold = (int)(probe % 2);
+                double count = fold == 0 ? (double)((R + 1) / 2) : (double)(R / 2);
+                for (uint64_t g = 0; g < G; ++g) {
+                    uint32_t cell = p->unit_cell[g];
+                    uint32_t w = p->cell_worker[cell], f = p->cell_firm[cell];
+                    double sw = a->worker[w] + a->firm[f], sf = b->worker[w] + b->firm[f];
+                    double mass = (p->unit_outcome[g] - 0.0 * (double)p->unit_count[g]) * res->residual_mass[g] / count;
+                    double kappa[3] = {sw * sw, sf * sf, sw * sf};
+                    for (int j = 0; j < 3; ++j) ss_add(&ns.folds[6 * g + 3 * fold + j], mass * kappa[j]);
+                }
+            }
         }
     }
     /* finalize: engine.rs:3108-3272 */
@@ -841,6 +1013,14 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
         for (int j = 0; j < 4; ++j) res->sum.mcse[j] = sqrt(ss_fin(v[j]) / denom);
     }
     res->sum.max_complete_residual = max_complete;
+    if (num) {
+        num->point_rhs = 1 + 3 * R;
+        num->max_point_complete_residual = max_complete;
+        rc = numerical_replay(p, &opt, &s, &ns, res->draws, num);
+        free(ns.grad); free(ns.offs); free(ns.folds);
+        ns.grad = NULL; ns.offs = NULL; ns.folds = NULL;
+        if (rc) goto fail;
+    }
     res->sum.solve_seconds = now_seconds() - t0;
     free(atoms);
     free_cols(cols, 2 * maxw);
@@ -852,6 +1032,9 @@ int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_resu
 fail_mom:
     free(mom);
 fail:
+    free(ns.grad);
+    free(ns.offs);
+    free(ns.folds);
     free(atoms);
     free_cols(cols, 2 * maxw);
     free(wr);

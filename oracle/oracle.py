@@ -68,6 +68,66 @@ class CmgOptions(C.Structure):
                 ("max_hierarchy_nnz_factor", C.c_double), ("low_degree_threshold", C.c_double)]
 
 
+class NumericalResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("leverage_probes", C.c_uint32), ("target_probes", C.c_uint32),
+                ("fold_a", C.c_uint32), ("fold_b", C.c_uint32), ("failed_replay_probe", C.c_uint32),
+                ("replay_error_code", C.c_uint32), ("reserved", C.c_uint32),
+                ("certified_replay_rhs", C.c_uint64), ("attempted_replay_rhs", C.c_uint64),
+                ("replay_generator_words", C.c_uint64), ("point_rhs", C.c_uint64),
+                ("psd_adjustment", C.c_double), ("minimum_constrained", C.c_double),
+                ("minimum_residual_margin", C.c_double), ("maximum_sensitivity_ratio", C.c_double),
+                ("score_means", C.c_double * 6), ("conditional", C.c_double * 9),
+                ("leverage", C.c_double * 9), ("raw", C.c_double * 9), ("usable", C.c_double * 9),
+                ("mcse", C.c_double * 4), ("max_point_complete_residual", C.c_double),
+                ("max_replay_complete_residual", C.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = np.array(list(v)) if hasattr(v, "__len__") else v
+        for k in ("conditional", "leverage", "raw", "usable"):
+            out[k] = out[k].reshape(3, 3)
+        out["score_means"] = out["score_means"].reshape(2, 3)
+        return out
+
+
+def finite_derivative(u, r):
+    """numerical_mc.rs FiniteDerivative::new: dict or None."""
+    out = np.zeros(20)
+    if not lib().orc_finite_derivative(np.ascontiguousarray(u, np.float64).ctypes.data, r, out.ctypes.data):
+        return None
+    return dict(h=out[0], m=out[1], bias=out[2], variance=out[3], dh=out[4:9], db=out[9:14], dv=out[14:19],
+                nonsmooth=bool(out[19]), raw=out)
+
+
+def fd_observation(fd, control=0.0):
+    g = np.zeros(5)
+    if not lib().orc_fd_observation(fd["raw"].ctypes.data, control, g.ctypes.data):
+        return None
+    return g
+
+
+def cross_covariance(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros(9)
+    _check(lib().orc_cross_covariance(a.ctypes.data, b.ctypes.data, len(a), out.ctypes.data))
+    return out.reshape(3, 3)
+
+
+def numerical_finalize(conditional, leverage):
+    raw, usable, mcse = np.zeros(9), np.zeros(9), np.zeros(4)
+    status = C.c_uint32()
+    psd = C.c_double()
+    c = np.ascontiguousarray(conditional, np.float64).ravel()
+    lv = np.ascontiguousarray(leverage, np.float64).ravel()
+    lib().orc_numerical_finalize(c.ctypes.data, lv.ctypes.data, raw.ctypes.data, usable.ctypes.data,
+                                 mcse.ctypes.data, C.byref(status), C.byref(psd))
+    return dict(raw=raw.reshape(3, 3), usable=usable.reshape(3, 3), mcse=mcse, status=int(status.value),
+                psd_adjustment=psd.value)
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -113,6 +173,12 @@ def lib():
         L.orc_engine_result_free.argtypes = [C.c_void_p]
         L.orc_engine_result_summary.argtypes = [C.c_void_p, C.POINTER(EngineSummary)]
         L.orc_engine_result_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]
+        L.orc_engine_run_numerical.argtypes = [C.c_void_p, EngineOptions, C.POINTER(C.c_void_p),
+                                               C.POINTER(NumericalResult)]
+        L.orc_finite_derivative.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_fd_observation.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.orc_cross_covariance.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.orc_numerical_finalize.argtypes = [C.c_void_p] * 5 + [C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_time_probe_solves.argtypes = [C.c_void_p, EngineOptions, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.orc_frozen_digest.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)]
@@ -310,12 +376,16 @@ class Problem:
 
     def run(self, seed=8675309, probes=200, leverage_width=8, target_width=8, threads=1,
             fit_tolerance=1e-10, probe_tolerance=1e-6, max_iterations=10000,
-            recompute_interval=100, rank_tolerance=1e-10, block_tolerance=1e-10):
+            recompute_interval=100, rank_tolerance=1e-10, block_tolerance=1e-10, numerical=False):
         opt = EngineOptions(seed, probes, leverage_width, target_width, threads, fit_tolerance,
                             probe_tolerance, max_iterations, recompute_interval, rank_tolerance,
                             block_tolerance)
         h = C.c_void_p()
-        _check(lib().orc_engine_run(self._h, opt, C.byref(h)))
+        num = NumericalResult() if numerical else None
+        if numerical:
+            _check(lib().orc_engine_run_numerical(self._h, opt, C.byref(h), C.byref(num)))
+        else:
+            _check(lib().orc_engine_run(self._h, opt, C.byref(h)))
         try:
             s = EngineSummary()
             lib().orc_engine_result_summary(h.value, C.byref(s))
@@ -333,6 +403,8 @@ class Problem:
                 a = np.zeros(n)
                 _check(lib().orc_engine_result_array(h.value, i, a.ctypes.data, n))
                 res[name] = a.reshape(probes, 4) if name == "target_draws" else a
+            if numerical:
+                res["numerical"] = num.as_dict()
             return res
         finally:
             lib().orc_engine_result_free(h.value)

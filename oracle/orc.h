@@ -16,6 +16,9 @@
  *     closed form.
  *   - Leave-out selection: brute force (articulation and bridge deletion
  *     plus BFS) on small panels.
+ *   - Numerical MCSE: finite differences of the linearised functions, the
+ *     covariance-policy cases and fold enumeration of
+ *     rust/crates/vckss-core/tests/numerical_mc.rs, eigendecomposition.
  * Parity unpinned: the CMG_FULL_V2 refinement ladder and the harmonic lift
  * of auxiliary vertices (full_cmg.rs:970-1088, cmg_impl.rs.in:459-500).  No
  * reference vector makes a complete-residual gate fail, so these are
@@ -198,6 +201,29 @@ void orc_engine_result_summary(const orc_engine_result *r,
  *        9 leverage_iterations(per probe, as double) */
 int orc_engine_result_array(const orc_engine_result *r, int which, double *dst,
                             uint64_t capacity);
+
+/* ---- numerical MCSE attachment (engine/numerical.rs, numerical_mc.rs) --- */
+typedef struct {
+    /* 1 exact_zero, 2 ok_local, 3 ok_local_psd_adjusted, 4 unstable_nonpsd,
+     * 5 nonsmooth_adjustment, 6 nonfinite_derivative, 7 replay_failed */
+    uint32_t status, leverage_probes, target_probes, fold_a, fold_b;
+    uint32_t failed_replay_probe, replay_error_code, reserved;
+    uint64_t certified_replay_rhs, attempted_replay_rhs, replay_generator_words, point_rhs;
+    double psd_adjustment, minimum_constrained, minimum_residual_margin, maximum_sensitivity_ratio;
+    double score_means[6]; /* [fold][worker, firm, covariance] */
+    double conditional[9], leverage[9], raw[9], usable[9], mcse[4];
+    double max_point_complete_residual, max_replay_complete_residual;
+} orc_numerical_result;
+
+/* The engine run with the numerical attachment (engine.rs:1262-1545). */
+int orc_engine_run_numerical(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
+                             orc_numerical_result *num);
+/* numerical_mc.rs pieces, exposed for the pins. */
+int orc_finite_derivative(const double u[5], uint32_t r, double out[20]);
+int orc_fd_observation(const double fd[20], double control, double grad[5]);
+int orc_cross_covariance(const double *a, const double *b, uint64_t n, double out[9]);
+void orc_numerical_finalize(const double cond[9], const double lev[9], double raw[9], double usable[9],
+                            double mcse[4], uint32_t *status, double *psd_adjustment);
 
 /* Timed probe solves on an existing problem: solves `count` leverage probes
  * (first probe `first`) with `threads` host threads; returns seconds. */
