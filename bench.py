@@ -173,6 +173,15 @@ def main():
         kss = {"wall_seconds": round(kw, 3), "probes": args.kss_probes,
                "corrected": res["corrected"], "numerical_mcse": res["numerical_mcse"]}
         log(f"KSS estimate ({args.kss_probes} probes): {kw:.2f}s, corrected {res['corrected']}")
+        # the command's default adds the numerical MCSE (replayed leverage probes)
+        t0 = time.perf_counter()
+        _, num = eng.solve_numerical(probes=args.kss_probes, tolerance=args.tolerance,
+                                     fit_tolerance=args.tolerance)
+        nw = time.perf_counter() - t0
+        kss["numerical_mcse_wall_seconds"] = round(nw, 3)
+        kss["numerical_mcse"] = [float(v) for v in num["mcse"]]
+        kss["numerical_mcse_status"] = int(num["status"])
+        log(f"KSS estimate with numerical MCSE replay: {nw:.2f}s, status {num['status']}, mcse {list(num['mcse'])}")
         if args.compare_plain:
             # the same estimate on the plain single-RHS FP64 path, identical draws
             eng.configure(64, 1, 1)

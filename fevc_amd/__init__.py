@@ -83,6 +83,56 @@ class SolveRequestV5(C.Structure):
         ("reserved_5", C.c_uint32)]
 
 
+class SolveRequestV4(C.Structure):
+    """The V4 point prefix of the solve request (the numerical requests embed it)."""
+    _fields_ = SolveRequestV5._fields_[:-4]
+
+
+class NumericalRequestV1(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("struct_size", C.c_uint32), ("schema_version", C.c_uint32),
+                ("threads", C.c_uint32), ("tolerance_supplied", C.c_uint32),
+                ("projection_requested", C.c_uint32), ("component_requested", C.c_uint32),
+                ("reserved", C.c_uint32), ("controls_count", C.c_uint32), ("reserved_2", C.c_uint32),
+                ("point", SolveRequestV4)]
+
+
+class NumericalRequestV2(C.Structure):
+    _fields_ = [("v1", NumericalRequestV1), ("execution_mode", C.c_uint32),
+                ("component_batch_mode", C.c_uint32), ("full_cmg", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class NumericalResultV1(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("schema_version", C.c_uint32), ("generation", C.c_uint64),
+                ("status", C.c_uint32), ("engine", C.c_uint32), ("leverage_probes", C.c_uint32),
+                ("target_probes", C.c_uint32), ("target_fold_a", C.c_uint32), ("target_fold_b", C.c_uint32),
+                ("certified_replay_rhs", C.c_uint64), ("executed_replay_rhs", C.c_uint64),
+                ("attempted_replay_rhs", C.c_uint64), ("replay_generator_words", C.c_uint64),
+                ("allocation_bound_bytes", C.c_uint64), ("failed_replay_probe", C.c_uint32),
+                ("replay_error_code", C.c_uint32), ("psd_adjustment", C.c_double),
+                ("minimum_constrained", C.c_double), ("minimum_residual_margin", C.c_double),
+                ("maximum_sensitivity_ratio", C.c_double), ("score_means", C.c_double * 6),
+                ("conditional", C.c_double * 9), ("leverage", C.c_double * 9), ("raw", C.c_double * 9),
+                ("usable", C.c_double * 9), ("mcse", C.c_double * 4), ("point_rhs", C.c_uint64),
+                ("maximum_point_complete_residual", C.c_double),
+                ("maximum_replay_complete_residual", C.c_double)]
+
+
+class NumericalRhsV1(C.Structure):
+    _fields_ = [("schema_version", C.c_uint32), ("phase", C.c_uint32), ("probe", C.c_uint32),
+                ("reserved", C.c_uint32), ("complete_residual", C.c_double)]
+
+
+class FullCmgModelReceiptV1(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("schema_version", C.c_uint32), ("generation", C.c_uint64),
+                ("controls_count", C.c_uint32), ("nuisance_mode", C.c_uint32),
+                ("logical_rhs_count", C.c_uint64), ("explicit_options_rhs_count", C.c_uint64),
+                ("controlled_rhs_count", C.c_uint64), ("control_refinement_rhs_count", C.c_uint64)]
+
+
+assert C.sizeof(NumericalRequestV1) == 328 and C.sizeof(NumericalRequestV2) == 344
+assert C.sizeof(NumericalResultV1) == 512 and C.sizeof(NumericalRhsV1) == 24
+
+
 class SolveRequestInterruptV5(C.Structure):
     _fields_ = [("options", SolveRequestV5), ("interrupt_poll", C.c_void_p),
                 ("interrupt_context", C.c_void_p), ("checkpoint_interval", C.c_uint32),
@@ -146,6 +196,19 @@ SIGNATURES = [
     ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("vckss_mi355x_counters_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_set_chunk_v1", C.c_int32, [C.c_uint64, C.c_uint32]),
+    ("vckss_rust_numerical_schema_v1", C.c_uint32, []),
+    ("vckss_rust_numerical_schema_v2", C.c_uint32, []),
+    ("vckss_rust_engine_numerical_preflight_v1", C.c_int32, [C.c_void_p]),
+    ("vckss_rust_engine_numerical_preflight_v2", C.c_int32, [C.c_void_p]),
+    ("vckss_rust_engine_solve_numerical_v1", C.c_int32, [C.c_uint64, C.c_void_p]),
+    ("vckss_rust_engine_solve_numerical_interrupt_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_solve_numerical_interrupt_v2", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_numerical_result_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_numerical_rhs_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("vckss_rust_engine_numerical_cmg_work_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -204,6 +267,27 @@ def default_prepare_request(rows: int) -> PrepareRequestInterruptV3:
     _check(library().vckss_rust_engine_default_prepare_request_interrupt_v3(C.byref(req), C.sizeof(req)))
     req.options.rows = rows
     return req
+
+
+def default_numerical_request(seed=8675309, probes=200, tolerance=None, fit_tolerance=1e-10,
+                              maximum_iterations=10000, residual_replacement_interval=100):
+    """A V2 numerical MCSE request on the CMG_FULL_V2 path (numerical_api.rs:199-235): the
+    default solve request's V4 prefix as the point, JLA, full CMG."""
+    base = default_solve_request().options
+    r = NumericalRequestV2()
+    C.memmove(C.byref(r.v1.point), C.byref(base), C.sizeof(SolveRequestV4))
+    p = r.v1.point
+    p.struct_size = C.sizeof(SolveRequestV4)
+    p.algorithm = 2
+    p.seed, p.probes, p.pcg_tolerance = seed, probes, fit_tolerance
+    p.maximum_iterations, p.residual_replacement_interval = maximum_iterations, residual_replacement_interval
+    v1 = r.v1
+    v1.abi_version, v1.struct_size, v1.schema_version, v1.threads = 1, C.sizeof(r), 2, 1
+    if tolerance is not None:
+        p.pcg_tolerance = tolerance
+        v1.tolerance_supplied = 1
+    r.full_cmg = 1
+    return r
 
 
 def default_solve_request() -> SolveRequestInterruptV5:
@@ -277,6 +361,41 @@ class Engine:
             req.interrupt_poll = C.cast(cb, C.c_void_p)
         _check(library().vckss_rust_engine_solve_interrupt_v5(self.generation, C.byref(req)))
         return self.result()
+
+    def numerical_request(self, **kw):
+        return default_numerical_request(**kw)
+
+    def solve_numerical(self, request=None, **kw):
+        """Point estimate plus the numerical MCSE attachment; returns (result, numerical)."""
+        r = request if request is not None else self.numerical_request(**kw)
+        _check(library().vckss_rust_engine_solve_numerical_interrupt_v2(self.generation, C.byref(r), None, None, 0))
+        return self.result(), self.numerical_result()
+
+    def numerical_result(self):
+        o = NumericalResultV1()
+        _check(library().vckss_rust_engine_numerical_result_v1(self.generation, C.byref(o), C.sizeof(o)))
+        out = {}
+        for name, _ in o._fields_:
+            v = getattr(o, name)
+            out[name] = np.array(list(v)) if hasattr(v, "__len__") else v
+        for k in ("conditional", "leverage", "raw", "usable"):
+            out[k] = out[k].reshape(3, 3)
+        out["score_means"] = out["score_means"].reshape(2, 3)
+        return out
+
+    def numerical_rhs(self):
+        """(phase, probe, complete residual) of every certified replay RHS; the
+        destination must hold them all (numerical_api.rs:541-608)."""
+        cap = self.numerical_result()["certified_replay_rhs"]
+        n = C.c_uint64()
+        out = (NumericalRhsV1 * max(cap, 1))()
+        _check(library().vckss_rust_engine_numerical_rhs_v1(self.generation, out, cap, C.byref(n)))
+        return [(r.phase, r.probe, r.complete_residual) for r in out[:n.value]]
+
+    def numerical_cmg_work(self):
+        o = FullCmgModelReceiptV1()
+        _check(library().vckss_rust_engine_numerical_cmg_work_v1(self.generation, C.byref(o), C.sizeof(o)))
+        return {name: getattr(o, name) for name, _ in o._fields_}
 
     def result(self):
         r = ResultV1()

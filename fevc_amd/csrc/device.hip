@@ -10,6 +10,7 @@
 // a contiguous segmented sum, prolongation labels are monotone); the level-0
 // renumbering is undone at the firm/auxiliary boundary of the hybrid system.
 #include "device_impl.hpp"
+#include "numerical.hpp"
 
 namespace vk {
 namespace {
@@ -368,6 +369,102 @@ double component_identity(const double c[4]) {
 }
 
 // Host permutation of K-lane vectors between original and renumbered order.
+// Numerical MCSE replay (engine/numerical.rs:160-330): the leverage probes
+// are solved again (same Counter-V1 draws, same batches as the leverage
+// phase) and every probe is scored against the unit linearisation and the
+// target folds; the scores' fold cross-covariance plus the target draws'
+// covariance give the MCSE.  A replay solve that fails its PCG or complete
+// residual gate stops the replay with status 7.
+void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
+  NumericalResult &num = res.numerical;
+  const uint64_t R = opt.probes;
+  const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
+  const uint32_t KL = lanes_for(lw);
+  num.leverage_probes = num.target_probes = (uint32_t)R;
+  num.fold_a = (uint32_t)((R + 1) / 2);
+  num.fold_b = (uint32_t)(R / 2);
+  num.point_rhs = 1 + 3 * R;
+  num.maximum_point_complete_residual = res.max_complete_residual;
+  std::vector<double> draws(3 * R), scores(6 * R, 0.0);
+  for (uint64_t t = 0; t < R; ++t)
+    for (int j = 0; j < 3; ++j) draws[3 * t + j] = res.draws[4 * t + j];
+  Matrix3 conditional;
+  if (!cross_covariance(draws, draws, conditional)) fail(kInvalidInput, "numerical_mc", "nonfinite target draws");
+  uint32_t status = num.status;
+  if (status == 2) {
+    E.nscores.alloc(6 * 64);
+    for (uint64_t first = 0; first < R; first += lw) {
+      const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
+      hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
+                         E.unit_count.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
+      std::vector<ColumnReceipt> rc;
+      bool failed = false;
+      with_k(KL, [&](auto kk) {
+        constexpr int K = decltype(kk)::value;
+        hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W,
+                           E.worker_ptr.p, (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p,
+                           E.G, k, E.wrhs.p);
+        segments<double, K, 0>(E, E.firm_chunks,
+                               dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G,
+                                                   k, E.frhs.p});
+        num.attempted_replay_rhs += k;
+        try {
+          solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
+        } catch (const Error &e) {
+          if (e.code < kPcgCurvature || e.code > kFullResidual) throw;
+          failed = true;
+          num.failed_replay_probe = (uint32_t)first;
+          num.replay_error_code = (uint32_t)e.code;
+          return;
+        }
+        const unsigned g = reduce_grid(E.G * K);
+        hipLaunchKernelGGL((dev::k_numerical_scores<K>), g, kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+                           E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p, E.ngrad.p,
+                           E.noffs.p, E.nfolds.p, E.rpartial.p);
+        finish<K, 6>(E, E.rpartial.p, g, dev::FinScores{E.nscores.p});
+      });
+      for (uint64_t g = 0; g < E.G; ++g) num.replay_generator_words += (E.P->unit_count[g] + 63) / 64 * k;
+      if (failed) {
+        status = 7;
+        break;
+      }
+      std::vector<double> s;
+      E.nscores.download(s, 6 * (size_t)KL, E.stream);
+      for (uint32_t c = 0; c < k; ++c) {
+        const uint64_t probe = first + c;
+        for (int f = 0; f < 2; ++f)
+          for (int j = 0; j < 3; ++j) scores[(f * R + probe) * 3 + j] = s[c * 6 + f * 3 + j];
+        num.replay_probes.push_back((uint32_t)probe);
+        num.replay_complete.push_back(rc[c].complete);
+        num.maximum_replay_complete_residual = std::max(num.maximum_replay_complete_residual, rc[c].complete);
+      }
+      num.certified_replay_rhs += k;
+    }
+  }
+  for (int f = 0; f < 2; ++f) {
+    const std::vector<double> fold(scores.begin() + f * 3 * R, scores.begin() + (f + 1) * 3 * R);
+    const auto m = score_means(fold);
+    for (int j = 0; j < 3; ++j) num.score_means[f][j] = m[j];
+  }
+  Matrix3 leverage;
+  leverage.fill(NAN);
+  if (status == 2) {
+    const std::vector<double> a(scores.begin(), scores.begin() + 3 * R), b(scores.begin() + 3 * R, scores.end());
+    if (!cross_covariance(a, b, leverage)) {
+      status = 6;
+      leverage.fill(NAN);
+    }
+  }
+  const Covariance cov = finalize_covariance(conditional, leverage);
+  std::copy(cov.conditional.begin(), cov.conditional.end(), num.conditional);
+  std::copy(cov.leverage.begin(), cov.leverage.end(), num.leverage);
+  std::copy(cov.raw.begin(), cov.raw.end(), num.raw);
+  std::copy(cov.usable.begin(), cov.usable.end(), num.usable);
+  std::copy(cov.mcse.begin(), cov.mcse.end(), num.mcse);
+  num.psd_adjustment = cov.psd_adjustment;
+  num.status = status != 2 ? status : cov.status;
+}
+
 // Device layout of the hierarchy and the firm passes for chunks of at most
 // `chunk` entries (renumbering, CSR, labels, chunk tables).
 void relayout(Impl &E, uint32_t chunk) {
@@ -571,6 +668,37 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     E.cellw.download(res.cell_weight, E.C, E.stream);
   }
 
+  // ---- numerical attachment state (engine/numerical.rs:62-122)
+  NumericalResult &num = res.numerical;
+  if (opt.numerical) {
+    res.has_numerical = true;
+    E.ngrad.alloc(5 * E.G);
+    E.noffs.alloc(E.G);
+    E.nfolds.alloc(12 * E.G);
+    VK_HIP(hipMemsetAsync(E.nfolds.p, 0, 12 * E.G * sizeof(double), E.stream));
+    const unsigned sg = reduce_grid(E.G);
+    E.npart.alloc(5 * (size_t)sg);
+    hipLaunchKernelGGL(dev::k_numerical_state, sg, kBlock, 0, E.stream, E.G, E.moments.p, (double)R, E.ngrad.p,
+                       E.noffs.p, E.npart.p);
+    std::vector<double> part;
+    E.npart.download(part, 5 * (size_t)sg, E.stream);
+    num.minimum_constrained = INFINITY;
+    num.minimum_residual_margin = INFINITY;
+    num.maximum_sensitivity_ratio = 0.0;
+    double last = -1.0;
+    bool clipped = false;
+    for (unsigned b = 0; b < sg; ++b) {
+      num.minimum_constrained = std::min(num.minimum_constrained, part[5 * b]);
+      num.minimum_residual_margin = std::min(num.minimum_residual_margin, part[5 * b + 1]);
+      num.maximum_sensitivity_ratio = std::max(num.maximum_sensitivity_ratio, part[5 * b + 2]);
+      last = std::max(last, part[5 * b + 3]);
+      clipped = clipped || part[5 * b + 4] != 0.0;
+    }
+    // the last unit with an event sets the status (numerical.rs:88-120)
+    num.status = last >= 0.0 ? (uint32_t)((uint64_t)last % 8) : (clipped ? 5u : 2u);
+    num.allocation_bytes = (5 + 1 + 12) * E.G * sizeof(double) + 6 * R * sizeof(double);
+  }
+
   // ---- target probes (engine.rs:1373-1484)
   auto tt = std::chrono::steady_clock::now();
   E.dir.alloc((uint64_t)KT * E.C);
@@ -601,6 +729,10 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bf, kBlock, 0, E.stream, E.F, E.frhs.p, E.rpartial.p);
       finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.status.p});
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
+      if (opt.numerical)
+        hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, first, R,
+                           E.unit_cell.p, E.cell_worker.p, E.cell_firm.p, E.unit_outcome.p, E.adjust.p + 4 * E.G,
+                           E.worker.p, E.fsol.p, E.nfolds.p);
       hipLaunchKernelGGL((dev::k_contract<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_worker.p, E.cell_firm.p,
                          E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p);
       finish<KTc, 3>(E, E.rpartial.p, cg, dev::FinDraws{E.draws.p});
@@ -639,6 +771,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   res.accounting_residual = acc;
   if (acc > 4096.0 * 2.220446049250313e-16)
     fail(kTargetIdentity, "jla_accounting", "component accounting residual exceeds the roundoff gate");
+  if (opt.numerical) numerical_replay(E, opt, res);
   res.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   res.vcycles = E.vcycles;
   return res;

@@ -24,6 +24,23 @@ struct SolveOptions {
   double rank_tolerance = 1e-10;
   double block_tolerance = 1e-10;
   int precision = 64;             // 64: FP64 V-cycle; 32: FP32 V-cycle, FP64 Krylov
+  bool numerical = false;         // numerical MCSE attachment (replayed leverage probes)
+};
+
+// Numerical MCSE attachment (engine/numerical.rs:13-32, numerical_api.rs:54-90).
+struct NumericalResult {
+  uint32_t status = 0;  // 1 exact zero .. 7 replay failed (numerical_api.rs:59-60)
+  uint32_t leverage_probes = 0, target_probes = 0, fold_a = 0, fold_b = 0;
+  uint64_t certified_replay_rhs = 0, attempted_replay_rhs = 0, replay_generator_words = 0,
+           allocation_bytes = 0, point_rhs = 0;
+  uint32_t failed_replay_probe = UINT32_MAX, replay_error_code = 0;
+  double psd_adjustment = 0, minimum_constrained = 0, minimum_residual_margin = 0,
+         maximum_sensitivity_ratio = 0;
+  double score_means[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  double conditional[9] = {}, leverage[9] = {}, raw[9] = {}, usable[9] = {}, mcse[4] = {};
+  double maximum_point_complete_residual = 0, maximum_replay_complete_residual = 0;
+  std::vector<uint32_t> replay_probes;        // certified replay right-hand sides, in order
+  std::vector<double> replay_complete;        // their complete residuals
 };
 
 struct ColumnReceipt {
@@ -41,6 +58,8 @@ struct SolveResult {
   ColumnReceipt fit;
   double seconds_total = 0.0, seconds_leverage = 0.0, seconds_target = 0.0;
   uint64_t vcycles = 0;
+  bool has_numerical = false;
+  NumericalResult numerical;
 };
 
 bool device_available();

@@ -218,6 +218,9 @@ struct DeviceEngine::Impl {
   DBuf<double> lane_a, lane_b, lane_c, reduced_res, complete_res, rhs_norm, sums;
   DBuf<int32_t> atoms;
   DBuf<double> moments, adjust, cellw, dir, totals, draws, fitted;
+  // numerical MCSE attachment: unit gradients (5 x G), offsets, fold sums and
+  // errors (12 x G), state partials, replay scores (6 per lane)
+  DBuf<double> ngrad, noffs, nfolds, npart, nscores;
   DBuf<int> status;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -27,7 +27,7 @@ namespace dev {
 
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 4096;
-constexpr int kMaxReduceValues = 4;
+constexpr int kMaxReduceValues = 6;
 // Block partials: value v of lane j from block slot b sits at
 // [(v * K + j) * kPartialStride + b], so the final per-lane sum reads
 // consecutive slots (pass-1 blocks, then combine blocks).

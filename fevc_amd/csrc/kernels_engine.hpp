@@ -739,6 +739,203 @@ static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, co
   }
 }
 
+// ------------------------------------------------ numerical MCSE attachment
+// engine/numerical.rs and numerical_mc.rs: a local linearisation of every
+// unit's leverage adjustment in its five moment means, fold sums of the
+// target solutions, and replayed leverage probes scored against both.
+
+// FiniteDerivative::new and observation(0) (numerical_mc.rs:54-125): the
+// gradient of 1/m + bias/m^2 - variance/m^3 in the five means.  Returns 0
+// (defined), 1 (no finite derivative), 5 (nonsmooth: variance < 0) or 2 (no
+// finite gradient); h, m and the variance are set unless it returns 1.
+__device__ inline int unit_linearisation(const double (&u)[5], double r, double (&grad)[5], double &h_out,
+                                         double &m_out, double &var_out) {
+  const double p = u[0], mm = u[1], a = u[2], b = u[3], c = u[4];
+  const double s = p + mm;
+  if (!(s > 0.0)) return 1;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    if (!isfinite(u[j])) return 1;
+  const double h = p / s, m = mm / s;
+  const double bias = (m * a - h * b + (m - h) * c) / r;
+  const double var = (m * m * a + h * h * b - 2.0 * h * m * c) / r;
+  double dh[5] = {m / s, -h / s, 0.0, 0.0, 0.0}, db[5], dv[5];
+  bool finite = isfinite(h) && isfinite(m) && isfinite(bias) && isfinite(var);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const double da = j == 2, dbb = j == 3, dc = j == 4;
+    db[j] = (-(a + b + 2.0 * c) * dh[j] + m * da - h * dbb + (m - h) * dc) / r;
+    dv[j] = ((-2.0 * m * a + 2.0 * h * b - 2.0 * c * (m - h)) * dh[j] + m * m * da + h * h * dbb -
+             2.0 * h * m * dc) / r;
+    finite = finite && isfinite(dh[j]) && isfinite(db[j]) && isfinite(dv[j]);
+  }
+  if (!finite) return 1;
+  h_out = h;
+  m_out = m;
+  var_out = var;
+  if (var < 0.0) return 5;
+  if (m <= 0.0) return 2;
+  const double inv = 1.0 / m, inv2 = inv * inv, inv3 = inv2 * inv, inv4 = inv2 * inv2;
+  const double factor = -inv2 - 2.0 * bias * inv3 + 3.0 * var * inv4;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    grad[j] = -factor * dh[j] + inv2 * db[j] - inv3 * dv[j];
+    if (!isfinite(grad[j])) return 2;
+  }
+  return 0;
+}
+
+// Per unit: gradient (5 x units), offset gradient . u (Neumaier), and block
+// partials for the attachment's diagnostics: [min u0 + u1, min m, max
+// sqrt(variance)/m, last unit with an event (unit * 8 + code, or -1), any
+// clipped adjustment variance (engine.rs:2327-2338)].
+static __global__ void __launch_bounds__(kBlock) k_numerical_state(uint64_t units, const double *__restrict__ m,
+                                                                   double r, double *__restrict__ grad,
+                                                                   double *__restrict__ offs,
+                                                                   double *__restrict__ part) {
+  __shared__ double lds[3][kBlock];
+  __shared__ long long lev[kBlock], lclip[kBlock];
+  double mn_c = INFINITY, mn_m = INFINITY, mx_s = 0.0;
+  long long last = -1, clipped = 0;
+  VK_GRID_LOOP(g, units) {
+    double u[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) u[q] = (m[(2 * q) * units + g] + m[(2 * q + 1) * units + g]) / r;
+    // the adjustment's own variance (engine.rs:2299-2338) marks clipping
+    {
+      const double pf = m[0 * units + g] + m[1 * units + g], mf = m[2 * units + g] + m[3 * units + g];
+      const double total = (pf + mf) / r;
+      const double proj = (pf / r) / total, res = (mf / r) / total;
+      const double var = (res * res * u[2] + proj * proj * u[3] - 2.0 * proj * res * u[4]) / r;
+      if (var < 0.0) clipped = 1;
+    }
+    double gr[5] = {0, 0, 0, 0, 0}, h = 0.0, mm = 0.0, var = 0.0;
+    const int code = unit_linearisation(u, r, gr, h, mm, var);
+    if (code != 1) {  // numerical.rs:95-100: after a finite derivative exists
+      mn_c = fmin(mn_c, u[0] + u[1]);
+      mn_m = fmin(mn_m, mm);
+      mx_s = fmax(mx_s, sqrt(fmax(var, 0.0)) / mm);
+    }
+    // event status: 5 nonsmooth adjustment, 6 nonfinite derivative
+    if (code) last = (long long)g * 8 + (code == 5 ? 5 : 6);
+    double s = 0.0, e = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      grad[(uint64_t)q * units + g] = code ? 0.0 : gr[q];
+      if (!code) neumaier_add(s, e, gr[q] * u[q]);
+    }
+    offs[g] = s + e;
+  }
+  lds[0][threadIdx.x] = mn_c;
+  lds[1][threadIdx.x] = mn_m;
+  lds[2][threadIdx.x] = mx_s;
+  lev[threadIdx.x] = last;
+  lclip[threadIdx.x] = clipped;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      lds[0][threadIdx.x] = fmin(lds[0][threadIdx.x], lds[0][threadIdx.x + w]);
+      lds[1][threadIdx.x] = fmin(lds[1][threadIdx.x], lds[1][threadIdx.x + w]);
+      lds[2][threadIdx.x] = fmax(lds[2][threadIdx.x], lds[2][threadIdx.x + w]);
+      lev[threadIdx.x] = lev[threadIdx.x] > lev[threadIdx.x + w] ? lev[threadIdx.x] : lev[threadIdx.x + w];
+      lclip[threadIdx.x] |= lclip[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double *o = part + 5 * (uint64_t)blockIdx.x;
+    o[0] = lds[0][0];
+    o[1] = lds[1][0];
+    o[2] = lds[2][0];
+    o[3] = (double)lev[0];
+    o[4] = (double)lclip[0];
+  }
+}
+
+// Target folds (engine/numerical.rs:124-157): for every probe of a target
+// batch in probe order, unit g adds mass * (sw^2, sf^2, sw sf) to its fold's
+// compensated sums, mass = outcome_g * residual_mass_g / fold size; sw and sf
+// are the worker- and firm-score solutions (columns 2q and 2q + 1) at the
+// unit's cell.  folds: sums then errors, [(fold * 3 + j) * units + g].
+template <int K2>
+__global__ void __launch_bounds__(kBlock) k_numerical_folds(uint64_t units, uint32_t probes, uint64_t first,
+                                                            uint64_t R, const uint32_t *__restrict__ unit_cell,
+                                                            const uint32_t *__restrict__ cell_worker,
+                                                            const uint32_t *__restrict__ cell_firm,
+                                                            const double *__restrict__ unit_outcome,
+                                                            const double *__restrict__ residual_mass,
+                                                            const double *__restrict__ worker,
+                                                            const double *__restrict__ firm,
+                                                            double *__restrict__ folds) {
+  VK_GRID_LOOP(g, units) {
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double base = unit_outcome[g] * residual_mass[g];
+    for (uint32_t q = 0; q < probes; ++q) {
+      const uint64_t probe = first + q;
+      const int fold = (int)(probe % 2);
+      const double count = fold == 0 ? (double)((R + 1) / 2) : (double)(R / 2);
+      const double sw = worker[w * K2 + 2 * q] + firm[f * K2 + 2 * q];
+      const double sf = worker[w * K2 + 2 * q + 1] + firm[f * K2 + 2 * q + 1];
+      const double mass = base / count;
+      const double kappa[3] = {sw * sw, sf * sf, sw * sf};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double &s = folds[(uint64_t)(fold * 3 + j) * units + g];
+        double &e = folds[(uint64_t)(6 + fold * 3 + j) * units + g];
+        neumaier_add(s, e, mass * kappa[j]);
+      }
+    }
+  }
+}
+
+// Replay scores (engine/numerical.rs:246-291): per replayed probe (lane) the
+// six sums over units of -fold[g][f][j] * alpha_g, alpha_g = gradient_g . u_g
+// - offset_g with u_g the unit's (p^2, m^2, p^4, m^4, p^2 m^2).
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_numerical_scores(uint64_t units, uint32_t kcols,
+                                                             const uint32_t *__restrict__ unit_cell,
+                                                             const uint64_t *__restrict__ unit_count,
+                                                             const uint32_t *__restrict__ cell_worker,
+                                                             const uint32_t *__restrict__ cell_firm,
+                                                             const double *__restrict__ worker,
+                                                             const double *__restrict__ firm,
+                                                             const int32_t *__restrict__ atoms,
+                                                             const double *__restrict__ grad,
+                                                             const double *__restrict__ offs,
+                                                             const double *__restrict__ folds,
+                                                             double *__restrict__ partial) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  VK_GRID_LOOP(t, units * K) {
+    const uint64_t g = t / K;
+    const int j = t % K;
+    if (j >= (int)kcols) continue;
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double root = sqrt((double)unit_count[g]);
+    const double p = root * (worker[w * K + j] + firm[f * K + j]);
+    const double mres = (double)atoms[g * K + j] / root - p;
+    const double p2 = p * p, m2 = mres * mres;
+    const double u[5] = {p2, m2, p2 * p2, m2 * m2, p2 * m2};
+    double s = 0.0, e = 0.0;
+    neumaier_add(s, e, -offs[g]);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) neumaier_add(s, e, grad[(uint64_t)q * units + g] * u[q]);
+    const double alpha = s + e;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) acc[v] -= (folds[(uint64_t)v * units + g] + folds[(uint64_t)(6 + v) * units + g]) * alpha;
+  }
+  block_partial<K, 6>(acc, partial);
+}
+struct FinScores {
+  double *__restrict__ out;  // [lane][fold * 3 + j]
+  __device__ bool enabled() const { return true; }
+  __device__ void reduced(int j, const double (&tot)[6]) const {
+#pragma unroll
+    for (int v = 0; v < 6; ++v) out[j * 6 + v] = tot[v];
+  }
+};
+
 // cell_correction_weight[c] = compensated sum over the cell's units ascending.
 static __global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict__ cu_ptr, const uint32_t *__restrict__ cu,
                               const double *__restrict__ contrib, double *__restrict__ out) {

@@ -142,19 +142,12 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   ctx.H = build_hierarchy(ctx.HS.graph, copt);
 }
 
-SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ctx) {
-  const auto &v4 = q.v4;
+// Options of a point request on the CMG_FULL_V2 path (the V4 prefix shared
+// by the V5 solve and the numerical requests).
+SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t tolerance_supplied, const Context &ctx) {
   const auto &v3 = v4.v3;
   const auto &v2 = v3.v2;
   const auto &v1 = v2.v1;
-  if (v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_solve", "unsupported ABI version");
-  if (v1.struct_size < sizeof(VckssEngineSolveRequestV5))
-    fail(kAbiMismatch, "engine_solve", "V5 solve request reports a short structure size");
-  if (q.reserved_5 != 0 || q.tolerance_supplied > 1 || q.full_cmg_v2 > 1)
-    fail(kAbiMismatch, "engine_solve", "invalid V5 solve request flags or reserved field");
-  if (q.full_cmg_v2 != 1)
-    fail(kUnsupported, "engine_solve", "the MI355X engine implements the CMG_FULL_V2 route only");
-  if (q.threads == 0) fail(kInvalidInput, "cmg_full_v2", "CMG_FULL_V2 requires a positive caller-declared thread count");
   // ffi_engine.rs:10335-10425 eligibility, restricted to the compressed path
   const bool eligible =
       v2.algorithm == VCKSS_ALGORITHM_JLA &&
@@ -180,7 +173,7 @@ SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ct
   o.seed = v1.seed;
   o.probes = v1.probes;
   o.fit_tolerance = v1.pcg_tolerance;
-  o.probe_tolerance = q.tolerance_supplied ? v1.pcg_tolerance : 1.0e-6;  // full_cmg.rs:29
+  o.probe_tolerance = tolerance_supplied ? v1.pcg_tolerance : 1.0e-6;  // full_cmg.rs:29
   for (double t : {o.fit_tolerance, o.probe_tolerance})
     if (!std::isfinite(t) || t < 1e-15 || t > 1e-4) fail(kInvalidInput, "cmg_full_v2", "tolerance must lie in [1e-15, 1e-4]");
   o.max_iterations = v1.maximum_iterations;
@@ -199,6 +192,54 @@ SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ct
   return o;
 }
 
+SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ctx) {
+  const auto &v1 = q.v4.v3.v2.v1;
+  if (v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_solve", "unsupported ABI version");
+  if (v1.struct_size < sizeof(VckssEngineSolveRequestV5))
+    fail(kAbiMismatch, "engine_solve", "V5 solve request reports a short structure size");
+  if (q.reserved_5 != 0 || q.tolerance_supplied > 1 || q.full_cmg_v2 > 1)
+    fail(kAbiMismatch, "engine_solve", "invalid V5 solve request flags or reserved field");
+  if (q.full_cmg_v2 != 1)
+    fail(kUnsupported, "engine_solve", "the MI355X engine implements the CMG_FULL_V2 route only");
+  if (q.threads == 0) fail(kInvalidInput, "cmg_full_v2", "CMG_FULL_V2 requires a positive caller-declared thread count");
+  return point_options(q.v4, q.tolerance_supplied, ctx);
+}
+
+// numerical_api.rs:113-196 (validate_point) for the path's requests: schema,
+// reserved fields and the point prefix; controls, projection and component
+// attachments are outside the path.
+void validate_numerical_v1(const VckssNumericalRequestV1 &r, uint32_t schema) {
+  if (r.abi_version != VCKSS_RUST_ABI_VERSION_V1 || r.point.v3.v2.v1.abi_version != VCKSS_RUST_ABI_VERSION_V1)
+    fail(kAbiMismatch, "numerical_preflight", "unsupported ABI version");
+  if (r.schema_version != schema || r.reserved != 0 || r.reserved_2 != 0 || r.tolerance_supplied > 1 ||
+      r.projection_requested > 1 || r.component_requested > 1)
+    fail(kAbiMismatch, "numerical_preflight", "invalid numerical V1 schema or reserved fields");
+  if (r.point.v3.v2.v1.struct_size < sizeof(VckssEngineSolveRequestV4))
+    fail(kAbiMismatch, "numerical_preflight", "numerical point prefix reports a short structure size");
+  if (r.point.v3.reserved_3 != 0 || r.point.reserved_4 != 0)
+    fail(kAbiMismatch, "numerical_preflight", "reserved numerical point prefix fields must be zero");
+  if (r.threads == 0 || r.point.v3.v2.v1.probes < 2)
+    fail(kInvalidInput, "numerical_preflight", "positive threads and at least two probes required");
+  if (r.projection_requested || r.component_requested || r.controls_count)
+    fail(kUnsupported, "numerical_preflight",
+         "the MI355X engine attaches numerical MCSE to the no-control compressed path only");
+}
+
+void validate_numerical_v2(const VckssNumericalRequestV2 &r) {
+  if (r.v1.schema_version != 2 || r.reserved != 0 || r.full_cmg > 1 || r.component_batch_mode > 1)
+    fail(kAbiMismatch, "numerical_preflight", "invalid numerical V2 schema or executor intent");
+  validate_numerical_v1(r.v1, 2);
+  if (r.full_cmg != 1 || r.execution_mode != 0)
+    fail(kUnsupported, "numerical_preflight", "the MI355X engine runs the numerical attachment on CMG_FULL_V2 only");
+}
+
+// Point options also checked without a context (preflight).
+void preflight_point(VckssEngineSolveRequestV4 point, uint32_t tolerance_supplied) {
+  if (point.v3.engine == VCKSS_ENGINE_COMPRESSED) point.v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
+  Context probe;
+  (void)point_options(point, tolerance_supplied, probe);
+}
+
 std::vector<double> builtin_smoother() {
   return std::vector<double>(kBuiltinSmoother, kBuiltinSmoother + kBuiltinSmootherCount);
 }
@@ -212,8 +253,13 @@ DeviceEngine &device_of(Context &ctx) {
   return *ctx.dev;
 }
 
+void run_options(Context &ctx, const SolveOptions &o, VckssInterruptPollV1 poll, void *poll_ctx);
+
 void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, VckssInterruptPollV1 poll, void *poll_ctx) {
-  SolveOptions o = solve_options(q, ctx);
+  run_options(ctx, solve_options(q, ctx), poll, poll_ctx);
+}
+
+void run_options(Context &ctx, const SolveOptions &o, VckssInterruptPollV1 poll, void *poll_ctx) {
   if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK) fail(kUserBreak, "engine_solve", "user break");
   DeviceEngine &dev = device_of(ctx);
   ctx.state = kSolving;
@@ -430,6 +476,168 @@ int32_t vckss_rust_engine_result_v1(uint64_t generation, VckssEngineResultV1 *ou
     put(o.correction, r.correction);
     put(o.corrected, r.corrected);
     put(o.numerical_mcse, r.mcse);
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- numerical MCSE attachment (numerical_api.rs:262-664)
+uint32_t vckss_rust_numerical_schema_v1(void) { return 1; }
+uint32_t vckss_rust_numerical_schema_v2(void) { return 2; }
+
+int32_t vckss_rust_engine_numerical_preflight_v1(const VckssNumericalRequestV1 *request) {
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "numerical_preflight", "request pointer is null");
+    VckssNumericalRequestV1 r;
+    std::memcpy(&r, request, sizeof r);
+    validate_numerical_v1(r, 1);
+    preflight_point(r.point, r.tolerance_supplied);
+  });
+}
+
+int32_t vckss_rust_engine_numerical_preflight_v2(const VckssNumericalRequestV2 *request) {
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "numerical_preflight", "request pointer is null");
+    VckssNumericalRequestV2 r;
+    std::memcpy(&r, request, sizeof r);
+    validate_numerical_v2(r);
+    preflight_point(r.v1.point, r.v1.tolerance_supplied);
+  });
+}
+
+static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &r, VckssInterruptPollV1 poll,
+                            void *poll_ctx) {
+  Context &ctx = live(generation);
+  VckssEngineSolveRequestV4 point = r.point;
+  // numerical_api.rs:157-161 admits the compressed engine for the attachment
+  if (point.v3.engine == VCKSS_ENGINE_COMPRESSED) point.v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
+  SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
+  o.numerical = true;
+  run_options(ctx, o, poll, poll_ctx);
+}
+
+int32_t vckss_rust_engine_solve_numerical_v1(uint64_t generation, const VckssNumericalRequestV1 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "numerical_solve", "request pointer is null");
+    VckssNumericalRequestV1 r;
+    std::memcpy(&r, request, sizeof r);
+    validate_numerical_v1(r, 1);
+    solve_numerical(generation, r, nullptr, nullptr);
+  });
+}
+
+int32_t vckss_rust_engine_solve_numerical_interrupt_v1(uint64_t generation, const VckssNumericalRequestV1 *request,
+                                                       VckssInterruptPollV1 callback, void *context,
+                                                       uint32_t interval) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "numerical_solve", "request pointer is null");
+    VckssNumericalRequestV1 r;
+    std::memcpy(&r, request, sizeof r);
+    validate_numerical_v1(r, 1);
+    (void)interval;
+    solve_numerical(generation, r, callback, context);
+  });
+}
+
+int32_t vckss_rust_engine_solve_numerical_interrupt_v2(uint64_t generation, const VckssNumericalRequestV2 *request,
+                                                       VckssInterruptPollV1 callback, void *context,
+                                                       uint32_t interval) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "numerical_solve", "request pointer is null");
+    VckssNumericalRequestV2 r;
+    std::memcpy(&r, request, sizeof r);
+    validate_numerical_v2(r);
+    (void)interval;
+    solve_numerical(generation, r.v1, callback, context);
+  });
+}
+
+static const NumericalResult &numerical_of(Context &ctx) {
+  if (!ctx.has_result) fail(kStaleContext, "numerical_result", "no solved result is available");
+  if (!ctx.result.has_numerical) fail(kUnsupported, "numerical_result", "result has no numerical V1 attachment");
+  return ctx.result.numerical;
+}
+
+int32_t vckss_rust_engine_numerical_result_v1(uint64_t generation, VckssNumericalResultV1 *output,
+                                              uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    require_capacity<VckssNumericalResultV1>(output, output_capacity_bytes, "numerical result");
+    const NumericalResult &n = numerical_of(ctx);
+    VckssNumericalResultV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    o.status = n.status;
+    o.engine = VCKSS_ENGINE_COMPRESSED;
+    o.leverage_probes = n.leverage_probes;
+    o.target_probes = n.target_probes;
+    o.target_fold_a = n.fold_a;
+    o.target_fold_b = n.fold_b;
+    o.certified_replay_rhs = n.certified_replay_rhs;
+    o.executed_replay_rhs = n.certified_replay_rhs;
+    o.attempted_replay_rhs = n.attempted_replay_rhs;
+    o.replay_generator_words = n.replay_generator_words;
+    o.allocation_bound_bytes = n.allocation_bytes;
+    o.failed_replay_probe = n.failed_replay_probe;
+    o.replay_error_code = n.replay_error_code;
+    o.psd_adjustment = n.psd_adjustment;
+    o.minimum_constrained = n.minimum_constrained;
+    o.minimum_residual_margin = n.minimum_residual_margin;
+    o.maximum_sensitivity_ratio = n.maximum_sensitivity_ratio;
+    std::memcpy(o.score_means, n.score_means, sizeof o.score_means);
+    std::memcpy(o.conditional, n.conditional, sizeof o.conditional);
+    std::memcpy(o.leverage, n.leverage, sizeof o.leverage);
+    std::memcpy(o.raw, n.raw, sizeof o.raw);
+    std::memcpy(o.usable, n.usable, sizeof o.usable);
+    std::memcpy(o.mcse, n.mcse, sizeof o.mcse);
+    o.point_rhs = n.point_rhs;
+    o.maximum_point_complete_residual = n.maximum_point_complete_residual;
+    o.maximum_replay_complete_residual = n.maximum_replay_complete_residual;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+int32_t vckss_rust_engine_numerical_rhs_v1(uint64_t generation, VckssNumericalRhsV1 *output, uint64_t capacity,
+                                           uint64_t *written) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!written) fail(kAbiMismatch, "numerical_rhs", "numerical RHS count destination is null");
+    Context &ctx = live(generation);
+    const NumericalResult &n = numerical_of(ctx);
+    const uint64_t count = n.replay_probes.size();
+    if (capacity < count || (count && !output)) fail(kAbiMismatch, "numerical_rhs", "invalid or short numerical RHS destination");
+    for (uint64_t i = 0; i < count; ++i) {
+      VckssNumericalRhsV1 r;
+      std::memset(&r, 0, sizeof r);
+      r.schema_version = 1;
+      r.phase = 1;  // leverage replay
+      r.probe = n.replay_probes[i];
+      r.complete_residual = n.replay_complete[i];
+      std::memcpy(output + i, &r, sizeof r);
+    }
+    *written = count;
+  });
+}
+
+int32_t vckss_rust_engine_numerical_cmg_work_v1(uint64_t generation, VckssFullCmgModelReceiptV1 *output,
+                                                uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    require_capacity<VckssFullCmgModelReceiptV1>(output, output_capacity_bytes, "numerical CMG work");
+    const NumericalResult &n = numerical_of(ctx);
+    VckssFullCmgModelReceiptV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    o.nuisance_mode = VCKSS_NUISANCE_JOINT;
+    o.logical_rhs_count = n.point_rhs + n.certified_replay_rhs;
     std::memcpy(output, &o, sizeof o);
   });
 }

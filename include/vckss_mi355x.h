@@ -230,6 +230,62 @@ typedef struct VckssEngineResultV1 {
     VckssComponentVectorV1 numerical_mcse;
 } VckssEngineResultV1;
 
+/* vckss_rust.h:2337-2346: numerical MCSE request (opt-in attachment) */
+typedef struct VckssNumericalRequestV1 {
+    uint32_t abi_version, struct_size, schema_version, threads;
+    uint32_t tolerance_supplied, projection_requested, component_requested, reserved;
+    uint32_t controls_count, reserved_2;
+    VckssEngineSolveRequestV4 point;
+} VckssNumericalRequestV1;
+
+/* vckss_rust.h:2348-2351 */
+typedef struct VckssNumericalRequestV2 {
+    VckssNumericalRequestV1 v1;
+    uint32_t execution_mode, component_batch_mode, full_cmg, reserved;
+} VckssNumericalRequestV2;
+
+/* vckss_rust.h:2359-2370: status 1 exact_zero, 2 ok_local, 3 ok_local_psd_adjusted,
+ * 4 unstable_nonpsd, 5 nonsmooth_adjustment, 6 nonfinite_derivative, 7 replay_failed */
+typedef struct VckssNumericalResultV1 {
+    uint32_t struct_size, schema_version;
+    uint64_t generation;
+    uint32_t status, engine, leverage_probes, target_probes, target_fold_a, target_fold_b;
+    uint64_t certified_replay_rhs, executed_replay_rhs, attempted_replay_rhs, replay_generator_words,
+        allocation_bound_bytes;
+    uint32_t failed_replay_probe, replay_error_code;
+    double psd_adjustment, minimum_constrained, minimum_residual_margin, maximum_sensitivity_ratio;
+    double score_means[2][3];
+    double conditional[3][3], leverage[3][3], raw[3][3], usable[3][3], mcse[4];
+    uint64_t point_rhs;
+    double maximum_point_complete_residual, maximum_replay_complete_residual;
+} VckssNumericalResultV1;
+
+/* vckss_rust.h:2371-2374 */
+typedef struct VckssNumericalRhsV1 {
+    uint32_t schema_version, phase, probe, reserved;
+    double complete_residual;
+} VckssNumericalRhsV1;
+
+/* vckss_rust.h:1512-1522 */
+typedef struct VckssFullCmgModelReceiptV1 {
+    uint32_t struct_size;
+    uint32_t schema_version;
+    uint64_t generation;
+    uint32_t controls_count;
+    uint32_t nuisance_mode;
+    uint64_t logical_rhs_count;
+    uint64_t explicit_options_rhs_count;
+    uint64_t controlled_rhs_count;
+    uint64_t control_refinement_rhs_count;
+} VckssFullCmgModelReceiptV1;
+
+#if !defined(__cplusplus)
+_Static_assert(sizeof(VckssNumericalRequestV1) == 328, "numerical request layout");
+_Static_assert(sizeof(VckssNumericalRequestV2) == 344, "numerical V2 request layout");
+_Static_assert(sizeof(VckssNumericalResultV1) == 512, "numerical result layout");
+_Static_assert(sizeof(VckssNumericalRhsV1) == 24, "numerical RHS layout");
+#endif
+
 /* vckss_rust.h:1635-1640 */
 typedef struct VckssEngineSnapshotV1 {
     uint32_t struct_size;
@@ -272,6 +328,27 @@ int32_t vckss_rust_engine_release_v1(uint64_t generation);
 int32_t vckss_rust_engine_clear_abandoned_v1(void);
 int32_t vckss_rust_engine_snapshot_v1(VckssEngineSnapshotV1 *output, uint32_t output_capacity_bytes);
 const char *vckss_rust_engine_last_error(void);
+
+/* vckss_rust.h:2348-2388: the numerical MCSE attachment on the same path.
+ * The point estimate equals the plain solve's; the leverage probes are then
+ * replayed and scored against the unit linearisation and target folds. */
+uint32_t vckss_rust_numerical_schema_v1(void);
+uint32_t vckss_rust_numerical_schema_v2(void);
+int32_t vckss_rust_engine_numerical_preflight_v1(const VckssNumericalRequestV1 *request);
+int32_t vckss_rust_engine_numerical_preflight_v2(const VckssNumericalRequestV2 *request);
+int32_t vckss_rust_engine_solve_numerical_v1(uint64_t generation, const VckssNumericalRequestV1 *request);
+int32_t vckss_rust_engine_solve_numerical_interrupt_v1(uint64_t generation, const VckssNumericalRequestV1 *request,
+                                                       VckssInterruptPollV1 callback, void *context,
+                                                       uint32_t interval);
+int32_t vckss_rust_engine_solve_numerical_interrupt_v2(uint64_t generation, const VckssNumericalRequestV2 *request,
+                                                       VckssInterruptPollV1 callback, void *context,
+                                                       uint32_t interval);
+int32_t vckss_rust_engine_numerical_result_v1(uint64_t generation, VckssNumericalResultV1 *output,
+                                              uint32_t output_capacity_bytes);
+int32_t vckss_rust_engine_numerical_rhs_v1(uint64_t generation, VckssNumericalRhsV1 *output, uint64_t capacity,
+                                           uint64_t *written);
+int32_t vckss_rust_engine_numerical_cmg_work_v1(uint64_t generation, VckssFullCmgModelReceiptV1 *output,
+                                                uint32_t output_capacity_bytes);
 
 #ifdef __cplusplus
 }

@@ -200,3 +200,34 @@ def test_full_reference_abi_is_exported():
     g = getattr(lib, schema[0]["name"])
     g.restype = C.c_uint32
     assert g() == 0
+
+
+def test_numerical_preflight_validation():
+    """numerical_api.rs:113-196, 237-279: schema and reserved fields are ABI
+    errors (10); projection, component, control and non-full-CMG requests are
+    unsupported (11); one probe is invalid input (20)."""
+    import ctypes as C
+    lib = F.library()
+    assert lib.vckss_rust_numerical_schema_v1() == 1 and lib.vckss_rust_numerical_schema_v2() == 2
+
+    def status(**changes):
+        r = F.default_numerical_request(probes=16, tolerance=1e-8)
+        for path, value in changes.items():
+            obj = r
+            parts = path.split(".")
+            for part in parts[:-1]:
+                obj = getattr(obj, part)
+            setattr(obj, parts[-1], value)
+        return lib.vckss_rust_engine_numerical_preflight_v2(C.byref(r))
+
+    assert status() == 0
+    assert status(**{"v1.schema_version": 1}) == 10
+    assert status(**{"v1.reserved": 1}) == 10
+    assert status(reserved=1) == 10
+    assert status(full_cmg=2) == 10
+    assert status(**{"v1.projection_requested": 1}) == 11
+    assert status(**{"v1.controls_count": 1}) == 11
+    assert status(full_cmg=0) == 11
+    assert status(**{"v1.point.probes": 1}) == 20
+    assert status(**{"v1.threads": 0}) == 20
+    assert status(**{"v1.point.deletion_mode": 0}) == 11
