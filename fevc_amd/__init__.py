@@ -233,6 +233,8 @@ SIGNATURES = [
     ("vckss_mi355x_hierarchy_free_v1", C.c_int32, [C.c_uint64]),
     ("vckss_mi355x_layout_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p] + [C.c_void_p] * 12),
+    ("vckss_mi355x_chunk_table_v1", C.c_int32,
+     [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("vckss_mi355x_prepare_host_v1", C.c_int32,
      [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64)]),
 ]
@@ -543,6 +545,23 @@ class Engine:
             self.release()
         except Exception:
             pass
+
+
+def chunk_table(ptr, chunk=64, group=1):
+    """Chunk table of segments with pointers `ptr` (host only): (desc as an
+    (chunks, 4) array, multi ranges as (multi, 2)).  With group > 1, runs of
+    short segments share a chunk tagged 0xffffff00 + count."""
+    lib = library()
+    ptr = np.ascontiguousarray(ptr, np.uint32)
+    segments = max(len(ptr) - 1, 0)
+    s = np.zeros(2, np.uint64)
+    _check(lib.vckss_mi355x_chunk_table_v1(ptr.ctypes.data, segments, chunk, group, s.ctypes.data, None, None))
+    desc = np.zeros(4 * int(s[0]), np.uint32)
+    multi = np.zeros(2 * int(s[1]), np.uint32)
+    _check(lib.vckss_mi355x_chunk_table_v1(ptr.ctypes.data, segments, chunk, group, s.ctypes.data,
+                                           desc.ctypes.data if desc.size else None,
+                                           multi.ctypes.data if multi.size else None))
+    return desc.reshape(-1, 4), multi.reshape(-1, 2)
 
 
 def hierarchy_layout(n, u, v, w, chunk=64, direct_threshold=700, max_levels=128):

@@ -18,6 +18,15 @@ namespace {
 // ------------------------------------------------------------------ V-cycle
 template <typename T, int K>
 struct Cycle {
+  // The FP32 segment operations carry four probe lanes per thread (float4:
+  // 16-byte loads and stores, four entries of a row walked per wavefront);
+  // the scalar per-row values (matrix entries, inverse diagonals) stay float.
+  static constexpr bool kPacked = std::is_same_v<T, float> && K >= 16;
+  using VT = std::conditional_t<kPacked, float4, T>;
+  static constexpr int KV = kPacked ? K / 4 : K;
+  static VT *vp(T *p) { return reinterpret_cast<VT *>(p); }
+  static const VT *vp(const T *p) { return reinterpret_cast<const VT *>(p); }
+
   Impl &E;
   std::vector<LevelVecs<T>> &V;
   const T *braw0;  // level-0 right-hand side
@@ -56,23 +65,26 @@ struct Cycle {
     const uint32_t nx = (E.sm.shaped && L.next) ? L.next : (uint32_t)l + 1;
     const bool composite = nx != l + 1;
     const uint32_t *lab = composite ? L.clab.p : L.lab.p;
-    const DevChunks &aggs = composite ? L.caggs : L.aggs;
+    constexpr bool g = grouped<KV>();
+    const DevChunks &rows = g ? L.rows_g : L.rows;
+    const DevChunks &aggs = composite ? (g ? L.caggs_g : L.caggs) : (g ? L.aggs_g : L.aggs);
     DevLevel &Lc = E.levels[nx];
     LevelVecs<T> &c = V[nx];
     for (uint32_t it = 0; it < iterations; ++it) {
       if (it == 0) {
-        segments<T, K, 0>(E, L.rows,
-                          dev::OpScaleResidual<T, K>{{L.col.p, val(L)}, braw, mean, inv(L), b, v.x.p, v.r.p});
+        segments<VT, KV, 0>(E, rows, dev::OpScaleResidual<VT, KV>{{L.col.p, val(L)}, vp(braw), mean, inv(L),
+                                                                      vp(b), vp(v.x.p), vp(v.r.p)});
       } else {
-        segments<T, K, 0>(E, L.rows, dev::OpSmooth<T, K>{{L.col.p, val(L)}, b, v.x.p, inv(L), v.y.p});
+        segments<VT, KV, 0>(E, rows,
+                            dev::OpSmooth<VT, KV>{{L.col.p, val(L)}, vp(b), vp(v.x.p), inv(L), vp(v.y.p)});
         std::swap(v.x, v.y);
-        segments<T, K, 0>(E, L.rows, dev::OpResidual<T, K>{{L.col.p, val(L)}, b, v.x.p, v.r.p});
+        segments<VT, KV, 0>(E, rows, dev::OpResidual<VT, KV>{{L.col.p, val(L)}, vp(b), vp(v.x.p), vp(v.r.p)});
       }
       // The coarse right-hand side sums residuals of a Laplacian (zero row
       // sums) of a centred right-hand side, so its lane means vanish up to
       // rounding: the reference's coarse centring (cycle.rs:459-478) is a
       // rounding-level projection here and is not recomputed.
-      segments<T, K, 0>(E, aggs, dev::OpAggregate<T, K>{v.r.p, c.braw.p, nullptr, Lc.n});
+      segments<VT, KV, 0>(E, aggs, dev::OpAggregate<VT, KV>{vp(v.r.p), vp(c.braw.p), nullptr, Lc.n});
       level(nx, E.levels[nx - 1].repeat, nullptr);
       if (l == 0) {
         // the finest prolong-smooth pass is the kernel the roofline reports:
@@ -88,12 +100,14 @@ struct Cycle {
           E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * L.rows.count + (s + 4) * L.n +
                           K * s * (3.0 * L.n + Lc.n);
         }
-        segments<T, K, 0>(
-            E, L.rows,
-            dev::OpProlongSmooth<T, K, true>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), lab, v.y.p}, a, e);
+        segments<VT, KV, 0>(E, rows,
+                            dev::OpProlongSmooth<VT, KV, true>{{L.col.p, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p),
+                                                               inv(L), lab, vp(v.y.p)},
+                            a, e);
       } else {
-        segments<T, K, 0>(E, L.rows,
-                          dev::OpProlongSmooth<T, K>{{L.col.p, val(L)}, v.r.p, v.x.p, c.x.p, inv(L), lab, v.y.p});
+        segments<VT, KV, 0>(E, rows,
+                            dev::OpProlongSmooth<VT, KV>{{L.col.p, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p), inv(L),
+                                                         lab, vp(v.y.p)});
       }
       std::swap(v.x, v.y);
     }
@@ -142,6 +156,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   const uint32_t n = E.n0;
   const uint64_t m0 = (uint64_t)n * K;
   const DevLevel &L0 = E.levels[0];
+  const DevChunks &rows0 = grouped<K>() ? L0.rows_g : L0.rows;
   dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval};
   int *flags = E.flags.p;
   const int *active = flags + dev::kFlagActive;
@@ -161,7 +176,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
     hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
                        E.lane_a.p, nullptr);
-    segments<double, K, 4>(E, L0.rows, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
+    segments<double, K, 4>(E, rows0, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
   }
   E.gate = active;
   auto precondition = [&]() {
@@ -186,14 +201,14 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   precondition();
   for (uint32_t it = 1; it <= max_it; ++it) {
     E.iterations_launched++;
-    segments<double, K, 1>(E, L0.rows, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
+    segments<double, K, 1>(E, rows0, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
     hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
                        E.rpartial.p, E.st.p, active);
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
     hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags, active);
     hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.lane_a.p,
                        active);
-    segments<double, K, 3>(E, L0.rows,
+    segments<double, K, 3>(E, rows0,
                            dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags});
     hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm, flags);
     const int slot = it % kPollSlots;
@@ -491,9 +506,11 @@ void relayout(Impl &E, uint32_t chunk) {
     D.col.upload(Y.col); D.val.upload(Y.val); D.val32.upload(Y.val);
     D.inv.upload(Y.inv); D.inv32.upload(Y.inv);
     D.rows.upload(Y.rows);
+    D.rows_g.upload(chunk_segments(Y.row_ptr, chunk, kSegmentGroup));
     if (!Y.lab.empty()) {
       D.lab.upload(Y.lab);
       D.aggs.upload(Y.aggs);
+      D.aggs_g.upload(chunk_segments(Y.agg_ptr, chunk, kSegmentGroup));
     }
     D.perm = Y.perm;
     D.lab_host = Y.lab;

@@ -99,15 +99,18 @@ inline uint32_t lanes_for(uint32_t columns) {
 }
 
 struct DevChunks {
-  DBuf<uint32_t> desc, mrange;
+  DBuf<uint32_t> desc, mrange, ptr;
   uint32_t count = 0, n_multi = 0;
   void upload(const ChunkTable &t) {
     desc.upload(t.desc);
     mrange.upload(t.mrange);
+    ptr.upload(t.ptr);
     count = static_cast<uint32_t>(t.chunks());
     n_multi = t.multi();
   }
-  dev::Chunks view() const { return {reinterpret_cast<const uint4 *>(desc.p), mrange.p, count, n_multi}; }
+  dev::Chunks view() const {
+    return {reinterpret_cast<const uint4 *>(desc.p), mrange.p, ptr.p, count, n_multi};
+  }
 };
 
 struct DevLevel {
@@ -120,14 +123,22 @@ struct DevLevel {
   DBuf<double> linv;  // learned smoother inverse diagonal omega / d (smoothed levels)
   DBuf<float> linv32;
   DevChunks rows, aggs;
+  DevChunks rows_g, aggs_g;  // the same with runs of short segments grouped (kSegmentGroup)
   std::vector<uint32_t> perm;  // host copy: renumbered -> original vertex
   std::vector<uint32_t> lab_host;  // host copy of lab (renumbered coarse vertex)
   // Cycle shape: the next level the tuned cycle visits after this one; when it
   // skips levels, the composite aggregation to it (clab, caggs).
   uint32_t next = 0;
   DBuf<uint32_t> clab;
-  DevChunks caggs;
+  DevChunks caggs, caggs_g;
 };
+
+// Short segments per grouped chunk.  Operations with at least this many
+// slices per chunk (dev::Shape<K>::V) use the grouped tables: each slice then
+// walks its own short row, instead of all slices sharing one.
+constexpr uint32_t kSegmentGroup = 4;
+template <int K>
+constexpr bool grouped() { return dev::Shape<K>::V >= (int)kSegmentGroup; }
 
 template <typename T>
 struct LevelVecs {
@@ -289,6 +300,7 @@ inline void set_cycle_shape(DeviceEngine::Impl &E, const std::vector<uint32_t> &
     D.clab.upload(lab);
     const ChunkTable table = chunk_segments(ptr, E.chunk);
     D.caggs.upload(table);
+    D.caggs_g.upload(chunk_segments(ptr, E.chunk, kSegmentGroup));
     E.max_chunks = std::max<uint64_t>(E.max_chunks, table.chunks());
     E.max_multi = std::max(E.max_multi, table.multi());
   }

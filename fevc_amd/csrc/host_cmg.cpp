@@ -509,12 +509,27 @@ Hierarchy build_hierarchy(const Laplacian &graph, const CmgOptions &opt) {
   return H;
 }
 
-ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk) {
+ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk, uint32_t group) {
   if (chunk == 0) fail(kInvalidInput, "device_layout", "chunk size must be positive");
+  if (group == 0 || group >= kSingleChunk - kGroupChunk) fail(kInvalidInput, "device_layout", "bad group size");
   ChunkTable t;
   const uint64_t segments = ptr.empty() ? 0 : ptr.size() - 1;
+  const uint64_t short_len = chunk / 4;
+  if (group > 1) t.ptr = ptr;
   for (uint64_t s = 0; s < segments; ++s) {
     const uint64_t begin = ptr[s], len = ptr[s + 1] - begin;
+    if (group > 1 && len <= short_len) {
+      uint64_t e = s + 1;
+      while (e < segments && e - s < group && ptr[e + 1] - ptr[e] <= short_len) ++e;
+      if (e - s > 1) {
+        t.desc.push_back(static_cast<uint32_t>(s));
+        t.desc.push_back(ptr[s]);
+        t.desc.push_back(ptr[e]);
+        t.desc.push_back(kGroupChunk + static_cast<uint32_t>(e - s));
+        s = e - 1;
+        continue;
+      }
+    }
     const uint64_t parts = std::max<uint64_t>(1, (len + chunk - 1) / chunk);
     const uint32_t first = static_cast<uint32_t>(t.chunks());
     const uint32_t multi = parts > 1 ? t.multi() : kSingleChunk;
@@ -525,6 +540,7 @@ ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk) {
       t.desc.push_back(multi);
     }
     if (parts > 1) {
+      if (t.multi() >= kGroupChunk) fail(kInvalidInput, "device_layout", "too many multi-chunk segments");
       t.mrange.push_back(first);
       t.mrange.push_back(static_cast<uint32_t>(t.chunks()));
     }

@@ -73,17 +73,21 @@ Hierarchy build_hierarchy(const Laplacian &g, const CmgOptions &opt);
 // Segments (CSR rows, or aggregates over their contiguous members) are cut
 // into chunks of at most `chunk` entries so that no GPU thread group walks a
 // long row or a large aggregate alone.  A segment with several chunks gets a
-// "multi" index: its chunk partials are combined by whichever chunk group
-// finishes last.
+// "multi" index: its chunk partials are combined in a second pass.  With
+// `group` > 1, runs of up to `group` consecutive short segments (at most
+// chunk / 4 entries each) share one chunk tagged kGroupChunk + count, so that
+// the slices of one wavefront walk several short rows at once.
 constexpr uint32_t kSingleChunk = 0xffffffffu;
+constexpr uint32_t kGroupChunk = 0xffffff00u;
 
 struct ChunkTable {
-  std::vector<uint32_t> desc;    // 4 per chunk: segment, begin, end, multi index or kSingleChunk
+  std::vector<uint32_t> desc;    // 4 per chunk: segment, begin, end, tag
   std::vector<uint32_t> mrange;  // 2 per multi-chunk segment: first chunk, end chunk
+  std::vector<uint32_t> ptr;     // segment pointers (grouped tables only)
   uint64_t chunks() const { return desc.size() / 4; }
   uint32_t multi() const { return static_cast<uint32_t>(mrange.size() / 2); }
 };
-ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk);
+ChunkTable chunk_segments(const std::vector<uint32_t> &ptr, uint32_t chunk, uint32_t group = 1);
 
 // One level renumbered so that every aggregate is a contiguous run of
 // vertices, runs ordered like the coarse level's own renumbering; the last

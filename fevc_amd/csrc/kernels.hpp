@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstdint>
 
 namespace vk {
@@ -33,6 +35,7 @@ constexpr int kMaxReduceValues = 6;
 // consecutive slots (pass-1 blocks, then combine blocks).
 constexpr uint32_t kPartialStride = 2 * kMaxReduceBlocks;
 constexpr uint32_t kSingle = 0xffffffffu;
+constexpr uint32_t kGroupBase = 0xffffff00u;
 
 // Threads per chunk: V slices of K lanes, G = K * V <= 64 (one wavefront).
 template <int K>
@@ -40,6 +43,26 @@ struct Shape {
   static constexpr int V = K >= 64 ? 1 : (K >= 4 ? 64 / K : 16);
   static constexpr int G = K * V;
 };
+
+// Lane vectors: a segment op runs on K "lanes" of type T, where T may pack
+// four FP32 probe columns (float4, 16-byte loads); values that are one per
+// row (matrix values, inverse diagonals) stay scalar.
+template <typename T>
+struct ScalarOf {
+  using type = T;
+};
+template <>
+struct ScalarOf<float4> {
+  using type = float;
+};
+template <typename T>
+using scalar_t = typename ScalarOf<T>::type;
+
+__device__ inline float shfl_xor(float v, int off) { return __shfl_xor(v, off); }
+__device__ inline double shfl_xor(double v, int off) { return __shfl_xor(v, off); }
+__device__ inline float4 shfl_xor(float4 v, int off) {
+  return make_float4(__shfl_xor(v.x, off), __shfl_xor(v.y, off), __shfl_xor(v.z, off), __shfl_xor(v.w, off));
+}
 
 // ---------------------------------------------------------------- RNG
 // Counter-V1 Philox4x64-10 with SplitMix64 keys (rng.rs:14-99, 309-327).
@@ -173,9 +196,14 @@ __global__ void __launch_bounds__(kBlock) k_finish(const double *__restrict__ pa
 
 // ---------------------------------------------------------------- segments
 
+// A chunk descriptor is (segment, begin, end, tag): tag kSingle for a whole
+// segment, a multi index for one chunk of a long segment, or kGroupBase + c
+// for a group of c consecutive short segments starting at `segment` (their
+// entry ranges come from the segment pointers `ptr`).
 struct Chunks {
-  const uint4 *__restrict__ desc;     // segment, begin, end, multi index or kSingle
+  const uint4 *__restrict__ desc;
   const uint32_t *__restrict__ mrange;  // first and end chunk of every multi segment
+  const uint32_t *__restrict__ ptr;     // segment pointers (tables with groups)
   uint32_t count, n_multi;
 };
 
@@ -189,17 +217,20 @@ struct SegScratch {
 template <typename T, int K>
 __device__ inline T slice_sum(T v) {
 #pragma unroll
-  for (int off = K; off < Shape<K>::G; off <<= 1) v += __shfl_xor(v, off);
+  for (int off = K; off < Shape<K>::G; off <<= 1) v += shfl_xor(v, off);
   return v;
 }
 
 // Pass 1 of a segment operation.  Op supplies
-//   T partial(const uint4 &d, int j, int s)  sum over the chunk's entries
-//                                            d.y + s, d.y + s + V, ... < d.z
+//   T partial<STEP = V>(const uint4 &d, int j, int s)
+//                                            sum over the chunk's entries
+//                                            d.y + s, d.y + s + STEP, ... < d.z
 //   void finish(uint32_t seg, int j, T total, double (&acc)[NA])
 //   void reduced(int j, const double (&tot)[NV])   (NV > 0 only)
 //   bool enabled()                           uniform early exit
-// Single-chunk segments are finished here; multi-chunk segments leave their
+// Single-chunk segments and groups of short segments are finished here (a
+// group puts one short segment on each slice, so a wavefront walks several
+// rows at once); multi-chunk segments leave their
 // chunk partials for pass 2 (the kernel boundary publishes them, so no
 // device-scope fence is needed).  With NV > 0 every block stores its
 // reduction partial (no ticket).
@@ -225,6 +256,16 @@ __global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratc
       d.y = __builtin_amdgcn_readfirstlane(d.y);
       d.z = __builtin_amdgcn_readfirstlane(d.z);
       d.w = __builtin_amdgcn_readfirstlane(d.w);
+    }
+    if (d.w > kGroupBase && d.w != kSingle) {
+      // a group of short segments: slice s finishes segments s, s + V, ...
+      // of the group, each walked by its own K threads
+      for (uint32_t q = s; q < d.w - kGroupBase; q += Shape<K>::V) {
+        const uint32_t seg = d.x + q;
+        const uint4 e = make_uint4(seg, ch.ptr[seg], ch.ptr[seg + 1], kSingle);
+        op.finish(seg, j, op.template partial<1>(e, j, 0), acc);
+      }
+      continue;
     }
     const T part = slice_sum<T, K>(op.partial(d, j, s));
     if (s != 0) continue;
@@ -286,15 +327,16 @@ struct NoLane {};
 template <typename T, int K, class D>
 struct CsrOp {
   const uint32_t *__restrict__ col;
-  const T *__restrict__ val;
+  const scalar_t<T> *__restrict__ val;
   __device__ NoLane lane(int) const { return {}; }
+  template <int STEP = Shape<K>::V>
   __device__ T partial(const uint4 &d, int j, int s) const {
     const D &op = static_cast<const D &>(*this);
     const auto ln = op.lane(j);
     const T xi = op.self(d.x, j, ln);
     T acc = T(0);
 #pragma unroll 8
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) acc += val[k] * (xi - op.nbr(col[k], j, ln));
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));
     return acc;
   }
   __device__ bool enabled() const { return true; }
@@ -312,9 +354,15 @@ template <typename T, int K>
 struct OpScaleResidual : CsrOp<T, K, OpScaleResidual<T, K>> {
   const T *__restrict__ braw;
   const double *__restrict__ mean;
-  const T *__restrict__ inv;
+  const scalar_t<T> *__restrict__ inv;
   T *__restrict__ b, *__restrict__ x, *__restrict__ r;
-  __device__ T lane(int j) const { return mean ? (T)mean[j] : T(0); }  // the lane's coarse mean
+  __device__ T lane(int j) const {  // the lane's coarse mean
+    if (!mean) return T(0);
+    if constexpr (std::is_same_v<T, float4>)
+      return make_float4((float)mean[4 * j], (float)mean[4 * j + 1], (float)mean[4 * j + 2], (float)mean[4 * j + 3]);
+    else
+      return (T)mean[j];
+  }
   __device__ T self(uint32_t i, int j, T mu) const { return inv[i] * (braw[(uint64_t)i * K + j] - mu); }
   __device__ T nbr(uint32_t c, int j, T mu) const { return inv[c] * (braw[(uint64_t)c * K + j] - mu); }
   template <int NA>
@@ -330,7 +378,8 @@ struct OpScaleResidual : CsrOp<T, K, OpScaleResidual<T, K>> {
 // Later visits: y = x + D^-1 (b - A x)
 template <typename T, int K>
 struct OpSmooth : CsrOp<T, K, OpSmooth<T, K>> {
-  const T *__restrict__ b, *__restrict__ x, *__restrict__ inv;
+  const T *__restrict__ b, *__restrict__ x;
+  const scalar_t<T> *__restrict__ inv;
   T *__restrict__ y;
   __device__ T self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
   __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
@@ -362,7 +411,8 @@ struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
 // finest-level instance (its own kernel symbol for profiling).
 template <typename T, int K, bool Finest = false>
 struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
-  const T *__restrict__ r, *__restrict__ x, *__restrict__ e, *__restrict__ inv;
+  const T *__restrict__ r, *__restrict__ x, *__restrict__ e;
+  const scalar_t<T> *__restrict__ inv;
   const uint32_t *__restrict__ lab;
   T *__restrict__ y;
   __device__ T self(uint32_t i, int j, NoLane = {}) const { return e[(uint64_t)lab[i] * K + j]; }
@@ -395,16 +445,17 @@ struct OpAggregate {
   T *__restrict__ out;
   double *__restrict__ mean;
   uint32_t coarse;
+  template <int STEP = Shape<K>::V>
   __device__ T partial(const uint4 &d, int j, int s) const {
     T acc = T(0);
-    for (uint32_t m = d.y + s; m < d.z; m += Shape<K>::V) acc += r[(uint64_t)m * K + j];
+    for (uint32_t m = d.y + s; m < d.z; m += STEP) acc += r[(uint64_t)m * K + j];
     return acc;
   }
   __device__ bool enabled() const { return true; }
   template <int NA>
   __device__ void finish(uint32_t c, int j, T sum, double (&acc)[NA]) const {
     out[(uint64_t)c * K + j] = sum;
-    acc[0] += (double)sum;
+    if constexpr (!std::is_same_v<T, float4>) acc[0] += (double)sum;
   }
   template <int NV>
   __device__ void reduced(int j, const double (&tot)[NV]) const {

@@ -535,11 +535,12 @@ template <int K, class D>
 struct FirmCellOp {
   const uint32_t *__restrict__ fcells, *__restrict__ cell_worker;
   const double *__restrict__ cell_weight;
+  template <int STEP = Shape<K>::V>
   __device__ double partial(const uint4 &d, int j, int s) const {
     const D &op = static_cast<const D &>(*this);
     double acc = 0.0;
 #pragma unroll 4
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) {
       const uint32_t c = fcells[k];
       acc += cell_weight[c] * op.term(cell_worker[c], j);
     }
@@ -561,10 +562,11 @@ struct OpFirmUnits {
   uint64_t units;
   uint32_t kcols;
   double *__restrict__ out;
+  template <int STEP = Shape<K>::V>
   __device__ double partial(const uint4 &d, int j, int s) const {
     double acc = 0.0;
     if (j >= (int)kcols) return acc;
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) {
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) {
       const uint32_t c = fcells[k];
       for (uint64_t q = cu_ptr[c]; q < cu_ptr[c + 1]; ++q) acc += (double)atoms[(uint64_t)cu[q] * K + j];
     }
@@ -1010,11 +1012,12 @@ struct OpFirmTargetScores {
   const double *__restrict__ dir, *__restrict__ cell_target, *__restrict__ totals;
   double target_total;
   double *__restrict__ out;
+  template <int STEP = Shape<2 * KT>::V>
   __device__ double partial(const uint4 &d, int l, int s) const {
     double acc = 0.0;
     if ((l & 1) == 0) return acc;
     const int j = l / 2;
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<2 * KT>::V) {
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) {
       const uint32_t c = fcells[k];
       acc += dir[(uint64_t)c * KT + j] - cell_target[c] / target_total * totals[j];
     }

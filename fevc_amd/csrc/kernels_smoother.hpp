@@ -216,9 +216,10 @@ struct OpAdjacency {
   const double *__restrict__ scale;
   double *__restrict__ out;
   bool accumulate;
+  template <int STEP = Shape<K>::V>
   __device__ double partial(const uint4 &d, int j, int s) const {
     double acc = 0.0;
-    for (uint32_t k = d.y + s; k < d.z; k += Shape<K>::V) acc += val[k] * x[(uint64_t)col[k] * K + j];
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * x[(uint64_t)col[k] * K + j];
     return acc;
   }
   __device__ bool enabled() const { return true; }

@@ -1054,6 +1054,22 @@ int32_t vckss_mi355x_layout_v1(uint64_t handle, uint32_t level, uint32_t chunk, 
   });
 }
 
+int32_t vckss_mi355x_chunk_table_v1(const uint32_t *ptr, uint64_t segments, uint32_t chunk, uint32_t group,
+                                    uint64_t *sizes, uint32_t *desc, uint32_t *mrange) {
+  return guarded([&] {
+    if (!sizes || (!ptr && segments)) fail(kInvalidInput, "chunk_table", "null pointer");
+    std::vector<uint32_t> p(segments + 1, 0);
+    if (segments) std::memcpy(p.data(), ptr, p.size() * 4);
+    for (uint64_t s = 0; s < segments; ++s)
+      if (p[s + 1] < p[s]) fail(kInvalidInput, "chunk_table", "segment pointers must not decrease");
+    const ChunkTable t = chunk_segments(p, chunk, group);
+    sizes[0] = t.chunks();
+    sizes[1] = t.multi();
+    if (desc && !t.desc.empty()) std::memcpy(desc, t.desc.data(), t.desc.size() * 4);
+    if (mrange && !t.mrange.empty()) std::memcpy(mrange, t.mrange.data(), t.mrange.size() * 4);
+  });
+}
+
 int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] { g_hierarchies.erase(handle); });

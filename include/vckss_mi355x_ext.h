@@ -162,6 +162,14 @@ int32_t vckss_mi355x_layout_v1(uint64_t handle, uint32_t level, uint32_t chunk, 
                                uint32_t *row_multi, uint32_t *agg_chunks, uint32_t *agg_multi,
                                double *terminal_inverse);
 
+/* Host only (no GPU): the chunk table of `segments` segments with pointers
+ * ptr[0..segments] for chunks of at most `chunk` entries; with group > 1,
+ * runs of up to `group` consecutive segments of at most chunk / 4 entries
+ * share one chunk tagged 0xffffff00 + count.  sizes: chunks, multi-chunk
+ * segments.  Call with null desc / mrange to read the sizes first. */
+int32_t vckss_mi355x_chunk_table_v1(const uint32_t *ptr, uint64_t segments, uint32_t chunk, uint32_t group,
+                                    uint64_t *sizes, uint32_t *desc, uint32_t *mrange);
+
 /* Host preparation only (no GPU): same as the prepare entry point but stops
  * before device upload; the generation can be queried with dims/arrays of
  * the prepared problem (which 20..): 20 unit_rank 21 unit_count

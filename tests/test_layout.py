@@ -116,6 +116,64 @@ def test_layout_invariants(oracle, case):
         assert any(len(l["row_multi"]) for l in lays)
 
 
+def _check_grouped(chunks, multi, ptr, cap, group):
+    """Grouped chunk table: every segment is finished exactly once, either in
+    a group of 2..group consecutive short segments (at most cap / 4 entries
+    each) or by the plain single/multi chunks."""
+    segs = len(ptr) - 1
+    tag = chunks[:, 3]
+    is_group = (tag > 0xFFFFFF00) & (tag != 0xFFFFFFFF)
+    owner = np.zeros(segs, np.int64)
+    for s, b, e, m in chunks[is_group]:
+        c = int(m) - 0xFFFFFF00
+        assert 2 <= c <= group
+        assert b == ptr[s] and e == ptr[s + c]
+        assert np.all(np.diff(ptr[s:s + c + 1]) <= cap // 4)
+        owner[s:s + c] += 1
+    plain = chunks[~is_group]
+    grouped = owner > 0
+    assert np.all(owner <= 1)
+    assert np.all(np.isin(plain[:, 0], np.flatnonzero(grouped), invert=True))
+    rest = np.flatnonzero(~grouped)
+    assert np.array_equal(np.unique(plain[:, 0]), rest)
+    # the plain chunks of the other segments are those of the ungrouped table
+    full, _ = F.chunk_table(ptr, cap, 1)
+    keep = np.isin(full[:, 0], rest)
+    assert np.array_equal(plain[:, :3], full[keep, :3])
+    for q in np.flatnonzero(plain[:, 3] != 0xFFFFFFFF):
+        first, end = multi[plain[q, 3]]
+        idx = np.flatnonzero(~is_group)[q]
+        assert first <= idx < end and np.all(chunks[first:end, 0] == plain[q, 0])
+
+
+@pytest.mark.parametrize("group", [1, 2, 4, 16])
+def test_grouped_chunk_tables(oracle, group):
+    rng = np.random.default_rng(group)
+    deg = np.concatenate([rng.integers(0, 20, 3000), [0, 0, 0, 500, 5000, 1, 16, 17, 64, 65]])
+    rng.shuffle(deg)
+    ptrs = [np.concatenate([[0], np.cumsum(deg)]), np.zeros(1, np.int64), np.array([0, 0, 0, 3]),
+            np.concatenate([[0], np.cumsum(np.full(37, 16))])]
+    n, u, v, w = _panel_graph(oracle)
+    lays = F.hierarchy_layout(n, u, v, w, chunk=64)
+    ptrs += [lays[0]["row_ptr"].astype(np.int64), lays[0]["agg_ptr"].astype(np.int64)]
+    for ptr in ptrs:
+        for cap in (16, 64):
+            chunks, multi = F.chunk_table(ptr, cap, group)
+            if group == 1:
+                _check_chunks(chunks, multi, ptr, cap)
+            else:
+                _check_grouped(chunks, multi, ptr, cap, group)
+    # the finest level of a panel has mostly short rows: grouping cuts the chunk count
+    full, _ = F.chunk_table(lays[0]["row_ptr"], 64, 1)
+    grouped, _ = F.chunk_table(lays[0]["row_ptr"], 64, 4)
+    if group == 4:
+        assert len(grouped) < 0.7 * len(full)
+    with pytest.raises(F.EngineError):
+        F.chunk_table(np.array([0, 5, 3]), 64, group)
+    with pytest.raises(F.EngineError):
+        F.chunk_table(np.array([0, 5]), 64, 0)
+
+
 def _vcycle(lays, levels, r0):
     """numpy restatement of the device V-cycle over the renumbered layout."""
     def matvec(lay, x):
