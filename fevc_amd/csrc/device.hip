@@ -65,6 +65,7 @@ struct Cycle {
     const uint32_t nx = (E.sm.shaped && L.next) ? L.next : (uint32_t)l + 1;
     const bool composite = nx != l + 1;
     const uint32_t *lab = composite ? L.clab.p : L.lab.p;
+    const uint32_t *pcol = composite ? L.cpcol.p : L.pcol.p;
     constexpr bool g = grouped<KV>();
     const DevChunks &rows = g ? L.rows_g : L.rows;
     const DevChunks &aggs = composite ? (g ? L.caggs_g : L.caggs) : (g ? L.aggs_g : L.aggs);
@@ -101,12 +102,12 @@ struct Cycle {
                           K * s * (3.0 * L.n + Lc.n);
         }
         segments<VT, KV, 0>(E, rows,
-                            dev::OpProlongSmooth<VT, KV, true>{{L.col.p, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p),
+                            dev::OpProlongSmooth<VT, KV, true>{{pcol, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p),
                                                                inv(L), lab, vp(v.y.p)},
                             a, e);
       } else {
         segments<VT, KV, 0>(E, rows,
-                            dev::OpProlongSmooth<VT, KV>{{L.col.p, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p), inv(L),
+                            dev::OpProlongSmooth<VT, KV>{{pcol, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p), inv(L),
                                                          lab, vp(v.y.p)});
       }
       std::swap(v.x, v.y);
@@ -509,6 +510,8 @@ void relayout(Impl &E, uint32_t chunk) {
     D.rows_g.upload(chunk_segments(Y.row_ptr, chunk, kSegmentGroup));
     if (!Y.lab.empty()) {
       D.lab.upload(Y.lab);
+      D.pcol.alloc(D.nnz);
+      hipLaunchKernelGGL(dev::k_compose, grid_for(D.nnz), kBlock, 0, E.stream, D.nnz, D.col.p, D.lab.p, D.pcol.p);
       D.aggs.upload(Y.aggs);
       D.aggs_g.upload(chunk_segments(Y.agg_ptr, chunk, kSegmentGroup));
     }

@@ -70,7 +70,7 @@ struct DBuf {
 
 // Grid for `threads` threads, capped, rounded up to a multiple of the 8 XCDs
 // (the segment kernels map block ranges per XCD).
-inline unsigned grid_for(uint64_t threads, unsigned cap = 8192) {
+inline unsigned grid_for(uint64_t threads, unsigned cap = VK_GRID_CAP) {
   uint64_t g = (threads + kBlock - 1) / kBlock;
   g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
   return static_cast<unsigned>((g + 7) / 8 * 8);
@@ -131,12 +131,14 @@ struct DevLevel {
   uint32_t next = 0;
   DBuf<uint32_t> clab;
   DevChunks caggs, caggs_g;
+  // coarse column of every CSR entry, lab[col[k]] and clab[col[k]]
+  DBuf<uint32_t> pcol, cpcol;
 };
 
 // Short segments per grouped chunk.  Operations with at least this many
 // slices per chunk (dev::Shape<K>::V) use the grouped tables: each slice then
 // walks its own short row, instead of all slices sharing one.
-constexpr uint32_t kSegmentGroup = 4;
+constexpr uint32_t kSegmentGroup = VK_SEG_GROUP;
 template <int K>
 constexpr bool grouped() { return dev::Shape<K>::V >= (int)kSegmentGroup; }
 
@@ -298,6 +300,8 @@ inline void set_cycle_shape(DeviceEngine::Impl &E, const std::vector<uint32_t> &
     for (uint32_t x : lab) ptr[x + 1]++;
     for (uint32_t c = 0; c < nb; ++c) ptr[c + 1] += ptr[c];
     D.clab.upload(lab);
+    D.cpcol.alloc(D.nnz);
+    hipLaunchKernelGGL(dev::k_compose, grid_for(D.nnz), kBlock, 0, E.stream, D.nnz, D.col.p, D.clab.p, D.cpcol.p);
     const ChunkTable table = chunk_segments(ptr, E.chunk);
     D.caggs.upload(table);
     D.caggs_g.upload(chunk_segments(ptr, E.chunk, kSegmentGroup));

@@ -27,6 +27,18 @@
 namespace vk {
 namespace dev {
 
+// Build-time tuning knobs (measured defaults): unroll of the CSR walk, grid
+// cap of streaming launches, short segments per grouped chunk.
+#ifndef VK_SEG_UNROLL
+#define VK_SEG_UNROLL 4
+#endif
+#ifndef VK_GRID_CAP
+#define VK_GRID_CAP 8192
+#endif
+#ifndef VK_SEG_GROUP
+#define VK_SEG_GROUP 4
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 4096;
 constexpr int kMaxReduceValues = 6;
@@ -335,7 +347,7 @@ struct CsrOp {
     const auto ln = op.lane(j);
     const T xi = op.self(d.x, j, ln);
     T acc = T(0);
-#pragma unroll 8
+#pragma unroll VK_SEG_UNROLL
     for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));
     return acc;
   }
@@ -407,7 +419,9 @@ struct OpResidual : CsrOp<T, K, OpResidual<T, K>> {
 // y = x' + D^-1 (b - A x'), x' = x + P e  (prolong-add, matvec, Jacobi).
 // With r = b - A x kept from the restriction, A x' = (b - r) + A P e, so
 // y = x' + D^-1 (r - A P e): the walk gathers only the coarse correction e
-// (small, cache resident), never the fine iterate.  `Finest` only names the
+// (small, cache resident), never the fine iterate.  The CSR columns handed to
+// this operator are already coarse (lab[col[k]], composed once per layout),
+// so a neighbour costs one dependent load, not two.  `Finest` only names the
 // finest-level instance (its own kernel symbol for profiling).
 template <typename T, int K, bool Finest = false>
 struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
@@ -416,7 +430,7 @@ struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
   const uint32_t *__restrict__ lab;
   T *__restrict__ y;
   __device__ T self(uint32_t i, int j, NoLane = {}) const { return e[(uint64_t)lab[i] * K + j]; }
-  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return e[(uint64_t)lab[c] * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return e[(uint64_t)c * K + j]; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ape, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
@@ -526,6 +540,14 @@ __global__ void k_to_double(uint64_t count, const float *__restrict__ a, double 
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
        t += (uint64_t)gridDim.x * blockDim.x)
     b[t] = (double)a[t];
+}
+
+// Index composition out[k] = map[a[k]] (coarse column of every CSR entry).
+static __global__ void k_compose(uint64_t count, const uint32_t *__restrict__ a, const uint32_t *__restrict__ map,
+                                 uint32_t *__restrict__ out) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    out[t] = map[a[t]];
 }
 
 // Row permutation of K-lane vectors: out[p] = in[perm[p]] (gather).
