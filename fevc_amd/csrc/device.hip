@@ -71,21 +71,31 @@ struct Cycle {
     const DevChunks &aggs = composite ? (g ? L.caggs_g : L.caggs) : (g ? L.aggs_g : L.aggs);
     DevLevel &Lc = E.levels[nx];
     LevelVecs<T> &c = V[nx];
+    // The packed FP32 cycle is never centred (mean == nullptr): b is braw
+    // itself, and the producer of braw also wrote x0 = D^-1 braw into v.x.
+    if (kPacked && mean) fail(kInvariant, "vcycle", "the packed FP32 cycle takes uncentred right-hand sides");
+    const T *bb = kPacked ? braw : b;
     for (uint32_t it = 0; it < iterations; ++it) {
       if (it == 0) {
-        segments<VT, KV, 0>(E, rows, dev::OpScaleResidual<VT, KV>{{L.col.p, val(L)}, vp(braw), mean, inv(L),
-                                                                      vp(b), vp(v.x.p), vp(v.r.p)});
+        if constexpr (kPacked)
+          segments<VT, KV, 0>(E, rows,
+                              dev::OpFirstResidual<VT, KV>{{L.col.p, val(L)}, vp(braw), vp(v.x.p), vp(v.r.p)});
+        else
+          segments<VT, KV, 0>(E, rows, dev::OpScaleResidual<VT, KV>{{L.col.p, val(L)}, vp(braw), mean, inv(L),
+                                                                        vp(b), vp(v.x.p), vp(v.r.p)});
       } else {
         segments<VT, KV, 0>(E, rows,
-                            dev::OpSmooth<VT, KV>{{L.col.p, val(L)}, vp(b), vp(v.x.p), inv(L), vp(v.y.p)});
+                            dev::OpSmooth<VT, KV>{{L.col.p, val(L)}, vp(bb), vp(v.x.p), inv(L), vp(v.y.p)});
         std::swap(v.x, v.y);
-        segments<VT, KV, 0>(E, rows, dev::OpResidual<VT, KV>{{L.col.p, val(L)}, vp(b), vp(v.x.p), vp(v.r.p)});
+        segments<VT, KV, 0>(E, rows, dev::OpResidual<VT, KV>{{L.col.p, val(L)}, vp(bb), vp(v.x.p), vp(v.r.p)});
       }
       // The coarse right-hand side sums residuals of a Laplacian (zero row
       // sums) of a centred right-hand side, so its lane means vanish up to
       // rounding: the reference's coarse centring (cycle.rs:459-478) is a
       // rounding-level projection here and is not recomputed.
-      segments<VT, KV, 0>(E, aggs, dev::OpAggregate<VT, KV>{vp(v.r.p), vp(c.braw.p), nullptr, Lc.n});
+      const bool x0 = kPacked && !Lc.terminal;  // the child's first iterate, written with its braw
+      segments<VT, KV, 0>(E, aggs, dev::OpAggregate<VT, KV>{vp(v.r.p), vp(c.braw.p), nullptr, Lc.n,
+                                                            x0 ? inv(Lc) : nullptr, x0 ? vp(c.x.p) : nullptr});
       level(nx, E.levels[nx - 1].repeat, nullptr);
       if (l == 0) {
         // the finest prolong-smooth pass is the kernel the roofline reports:
@@ -98,7 +108,7 @@ struct Cycle {
           // algorithmic bytes: CSR once (values, columns, chunk descriptors),
           // inverse diagonal and labels, r and x read, y written, e read
           const double s = sizeof(T);
-          E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * L.rows.count + (s + 4) * L.n +
+          E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * rows.count + (s + 4) * L.n +
                           K * s * (3.0 * L.n + Lc.n);
         }
         segments<VT, KV, 0>(E, rows,
@@ -131,9 +141,10 @@ CycleOut apply_cycle_raw(Impl &E, const double *r, const double *mean_r, double 
     return {E.lv64[0].x.p, nullptr};
   }
   const uint64_t m0 = E.n0 * K;
-  hipLaunchKernelGGL((dev::k_centre_split<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, r, mean_r, rc,
-                     E.lv32[0].braw.p, E.gate);
   Cycle<float, K> cyc{E, E.lv32, E.lv32[0].braw.p, E.lv32[0].b.p};
+  const bool x0 = Cycle<float, K>::kPacked;  // level 0's first iterate, written with its braw
+  hipLaunchKernelGGL((dev::k_centre_split<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, r, mean_r, rc,
+                     E.lv32[0].braw.p, x0 ? cyc.inv(E.levels[0]) : nullptr, x0 ? E.lv32[0].x.p : nullptr, E.gate);
   cyc.level(0, 1, nullptr);
   return {nullptr, E.lv32[0].x.p};
 }

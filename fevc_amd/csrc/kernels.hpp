@@ -387,6 +387,23 @@ struct OpScaleResidual : CsrOp<T, K, OpScaleResidual<T, K>> {
   }
 };
 
+// First visit of an uncentred level whose x0 = D^-1 braw was written by the
+// producer of braw (the restriction, or the FP64 -> FP32 split on level 0):
+// r = braw - A x0.  b is braw itself and x is x0, so neither is rewritten,
+// and a neighbour costs one gather instead of two.
+template <typename T, int K>
+struct OpFirstResidual : CsrOp<T, K, OpFirstResidual<T, K>> {
+  const T *__restrict__ braw, *__restrict__ x0;
+  T *__restrict__ r;
+  __device__ T self(uint32_t i, int j, NoLane = {}) const { return x0[(uint64_t)i * K + j]; }
+  __device__ T nbr(uint32_t c, int j, NoLane = {}) const { return x0[(uint64_t)c * K + j]; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
+    const uint64_t q = (uint64_t)i * K + j;
+    r[q] = braw[q] - ax;
+  }
+};
+
 // Later visits: y = x + D^-1 (b - A x)
 template <typename T, int K>
 struct OpSmooth : CsrOp<T, K, OpSmooth<T, K>> {
@@ -452,13 +469,17 @@ struct OpProduct : CsrOp<T, K, OpProduct<T, K>> {
 };
 
 // Restriction: coarse[c] = sum over the aggregate's contiguous members of r
-// (coarsen.rs:476-503), plus the per-lane coarse mean for the centring.
+// (coarsen.rs:476-503), plus the per-lane coarse mean for the centring.  With
+// `x0`, also the coarse level's first iterate x0 = D_c^-1 coarse (when the
+// coarse level is not centred).
 template <typename T, int K>
 struct OpAggregate {
   const T *__restrict__ r;
   T *__restrict__ out;
   double *__restrict__ mean;
   uint32_t coarse;
+  const scalar_t<T> *__restrict__ inv = nullptr;
+  T *__restrict__ x0 = nullptr;
   template <int STEP = Shape<K>::V>
   __device__ T partial(const uint4 &d, int j, int s) const {
     T acc = T(0);
@@ -469,6 +490,7 @@ struct OpAggregate {
   template <int NA>
   __device__ void finish(uint32_t c, int j, T sum, double (&acc)[NA]) const {
     out[(uint64_t)c * K + j] = sum;
+    if (x0) x0[(uint64_t)c * K + j] = inv[c] * sum;
     if constexpr (!std::is_same_v<T, float4>) acc[0] += (double)sum;
   }
   template <int NV>
@@ -521,16 +543,18 @@ static __global__ void k_stream_copy(uint64_t count, const double4 *__restrict__
 
 // ---------------------------------------------------------------- precision casts
 // rc = r - mean (FP64, kept for the Krylov recurrences) and its FP32 copy for
-// the V-cycle.
+// the V-cycle; with `inv`, also the cycle's first iterate x0 = D^-1 rf.
 template <int K>
 __global__ void k_centre_split(uint64_t rows, const double *__restrict__ r, const double *__restrict__ mean,
-                               double *__restrict__ rc, float *__restrict__ rf, const int *__restrict__ gate) {
+                               double *__restrict__ rc, float *__restrict__ rf, const float *__restrict__ inv,
+                               float *__restrict__ x0, const int *__restrict__ gate) {
   if (gate && *gate == 0) return;
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < rows * K;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const double v = mean ? r[t] - mean[t % K] : r[t];
     rc[t] = v;
     rf[t] = (float)v;
+    if (inv) x0[t] = inv[t / K] * (float)v;
   }
 }
 template <int K>
