@@ -211,6 +211,7 @@ SIGNATURES = [
     ("vckss_rust_engine_numerical_cmg_work_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_exact_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_params_v1", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_omega_v1", C.c_int32,
@@ -363,6 +364,26 @@ class Engine:
             req.interrupt_poll = C.cast(cb, C.c_void_p)
         _check(library().vckss_rust_engine_solve_interrupt_v5(self.generation, C.byref(req)))
         return self.result()
+
+    def solve_exact(self, fit_tolerance=1e-10, exact_limit=None, algorithm=1, poll=None, **overrides):
+        """Dense exact estimator (algorithm 1 = EXACT; 0 = AUTO routes here when
+        W + F - 1 <= exact_limit): (result, receipt)."""
+        kw = dict(overrides)
+        if exact_limit is not None:
+            kw["exact_estimator_limit"] = exact_limit
+        res = self.solve(fit_tolerance=fit_tolerance, poll=poll, algorithm=algorithm, **kw)
+        return res, self.exact_receipt()
+
+    def exact_receipt(self):
+        out = np.zeros(13)
+        _check(library().vckss_mi355x_exact_receipt_v1(self.generation, out.ctypes.data, 13))
+        keys = ("parameters", "deletion_units", "cells", "max_leverage", "deletion_rank_gap",
+                "information_rcond", "inverse_relres", "fit_relres", "fit_tolerance",
+                "firm_zero_sum_residual", "weighted_rss", "seconds_factor", "seconds_cells")
+        d = dict(zip(keys, out.tolist()))
+        for k in ("parameters", "deletion_units", "cells"):
+            d[k] = int(d[k])
+        return d
 
     def numerical_request(self, **kw):
         return default_numerical_request(**kw)

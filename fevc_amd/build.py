@@ -14,7 +14,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["device.hip", "smoother.hip", "host_prepare.cpp", "host_cmg.cpp", "numerical.cpp", "vckss_abi.cpp",
+SOURCES = ["device.hip", "smoother.hip", "exact.hip", "host_prepare.cpp", "host_cmg.cpp", "numerical.cpp", "vckss_abi.cpp",
            "vckss_abi_refusals.cpp"]
 PARAMS = os.path.join(HERE, "smoother_params.json")
 OUTPUT = os.path.join(HERE, "libvckss_mi355x.so")

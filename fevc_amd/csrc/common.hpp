@@ -44,6 +44,8 @@ enum Code : int32_t {
   kNonfiniteCorrected = 79,
   kStaleContext = 80,
   kContextPoisoned = 81,
+  kSingularInformation = 82,
+  kInverseResidual = 83,
   kInvariant = 90,
   kPanic = 99,
 };

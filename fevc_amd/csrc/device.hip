@@ -360,6 +360,8 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   }
 }
 
+}  // namespace
+
 // jla.rs:286-364 on the host (one pass over the rows after the full fit).
 void plugin_components(const Prepared &P, const std::vector<double> &alpha, const std::vector<double> &psi,
                        double out[4]) {
@@ -394,6 +396,8 @@ double component_identity(const double c[4]) {
   const double scale = std::max({std::fabs(c[0]), std::fabs(c[1]), std::fabs(2.0 * c[2]), std::fabs(c[3]), 1.0});
   return std::fabs(c[3] - implied) / scale;
 }
+
+namespace {
 
 // Host permutation of K-lane vectors between original and renumbered order.
 // Numerical MCSE replay (engine/numerical.rs:160-330): the leverage probes

@@ -25,6 +25,15 @@ struct SolveOptions {
   double block_tolerance = 1e-10;
   int precision = 64;             // 64: FP64 V-cycle; 32: FP32 V-cycle, FP64 Krylov
   bool numerical = false;         // numerical MCSE attachment (replayed leverage probes)
+  bool exact = false;             // dense exact estimator instead of JLA probes
+};
+
+// Receipt of the dense exact estimator (exact_estimator.rs:127-153).
+struct ExactReceipt {
+  uint64_t parameters = 0, deletion_units = 0, cells = 0;
+  double max_leverage = 0, deletion_rank_gap = 0, information_rcond = 0, inverse_relres = 0,
+         fit_relres = 0, fit_tolerance = 0, firm_zero_sum_residual = 0;
+  double seconds_factor = 0, seconds_cells = 0;
 };
 
 // Numerical MCSE attachment (engine/numerical.rs:13-32, numerical_api.rs:54-90).
@@ -60,6 +69,8 @@ struct SolveResult {
   uint64_t vcycles = 0;
   bool has_numerical = false;
   NumericalResult numerical;
+  bool has_exact = false;
+  ExactReceipt exact;
 };
 
 bool device_available();
@@ -69,6 +80,9 @@ class DeviceEngine {
   DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h);
   ~DeviceEngine();
   SolveResult run(const SolveOptions &opt);
+  // Dense exact estimator (match deletion, joint nuisance, no controls):
+  // exact leverages and target bilinear forms from the inverse information.
+  SolveResult run_exact(const SolveOptions &opt);
 
   // Diagnostics used by the parity tests (no counterpart in the reference).
   void atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint32_t columns, bool target,

@@ -329,6 +329,16 @@ struct PlainCycle {
 
 using Impl = DeviceEngine::Impl;
 
+}  // namespace gpu
+
+// Plug-in components of worker and firm effects (jla.rs:286-364) and the
+// relative accounting residual of a component vector.
+void plugin_components(const Prepared &P, const std::vector<double> &alpha, const std::vector<double> &psi,
+                       double out[4]);
+double component_identity(const double c[4]);
+
+namespace gpu {
+
 // Final per-lane reduction of `items` block partials (one block per lane).
 template <int K, int NV, class Fin>
 void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, const int *gate = nullptr) {

@@ -572,6 +572,7 @@ DeviceEngine::SmootherStatus DeviceEngine::smoother_status() const {
 // them (curvature or preconditioner breakdown) is solved again from the start
 // with the plain cycle, which then stays in use.
 SolveResult DeviceEngine::run(const SolveOptions &opt) {
+  if (opt.exact) return run_exact(opt);
   Impl &E = *impl_;
   SmootherState &S = E.sm;
   if (S.mode == 1 && !S.calibrated) smoother_mode(1, S.params, opt.precision);

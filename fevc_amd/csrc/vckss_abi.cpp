@@ -149,8 +149,16 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
   const auto &v2 = v3.v2;
   const auto &v1 = v2.v1;
   // ffi_engine.rs:10335-10425 eligibility, restricted to the compressed path
+  // ffi_engine.rs:5712-5728: AUTO takes the exact estimator when the
+  // identified dimension W + F - 1 is within exact_estimator_limit.
+  // Without a prepared problem (preflight) AUTO stays unresolved.
+  const bool prepared = ctx.P.workers > 0 && ctx.P.firms > 0;
+  const uint64_t parameters = prepared ? ctx.P.workers + ctx.P.firms - 1 : 0;
+  const bool exact = v2.algorithm == VCKSS_ALGORITHM_EXACT ||
+                     (prepared && v2.algorithm == VCKSS_ALGORITHM_AUTO && parameters <= v2.exact_estimator_limit);
   const bool eligible =
-      v2.algorithm == VCKSS_ALGORITHM_JLA &&
+      (v2.algorithm == VCKSS_ALGORITHM_JLA || v2.algorithm == VCKSS_ALGORITHM_AUTO ||
+       v2.algorithm == VCKSS_ALGORITHM_EXACT) &&
       (v3.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED || v3.engine == VCKSS_ENGINE_GENERIC) &&
       v1.deletion_mode == VCKSS_DELETION_MATCH && v2.nuisance_mode == VCKSS_NUISANCE_JOINT &&
       v1.rng_contract == VCKSS_RNG_COUNTER_V1 &&
@@ -166,10 +174,20 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
       (v3.frequency_use == VCKSS_REQUEST_FREQUENCY_UNIT || v3.frequency_use == VCKSS_REQUEST_FREQUENCY_LITERAL);
   if (!eligible)
     fail(kUnsupported, "cmg_full_v2",
-         "the MI355X engine requires JLA, auto/generic engine, auto/CMG solver, match deletion, joint nuisance, "
-         "movers, automatic batches and no probe order");
-  if (v1.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+         "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match deletion, joint "
+         "nuisance, movers, automatic batches and no probe order");
   SolveOptions o;
+  o.exact = exact;
+  if (exact) {
+    // ffi_engine.rs:5733-5751: dimension and block-size limits of the exact route
+    if (prepared && parameters > v2.exact_estimator_limit)
+      fail(kResourceLimit, "exact_estimator", "identified coefficient dimension exceeds exact_limit()");
+    for (uint64_t g = 0; g < ctx.P.units; ++g)
+      if (ctx.P.unit_ptr[g + 1] - ctx.P.unit_ptr[g] > v2.blocksize_limit)
+        fail(kResourceLimit, "exact_estimator", "a deletion block exceeds blocksize_limit()");
+  } else if (v1.probes < 2) {
+    fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+  }
   o.seed = v1.seed;
   o.probes = v1.probes;
   o.fit_tolerance = v1.pcg_tolerance;
@@ -511,6 +529,8 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   // numerical_api.rs:157-161 admits the compressed engine for the attachment
   if (point.v3.engine == VCKSS_ENGINE_COMPRESSED) point.v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
   SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
+  if (o.exact)
+    fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
   o.numerical = true;
   run_options(ctx, o, poll, poll_ctx);
 }
@@ -884,6 +904,22 @@ int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint
     uint32_t v[10] = {(uint32_t)s.mode, s.active ? 1u : 0u, s.breakdown ? 1u : 0u, s.plain_iterations,
                       s.learned_iterations, s.shaped ? 1u : 0u, UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
     for (size_t k = 0; k < s.calibration.size() && k < 4; ++k) v[6 + k] = s.calibration[k];
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!out || capacity < 13) fail(kAbiMismatch, "exact_receipt", "capacity must be at least 13");
+    if (!ctx.has_result) fail(kStaleContext, "exact_receipt", "no solved result is available");
+    if (!ctx.result.has_exact) fail(kUnsupported, "exact_receipt", "the result is not an exact estimate");
+    const ExactReceipt &x = ctx.result.exact;
+    const double v[13] = {(double)x.parameters, (double)x.deletion_units, (double)x.cells, x.max_leverage,
+                          x.deletion_rank_gap, x.information_rcond, x.inverse_relres, x.fit_relres,
+                          x.fit_tolerance, x.firm_zero_sum_residual, ctx.result.weighted_rss, x.seconds_factor,
+                          x.seconds_cells};
     std::memcpy(out, v, sizeof v);
   });
 }

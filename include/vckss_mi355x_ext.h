@@ -105,6 +105,15 @@ int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const doubl
  * cycle shape in use, then the calibration iterations of the reference cycle,
  * the learned weights, the tuned shape and both (UINT32_MAX if not tried). */
 int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity);
+/* Receipt of an exact-route solve (algorithm EXACT, or AUTO within
+ * exact_estimator_limit; exact_estimator.rs:127-153): out[13] = identified
+ * parameters, deletion units, cells, maximum block leverage, smallest block
+ * maker eigenvalue, certified lower bound of the scaled quotient reciprocal
+ * condition number, scaled inverse residual, complete fit residual and its
+ * tolerance, firm zero-sum residual, weighted RSS, seconds for the
+ * factorisation and fit, seconds for the cell pass.  11 if the result is a
+ * JLA estimate. */
+int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t capacity);
 /* Levels the cycle visits (all levels unless the tuned shape is in use). */
 int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint32_t capacity, uint32_t *count);
 /* Built-in trained parameters (count = 0 when the library carries none). */
