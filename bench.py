@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
     ap.add_argument("--compare-plain", type=int, default=1,
                     help="also run the KSS estimate on the plain single-RHS FP64 path and report the error")
+    ap.add_argument("--exact", type=int, default=1, help="time the dense exact estimator on the small panel")
     ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--scale", type=float, default=1.0, help="panel size factor (1.0 = full size)")
@@ -202,6 +203,31 @@ def main():
             log(f"plain FP64 KSS: {pw:.2f}s; component rel. error {rel}; leverage abs. error {lev}")
     eng.release()
 
+    exact = None
+    if args.exact:
+        # dense exact estimator on the small plain-method panel (2000 workers x
+        # 5 periods, 200 firms, mobility 0.2, seed 1): exact leverages, EXACT
+        # algorithm with the dimension limit raised above its W + F - 1
+        small = synth.PanelSpec(workers=2000, firms=200, periods=5, mobility=0.2, seed=1,
+                                name="akm-2k-workers-200-firms-T5")
+        sd = synth.generate(small)
+        se = F.Engine(*[np.asarray(sd[k]) for k in ("worker", "firm", "deletion", "outcome", "frequency",
+                                                    "target")])
+        try:
+            se.solve_exact(exact_limit=10000)  # warm (device set-up, code objects)
+            t0 = time.perf_counter()
+            res, rec = se.solve_exact(exact_limit=10000)
+            ew = time.perf_counter() - t0
+        finally:
+            se.release()
+        exact = {"panel": small.name, "parameters": rec["parameters"], "deletion_units": rec["deletion_units"],
+                 "wall_seconds": round(ew, 4), "factor_fit_seconds": round(rec["seconds_factor"], 4),
+                 "cell_pass_seconds": round(rec["seconds_cells"], 4),
+                 "max_leverage": rec["max_leverage"], "information_rcond_lower_bound": rec["information_rcond"],
+                 "corrected": res["corrected"]}
+        log(f"exact estimator ({small.name}, {rec['parameters']} parameters): {ew * 1e3:.1f} ms, "
+            f"corrected {res['corrected']}")
+
     cpu = None
     if args.cpu_solves > 0:
         from oracle import oracle as O
@@ -256,6 +282,7 @@ def main():
         "plain_fp64_single_rhs": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,
+        "exact_estimate": exact,
         "stream_copy_GBs": round(2 * copy_bytes / copy_ms / 1e6, 1),
         "prepare_seconds": round(prep_s, 2),
     }
