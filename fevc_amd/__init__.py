@@ -212,6 +212,8 @@ SIGNATURES = [
     ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_exact_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_graphs_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_params_v1", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_omega_v1", C.c_int32,
@@ -373,6 +375,14 @@ class Engine:
             kw["exact_estimator_limit"] = exact_limit
         res = self.solve(fit_tolerance=fit_tolerance, poll=poll, algorithm=algorithm, **kw)
         return res, self.exact_receipt()
+
+    def graphs(self, enable=None):
+        """Enable (True) or disable (False) HIP-graph iterations, or query
+        (None); returns (captures, graph launches) so far."""
+        c, l = C.c_uint64(), C.c_uint64()
+        mode = 2 if enable is None else int(bool(enable))
+        _check(library().vckss_mi355x_graphs_v1(self.generation, mode, C.byref(c), C.byref(l)))
+        return c.value, l.value
 
     def exact_receipt(self):
         out = np.zeros(13)

@@ -158,6 +158,113 @@ const double *apply_cycle(Impl &E, const double *r, const double *mean_r, double
   return E.z.p;
 }
 
+// ------------------------------------------------------------- HIP graphs
+// Fingerprint of everything the captured PCG kernels read as arguments or
+// launch shape: widths, flags, tolerances, buffer addresses, level tables and
+// their counts.  The V-cycle swaps each level's x and y buffers on the host
+// while it enqueues, so the pair enters as (min, max): a captured sequence is
+// self-contained (every cycle buffer it reads is written inside it), so one
+// graph serves either assignment.
+struct Fingerprint {
+  uint64_t h = 1469598103934665603ULL;
+  void add(uint64_t v) {
+    for (int k = 0; k < 8; ++k) {
+      h ^= (v >> (8 * k)) & 0xff;
+      h *= 1099511628211ULL;
+    }
+  }
+  void ptr(const void *p) { add((uint64_t)(uintptr_t)p); }
+  void real(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    add(u);
+  }
+  void chunks(const DevChunks &c) {
+    ptr(c.desc.p);
+    ptr(c.mrange.p);
+    ptr(c.ptr.p);
+    add(c.count);
+    add(c.n_multi);
+  }
+  template <typename T>
+  void pair(const DBuf<T> &a, const DBuf<T> &b) {
+    ptr(std::min(a.p, b.p));
+    ptr(std::max(a.p, b.p));
+  }
+};
+
+template <int K>
+uint64_t pcg_fingerprint(const Impl &E, const double *b, const dev::PcgParams &prm, bool fp32) {
+  Fingerprint f;
+  f.add(K);
+  f.add(fp32);
+  f.add(E.sm.active);
+  f.add(E.sm.shaped);
+  f.add(E.n0);
+  f.add(prm.n);
+  f.real(prm.rel);
+  f.real(prm.abs);
+  f.real(prm.bound);
+  f.add((uint64_t)(int64_t)prm.max_iterations);
+  f.add((uint64_t)(int64_t)prm.interval);
+  for (const void *p : {(const void *)b, (const void *)E.proj.p, (const void *)E.x.p, (const void *)E.r.p,
+                        (const void *)E.rc.p, (const void *)E.p.p, (const void *)E.ap.p, (const void *)E.z.p,
+                        (const void *)E.st.p, (const void *)E.lane_a.p, (const void *)E.lane_b.p,
+                        (const void *)E.flags.p, (const void *)E.rpartial.p, (const void *)E.bpartial.p,
+                        (const void *)E.seg_partial.p, (const void *)E.use.p, (const void *)E.ginv.p,
+                        (const void *)E.gate})
+    f.ptr(p);
+  for (const DevLevel &L : E.levels) {
+    f.add(L.n);
+    f.add(L.nnz);
+    f.add(L.terminal);
+    f.add(L.repeat);
+    f.add(L.next);
+    for (const void *p : {(const void *)L.col.p, (const void *)L.val.p, (const void *)L.val32.p,
+                          (const void *)L.inv.p, (const void *)L.inv32.p, (const void *)L.linv.p,
+                          (const void *)L.linv32.p, (const void *)L.lab.p, (const void *)L.clab.p,
+                          (const void *)L.pcol.p, (const void *)L.cpcol.p})
+      f.ptr(p);
+    for (const DevChunks *c : {&L.rows, &L.rows_g, &L.aggs, &L.aggs_g, &L.caggs, &L.caggs_g}) f.chunks(*c);
+  }
+  for (const auto &v : E.lv32) {
+    f.ptr(v.braw.p);
+    f.ptr(v.b.p);
+    f.ptr(v.r.p);
+    f.ptr(v.mean.p);
+    f.pair(v.x, v.y);
+  }
+  for (const auto &v : E.lv64) {
+    f.ptr(v.braw.p);
+    f.ptr(v.b.p);
+    f.ptr(v.r.p);
+    f.ptr(v.mean.p);
+    f.pair(v.x, v.y);
+  }
+  return f.h;
+}
+
+// Record what `enqueue` launches on the engine stream into an executable graph.
+template <class F>
+hipGraphExec_t capture(Impl &E, F &&enqueue) {
+  hipGraph_t g = nullptr;
+  VK_HIP(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue();
+  } catch (...) {
+    (void)hipStreamEndCapture(E.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  VK_HIP(hipStreamEndCapture(E.stream, &g));
+  hipGraphExec_t x = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  VK_HIP(rc);
+  E.graph_captures++;
+  return x;
+}
+
 // Lockstep certified PCG (pcg.rs:1537-1909) on the finest hybrid Laplacian
 // (renumbered).  b: n0 x K right-hand sides.  With `warm`, E.x holds the
 // initial guesses.  The host polls the any-lane-active flag with a lag of
@@ -210,9 +317,8 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
                          E.st.p, active);
     }
   };
-  precondition();
-  for (uint32_t it = 1; it <= max_it; ++it) {
-    E.iterations_launched++;
+  // iteration body: A p, update, centring, fresh-residual gate, limit
+  auto body = [&]() {
     segments<double, K, 1>(E, rows0, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
     hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
                        E.rpartial.p, E.st.p, active);
@@ -223,6 +329,46 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     segments<double, K, 3>(E, rows0,
                            dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags});
     hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm, flags);
+  };
+  // With graphs, each iteration launches the two captured sequences; the flag
+  // copy and the host poll stay between them, as without graphs.  Timed runs
+  // keep individual launches (their per-launch events).
+  Impl::PcgGraphs *graphs = nullptr;
+  if (E.use_graphs && !E.timing) {
+    const uint64_t key = pcg_fingerprint<K>(E, b, prm, fp32);
+    auto found = E.pcg_graphs.find(key);
+    if (found == E.pcg_graphs.end()) {
+      if (E.pcg_graphs.size() >= 16) {
+        VK_HIP(hipStreamSynchronize(E.stream));  // no cached graph may still be running
+        E.drop_graphs();
+      }
+      Impl::PcgGraphs g;
+      g.body = capture(E, body);
+      const uint64_t v = E.vcycles;
+      g.pre = capture(E, precondition);
+      E.vcycles = v;
+      found = E.pcg_graphs.emplace(key, g).first;
+    }
+    graphs = &found->second;
+  }
+  auto precondition_step = [&]() {
+    if (graphs) {
+      VK_HIP(hipGraphLaunch(graphs->pre, E.stream));
+      E.vcycles++;
+      E.graph_launches++;
+    } else {
+      precondition();
+    }
+  };
+  precondition_step();
+  for (uint32_t it = 1; it <= max_it; ++it) {
+    E.iterations_launched++;
+    if (graphs) {
+      VK_HIP(hipGraphLaunch(graphs->body, E.stream));
+      E.graph_launches++;
+    } else {
+      body();
+    }
     const int slot = it % kPollSlots;
     VK_HIP(hipMemcpyAsync(E.host_flags + slot, active, sizeof(int), hipMemcpyDeviceToHost, E.stream));
     VK_HIP(hipEventRecord(E.poll_events[slot], E.stream));
@@ -231,7 +377,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
       VK_HIP(hipEventSynchronize(E.poll_events[seen]));
       if (E.host_flags[seen] == 0) break;
     }
-    precondition();
+    precondition_step();
   }
   E.gate = E.always();
   std::vector<dev::ColState> host(K);
@@ -605,6 +751,7 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
 DeviceEngine::~DeviceEngine() {
   if (impl_ && impl_->stream) {
     (void)hipStreamSynchronize(impl_->stream);
+    impl_->drop_graphs();
     for (auto &e : impl_->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto &ev : impl_->poll_events)
       if (ev) (void)hipEventDestroy(ev);
@@ -978,6 +1125,20 @@ DeviceEngine::Tuning DeviceEngine::autotune(const SolveOptions &base, uint32_t b
 }
 
 void DeviceEngine::set_chunk(uint32_t chunk) { relayout(*impl_, chunk); }
+
+void DeviceEngine::set_graphs(bool enable) {
+  Impl &E = *impl_;
+  E.use_graphs = enable;
+  if (!enable) {
+    VK_HIP(hipStreamSynchronize(E.stream));
+    E.drop_graphs();
+  }
+}
+
+void DeviceEngine::graph_counters(uint64_t &captures, uint64_t &launches) const {
+  captures = impl_->graph_captures;
+  launches = impl_->graph_launches;
+}
 
 void DeviceEngine::counters(uint64_t out[6]) const {
   const Impl &E = *impl_;

@@ -105,6 +105,10 @@ class DeviceEngine {
   };
   Tuning autotune(const SolveOptions &base, uint32_t batches);
   void set_chunk(uint32_t chunk);
+  // HIP-graph capture of the PCG iteration (two graphs per configuration,
+  // cached); returns the captures and graph launches so far.
+  void set_graphs(bool enable);
+  void graph_counters(uint64_t &captures, uint64_t &launches) const;
   // Plain device-to-device stream copy of `bytes` (counter calibration);
   // returns milliseconds.
   double stream_copy(uint64_t bytes);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <cstdio>
 #include <string>
@@ -243,6 +244,22 @@ struct DeviceEngine::Impl {
   uint64_t batches = 0, refine_rounds = 0, refined_columns = 0, iterations_launched = 0;
   SmootherState sm;
   int precision_hint = 32;  // V-cycle precision of the calibration batches
+  // HIP graphs of the PCG iteration: the body (A p .. iteration limit) and
+  // the preconditioning sequence (lane means, V-cycle, rho, direction), keyed
+  // by a fingerprint of every pointer, count and flag their kernels read.
+  bool use_graphs = false;
+  struct PcgGraphs {
+    hipGraphExec_t body = nullptr, pre = nullptr;
+  };
+  std::map<uint64_t, PcgGraphs> pcg_graphs;
+  uint64_t graph_captures = 0, graph_launches = 0;
+  void drop_graphs() {
+    for (auto &g : pcg_graphs) {
+      if (g.second.body) (void)hipGraphExecDestroy(g.second.body);
+      if (g.second.pre) (void)hipGraphExecDestroy(g.second.pre);
+    }
+    pcg_graphs.clear();
+  }
 
   const int *always() const { return flags.p + dev::kFlagOne; }
 

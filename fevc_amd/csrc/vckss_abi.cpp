@@ -908,6 +908,20 @@ int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint
   });
 }
 
+int32_t vckss_mi355x_graphs_v1(uint64_t generation, uint32_t enable, uint64_t *captures, uint64_t *launches) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (enable > 2) fail(kInvalidInput, "graphs", "enable must be 0, 1 or 2 (query)");
+    DeviceEngine &dev = device_of(ctx);
+    if (enable < 2) dev.set_graphs(enable == 1);
+    uint64_t c = 0, l = 0;
+    dev.graph_counters(c, l);
+    if (captures) *captures = c;
+    if (launches) *launches = l;
+  });
+}
+
 int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t capacity) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {

@@ -105,6 +105,14 @@ int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const doubl
  * cycle shape in use, then the calibration iterations of the reference cycle,
  * the learned weights, the tuned shape and both (UINT32_MAX if not tried). */
 int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity);
+/* HIP-graph capture of the PCG iteration (no reference counterpart): enable
+ * 1 captures the iteration body and the preconditioning sequence once per
+ * configuration (cached by a fingerprint of every pointer, count and flag
+ * their kernels read) and launches the two graphs per iteration; 0 launches
+ * kernels one by one and frees the graphs; 2 only queries.  Timed runs
+ * (`vckss_mi355x_time_leverage_v1`) always launch one by one.  captures and
+ * launches (optional) count graph captures and graph launches so far. */
+int32_t vckss_mi355x_graphs_v1(uint64_t generation, uint32_t enable, uint64_t *captures, uint64_t *launches);
 /* Receipt of an exact-route solve (algorithm EXACT, or AUTO within
  * exact_estimator_limit; exact_estimator.rs:127-153): out[13] = identified
  * parameters, deletion units, cells, maximum block leverage, smallest block
