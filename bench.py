@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--compare-plain", type=int, default=1,
                     help="also run the KSS estimate on the plain single-RHS FP64 path and report the error")
     ap.add_argument("--exact", type=int, default=1, help="time the dense exact estimator on the small panel")
+    ap.add_argument("--learned-eval", type=int, default=0,
+                    help="leverage batches of the held-out 10M-worker smoother evaluation (0 skips)")
     ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--scale", type=float, default=1.0, help="panel size factor (1.0 = full size)")
@@ -228,6 +230,55 @@ def main():
         log(f"exact estimator ({small.name}, {rec['parameters']} parameters): {ew * 1e3:.1f} ms, "
             f"corrected {res['corrected']}")
 
+    learned_eval = None
+    if args.learned_eval > 0:
+        # held-out evaluation of the built-in smoother weights: 10M workers x
+        # 3 periods, 500k firms with Pareto(1.05) sizes, mobility 0.1, 256
+        # probes, reference cycle against the tuned preconditioner
+        big = synth.PanelSpec(workers=10_000_000, firms=500_000, periods=3, mobility=0.1, pareto_shape=1.05,
+                              size_cap=5000.0, seed=7, name="akm-10M-workers-500k-firms-T3")
+        log(f"learned smoother evaluation: generate {big.name}")
+        bd = synth.generate(big)
+        bcols = [bd[k] for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+        del bd
+        t0 = time.perf_counter()
+        be = F.Engine(*bcols)
+        bprep = time.perf_counter() - t0
+        del bcols
+        log(f"learned smoother evaluation: prepared in {bprep:.1f}s: {be.dims}")
+        try:
+            be.configure(args.precision, 64, 32)
+            learned_eval = {"panel": big.name, "dims": be.dims, "prepare_seconds": round(bprep, 1),
+                            "leverage_batches": args.learned_eval, "probes": 256}
+            for mode, name in ((0, "plain"), (1, "learned")):
+                be.smoother(mode)
+                entry = {}
+                if mode:
+                    st = be.smoother_status()
+                    entry["calibration"] = st["calibration"]
+                    entry["shape"] = st["shape"]
+                    entry["active"] = st["active"]
+                be.time_leverage(64, args.precision, args.tolerance, 1)
+                t0 = time.perf_counter()
+                tl = be.time_leverage(64, args.precision, args.tolerance, args.learned_eval)
+                lw = time.perf_counter() - t0
+                entry["solves_per_s"] = round(tl["solves"] / lw, 1)
+                entry["iterations_per_solve"] = round(tl["iterations"] / max(tl["solves"], 1), 2)
+                t0 = time.perf_counter()
+                res = be.solve(probes=256, tolerance=args.tolerance, fit_tolerance=args.tolerance)
+                entry["kss_wall_seconds"] = round(time.perf_counter() - t0, 3)
+                entry["corrected"] = res["corrected"]
+                learned_eval[name] = entry
+                log(f"learned smoother evaluation, {name}: {entry['solves_per_s']:.0f} solves/s, "
+                    f"{entry['iterations_per_solve']:.1f} it/solve, KSS {entry['kss_wall_seconds']:.2f}s")
+            pl, le = learned_eval["plain"], learned_eval["learned"]
+            learned_eval["solve_speedup"] = round(le["solves_per_s"] / pl["solves_per_s"], 3)
+            learned_eval["kss_speedup"] = round(pl["kss_wall_seconds"] / le["kss_wall_seconds"], 3)
+            learned_eval["max_rel_component_difference"] = max(
+                abs(a - b) / max(abs(b), 1e-300) for a, b in zip(le["corrected"], pl["corrected"]))
+        finally:
+            be.release()
+
     cpu = None
     if args.cpu_solves > 0:
         from oracle import oracle as O
@@ -283,6 +334,7 @@ def main():
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,
         "exact_estimate": exact,
+        "learned_eval_10M": learned_eval,
         "stream_copy_GBs": round(2 * copy_bytes / copy_ms / 1e6, 1),
         "prepare_seconds": round(prep_s, 2),
     }
