@@ -308,13 +308,17 @@ class Engine:
     """
 
     def __init__(self, worker, firm, deletion, outcome, frequency, target, *, upload=True,
-                 poll=None):
+                 poll=None, deletion_mode="match"):
+        """deletion_mode "match" (the default) or "observation" (the dense
+        exact route only)."""
         lib = library()
+        self.deletion_mode = {"match": 1, "observation": 2}[deletion_mode]
         cols = [_f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency), _f64(target)]
         self._cols = cols
         rows = len(cols[0])
         req = default_prepare_request(rows)
         req.options.cleanup_abandoned = 1
+        req.options.deletion_mode = self.deletion_mode
         self._poll = Poll(poll) if poll else None
         if self._poll:
             req.interrupt_poll = C.cast(self._poll, C.c_void_p)
@@ -351,6 +355,7 @@ class Engine:
         o.algorithm = 2  # JLA
         o.full_cmg_v2 = 1
         o.threads = 1
+        o.deletion_mode = self.deletion_mode
         o.seed = seed
         o.probes = probes
         o.pcg_tolerance = fit_tolerance

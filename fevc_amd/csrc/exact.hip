@@ -20,7 +20,9 @@
 // A match block's rows share one cell, so its maker I - h_c q q' (q = the
 // rows' square-root frequencies, exact_estimator.rs:1448-1696) has smallest
 // eigenvalue 1 - h_c n_g and its deleted residual sums to
-// (sum f e)/(1 - h_c n_g): the host combines the blocks in closed form.
+// (sum f e)/(1 - h_c n_g): the host combines the blocks in closed form.  Under
+// observation deletion every physical copy is a block of its own (maker
+// 1 - h_c, exact_estimator.rs:914-987).
 #include <chrono>
 #include <limits>
 
@@ -557,10 +559,28 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   std::vector<double> cell(4 * (uint64_t)C);
   VK_HIP(hipMemcpyAsync(cell.data(), dcell.p, cell.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   VK_HIP(hipStreamSynchronize(st));
-  // match blocks in closed form (exact_estimator.rs:1448-1696)
   Neumaier corr[3];
   double max_leverage = 0.0, gap = std::numeric_limits<double>::infinity();
-  for (uint64_t g = 0; g < P.units; ++g) {
+  if (P.observation) {
+    // observation deletion (exact_estimator.rs:914-987): every physical copy
+    // of a stored row is its own block, of leverage h_c and maker 1 - h_c
+    for (uint64_t r = 0; r < P.rows; ++r) {
+      const uint32_t c = P.row_cell[r];
+      const double h = cell[4 * (uint64_t)c];
+      const double maker = 1.0 - h;
+      if (!std::isfinite(h) || h < -100.0 * opt.rank_tolerance)
+        fail(kNonestimable, "exact_estimator", "physical observation leverage is invalid");
+      if (maker <= opt.block_tolerance)
+        fail(kNonestimable, "exact_estimator", "physical observation deletion has leverage at or above one");
+      const double e = P.outcome[r] - beta[P.cell_worker[c]] - beta[W + P.cell_firm[c]];
+      const double scale = (double)P.frequency[r] * P.outcome[r] * e / maker;
+      for (int k = 0; k < 3; ++k) corr[k].add(scale * cell[4 * (uint64_t)c + 1 + k]);
+      max_leverage = std::max(max_leverage, h);
+      gap = std::min(gap, maker);
+    }
+  }
+  // match blocks in closed form (exact_estimator.rs:1448-1696)
+  for (uint64_t g = 0; g < (P.observation ? 0 : P.units); ++g) {
     const uint32_t c = P.unit_cell[g];
     const double h = cell[4 * (uint64_t)c], n = (double)P.unit_count[g];
     const double maker = 1.0 - h * n;
@@ -585,7 +605,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   res.has_exact = true;
   ExactReceipt &x = res.exact;
   x.parameters = (uint64_t)W + F - 1;
-  x.deletion_units = P.units;
+  x.deletion_units = P.observation ? P.physical_total : P.units;
   x.cells = C;
   x.max_leverage = max_leverage;
   x.deletion_rank_gap = gap;

@@ -326,6 +326,73 @@ void select(const Graph &g, std::vector<uint8_t> &active, SelectionReceipt &rec)
     }
 }
 
+// graph.rs:82-242 (select_observation_deletion_graph): largest component,
+// then to a fixed point drop workers with one physical observation, else the
+// articulation workers; stayers with several copies stay.
+void select_observation(const Graph &g, std::vector<uint8_t> &active, SelectionReceipt &rec) {
+  active.assign(g.rows, 1);
+  rec.input_rows = g.rows;
+  uint64_t comps = largest_component(g, active);
+  rec.initial_components = rec.maximum_components = comps;
+  rec.initial_component_rows = count_active(active);
+  rec.mover_input_rows = count_active(active);
+  {
+    std::vector<uint8_t> seen(g.K, 0);
+    for (uint64_t r = 0; r < g.rows; ++r)
+      if (active[r] && !seen[g.row_coord[r]]) { seen[g.row_coord[r]] = 1; rec.initial_deletion_edges++; }
+  }
+  auto physical = [&]() {
+    std::vector<uint64_t> p(g.W, 0);
+    for (uint64_t r = 0; r < g.rows; ++r)
+      if (active[r]) p[g.w[r]] += g.freq[r];
+    return p;
+  };
+  const uint64_t bound = g.W + 1;
+  for (;;) {
+    if (rec.fixed_point_iterations > bound)
+      fail(kGraphCertificate, "graph", "observation-deletion pruning exceeded its finite iteration bound");
+    rec.maximum_components = std::max<uint64_t>(rec.maximum_components, largest_component(g, active));
+    if (!count_active(active)) fail(kGraphEmpty, "graph", "no component remains after observation-deletion pruning");
+    const auto phys = physical();
+    uint64_t removed = 0;
+    for (uint64_t w = 0; w < g.W; ++w) removed += phys[w] == 1;
+    if (removed) {
+      for (uint64_t r = 0; r < g.rows; ++r)
+        if (active[r] && phys[g.w[r]] == 1) active[r] = 0;
+      rec.insufficient_workers_removed += removed;
+      rec.degree_iterations++;
+      rec.fixed_point_iterations++;
+      continue;
+    }
+    auto art = articulation_workers(g, active);
+    uint64_t arts = 0;
+    for (uint8_t x : art) arts += x;
+    if (!arts) break;
+    for (uint64_t r = 0; r < g.rows; ++r)
+      if (active[r] && art[g.w[r]]) active[r] = 0;
+    rec.articulation_workers_removed += arts;
+    rec.articulation_iterations++;
+    rec.fixed_point_iterations++;
+  }
+  rec.maximum_components = std::max<uint64_t>(rec.maximum_components, largest_component(g, active));
+  for (uint64_t p : physical())
+    if (p == 1) fail(kGraphCertificate, "graph", "final observation-deletion graph still contains a physical-singleton worker");
+  {
+    auto art = articulation_workers(g, active);
+    for (uint8_t x : art)
+      if (x) fail(kGraphCertificate, "graph", "final observation-deletion graph still contains a worker articulation");
+  }
+  rec.retained_rows = count_active(active);
+  if (!rec.retained_rows) fail(kGraphEmpty, "graph", "no component remains after observation-deletion graph pruning");
+  rec.retained_physical_mass = 0;
+  std::vector<uint8_t> seen(g.K, 0);
+  for (uint64_t r = 0; r < g.rows; ++r)
+    if (active[r]) {
+      rec.retained_physical_mass += g.freq[r];
+      if (!seen[g.row_coord[r]]) { seen[g.row_coord[r]] = 1; rec.retained_deletion_edges++; }
+    }
+}
+
 int cmp_total(double a, double b) {
   int64_t x = total_key(a), y = total_key(b);
   return x < y ? -1 : (x > y ? 1 : 0);
@@ -333,7 +400,7 @@ int cmp_total(double a, double b) {
 
 }  // namespace
 
-Prepared prepare(const Columns &cols, const uint8_t *mask) {
+Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   const uint64_t n = cols.rows;
   if (n == 0) fail(kInvalidInput, "ingest", "input sample is empty");
   Prepared P;
@@ -389,8 +456,11 @@ Prepared prepare(const Columns &cols, const uint8_t *mask) {
     }
   }
   P.input_topology_checksum = fnv_checksum(n, cw.data(), cf.data(), cd.data(), cols.frequency);
+  P.observation = observation;
   if (mask) {
     P.mask.assign(mask, mask + n);
+  } else if (observation) {
+    select_observation(g, P.mask, P.selection);
   } else {
     select(g, P.mask, P.selection);
   }

@@ -33,6 +33,7 @@ struct Prepared {
   uint64_t input_topology_checksum = 0;
   std::vector<uint8_t> mask;
   SelectionReceipt selection;
+  bool observation = false;  // observation deletion (else match deletion)
 
   uint64_t rows = 0, workers = 0, firms = 0, cells = 0, units = 0, exact_strata = 0, strata = 0;
   std::vector<uint32_t> row_worker, row_firm, row_deletion, row_cell;
@@ -64,6 +65,9 @@ struct Prepared {
 
 // Validate, canonicalize, select (unless `mask` is given), compress, plan.
 // Throws vk::Error with the reference status code.
-Prepared prepare(const Columns &cols, const uint8_t *mask);
+// With `observation`, the observation-deletion graph selection (graph.rs:82-242)
+// instead of the match-deletion one; the exact estimator then deletes every
+// physical observation on its own.
+Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation = false);
 
 }  // namespace vk

@@ -114,8 +114,8 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
     fail(kAbiMismatch, "engine_prepare", "prepare request reports a short structure size");
   if (o.v3.v2.reserved || o.v3.reserved_2 || o.reserved_3 || req.reserved)
     fail(kAbiMismatch, "engine_prepare", "reserved fields must be zero");
-  if (o.v3.deletion_mode != VCKSS_DELETION_MATCH)
-    fail(kUnsupported, "engine_prepare", "the MI355X engine supports match deletion only");
+  if (o.v3.deletion_mode != VCKSS_DELETION_MATCH && o.v3.deletion_mode != VCKSS_DELETION_OBSERVATION)
+    fail(kInvalidInput, "engine_prepare", "unknown deletion mode");
   if (o.v3.controls_count != 0 || columns->v2.controls_count != 0)
     fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
   if (columns->probeorder_supplied)
@@ -134,7 +134,7 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   auto frequency = integer_column(c.frequency, rows, "frequency weight", kInvalidWeight, false);
   if (!c.outcome || !c.target_weight) fail(kInvalidInput, "engine_ingest", "outcome or target column is null");
   Columns cols{rows, worker.data(), firm.data(), deletion.data(), c.outcome, frequency.data(), c.target_weight};
-  ctx.P = prepare(cols, nullptr);
+  ctx.P = prepare(cols, nullptr, o.v3.deletion_mode == VCKSS_DELETION_OBSERVATION);
   if (req.interrupt_poll && req.interrupt_poll(req.interrupt_context) == VCKSS_INTERRUPT_USER_BREAK)
     fail(kUserBreak, "engine_prepare", "user break");
   ctx.HS = build_hybrid(ctx.P);
@@ -156,11 +156,17 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
   const uint64_t parameters = prepared ? ctx.P.workers + ctx.P.firms - 1 : 0;
   const bool exact = v2.algorithm == VCKSS_ALGORITHM_EXACT ||
                      (prepared && v2.algorithm == VCKSS_ALGORITHM_AUTO && parameters <= v2.exact_estimator_limit);
+  // ffi_engine.rs:5705-5710: the solve's deletion mode is the prepared graph's
+  if (prepared && (v1.deletion_mode == VCKSS_DELETION_OBSERVATION) != ctx.P.observation)
+    fail(kInvalidInput, "engine_solve", "solve deletion mode differs from the prepared graph mode");
+  // observation deletion: the exact route only (its JLA route is the generic engine)
+  const bool deletion_ok =
+      v1.deletion_mode == VCKSS_DELETION_MATCH || (v1.deletion_mode == VCKSS_DELETION_OBSERVATION && exact);
   const bool eligible =
       (v2.algorithm == VCKSS_ALGORITHM_JLA || v2.algorithm == VCKSS_ALGORITHM_AUTO ||
        v2.algorithm == VCKSS_ALGORITHM_EXACT) &&
       (v3.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED || v3.engine == VCKSS_ENGINE_GENERIC) &&
-      v1.deletion_mode == VCKSS_DELETION_MATCH && v2.nuisance_mode == VCKSS_NUISANCE_JOINT &&
+      deletion_ok && v2.nuisance_mode == VCKSS_NUISANCE_JOINT &&
       v1.rng_contract == VCKSS_RNG_COUNTER_V1 &&
       (v1.solver_route == VCKSS_ROUTE_AUTO || v1.solver_route == VCKSS_ROUTE_CMG_PCG) &&
       v3.batch_mode == VCKSS_BATCH_MODE_AUTO && v4.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO &&
@@ -174,15 +180,15 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
       (v3.frequency_use == VCKSS_REQUEST_FREQUENCY_UNIT || v3.frequency_use == VCKSS_REQUEST_FREQUENCY_LITERAL);
   if (!eligible)
     fail(kUnsupported, "cmg_full_v2",
-         "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match deletion, joint "
-         "nuisance, movers, automatic batches and no probe order");
+         "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match deletion (or "
+         "observation deletion on the exact route), joint nuisance, movers, automatic batches and no probe order");
   SolveOptions o;
   o.exact = exact;
   if (exact) {
     // ffi_engine.rs:5733-5751: dimension and block-size limits of the exact route
     if (prepared && parameters > v2.exact_estimator_limit)
       fail(kResourceLimit, "exact_estimator", "identified coefficient dimension exceeds exact_limit()");
-    for (uint64_t g = 0; g < ctx.P.units; ++g)
+    for (uint64_t g = 0; g < (ctx.P.observation ? 0 : ctx.P.units); ++g)
       if (ctx.P.unit_ptr[g + 1] - ctx.P.unit_ptr[g] > v2.blocksize_limit)
         fail(kResourceLimit, "exact_estimator", "a deletion block exceeds blocksize_limit()");
   } else if (v1.probes < 2) {

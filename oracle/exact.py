@@ -101,9 +101,11 @@ def _zero_sum_inverse(info, workers, firms, tolerance):
 
 
 def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_tolerance=1e-10,
-                    block_tolerance=1e-10, solver_tolerance=1e-10):
-    """run_exact_estimator (exact_estimator.rs:490-1244) for match deletion,
-    joint nuisance, no controls and no centering, on retained stored rows."""
+                    block_tolerance=1e-10, solver_tolerance=1e-10, mode="match"):
+    """run_exact_estimator (exact_estimator.rs:490-1244) for match deletion
+    (or, with mode="observation", physical-observation deletion,
+    exact_estimator.rs:914-987), joint nuisance, no controls and no
+    centering, on retained stored rows."""
     w = _codes(worker)
     f = _codes(firm)
     g = _codes(deletion)
@@ -113,10 +115,11 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     rows = y.size
     W, F = int(w.max()) + 1, int(f.max()) + 1
     p = W + F
-    for code in range(int(g.max()) + 1):
-        use = g == code
-        if np.unique(w[use]).size != 1 or np.unique(f[use]).size != 1:
-            raise ExactError(20, "a deletion unit spans two cells")
+    if mode == "match":
+        for code in range(int(g.max()) + 1):
+            use = g == code
+            if np.unique(w[use]).size != 1 or np.unique(f[use]).size != 1:
+                raise ExactError(20, "a deletion unit spans two cells")
     # build_design: worker dummy, firm coordinates e_f - 1/F
     x = np.zeros((rows, p))
     x[np.arange(rows), w] = 1.0
@@ -158,6 +161,25 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     max_leverage = 0.0
     rank_gap = np.inf
     h_unit = np.zeros(int(g.max()) + 1)
+    if mode == "observation":
+        design_inverse = x @ inverse
+        for row in range(rows):
+            z = design_inverse[row]
+            lev = float(x[row] @ z)
+            if not np.isfinite(lev) or lev < -100.0 * rank_tolerance:
+                raise ExactError(NONESTIMABLE_DELETION, "physical observation leverage is invalid")
+            maker = 1.0 - lev
+            if maker <= block_tolerance:
+                raise ExactError(NONESTIMABLE_DELETION, "physical observation deletion has leverage at or above one")
+            max_leverage = max(max_leverage, lev)
+            rank_gap = min(rank_gap, maker)
+            scale = fr[row] * y[row] * residual[row] / maker
+            correction += scale * _bilinear(z, z, W, worker_share, firm_share, cell_w, cell_f, cell_mass)
+        correction[3] = correction[0] + correction[1] + 2.0 * correction[2]
+        return dict(plugin=plugin, correction=correction, corrected=plugin - correction, beta=beta,
+                    residual=residual, max_leverage=max_leverage, deletion_rank_gap=rank_gap,
+                    deletion_units=int(fr.sum()), weighted_rss=float(np.sum(fr * residual ** 2)),
+                    information_rcond=rcond, inverse_relres=relres, fit_relres=fit_rel, workers=W, firms=F)
     for code in range(int(g.max()) + 1):
         idx = np.flatnonzero(g == code)
         root = np.sqrt(fr[idx])

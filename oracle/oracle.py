@@ -160,6 +160,7 @@ def lib():
                                     C.c_void_p, PcgOptions, C.POINTER(PcgDiag)]
         L.orc_prepare.argtypes = [C.c_uint64] + [C.c_void_p] * 6 + [C.POINTER(C.c_void_p)]
         L.orc_prepare_masked.argtypes = [C.c_uint64] + [C.c_void_p] * 7 + [C.POINTER(C.c_void_p)]
+        L.orc_prepare_observation.argtypes = [C.c_uint64] + [C.c_void_p] * 6 + [C.POINTER(C.c_void_p)]
         L.orc_problem_free.argtypes = [C.c_void_p]
         L.orc_problem_dims.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_problem_target_total.restype = C.c_double
@@ -334,12 +335,14 @@ class Problem:
               "cell_outcome_sum": (10, np.float64, 5), "cell_target_sum": (11, np.float64, 5),
               "unit_outcome_sum": (12, np.float64, 6)}
 
-    def __init__(self, worker, firm, deletion, outcome, frequency, target, mask=None):
+    def __init__(self, worker, firm, deletion, outcome, frequency, target, mask=None, observation=False):
         cols = [np.ascontiguousarray(worker, np.uint64), np.ascontiguousarray(firm, np.uint64),
                 np.ascontiguousarray(deletion, np.uint64), np.ascontiguousarray(outcome, np.float64),
                 np.ascontiguousarray(frequency, np.uint64), np.ascontiguousarray(target, np.float64)]
         h = C.c_void_p()
-        if mask is None:
+        if observation:
+            _check(lib().orc_prepare_observation(len(cols[0]), *[c.ctypes.data for c in cols], C.byref(h)))
+        elif mask is None:
             _check(lib().orc_prepare(len(cols[0]), *[c.ctypes.data for c in cols], C.byref(h)))
         else:
             m = np.ascontiguousarray(mask, dtype=np.uint8)

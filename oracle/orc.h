@@ -97,6 +97,11 @@ int orc_prepare_masked(uint64_t rows, const uint64_t *worker, const uint64_t *fi
                        const uint64_t *deletion, const double *outcome,
                        const uint64_t *frequency, const double *target, const uint8_t *mask,
                        orc_problem **out);
+/* Observation-deletion graph selection (graph.rs:82-242) instead of the
+ * match-deletion one; compression and plan as orc_prepare. */
+int orc_prepare_observation(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                            const uint64_t *deletion, const double *outcome,
+                            const uint64_t *frequency, const double *target, orc_problem **out);
 void orc_problem_free(orc_problem *p);
 /* dims[0..]: input_rows, retained_rows, retained_physical, workers, firms,
  * cells, deletion_units, target_strata_exact, plan_target_strata */

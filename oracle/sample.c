@@ -365,6 +365,62 @@ static int select_match(const sel_ctx *s, uint8_t *active, orc_selection_receipt
     return ORC_OK;
 }
 
+/* graph.rs:82-242 (select_observation_deletion_graph_with_interrupt): the
+ * largest component, then to a fixed point drop workers with one physical
+ * observation, else the articulation workers of the coordinate graph. */
+static int select_observation(const sel_ctx *s, uint8_t *active, orc_selection_receipt *rec) {
+    memset(rec, 0, sizeof *rec);
+    rec->input_rows = s->rows;
+    for (uint64_t r = 0; r < s->rows; ++r) active[r] = 1;
+    int rc;
+    uint64_t comps;
+    if ((rc = largest_component(s, active, &comps))) return rc;
+    const uint64_t bound = s->W + 1;
+    uint64_t *phys = orc_alloc(s->W, 8);
+    for (;;) {
+        if (rec->fixed_point_iterations > bound) {
+            free(phys);
+            return orc_fail(ORC_GRAPH_CERTIFICATE, "graph", "observation pruning exceeded its bound");
+        }
+        if ((rc = largest_component(s, active, &comps))) { free(phys); return rc; }
+        memset(phys, 0, s->W * 8);
+        for (uint64_t r = 0; r < s->rows; ++r)
+            if (active[r]) phys[s->w[r]] += s->freq[r];
+        uint64_t removed = 0;
+        for (uint64_t w = 0; w < s->W; ++w) removed += phys[w] == 1;
+        if (removed) {
+            for (uint64_t r = 0; r < s->rows; ++r)
+                if (active[r] && phys[s->w[r]] == 1) active[r] = 0;
+            rec->insufficient_workers_removed += removed;
+            rec->degree_iterations++;
+            rec->fixed_point_iterations++;
+            continue;
+        }
+        uint8_t *art = worker_articulations(s, active);
+        uint64_t arts = 0;
+        for (uint64_t w = 0; w < s->W; ++w) arts += art[w];
+        if (!arts) { free(art); break; }
+        for (uint64_t r = 0; r < s->rows; ++r)
+            if (active[r] && art[s->w[r]]) active[r] = 0;
+        free(art);
+        rec->articulation_workers_removed += arts;
+        rec->articulation_iterations++;
+        rec->fixed_point_iterations++;
+    }
+    free(phys);
+    if ((rc = largest_component(s, active, &comps))) return rc;
+    uint64_t retained = 0, mass = 0;
+    for (uint64_t r = 0; r < s->rows; ++r)
+        if (active[r]) {
+            ++retained;
+            mass += s->freq[r];
+        }
+    if (!retained) return orc_fail(ORC_GRAPH_EMPTY, "graph", "no component remains");
+    rec->retained_rows = retained;
+    rec->retained_physical_mass = mass;
+    return ORC_OK;
+}
+
 /* ---------------- compression (problem.rs) ---------------- */
 
 typedef struct {
@@ -486,12 +542,29 @@ int orc_prepare(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
     return orc_prepare_masked(rows, worker, firm, deletion, outcome, frequency, target, NULL, out);
 }
 
+static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                        const uint64_t *deletion, const double *outcome, const uint64_t *frequency,
+                        const double *target, const uint8_t *mask, int observation, orc_problem **out);
+
 /* As orc_prepare; a non-NULL mask replaces graph selection (problem.rs:194
  * `compress(active)` called with a caller mask, as the reference unit tests do). */
 int orc_prepare_masked(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
                        const uint64_t *deletion, const double *outcome,
                        const uint64_t *frequency, const double *target, const uint8_t *mask,
                        orc_problem **out) {
+    return prepare_impl(rows, worker, firm, deletion, outcome, frequency, target, mask, 0, out);
+}
+
+/* As orc_prepare with the observation-deletion graph selection. */
+int orc_prepare_observation(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                            const uint64_t *deletion, const double *outcome,
+                            const uint64_t *frequency, const double *target, orc_problem **out) {
+    return prepare_impl(rows, worker, firm, deletion, outcome, frequency, target, NULL, 1, out);
+}
+
+static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
+                        const uint64_t *deletion, const double *outcome, const uint64_t *frequency,
+                        const double *target, const uint8_t *mask, int observation, orc_problem **out) {
     /* types.rs:185-263 (validate) */
     if (rows == 0) return orc_fail(ORC_INVALID_INPUT, "ingest", "input sample is empty");
     uint64_t physical = 0;
@@ -544,7 +617,8 @@ int orc_prepare_masked(uint64_t rows, const uint64_t *worker, const uint64_t *fi
     p->mask = orc_alloc(rows, 1);
     if (mask) {
         memcpy(p->mask, mask, rows);
-    } else if ((rc = select_match(&s, p->mask, &p->selection))) {
+    } else if ((rc = observation ? select_observation(&s, p->mask, &p->selection)
+                                 : select_match(&s, p->mask, &p->selection))) {
         goto fail_early;
     }
 

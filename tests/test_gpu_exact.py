@@ -124,3 +124,46 @@ def test_exact_limits_and_refusals():
         assert ei.value.code == 11
     finally:
         e.release()
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_observation(spec):
+    from oracle import exact as X
+    from oracle import oracle as O
+    cols = columns(spec)
+    keep = O.Problem(*cols, observation=True).mask()
+    return keep, X.exact_estimator(*[c[keep] for c in cols], mode="observation")
+
+
+@pytest.mark.parametrize("spec", [SMALL, CONFIG0], ids=["small", "config0"])
+def test_exact_observation_deletion_matches_oracle(oracle, spec):
+    keep, want = oracle_observation(spec)
+    e = F.Engine(*columns(spec), deletion_mode="observation")
+    try:
+        assert np.array_equal(e.mask(), keep)
+        print(f"observation {spec.name}: retained rows {int(keep.sum())}", flush=True)
+        res, rec = e.solve_exact(exact_limit=10000)
+    finally:
+        e.release()
+    res = _arrays(res)
+    for k in ("plugin", "correction", "corrected"):
+        _close(res[k], want[k])
+    assert rec["deletion_units"] == want["deletion_units"]
+    assert abs(rec["max_leverage"] - want["max_leverage"]) <= 1e-10
+    assert abs(rec["deletion_rank_gap"] - want["deletion_rank_gap"]) <= 1e-10
+
+
+def test_observation_deletion_refusals():
+    e = F.Engine(*columns(SMALL), deletion_mode="observation")
+    try:
+        with pytest.raises(F.EngineError) as ei:  # its JLA route is not on this engine
+            e.solve(probes=16, tolerance=1e-8)
+        assert ei.value.code == 11
+        e.deletion_mode = 1  # a solve whose deletion mode differs from the prepared graph's
+        with pytest.raises(F.EngineError) as ei:
+            e.solve_exact(exact_limit=10000)
+        assert ei.value.code == 20
+        e.deletion_mode = 2
+        e.solve_exact(exact_limit=10000)
+    finally:
+        e.release()

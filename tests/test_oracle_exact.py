@@ -114,3 +114,108 @@ def test_nonestimable_block_is_rejected():
     with pytest.raises(X.ExactError) as e:
         X.exact_estimator(w, f, np.arange(5), [1.0, 0.5, -0.2, 0.3, 0.9], np.ones(5), np.ones(5))
     assert e.value.code == X.NONESTIMABLE_DELETION
+
+
+# ---------------------------------------------------------------- observation deletion
+# graph.rs:1268-1298: (worker, firm, deletion, frequency) rows and the retained mask
+OBSERVATION_CASES = [
+    ([(1, 1, 1, 2), (2, 1, 2, 1), (2, 2, 3, 1), (3, 1, 4, 1), (3, 2, 5, 1)], [True] * 5),
+    ([(1, 1, 1, 1), (2, 1, 2, 1), (2, 2, 3, 1), (3, 1, 4, 1), (3, 2, 5, 1)], [False, True, True, True, True]),
+]
+
+
+def _observation_columns(rows):
+    a = np.array(rows, dtype=np.float64)
+    n = len(rows)
+    return [a[:, 0], a[:, 1], a[:, 2], np.linspace(-1.0, 1.0, n), a[:, 3], np.ones(n)]
+
+
+@pytest.mark.parametrize("case", range(len(OBSERVATION_CASES)))
+def test_observation_selection_reference_cases(case):
+    from oracle import oracle as O
+    rows, want = OBSERVATION_CASES[case]
+    cols = _observation_columns(rows)
+    assert list(O.Problem(*cols, observation=True).mask()) == want
+    import fevc_amd as F
+    e = F.Engine(*cols, upload=False, deletion_mode="observation")
+    try:
+        assert list(e.mask()) == want
+    finally:
+        e.release()
+
+
+def _observation_panel(seed, workers=120, firms=20, periods=4):
+    from fevc_amd import synth
+    from oracle import oracle as O
+    d = synth.generate(synth.PanelSpec(workers=workers, firms=firms, periods=periods, mobility=0.3,
+                                       seed=seed, name="exact-observation"))
+    cols = [np.asarray(d[k]) for k in COLS]
+    keep = O.Problem(*cols, observation=True).mask()
+    return cols, keep
+
+
+def _worker_articulations(w, f):
+    """Brute force: workers whose removal disconnects the worker-firm graph."""
+    def components(rows_w, rows_f):
+        nodes = {("w", a) for a in rows_w} | {("f", b) for b in rows_f}
+        parent = {v: v for v in nodes}
+
+        def find(v):
+            while parent[v] != v:
+                parent[v] = parent[parent[v]]
+                v = parent[v]
+            return v
+        for a, b in zip(rows_w, rows_f):
+            parent[find(("w", a))] = find(("f", b))
+        return len({find(v) for v in nodes})
+    base = components(w, f)
+    out = []
+    for a in np.unique(w):
+        keep = w != a
+        if keep.any() and components(w[keep], f[keep]) > base:
+            out.append(a)
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_observation_selection_host_matches_oracle(seed):
+    import fevc_amd as F
+    cols, keep = _observation_panel(seed)
+    e = F.Engine(*cols, upload=False, deletion_mode="observation")
+    try:
+        assert np.array_equal(e.mask(), keep)
+    finally:
+        e.release()
+    w, f, fr = cols[0][keep], cols[1][keep], cols[4][keep]
+    # the retained graph: connected, no physical-singleton worker, no worker articulation
+    phys = {a: fr[w == a].sum() for a in np.unique(w)}
+    assert min(phys.values()) >= 2
+    assert not _worker_articulations(w, f)
+    # stayers with several copies are kept
+    firms_per_worker = {a: np.unique(f[w == a]).size for a in np.unique(w)}
+    assert min(firms_per_worker.values()) == 1
+
+
+def test_observation_restatements_agree():
+    cols, keep = _observation_panel(3)
+    w, f, g, y, fr, t = [c[keep] for c in cols]
+    a = X.exact_estimator(w, f, g, y, fr, t, mode="observation")
+    b = X.dense_kss(y, w, f, frequency=fr, target=t, deletion="observation")
+    for k in ("plugin", "correction", "corrected"):
+        assert np.allclose(a[k], b[k], rtol=1e-10, atol=1e-12), (k, a[k], b[k])
+    assert a["deletion_units"] == b["deletion_units"] == int(fr.sum())
+    assert a["max_leverage"] == pytest.approx(b["leverage_max"], abs=1e-12)
+
+
+def test_observation_frequency_deletes_one_physical_copy():
+    cols, keep = _observation_panel(4)
+    w, f, g, y, fr, t = [c[keep] for c in cols]
+    rng = np.random.default_rng(8)
+    fr = rng.integers(2, 4, fr.size)
+    t = rng.uniform(0.5, 2.0, fr.size)
+    a = X.exact_estimator(w, f, g, y, fr, t, mode="observation")
+    idx = np.repeat(np.arange(fr.size), fr)
+    b = X.exact_estimator(w[idx], f[idx], np.arange(idx.size), y[idx], np.ones(idx.size), (t / fr)[idx],
+                          mode="observation")
+    for k in ("plugin", "correction"):
+        assert np.allclose(a[k], b[k], rtol=1e-10, atol=1e-12), k
