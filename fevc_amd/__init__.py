@@ -374,8 +374,9 @@ class Engine:
 
     def solve_exact(self, fit_tolerance=1e-10, exact_limit=None, algorithm=1, poll=None, **overrides):
         """Dense exact estimator (algorithm 1 = EXACT; 0 = AUTO routes here when
-        W + F - 1 <= exact_limit): (result, receipt)."""
-        kw = dict(overrides)
+        W + F - 1 <= exact_limit), a V5 solve with full_cmg_v2 = 0: (result, receipt)."""
+        kw = dict(full_cmg_v2=0)
+        kw.update(overrides)
         if exact_limit is not None:
             kw["exact_estimator_limit"] = exact_limit
         res = self.solve(fit_tolerance=fit_tolerance, poll=poll, algorithm=algorithm, **kw)

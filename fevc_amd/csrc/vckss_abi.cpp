@@ -223,10 +223,22 @@ SolveOptions solve_options(const VckssEngineSolveRequestV5 &q, const Context &ct
     fail(kAbiMismatch, "engine_solve", "V5 solve request reports a short structure size");
   if (q.reserved_5 != 0 || q.tolerance_supplied > 1 || q.full_cmg_v2 > 1)
     fail(kAbiMismatch, "engine_solve", "invalid V5 solve request flags or reserved field");
-  if (q.full_cmg_v2 != 1)
-    fail(kUnsupported, "engine_solve", "the MI355X engine implements the CMG_FULL_V2 route only");
-  if (q.threads == 0) fail(kInvalidInput, "cmg_full_v2", "CMG_FULL_V2 requires a positive caller-declared thread count");
-  return point_options(q.v4, q.tolerance_supplied, ctx);
+  const auto &algorithm = q.v4.v3.v2.algorithm;
+  if (q.full_cmg_v2 == 1) {
+    if (q.threads == 0)
+      fail(kInvalidInput, "cmg_full_v2", "CMG_FULL_V2 requires a positive caller-declared thread count");
+    // ffi_engine.rs:10366-10410: CMG_FULL_V2 takes the JLA algorithm only
+    if (algorithm != VCKSS_ALGORITHM_JLA)
+      fail(kUnsupported, "cmg_full_v2",
+           "CMG_FULL_V2 requires JLA, auto/generic engine, auto/CMG solver and joint/fixed-offset point estimation");
+    return point_options(q.v4, q.tolerance_supplied, ctx);
+  }
+  // full_cmg_v2 = 0: the dense exact estimator (EXACT, or AUTO within the
+  // exact limit); the legacy direct-solver JLA route is not implemented
+  SolveOptions o = point_options(q.v4, q.tolerance_supplied, ctx);
+  if (!o.exact)
+    fail(kUnsupported, "engine_solve", "the MI355X engine runs JLA on the CMG_FULL_V2 route only (full_cmg_v2 = 1)");
+  return o;
 }
 
 // numerical_api.rs:113-196 (validate_point) for the path's requests: schema,

@@ -77,17 +77,26 @@ def test_auto_routes_by_exact_limit():
     try:
         exact, rec = e.solve_exact(exact_limit=10000)
         exact = _arrays(exact)
-        auto = _arrays(e.solve(algorithm=0, exact_estimator_limit=rec["parameters"]))
+        auto, _ = e.solve_exact(algorithm=0, exact_limit=rec["parameters"])
+        auto = _arrays(auto)
         assert e.exact_receipt()["parameters"] == rec["parameters"]
         for k in ("plugin", "correction", "corrected"):
             assert np.array_equal(auto[k], exact[k])
-        # one below the dimension: AUTO takes the JLA route
-        jla = _arrays(e.solve(algorithm=0, exact_estimator_limit=rec["parameters"] - 1, probes=64,
-                              tolerance=1e-10))
-        with pytest.raises(F.EngineError) as ei:
+        jla = _arrays(e.solve(probes=64, tolerance=1e-10))
+        with pytest.raises(F.EngineError) as ei:  # the last solve was not exact
             e.exact_receipt()
         assert ei.value.code == 11
         assert np.all(jla["numerical_mcse"] > 0)
+        # one below the dimension AUTO resolves to the legacy JLA route (full_cmg_v2 = 0),
+        # which this engine does not run
+        with pytest.raises(F.EngineError) as ei:
+            e.solve_exact(algorithm=0, exact_limit=rec["parameters"] - 1)
+        assert ei.value.code == 11
+        # ffi_engine.rs:10366-10410: CMG_FULL_V2 takes JLA only
+        for algorithm in (0, 1):
+            with pytest.raises(F.EngineError) as ei:
+                e.solve_exact(algorithm=algorithm, exact_limit=10000, full_cmg_v2=1)
+            assert ei.value.code == 11
     finally:
         e.release()
 
