@@ -310,7 +310,7 @@ class Engine:
     def __init__(self, worker, firm, deletion, outcome, frequency, target, *, upload=True,
                  poll=None, deletion_mode="match"):
         """deletion_mode "match" (the default) or "observation" (the dense
-        exact route only)."""
+        exact route, and the JLA route without numerical MCSE)."""
         lib = library()
         self.deletion_mode = {"match": 1, "observation": 2}[deletion_mode]
         cols = [_f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency), _f64(target)]
@@ -402,7 +402,9 @@ class Engine:
         return d
 
     def numerical_request(self, **kw):
-        return default_numerical_request(**kw)
+        r = default_numerical_request(**kw)
+        r.v1.point.deletion_mode = self.deletion_mode
+        return r
 
     def solve_numerical(self, request=None, **kw):
         """Point estimate plus the numerical MCSE attachment; returns (result, numerical)."""

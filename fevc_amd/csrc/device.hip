@@ -573,7 +573,7 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
     for (uint64_t first = 0; first < R; first += lw) {
       const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
       hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                         E.unit_count.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
+                         E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
       std::vector<ColumnReceipt> rc;
       bool failed = false;
       with_k(KL, [&](auto kk) {
@@ -724,6 +724,13 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.cell_unit_ptr.upload(p.cell_unit_ptr); E.cell_units.upload(p.cell_units);
   E.unit_cell.upload(p.unit_cell); E.unit_count.upload(p.unit_count); E.unit_rank.upload(p.unit_rank);
   E.unit_outcome.upload(p.unit_outcome);
+  if (p.observation) {
+    std::vector<uint64_t> phys(p.rows + 1, 0);
+    for (uint64_t r = 0; r < p.rows; ++r) phys[r + 1] = phys[r] + p.frequency[r];
+    E.unit_offset.upload(p.unit_offset);
+    E.row_phys.upload(phys);
+    E.row_outcome.upload(p.outcome);
+  }
   E.stratum_per_copy.upload(p.stratum_per_copy); E.stratum_rank.upload(p.stratum_rank);
   E.stratum_count.upload(p.stratum_count); E.cell_stratum_ptr.upload(p.cell_stratum_ptr);
   E.cell_target.upload(p.cell_target);
@@ -764,6 +771,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+  if (P.observation && opt.numerical)
+    fail(kUnsupported, "numerical_solve", "no numerical attachment under observation deletion");
   const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
   const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
   const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
@@ -805,10 +814,14 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   E.moments.alloc(10 * E.G);
   VK_HIP(hipMemsetAsync(E.moments.p, 0, 10 * E.G * sizeof(double), E.stream));
   E.atoms.alloc(std::max<uint64_t>((uint64_t)KL * E.G, (uint64_t)KT * E.S));
+  if (P.observation) {
+    E.obs_corr.alloc(4 * P.physical_total);
+    VK_HIP(hipMemsetAsync(E.obs_corr.p, 0, 4 * P.physical_total * sizeof(double), E.stream));
+  }
   for (uint64_t first = 0; first < R; first += lw) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
+                       E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
@@ -818,9 +831,15 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
                                                  E.frhs.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
-                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
-                         E.moments.p, E.status.p);
+      if (P.observation)
+        hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k,
+                           first, opt.seed, kTagLeverage, E.unit_cell.p, E.unit_count.p, E.unit_rank.p,
+                           E.unit_offset.p, E.row_phys.p, P.physical_total, E.cell_worker.p, E.cell_firm.p,
+                           E.worker.p, E.fsol.p, E.moments.p, E.obs_corr.p, E.status.p);
+      else
+        hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k,
+                           E.unit_cell.p, E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p,
+                           E.atoms.p, E.moments.p, E.status.p);
     });
     for (auto &c : rc) {
       res.leverage.push_back(c);
@@ -833,9 +852,14 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   // ---- leverage adjustment (engine.rs:2269-2410)
   E.adjust.alloc(7 * E.G);
   E.cellw.alloc(E.C);
-  hipLaunchKernelGGL(dev::k_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.unit_cell.p,
-                     E.unit_count.p, E.unit_outcome.p, E.fitted.p,
-                     dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
+  if (P.observation)  // generic_jla.rs:5893-6017
+    hipLaunchKernelGGL(dev::k_obs_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.obs_corr.p,
+                       P.physical_total, E.row_phys.p, E.unit_count.p, E.unit_cell.p, E.row_outcome.p, E.fitted.p,
+                       dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
+  else
+    hipLaunchKernelGGL(dev::k_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.unit_cell.p,
+                       E.unit_count.p, E.unit_outcome.p, E.fitted.p,
+                       dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
   E.check_status("jla_leverage_adjustment");
   hipLaunchKernelGGL(dev::k_cell_weight, grid_for(E.C), kBlock, 0, E.stream, E.C, E.cell_unit_ptr.p,
                      E.cell_units.p, E.adjust.p + 6 * E.G, E.cellw.p);
@@ -889,7 +913,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
     VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(int32_t), E.stream));
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
-                       E.stratum_rank.p, E.stratum_count.p, opt.seed, kTagTarget, first, k, KT, E.atoms.p);
+                       E.stratum_rank.p, E.stratum_count.p, (const uint64_t *)nullptr, opt.seed, kTagTarget, first, k, KT, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KT, [&](auto kt) {
       // target batches hold at most 32 probes (64 columns)
@@ -966,7 +990,8 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
   DBuf<int32_t> buf;
   buf.alloc(n * columns);
   hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
-                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p, seed,
+                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p,
+                     target ? (const uint64_t *)nullptr : E.unit_offset.p, seed,
                      tag, first_probe, columns, columns, buf.p);
   std::vector<int32_t> h;
   buf.download(h, n * columns, E.stream);
@@ -1069,7 +1094,7 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   for (uint32_t b = 0; b < batches; ++b) {
     const uint64_t first = (uint64_t)b * lw;
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((lw + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, opt.seed, kTagLeverage, first, lw, KL, E.atoms.p);
+                       E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, lw, KL, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;

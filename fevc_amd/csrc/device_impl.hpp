@@ -210,6 +210,11 @@ struct DeviceEngine::Impl {
   DBuf<uint32_t> firm_cells, cell_worker, cell_firm, cell_units, unit_cell, aux_nbr;
   DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
   DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
+  // observation deletion (null otherwise): per row the first copy in its
+  // class, its first physical copy (prefix of frequencies), its outcome; per
+  // physical copy the probe sums of s p and s p^3 (sum, correction)
+  DBuf<uint64_t> unit_offset, row_phys;
+  DBuf<double> row_outcome, obs_corr;
   std::vector<DevLevel> levels;
   DBuf<uint32_t> perm0, pos0;  // level-0 renumbering
   DevChunks firm_chunks;       // firms over their cells (firm_cells order)

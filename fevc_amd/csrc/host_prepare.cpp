@@ -398,6 +398,50 @@ int cmp_total(double a, double b) {
   return x < y ? -1 : (x > y ? 1 : 0);
 }
 
+// Observation deletion plan (generic_jla.rs:4158-4330, statistical_batches.rs:15-53):
+// semantic ranks over (worker, firm, per-copy target mass, outcome), classes
+// of equal rank (rows ascending), and one unit per stored row holding the
+// class rank and the row's copy offset in its class, so that the row's atom
+// is the sum of its own physical copies' Rademacher signs.
+void observation_plan(Prepared &P, std::vector<uint32_t> &order) {
+  const uint64_t R = P.rows;
+  auto cmp = [&](uint32_t a, uint32_t b) {
+    if (P.row_worker[a] != P.row_worker[b]) return P.row_worker[a] < P.row_worker[b] ? -1 : 1;
+    if (P.row_firm[a] != P.row_firm[b]) return P.row_firm[a] < P.row_firm[b] ? -1 : 1;
+    int c = cmp_total(P.per_copy[a], P.per_copy[b]);
+    if (c) return c;
+    return cmp_total(P.outcome[a], P.outcome[b]);
+  };
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    int c = cmp(a, b);
+    return c ? c < 0 : a < b;
+  });
+  P.row_rank.assign(R, 0);
+  P.unit_offset.assign(R, 0);
+  uint64_t rank = 0, offset = 0;
+  for (uint64_t i = 0; i < R; ++i) {
+    if (i == 0 || cmp(order[i - 1], order[i]) != 0) {
+      ++rank;
+      offset = 0;
+    }
+    P.row_rank[order[i]] = rank;
+    P.unit_offset[order[i]] = offset;  // rows of a class ascend in `order`
+    offset += P.frequency[order[i]];
+  }
+  P.units = R;
+  P.unit_ptr.resize(R + 1);
+  std::iota(P.unit_ptr.begin(), P.unit_ptr.end(), 0ull);
+  P.unit_rows.resize(R);
+  std::iota(P.unit_rows.begin(), P.unit_rows.end(), 0u);
+  P.unit_cell = P.row_cell;
+  P.unit_count = P.frequency;
+  P.unit_rank = P.row_rank;
+  P.unit_outcome.resize(R);
+  for (uint64_t i = 0; i < R; ++i) P.unit_outcome[i] = static_cast<double>(P.frequency[i]) * P.outcome[i];
+  P.unit_target = P.target;
+}
+
 }  // namespace
 
 Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
@@ -585,6 +629,9 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
       fail(kInvalidTargetWeight, "jla_plan", "per-copy target mass is invalid at zero-based row " + std::to_string(i));
     P.per_copy[i] = v;
   }
+  if (P.observation) {
+    observation_plan(P, order);
+  } else {
   auto semantic_cmp = [&](uint32_t a, uint32_t b) {
     if (P.row_worker[a] != P.row_worker[b]) return P.row_worker[a] < P.row_worker[b] ? -1 : 1;
     if (P.row_firm[a] != P.row_firm[b]) return P.row_firm[a] < P.row_firm[b] ? -1 : 1;
@@ -628,6 +675,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
     P.unit_target[u] = ts.value();
     P.unit_rank[u] = minr;
   }
+  }
   std::iota(order.begin(), order.end(), 0u);
   std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     if (P.row_cell[a] != P.row_cell[b]) return P.row_cell[a] < P.row_cell[b];
@@ -657,13 +705,14 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   }
   P.strata = P.stratum_cell.size();
   // gather indices for the device: units and strata of each cell
+  // (P.units: the deletion units, or the stored rows under observation deletion)
   P.cell_unit_ptr.assign(C + 1, 0);
-  for (uint64_t u = 0; u < G; ++u) P.cell_unit_ptr[P.unit_cell[u] + 1]++;
+  for (uint64_t u = 0; u < P.units; ++u) P.cell_unit_ptr[P.unit_cell[u] + 1]++;
   for (uint64_t c = 0; c < C; ++c) P.cell_unit_ptr[c + 1] += P.cell_unit_ptr[c];
-  P.cell_units.resize(G);
+  P.cell_units.resize(P.units);
   {
     std::vector<uint64_t> cur(P.cell_unit_ptr.begin(), P.cell_unit_ptr.end() - 1);
-    for (uint64_t u = 0; u < G; ++u) P.cell_units[cur[P.unit_cell[u]]++] = static_cast<uint32_t>(u);
+    for (uint64_t u = 0; u < P.units; ++u) P.cell_units[cur[P.unit_cell[u]]++] = static_cast<uint32_t>(u);
   }
   P.cell_stratum_ptr.assign(C + 1, 0);
   for (uint64_t s = 0; s < P.strata; ++s) P.cell_stratum_ptr[P.stratum_cell[s] + 1]++;

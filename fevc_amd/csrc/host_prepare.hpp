@@ -54,6 +54,7 @@ struct Prepared {
   std::vector<uint64_t> row_rank;
   std::vector<uint32_t> unit_cell;
   std::vector<uint64_t> unit_count, unit_rank;
+  std::vector<uint64_t> unit_offset;  // observation deletion: first copy of the row in its class
   std::vector<double> unit_outcome, unit_target;
   std::vector<uint32_t> stratum_cell;
   std::vector<double> stratum_per_copy, stratum_target;

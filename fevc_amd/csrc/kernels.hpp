@@ -102,23 +102,27 @@ __device__ inline void philox(uint64_t k0, uint64_t k1, uint64_t c[4]) {
 // Rademacher atoms 2*popcount - n for `columns` probes starting at
 // `first_probe`, as exact integers, interleaved: out[row * ld + col].  The
 // probe block varies fastest across threads so that stores are contiguous.
-// Callers guarantee trials < 2^31.
+// Row r sums the signs of copies [offset[r], offset[r] + trials[r]) of its
+// entity, copy j being bit j % 64 of Counter-V1 word j / 64 (rng.rs:121-179;
+// generic_jla.rs:10321-10329 with a copy offset); offset may be null (all
+// zero).  Callers guarantee trials < 2^31.
 static __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
-                        const uint64_t *__restrict__ trials, uint64_t seed, uint64_t tag,
-                        uint64_t first_probe, uint32_t columns, uint32_t ld, int32_t *__restrict__ out) {
+                        const uint64_t *__restrict__ trials, const uint64_t *__restrict__ offset, uint64_t seed,
+                        uint64_t tag, uint64_t first_probe, uint32_t columns, uint32_t ld,
+                        int32_t *__restrict__ out) {
   const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
   const uint64_t blocks = last_block - first_block + 1;
   const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n * blocks;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t row = t / blocks, pb = first_block + t % blocks;
-    const uint64_t tr = trials[row], complete = tr / 64, rem = tr % 64;
+    const uint64_t tr = trials[row], lo = offset ? offset[row] : 0, hi = lo + tr;
     uint64_t pos[4] = {0, 0, 0, 0};
-    for (uint64_t w = 0; w <= complete; ++w) {
-      if (w == complete && rem == 0) break;
+    for (uint64_t w = lo / 64; w * 64 < hi; ++w) {
+      const uint64_t b0 = lo > w * 64 ? lo - w * 64 : 0, b1 = hi - w * 64 < 64 ? hi - w * 64 : 64;
+      const uint64_t mask = (b1 == 64 ? ~0ULL : ((1ULL << b1) - 1)) & ~((1ULL << b0) - 1);
       uint64_t c[4] = {entity[row], pb, w, tag};
       philox(k0, k1, c);
-      const uint64_t mask = (w == complete) ? ((1ULL << rem) - 1) : ~0ULL;
 #pragma unroll
       for (int l = 0; l < 4; ++l) pos[l] += __popcll(c[l] & mask);
     }

@@ -741,6 +741,161 @@ static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, co
   }
 }
 
+// ------------------------------------------------ observation deletion
+// Generic-engine observation deletion without controls.  One unit per stored
+// row, whose atom sums the signs of the row's own physical copies (k_atoms
+// with copy offsets).
+
+// ObservationMoments and ObservationCorrelations (statistical_batches.rs:79-206):
+// per stored row the probe sums of p^2 and p^4, per physical copy those of
+// s p and s p^3, p = alpha_w + psi_f of the row's cell and s the copy's sign,
+// bit (copy % 64) of Counter-V1 word (entity, probe, copy / 64)
+// (generic_jla.rs:10321-10329).  m: 4 x rows (p^2 sum, correction, p^4 sum,
+// correction); corr: 4 x physical copies (s p sum, correction, s p^3 sum,
+// correction); one compensated addition per batch, batches in probe order.
+template <int K>
+__global__ void k_obs_moments(uint64_t rows, uint32_t kcols, uint64_t first_probe, uint64_t seed, uint64_t tag,
+                              const uint32_t *__restrict__ unit_cell, const uint64_t *__restrict__ unit_count,
+                              const uint64_t *__restrict__ unit_rank, const uint64_t *__restrict__ unit_offset,
+                              const uint64_t *__restrict__ row_phys, uint64_t physical,
+                              const uint32_t *__restrict__ cell_worker, const uint32_t *__restrict__ cell_firm,
+                              const double *__restrict__ worker, const double *__restrict__ firm,
+                              double *__restrict__ m, double *__restrict__ corr, int *__restrict__ status) {
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+  const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
+  VK_GRID_LOOP(t, rows * TPR) {
+    const uint64_t g = t / TPR;
+    const int j0 = (int)(t % TPR) * LP;
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    double p[LP], v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < LP; ++q) {
+      const int j = j0 + q;
+      p[q] = 0.0;
+      if (j < (int)kcols) {
+        p[q] = worker[w * K + j] + firm[f * K + j];
+        if (!isfinite(p[q])) atomicCAS(status, 0, 73);
+        const double p2 = p[q] * p[q];
+        v[0] += p2;
+        v[1] += p2 * p2;
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < TPR; off <<= 1) {
+      v[0] += __shfl_xor(v[0], off);
+      v[1] += __shfl_xor(v[1], off);
+    }
+    if (j0 == 0) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        double s = m[(uint64_t)(2 * q) * rows + g], e = m[(uint64_t)(2 * q + 1) * rows + g];
+        neumaier_add(s, e, v[q]);
+        m[(uint64_t)(2 * q) * rows + g] = s;
+        m[(uint64_t)(2 * q + 1) * rows + g] = e;
+      }
+    }
+    // every thread of the row runs the same copy loops, so the butterflies
+    // below see the whole group
+    const uint64_t lo = unit_offset[g], hi = lo + unit_count[g], entity = unit_rank[g], base = row_phys[g];
+    for (uint64_t word = lo / 64; word * 64 < hi; ++word) {
+      uint64_t bits[LP], blk[4] = {0, 0, 0, 0}, cached = ~0ULL;
+#pragma unroll
+      for (int q = 0; q < LP; ++q) {
+        const uint64_t probe = first_probe + (uint64_t)(j0 + q), pb = probe / 4;
+        if (pb != cached) {
+          blk[0] = entity; blk[1] = pb; blk[2] = word; blk[3] = tag;
+          philox(k0, k1, blk);
+          cached = pb;
+        }
+        bits[q] = blk[probe % 4];
+      }
+      const uint64_t c0 = lo > word * 64 ? lo : word * 64, c1 = hi < word * 64 + 64 ? hi : word * 64 + 64;
+      for (uint64_t copy = c0; copy < c1; ++copy) {
+        double a[2] = {0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < LP; ++q) {
+          if (j0 + q < (int)kcols) {
+            const double sgn = ((bits[q] >> (copy % 64)) & 1) ? 1.0 : -1.0;
+            a[0] += sgn * p[q];
+            a[1] += sgn * (p[q] * p[q] * p[q]);
+          }
+        }
+#pragma unroll
+        for (int off = 1; off < TPR; off <<= 1) {
+          a[0] += __shfl_xor(a[0], off);
+          a[1] += __shfl_xor(a[1], off);
+        }
+        if (j0 == 0) {
+          const uint64_t x = base + (copy - lo);
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            double s = corr[(uint64_t)(2 * q) * physical + x], e = corr[(uint64_t)(2 * q + 1) * physical + x];
+            neumaier_add(s, e, a[q]);
+            corr[(uint64_t)(2 * q) * physical + x] = s;
+            corr[(uint64_t)(2 * q + 1) * physical + x] = e;
+          }
+        }
+      }
+    }
+  }
+}
+
+// observation_deleted_adjustment (generic_jla.rs:5893-6017) with
+// FiveMoments::from_totals and finite (:441-513), no control leverage: per
+// copy the finite-projection inverse 1/m + b/m^2 - v/m^3, d = e (sum over
+// the row's copies) / f.  out (7 x rows): means over the copies of the
+// projection share, residual share, bias and variance; e; d; and f y d, the
+// row's share of its cell's correction weight.
+static __global__ void k_obs_adjust(uint64_t rows, const double *__restrict__ m, const double *__restrict__ corr,
+                                    uint64_t physical, const uint64_t *__restrict__ row_phys,
+                                    const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ unit_cell,
+                                    const double *__restrict__ row_outcome, const double *__restrict__ fitted_cell,
+                                    AdjustParams prm, double *__restrict__ out, int *__restrict__ status) {
+  VK_GRID_LOOP(g, rows) {
+    const double pf = m[0 * rows + g] + m[1 * rows + g];
+    const double ps = m[2 * rows + g] + m[3 * rows + g];
+    double is = 0.0, ic = 0.0, ls = 0.0, lc = 0.0, rs = 0.0, rc = 0.0, bs = 0.0, bc = 0.0, vs = 0.0, vc = 0.0;
+    int fail = 0;
+    for (uint64_t x = row_phys[g]; x < row_phys[g + 1]; ++x) {
+      const double first = corr[0 * physical + x] + corr[1 * physical + x];
+      const double third = corr[2 * physical + x] + corr[3 * physical + x];
+      const double t0 = pf, t1 = prm.probes + pf - 2.0 * first, t2 = ps;
+      const double t3 = prm.probes + 6.0 * pf + ps - 4.0 * first - 4.0 * third, t4 = pf + ps - 2.0 * third;
+      const double total = (t0 + t1) / prm.probes;
+      if (!isfinite(total) || total <= prm.block_tolerance) { fail = 74; break; }
+      const double proj = (t0 / prm.probes) / total, res = (t1 / prm.probes) / total;
+      const double p4 = t2 / prm.probes, m4 = t3 / prm.probes, mixed = t4 / prm.probes;
+      const double bias = (res * p4 - proj * m4 + (res - proj) * mixed) / prm.probes;
+      double var = (res * res * p4 + proj * proj * m4 - 2.0 * proj * res * mixed) / prm.probes;
+      if (!isfinite(proj) || !isfinite(res) || !isfinite(bias) || !isfinite(var) ||
+          var < -100.0 * prm.rank_tolerance) { fail = 73; break; }
+      var = fmax(var, 0.0);
+      if (!isfinite(res) || res <= prm.block_tolerance) { fail = 75; break; }
+      const double r = 1.0 / res;
+      const double inv = r + bias * (r * r) - var * (r * r * r);
+      if (!isfinite(inv) || inv <= 0.0) { fail = 76; break; }
+      neumaier_add(is, ic, inv);
+      neumaier_add(ls, lc, proj);
+      neumaier_add(rs, rc, res);
+      neumaier_add(bs, bc, bias);
+      neumaier_add(vs, vc, var);
+    }
+    if (fail) { atomicCAS(status, 0, fail); continue; }
+    const double f = (double)unit_count[g];
+    const double y = row_outcome[g], e = y - fitted_cell[unit_cell[g]];
+    const double deleted = e * ((is + ic) / f);
+    if (!isfinite(deleted)) { atomicCAS(status, 0, 71); continue; }
+    out[0 * rows + g] = (ls + lc) / f;
+    out[1 * rows + g] = (rs + rc) / f;
+    out[2 * rows + g] = (bs + bc) / f;
+    out[3 * rows + g] = (vs + vc) / f;
+    out[4 * rows + g] = e;
+    out[5 * rows + g] = deleted;
+    out[6 * rows + g] = f * y * deleted;
+  }
+}
+
 // ------------------------------------------------ numerical MCSE attachment
 // engine/numerical.rs and numerical_mc.rs: a local linearisation of every
 // unit's leverage adjustment in its five moment means, fold sums of the

@@ -159,9 +159,10 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
   // ffi_engine.rs:5705-5710: the solve's deletion mode is the prepared graph's
   if (prepared && (v1.deletion_mode == VCKSS_DELETION_OBSERVATION) != ctx.P.observation)
     fail(kInvalidInput, "engine_solve", "solve deletion mode differs from the prepared graph mode");
-  // observation deletion: the exact route only (its JLA route is the generic engine)
+  // observation deletion: the exact route, or the generic engine's JLA route
+  // without controls (generic_jla.rs:5470-6017)
   const bool deletion_ok =
-      v1.deletion_mode == VCKSS_DELETION_MATCH || (v1.deletion_mode == VCKSS_DELETION_OBSERVATION && exact);
+      v1.deletion_mode == VCKSS_DELETION_MATCH || v1.deletion_mode == VCKSS_DELETION_OBSERVATION;
   const bool eligible =
       (v2.algorithm == VCKSS_ALGORITHM_JLA || v2.algorithm == VCKSS_ALGORITHM_AUTO ||
        v2.algorithm == VCKSS_ALGORITHM_EXACT) &&
@@ -180,8 +181,8 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
       (v3.frequency_use == VCKSS_REQUEST_FREQUENCY_UNIT || v3.frequency_use == VCKSS_REQUEST_FREQUENCY_LITERAL);
   if (!eligible)
     fail(kUnsupported, "cmg_full_v2",
-         "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match deletion (or "
-         "observation deletion on the exact route), joint nuisance, movers, automatic batches and no probe order");
+         "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match or observation "
+         "deletion, joint nuisance, movers, automatic batches and no probe order");
   SolveOptions o;
   o.exact = exact;
   if (exact) {
@@ -274,6 +275,8 @@ void preflight_point(VckssEngineSolveRequestV4 point, uint32_t tolerance_supplie
   if (point.v3.engine == VCKSS_ENGINE_COMPRESSED) point.v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
   Context probe;
   (void)point_options(point, tolerance_supplied, probe);
+  if (point.v3.v2.v1.deletion_mode == VCKSS_DELETION_OBSERVATION)
+    fail(kUnsupported, "numerical_preflight", "the MI355X engine attaches numerical MCSE under match deletion only");
 }
 
 std::vector<double> builtin_smoother() {
@@ -549,6 +552,8 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
+  if (ctx.P.observation)
+    fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE under match deletion only");
   o.numerical = true;
   run_options(ctx, o, poll, poll_ctx);
 }
@@ -780,6 +785,8 @@ int32_t vckss_mi355x_array_v1(uint64_t generation, uint32_t which, double *out, 
         case 23: tmp.assign(P.stratum_count.begin(), P.stratum_count.end()); break;
         case 24: src = &P.cell_weight; break;
         case 25: tmp.assign(P.unit_cell.begin(), P.unit_cell.end()); break;
+        case 26: tmp.assign(P.unit_offset.begin(), P.unit_offset.end()); tmp.resize(P.units, 0.0); break;
+        case 27: tmp.assign(P.cell_units.begin(), P.cell_units.end()); break;
         default: fail(kInvalidInput, "array", "unknown array");
       }
     }

@@ -190,7 +190,9 @@ int32_t vckss_mi355x_chunk_table_v1(const uint32_t *ptr, uint64_t segments, uint
 /* Host preparation only (no GPU): same as the prepare entry point but stops
  * before device upload; the generation can be queried with dims/arrays of
  * the prepared problem (which 20..): 20 unit_rank 21 unit_count
- * 22 stratum_rank 23 stratum_count 24 cell_weight 25 unit_cell. */
+ * 22 stratum_rank 23 stratum_count 24 cell_weight 25 unit_cell
+ * 26 unit copy offset (observation deletion; zero under match deletion)
+ * 27 the units of each cell, cells ascending (the device's gather list). */
 int32_t vckss_mi355x_prepare_host_v1(const VckssEnginePrepareRequestInterruptV3 *request,
                                      const VckssEngineColumnsV3 *columns, uint64_t *output_handle);
 

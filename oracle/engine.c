@@ -769,6 +769,158 @@ out:
     return rc;
 }
 
+/* ---- observation deletion (generic engine, no controls) ----
+ * One unit per stored row.  Its atom is the sum of the Rademacher signs of
+ * the row's physical copies, copy j of a class being bit j % 64 of Counter-V1
+ * word j / 64 of the class entity (generic_jla.rs:10321-10329, 5660-5706). */
+static int copy_sign(uint64_t seed, uint64_t probe, uint64_t entity, uint64_t copy) {
+    uint64_t word = orc_rng_word(seed, ORC_TAG_LEVERAGE, probe, entity, copy / 64);
+    return ((word >> (copy % 64)) & 1) ? 1 : -1;
+}
+
+static void observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k,
+                              int64_t *atoms) {
+    for (uint64_t c = 0; c < k; ++c)
+        for (uint64_t g = 0; g < p->units; ++g) {
+            int64_t a = 0;
+            for (uint64_t j = 0; j < p->unit_count[g]; ++j)
+                a += copy_sign(seed, first + c, p->unit_rank[g], p->unit_offset[g] + j);
+            atoms[c * p->units + g] = a;
+        }
+}
+
+int orc_observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t columns, int64_t *out) {
+    if (!p->observation)
+        return orc_fail(ORC_INVALID_INPUT, "observation_atoms", "the problem was prepared for match deletion");
+    observation_atoms(p, seed, first, columns, out);
+    return ORC_OK;
+}
+
+/* ObservationMoments and ObservationCorrelations (generic_jla.rs:5540-5892):
+ * per stored row the probe sums of p^2 and p^4, per physical copy those of
+ * s p and s p^3, p the row's prediction and s the copy's sign. */
+typedef struct {
+    uint64_t *phys;          /* copies of row g: [phys[g], phys[g+1]) */
+    ssum *square, *fourth;   /* per row */
+    ssum *first, *third;     /* per physical copy */
+} obs_state;
+
+static void observation_state_init(const orc_problem *p, obs_state *o) {
+    uint64_t G = p->units;
+    o->phys = orc_alloc(G + 1, 8);
+    for (uint64_t g = 0; g < G; ++g) o->phys[g + 1] = o->phys[g] + p->unit_count[g];
+    o->square = orc_alloc(G, sizeof(ssum));
+    o->fourth = orc_alloc(G, sizeof(ssum));
+    o->first = orc_alloc(o->phys[G], sizeof(ssum));
+    o->third = orc_alloc(o->phys[G], sizeof(ssum));
+}
+
+static void observation_state_free(obs_state *o) {
+    free(o->phys); free(o->square); free(o->fourth); free(o->first); free(o->third);
+    o->phys = NULL; o->square = o->fourth = o->first = o->third = NULL;
+}
+
+/* statistical_batches.rs:79-206, probes in order */
+static void observation_moments(const orc_problem *p, obs_state *o, uint64_t seed, uint64_t first,
+                                uint64_t k, const column_t *cols) {
+    for (uint64_t g = 0; g < p->units; ++g) {
+        uint32_t cell = p->unit_cell[g];
+        for (uint64_t c = 0; c < k; ++c) {
+            double pred = cols[c].worker[p->cell_worker[cell]] + cols[c].firm[p->cell_firm[cell]];
+            ss_add(&o->square[g], pred * pred);
+            ss_add(&o->fourth[g], pred * pred * pred * pred);
+            for (uint64_t j = 0; j < p->unit_count[g]; ++j) {
+                double s = (double)copy_sign(seed, first + c, p->unit_rank[g], p->unit_offset[g] + j);
+                ss_add(&o->first[o->phys[g] + j], s * pred);
+                ss_add(&o->third[o->phys[g] + j], s * pred * pred * pred);
+            }
+        }
+    }
+}
+
+/* observation_deleted_adjustment (generic_jla.rs:5893-6017) with
+ * FiveMoments::from_totals and finite (:441-513), no control leverage.  The
+ * target contraction (statistical_batches.rs:318-331) sums
+ * f y d p^2 over rows, d = e (sum over copies of the finite inverse) / f, and
+ * p is one per cell: it is the cell-weight contraction of the match route
+ * with weight[cell] = sum over the cell's rows of f y d.  Per row: projection
+ * share, residual share, bias and variance are means over the copies;
+ * residual mass = e; deleted mass = d. */
+static int observation_adjustment(const orc_problem *p, const obs_state *o, const orc_engine_options *opt,
+                                  orc_engine_result *res) {
+    uint64_t G = p->units, C = p->cells;
+    double probes = (double)res->probes;
+    res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
+    res->finite_bias = orc_alloc(G, 8); res->finite_variance = orc_alloc(G, 8);
+    res->residual_mass = orc_alloc(G, 8); res->deleted_mass = orc_alloc(G, 8);
+    ssum *cw = orc_alloc(C, sizeof *cw);
+    int rc = ORC_OK;
+    for (uint64_t g = 0; g < G && !rc; ++g) {
+        double pf = ss_fin(o->square[g]), ps = ss_fin(o->fourth[g]);
+        ssum inverse_sum = {0, 0}, lev = {0, 0}, rs = {0, 0}, bs = {0, 0}, vs = {0, 0};
+        for (uint64_t q = o->phys[g]; q < o->phys[g + 1]; ++q) {
+            double first = ss_fin(o->first[q]), third = ss_fin(o->third[q]);
+            double tot[5] = {pf, probes + pf - 2.0 * first, ps,
+                             probes + 6.0 * pf + ps - 4.0 * first - 4.0 * third, pf + ps - 2.0 * third};
+            double total = (tot[0] + tot[1]) / probes;
+            if (!isfinite(total) || total <= opt->block_tolerance) {
+                rc = orc_fail(ORC_JLA_CONSTRAINT, "generic_jla_leverage", "projection and residual masses are not positive");
+                break;
+            }
+            double projection = (tot[0] / probes) / total, residual = (tot[1] / probes) / total;
+            double p4 = tot[2] / probes, m4 = tot[3] / probes, mixed = tot[4] / probes;
+            double bias = (residual * p4 - projection * m4 + (residual - projection) * mixed) / probes;
+            double variance = (residual * residual * p4 + projection * projection * m4 -
+                               2.0 * projection * residual * mixed) / probes;
+            if (!isfinite(projection) || !isfinite(residual) || !isfinite(bias) || !isfinite(variance) ||
+                variance < -100.0 * opt->rank_tolerance) {
+                rc = orc_fail(ORC_JLA_MOMENT, "generic_jla_leverage", "finite-projection moment failed");
+                break;
+            }
+            variance = fmax(variance, 0.0);
+            if (!isfinite(residual) || residual <= opt->block_tolerance) {
+                rc = orc_fail(ORC_NONESTIMABLE, "generic_jla_observation_adjustment",
+                              "estimated residual leverage is nonpositive");
+                break;
+            }
+            double r = 1.0 / residual;
+            double inverse = r + bias * r * r - variance * r * r * r;
+            if (!isfinite(inverse) || inverse <= 0.0) {
+                rc = orc_fail(ORC_BLOCK_INVERSE, "generic_jla_observation_adjustment",
+                              "finite-projection observation inverse is nonpositive");
+                break;
+            }
+            ss_add(&inverse_sum, inverse);
+            ss_add(&lev, projection);
+            ss_add(&rs, residual);
+            ss_add(&bs, bias);
+            ss_add(&vs, variance);
+        }
+        if (rc) break;
+        double f = (double)p->unit_count[g];
+        uint32_t cell = p->unit_cell[g];
+        double e = p->outcome[g] - res->fitted_cell[cell];
+        double deleted = e * (ss_fin(inverse_sum) / f);
+        if (!isfinite(deleted)) {
+            rc = orc_fail(ORC_CORRECTION_NONFINITE, "generic_jla_observation_adjustment", "observation deleted residual is nonfinite");
+            break;
+        }
+        res->projection_share[g] = ss_fin(lev) / f;
+        res->residual_share[g] = ss_fin(rs) / f;
+        res->finite_bias[g] = ss_fin(bs) / f;
+        res->finite_variance[g] = ss_fin(vs) / f;
+        res->residual_mass[g] = e;
+        res->deleted_mass[g] = deleted;
+        ss_add(&cw[cell], f * p->outcome[g] * deleted);
+    }
+    if (!rc) {
+        res->cell_weight = orc_alloc(C, 8);
+        for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);
+    }
+    free(cw);
+    return rc;
+}
+
 static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
                       orc_numerical_result *num) {
     if (opt.probes < 2)
@@ -797,6 +949,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     }
     column_t *fit = alloc_cols(p, 1);
     five_moments *mom = NULL;
+    obs_state obs = {NULL, NULL, NULL, NULL, NULL};
     rc = solve_batch(&s, wr, fr, 1, opt.fit_tolerance, 0.01, &opt, fit);
     if (rc) {
         free_cols(fit, 1);
@@ -819,14 +972,22 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     free_cols(fit, 1);
     if (rc) goto fail;
 
-    /* leverage loop: engine.rs:1150-1246 */
+    /* leverage loop: engine.rs:1150-1246; observation deletion:
+     * generic_jla.rs:5540-5892 */
+    if (p->observation && num) {
+        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment under observation deletion");
+        goto fail;
+    }
+    if (p->observation) observation_state_init(p, &obs);
     mom = orc_alloc(G, sizeof *mom);
     res->lev_iters = orc_alloc(R, 8);
     uint64_t width = opt.leverage_width ? opt.leverage_width : 1;
     for (uint64_t first = 0; first < R; first += width) {
         uint64_t k = R - first < width ? R - first : width;
-        if ((rc = orc_rademacher_fill(opt.seed, ORC_TAG_LEVERAGE, first, k, p->unit_rank,
-                                      p->unit_count, G, atoms)))
+        if (p->observation)
+            observation_atoms(p, opt.seed, first, k, atoms);
+        else if ((rc = orc_rademacher_fill(opt.seed, ORC_TAG_LEVERAGE, first, k, p->unit_rank,
+                                           p->unit_count, G, atoms)))
             goto fail_mom;
         for (uint64_t c = 0; c < k; ++c) leverage_rhs(p, atoms + c * G, wr + c * W, fr + c * F);
         if ((rc = solve_batch(&s, wr, fr, k, opt.probe_tolerance, 1.0, &opt, cols))) goto fail_mom;
@@ -834,6 +995,10 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             res->lev_iters[first + c] = (double)cols[c].iterations;
             res->sum.leverage_iterations += cols[c].iterations;
             max_complete = fmax(max_complete, cols[c].complete);
+        }
+        if (p->observation) {
+            observation_moments(p, &obs, opt.seed, first, k, cols);
+            continue;
         }
         for (uint64_t g = 0; g < G; ++g) {
             uint32_t cell = p->unit_cell[g];
@@ -852,7 +1017,10 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     }
     /* leverage adjustment: engine.rs:2269-2410 (outcome mean 0, centering None) */
     int clipped = 0;
-    {
+    if (p->observation) {
+        rc = observation_adjustment(p, &obs, &opt, res);
+        if (rc) goto fail_mom;
+    } else {
         double probes = (double)R;
         res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
         res->finite_bias = orc_alloc(G, 8); res->finite_variance = orc_alloc(G, 8);
@@ -1022,6 +1190,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         if (rc) goto fail;
     }
     res->sum.solve_seconds = now_seconds() - t0;
+    observation_state_free(&obs);
     free(atoms);
     free_cols(cols, 2 * maxw);
     free(wr);
@@ -1032,6 +1201,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
 fail_mom:
     free(mom);
 fail:
+    observation_state_free(&obs);
     free(ns.grad);
     free(ns.offs);
     free(ns.folds);

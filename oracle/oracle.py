@@ -138,6 +138,7 @@ def lib():
         L = C.CDLL(path)
         L.orc_last_error.restype = C.c_char_p
         L.orc_rng_word.restype = C.c_uint64
+        L.orc_observation_atoms.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]
         L.orc_graph_from_edges.restype = C.c_void_p
         L.orc_graph_from_edges.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_graph_vertices.restype = C.c_uint64
@@ -333,7 +334,7 @@ class Problem:
               "stratum_cell": (6, np.uint32, 8), "stratum_count": (7, np.uint64, 8),
               "stratum_rank": (8, np.uint64, 8), "stratum_per_copy": (9, np.float64, 8),
               "cell_outcome_sum": (10, np.float64, 5), "cell_target_sum": (11, np.float64, 5),
-              "unit_outcome_sum": (12, np.float64, 6)}
+              "unit_outcome_sum": (12, np.float64, 6), "unit_offset": (13, np.uint64, 6)}
 
     def __init__(self, worker, firm, deletion, outcome, frequency, target, mask=None, observation=False):
         cols = [np.ascontiguousarray(worker, np.uint64), np.ascontiguousarray(firm, np.uint64),
@@ -368,6 +369,11 @@ class Problem:
         out = np.zeros(n, dtype)
         _check(lib().orc_problem_array(self._h, which, out.ctypes.data))
         return out
+
+    def observation_atoms(self, seed, first, columns):
+        out = np.zeros(columns * self.units, np.int64)
+        _check(lib().orc_observation_atoms(self._h, seed, first, columns, out.ctypes.data))
+        return out.reshape(columns, self.units)
 
     def hybrid_graph(self):
         h = C.c_void_p()

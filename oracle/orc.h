@@ -103,6 +103,9 @@ int orc_prepare_observation(uint64_t rows, const uint64_t *worker, const uint64_
                             const uint64_t *deletion, const double *outcome,
                             const uint64_t *frequency, const double *target, orc_problem **out);
 void orc_problem_free(orc_problem *p);
+/* Observation deletion: leverage atoms of `columns` probes from `first`, one
+ * per stored row, out[col * rows + row] (generic_jla.rs:5660-5706). */
+int orc_observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t columns, int64_t *out);
 /* dims[0..]: input_rows, retained_rows, retained_physical, workers, firms,
  * cells, deletion_units, target_strata_exact, plan_target_strata */
 void orc_problem_dims(const orc_problem *p, uint64_t dims[9]);
@@ -113,7 +116,8 @@ void orc_problem_mask(const orc_problem *p, uint8_t *mask);
  *        3 unit_cell(u32) 4 unit_count(u64) 5 unit_rank(u64)
  *        6 stratum_cell(u32) 7 stratum_count(u64) 8 stratum_rank(u64)
  *        9 stratum_per_copy(f64) 10 cell_outcome_sum(f64)
- *        11 cell_target_sum(f64) 12 unit_outcome_sum(f64) */
+ *        11 cell_target_sum(f64) 12 unit_outcome_sum(f64)
+ *        13 unit_offset(u64; observation deletion, else zero) */
 int orc_problem_array(const orc_problem *p, int which, void *dst);
 
 /* ---- CMG (rust/vendor/cmg) --------------------------------------------- */

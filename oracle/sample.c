@@ -481,6 +481,22 @@ static int cmp_semantic_rows(const void *a, const void *b, void *arg) {
     return x < y ? -1 : (x > y);
 }
 
+/* generic_jla.rs:4305-4334 (compare_observation_rows), no controls and no
+ * probe order: worker, firm, canonical per-copy target mass, outcome */
+static int cmp_observation_key(const orc_problem *p, uint64_t x, uint64_t y) {
+    if (p->row_worker[x] != p->row_worker[y]) return p->row_worker[x] < p->row_worker[y] ? -1 : 1;
+    if (p->row_firm[x] != p->row_firm[y]) return p->row_firm[x] < p->row_firm[y] ? -1 : 1;
+    int c = orc_total_cmp(orc_canonical_zero(p->per_copy[x]), orc_canonical_zero(p->per_copy[y]));
+    if (c) return c;
+    return orc_total_cmp(p->outcome[x], p->outcome[y]);
+}
+static int cmp_observation_rows(const void *a, const void *b, void *arg) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    int c = cmp_observation_key(arg, x, y);
+    if (c) return c;
+    return x < y ? -1 : (x > y);
+}
+
 /* jla.rs:636-646 (target_plan ordering) */
 static int cmp_target_plan(const void *a, const void *b, void *arg) {
     const orc_problem *p = arg;
@@ -742,6 +758,42 @@ static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *f
         }
         p->per_copy[i] = v;
     }
+    p->observation = observation;
+    if (observation) {
+        /* generic_jla.rs:4267-4303 (observation_row_ranks), 4158-4215
+         * (observation_classes: rows of equal rank, ascending) and
+         * statistical_batches.rs:15-53 (per-row copy offsets) */
+        for (uint64_t i = 0; i < R; ++i) order[i] = i;
+        qsort_r(order, R, 8, cmp_observation_rows, p);
+        p->row_rank = orc_alloc(R, 8);
+        p->unit_offset = orc_alloc(R, 8);
+        uint64_t current = 0, offset = 0;
+        for (uint64_t i = 0; i < R; ++i) {
+            if (i == 0 || cmp_observation_key(p, order[i - 1], order[i]) != 0) {
+                ++current;
+                offset = 0;
+            }
+            p->row_rank[order[i]] = current;
+            p->unit_offset[order[i]] = offset;
+            offset += p->frequency[order[i]];
+        }
+        G = R;
+        p->units = R;
+        free(p->unit_ptr);
+        p->unit_ptr = orc_alloc(R + 1, 8);
+        for (uint64_t i = 0; i <= R; ++i) p->unit_ptr[i] = i;
+        for (uint64_t i = 0; i < R; ++i) p->unit_rows[i] = i;
+        p->unit_cell = orc_alloc(R, 4); p->unit_count = orc_alloc(R, 8);
+        p->unit_outcome = orc_alloc(R, 8); p->unit_target = orc_alloc(R, 8);
+        p->unit_rank = orc_alloc(R, 8);
+        for (uint64_t i = 0; i < R; ++i) {
+            p->unit_cell[i] = p->row_cell[i];
+            p->unit_count[i] = p->frequency[i];
+            p->unit_outcome[i] = (double)p->frequency[i] * p->outcome[i];
+            p->unit_target[i] = p->target[i];
+            p->unit_rank[i] = p->row_rank[i];
+        }
+    } else {
     /* jla.rs:413-480 (semantic_row_ranks) */
     for (uint64_t i = 0; i < R; ++i) order[i] = i;
     qsort_r(order, R, 8, cmp_semantic_rows, p);
@@ -780,6 +832,7 @@ static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *f
         p->unit_outcome[g] = os + oc;
         p->unit_target[g] = ts + tc;
         p->unit_rank[g] = min_rank;
+    }
     }
     /* jla.rs:651-790 (target_plan) */
     for (uint64_t i = 0; i < R; ++i) order[i] = i;
@@ -835,7 +888,7 @@ void orc_problem_free(orc_problem *p) {
                     p->worker_ptr, p->firm_ptr, p->firm_cells, p->unit_ptr, p->unit_rows,
                     p->per_copy, p->row_rank, p->unit_cell, p->unit_count, p->unit_outcome,
                     p->unit_target, p->unit_rank, p->stratum_cell, p->stratum_per_copy,
-                    p->stratum_count, p->stratum_target, p->stratum_rank};
+                    p->stratum_count, p->stratum_target, p->stratum_rank, p->unit_offset};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(p);
 }
@@ -870,6 +923,10 @@ int orc_problem_array(const orc_problem *p, int which, void *dst) {
     case 10: memcpy(dst, p->cell_outcome_sum, p->cells * 8); break;
     case 11: memcpy(dst, p->cell_target_sum, p->cells * 8); break;
     case 12: memcpy(dst, p->unit_outcome, p->units * 8); break;
+    case 13:
+        if (p->unit_offset) memcpy(dst, p->unit_offset, p->units * 8);
+        else memset(dst, 0, p->units * 8);
+        break;
     default: return orc_fail(ORC_INVALID_INPUT, "problem", "unknown array %d", which);
     }
     return ORC_OK;

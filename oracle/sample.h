@@ -41,6 +41,10 @@ struct orc_problem {
     uint32_t *stratum_cell;
     double *stratum_per_copy, *stratum_target;
     uint64_t *stratum_count, *stratum_rank;
+    /* observation deletion: one unit per stored row; its first copy in the
+     * row's observation class (generic_jla.rs:4158-4215) */
+    int observation;
+    uint64_t *unit_offset;
 };
 
 #endif

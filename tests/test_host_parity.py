@@ -57,9 +57,9 @@ def kat_columns():
                ("worker", "firm", "deletion", "outcome", "frequency", "target")]
 
 
-def _compare_prepared(oracle, cols):
-    p = oracle.Problem(*cols)
-    e = F.Engine(*cols, upload=False)
+def _compare_prepared(oracle, cols, observation=False):
+    p = oracle.Problem(*cols, observation=observation)
+    e = F.Engine(*cols, upload=False, deletion_mode="observation" if observation else "match")
     try:
         d = e.dims
         assert (d["input_rows"], d["rows"], d["physical"], d["workers"], d["firms"], d["cells"],
@@ -70,7 +70,8 @@ def _compare_prepared(oracle, cols):
         np.testing.assert_array_equal(e.mask(), p.mask())
         for which, name, n in [(20, "unit_rank", p.units), (21, "unit_count", p.units),
                                (22, "stratum_rank", p.strata), (23, "stratum_count", p.strata),
-                               (24, "cell_weight", p.cells), (25, "unit_cell", p.units)]:
+                               (24, "cell_weight", p.cells), (25, "unit_cell", p.units),
+                               (26, "unit_offset", p.units)]:
             got = e.array(which, n)
             want = p.array(name).astype(np.float64)
             assert np.array_equal(got, want), name
@@ -98,6 +99,36 @@ def test_prepare_synthetic_panels(oracle, seed):
     p, e = _compare_prepared(oracle, cols)
     assert e.dims["cells"] > 1000
     e.release()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_prepare_observation_plan(oracle, seed):
+    """Observation deletion: one unit per stored row with its class rank
+    (generic_jla.rs:4267-4303) and copy offset; strata keyed by the least
+    observation rank (:3902-3995).  Duplicated rows share a class and take
+    consecutive copies."""
+    from fevc_amd import synth
+    spec = synth.PanelSpec(workers=800, firms=60, periods=4, mobility=0.3, seed=seed, name=f"o{seed}")
+    data = synth.generate(spec)
+    cols = [np.asarray(data[k], float) for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    rng = np.random.default_rng(seed)
+    cols[4] = rng.integers(1, 4, cols[0].size).astype(float)
+    cols[5] = cols[4] * rng.uniform(0.5, 1.5, cols[0].size)
+    # duplicate some rows (same worker, firm, outcome and per-copy target mass)
+    dup = rng.choice(cols[0].size, 40, replace=False)
+    cols = [np.concatenate((c, c[dup])) for c in cols]
+    cols[2][-40:] = cols[2].max() + 1 + np.arange(40)  # their own deletion identifiers
+    p, e = _compare_prepared(oracle, cols, observation=True)
+    try:
+        assert e.dims["units"] == e.dims["rows"]
+        # the device's cell -> unit gather lists every stored row once, cells ascending
+        gather = e.array(27, e.dims["units"]).astype(np.int64)
+        assert np.array_equal(np.sort(gather), np.arange(e.dims["units"]))
+        assert np.all(np.diff(e.array(25, e.dims["units"])[gather]) >= 0)
+        offsets = e.array(26, e.dims["units"])
+        assert offsets.max() >= 1  # a duplicated row continues its class's copies
+    finally:
+        e.release()
 
 
 def test_prepare_refuses_bad_input():
