@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--compare-plain", type=int, default=1,
                     help="also run the KSS estimate on the plain single-RHS FP64 path and report the error")
     ap.add_argument("--exact", type=int, default=1, help="time the dense exact estimator on the small panel")
+    ap.add_argument("--observation", type=int, default=1,
+                    help="KSS estimate under observation deletion on one 40k-worker subsample (0 skips)")
     ap.add_argument("--learned-eval", type=int, default=0,
                     help="leverage batches of the held-out 10M-worker smoother evaluation (0 skips)")
     ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
@@ -297,6 +299,33 @@ def main():
         log(f"cpu baseline: {cpu['value']:.2f} solves/s")
 
     traffic = measured_traffic(args.width, args.precision)
+    observation = None
+    if args.observation:
+        # one subsample of the many-subsample observation-deletion workload at its
+        # median size: 40 000 workers x 4 periods, 2 000 firms (W / 20), mobility
+        # 0.15, Pareto firm sizes; 128 probes, probe solves to rel.res 1e-8
+        sub = synth.PanelSpec(workers=40000, firms=2000, periods=4, mobility=0.15, seed=1,
+                              name="akm-40k-workers-2k-firms-T4-observation")
+        od = synth.generate(sub)
+        ocols = [np.asarray(od[k]) for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+        observation = {"panel": sub.name, "probes": 128}
+        for mode in ("observation", "match"):
+            oe = F.Engine(*ocols, deletion_mode=mode)
+            try:
+                oe.configure(args.precision, 64, 32)
+                oe.solve(probes=16, tolerance=args.tolerance)  # warm: smoother calibration, code objects
+                t0 = time.perf_counter()
+                ores = oe.solve(probes=128, tolerance=args.tolerance)
+                ow = time.perf_counter() - t0
+                entry = {"rows": oe.dims["rows"], "deletion_units": oe.dims["units"],
+                         "physical": oe.dims["physical"], "wall_seconds": round(ow, 4),
+                         "probe_solves_per_s": round(3 * 128 / ow, 1), "corrected": ores["corrected"],
+                         "numerical_mcse": ores["numerical_mcse"]}
+            finally:
+                oe.release()
+            observation[mode] = entry
+            log(f"{mode} deletion KSS ({sub.name}, 128 probes): {ow:.3f}s, corrected {ores['corrected']}")
+
     line = {
         "metric": "probe solves/s at rel.res 1e-8 (leverage probes, learned+batched CMG-PCG)",
         "value": round(value, 3),
@@ -334,6 +363,7 @@ def main():
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,
         "exact_estimate": exact,
+        "observation_estimate": observation,
         "learned_eval_10M": learned_eval,
         "stream_copy_GBs": round(2 * copy_bytes / copy_ms / 1e6, 1),
         "prepare_seconds": round(prep_s, 2),
