@@ -159,6 +159,36 @@ class SnapshotV1(C.Structure):
 
 
 # (name, restype, argtypes) of every exported symbol declared in include/*.h
+class StayerRequestV1(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("struct_size", C.c_uint32), ("rows", C.c_uint64),
+                ("controls_count", C.c_uint32), ("reserved", C.c_uint32), ("caller_copy_bytes", C.c_uint64)]
+
+
+class StayerRequestInterruptV1(C.Structure):
+    _fields_ = [("options", StayerRequestV1), ("interrupt_poll", C.c_void_p), ("interrupt_context", C.c_void_p),
+                ("checkpoint_interval", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class StayerColumnsV1(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("reserved", C.c_uint32), ("rows", C.c_uint64),
+                ("firm", C.c_void_p), ("worker", C.c_void_p), ("outcome", C.c_void_p), ("frequency", C.c_void_p),
+                ("target_weight", C.c_void_p), ("controls", C.c_void_p), ("controls_count", C.c_uint32),
+                ("reserved_2", C.c_uint32)]
+
+
+class StayerReceiptV1(C.Structure):
+    _fields_ = ([("struct_size", C.c_uint32), ("schema_version", C.c_uint32), ("generation", C.c_uint64)] +
+                [(k, C.c_uint64) for k in ("mover_stored_rows", "stayer_stored_rows", "combined_stored_rows",
+                                           "mover_physical_mass", "stayer_physical_mass", "combined_physical_mass",
+                                           "mover_workers", "stayer_workers", "combined_workers", "firms",
+                                           "mover_deletion_units", "stayer_deletion_units",
+                                           "combined_deletion_units")] +
+                [(k, C.c_double) for k in ("mover_target_mass", "stayer_target_mass", "combined_target_mass")] +
+                [(k, C.c_uint64) for k in ("topology_checksum", "memory_limit_bytes", "caller_copy_bytes",
+                                           "augmentation_peak_forecast_bytes", "augmented_resident_bytes",
+                                           "total_prepared_resident_bytes")])
+
+
 SIGNATURES = [
     ("vckss_rust_abi_version", C.c_uint32, []),
     ("vckss_rust_backend_version", C.c_char_p, []),
@@ -172,6 +202,14 @@ SIGNATURES = [
     ("vckss_rust_engine_prepare_interrupt_v4", C.c_int32,
      [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64), C.c_uint32]),
     ("vckss_rust_engine_retained_mask_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint64]),
+    ("vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1", C.c_int32,
+     [C.POINTER(StayerRequestInterruptV1), C.c_uint32]),
+    ("vckss_rust_engine_augment_stayers_v1", C.c_int32,
+     [C.c_uint64, C.POINTER(StayerRequestV1), C.POINTER(StayerColumnsV1)]),
+    ("vckss_rust_engine_augment_stayers_interrupt_v1", C.c_int32,
+     [C.c_uint64, C.POINTER(StayerRequestInterruptV1), C.POINTER(StayerColumnsV1)]),
+    ("vckss_rust_engine_stayer_augmentation_receipt_v1", C.c_int32,
+     [C.c_uint64, C.POINTER(StayerReceiptV1), C.c_uint32]),
     ("vckss_rust_engine_solve_v5", C.c_int32, [C.c_uint64, C.POINTER(SolveRequestV5)]),
     ("vckss_rust_engine_solve_interrupt_v5", C.c_int32,
      [C.c_uint64, C.POINTER(SolveRequestInterruptV5)]),
@@ -313,6 +351,7 @@ class Engine:
         exact route, and the JLA route without numerical MCSE)."""
         lib = library()
         self.deletion_mode = {"match": 1, "observation": 2}[deletion_mode]
+        self.stayers_mode = 1  # movers; 2 (all) after augment_stayers
         cols = [_f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency), _f64(target)]
         self._cols = cols
         rows = len(cols[0])
@@ -339,6 +378,32 @@ class Engine:
                 "terminal_dimension", "topology_checksum"]
         self.dims = {k: int(v) for k, v in zip(keys, d)}
 
+    def augment_stayers(self, firm, worker, outcome, frequency, target):
+        """Append eligible stayer rows (1-based dense retained-mover firm and
+        stayer-worker indices); later solves take stayers_mode ALL."""
+        cols = [_f64(firm), _f64(worker), _f64(outcome), _f64(frequency), _f64(target)]
+        self._stayer_cols = cols
+        rows = len(cols[0])
+        r = StayerRequestInterruptV1()
+        _check(library().vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1(C.byref(r),
+                                                                                             C.sizeof(r)))
+        r.options.rows = rows
+        c = StayerColumnsV1()
+        c.struct_size, c.rows = C.sizeof(c), rows
+        if rows:
+            c.firm, c.worker, c.outcome, c.frequency, c.target_weight = [a.ctypes.data for a in cols]
+        _check(library().vckss_rust_engine_augment_stayers_interrupt_v1(self.generation, C.byref(r), C.byref(c)))
+        self.stayers_mode = 2
+        d = np.zeros(15, np.uint64)
+        _check(library().vckss_mi355x_dims_v1(self.generation, d.ctypes.data, 15))
+        self.dims.update({k: int(v) for k, v in zip(list(self.dims), d)})
+        return self.stayer_receipt()
+
+    def stayer_receipt(self):
+        o = StayerReceiptV1()
+        _check(library().vckss_rust_engine_stayer_augmentation_receipt_v1(self.generation, C.byref(o), C.sizeof(o)))
+        return {name: getattr(o, name) for name, _ in o._fields_}
+
     def mask(self):
         m = np.zeros(self.dims["input_rows"], np.uint8)
         _check(library().vckss_rust_engine_retained_mask_v1(self.generation, m.ctypes.data, m.size))
@@ -356,6 +421,7 @@ class Engine:
         o.full_cmg_v2 = 1
         o.threads = 1
         o.deletion_mode = self.deletion_mode
+        o.stayers_mode = self.stayers_mode
         o.seed = seed
         o.probes = probes
         o.pcg_tolerance = fit_tolerance
@@ -404,6 +470,7 @@ class Engine:
     def numerical_request(self, **kw):
         r = default_numerical_request(**kw)
         r.v1.point.deletion_mode = self.deletion_mode
+        r.v1.point.stayers_mode = self.stayers_mode
         return r
 
     def solve_numerical(self, request=None, **kw):

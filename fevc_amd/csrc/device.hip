@@ -724,12 +724,20 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.cell_unit_ptr.upload(p.cell_unit_ptr); E.cell_units.upload(p.cell_units);
   E.unit_cell.upload(p.unit_cell); E.unit_count.upload(p.unit_count); E.unit_rank.upload(p.unit_rank);
   E.unit_outcome.upload(p.unit_outcome);
-  if (p.observation) {
-    std::vector<uint64_t> phys(p.rows + 1, 0);
-    for (uint64_t r = 0; r < p.rows; ++r) phys[r + 1] = phys[r] + p.frequency[r];
+  E.Gm = p.match_units;
+  E.Go = p.units - p.match_units;
+  if (E.Go) {
+    const uint64_t first_row = p.rows - E.Go;  // observation units are the last stored rows
+    std::vector<uint64_t> phys(E.Go + 1, 0);
+    std::vector<double> y(E.Go);
+    for (uint64_t i = 0; i < E.Go; ++i) {
+      phys[i + 1] = phys[i] + p.frequency[first_row + i];
+      y[i] = p.outcome[first_row + i];
+    }
+    E.obs_physical = phys[E.Go];
     E.unit_offset.upload(p.unit_offset);
     E.row_phys.upload(phys);
-    E.row_outcome.upload(p.outcome);
+    E.row_outcome.upload(y);
   }
   E.stratum_per_copy.upload(p.stratum_per_copy); E.stratum_rank.upload(p.stratum_rank);
   E.stratum_count.upload(p.stratum_count); E.cell_stratum_ptr.upload(p.cell_stratum_ptr);
@@ -771,8 +779,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
-  if (P.observation && opt.numerical)
-    fail(kUnsupported, "numerical_solve", "no numerical attachment under observation deletion");
+  if (E.Go && opt.numerical)
+    fail(kUnsupported, "numerical_solve", "no numerical attachment with observation units");
   const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
   const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
   const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
@@ -814,9 +822,11 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   E.moments.alloc(10 * E.G);
   VK_HIP(hipMemsetAsync(E.moments.p, 0, 10 * E.G * sizeof(double), E.stream));
   E.atoms.alloc(std::max<uint64_t>((uint64_t)KL * E.G, (uint64_t)KT * E.S));
-  if (P.observation) {
-    E.obs_corr.alloc(4 * P.physical_total);
-    VK_HIP(hipMemsetAsync(E.obs_corr.p, 0, 4 * P.physical_total * sizeof(double), E.stream));
+  if (E.Go) {
+    E.obs_m.alloc(4 * E.Go);
+    VK_HIP(hipMemsetAsync(E.obs_m.p, 0, 4 * E.Go * sizeof(double), E.stream));
+    E.obs_corr.alloc(4 * E.obs_physical);
+    VK_HIP(hipMemsetAsync(E.obs_corr.p, 0, 4 * E.obs_physical * sizeof(double), E.stream));
   }
   for (uint64_t first = 0; first < R; first += lw) {
     const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
@@ -831,15 +841,16 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
                                                  E.frhs.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
-      if (P.observation)
-        hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k,
-                           first, opt.seed, kTagLeverage, E.unit_cell.p, E.unit_count.p, E.unit_rank.p,
-                           E.unit_offset.p, E.row_phys.p, P.physical_total, E.cell_worker.p, E.cell_firm.p,
-                           E.worker.p, E.fsol.p, E.moments.p, E.obs_corr.p, E.status.p);
-      else
-        hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, k,
+      if (E.Gm)
+        hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.Gm * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Gm, k,
                            E.unit_cell.p, E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p,
                            E.atoms.p, E.moments.p, E.status.p);
+      if (E.Go)  // generic_jla.rs:5540-5892, units [Gm, G)
+        hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.Go * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Go,
+                           k, first, opt.seed, kTagLeverage, E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm,
+                           E.unit_rank.p + E.Gm, E.unit_offset.p + E.Gm, E.row_phys.p, E.obs_physical,
+                           E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.obs_m.p, E.obs_corr.p,
+                           E.status.p);
     });
     for (auto &c : rc) {
       res.leverage.push_back(c);
@@ -850,24 +861,36 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   res.seconds_leverage = std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
 
   // ---- leverage adjustment (engine.rs:2269-2410)
-  E.adjust.alloc(7 * E.G);
+  E.adjust.alloc(7 * std::max<uint64_t>(E.Gm, 1));
   E.cellw.alloc(E.C);
-  if (P.observation)  // generic_jla.rs:5893-6017
-    hipLaunchKernelGGL(dev::k_obs_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.obs_corr.p,
-                       P.physical_total, E.row_phys.p, E.unit_count.p, E.unit_cell.p, E.row_outcome.p, E.fitted.p,
-                       dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
-  else
-    hipLaunchKernelGGL(dev::k_adjust, grid_for(E.G), kBlock, 0, E.stream, E.G, E.moments.p, E.unit_cell.p,
-                       E.unit_count.p, E.unit_outcome.p, E.fitted.p,
-                       dev::AdjustParams{(double)R, opt.block_tolerance, opt.rank_tolerance}, E.adjust.p, E.status.p);
+  const dev::AdjustParams aprm{(double)R, opt.block_tolerance, opt.rank_tolerance};
+  if (E.Gm)
+    hipLaunchKernelGGL(dev::k_adjust, grid_for(E.Gm), kBlock, 0, E.stream, E.Gm, E.moments.p, E.unit_cell.p,
+                       E.unit_count.p, E.unit_outcome.p, E.fitted.p, aprm, E.adjust.p, E.status.p);
+  if (E.Go) {  // generic_jla.rs:5893-6017
+    E.obs_adjust.alloc(7 * E.Go);
+    hipLaunchKernelGGL(dev::k_obs_adjust, grid_for(E.Go), kBlock, 0, E.stream, E.Go, E.obs_m.p, E.obs_corr.p,
+                       E.obs_physical, E.row_phys.p, E.unit_count.p + E.Gm, E.unit_cell.p + E.Gm, E.row_outcome.p,
+                       E.fitted.p, aprm, E.obs_adjust.p, E.status.p);
+  }
   E.check_status("jla_leverage_adjustment");
+  // per-unit correction weight contributions, match units then observation units
+  E.contrib.alloc(E.G);
+  if (E.Gm)
+    VK_HIP(hipMemcpyAsync(E.contrib.p, E.adjust.p + 6 * E.Gm, E.Gm * sizeof(double), hipMemcpyDeviceToDevice,
+                          E.stream));
+  if (E.Go)
+    VK_HIP(hipMemcpyAsync(E.contrib.p + E.Gm, E.obs_adjust.p + 6 * E.Go, E.Go * sizeof(double),
+                          hipMemcpyDeviceToDevice, E.stream));
   hipLaunchKernelGGL(dev::k_cell_weight, grid_for(E.C), kBlock, 0, E.stream, E.C, E.cell_unit_ptr.p,
-                     E.cell_units.p, E.adjust.p + 6 * E.G, E.cellw.p);
+                     E.cell_units.p, E.contrib.p, E.cellw.p);
   {
-    std::vector<double> adj;
-    E.adjust.download(adj, 6 * E.G, E.stream);
+    std::vector<double> adj, oadj;
+    if (E.Gm) E.adjust.download(adj, 6 * E.Gm, E.stream);
+    if (E.Go) E.obs_adjust.download(oadj, 6 * E.Go, E.stream);
     auto slice = [&](int q, std::vector<double> &out) {
-      out.assign(adj.begin() + q * E.G, adj.begin() + (q + 1) * E.G);
+      out.assign(adj.begin() + q * E.Gm, adj.begin() + (q + 1) * E.Gm);
+      out.insert(out.end(), oadj.begin() + q * E.Go, oadj.begin() + (q + 1) * E.Go);
     };
     slice(0, res.projection_share); slice(1, res.residual_share); slice(2, res.finite_bias);
     slice(3, res.finite_variance); slice(4, res.residual_mass); slice(5, res.deleted_mass);

@@ -205,16 +205,22 @@ struct DeviceEngine::Impl {
   const Hierarchy *H = nullptr;
   uint32_t chunk = kChunkEntries;
   uint64_t W = 0, F = 0, C = 0, G = 0, S = 0, n0 = 0, aux = 0;
+  // units [0, Gm) are match units, [Gm, G) observation units (Go of them, the
+  // last Go stored rows, obs_physical physical copies)
+  uint64_t Gm = 0, Go = 0, obs_physical = 0;
   double target_total = 0, bound = 0;
   DBuf<uint64_t> worker_ptr, firm_ptr, cell_unit_ptr, cell_stratum_ptr, aux_ptr;
   DBuf<uint32_t> firm_cells, cell_worker, cell_firm, cell_units, unit_cell, aux_nbr;
   DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
   DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
-  // observation deletion (null otherwise): per row the first copy in its
-  // class, its first physical copy (prefix of frequencies), its outcome; per
-  // physical copy the probe sums of s p and s p^3 (sum, correction)
+  // observation units (null without them): per unit the first copy in its
+  // class (all units; zero for match units); per observation unit its first
+  // physical copy (prefix of frequencies) and outcome; per observation unit
+  // the probe sums of p^2 and p^4 and per physical copy those of s p and s p^3
+  // (sum, correction); the observation adjustment; the per-unit correction
+  // weight contributions of both kinds
   DBuf<uint64_t> unit_offset, row_phys;
-  DBuf<double> row_outcome, obs_corr;
+  DBuf<double> row_outcome, obs_m, obs_corr, obs_adjust, contrib;
   std::vector<DevLevel> levels;
   DBuf<uint32_t> perm0, pos0;  // level-0 renumbering
   DevChunks firm_chunks;       // firms over their cells (firm_cells order)

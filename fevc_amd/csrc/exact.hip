@@ -561,10 +561,16 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   VK_HIP(hipStreamSynchronize(st));
   Neumaier corr[3];
   double max_leverage = 0.0, gap = std::numeric_limits<double>::infinity();
-  if (P.observation) {
-    // observation deletion (exact_estimator.rs:914-987): every physical copy
-    // of a stored row is its own block, of leverage h_c and maker 1 - h_c
-    for (uint64_t r = 0; r < P.rows; ++r) {
+  // units [0, match_units) are match blocks; the observation units are the
+  // last stored rows (all rows under observation deletion, the stayers in a
+  // stayer hybrid, exact_estimator.rs:914-987 and its stayer plan)
+  const uint64_t observation_units = P.units - P.match_units;
+  uint64_t deleted_copies = 0;
+  {
+    // observation deletion: every physical copy of a stored row is its own
+    // block, of leverage h_c and maker 1 - h_c
+    for (uint64_t r = P.rows - observation_units; r < P.rows; ++r) {
+      deleted_copies += P.frequency[r];
       const uint32_t c = P.row_cell[r];
       const double h = cell[4 * (uint64_t)c];
       const double maker = 1.0 - h;
@@ -580,7 +586,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
     }
   }
   // match blocks in closed form (exact_estimator.rs:1448-1696)
-  for (uint64_t g = 0; g < (P.observation ? 0 : P.units); ++g) {
+  for (uint64_t g = 0; g < P.match_units; ++g) {
     const uint32_t c = P.unit_cell[g];
     const double h = cell[4 * (uint64_t)c], n = (double)P.unit_count[g];
     const double maker = 1.0 - h * n;
@@ -605,7 +611,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   res.has_exact = true;
   ExactReceipt &x = res.exact;
   x.parameters = (uint64_t)W + F - 1;
-  x.deletion_units = P.observation ? P.physical_total : P.units;
+  x.deletion_units = P.match_units + deleted_copies;
   x.cells = C;
   x.max_leverage = max_leverage;
   x.deletion_rank_gap = gap;

@@ -398,12 +398,10 @@ int cmp_total(double a, double b) {
   return x < y ? -1 : (x > y ? 1 : 0);
 }
 
-// Observation deletion plan (generic_jla.rs:4158-4330, statistical_batches.rs:15-53):
-// semantic ranks over (worker, firm, per-copy target mass, outcome), classes
-// of equal rank (rows ascending), and one unit per stored row holding the
-// class rank and the row's copy offset in its class, so that the row's atom
-// is the sum of its own physical copies' Rademacher signs.
-void observation_plan(Prepared &P, std::vector<uint32_t> &order) {
+// Observation semantic ranks (generic_jla.rs:4267-4334, no controls): dense
+// 1-based ranks over (worker, firm, per-copy target mass, outcome); `order`
+// returns the rows sorted by (rank, row).
+std::vector<uint64_t> observation_ranks(const Prepared &P, std::vector<uint32_t> &order) {
   const uint64_t R = P.rows;
   auto cmp = [&](uint32_t a, uint32_t b) {
     if (P.row_worker[a] != P.row_worker[b]) return P.row_worker[a] < P.row_worker[b] ? -1 : 1;
@@ -412,24 +410,37 @@ void observation_plan(Prepared &P, std::vector<uint32_t> &order) {
     if (c) return c;
     return cmp_total(P.outcome[a], P.outcome[b]);
   };
+  order.resize(R);
   std::iota(order.begin(), order.end(), 0u);
   std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     int c = cmp(a, b);
     return c ? c < 0 : a < b;
   });
-  P.row_rank.assign(R, 0);
-  P.unit_offset.assign(R, 0);
-  uint64_t rank = 0, offset = 0;
+  std::vector<uint64_t> rank(R, 0);
+  uint64_t cur = 0;
   for (uint64_t i = 0; i < R; ++i) {
-    if (i == 0 || cmp(order[i - 1], order[i]) != 0) {
-      ++rank;
-      offset = 0;
-    }
-    P.row_rank[order[i]] = rank;
+    if (i == 0 || cmp(order[i - 1], order[i]) != 0) ++cur;
+    rank[order[i]] = cur;
+  }
+  return rank;
+}
+
+// Observation deletion plan (generic_jla.rs:4158-4215, statistical_batches.rs:15-53):
+// classes of equal observation rank (rows ascending), and one unit per stored
+// row holding the class rank and the row's copy offset in its class, so that
+// the row's atom is the sum of its own physical copies' Rademacher signs.
+void observation_plan(Prepared &P, std::vector<uint32_t> &order) {
+  const uint64_t R = P.rows;
+  P.row_rank = observation_ranks(P, order);
+  P.unit_offset.assign(R, 0);
+  uint64_t offset = 0;
+  for (uint64_t i = 0; i < R; ++i) {
+    if (i == 0 || P.row_rank[order[i - 1]] != P.row_rank[order[i]]) offset = 0;
     P.unit_offset[order[i]] = offset;  // rows of a class ascend in `order`
     offset += P.frequency[order[i]];
   }
   P.units = R;
+  P.match_units = 0;
   P.unit_ptr.resize(R + 1);
   std::iota(P.unit_ptr.begin(), P.unit_ptr.end(), 0ull);
   P.unit_rows.resize(R);
@@ -717,6 +728,103 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   P.cell_stratum_ptr.assign(C + 1, 0);
   for (uint64_t s = 0; s < P.strata; ++s) P.cell_stratum_ptr[P.stratum_cell[s] + 1]++;
   for (uint64_t c = 0; c < C; ++c) P.cell_stratum_ptr[c + 1] += P.cell_stratum_ptr[c];
+  if (!P.observation) P.match_units = P.units;
+  return P;
+}
+
+Prepared augment_stayers(const Prepared &M, const StayerColumns &s, StayerReceipt &rec) {
+  if (M.observation)
+    fail(kUnsupported, "session_stayer_augmentation", "the stayer hybrid requires match-deletion mover preparation");
+  if (M.stayers)
+    fail(kContextPoisoned, "session_stayer_augmentation", "the prepared generation already owns a stayer augmentation");
+  const uint64_t S = s.rows;
+  uint64_t SW = 0;
+  for (uint64_t r = 0; r < S; ++r) SW = std::max<uint64_t>(SW, s.worker[r]);
+  if ((S == 0) != (SW == 0))
+    fail(kInvalidIdentifier, "stayer_augmentation",
+         "eligible stayer workers must be a nonempty dense map when rows are supplied");
+  std::vector<uint64_t> worker_firm(SW, 0), worker_physical(SW, 0);
+  uint64_t stayer_physical = 0;
+  Neumaier stayer_target;
+  for (uint64_t r = 0; r < S; ++r) {
+    const uint64_t f = s.firm[r], w = s.worker[r];
+    if (f == 0 || f > M.firms)
+      fail(kInvalidIdentifier, "stayer_augmentation",
+           "stayer row " + std::to_string(r) + " is not attached to a retained mover firm");
+    if (w == 0 || w > SW)
+      fail(kInvalidIdentifier, "stayer_augmentation", "stayer row " + std::to_string(r) + " has a non-dense worker identifier");
+    if (worker_firm[w - 1] && worker_firm[w - 1] != f)
+      fail(kInvalidIdentifier, "stayer_augmentation",
+           "eligible stayer worker " + std::to_string(w) + " spans more than one firm");
+    worker_firm[w - 1] = f;
+    worker_physical[w - 1] += s.frequency[r];
+    stayer_physical += s.frequency[r];
+    stayer_target.add(s.target[r]);
+  }
+  for (uint64_t w = 0; w < SW; ++w)
+    if (!worker_firm[w]) fail(kInvalidIdentifier, "stayer_augmentation", "eligible stayer worker identifiers are not dense");
+  for (uint64_t w = 0; w < SW; ++w)
+    if (worker_physical[w] < 2)
+      fail(kInvalidInput, "stayer_augmentation",
+           "every eligible stayer worker must represent at least two physical observations");
+  // combined columns: the mover problem's dense identifiers, then the stayers
+  // with worker M.workers + w and one deletion unit per stored row
+  const uint64_t RM = M.rows, R = RM + S;
+  std::vector<uint64_t> cw(R), cf(R), cd(R), cq(R);
+  std::vector<double> cy(R), ct(R);
+  for (uint64_t r = 0; r < RM; ++r) {
+    cw[r] = M.row_worker[r] + 1ull; cf[r] = M.row_firm[r] + 1ull; cd[r] = M.row_deletion[r] + 1ull;
+    cy[r] = M.outcome[r]; cq[r] = M.frequency[r]; ct[r] = M.target[r];
+  }
+  for (uint64_t r = 0; r < S; ++r) {
+    cw[RM + r] = M.workers + s.worker[r]; cf[RM + r] = s.firm[r]; cd[RM + r] = M.units + r + 1;
+    cy[RM + r] = s.outcome[r]; cq[RM + r] = s.frequency[r]; ct[RM + r] = s.target[r];
+  }
+  std::vector<uint8_t> all(R, 1);
+  Columns cols{R, cw.data(), cf.data(), cd.data(), cy.data(), cq.data(), ct.data()};
+  Prepared P = prepare(cols, all.data(), false);
+  if (P.rows != R || P.workers != M.workers + SW || P.firms != M.firms || P.units != M.units + S)
+    fail(kInvariant, "stayer_augmentation", "combined exact problem dimensions do not reconcile");
+  for (uint64_t r = 0; r < RM; ++r)
+    if (P.row_worker[r] != M.row_worker[r] || P.row_firm[r] != M.row_firm[r] ||
+        P.row_deletion[r] != M.row_deletion[r] || P.frequency[r] != M.frequency[r])
+      fail(kInvariant, "stayer_augmentation", "combined preparation changed a retained mover row");
+  // keep the mover generation's identity (input rows, mask, selection)
+  P.input_rows = M.input_rows; P.input_physical = M.input_physical; P.input_target_total = M.input_target_total;
+  P.input_topology_checksum = M.input_topology_checksum; P.mask = M.mask; P.selection = M.selection;
+  // the hybrid plan (generic_jla.rs:2170-2213): mover match groups keep their
+  // semantic entities; each stayer row is an observation unit whose entity is
+  // its observation class (ranked over all rows, generic_jla.rs:4217-4265) and
+  // whose copies continue those of the class's earlier stayer rows
+  P.stayers = true;
+  P.match_units = M.units;
+  P.unit_offset.assign(P.units, 0);
+  if (S) {
+    std::vector<uint32_t> order;
+    const std::vector<uint64_t> orank = observation_ranks(P, order);
+    uint64_t offset = 0, last = 0;
+    for (uint64_t i = 0; i < R; ++i) {
+      const uint32_t row = order[i];
+      if (row < RM) continue;
+      if (orank[row] != last) offset = 0;
+      last = orank[row];
+      const uint64_t u = M.units + (row - RM);
+      P.unit_rank[u] = orank[row];
+      P.unit_offset[u] = offset;
+      offset += P.frequency[row];
+    }
+  }
+  rec = StayerReceipt{};
+  rec.mover_stored_rows = RM; rec.stayer_stored_rows = S; rec.combined_stored_rows = R;
+  rec.mover_physical_mass = M.physical_total; rec.stayer_physical_mass = stayer_physical;
+  rec.combined_physical_mass = P.physical_total;
+  rec.mover_workers = M.workers; rec.stayer_workers = SW; rec.combined_workers = P.workers; rec.firms = P.firms;
+  rec.mover_deletion_units = M.units; rec.stayer_deletion_units = stayer_physical;
+  rec.combined_deletion_units = M.units + stayer_physical;
+  rec.mover_target_mass = M.target_total; rec.stayer_target_mass = stayer_target.value();
+  rec.combined_target_mass = P.target_total; rec.topology_checksum = P.topology_checksum;
+  if (!std::isfinite(rec.stayer_target_mass) || rec.stayer_target_mass < 0.0)
+    fail(kInvalidTargetWeight, "stayer_augmentation", "stayer target mass is not finite and nonnegative");
   return P;
 }
 

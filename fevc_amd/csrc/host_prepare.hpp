@@ -34,6 +34,10 @@ struct Prepared {
   std::vector<uint8_t> mask;
   SelectionReceipt selection;
   bool observation = false;  // observation deletion (else match deletion)
+  bool stayers = false;      // a mover problem augmented with eligible stayers
+  // units [0, match_units) are match deletion units; the rest are observation
+  // units, one per stored row, in row order, ending with the last row
+  uint64_t match_units = 0;
 
   uint64_t rows = 0, workers = 0, firms = 0, cells = 0, units = 0, exact_strata = 0, strata = 0;
   std::vector<uint32_t> row_worker, row_firm, row_deletion, row_cell;
@@ -70,5 +74,26 @@ struct Prepared {
 // instead of the match-deletion one; the exact estimator then deletes every
 // physical observation on its own.
 Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation = false);
+
+// Eligible stayer rows: 1-based dense retained-mover firm and stayer worker
+// indices (stayer_hybrid.rs:23-34).
+struct StayerColumns {
+  uint64_t rows = 0;
+  const uint64_t *firm = nullptr, *worker = nullptr, *frequency = nullptr;
+  const double *outcome = nullptr, *target = nullptr;
+};
+struct StayerReceipt {
+  uint64_t mover_stored_rows = 0, stayer_stored_rows = 0, combined_stored_rows = 0;
+  uint64_t mover_physical_mass = 0, stayer_physical_mass = 0, combined_physical_mass = 0;
+  uint64_t mover_workers = 0, stayer_workers = 0, combined_workers = 0, firms = 0;
+  uint64_t mover_deletion_units = 0, stayer_deletion_units = 0, combined_deletion_units = 0;
+  double mover_target_mass = 0.0, stayer_target_mass = 0.0, combined_target_mass = 0.0;
+  uint64_t topology_checksum = 0;
+};
+// stayer_hybrid.rs:71-348 (prepare_exact_stayer_hybrid): validate the stayer
+// rows against the prepared mover problem M, recompress movers and stayers
+// together (every row retained), and address the stayer rows as observation
+// units.  Throws vk::Error with the reference status code.
+Prepared augment_stayers(const Prepared &M, const StayerColumns &s, StayerReceipt &rec);
 
 }  // namespace vk

@@ -40,6 +40,8 @@ struct Context {
   std::unique_ptr<DeviceEngine> dev;
   bool has_result = false;
   SolveResult result;
+  StayerReceipt stayer_receipt;
+  uint64_t stayer_copy_bytes = 0;
   uint32_t precision = 64, leverage_width = 32, target_width = 16;
 };
 
@@ -156,6 +158,12 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
   const uint64_t parameters = prepared ? ctx.P.workers + ctx.P.firms - 1 : 0;
   const bool exact = v2.algorithm == VCKSS_ALGORITHM_EXACT ||
                      (prepared && v2.algorithm == VCKSS_ALGORITHM_AUTO && parameters <= v2.exact_estimator_limit);
+  // ffi_engine.rs:4840-4861: a stayer-augmented generation takes stayers-all
+  // solves only, and stayers-all needs the augmentation
+  if (prepared && v3.stayers_mode == VCKSS_STAYERS_MOVERS && ctx.P.stayers)
+    fail(kInvalidInput, "engine_solve", "a mover-only solve cannot consume a stayer-augmented generation");
+  if (prepared && v3.stayers_mode == VCKSS_STAYERS_ALL && !ctx.P.stayers)
+    fail(kInvalidInput, "engine_solve", "a stayers-all solve requires a reconciled stayer augmentation");
   // ffi_engine.rs:5705-5710: the solve's deletion mode is the prepared graph's
   if (prepared && (v1.deletion_mode == VCKSS_DELETION_OBSERVATION) != ctx.P.observation)
     fail(kInvalidInput, "engine_solve", "solve deletion mode differs from the prepared graph mode");
@@ -172,7 +180,8 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
       (v1.solver_route == VCKSS_ROUTE_AUTO || v1.solver_route == VCKSS_ROUTE_CMG_PCG) &&
       v3.batch_mode == VCKSS_BATCH_MODE_AUTO && v4.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO &&
       v4.target_batch_mode == VCKSS_BATCH_MODE_AUTO && v1.leverage_batch_width == 0 &&
-      v1.target_batch_width == 0 && v3.stayers_mode == VCKSS_STAYERS_MOVERS &&
+      v1.target_batch_width == 0 &&
+      (v3.stayers_mode == VCKSS_STAYERS_MOVERS || v3.stayers_mode == VCKSS_STAYERS_ALL) &&
       (v3.target_weight_mode == VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT ||
        v3.target_weight_mode == VCKSS_TARGET_WEIGHT_STORED_ROW_EXPLICIT) &&
       (v3.deletion_unit_source == VCKSS_DELETION_SOURCE_CELL_DEFAULT ||
@@ -449,6 +458,139 @@ static int32_t prepare_common(const VckssEnginePrepareRequestInterruptV3 *reques
   });
 }
 
+// ---- stayer hybrid: ffi_engine.rs:2867-2879, 3667-3676, 4171-4262, 9672-9760;
+// session_retained.rs:850-943; context.rs:385-440 (a Prepared generation only;
+// a failed augmentation leaves it Failed)
+int32_t vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1(
+    VckssStayerAugmentationRequestInterruptV1 *output, uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssStayerAugmentationRequestInterruptV1>(output, output_capacity_bytes,
+                                                                "default interrupt stayer augmentation request");
+    VckssStayerAugmentationRequestInterruptV1 d;
+    std::memset(&d, 0, sizeof d);
+    d.options.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+    d.options.struct_size = sizeof(VckssStayerAugmentationRequestInterruptV1);
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+static void augment_stayers_value(uint64_t generation, const VckssStayerAugmentationRequestV1 &r,
+                                  const VckssStayerAugmentationColumnsV1 *columns, VckssInterruptPollV1 poll,
+                                  void *poll_ctx) {
+  if (r.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_abi", "unsupported ABI version");
+  if (r.struct_size < sizeof(VckssStayerAugmentationRequestV1) || r.reserved != 0)
+    fail(kAbiMismatch, "engine_abi", "stayer augmentation request is short or reserves nonzero fields");
+  if (!columns) fail(kInvalidInput, "engine_stayer_augmentation", "stayer augmentation column descriptor is null");
+  VckssStayerAugmentationColumnsV1 c;
+  std::memcpy(&c, columns, sizeof c);
+  if (c.struct_size < sizeof(VckssStayerAugmentationColumnsV1))
+    fail(kAbiMismatch, "engine_abi", "stayer augmentation column descriptor reports a short structure size");
+  if (c.reserved != 0 || c.reserved_2 != 0)
+    fail(kAbiMismatch, "engine_abi", "reserved stayer augmentation column fields must be zero");
+  if (r.rows != c.rows || r.controls_count != c.controls_count)
+    fail(kInvalidInput, "engine_stayer_augmentation", "stayer request and column dimensions disagree");
+  if (c.rows == 0) {
+    if (c.firm || c.worker || c.outcome || c.frequency || c.target_weight || c.controls)
+      fail(kInvalidInput, "engine_stayer_augmentation", "zero stayer rows require null numeric and control pointers");
+  } else if (c.controls_count == 0 && c.controls) {
+    fail(kInvalidInput, "engine_stayer_augmentation", "zero controls require a null control-pointer array");
+  }
+  if (c.controls_count != 0)
+    fail(kUnsupported, "engine_stayer_augmentation", "the MI355X engine supports the no-control route only");
+  const uint64_t rows = c.rows;
+  std::vector<uint64_t> firm, worker, frequency;
+  std::vector<double> outcome, target;
+  if (rows) {
+    firm = integer_column(c.firm, rows, "stayer firm identifier", kInvalidIdentifier, false);
+    worker = integer_column(c.worker, rows, "stayer worker identifier", kInvalidIdentifier, false);
+    frequency = integer_column(c.frequency, rows, "stayer frequency weight", kInvalidWeight, false);
+    if (!c.outcome || !c.target_weight)
+      fail(kInvalidInput, "engine_stayer_augmentation", "stayer outcome or target column is null");
+    outcome.assign(c.outcome, c.outcome + rows);
+    target.assign(c.target_weight, c.target_weight + rows);
+  }
+  Context &ctx = live(generation);
+  if (ctx.state != kPrepared)
+    fail(kContextPoisoned, "context_augment", "generation " + std::to_string(generation) +
+                                                  " is not in the Prepared state");
+  if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK)
+    fail(kUserBreak, "engine_stayer_augmentation", "user break");
+  StayerColumns sc{rows, firm.data(), worker.data(), frequency.data(), outcome.data(), target.data()};
+  try {
+    StayerReceipt rec;
+    Prepared P = augment_stayers(ctx.P, sc, rec);
+    HybridSystem HS = build_hybrid(P);
+    CmgOptions copt;  // full_cmg.rs:467-476: max_levels 128
+    Hierarchy H = build_hierarchy(HS.graph, copt);
+    const bool upload = static_cast<bool>(ctx.dev);
+    ctx.dev.reset();
+    ctx.P = std::move(P);
+    ctx.HS = std::move(HS);
+    ctx.H = std::move(H);
+    if (upload) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
+    ctx.stayer_receipt = rec;
+    ctx.stayer_copy_bytes = r.caller_copy_bytes;
+  } catch (...) {
+    ctx.state = kFailed;
+    throw;
+  }
+}
+
+int32_t vckss_rust_engine_augment_stayers_v1(uint64_t generation, const VckssStayerAugmentationRequestV1 *request,
+                                             const VckssStayerAugmentationColumnsV1 *columns) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_stayer_augmentation", "request pointer is null");
+    VckssStayerAugmentationRequestV1 r;
+    std::memcpy(&r, request, sizeof r);
+    augment_stayers_value(generation, r, columns, nullptr, nullptr);
+  });
+}
+
+int32_t vckss_rust_engine_augment_stayers_interrupt_v1(uint64_t generation,
+                                                       const VckssStayerAugmentationRequestInterruptV1 *request,
+                                                       const VckssStayerAugmentationColumnsV1 *columns) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_stayer_augmentation", "request pointer is null");
+    VckssStayerAugmentationRequestInterruptV1 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.reserved != 0) fail(kAbiMismatch, "engine_abi", "reserved interrupt fields must be zero");
+    augment_stayers_value(generation, r.options, columns, r.interrupt_poll, r.interrupt_context);
+  });
+}
+
+// ffi_engine.rs:8737-8770
+int32_t vckss_rust_engine_stayer_augmentation_receipt_v1(uint64_t generation, VckssStayerAugmentationReceiptV1 *output,
+                                                         uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssStayerAugmentationReceiptV1>(output, output_capacity_bytes, "stayer augmentation receipt");
+    Context &ctx = live(generation);
+    if (!ctx.P.stayers)
+      fail(kInvalidInput, "engine_stayer_augmentation", "the generation owns no stayer augmentation");
+    const StayerReceipt &s = ctx.stayer_receipt;
+    VckssStayerAugmentationReceiptV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    o.mover_stored_rows = s.mover_stored_rows; o.stayer_stored_rows = s.stayer_stored_rows;
+    o.combined_stored_rows = s.combined_stored_rows;
+    o.mover_physical_mass = s.mover_physical_mass; o.stayer_physical_mass = s.stayer_physical_mass;
+    o.combined_physical_mass = s.combined_physical_mass;
+    o.mover_workers = s.mover_workers; o.stayer_workers = s.stayer_workers; o.combined_workers = s.combined_workers;
+    o.firms = s.firms;
+    o.mover_deletion_units = s.mover_deletion_units; o.stayer_deletion_units = s.stayer_deletion_units;
+    o.combined_deletion_units = s.combined_deletion_units;
+    o.mover_target_mass = s.mover_target_mass; o.stayer_target_mass = s.stayer_target_mass;
+    o.combined_target_mass = s.combined_target_mass;
+    o.topology_checksum = s.topology_checksum;
+    o.caller_copy_bytes = ctx.stayer_copy_bytes;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
 int32_t vckss_rust_engine_prepare_interrupt_v4(const VckssEnginePrepareRequestInterruptV3 *request,
                                                const VckssEngineColumnsV3 *columns, uint64_t *output_handle,
                                                uint32_t output_capacity_bytes) {
@@ -552,8 +694,9 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
-  if (ctx.P.observation)
-    fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE under match deletion only");
+  if (ctx.P.observation || ctx.P.stayers)
+    fail(kUnsupported, "numerical_solve",
+         "the MI355X engine attaches numerical MCSE under match deletion without stayers only");
   o.numerical = true;
   run_options(ctx, o, poll, poll_ctx);
 }

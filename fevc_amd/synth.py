@@ -118,3 +118,36 @@ def generate(spec: PanelSpec):
         worker=w64, firm=f64, deletion=w64 * np.uint64(F + 1) + f64,
         outcome=y.astype(np.float64), frequency=np.ones(W * T, np.uint64),
         target=np.ones(W * T, np.float64), alpha=alpha, psi=psi)
+
+
+def eligible_stayers(worker, firm, outcome, frequency, target, mover_mask):
+    """Eligible stayer rows of a panel for the stayer hybrid, in the form the
+    augmentation entry point takes (stayer_hybrid.rs:23-34): workers outside
+    the retained mover sample whose rows all lie in one retained mover firm
+    and who represent at least two physical observations.  Returns
+    (firm, worker, outcome, frequency, target) with 1-based dense
+    retained-mover firm indices (retained firm identifiers in ascending order,
+    the prepared problem's firm order) and 1-based dense stayer-worker
+    indices (ascending identifiers)."""
+    worker, firm = np.asarray(worker), np.asarray(firm)
+    mask = np.asarray(mover_mask, bool)
+    retained_firms = np.unique(firm[mask])
+    retained_workers = set(np.unique(worker[mask]).tolist())
+    frequency = np.asarray(frequency, float)
+    rows = []
+    order = np.argsort(worker, kind="stable")
+    bounds = np.flatnonzero(np.diff(worker[order])) + 1
+    for idx in np.split(order, bounds):
+        w = worker[idx[0]]
+        if w in retained_workers or np.unique(firm[idx]).size != 1:
+            continue
+        if frequency[idx].sum() < 2 or not np.isin(firm[idx[0]], retained_firms):
+            continue
+        rows.append(np.sort(idx))
+    if not rows:
+        return tuple(np.zeros(0) for _ in range(5))
+    rows_all = np.concatenate(rows)
+    firm_index = np.searchsorted(retained_firms, firm[rows_all]) + 1.0
+    worker_index = np.concatenate([np.full(r.size, k + 1.0) for k, r in enumerate(rows)])
+    return (firm_index, worker_index, np.asarray(outcome, float)[rows_all], frequency[rows_all],
+            np.asarray(target, float)[rows_all])

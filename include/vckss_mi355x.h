@@ -212,6 +212,68 @@ typedef struct VckssEngineSolveRequestInterruptV5 {
     uint32_t reserved;
 } VckssEngineSolveRequestInterruptV5;
 
+/* vckss_rust.h:369-398: eligible stayer rows, 1-based dense retained-mover
+ * firm and stayer-worker indices as exact binary64 integers */
+typedef struct VckssStayerAugmentationRequestV1 {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    uint64_t rows;
+    uint32_t controls_count;
+    uint32_t reserved;
+    uint64_t caller_copy_bytes;
+} VckssStayerAugmentationRequestV1;
+
+typedef struct VckssStayerAugmentationRequestInterruptV1 {
+    VckssStayerAugmentationRequestV1 options;
+    VckssInterruptPollV1 interrupt_poll;
+    void *interrupt_context;
+    uint32_t checkpoint_interval;
+    uint32_t reserved;
+} VckssStayerAugmentationRequestInterruptV1;
+
+typedef struct VckssStayerAugmentationColumnsV1 {
+    uint32_t struct_size;
+    uint32_t reserved;
+    uint64_t rows;
+    const double *firm;
+    const double *worker;
+    const double *outcome;
+    const double *frequency;
+    const double *target_weight;
+    const double *const *controls;
+    uint32_t controls_count;
+    uint32_t reserved_2;
+} VckssStayerAugmentationColumnsV1;
+
+/* vckss_rust.h:770-796 */
+typedef struct VckssStayerAugmentationReceiptV1 {
+    uint32_t struct_size;
+    uint32_t schema_version;
+    uint64_t generation;
+    uint64_t mover_stored_rows;
+    uint64_t stayer_stored_rows;
+    uint64_t combined_stored_rows;
+    uint64_t mover_physical_mass;
+    uint64_t stayer_physical_mass;
+    uint64_t combined_physical_mass;
+    uint64_t mover_workers;
+    uint64_t stayer_workers;
+    uint64_t combined_workers;
+    uint64_t firms;
+    uint64_t mover_deletion_units;
+    uint64_t stayer_deletion_units;
+    uint64_t combined_deletion_units;
+    double mover_target_mass;
+    double stayer_target_mass;
+    double combined_target_mass;
+    uint64_t topology_checksum;
+    uint64_t memory_limit_bytes;
+    uint64_t caller_copy_bytes;
+    uint64_t augmentation_peak_forecast_bytes;
+    uint64_t augmented_resident_bytes;
+    uint64_t total_prepared_resident_bytes;
+} VckssStayerAugmentationReceiptV1;
+
 /* vckss_rust.h:1037-1052 */
 typedef struct VckssComponentVectorV1 {
     double worker;
@@ -313,6 +375,20 @@ int32_t vckss_rust_engine_default_solve_request_interrupt_v5(
 int32_t vckss_rust_engine_prepare_interrupt_v4(
     const VckssEnginePrepareRequestInterruptV3 *request, const VckssEngineColumnsV3 *columns,
     uint64_t *output_handle, uint32_t output_capacity_bytes);
+/* vckss_rust.h:1717-1720, 1805-1819: the stayer hybrid (stayers(both)):
+ * eligible stayer rows appended to a prepared match-deletion generation;
+ * solves then take stayers_mode ALL (ffi_engine.rs:2867-2879, 3667-3676,
+ * 4171-4262, 8737-8770) */
+int32_t vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1(
+    VckssStayerAugmentationRequestInterruptV1 *output, uint32_t output_capacity_bytes);
+int32_t vckss_rust_engine_augment_stayers_v1(uint64_t generation, const VckssStayerAugmentationRequestV1 *request,
+                                             const VckssStayerAugmentationColumnsV1 *columns);
+int32_t vckss_rust_engine_augment_stayers_interrupt_v1(uint64_t generation,
+                                                       const VckssStayerAugmentationRequestInterruptV1 *request,
+                                                       const VckssStayerAugmentationColumnsV1 *columns);
+int32_t vckss_rust_engine_stayer_augmentation_receipt_v1(uint64_t generation, VckssStayerAugmentationReceiptV1 *output,
+                                                         uint32_t output_capacity_bytes);
+
 /* vckss_rust.h:1928-1932 */
 int32_t vckss_rust_engine_retained_mask_v1(uint64_t generation, uint8_t *output,
                                            uint64_t output_capacity_bytes);

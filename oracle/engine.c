@@ -778,10 +778,15 @@ static int copy_sign(uint64_t seed, uint64_t probe, uint64_t entity, uint64_t co
     return ((word >> (copy % 64)) & 1) ? 1 : -1;
 }
 
+/* Units [match_units, units) are observation units: the last stored rows. */
+static uint64_t observation_row(const orc_problem *p, uint64_t g) {
+    return g - p->match_units + (p->rows - (p->units - p->match_units));
+}
+
 static void observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k,
                               int64_t *atoms) {
     for (uint64_t c = 0; c < k; ++c)
-        for (uint64_t g = 0; g < p->units; ++g) {
+        for (uint64_t g = p->match_units; g < p->units; ++g) {
             int64_t a = 0;
             for (uint64_t j = 0; j < p->unit_count[g]; ++j)
                 a += copy_sign(seed, first + c, p->unit_rank[g], p->unit_offset[g] + j);
@@ -789,30 +794,43 @@ static void observation_atoms(const orc_problem *p, uint64_t seed, uint64_t firs
         }
 }
 
-int orc_observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t columns, int64_t *out) {
-    if (!p->observation)
-        return orc_fail(ORC_INVALID_INPUT, "observation_atoms", "the problem was prepared for match deletion");
-    observation_atoms(p, seed, first, columns, out);
+/* Leverage atoms of every unit, out[col * units + unit]: match units by the
+ * compressed Counter-V1 sums (rng.rs:202-305), observation units by their
+ * copies' signs. */
+static int leverage_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k, int64_t *atoms) {
+    uint64_t G = p->units, Gm = p->match_units;
+    if (Gm) {
+        int64_t *tmp = orc_alloc(Gm * k, 8);
+        int rc = orc_rademacher_fill(seed, ORC_TAG_LEVERAGE, first, k, p->unit_rank, p->unit_count, Gm, tmp);
+        for (uint64_t c = 0; !rc && c < k; ++c) memcpy(atoms + c * G, tmp + c * Gm, Gm * 8);
+        free(tmp);
+        if (rc) return rc;
+    }
+    if (G > Gm) observation_atoms(p, seed, first, k, atoms);
     return ORC_OK;
+}
+
+int orc_observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t columns, int64_t *out) {
+    return leverage_atoms(p, seed, first, columns, out);
 }
 
 /* ObservationMoments and ObservationCorrelations (generic_jla.rs:5540-5892):
  * per stored row the probe sums of p^2 and p^4, per physical copy those of
  * s p and s p^3, p the row's prediction and s the copy's sign. */
 typedef struct {
-    uint64_t *phys;          /* copies of row g: [phys[g], phys[g+1]) */
-    ssum *square, *fourth;   /* per row */
+    uint64_t *phys;          /* copies of observation unit i: [phys[i], phys[i+1]) */
+    ssum *square, *fourth;   /* per observation unit */
     ssum *first, *third;     /* per physical copy */
 } obs_state;
 
 static void observation_state_init(const orc_problem *p, obs_state *o) {
-    uint64_t G = p->units;
-    o->phys = orc_alloc(G + 1, 8);
-    for (uint64_t g = 0; g < G; ++g) o->phys[g + 1] = o->phys[g] + p->unit_count[g];
-    o->square = orc_alloc(G, sizeof(ssum));
-    o->fourth = orc_alloc(G, sizeof(ssum));
-    o->first = orc_alloc(o->phys[G], sizeof(ssum));
-    o->third = orc_alloc(o->phys[G], sizeof(ssum));
+    uint64_t Go = p->units - p->match_units;
+    o->phys = orc_alloc(Go + 1, 8);
+    for (uint64_t i = 0; i < Go; ++i) o->phys[i + 1] = o->phys[i] + p->unit_count[p->match_units + i];
+    o->square = orc_alloc(Go, sizeof(ssum));
+    o->fourth = orc_alloc(Go, sizeof(ssum));
+    o->first = orc_alloc(o->phys[Go], sizeof(ssum));
+    o->third = orc_alloc(o->phys[Go], sizeof(ssum));
 }
 
 static void observation_state_free(obs_state *o) {
@@ -823,16 +841,17 @@ static void observation_state_free(obs_state *o) {
 /* statistical_batches.rs:79-206, probes in order */
 static void observation_moments(const orc_problem *p, obs_state *o, uint64_t seed, uint64_t first,
                                 uint64_t k, const column_t *cols) {
-    for (uint64_t g = 0; g < p->units; ++g) {
+    for (uint64_t i = 0; i < p->units - p->match_units; ++i) {
+        uint64_t g = p->match_units + i;
         uint32_t cell = p->unit_cell[g];
         for (uint64_t c = 0; c < k; ++c) {
             double pred = cols[c].worker[p->cell_worker[cell]] + cols[c].firm[p->cell_firm[cell]];
-            ss_add(&o->square[g], pred * pred);
-            ss_add(&o->fourth[g], pred * pred * pred * pred);
+            ss_add(&o->square[i], pred * pred);
+            ss_add(&o->fourth[i], pred * pred * pred * pred);
             for (uint64_t j = 0; j < p->unit_count[g]; ++j) {
                 double s = (double)copy_sign(seed, first + c, p->unit_rank[g], p->unit_offset[g] + j);
-                ss_add(&o->first[o->phys[g] + j], s * pred);
-                ss_add(&o->third[o->phys[g] + j], s * pred * pred * pred);
+                ss_add(&o->first[o->phys[i] + j], s * pred);
+                ss_add(&o->third[o->phys[i] + j], s * pred * pred * pred);
             }
         }
     }
@@ -847,18 +866,14 @@ static void observation_moments(const orc_problem *p, obs_state *o, uint64_t see
  * share, residual share, bias and variance are means over the copies;
  * residual mass = e; deleted mass = d. */
 static int observation_adjustment(const orc_problem *p, const obs_state *o, const orc_engine_options *opt,
-                                  orc_engine_result *res) {
-    uint64_t G = p->units, C = p->cells;
+                                  orc_engine_result *res, ssum *cw) {
     double probes = (double)res->probes;
-    res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
-    res->finite_bias = orc_alloc(G, 8); res->finite_variance = orc_alloc(G, 8);
-    res->residual_mass = orc_alloc(G, 8); res->deleted_mass = orc_alloc(G, 8);
-    ssum *cw = orc_alloc(C, sizeof *cw);
     int rc = ORC_OK;
-    for (uint64_t g = 0; g < G && !rc; ++g) {
-        double pf = ss_fin(o->square[g]), ps = ss_fin(o->fourth[g]);
+    for (uint64_t i = 0; i < p->units - p->match_units && !rc; ++i) {
+        uint64_t g = p->match_units + i, row = observation_row(p, g);
+        double pf = ss_fin(o->square[i]), ps = ss_fin(o->fourth[i]);
         ssum inverse_sum = {0, 0}, lev = {0, 0}, rs = {0, 0}, bs = {0, 0}, vs = {0, 0};
-        for (uint64_t q = o->phys[g]; q < o->phys[g + 1]; ++q) {
+        for (uint64_t q = o->phys[i]; q < o->phys[i + 1]; ++q) {
             double first = ss_fin(o->first[q]), third = ss_fin(o->third[q]);
             double tot[5] = {pf, probes + pf - 2.0 * first, ps,
                              probes + 6.0 * pf + ps - 4.0 * first - 4.0 * third, pf + ps - 2.0 * third};
@@ -899,7 +914,7 @@ static int observation_adjustment(const orc_problem *p, const obs_state *o, cons
         if (rc) break;
         double f = (double)p->unit_count[g];
         uint32_t cell = p->unit_cell[g];
-        double e = p->outcome[g] - res->fitted_cell[cell];
+        double e = p->outcome[row] - res->fitted_cell[cell];
         double deleted = e * (ss_fin(inverse_sum) / f);
         if (!isfinite(deleted)) {
             rc = orc_fail(ORC_CORRECTION_NONFINITE, "generic_jla_observation_adjustment", "observation deleted residual is nonfinite");
@@ -911,13 +926,8 @@ static int observation_adjustment(const orc_problem *p, const obs_state *o, cons
         res->finite_variance[g] = ss_fin(vs) / f;
         res->residual_mass[g] = e;
         res->deleted_mass[g] = deleted;
-        ss_add(&cw[cell], f * p->outcome[g] * deleted);
+        ss_add(&cw[cell], f * p->outcome[row] * deleted);
     }
-    if (!rc) {
-        res->cell_weight = orc_alloc(C, 8);
-        for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);
-    }
-    free(cw);
     return rc;
 }
 
@@ -974,21 +984,18 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
 
     /* leverage loop: engine.rs:1150-1246; observation deletion:
      * generic_jla.rs:5540-5892 */
-    if (p->observation && num) {
-        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment under observation deletion");
+    const uint64_t Gm = p->match_units, Go = G - Gm;
+    if (Go && num) {
+        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment with observation units");
         goto fail;
     }
-    if (p->observation) observation_state_init(p, &obs);
+    if (Go) observation_state_init(p, &obs);
     mom = orc_alloc(G, sizeof *mom);
     res->lev_iters = orc_alloc(R, 8);
     uint64_t width = opt.leverage_width ? opt.leverage_width : 1;
     for (uint64_t first = 0; first < R; first += width) {
         uint64_t k = R - first < width ? R - first : width;
-        if (p->observation)
-            observation_atoms(p, opt.seed, first, k, atoms);
-        else if ((rc = orc_rademacher_fill(opt.seed, ORC_TAG_LEVERAGE, first, k, p->unit_rank,
-                                           p->unit_count, G, atoms)))
-            goto fail_mom;
+        if ((rc = leverage_atoms(p, opt.seed, first, k, atoms))) goto fail_mom;
         for (uint64_t c = 0; c < k; ++c) leverage_rhs(p, atoms + c * G, wr + c * W, fr + c * F);
         if ((rc = solve_batch(&s, wr, fr, k, opt.probe_tolerance, 1.0, &opt, cols))) goto fail_mom;
         for (uint64_t c = 0; c < k; ++c) {
@@ -996,11 +1003,8 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             res->sum.leverage_iterations += cols[c].iterations;
             max_complete = fmax(max_complete, cols[c].complete);
         }
-        if (p->observation) {
-            observation_moments(p, &obs, opt.seed, first, k, cols);
-            continue;
-        }
-        for (uint64_t g = 0; g < G; ++g) {
+        if (Go) observation_moments(p, &obs, opt.seed, first, k, cols);
+        for (uint64_t g = 0; g < Gm; ++g) {
             uint32_t cell = p->unit_cell[g];
             double freq = (double)p->unit_count[g];
             for (uint64_t c = 0; c < k; ++c) {
@@ -1017,16 +1021,13 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     }
     /* leverage adjustment: engine.rs:2269-2410 (outcome mean 0, centering None) */
     int clipped = 0;
-    if (p->observation) {
-        rc = observation_adjustment(p, &obs, &opt, res);
-        if (rc) goto fail_mom;
-    } else {
+    {
         double probes = (double)R;
         res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
         res->finite_bias = orc_alloc(G, 8); res->finite_variance = orc_alloc(G, 8);
         res->residual_mass = orc_alloc(G, 8); res->deleted_mass = orc_alloc(G, 8);
         ssum *cw = orc_alloc(C, sizeof *cw);
-        for (uint64_t g = 0; g < G; ++g) {
+        for (uint64_t g = 0; g < Gm; ++g) {
             double pf = ss_fin(mom[g].proj), mf = ss_fin(mom[g].res);
             double total_mean = (pf + mf) / probes;
             if (!isfinite(total_mean) || total_mean <= opt.block_tolerance) {
@@ -1071,6 +1072,11 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             res->finite_variance[g] = variance;
             res->residual_mass[g] = resid_outcome;
             res->deleted_mass[g] = deleted;
+        }
+        /* observation units: generic_jla.rs:5893-6017 */
+        if (Go && (rc = observation_adjustment(p, &obs, &opt, res, cw))) {
+            free(cw);
+            goto fail_mom;
         }
         res->cell_weight = orc_alloc(C, 8);
         for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);

@@ -101,11 +101,13 @@ def _zero_sum_inverse(info, workers, firms, tolerance):
 
 
 def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_tolerance=1e-10,
-                    block_tolerance=1e-10, solver_tolerance=1e-10, mode="match"):
+                    block_tolerance=1e-10, solver_tolerance=1e-10, mode="match", stayer_rows=None):
     """run_exact_estimator (exact_estimator.rs:490-1244) for match deletion
     (or, with mode="observation", physical-observation deletion,
-    exact_estimator.rs:914-987), joint nuisance, no controls and no
-    centering, on retained stored rows."""
+    exact_estimator.rs:914-987; with mode="hybrid", the stayer hybrid of
+    stayer_hybrid.rs:71-348: match blocks for the rows outside the boolean
+    `stayer_rows`, physical-observation deletion for the stayer rows), joint
+    nuisance, no controls and no centering, on retained stored rows."""
     w = _codes(worker)
     f = _codes(firm)
     g = _codes(deletion)
@@ -115,9 +117,11 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     rows = y.size
     W, F = int(w.max()) + 1, int(f.max()) + 1
     p = W + F
-    if mode == "match":
-        for code in range(int(g.max()) + 1):
-            use = g == code
+    observed = np.ones(rows, bool) if mode == "observation" else (
+        np.asarray(stayer_rows, bool) if mode == "hybrid" else np.zeros(rows, bool))
+    if mode != "observation":
+        for code in np.unique(g[~observed]):
+            use = (g == code) & ~observed
             if np.unique(w[use]).size != 1 or np.unique(f[use]).size != 1:
                 raise ExactError(20, "a deletion unit spans two cells")
     # build_design: worker dummy, firm coordinates e_f - 1/F
@@ -161,9 +165,9 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     max_leverage = 0.0
     rank_gap = np.inf
     h_unit = np.zeros(int(g.max()) + 1)
-    if mode == "observation":
+    if observed.any():
         design_inverse = x @ inverse
-        for row in range(rows):
+        for row in np.flatnonzero(observed):
             z = design_inverse[row]
             lev = float(x[row] @ z)
             if not np.isfinite(lev) or lev < -100.0 * rank_tolerance:
@@ -175,13 +179,9 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
             rank_gap = min(rank_gap, maker)
             scale = fr[row] * y[row] * residual[row] / maker
             correction += scale * _bilinear(z, z, W, worker_share, firm_share, cell_w, cell_f, cell_mass)
-        correction[3] = correction[0] + correction[1] + 2.0 * correction[2]
-        return dict(plugin=plugin, correction=correction, corrected=plugin - correction, beta=beta,
-                    residual=residual, max_leverage=max_leverage, deletion_rank_gap=rank_gap,
-                    deletion_units=int(fr.sum()), weighted_rss=float(np.sum(fr * residual ** 2)),
-                    information_rcond=rcond, inverse_relres=relres, fit_relres=fit_rel, workers=W, firms=F)
-    for code in range(int(g.max()) + 1):
-        idx = np.flatnonzero(g == code)
+    match_codes = np.unique(g[~observed])
+    for code in match_codes:
+        idx = np.flatnonzero((g == code) & ~observed)
         root = np.sqrt(fr[idx])
         bd = root[:, None] * x[idx]
         ty = root * y[idx]
@@ -205,7 +205,8 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     correction[3] = correction[0] + correction[1] + 2.0 * correction[2]
     return dict(plugin=plugin, correction=correction, corrected=plugin - correction, beta=beta,
                 residual=residual, max_leverage=max_leverage, deletion_rank_gap=rank_gap,
-                deletion_units=int(g.max()) + 1, weighted_rss=float(np.sum(fr * residual ** 2)),
+                deletion_units=int(match_codes.size + fr[observed].sum()),
+                weighted_rss=float(np.sum(fr * residual ** 2)),
                 information_rcond=rcond, inverse_relres=relres, fit_relres=fit_rel,
                 unit_leverage=h_unit, workers=W, firms=F)
 

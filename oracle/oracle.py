@@ -139,6 +139,7 @@ def lib():
         L.orc_last_error.restype = C.c_char_p
         L.orc_rng_word.restype = C.c_uint64
         L.orc_observation_atoms.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]
+        L.orc_prepare_stayers.argtypes = [C.c_void_p, C.c_uint64] + [C.c_void_p] * 6
         L.orc_graph_from_edges.restype = C.c_void_p
         L.orc_graph_from_edges.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_graph_vertices.restype = C.c_uint64
@@ -349,13 +350,29 @@ class Problem:
             m = np.ascontiguousarray(mask, dtype=np.uint8)
             _check(lib().orc_prepare_masked(len(cols[0]), *[c.ctypes.data for c in cols],
                                             m.ctypes.data, C.byref(h)))
-        self._h = h.value
+        self._adopt(h.value)
+
+    def _adopt(self, handle):
+        self._h = handle
         d = np.zeros(9, np.uint64)
         lib().orc_problem_dims(self._h, d.ctypes.data)
         (self.input_rows, self.rows, self.physical, self.workers, self.firms, self.cells,
          self.units, self.exact_strata, self.strata) = [int(x) for x in d]
         self.target_total = float(lib().orc_problem_target_total(self._h))
         self.topology_checksum = int(lib().orc_problem_topology_checksum(self._h))
+
+    def with_stayers(self, firm, worker, outcome, frequency, target):
+        """The stayer hybrid of this (match-deletion) problem: eligible stayer
+        rows with 1-based dense retained-mover firm and stayer-worker indices."""
+        cols = [np.ascontiguousarray(firm, np.uint64), np.ascontiguousarray(worker, np.uint64),
+                np.ascontiguousarray(outcome, np.float64), np.ascontiguousarray(frequency, np.uint64),
+                np.ascontiguousarray(target, np.float64)]
+        h = C.c_void_p()
+        _check(lib().orc_prepare_stayers(self._h, len(cols[0]), *[c.ctypes.data for c in cols], C.byref(h)))
+        out = Problem.__new__(Problem)
+        out._adopt(h.value)
+        out.mover = self
+        return out
 
     def mask(self):
         m = np.zeros(self.input_rows, np.uint8)

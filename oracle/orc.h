@@ -102,6 +102,12 @@ int orc_prepare_masked(uint64_t rows, const uint64_t *worker, const uint64_t *fi
 int orc_prepare_observation(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
                             const uint64_t *deletion, const double *outcome,
                             const uint64_t *frequency, const double *target, orc_problem **out);
+/* The stayer hybrid (stayer_hybrid.rs:71-348): eligible stayer rows (1-based
+ * dense retained-mover firm and stayer-worker indices) appended to the match
+ * problem m; stayer rows become observation units. */
+int orc_prepare_stayers(const orc_problem *m, uint64_t rows, const uint64_t *firm, const uint64_t *worker,
+                        const double *outcome, const uint64_t *frequency, const double *target,
+                        orc_problem **out);
 void orc_problem_free(orc_problem *p);
 /* Observation deletion: leverage atoms of `columns` probes from `first`, one
  * per stored row, out[col * rows + row] (generic_jla.rs:5660-5706). */

@@ -497,6 +497,18 @@ static int cmp_observation_rows(const void *a, const void *b, void *arg) {
     return x < y ? -1 : (x > y);
 }
 
+/* Dense 1-based observation ranks of every stored row; `order` returns the
+ * rows sorted by (rank, row) (generic_jla.rs:4267-4303). */
+static void observation_ranks(const orc_problem *p, uint64_t *order, uint64_t *rank) {
+    for (uint64_t i = 0; i < p->rows; ++i) order[i] = i;
+    qsort_r(order, p->rows, 8, cmp_observation_rows, (void *)p);
+    uint64_t current = 0;
+    for (uint64_t i = 0; i < p->rows; ++i) {
+        if (i == 0 || cmp_observation_key(p, order[i - 1], order[i]) != 0) ++current;
+        rank[order[i]] = current;
+    }
+}
+
 /* jla.rs:636-646 (target_plan ordering) */
 static int cmp_target_plan(const void *a, const void *b, void *arg) {
     const orc_problem *p = arg;
@@ -576,6 +588,93 @@ int orc_prepare_observation(uint64_t rows, const uint64_t *worker, const uint64_
                             const uint64_t *deletion, const double *outcome,
                             const uint64_t *frequency, const double *target, orc_problem **out) {
     return prepare_impl(rows, worker, firm, deletion, outcome, frequency, target, NULL, 1, out);
+}
+
+/* stayer_hybrid.rs:71-348 (prepare_exact_stayer_hybrid): eligible stayer
+ * rows, 1-based dense retained-mover firm and stayer-worker indices, appended
+ * to the prepared mover problem m (identifiers of m's dense levels; one
+ * deletion unit per stayer row) and recompressed with every row retained.
+ * Stayer rows become observation units (generic_jla.rs:2170-2213): entity
+ * the row's observation class ranked over all rows, copies numbered on over
+ * the class's stayer rows (:4217-4265). */
+int orc_prepare_stayers(const orc_problem *m, uint64_t rows, const uint64_t *firm, const uint64_t *worker,
+                        const double *outcome, const uint64_t *frequency, const double *target,
+                        orc_problem **out) {
+    if (m->observation)
+        return orc_fail(ORC_UNSUPPORTED, "session_stayer_augmentation",
+                        "the stayer hybrid requires match-deletion mover preparation");
+    uint64_t SW = 0;
+    for (uint64_t r = 0; r < rows; ++r) SW = worker[r] > SW ? worker[r] : SW;
+    if ((rows == 0) != (SW == 0))
+        return orc_fail(ORC_INVALID_IDENTIFIER, "stayer_augmentation",
+                        "eligible stayer workers must be a nonempty dense map when rows are supplied");
+    uint64_t *wf = orc_alloc(SW + 1, 8), *wp = orc_alloc(SW + 1, 8);
+    int rc = ORC_OK;
+    for (uint64_t r = 0; r < rows && !rc; ++r) {
+        if (firm[r] == 0 || firm[r] > m->firms)
+            rc = orc_fail(ORC_INVALID_IDENTIFIER, "stayer_augmentation", "stayer row is not attached to a retained mover firm");
+        else if (worker[r] == 0 || worker[r] > SW)
+            rc = orc_fail(ORC_INVALID_IDENTIFIER, "stayer_augmentation", "stayer row has a non-dense worker identifier");
+        else if (wf[worker[r]] && wf[worker[r]] != firm[r])
+            rc = orc_fail(ORC_INVALID_IDENTIFIER, "stayer_augmentation", "eligible stayer worker spans more than one firm");
+        else {
+            wf[worker[r]] = firm[r];
+            wp[worker[r]] += frequency[r];
+        }
+    }
+    for (uint64_t w = 1; w <= SW && !rc; ++w)
+        if (!wf[w]) rc = orc_fail(ORC_INVALID_IDENTIFIER, "stayer_augmentation", "eligible stayer worker identifiers are not dense");
+    for (uint64_t w = 1; w <= SW && !rc; ++w)
+        if (wp[w] < 2)
+            rc = orc_fail(ORC_INVALID_INPUT, "stayer_augmentation",
+                          "every eligible stayer worker must represent at least two physical observations");
+    free(wf);
+    free(wp);
+    if (rc) return rc;
+    uint64_t RM = m->rows, R = RM + rows;
+    uint64_t *cw = orc_alloc(R, 8), *cf = orc_alloc(R, 8), *cd = orc_alloc(R, 8), *cq = orc_alloc(R, 8);
+    double *cy = orc_alloc(R, 8), *ct = orc_alloc(R, 8);
+    uint8_t *all = orc_alloc(R, 1);
+    for (uint64_t r = 0; r < RM; ++r) {
+        cw[r] = (uint64_t)m->row_worker[r] + 1; cf[r] = (uint64_t)m->row_firm[r] + 1;
+        cd[r] = (uint64_t)m->row_deletion[r] + 1; cy[r] = m->outcome[r]; cq[r] = m->frequency[r];
+        ct[r] = m->target[r];
+    }
+    for (uint64_t r = 0; r < rows; ++r) {
+        cw[RM + r] = m->workers + worker[r]; cf[RM + r] = firm[r]; cd[RM + r] = m->units + r + 1;
+        cy[RM + r] = outcome[r]; cq[RM + r] = frequency[r]; ct[RM + r] = target[r];
+    }
+    memset(all, 1, R);
+    orc_problem *p = NULL;
+    rc = prepare_impl(R, cw, cf, cd, cy, cq, ct, all, 0, &p);
+    free(cw); free(cf); free(cd); free(cq); free(cy); free(ct); free(all);
+    if (rc) return rc;
+    if (p->rows != R || p->workers != m->workers + SW || p->firms != m->firms || p->units != m->units + rows) {
+        orc_problem_free(p);
+        return orc_fail(ORC_INVARIANT, "stayer_augmentation", "combined exact problem dimensions do not reconcile");
+    }
+    p->stayers = 1;
+    p->match_units = m->units;
+    p->unit_offset = orc_alloc(p->units, 8);
+    if (rows) {
+        uint64_t *order = orc_alloc(R, 8), *rank = orc_alloc(R, 8);
+        observation_ranks(p, order, rank);
+        uint64_t offset = 0, last = 0;
+        for (uint64_t i = 0; i < R; ++i) {
+            uint64_t row = order[i];
+            if (row < RM) continue;
+            if (rank[row] != last) offset = 0;
+            last = rank[row];
+            uint64_t u = m->units + (row - RM);
+            p->unit_rank[u] = rank[row];
+            p->unit_offset[u] = offset;
+            offset += p->frequency[row];
+        }
+        free(order);
+        free(rank);
+    }
+    *out = p;
+    return ORC_OK;
 }
 
 static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *firm,
@@ -764,16 +863,12 @@ static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *f
          * (observation_classes: rows of equal rank, ascending) and
          * statistical_batches.rs:15-53 (per-row copy offsets) */
         for (uint64_t i = 0; i < R; ++i) order[i] = i;
-        qsort_r(order, R, 8, cmp_observation_rows, p);
         p->row_rank = orc_alloc(R, 8);
+        observation_ranks(p, order, p->row_rank);
         p->unit_offset = orc_alloc(R, 8);
-        uint64_t current = 0, offset = 0;
+        uint64_t offset = 0;
         for (uint64_t i = 0; i < R; ++i) {
-            if (i == 0 || cmp_observation_key(p, order[i - 1], order[i]) != 0) {
-                ++current;
-                offset = 0;
-            }
-            p->row_rank[order[i]] = current;
+            if (i == 0 || p->row_rank[order[i - 1]] != p->row_rank[order[i]]) offset = 0;
             p->unit_offset[order[i]] = offset;
             offset += p->frequency[order[i]];
         }
@@ -872,6 +967,7 @@ static int prepare_impl(uint64_t rows, const uint64_t *worker, const uint64_t *f
     }
     free(order);
     free(cw); free(cf); free(cd); free(dw); free(df);
+    p->match_units = observation ? 0 : p->units;
     *out = p;
     return ORC_OK;
 fail_early:

@@ -45,6 +45,11 @@ struct orc_problem {
      * row's observation class (generic_jla.rs:4158-4215) */
     int observation;
     uint64_t *unit_offset;
+    /* units [0, match_units) are match units, the rest observation units
+     * (the last stored rows); stayers: a mover problem augmented with
+     * eligible stayers (stayer_hybrid.rs:71-348) */
+    uint64_t match_units;
+    int stayers;
 };
 
 #endif

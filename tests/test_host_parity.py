@@ -131,6 +131,91 @@ def test_prepare_observation_plan(oracle, seed):
         e.release()
 
 
+def test_stayer_hybrid_plan(oracle):
+    """Stayer augmentation (stayer_hybrid.rs:71-348): the host's combined
+    problem and hybrid plan (mover match units with their semantic entities,
+    stayer rows as observation units with class ranks and copy offsets,
+    strata) equal the oracle's."""
+    from fevc_amd import synth
+    spec = synth.PanelSpec(workers=800, firms=60, periods=4, mobility=0.3, seed=4, name="stayers")
+    data = synth.generate(spec)
+    cols = [np.asarray(data[k], float) for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    cols[4] = np.random.default_rng(4).integers(1, 4, cols[0].size).astype(float)
+    p = oracle.Problem(*cols)
+    st = synth.eligible_stayers(cols[0], cols[1], cols[3], cols[4], cols[5], p.mask())
+    h = p.with_stayers(*st)
+    e = F.Engine(*cols, upload=False)
+    try:
+        rec = e.augment_stayers(*st)
+        d = e.dims
+        assert (d["rows"], d["physical"], d["workers"], d["firms"], d["cells"], d["units"], d["strata"]) == (
+            h.rows, h.physical, h.workers, h.firms, h.cells, h.units, h.strata)
+        assert d["topology_checksum"] == h.topology_checksum
+        for which, name, n in [(20, "unit_rank", h.units), (21, "unit_count", h.units),
+                               (22, "stratum_rank", h.strata), (23, "stratum_count", h.strata),
+                               (24, "cell_weight", h.cells), (25, "unit_cell", h.units),
+                               (26, "unit_offset", h.units)]:
+            assert np.array_equal(e.array(which, n), h.array(name).astype(np.float64)), name
+        gather = e.array(27, h.units).astype(np.int64)
+        assert np.array_equal(np.sort(gather), np.arange(h.units))
+        assert (rec["stayer_stored_rows"], rec["stayer_workers"], rec["combined_workers"]) == (
+            st[0].size, int(st[1].max()), h.workers)
+        assert rec["stayer_physical_mass"] == int(st[3].sum())
+        assert rec["combined_deletion_units"] == p.units + int(st[3].sum())
+        assert abs(rec["combined_target_mass"] - rec["mover_target_mass"] - rec["stayer_target_mass"]) < 1e-9
+        # ffi_engine.rs:4840-4861: mover-only solves are refused once augmented
+        e.stayers_mode = 1
+        with pytest.raises(F.EngineError) as ei:
+            e.solve(probes=8)
+        assert ei.value.code == 20
+        # context.rs:385-404: a second augmentation needs the Prepared state
+        with pytest.raises(F.EngineError) as ei:
+            e.augment_stayers(*st)
+        assert ei.value.code in (81,)
+    finally:
+        e.release()
+
+
+def test_stayer_augmentation_refusals():
+    from fevc_amd import synth
+    spec = synth.PanelSpec(workers=400, firms=40, periods=4, mobility=0.3, seed=6, name="stayers-bad")
+    data = synth.generate(spec)
+    cols = [np.asarray(data[k], float) for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    good = [np.array([1.0, 1.0]), np.array([1.0, 1.0]), np.array([0.5, 0.7]), np.array([1.0, 1.0]),
+            np.array([1.0, 1.0])]
+    cases = [
+        ([np.array([0.0, 1.0])] + good[1:], 21),                     # firm not a retained mover firm
+        ([np.array([1.0, 2.0])] + good[1:], 21),                     # one stayer in two firms
+        (good[:1] + [np.array([1.0, 3.0])] + good[2:], 21),          # stayer workers not dense
+        ([good[0][:1], good[1][:1], good[2][:1], good[3][:1], good[4][:1]], 20),  # one physical observation
+    ]
+    for stayers, code in cases:
+        e = F.Engine(*cols, upload=False)
+        try:
+            with pytest.raises(F.EngineError) as ei:
+                e.augment_stayers(*stayers)
+            assert ei.value.code == code, (stayers, ei.value)
+        finally:
+            e.release()
+    # stayers-all without an augmentation (ffi_engine.rs:4850-4855)
+    e = F.Engine(*cols, upload=False)
+    try:
+        e.stayers_mode = 2
+        with pytest.raises(F.EngineError) as ei:
+            e.solve(probes=8)
+        assert ei.value.code == 20
+    finally:
+        e.release()
+    # the hybrid needs a match-deletion mover generation (session_retained.rs:866-872)
+    e = F.Engine(*cols, upload=False, deletion_mode="observation")
+    try:
+        with pytest.raises(F.EngineError) as ei:
+            e.augment_stayers(*good)
+        assert ei.value.code == 11
+    finally:
+        e.release()
+
+
 def test_prepare_refuses_bad_input():
     K, cols = kat_columns()
     bad = [c.astype(np.float64) for c in cols]
