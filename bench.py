@@ -325,6 +325,25 @@ def main():
                 oe.release()
             observation[mode] = entry
             log(f"{mode} deletion KSS ({sub.name}, 128 probes): {ow:.3f}s, corrected {ores['corrected']}")
+        # stayers(both): the match-deletion movers plus the eligible stayers
+        he = F.Engine(*ocols)
+        try:
+            st = synth.eligible_stayers(ocols[0], ocols[1], ocols[3], ocols[4], ocols[5], he.mask())
+            rec = he.augment_stayers(*st)
+            he.configure(args.precision, 64, 32)
+            he.solve(probes=16, tolerance=args.tolerance)
+            t0 = time.perf_counter()
+            hres = he.solve(probes=128, tolerance=args.tolerance)
+            hw = time.perf_counter() - t0
+            observation["stayers_both"] = {
+                "mover_rows": rec["mover_stored_rows"], "stayer_rows": rec["stayer_stored_rows"],
+                "stayer_workers": rec["stayer_workers"], "deletion_units": rec["combined_deletion_units"],
+                "wall_seconds": round(hw, 4), "probe_solves_per_s": round(3 * 128 / hw, 1),
+                "corrected": hres["corrected"], "numerical_mcse": hres["numerical_mcse"]}
+        finally:
+            he.release()
+        log(f"stayer hybrid KSS ({sub.name}, {rec['stayer_workers']} stayers, 128 probes): {hw:.3f}s, "
+            f"corrected {hres['corrected']}")
 
     line = {
         "metric": "probe solves/s at rel.res 1e-8 (leverage probes, learned+batched CMG-PCG)",
