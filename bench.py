@@ -34,6 +34,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -167,8 +169,6 @@ def main():
 
     kss = None
     if args.kss_probes > 0:
-        import numpy as np
-
         eng.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
         eng.smoother(1 if args.learned else 0)
         t0 = time.perf_counter()
