@@ -9,6 +9,9 @@
 // Every level is renumbered so that aggregates are contiguous (restriction is
 // a contiguous segmented sum, prolongation labels are monotone); the level-0
 // renumbering is undone at the firm/auxiliary boundary of the hybrid system.
+#include <algorithm>
+#include <numeric>
+
 #include "device_impl.hpp"
 #include "numerical.hpp"
 
@@ -595,12 +598,19 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
           return;
         }
         const unsigned g = reduce_grid(E.G * K);
-        hipLaunchKernelGGL((dev::k_numerical_scores<K>), g, kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
-                           E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p, E.ngrad.p,
-                           E.noffs.p, E.nfolds.p, E.rpartial.p);
+        if (E.Go)  // observation units (generic_jla/numerical.rs:617-641)
+          hipLaunchKernelGGL((dev::k_obs_scores<K>), g, kBlock, 0, E.stream, E.Go, k, first, opt.seed, kTagLeverage,
+                             E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm, E.unit_rank.p + E.Gm,
+                             E.unit_offset.p + E.Gm, E.row_phys.p, E.obs_physical, E.cell_worker.p, E.cell_firm.p,
+                             E.worker.p, E.fsol.p, E.ngrad.p, E.ncopy.p, E.nfolds.p, E.rpartial.p);
+        else
+          hipLaunchKernelGGL((dev::k_numerical_scores<K>), g, kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+                             E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
+                             E.ngrad.p, E.noffs.p, E.nfolds.p, E.rpartial.p);
         finish<K, 6>(E, E.rpartial.p, g, dev::FinScores{E.nscores.p});
       });
-      for (uint64_t g = 0; g < E.G; ++g) num.replay_generator_words += (E.P->unit_count[g] + 63) / 64 * k;
+      for (uint64_t g = 0; g < E.Gm; ++g) num.replay_generator_words += (E.P->unit_count[g] + 63) / 64 * k;
+      num.replay_generator_words += E.obs_class_words * k;  // generic_jla/numerical.rs:490-519
       if (failed) {
         status = 7;
         break;
@@ -779,8 +789,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
-  if (E.Go && opt.numerical)
-    fail(kUnsupported, "numerical_solve", "no numerical attachment with observation units");
+  if (E.Go && E.Gm && opt.numerical)
+    fail(kUnsupported, "numerical_solve", "no numerical attachment with stayers");
   const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
   const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
   const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
@@ -899,7 +909,86 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
 
   // ---- numerical attachment state (engine/numerical.rs:62-122)
   NumericalResult &num = res.numerical;
-  if (opt.numerical) {
+  if (opt.numerical && E.Go) {
+    // observation units (generic_jla/numerical.rs:172-270): per copy the
+    // finite derivative of its inverse, pulled back to the row's p^2, p^4 and
+    // the copy's s p, s p^3 (numerical_mc.rs:137-145); units in class order
+    // (class rank, then row), so that the last event sets the status
+    res.has_numerical = true;
+    const uint64_t Go = E.Go, N = E.obs_physical;
+    std::vector<double> m4, corr;
+    E.obs_m.download(m4, 4 * Go, E.stream);
+    E.obs_corr.download(corr, 4 * N, E.stream);
+    std::vector<uint64_t> phys(Go + 1, 0);
+    for (uint64_t i = 0; i < Go; ++i) phys[i + 1] = phys[i] + P.unit_count[E.Gm + i];
+    std::vector<uint64_t> order(Go);
+    std::iota(order.begin(), order.end(), 0ull);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint64_t a, uint64_t b) { return P.unit_rank[E.Gm + a] < P.unit_rank[E.Gm + b]; });
+    E.obs_class_words = 0;
+    for (uint64_t k = 0; k < Go;) {
+      uint64_t mass = 0, q = k;
+      for (; q < Go && P.unit_rank[E.Gm + order[q]] == P.unit_rank[E.Gm + order[k]]; ++q)
+        mass += P.unit_count[E.Gm + order[q]];
+      E.obs_class_words += (mass + 63) / 64;
+      k = q;
+    }
+    std::vector<double> grad(3 * Go, 0.0), copy(2 * N, 0.0);
+    num.minimum_constrained = INFINITY;
+    num.minimum_residual_margin = INFINITY;
+    num.maximum_sensitivity_ratio = 0.0;
+    bool clipped = false;
+    uint32_t event = 0;
+    const double r = (double)R;
+    for (uint64_t k = 0; k < Go; ++k) {
+      const uint64_t i = order[k];
+      const double a2 = (m4[i] + m4[Go + i]) / r, a4 = (m4[2 * Go + i] + m4[3 * Go + i]) / r;
+      Neumaier g2, g4, off;
+      for (uint64_t x = phys[i]; x < phys[i + 1]; ++x) {
+        const double c1 = (corr[x] + corr[N + x]) / r, c3 = (corr[2 * N + x] + corr[3 * N + x]) / r;
+        const double u[5] = {a2, 1.0 + a2 - 2.0 * c1, a4, 1.0 + 6.0 * a2 + a4 - 4.0 * c1 - 4.0 * c3,
+                             a2 + a4 - 2.0 * c3};
+        num.minimum_constrained = std::min(num.minimum_constrained, u[0] + u[1]);
+        double gr[5], h = 0.0, mm = 0.0, var = 0.0;
+        const int code = dev::unit_linearisation(u, r, gr, h, mm, var);
+        if (code == 1) {
+          event = 6;
+          continue;
+        }
+        num.minimum_residual_margin = std::min(num.minimum_residual_margin, mm);
+        num.maximum_sensitivity_ratio = std::max(num.maximum_sensitivity_ratio, std::sqrt(std::max(var, 0.0)) / mm);
+        if (var < 0.0) clipped = true;
+        if (code == 5) {
+          event = 5;
+          continue;
+        }
+        if (code == 2) {
+          event = 6;
+          continue;
+        }
+        const double pb[4] = {gr[0] + gr[1] + 6.0 * gr[3] + gr[4], gr[2] + gr[3] + gr[4],
+                              -2.0 * gr[1] - 4.0 * gr[3], -4.0 * gr[3] - 2.0 * gr[4]};
+        g2.add(pb[0]);
+        g4.add(pb[1]);
+        off.add(pb[2] * c1);
+        off.add(pb[3] * c3);
+        copy[x] = pb[2];
+        copy[N + x] = pb[3];
+      }
+      off.add(g2.value() * a2);
+      off.add(g4.value() * a4);
+      grad[i] = g2.value();
+      grad[Go + i] = g4.value();
+      grad[2 * Go + i] = off.value();
+    }
+    // the adjustment's variance clipping first (generic_jla/numerical.rs:172-176), then the last event
+    num.status = event ? event : (clipped ? 5u : 2u);
+    E.ngrad.upload(grad);
+    E.ncopy.upload(copy);
+    E.nfolds.alloc(12 * Go);
+    VK_HIP(hipMemsetAsync(E.nfolds.p, 0, 12 * Go * sizeof(double), E.stream));
+    num.allocation_bytes = (3 * Go + 2 * N + 12 * Go) * sizeof(double) + 6 * R * sizeof(double);
+  } else if (opt.numerical) {
     res.has_numerical = true;
     E.ngrad.alloc(5 * E.G);
     E.noffs.alloc(E.G);
@@ -958,7 +1047,11 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bf, kBlock, 0, E.stream, E.F, E.frhs.p, E.rpartial.p);
       finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.status.p});
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
-      if (opt.numerical)
+      if (opt.numerical && E.Go)  // observation folds: y e kappa / fold size (generic_jla/numerical.rs:377-389)
+        hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.Go), kBlock, 0, E.stream, E.Go, k, first, R,
+                           E.unit_cell.p + E.Gm, E.cell_worker.p, E.cell_firm.p, E.row_outcome.p,
+                           E.obs_adjust.p + 4 * E.Go, E.worker.p, E.fsol.p, E.nfolds.p);
+      else if (opt.numerical)
         hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, first, R,
                            E.unit_cell.p, E.cell_worker.p, E.cell_firm.p, E.unit_outcome.p, E.adjust.p + 4 * E.G,
                            E.worker.p, E.fsol.p, E.nfolds.p);

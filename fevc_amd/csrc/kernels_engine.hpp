@@ -905,26 +905,26 @@ static __global__ void k_obs_adjust(uint64_t rows, const double *__restrict__ m,
 // gradient of 1/m + bias/m^2 - variance/m^3 in the five means.  Returns 0
 // (defined), 1 (no finite derivative), 5 (nonsmooth: variance < 0) or 2 (no
 // finite gradient); h, m and the variance are set unless it returns 1.
-__device__ inline int unit_linearisation(const double (&u)[5], double r, double (&grad)[5], double &h_out,
-                                         double &m_out, double &var_out) {
+__host__ __device__ inline int unit_linearisation(const double (&u)[5], double r, double (&grad)[5], double &h_out,
+                                                  double &m_out, double &var_out) {
   const double p = u[0], mm = u[1], a = u[2], b = u[3], c = u[4];
   const double s = p + mm;
   if (!(s > 0.0)) return 1;
 #pragma unroll
   for (int j = 0; j < 5; ++j)
-    if (!isfinite(u[j])) return 1;
+    if (!__builtin_isfinite(u[j])) return 1;
   const double h = p / s, m = mm / s;
   const double bias = (m * a - h * b + (m - h) * c) / r;
   const double var = (m * m * a + h * h * b - 2.0 * h * m * c) / r;
   double dh[5] = {m / s, -h / s, 0.0, 0.0, 0.0}, db[5], dv[5];
-  bool finite = isfinite(h) && isfinite(m) && isfinite(bias) && isfinite(var);
+  bool finite = __builtin_isfinite(h) && __builtin_isfinite(m) && __builtin_isfinite(bias) && __builtin_isfinite(var);
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     const double da = j == 2, dbb = j == 3, dc = j == 4;
     db[j] = (-(a + b + 2.0 * c) * dh[j] + m * da - h * dbb + (m - h) * dc) / r;
     dv[j] = ((-2.0 * m * a + 2.0 * h * b - 2.0 * c * (m - h)) * dh[j] + m * m * da + h * h * dbb -
              2.0 * h * m * dc) / r;
-    finite = finite && isfinite(dh[j]) && isfinite(db[j]) && isfinite(dv[j]);
+    finite = finite && __builtin_isfinite(dh[j]) && __builtin_isfinite(db[j]) && __builtin_isfinite(dv[j]);
   }
   if (!finite) return 1;
   h_out = h;
@@ -937,7 +937,7 @@ __device__ inline int unit_linearisation(const double (&u)[5], double r, double 
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     grad[j] = -factor * dh[j] + inv2 * db[j] - inv3 * dv[j];
-    if (!isfinite(grad[j])) return 2;
+    if (!__builtin_isfinite(grad[j])) return 2;
   }
   return 0;
 }
@@ -1084,6 +1084,64 @@ __global__ void __launch_bounds__(kBlock) k_numerical_scores(uint64_t units, uin
   }
   block_partial<K, 6>(acc, partial);
 }
+// Replay scores of observation units (generic_jla/numerical.rs:617-641): per
+// replayed probe (lane) the six sums over units of -fold[g][f][j] * alpha_g,
+// alpha_g = g2 p^2 + g4 p^4 - offset + (sum_copies q1 s) p + (sum_copies q3 s) p^3,
+// with the copies' signs s regenerated from the leverage Counter-V1 words.
+// grad: 3 x units (g2, g4, offset); copy: 2 x copies (q1, q3).
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_obs_scores(uint64_t units, uint32_t kcols, uint64_t first_probe,
+                                                       uint64_t seed, uint64_t tag,
+                                                       const uint32_t *__restrict__ unit_cell,
+                                                       const uint64_t *__restrict__ unit_count,
+                                                       const uint64_t *__restrict__ unit_rank,
+                                                       const uint64_t *__restrict__ unit_offset,
+                                                       const uint64_t *__restrict__ row_phys, uint64_t physical,
+                                                       const uint32_t *__restrict__ cell_worker,
+                                                       const uint32_t *__restrict__ cell_firm,
+                                                       const double *__restrict__ worker,
+                                                       const double *__restrict__ firm,
+                                                       const double *__restrict__ grad,
+                                                       const double *__restrict__ copy,
+                                                       const double *__restrict__ folds,
+                                                       double *__restrict__ partial) {
+  const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  VK_GRID_LOOP(t, units * K) {
+    const uint64_t g = t / K;
+    const int j = t % K;
+    if (j >= (int)kcols) continue;
+    const uint32_t c = unit_cell[g];
+    const uint64_t w = cell_worker[c], f = cell_firm[c];
+    const double p = worker[w * K + j] + firm[f * K + j];
+    const uint64_t probe = first_probe + (uint64_t)j;
+    const uint64_t lo = unit_offset[g], hi = lo + unit_count[g], base = row_phys[g];
+    double w1 = 0.0, w1e = 0.0, w3 = 0.0, w3e = 0.0;
+    for (uint64_t word = lo / 64; word * 64 < hi; ++word) {
+      uint64_t blk[4] = {unit_rank[g], probe / 4, word, tag};
+      philox(k0, k1, blk);
+      const uint64_t bits = blk[probe % 4];
+      const uint64_t c0 = lo > word * 64 ? lo : word * 64, c1 = hi < word * 64 + 64 ? hi : word * 64 + 64;
+      for (uint64_t cp = c0; cp < c1; ++cp) {
+        const double s = ((bits >> (cp % 64)) & 1) ? 1.0 : -1.0;
+        const uint64_t x = base + (cp - lo);
+        neumaier_add(w1, w1e, copy[x] * s);
+        neumaier_add(w3, w3e, copy[physical + x] * s);
+      }
+    }
+    double a = 0.0, ae = 0.0;
+    neumaier_add(a, ae, grad[g] * p * p);
+    neumaier_add(a, ae, grad[units + g] * (p * p * p * p));
+    neumaier_add(a, ae, -grad[2 * units + g]);
+    neumaier_add(a, ae, (w1 + w1e) * p);
+    neumaier_add(a, ae, (w3 + w3e) * (p * p * p));
+    const double alpha = a + ae;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) acc[v] -= (folds[(uint64_t)v * units + g] + folds[(uint64_t)(6 + v) * units + g]) * alpha;
+  }
+  block_partial<K, 6>(acc, partial);
+}
+
 struct FinScores {
   double *__restrict__ out;  // [lane][fold * 3 + j]
   __device__ bool enabled() const { return true; }

@@ -654,11 +654,22 @@ static int target_columns(const orc_problem *p, const int64_t *atoms, uint64_t p
 
 /* numerical attachment state (engine/numerical.rs:49-122) */
 typedef struct {
-    double *grad;  /* 5 per unit */
+    double *grad;  /* 5 per unit (observation units: g2, g4, offset) */
     double *offs;  /* per unit */
     ssum *folds;   /* 6 per unit: [fold][sw^2, sf^2, sw sf] */
     uint32_t status;
+    /* observation units (generic_jla/numerical.rs:178-270): copy pullbacks
+     * (q1, q3) per physical copy, the units in (class rank, row) order, the
+     * copy ranges and the Counter-V1 words per replay column */
+    double *copy;
+    uint64_t *order;
+    const uint64_t *phys;
+    uint64_t class_words;
 } num_state;
+
+static int leverage_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k, int64_t *atoms);
+static int copy_sign(uint64_t seed, uint64_t probe, uint64_t entity, uint64_t copy);
+static uint64_t observation_row(const orc_problem *p, uint64_t g);
 
 static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
                       orc_numerical_result *num);
@@ -693,14 +704,16 @@ static int numerical_replay(const orc_problem *p, const orc_engine_options *opt,
     double *wr = orc_alloc(W * batch, 8), *fr = orc_alloc(F * batch, 8);
     column_t *cols = alloc_cols(p, batch);
     num->failed_replay_probe = UINT32_MAX;
+    const uint64_t Gm = p->match_units, Go = G - Gm;
     if (status == 2) {
         for (uint64_t first = 0; first < R; first += batch) {
             uint64_t width = R - first < batch ? R - first : batch;
-            if ((rc = orc_rademacher_fill(opt->seed, ORC_TAG_LEVERAGE, first, width, p->unit_rank,
-                                          p->unit_count, G, atoms)))
-                goto out;
-            for (uint64_t g = 0; g < G; ++g)
+            if ((rc = leverage_atoms(p, opt->seed, first, width, atoms))) goto out;
+            for (uint64_t g = 0; g < Gm; ++g)
                 num->replay_generator_words += (p->unit_count[g] + 63) / 64 * width;
+            /* generic_jla/numerical.rs:490-519: one evaluation per column for
+             * every word a class's copies touch */
+            num->replay_generator_words += ns->class_words * width;
             for (uint64_t c = 0; c < width; ++c) leverage_rhs(p, atoms + c * G, wr + c * W, fr + c * F);
             num->attempted_replay_rhs += width;
             rc = solve_batch(s, wr, fr, width, opt->probe_tolerance, 1.0, opt, cols);
@@ -718,7 +731,28 @@ static int numerical_replay(const orc_problem *p, const orc_engine_options *opt,
                 num->certified_replay_rhs++;
                 num->max_replay_complete_residual = fmax(num->max_replay_complete_residual, cols[c].complete);
                 ssum sc[2][3] = {{{0, 0}, {0, 0}, {0, 0}}, {{0, 0}, {0, 0}, {0, 0}}};
-                for (uint64_t g = 0; g < G; ++g) {
+                /* observation units (generic_jla/numerical.rs:617-641) */
+                for (uint64_t k = 0; k < Go; ++k) {
+                    uint64_t i = ns->order[k], g = Gm + i;
+                    uint32_t cell = p->unit_cell[g];
+                    double pp = cols[c].worker[p->cell_worker[cell]] + cols[c].firm[p->cell_firm[cell]];
+                    ssum w1 = {0, 0}, w3 = {0, 0};
+                    for (uint64_t j = 0; j < p->unit_count[g]; ++j) {
+                        double q = (double)copy_sign(opt->seed, probe, p->unit_rank[g], p->unit_offset[g] + j);
+                        ss_add(&w1, ns->copy[2 * (ns->phys[i] + j)] * q);
+                        ss_add(&w3, ns->copy[2 * (ns->phys[i] + j) + 1] * q);
+                    }
+                    ssum alpha = {0, 0};
+                    ss_add(&alpha, ns->grad[3 * i] * pp * pp);
+                    ss_add(&alpha, ns->grad[3 * i + 1] * pp * pp * pp * pp);
+                    ss_add(&alpha, -ns->grad[3 * i + 2]);
+                    ss_add(&alpha, ss_fin(w1) * pp);
+                    ss_add(&alpha, ss_fin(w3) * pp * pp * pp);
+                    double al = ss_fin(alpha);
+                    for (int f = 0; f < 2; ++f)
+                        for (int j = 0; j < 3; ++j) ss_add(&sc[f][j], -ss_fin(ns->folds[6 * i + 3 * f + j]) * al);
+                }
+                for (uint64_t g = 0; g < Gm; ++g) {
                     uint32_t cell = p->unit_cell[g];
                     double root = sqrt((double)p->unit_count[g]);
                     double pp = root * (cols[c].worker[p->cell_worker[cell]] + cols[c].firm[p->cell_firm[cell]]);
@@ -781,6 +815,25 @@ static int copy_sign(uint64_t seed, uint64_t probe, uint64_t entity, uint64_t co
 /* Units [match_units, units) are observation units: the last stored rows. */
 static uint64_t observation_row(const orc_problem *p, uint64_t g) {
     return g - p->match_units + (p->rows - (p->units - p->match_units));
+}
+
+static const orc_problem *g_order_problem;
+static int cmp_observation_units(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    const orc_problem *p = g_order_problem;
+    uint64_t rx = p->unit_rank[p->match_units + x], ry = p->unit_rank[p->match_units + y];
+    if (rx != ry) return rx < ry ? -1 : 1;
+    return x < y ? -1 : (x > y);
+}
+/* Observation units (indices from match_units) in class order: ascending
+ * class rank, rows ascending within a class (generic_jla.rs:4158-4215). */
+static uint64_t *observation_order(const orc_problem *p) {
+    uint64_t Go = p->units - p->match_units;
+    uint64_t *order = orc_alloc(Go ? Go : 1, 8);
+    for (uint64_t i = 0; i < Go; ++i) order[i] = i;
+    g_order_problem = p;
+    qsort(order, Go, 8, cmp_observation_units);
+    return order;
 }
 
 static void observation_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k,
@@ -866,7 +919,7 @@ static void observation_moments(const orc_problem *p, obs_state *o, uint64_t see
  * share, residual share, bias and variance are means over the copies;
  * residual mass = e; deleted mass = d. */
 static int observation_adjustment(const orc_problem *p, const obs_state *o, const orc_engine_options *opt,
-                                  orc_engine_result *res, ssum *cw) {
+                                  orc_engine_result *res, ssum *cw, int *clipped) {
     double probes = (double)res->probes;
     int rc = ORC_OK;
     for (uint64_t i = 0; i < p->units - p->match_units && !rc; ++i) {
@@ -892,6 +945,7 @@ static int observation_adjustment(const orc_problem *p, const obs_state *o, cons
                 rc = orc_fail(ORC_JLA_MOMENT, "generic_jla_leverage", "finite-projection moment failed");
                 break;
             }
+            if (variance < 0.0) *clipped = 1;
             variance = fmax(variance, 0.0);
             if (!isfinite(residual) || residual <= opt->block_tolerance) {
                 rc = orc_fail(ORC_NONESTIMABLE, "generic_jla_observation_adjustment",
@@ -985,8 +1039,8 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     /* leverage loop: engine.rs:1150-1246; observation deletion:
      * generic_jla.rs:5540-5892 */
     const uint64_t Gm = p->match_units, Go = G - Gm;
-    if (Go && num) {
-        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment with observation units");
+    if (Go && Gm && num) {
+        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment with stayers");
         goto fail;
     }
     if (Go) observation_state_init(p, &obs);
@@ -1074,7 +1128,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             res->deleted_mass[g] = deleted;
         }
         /* observation units: generic_jla.rs:5893-6017 */
-        if (Go && (rc = observation_adjustment(p, &obs, &opt, res, cw))) {
+        if (Go && (rc = observation_adjustment(p, &obs, &opt, res, cw, &clipped))) {
             free(cw);
             goto fail_mom;
         }
@@ -1082,8 +1136,69 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);
         free(cw);
     }
+    /* numerical state of observation units: generic_jla/numerical.rs:172-270 */
+    if (num && Go) {
+        ns.status = clipped ? 5 : 2;
+        num->minimum_constrained = INFINITY;
+        num->minimum_residual_margin = INFINITY;
+        num->maximum_sensitivity_ratio = 0.0;
+        ns.grad = orc_alloc(3 * Go, 8);
+        ns.copy = orc_alloc(2 * obs.phys[Go], 8);
+        ns.folds = orc_alloc(6 * Go, sizeof(ssum));
+        ns.phys = obs.phys;
+        ns.order = observation_order(p);
+        for (uint64_t k = 0; k < Go; ++k)
+            if (k == 0 || p->unit_rank[Gm + ns.order[k]] != p->unit_rank[Gm + ns.order[k - 1]]) {
+                uint64_t mass = 0;
+                for (uint64_t q = k; q < Go && p->unit_rank[Gm + ns.order[q]] == p->unit_rank[Gm + ns.order[k]]; ++q)
+                    mass += p->unit_count[Gm + ns.order[q]];
+                ns.class_words += (mass + 63) / 64;
+            }
+        double r = (double)R;
+        for (uint64_t k = 0; k < Go; ++k) {
+            uint64_t i = ns.order[k];
+            double a2 = ss_fin(obs.square[i]) / r, a4 = ss_fin(obs.fourth[i]) / r;
+            ssum g2 = {0, 0}, g4 = {0, 0}, off = {0, 0};
+            for (uint64_t q = obs.phys[i]; q < obs.phys[i + 1]; ++q) {
+                double c1 = ss_fin(obs.first[q]) / r, c3 = ss_fin(obs.third[q]) / r;
+                double u[5] = {a2, 1.0 + a2 - 2.0 * c1, a4, 1.0 + 6.0 * a2 + a4 - 4.0 * c1 - 4.0 * c3,
+                               a2 + a4 - 2.0 * c3};
+                num->minimum_constrained = fmin(num->minimum_constrained, u[0] + u[1]);
+                double fd[20], gr[5];
+                if (!orc_finite_derivative(u, (uint32_t)R, fd)) {
+                    ns.status = 6;
+                    continue;
+                }
+                double ell = fd[1];
+                num->minimum_residual_margin = fmin(num->minimum_residual_margin, ell);
+                num->maximum_sensitivity_ratio = fmax(num->maximum_sensitivity_ratio, sqrt(fmax(fd[3], 0.0)) / ell);
+                if (fd[19] != 0.0) {
+                    ns.status = 5;
+                    continue;
+                }
+                if (!orc_fd_observation(fd, 0.0, gr)) {
+                    ns.status = 6;
+                    continue;
+                }
+                /* numerical_mc.rs:137-145 (copy_pullback) */
+                double pb[4] = {gr[0] + gr[1] + 6.0 * gr[3] + gr[4], gr[2] + gr[3] + gr[4],
+                                -2.0 * gr[1] - 4.0 * gr[3], -4.0 * gr[3] - 2.0 * gr[4]};
+                ss_add(&g2, pb[0]);
+                ss_add(&g4, pb[1]);
+                ss_add(&off, pb[2] * c1);
+                ss_add(&off, pb[3] * c3);
+                ns.copy[2 * q] = pb[2];
+                ns.copy[2 * q + 1] = pb[3];
+            }
+            ss_add(&off, ss_fin(g2) * a2);
+            ss_add(&off, ss_fin(g4) * a4);
+            ns.grad[3 * i] = ss_fin(g2);
+            ns.grad[3 * i + 1] = ss_fin(g4);
+            ns.grad[3 * i + 2] = ss_fin(off);
+        }
+    }
     /* numerical state: engine/numerical.rs:62-122 (after the adjustment) */
-    if (num) {
+    if (num && !Go) {
         ns.grad = orc_alloc(5 * G, 8);
         ns.offs = orc_alloc(G, 8);
         ns.folds = orc_alloc(6 * G, sizeof(ssum));
@@ -1158,7 +1273,17 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
                 uint64_t probe = first + c;
                 int fold = (int)(probe % 2);
                 double count = fold == 0 ? (double)((R + 1) / 2) : (double)(R / 2);
-                for (uint64_t g = 0; g < G; ++g) {
+                /* observation units: generic_jla/numerical.rs:377-389 */
+                for (uint64_t i = 0; i < Go; ++i) {
+                    uint64_t g = Gm + i, row = observation_row(p, g);
+                    uint32_t cell = p->unit_cell[g];
+                    uint32_t w = p->cell_worker[cell], f = p->cell_firm[cell];
+                    double sw = a->worker[w] + a->firm[f], sf = b->worker[w] + b->firm[f];
+                    double kappa[3] = {sw * sw, sf * sf, sw * sf};
+                    for (int j = 0; j < 3; ++j)
+                        ss_add(&ns.folds[6 * i + 3 * fold + j], p->outcome[row] * res->residual_mass[g] * kappa[j] / count);
+                }
+                for (uint64_t g = 0; g < Gm; ++g) {
                     uint32_t cell = p->unit_cell[g];
                     uint32_t w = p->cell_worker[cell], f = p->cell_firm[cell];
                     double sw = a->worker[w] + a->firm[f], sf = b->worker[w] + b->firm[f];
@@ -1191,8 +1316,8 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         num->point_rhs = 1 + 3 * R;
         num->max_point_complete_residual = max_complete;
         rc = numerical_replay(p, &opt, &s, &ns, res->draws, num);
-        free(ns.grad); free(ns.offs); free(ns.folds);
-        ns.grad = NULL; ns.offs = NULL; ns.folds = NULL;
+        free(ns.grad); free(ns.offs); free(ns.folds); free(ns.copy); free(ns.order);
+        ns.grad = NULL; ns.offs = NULL; ns.folds = NULL; ns.copy = NULL; ns.order = NULL;
         if (rc) goto fail;
     }
     res->sum.solve_seconds = now_seconds() - t0;
@@ -1208,6 +1333,8 @@ fail_mom:
     free(mom);
 fail:
     observation_state_free(&obs);
+    free(ns.copy);
+    free(ns.order);
     free(ns.grad);
     free(ns.offs);
     free(ns.folds);

@@ -101,6 +101,28 @@ def test_numerical_on_a_deep_hierarchy(oracle):
     _compare(want, got, 1e-5)
 
 
+def test_numerical_under_observation_deletion(oracle):
+    """Observation deletion (generic_jla/numerical.rs:172-751): per-copy
+    pullbacks, observation folds and replayed copy signs, against the oracle."""
+    from oracle import oracle as O
+    cols = list(columns(SPEC))
+    cols[4] = np.random.default_rng(9).integers(1, 4, len(cols[0])).astype(np.float64)
+    R, tol = 24, 1e-11
+    want = O.Problem(*cols, observation=True).run(probes=R, leverage_width=4, target_width=4, fit_tolerance=1e-10,
+                                                 probe_tolerance=tol, numerical=True)
+    for precision in (64, 32):
+        e = F.Engine(*cols, deletion_mode="observation")
+        try:
+            e.configure(precision, 64, 32)
+            print(f"observation numerical, precision {precision}", flush=True)
+            res, got = e.solve_numerical(probes=R, tolerance=tol)
+        finally:
+            e.release()
+        _compare(want, got, 1e-6)
+        for key in ("plugin", "correction", "corrected"):
+            assert np.allclose(res[key], want[key], rtol=1e-8, atol=0)
+
+
 def test_numerical_result_requires_the_attachment():
     e = F.Engine(*columns(SPEC))
     try:

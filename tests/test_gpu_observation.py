@@ -93,9 +93,6 @@ def test_observation_fp32_vcycle_and_widths(oracle):
 def test_observation_refusals():
     e = F.Engine(*panel(), deletion_mode="observation")
     try:
-        with pytest.raises(F.EngineError) as ei:  # numerical MCSE under observation deletion
-            e.solve_numerical(probes=8, tolerance=1e-8)
-        assert ei.value.code == 11
         e.deletion_mode = 1  # solve mode differs from the prepared graph's (ffi_engine.rs:5705-5710)
         with pytest.raises(F.EngineError) as ei:
             e.solve(probes=8, tolerance=1e-8)

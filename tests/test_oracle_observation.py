@@ -94,3 +94,42 @@ def test_observation_physical_split_preserves_results(oracle):
     other[3][row + 1] = np.nextafter(other[3][row + 1], np.inf)  # its own class
     _, moved = _run(oracle, other, probes=64)
     assert not np.allclose(base["correction"], moved["correction"], rtol=1e-6, atol=0)
+
+
+# ---------------------------------------------------------------- numerical MCSE
+# generic_jla/numerical.rs:172-751 for observation units.  Pins: the replayed
+# leverage probes regenerate the leverage phase's copy signs and predictions,
+# so every fold's scores average to zero (the offsets are the linearisation
+# at the probe means); the result does not depend on the replay batch width;
+# the reported MCSE is of the size of the estimator's spread across seeds.
+# The copy pullback coefficients (numerical_mc.rs:137-145) have no
+# reference vector: parity unpinned beyond these invariants.
+def test_observation_numerical_scores_are_centred(oracle):
+    cols = _panel(3)
+    p = oracle.Problem(*cols, observation=True)
+    r = p.run(probes=64, leverage_width=8, target_width=8, probe_tolerance=1e-11, fit_tolerance=1e-11,
+              numerical=True)
+    n = r["numerical"]
+    assert n["status"] == 2 and n["certified_replay_rhs"] == 64
+    scale = np.sqrt(np.abs(np.diag(np.asarray(n["leverage"]).reshape(3, 3))).max() * 64 * 63)
+    assert np.max(np.abs(np.asarray(n["score_means"]))) <= 1e-9 * scale
+    # one replay Counter-V1 word per class word and column
+    assert n["replay_generator_words"] % 64 == 0 and n["replay_generator_words"] > 0
+    q = p.run(probes=64, leverage_width=1, target_width=8, probe_tolerance=1e-11, fit_tolerance=1e-11,
+              numerical=True)["numerical"]
+    for k in ("conditional", "leverage", "usable", "mcse"):
+        assert np.allclose(np.asarray(q[k]), np.asarray(n[k]), rtol=1e-9, atol=1e-18), k
+    assert q["replay_generator_words"] == n["replay_generator_words"]
+
+
+def test_observation_numerical_mcse_has_the_spread_of_the_estimator(oracle):
+    cols = _panel(3)
+    p = oracle.Problem(*cols, observation=True)
+    est, mcse = [], []
+    for seed in range(24):
+        r = p.run(seed=2000 + seed, probes=64, leverage_width=8, target_width=8, probe_tolerance=1e-11,
+                  fit_tolerance=1e-11, numerical=True)
+        est.append(r["corrected"][:3])
+        mcse.append(r["numerical"]["mcse"][:3])
+    spread, reported = np.std(est, axis=0, ddof=1), np.mean(mcse, axis=0)
+    assert np.all(reported > 0.5 * spread) and np.all(reported < 2.0 * spread), (reported, spread)
