@@ -573,6 +573,7 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
   uint32_t status = num.status;
   if (status == 2) {
     E.nscores.alloc(6 * 64);
+    E.noscores.alloc(6 * 64);
     for (uint64_t first = 0; first < R; first += lw) {
       const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
       hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
@@ -597,17 +598,23 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
           num.replay_error_code = (uint32_t)e.code;
           return;
         }
-        const unsigned g = reduce_grid(E.G * K);
-        if (E.Go)  // observation units (generic_jla/numerical.rs:617-641)
+        // observation units (generic_jla/numerical.rs:617-641), then the match
+        // units (engine/numerical.rs:246-291); the two sums are added per probe
+        if (E.Go) {
+          const unsigned g = reduce_grid(E.Go * K);
           hipLaunchKernelGGL((dev::k_obs_scores<K>), g, kBlock, 0, E.stream, E.Go, k, first, opt.seed, kTagLeverage,
                              E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm, E.unit_rank.p + E.Gm,
                              E.unit_offset.p + E.Gm, E.row_phys.p, E.obs_physical, E.cell_worker.p, E.cell_firm.p,
-                             E.worker.p, E.fsol.p, E.ngrad.p, E.ncopy.p, E.nfolds.p, E.rpartial.p);
-        else
-          hipLaunchKernelGGL((dev::k_numerical_scores<K>), g, kBlock, 0, E.stream, E.G, k, E.unit_cell.p,
+                             E.worker.p, E.fsol.p, E.nograd.p, E.ncopy.p, E.nofolds.p, E.rpartial.p);
+          finish<K, 6>(E, E.rpartial.p, g, dev::FinScores{E.noscores.p});
+        }
+        if (E.Gm) {
+          const unsigned g = reduce_grid(E.Gm * K);
+          hipLaunchKernelGGL((dev::k_numerical_scores<K>), g, kBlock, 0, E.stream, E.Gm, k, E.unit_cell.p,
                              E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
                              E.ngrad.p, E.noffs.p, E.nfolds.p, E.rpartial.p);
-        finish<K, 6>(E, E.rpartial.p, g, dev::FinScores{E.nscores.p});
+          finish<K, 6>(E, E.rpartial.p, g, dev::FinScores{E.nscores.p});
+        }
       });
       for (uint64_t g = 0; g < E.Gm; ++g) num.replay_generator_words += (E.P->unit_count[g] + 63) / 64 * k;
       num.replay_generator_words += E.obs_class_words * k;  // generic_jla/numerical.rs:490-519
@@ -615,8 +622,12 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
         status = 7;
         break;
       }
-      std::vector<double> s;
-      E.nscores.download(s, 6 * (size_t)KL, E.stream);
+      std::vector<double> s(6 * (size_t)KL, 0.0), so;
+      if (E.Gm) E.nscores.download(s, 6 * (size_t)KL, E.stream);
+      if (E.Go) {
+        E.noscores.download(so, 6 * (size_t)KL, E.stream);
+        for (size_t q = 0; q < s.size(); ++q) s[q] = E.Gm ? so[q] + s[q] : so[q];
+      }
       for (uint32_t c = 0; c < k; ++c) {
         const uint64_t probe = first + c;
         for (int f = 0; f < 2; ++f)
@@ -789,8 +800,6 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   if (opt.probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
-  if (E.Go && E.Gm && opt.numerical)
-    fail(kUnsupported, "numerical_solve", "no numerical attachment with stayers");
   const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
   const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
   const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
@@ -909,6 +918,34 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
 
   // ---- numerical attachment state (engine/numerical.rs:62-122)
   NumericalResult &num = res.numerical;
+  if (opt.numerical && E.Gm) {
+    res.has_numerical = true;
+    E.ngrad.alloc(5 * E.Gm);
+    E.noffs.alloc(E.Gm);
+    E.nfolds.alloc(12 * E.Gm);
+    VK_HIP(hipMemsetAsync(E.nfolds.p, 0, 12 * E.Gm * sizeof(double), E.stream));
+    const unsigned sg = reduce_grid(E.Gm);
+    E.npart.alloc(5 * (size_t)sg);
+    hipLaunchKernelGGL(dev::k_numerical_state, sg, kBlock, 0, E.stream, E.Gm, E.moments.p, (double)R, E.ngrad.p,
+                       E.noffs.p, E.npart.p);
+    std::vector<double> part;
+    E.npart.download(part, 5 * (size_t)sg, E.stream);
+    num.minimum_constrained = INFINITY;
+    num.minimum_residual_margin = INFINITY;
+    num.maximum_sensitivity_ratio = 0.0;
+    double last = -1.0;
+    bool clipped = false;
+    for (unsigned b = 0; b < sg; ++b) {
+      num.minimum_constrained = std::min(num.minimum_constrained, part[5 * b]);
+      num.minimum_residual_margin = std::min(num.minimum_residual_margin, part[5 * b + 1]);
+      num.maximum_sensitivity_ratio = std::max(num.maximum_sensitivity_ratio, part[5 * b + 2]);
+      last = std::max(last, part[5 * b + 3]);
+      clipped = clipped || part[5 * b + 4] != 0.0;
+    }
+    // the last unit with an event sets the status (numerical.rs:88-120)
+    num.status = last >= 0.0 ? (uint32_t)((uint64_t)last % 8) : (clipped ? 5u : 2u);
+    num.allocation_bytes = (5 + 1 + 12) * E.Gm * sizeof(double) + 6 * R * sizeof(double);
+  }
   if (opt.numerical && E.Go) {
     // observation units (generic_jla/numerical.rs:172-270): per copy the
     // finite derivative of its inverse, pulled back to the row's p^2, p^4 and
@@ -934,9 +971,11 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       k = q;
     }
     std::vector<double> grad(3 * Go, 0.0), copy(2 * N, 0.0);
-    num.minimum_constrained = INFINITY;
-    num.minimum_residual_margin = INFINITY;
-    num.maximum_sensitivity_ratio = 0.0;
+    if (!E.Gm) {
+      num.minimum_constrained = INFINITY;
+      num.minimum_residual_margin = INFINITY;
+      num.maximum_sensitivity_ratio = 0.0;
+    }
     bool clipped = false;
     uint32_t event = 0;
     const double r = (double)R;
@@ -981,41 +1020,17 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       grad[Go + i] = g4.value();
       grad[2 * Go + i] = off.value();
     }
-    // the adjustment's variance clipping first (generic_jla/numerical.rs:172-176), then the last event
-    num.status = event ? event : (clipped ? 5u : 2u);
-    E.ngrad.upload(grad);
+    // after the match units' state: the observation adjustment's variance
+    // clipping (generic_jla/numerical.rs:172-176), then the last event
+    const uint32_t base = E.Gm ? num.status : 2u;
+    num.status = event ? event : (clipped ? 5u : base);
+    E.nograd.upload(grad);
     E.ncopy.upload(copy);
-    E.nfolds.alloc(12 * Go);
-    VK_HIP(hipMemsetAsync(E.nfolds.p, 0, 12 * Go * sizeof(double), E.stream));
-    num.allocation_bytes = (3 * Go + 2 * N + 12 * Go) * sizeof(double) + 6 * R * sizeof(double);
-  } else if (opt.numerical) {
-    res.has_numerical = true;
-    E.ngrad.alloc(5 * E.G);
-    E.noffs.alloc(E.G);
-    E.nfolds.alloc(12 * E.G);
-    VK_HIP(hipMemsetAsync(E.nfolds.p, 0, 12 * E.G * sizeof(double), E.stream));
-    const unsigned sg = reduce_grid(E.G);
-    E.npart.alloc(5 * (size_t)sg);
-    hipLaunchKernelGGL(dev::k_numerical_state, sg, kBlock, 0, E.stream, E.G, E.moments.p, (double)R, E.ngrad.p,
-                       E.noffs.p, E.npart.p);
-    std::vector<double> part;
-    E.npart.download(part, 5 * (size_t)sg, E.stream);
-    num.minimum_constrained = INFINITY;
-    num.minimum_residual_margin = INFINITY;
-    num.maximum_sensitivity_ratio = 0.0;
-    double last = -1.0;
-    bool clipped = false;
-    for (unsigned b = 0; b < sg; ++b) {
-      num.minimum_constrained = std::min(num.minimum_constrained, part[5 * b]);
-      num.minimum_residual_margin = std::min(num.minimum_residual_margin, part[5 * b + 1]);
-      num.maximum_sensitivity_ratio = std::max(num.maximum_sensitivity_ratio, part[5 * b + 2]);
-      last = std::max(last, part[5 * b + 3]);
-      clipped = clipped || part[5 * b + 4] != 0.0;
-    }
-    // the last unit with an event sets the status (numerical.rs:88-120)
-    num.status = last >= 0.0 ? (uint32_t)((uint64_t)last % 8) : (clipped ? 5u : 2u);
-    num.allocation_bytes = (5 + 1 + 12) * E.G * sizeof(double) + 6 * R * sizeof(double);
+    E.nofolds.alloc(12 * Go);
+    VK_HIP(hipMemsetAsync(E.nofolds.p, 0, 12 * Go * sizeof(double), E.stream));
+    num.allocation_bytes += (3 * Go + 2 * N + 12 * Go) * sizeof(double) + (E.Gm ? 0 : 6 * R * sizeof(double));
   }
+
 
   // ---- target probes (engine.rs:1373-1484)
   auto tt = std::chrono::steady_clock::now();
@@ -1050,10 +1065,10 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       if (opt.numerical && E.Go)  // observation folds: y e kappa / fold size (generic_jla/numerical.rs:377-389)
         hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.Go), kBlock, 0, E.stream, E.Go, k, first, R,
                            E.unit_cell.p + E.Gm, E.cell_worker.p, E.cell_firm.p, E.row_outcome.p,
-                           E.obs_adjust.p + 4 * E.Go, E.worker.p, E.fsol.p, E.nfolds.p);
-      else if (opt.numerical)
-        hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.G), kBlock, 0, E.stream, E.G, k, first, R,
-                           E.unit_cell.p, E.cell_worker.p, E.cell_firm.p, E.unit_outcome.p, E.adjust.p + 4 * E.G,
+                           E.obs_adjust.p + 4 * E.Go, E.worker.p, E.fsol.p, E.nofolds.p);
+      if (opt.numerical && E.Gm)
+        hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.Gm), kBlock, 0, E.stream, E.Gm, k, first, R,
+                           E.unit_cell.p, E.cell_worker.p, E.cell_firm.p, E.unit_outcome.p, E.adjust.p + 4 * E.Gm,
                            E.worker.p, E.fsol.p, E.nfolds.p);
       hipLaunchKernelGGL((dev::k_contract<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_worker.p, E.cell_firm.p,
                          E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p);

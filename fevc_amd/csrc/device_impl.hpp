@@ -223,7 +223,7 @@ struct DeviceEngine::Impl {
   DBuf<double> row_outcome, obs_m, obs_corr, obs_adjust, contrib;
   // numerical attachment of observation units: copy pullbacks (q1, q3 per
   // copy) and the Counter-V1 words one replay column reads (over classes)
-  DBuf<double> ncopy;
+  DBuf<double> ncopy, nograd, nofolds, noscores;
   uint64_t obs_class_words = 0;
   std::vector<DevLevel> levels;
   DBuf<uint32_t> perm0, pos0;  // level-0 renumbering

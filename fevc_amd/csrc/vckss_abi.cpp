@@ -284,8 +284,6 @@ void preflight_point(VckssEngineSolveRequestV4 point, uint32_t tolerance_supplie
   if (point.v3.engine == VCKSS_ENGINE_COMPRESSED) point.v3.engine = VCKSS_ENGINE_AUTO_OR_UNSPECIFIED;
   Context probe;
   (void)point_options(point, tolerance_supplied, probe);
-  if (point.v3.stayers_mode == VCKSS_STAYERS_ALL)
-    fail(kUnsupported, "numerical_preflight", "the MI355X engine attaches numerical MCSE without stayers only");
 }
 
 std::vector<double> builtin_smoother() {
@@ -694,8 +692,6 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
-  if (ctx.P.stayers)
-    fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE without stayers only");
   o.numerical = true;
   run_options(ctx, o, poll, poll_ctx);
 }

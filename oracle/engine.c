@@ -665,6 +665,8 @@ typedef struct {
     uint64_t *order;
     const uint64_t *phys;
     uint64_t class_words;
+    double *ograd;  /* 3 per observation unit: g2, g4, offset */
+    ssum *ofolds;   /* 6 per observation unit */
 } num_state;
 
 static int leverage_atoms(const orc_problem *p, uint64_t seed, uint64_t first, uint64_t k, int64_t *atoms);
@@ -743,14 +745,14 @@ static int numerical_replay(const orc_problem *p, const orc_engine_options *opt,
                         ss_add(&w3, ns->copy[2 * (ns->phys[i] + j) + 1] * q);
                     }
                     ssum alpha = {0, 0};
-                    ss_add(&alpha, ns->grad[3 * i] * pp * pp);
-                    ss_add(&alpha, ns->grad[3 * i + 1] * pp * pp * pp * pp);
-                    ss_add(&alpha, -ns->grad[3 * i + 2]);
+                    ss_add(&alpha, ns->ograd[3 * i] * pp * pp);
+                    ss_add(&alpha, ns->ograd[3 * i + 1] * pp * pp * pp * pp);
+                    ss_add(&alpha, -ns->ograd[3 * i + 2]);
                     ss_add(&alpha, ss_fin(w1) * pp);
                     ss_add(&alpha, ss_fin(w3) * pp * pp * pp);
                     double al = ss_fin(alpha);
                     for (int f = 0; f < 2; ++f)
-                        for (int j = 0; j < 3; ++j) ss_add(&sc[f][j], -ss_fin(ns->folds[6 * i + 3 * f + j]) * al);
+                        for (int j = 0; j < 3; ++j) ss_add(&sc[f][j], -ss_fin(ns->ofolds[6 * i + 3 * f + j]) * al);
                 }
                 for (uint64_t g = 0; g < Gm; ++g) {
                     uint32_t cell = p->unit_cell[g];
@@ -1039,10 +1041,6 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     /* leverage loop: engine.rs:1150-1246; observation deletion:
      * generic_jla.rs:5540-5892 */
     const uint64_t Gm = p->match_units, Go = G - Gm;
-    if (Go && Gm && num) {
-        rc = orc_fail(ORC_UNSUPPORTED, "numerical", "no numerical attachment with stayers");
-        goto fail;
-    }
     if (Go) observation_state_init(p, &obs);
     mom = orc_alloc(G, sizeof *mom);
     res->lev_iters = orc_alloc(R, 8);
@@ -1074,7 +1072,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         }
     }
     /* leverage adjustment: engine.rs:2269-2410 (outcome mean 0, centering None) */
-    int clipped = 0;
+    int clipped = 0, obs_clipped = 0;
     {
         double probes = (double)R;
         res->projection_share = orc_alloc(G, 8); res->residual_share = orc_alloc(G, 8);
@@ -1128,7 +1126,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             res->deleted_mass[g] = deleted;
         }
         /* observation units: generic_jla.rs:5893-6017 */
-        if (Go && (rc = observation_adjustment(p, &obs, &opt, res, cw, &clipped))) {
+        if (Go && (rc = observation_adjustment(p, &obs, &opt, res, cw, &obs_clipped))) {
             free(cw);
             goto fail_mom;
         }
@@ -1136,15 +1134,57 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         for (uint64_t c = 0; c < C; ++c) res->cell_weight[c] = ss_fin(cw[c]);
         free(cw);
     }
-    /* numerical state of observation units: generic_jla/numerical.rs:172-270 */
-    if (num && Go) {
+    /* numerical state: engine/numerical.rs:62-122 (after the adjustment) */
+    if (num && Gm) {
+        ns.grad = orc_alloc(5 * G, 8);
+        ns.offs = orc_alloc(G, 8);
+        ns.folds = orc_alloc(6 * G, sizeof(ssum));
         ns.status = clipped ? 5 : 2;
         num->minimum_constrained = INFINITY;
         num->minimum_residual_margin = INFINITY;
         num->maximum_sensitivity_ratio = 0.0;
-        ns.grad = orc_alloc(3 * Go, 8);
+        double r = (double)R;
+        for (uint64_t g = 0; g < Gm; ++g) {
+            double u[5] = {ss_fin(mom[g].proj) / r, ss_fin(mom[g].res) / r, ss_fin(mom[g].proj4) / r,
+                           ss_fin(mom[g].res4) / r, ss_fin(mom[g].mixed) / r};
+            double fd[20], gr[5];
+            if (!orc_finite_derivative(u, (uint32_t)R, fd)) {
+                ns.status = 6;
+                continue;
+            }
+            num->minimum_constrained = fmin(num->minimum_constrained, u[0] + u[1]);
+            num->minimum_residual_margin = fmin(num->minimum_residual_margin, fd[1]);
+            num->maximum_sensitivity_ratio = fmax(num->maximum_sensitivity_ratio, sqrt(fmax(fd[3], 0.0)) / fd[1]);
+            if (fd[19] != 0.0) {
+                ns.status = 5;
+                continue;
+            }
+            if (!orc_fd_observation(fd, 0.0, gr)) {
+                ns.status = 6;
+                continue;
+            }
+            ssum off = {0, 0};
+            for (int j = 0; j < 5; ++j) {
+                ns.grad[5 * g + j] = gr[j];
+                ss_add(&off, gr[j] * u[j]);
+            }
+            ns.offs[g] = ss_fin(off);
+        }
+    }
+    /* numerical state of observation units: generic_jla/numerical.rs:172-270 */
+    if (num && Go) {
+        /* after the match units' state (generic_jla.rs:6053-6234 runs first):
+         * the observation adjustment's clipping, then its events */
+        if (!Gm) {
+            ns.status = 2;
+            num->minimum_constrained = INFINITY;
+            num->minimum_residual_margin = INFINITY;
+            num->maximum_sensitivity_ratio = 0.0;
+        }
+        if (obs_clipped) ns.status = 5;
+        ns.ograd = orc_alloc(3 * Go, 8);
         ns.copy = orc_alloc(2 * obs.phys[Go], 8);
-        ns.folds = orc_alloc(6 * Go, sizeof(ssum));
+        ns.ofolds = orc_alloc(6 * Go, sizeof(ssum));
         ns.phys = obs.phys;
         ns.order = observation_order(p);
         for (uint64_t k = 0; k < Go; ++k)
@@ -1192,46 +1232,9 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             }
             ss_add(&off, ss_fin(g2) * a2);
             ss_add(&off, ss_fin(g4) * a4);
-            ns.grad[3 * i] = ss_fin(g2);
-            ns.grad[3 * i + 1] = ss_fin(g4);
-            ns.grad[3 * i + 2] = ss_fin(off);
-        }
-    }
-    /* numerical state: engine/numerical.rs:62-122 (after the adjustment) */
-    if (num && !Go) {
-        ns.grad = orc_alloc(5 * G, 8);
-        ns.offs = orc_alloc(G, 8);
-        ns.folds = orc_alloc(6 * G, sizeof(ssum));
-        ns.status = clipped ? 5 : 2;
-        num->minimum_constrained = INFINITY;
-        num->minimum_residual_margin = INFINITY;
-        num->maximum_sensitivity_ratio = 0.0;
-        double r = (double)R;
-        for (uint64_t g = 0; g < G; ++g) {
-            double u[5] = {ss_fin(mom[g].proj) / r, ss_fin(mom[g].res) / r, ss_fin(mom[g].proj4) / r,
-                           ss_fin(mom[g].res4) / r, ss_fin(mom[g].mixed) / r};
-            double fd[20], gr[5];
-            if (!orc_finite_derivative(u, (uint32_t)R, fd)) {
-                ns.status = 6;
-                continue;
-            }
-            num->minimum_constrained = fmin(num->minimum_constrained, u[0] + u[1]);
-            num->minimum_residual_margin = fmin(num->minimum_residual_margin, fd[1]);
-            num->maximum_sensitivity_ratio = fmax(num->maximum_sensitivity_ratio, sqrt(fmax(fd[3], 0.0)) / fd[1]);
-            if (fd[19] != 0.0) {
-                ns.status = 5;
-                continue;
-            }
-            if (!orc_fd_observation(fd, 0.0, gr)) {
-                ns.status = 6;
-                continue;
-            }
-            ssum off = {0, 0};
-            for (int j = 0; j < 5; ++j) {
-                ns.grad[5 * g + j] = gr[j];
-                ss_add(&off, gr[j] * u[j]);
-            }
-            ns.offs[g] = ss_fin(off);
+            ns.ograd[3 * i] = ss_fin(g2);
+            ns.ograd[3 * i + 1] = ss_fin(g4);
+            ns.ograd[3 * i + 2] = ss_fin(off);
         }
     }
     free(mom);
@@ -1281,7 +1284,7 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
                     double sw = a->worker[w] + a->firm[f], sf = b->worker[w] + b->firm[f];
                     double kappa[3] = {sw * sw, sf * sf, sw * sf};
                     for (int j = 0; j < 3; ++j)
-                        ss_add(&ns.folds[6 * i + 3 * fold + j], p->outcome[row] * res->residual_mass[g] * kappa[j] / count);
+                        ss_add(&ns.ofolds[6 * i + 3 * fold + j], p->outcome[row] * res->residual_mass[g] * kappa[j] / count);
                 }
                 for (uint64_t g = 0; g < Gm; ++g) {
                     uint32_t cell = p->unit_cell[g];
@@ -1316,8 +1319,9 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         num->point_rhs = 1 + 3 * R;
         num->max_point_complete_residual = max_complete;
         rc = numerical_replay(p, &opt, &s, &ns, res->draws, num);
-        free(ns.grad); free(ns.offs); free(ns.folds); free(ns.copy); free(ns.order);
-        ns.grad = NULL; ns.offs = NULL; ns.folds = NULL; ns.copy = NULL; ns.order = NULL;
+        free(ns.grad); free(ns.offs); free(ns.folds); free(ns.copy); free(ns.order); free(ns.ograd); free(ns.ofolds);
+        ns.grad = NULL; ns.offs = NULL; ns.folds = NULL; ns.copy = NULL; ns.order = NULL; ns.ograd = NULL;
+        ns.ofolds = NULL;
         if (rc) goto fail;
     }
     res->sum.solve_seconds = now_seconds() - t0;
@@ -1335,6 +1339,8 @@ fail:
     observation_state_free(&obs);
     free(ns.copy);
     free(ns.order);
+    free(ns.ograd);
+    free(ns.ofolds);
     free(ns.grad);
     free(ns.offs);
     free(ns.folds);

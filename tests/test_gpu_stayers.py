@@ -112,11 +112,20 @@ def test_zero_stayers_reproduce_movers():
         assert np.array_equal(np.asarray(ra[k]), np.asarray(rb[k])), k
 
 
-def test_hybrid_refuses_numerical_attachment():
-    e = _engine()
+def test_hybrid_numerical_matches_oracle(oracle):
+    """Numerical MCSE of the hybrid (mover blocks and stayer copies) against the
+    oracle, with the bars of tests/test_gpu_numerical.py."""
+    from test_gpu_numerical import _compare
+    R, tol = 24, 1e-11
+    h = oracle.Problem(*panel()).with_stayers(*stayers())
+    want = h.run(probes=R, leverage_width=4, target_width=4, fit_tolerance=1e-10, probe_tolerance=tol,
+                 numerical=True)
+    e = _engine(64, (64, 32))
     try:
-        with pytest.raises(F.EngineError) as ei:
-            e.solve_numerical(probes=8, tolerance=1e-8)
-        assert ei.value.code == 11
+        e.smoother(0)
+        res, got = e.solve_numerical(probes=R, tolerance=tol)
     finally:
         e.release()
+    _compare(want, got, 1e-6)
+    for key in ("plugin", "correction", "corrected"):
+        assert np.allclose(res[key], want[key], rtol=1e-8, atol=0)

@@ -86,3 +86,22 @@ def test_zero_stayers_reproduce_the_mover_result(oracle):
     b = h.run(probes=16, probe_tolerance=1e-10)
     for k in ("plugin", "correction", "corrected", "mcse"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_hybrid_numerical_scores_are_centred(oracle):
+    """Numerical MCSE of the hybrid: mover blocks (whose generic linearisation,
+    generic_jla/numerical.rs:300-347 with k = 1/(1 - h), equals the compressed
+    unit linearisation) plus stayer copies; the replay reproduces the leverage
+    phase, so every fold's scores average to zero, at any replay batch width."""
+    cols = hybrid_panel(5)
+    p = oracle.Problem(*cols)
+    st = synth.eligible_stayers(cols[0], cols[1], cols[3], cols[4], cols[5], p.mask())
+    h = p.with_stayers(*st)
+    runs = [h.run(probes=48, leverage_width=w, target_width=8, probe_tolerance=1e-11, fit_tolerance=1e-11,
+                  numerical=True)["numerical"] for w in (8, 1)]
+    n = runs[0]
+    assert n["status"] == 2 and n["certified_replay_rhs"] == 48
+    scale = np.sqrt(np.abs(np.diag(np.asarray(n["leverage"]).reshape(3, 3))).max() * 48 * 47)
+    assert np.max(np.abs(np.asarray(n["score_means"]))) <= 1e-9 * scale
+    for k in ("conditional", "leverage", "usable", "mcse"):
+        assert np.allclose(np.asarray(runs[1][k]), np.asarray(n[k]), rtol=1e-9, atol=1e-18), k
