@@ -189,6 +189,27 @@ class StayerReceiptV1(C.Structure):
                                            "total_prepared_resident_bytes")])
 
 
+class _Vec4(C.Structure):
+    _fields_ = [("worker", C.c_double), ("firm", C.c_double), ("covariance", C.c_double), ("total", C.c_double)]
+
+
+class StayerHybridResultV1(C.Structure):
+    _fields_ = ([("struct_size", C.c_uint32), ("schema_version", C.c_uint32), ("generation", C.c_uint64)] +
+                [(k, _Vec4) for k in ("plugin", "correction", "corrected", "mover_correction",
+                                      "stayer_correction")] +
+                [("weighted_rss", C.c_double)] +
+                [(k, C.c_uint64) for k in ("parameters", "full_parameters", "correction_parameters",
+                                           "deletion_units")] +
+                [(k, C.c_double) for k in ("max_leverage", "information_rcond", "inverse_relres",
+                                           "inverse_original_relres", "inverse_sqrt_relres", "maker_relres",
+                                           "full_fit_relres", "working_fit_relres", "fit_residual_tolerance",
+                                           "control_basis_relres", "control_basis_forward_error",
+                                           "deletion_rank_gap", "firm_zero_sum_residual")] +
+                [(k, C.c_uint64) for k in ("peak_forecast_bytes", "fit_peak_forecast_bytes",
+                                           "correction_peak_forecast_bytes", "topology_checksum")] +
+                [("accounting_residual", C.c_double)])
+
+
 SIGNATURES = [
     ("vckss_rust_abi_version", C.c_uint32, []),
     ("vckss_rust_backend_version", C.c_char_p, []),
@@ -210,6 +231,8 @@ SIGNATURES = [
      [C.c_uint64, C.POINTER(StayerRequestInterruptV1), C.POINTER(StayerColumnsV1)]),
     ("vckss_rust_engine_stayer_augmentation_receipt_v1", C.c_int32,
      [C.c_uint64, C.POINTER(StayerReceiptV1), C.c_uint32]),
+    ("vckss_rust_engine_stayer_hybrid_result_v1", C.c_int32,
+     [C.c_uint64, C.POINTER(StayerHybridResultV1), C.c_uint32]),
     ("vckss_rust_engine_solve_v5", C.c_int32, [C.c_uint64, C.POINTER(SolveRequestV5)]),
     ("vckss_rust_engine_solve_interrupt_v5", C.c_int32,
      [C.c_uint64, C.POINTER(SolveRequestInterruptV5)]),
@@ -403,6 +426,16 @@ class Engine:
         o = StayerReceiptV1()
         _check(library().vckss_rust_engine_stayer_augmentation_receipt_v1(self.generation, C.byref(o), C.sizeof(o)))
         return {name: getattr(o, name) for name, _ in o._fields_}
+
+    def stayer_hybrid_result(self):
+        """The exact stayer hybrid's result with its mover and stayer corrections."""
+        o = StayerHybridResultV1()
+        _check(library().vckss_rust_engine_stayer_hybrid_result_v1(self.generation, C.byref(o), C.sizeof(o)))
+        out = {}
+        for name, _ in o._fields_:
+            v = getattr(o, name)
+            out[name] = np.array([v.worker, v.firm, v.covariance, v.total]) if isinstance(v, _Vec4) else v
+        return out
 
     def mask(self):
         m = np.zeros(self.dims["input_rows"], np.uint8)

@@ -34,6 +34,9 @@ struct ExactReceipt {
   double max_leverage = 0, deletion_rank_gap = 0, information_rcond = 0, inverse_relres = 0,
          fit_relres = 0, fit_tolerance = 0, firm_zero_sum_residual = 0;
   double seconds_factor = 0, seconds_cells = 0;
+  // the correction's match-block and physical-copy parts (the stayer hybrid's
+  // mover and stayer corrections, ffi_engine.rs:8772-8800)
+  double mover_correction[4] = {0, 0, 0, 0}, stayer_correction[4] = {0, 0, 0, 0};
 };
 
 // Numerical MCSE attachment (engine/numerical.rs:13-32, numerical_api.rs:54-90).

@@ -559,7 +559,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   std::vector<double> cell(4 * (uint64_t)C);
   VK_HIP(hipMemcpyAsync(cell.data(), dcell.p, cell.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   VK_HIP(hipStreamSynchronize(st));
-  Neumaier corr[3];
+  Neumaier corr[3], corr_obs[3], corr_match[3];
   double max_leverage = 0.0, gap = std::numeric_limits<double>::infinity();
   // units [0, match_units) are match blocks; the observation units are the
   // last stored rows (all rows under observation deletion, the stayers in a
@@ -580,6 +580,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
         fail(kNonestimable, "exact_estimator", "physical observation deletion has leverage at or above one");
       const double e = P.outcome[r] - beta[P.cell_worker[c]] - beta[W + P.cell_firm[c]];
       const double scale = (double)P.frequency[r] * P.outcome[r] * e / maker;
+      for (int k = 0; k < 3; ++k) corr_obs[k].add(scale * cell[4 * (uint64_t)c + 1 + k]);
       for (int k = 0; k < 3; ++k) corr[k].add(scale * cell[4 * (uint64_t)c + 1 + k]);
       max_leverage = std::max(max_leverage, h);
       gap = std::min(gap, maker);
@@ -595,11 +596,20 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
     const double fy = P.unit_outcome[g];
     const double fe = fy - n * (beta[P.cell_worker[c]] + beta[W + P.cell_firm[c]]);
     const double scale = fy * fe / maker;
+    for (int k = 0; k < 3; ++k) corr_match[k].add(scale * cell[4 * (uint64_t)c + 1 + k]);
     for (int k = 0; k < 3; ++k) corr[k].add(scale * cell[4 * (uint64_t)c + 1 + k]);
     max_leverage = std::max(max_leverage, h * n);
     gap = std::min(gap, maker);
   }
   for (int k = 0; k < 3; ++k) res.correction[k] = corr[k].value();
+  for (int k = 0; k < 3; ++k) {
+    res.exact.mover_correction[k] = corr_match[k].value();
+    res.exact.stayer_correction[k] = corr_obs[k].value();
+  }
+  res.exact.mover_correction[3] =
+      res.exact.mover_correction[0] + res.exact.mover_correction[1] + 2.0 * res.exact.mover_correction[2];
+  res.exact.stayer_correction[3] =
+      res.exact.stayer_correction[0] + res.exact.stayer_correction[1] + 2.0 * res.exact.stayer_correction[2];
   res.correction[3] = res.correction[0] + res.correction[1] + 2.0 * res.correction[2];
   for (int k = 0; k < 4; ++k) {
     if (!std::isfinite(res.correction[k]))

@@ -10,6 +10,7 @@
 //     release is idempotent (ffi_engine.rs:7146, context.rs:19-120);
 //   * requests are copied and size-checked before use; outputs carry a
 //     capacity that is checked before any write.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -585,6 +586,56 @@ int32_t vckss_rust_engine_stayer_augmentation_receipt_v1(uint64_t generation, Vc
     o.combined_target_mass = s.combined_target_mass;
     o.topology_checksum = s.topology_checksum;
     o.caller_copy_bytes = ctx.stayer_copy_bytes;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ffi_engine.rs:6122-6145, 8772-8840: after an exact solve of a stayer hybrid
+int32_t vckss_rust_engine_stayer_hybrid_result_v1(uint64_t generation, VckssStayerHybridResultV1 *output,
+                                                  uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssStayerHybridResultV1>(output, output_capacity_bytes, "stayer hybrid result");
+    Context &ctx = live(generation);
+    if (!ctx.has_result) fail(kInvalidInput, "engine_result", "the generation has no solved result");
+    const SolveResult &r = ctx.result;
+    if (!r.has_exact || !ctx.P.stayers)
+      fail(kUnsupported, "engine_stayer_hybrid_result", "the solved generation has no stayer-hybrid result");
+    const ExactReceipt &x = r.exact;
+    double source = 0.0;
+    for (int k = 0; k < 4; ++k)
+      source = std::max(source, std::fabs(x.mover_correction[k] + x.stayer_correction[k] - r.correction[k]));
+    if (source > 1e-10)
+      fail(kInvariant, "engine_stayer_hybrid_result",
+           "mixed-deletion source corrections do not sum to the total correction");
+    auto vec = [](const double (&v)[4]) {
+      VckssComponentVectorV1 c;
+      c.worker = v[0]; c.firm = v[1]; c.covariance = v[2]; c.total = v[3];
+      return c;
+    };
+    auto identity = [](const double (&v)[4]) { return std::fabs(v[3] - (v[0] + v[1] + 2.0 * v[2])); };
+    VckssStayerHybridResultV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    o.plugin = vec(r.plugin);
+    o.correction = vec(r.correction);
+    o.corrected = vec(r.corrected);
+    o.mover_correction = vec(x.mover_correction);
+    o.stayer_correction = vec(x.stayer_correction);
+    o.weighted_rss = r.weighted_rss;
+    o.parameters = o.full_parameters = o.correction_parameters = x.parameters;
+    o.deletion_units = x.deletion_units;
+    o.max_leverage = x.max_leverage;
+    o.information_rcond = x.information_rcond;
+    o.inverse_relres = o.inverse_original_relres = x.inverse_relres;
+    o.full_fit_relres = o.working_fit_relres = x.fit_relres;
+    o.fit_residual_tolerance = x.fit_tolerance;
+    o.deletion_rank_gap = x.deletion_rank_gap;
+    o.firm_zero_sum_residual = x.firm_zero_sum_residual;
+    o.topology_checksum = ctx.P.topology_checksum;
+    o.accounting_residual = std::max({identity(r.plugin), identity(r.correction), identity(r.corrected)});
     std::memcpy(output, &o, sizeof o);
   });
 }

@@ -389,6 +389,46 @@ int32_t vckss_rust_engine_augment_stayers_interrupt_v1(uint64_t generation,
 int32_t vckss_rust_engine_stayer_augmentation_receipt_v1(uint64_t generation, VckssStayerAugmentationReceiptV1 *output,
                                                          uint32_t output_capacity_bytes);
 
+/* vckss_rust.h:1054-1086 */
+typedef struct VckssStayerHybridResultV1 {
+    uint32_t struct_size;
+    uint32_t schema_version;
+    uint64_t generation;
+    VckssComponentVectorV1 plugin;
+    VckssComponentVectorV1 correction;
+    VckssComponentVectorV1 corrected;
+    VckssComponentVectorV1 mover_correction;
+    VckssComponentVectorV1 stayer_correction;
+    double weighted_rss;
+    uint64_t parameters;
+    uint64_t full_parameters;
+    uint64_t correction_parameters;
+    uint64_t deletion_units;
+    double max_leverage;
+    double information_rcond;
+    double inverse_relres;
+    double inverse_original_relres;
+    double inverse_sqrt_relres;
+    double maker_relres;
+    double full_fit_relres;
+    double working_fit_relres;
+    double fit_residual_tolerance;
+    double control_basis_relres;
+    double control_basis_forward_error;
+    double deletion_rank_gap;
+    double firm_zero_sum_residual;
+    uint64_t peak_forecast_bytes;
+    uint64_t fit_peak_forecast_bytes;
+    uint64_t correction_peak_forecast_bytes;
+    uint64_t topology_checksum;
+    double accounting_residual;
+} VckssStayerHybridResultV1;
+
+/* vckss_rust.h:1978-1982: the exact stayer hybrid's receipt with its mover
+ * and stayer corrections (ffi_engine.rs:6122-6145, 8772-8840) */
+int32_t vckss_rust_engine_stayer_hybrid_result_v1(uint64_t generation, VckssStayerHybridResultV1 *output,
+                                                  uint32_t output_capacity_bytes);
+
 /* vckss_rust.h:1928-1932 */
 int32_t vckss_rust_engine_retained_mask_v1(uint64_t generation, uint8_t *output,
                                            uint64_t output_capacity_bytes);

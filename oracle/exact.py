@@ -165,6 +165,7 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     max_leverage = 0.0
     rank_gap = np.inf
     h_unit = np.zeros(int(g.max()) + 1)
+    correction_copies = np.zeros(4)
     if observed.any():
         design_inverse = x @ inverse
         for row in np.flatnonzero(observed):
@@ -179,6 +180,7 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
             rank_gap = min(rank_gap, maker)
             scale = fr[row] * y[row] * residual[row] / maker
             correction += scale * _bilinear(z, z, W, worker_share, firm_share, cell_w, cell_f, cell_mass)
+    correction_copies[:3] = correction[:3]
     match_codes = np.unique(g[~observed])
     for code in match_codes:
         idx = np.flatnonzero((g == code) & ~observed)
@@ -206,6 +208,7 @@ def exact_estimator(worker, firm, deletion, outcome, frequency, target, *, rank_
     return dict(plugin=plugin, correction=correction, corrected=plugin - correction, beta=beta,
                 residual=residual, max_leverage=max_leverage, deletion_rank_gap=rank_gap,
                 deletion_units=int(match_codes.size + fr[observed].sum()),
+                correction_stayers=correction_copies, correction_movers=correction - correction_copies,
                 weighted_rss=float(np.sum(fr * residual ** 2)),
                 information_rcond=rcond, inverse_relres=relres, fit_relres=fit_rel,
                 unit_leverage=h_unit, workers=W, firms=F)

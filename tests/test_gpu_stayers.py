@@ -87,8 +87,15 @@ def test_hybrid_exact_matches_oracle():
     try:
         e.augment_stayers(*st)
         res, rec = e.solve_exact(exact_limit=10000)
+        split = e.stayer_hybrid_result()  # ffi_engine.rs:8772-8840
     finally:
         e.release()
+    for k, wk in (("mover_correction", "correction_movers"), ("stayer_correction", "correction_stayers")):
+        w3 = np.asarray(want[wk])[:3]
+        assert np.all(np.abs(split[k][:3] - w3) <= 1e-9 * np.abs(w3) + 1e-12), (k, split[k], w3)
+    assert np.allclose(split["mover_correction"] + split["stayer_correction"], split["correction"], rtol=0,
+                       atol=1e-10)
+    assert split["deletion_units"] == want["deletion_units"] and split["schema_version"] == 1
     for k in ("plugin", "correction", "corrected"):
         got, w_ = np.asarray(res[k], float), np.asarray(want[k])
         assert np.all(np.abs(got - w_) <= 1e-9 * np.abs(w_) + 1e-12), (k, got, w_)
