@@ -32,6 +32,10 @@ namespace dev {
 #ifndef VK_SEG_UNROLL
 #define VK_SEG_UNROLL 4
 #endif
+// the same for the FP64 lanes of the Krylov passes (level-0 products)
+#ifndef VK_SEG_UNROLL_F64
+#define VK_SEG_UNROLL_F64 VK_SEG_UNROLL
+#endif
 #ifndef VK_GRID_CAP
 #define VK_GRID_CAP 8192
 #endif
@@ -351,8 +355,13 @@ struct CsrOp {
     const auto ln = op.lane(j);
     const T xi = op.self(d.x, j, ln);
     T acc = T(0);
+    if constexpr (std::is_same_v<T, double>) {
+#pragma unroll VK_SEG_UNROLL_F64
+      for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));
+    } else {
 #pragma unroll VK_SEG_UNROLL
-    for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));
+      for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));
+    }
     return acc;
   }
   __device__ bool enabled() const { return true; }
