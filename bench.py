@@ -344,6 +344,22 @@ def main():
             he.release()
         log(f"stayer hybrid KSS ({sub.name}, {rec['stayer_workers']} stayers, 128 probes): {hw:.3f}s, "
             f"corrected {hres['corrected']}")
+        if args.cpu_solves > 0:
+            # the oracle on the host cores beside them: the same 128-probe KSS
+            # estimate (fit + 128 leverage + 256 target solves) per route
+            from oracle import oracle as O
+            for mode, make in (("observation", lambda: O.Problem(*ocols, observation=True)),
+                               ("stayers_both", lambda: O.Problem(*ocols).with_stayers(*st))):
+                prob = make()
+                t0 = time.perf_counter()
+                prob.run(probes=128, leverage_width=8, target_width=8,
+                         threads=max(1, min(args.cpu_threads, os.cpu_count() or 1)),
+                         probe_tolerance=args.tolerance)
+                cw = time.perf_counter() - t0
+                observation[mode]["cpu_oracle_kss_seconds"] = round(cw, 3)
+                observation[mode]["cpu_oracle_probe_solves_per_s"] = round(3 * 128 / cw, 2)
+                observation[mode]["gpu_vs_cpu_oracle"] = round(cw / observation[mode]["wall_seconds"], 1)
+                log(f"oracle {mode} 128-probe KSS on the host cores: {cw:.2f}s")
 
     line = {
         "metric": "probe solves/s at rel.res 1e-8 (leverage probes, learned+batched CMG-PCG)",
