@@ -89,3 +89,27 @@ def test_graph_solve_time():
             print(f"width {width}: {a * 1e3:.1f} ms without graphs, {b * 1e3:.1f} ms with ({a / b:.2f}x)", flush=True)
     finally:
         e.release()
+
+
+def test_graph_solves_with_observation_units_are_bit_identical():
+    """Observation deletion and the stayer hybrid: the captured iterations
+    replay the same kernels, whatever the deletion units."""
+    d = synth.generate(synth.PanelSpec(workers=4000, firms=300, periods=5, mobility=0.2, seed=19, name="graphs-obs"))
+    cols = [np.asarray(d[k], float) for k in COLS]
+    cols[4] = np.random.default_rng(19).integers(1, 4, len(cols[0])).astype(float)
+    for kind in ("observation", "stayers"):
+        e = F.Engine(*cols, deletion_mode="observation" if kind == "observation" else "match")
+        try:
+            if kind == "stayers":
+                e.augment_stayers(*synth.eligible_stayers(cols[0], cols[1], cols[3], cols[4], cols[5], e.mask()))
+            e.configure(32, 32, 16)
+            e.graphs(False)
+            print(f"graphs with {kind} units", flush=True)
+            plain = e.solve(probes=32, tolerance=1e-9)
+            e.graphs(True)
+            graphed = e.solve(probes=32, tolerance=1e-9)
+            c, l = e.graphs()
+            assert c > 0 and l > 0
+            _same(plain, graphed)
+        finally:
+            e.release()
