@@ -59,6 +59,29 @@ def measured_traffic(width, precision):
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
+def host_description():
+    """CPU model, nproc and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "usable_cores": usable}
+
+
+def cpu_threads(requested):
+    """Oracle threads: every core this process may use, capped at the 16-core
+    host share of one GPU, unless given explicitly."""
+    if requested > 0:
+        return requested
+    return max(1, min(16, host_description()["usable_cores"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,8 +106,10 @@ def main():
                     help="KSS estimate under observation deletion on one 40k-worker subsample (0 skips)")
     ap.add_argument("--learned-eval", type=int, default=0,
                     help="leverage batches of the held-out 10M-worker smoother evaluation (0 skips)")
-    ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves (0 skips)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-solves", type=int, default=64, help="oracle probe solves per timed run (0 skips)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="warm oracle runs; the median is reported")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0: every core this process may use, at most 16 per GPU)")
     ap.add_argument("--scale", type=float, default=1.0, help="panel size factor (1.0 = full size)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -282,21 +307,32 @@ def main():
             be.release()
 
     cpu = None
+    threads = cpu_threads(args.cpu_threads)
     if args.cpu_solves > 0:
         from oracle import oracle as O
 
         log("cpu baseline: oracle prepare")
         p = O.Problem(*cols)
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        log(f"cpu baseline: {args.cpu_solves} probe solves on {threads} threads")
-        sec, it = p.time_probe_solves(args.cpu_solves, first=0, threads=threads,
-                                      probe_tolerance=args.tolerance)
-        cpu = {"value": round(args.cpu_solves / sec, 3), "unit": "probe solves/s", "cores": threads,
+        host = host_description()
+        log(f"cpu baseline: {host['cpu_model']}, nproc {host['nproc']}, usable {host['usable_cores']}; "
+            f"1 warm-up + {args.cpu_runs} timed runs of {args.cpu_solves} probe solves on {threads} threads")
+        p.time_probe_solves(args.cpu_solves, first=0, threads=threads, probe_tolerance=args.tolerance)
+        runs = []
+        for r in range(max(1, args.cpu_runs)):
+            sec, it = p.time_probe_solves(args.cpu_solves, first=(r + 1) * args.cpu_solves, threads=threads,
+                                          probe_tolerance=args.tolerance)
+            runs.append((sec, it))
+            log(f"cpu baseline run {r + 1}: {args.cpu_solves / sec:.2f} solves/s")
+        secs = sorted(sec for sec, _ in runs)
+        med = secs[len(secs) // 2]
+        its = sum(it for _, it in runs) / (len(runs) * args.cpu_solves)
+        cpu = {"value": round(args.cpu_solves / med, 3), "unit": "probe solves/s", "cores": threads,
                "kind": "port",
-               "sample": f"{args.cpu_solves} leverage probe solves (atoms, rhs, CMG-PCG to "
-                         f"{args.tolerance:g}, extraction) of the same panel, {sec:.1f}s wall, "
-                         f"{it / args.cpu_solves:.1f} it/solve"}
-        log(f"cpu baseline: {cpu['value']:.2f} solves/s")
+               "sample": f"median of {len(runs)} warm runs (after 1 warm-up) of {args.cpu_solves} leverage probe "
+                         f"solves each (atoms, rhs, CMG-PCG to {args.tolerance:g}, extraction) of the same panel; "
+                         f"median run {med:.2f}s, {its:.1f} it/solve",
+               "runs_seconds": [round(s_, 3) for s_ in secs], **host}
+        log(f"cpu baseline: {cpu['value']:.2f} solves/s (median)")
 
     traffic = measured_traffic(args.width, args.precision)
     observation = None
@@ -352,8 +388,7 @@ def main():
                                ("stayers_both", lambda: O.Problem(*ocols).with_stayers(*st))):
                 prob = make()
                 t0 = time.perf_counter()
-                prob.run(probes=128, leverage_width=8, target_width=8,
-                         threads=max(1, min(args.cpu_threads, os.cpu_count() or 1)),
+                prob.run(probes=128, leverage_width=8, target_width=8, threads=threads,
                          probe_tolerance=args.tolerance)
                 cw = time.perf_counter() - t0
                 observation[mode]["cpu_oracle_kss_seconds"] = round(cw, 3)

@@ -254,6 +254,9 @@ SIGNATURES = [
      [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_solve_columns_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("vckss_mi355x_stream_copy_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     ("vckss_mi355x_counters_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_set_chunk_v1", C.c_int32, [C.c_uint64, C.c_uint32]),
@@ -272,6 +275,7 @@ SIGNATURES = [
     ("vckss_rust_engine_numerical_cmg_work_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_smoother_force_v1", C.c_int32, [C.c_uint64, C.c_double]),
     ("vckss_mi355x_exact_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_graphs_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -384,6 +388,7 @@ class Engine:
         self._poll = Poll(poll) if poll else None
         if self._poll:
             req.interrupt_poll = C.cast(self._poll, C.c_void_p)
+            req.checkpoint_interval = 1
         c = ColumnsV3()
         c.struct_size = C.sizeof(c)
         c.rows = rows
@@ -447,7 +452,10 @@ class Engine:
                                                    target_width))
 
     def solve(self, seed=8675309, probes=200, tolerance=None, fit_tolerance=1e-10,
-              maximum_iterations=10000, residual_replacement_interval=100, poll=None, **overrides):
+              maximum_iterations=10000, residual_replacement_interval=100, poll=None, checkpoint_interval=1,
+              **overrides):
+        """poll: optional callable returning 0 (continue) or 1 (user break),
+        called at every `checkpoint_interval`-th checkpoint of the solve."""
         req = default_solve_request()
         o = req.options
         o.algorithm = 2  # JLA
@@ -468,6 +476,7 @@ class Engine:
         cb = Poll(poll) if poll else None
         if cb:
             req.interrupt_poll = C.cast(cb, C.c_void_p)
+            req.checkpoint_interval = checkpoint_interval
         _check(library().vckss_rust_engine_solve_interrupt_v5(self.generation, C.byref(req)))
         return self.result()
 
@@ -591,6 +600,26 @@ class Engine:
                                              rhs.ctypes.data, x.ctypes.data, it.ctypes.data))
         return x, it
 
+    def solve_columns(self, worker_rhs, firm_rhs, tolerance=1e-10, inner_ratio=1.0, precision=64,
+                      maximum_iterations=10000, replacement_interval=100):
+        """One CMG_FULL_V2 batch solve of host columns (workers x k, firms x k):
+        returns worker and firm solutions, receipts per column and the
+        refinement rounds / refined columns of the call."""
+        wr = np.ascontiguousarray(np.asarray(worker_rhs, np.float64).T)
+        fr = np.ascontiguousarray(np.asarray(firm_rhs, np.float64).T)
+        k = wr.shape[0]
+        wo = np.zeros_like(wr)
+        fo = np.zeros_like(fr)
+        rec = np.zeros((k, 5))
+        ref = np.zeros(2, np.uint64)
+        _check(library().vckss_mi355x_solve_columns_v1(self.generation, k, tolerance, inner_ratio, precision,
+                                                       maximum_iterations, replacement_interval, wr.ctypes.data,
+                                                       fr.ctypes.data, wo.ctypes.data, fo.ctypes.data,
+                                                       rec.ctypes.data, ref.ctypes.data))
+        receipts = {"iterations": rec[:, 0].astype(int), "restarts": rec[:, 1].astype(int),
+                    "reduced": rec[:, 2], "complete": rec[:, 3], "rhs_norm": rec[:, 4]}
+        return wo.T, fo.T, receipts, {"rounds": int(ref[0]), "refined": int(ref[1])}
+
     def time_leverage(self, width, precision, tolerance, batches):
         sec, ms, by = C.c_double(), C.c_double(), C.c_double()
         solves, iters, launches = C.c_uint64(), C.c_uint64(), C.c_uint64()
@@ -608,6 +637,11 @@ class Engine:
         else:
             p = np.ascontiguousarray(params, np.float64)
             _check(library().vckss_mi355x_smoother_v1(self.generation, mode, p.ctypes.data, len(p)))
+
+    def smoother_force(self, omega):
+        """Constant smoother weight omega on every smoothed level, uncalibrated
+        (host-forced fallback check)."""
+        _check(library().vckss_mi355x_smoother_force_v1(self.generation, float(omega)))
 
     def smoother_status(self):
         """mode, learned weights in use (active), breakdown fallback, calibration

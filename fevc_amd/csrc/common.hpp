@@ -60,6 +60,39 @@ struct Error : std::runtime_error {
   throw Error(code, phase, msg);
 }
 
+// Cooperative interruption of a call (ffi_engine.rs:1412-1503).  The caller's
+// poll runs synchronously on the calling thread at named checkpoints, once
+// every `interval` checkpoints (the first checkpoint polls): 0 continues, 1 is
+// a user break (status 1), any other value an invariant failure (90).  A null
+// poll needs a null context and a zero interval; a poll needs a positive one.
+struct Interrupt {
+  int32_t (*poll)(void *) = nullptr;
+  void *context = nullptr;
+  uint32_t interval = 0, remaining = 0;
+  uint64_t polls = 0;
+
+  Interrupt() = default;
+  Interrupt(int32_t (*p)(void *), void *c, uint32_t every) : poll(p), context(c), interval(every) {
+    if (!poll && (context || interval))
+      fail(kInvalidInput, "engine_interrupt", "a null interrupt callback requires null context and zero interval");
+    if (poll && !interval)
+      fail(kInvalidInput, "engine_interrupt", "a nonnull interrupt callback requires a positive checkpoint interval");
+  }
+  void checkpoint(const char *phase) {
+    if (!poll) return;
+    if (remaining) {
+      --remaining;
+      return;
+    }
+    remaining = interval - 1;
+    ++polls;
+    const int32_t v = poll(context);
+    if (v == 0) return;
+    if (v == 1) fail(kUserBreak, phase, "user requested interruption");
+    fail(kInvariant, "engine_interrupt", "interrupt callback returned unsupported status " + std::to_string(v));
+  }
+};
+
 // Neumaier compensated accumulator (engine.rs:327-352, graph.rs:863-871).
 struct Neumaier {
   double sum = 0.0, corr = 0.0;

@@ -471,6 +471,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
         any = true;
       }
     if (!any) break;
+    opt.checkpoint("cmg_full_v2_refinement");
     // warm start from the lifted firm solution (full_cmg.rs:970-1088)
     if (E.aux)
       hipLaunchKernelGGL((dev::k_lift<K>), grid_for(E.aux * K), kBlock, 0, E.stream, (uint32_t)F, (uint32_t)E.aux,
@@ -575,6 +576,7 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
     E.nscores.alloc(6 * 64);
     E.noscores.alloc(6 * 64);
     for (uint64_t first = 0; first < R; first += lw) {
+      opt.checkpoint("numerical_replay_batch");
       const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
       hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
                          E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
@@ -780,7 +782,7 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.lane_a.alloc(64); E.lane_b.alloc(64); E.lane_c.alloc(64);
   E.reduced_res.alloc(64); E.complete_res.alloc(64); E.rhs_norm.alloc(64); E.sums.alloc(128);
   E.status.alloc(1);
-  VK_HIP(hipMemset(E.status.p, 0, sizeof(int)));
+  E.reset_status();
   E.totals.alloc(128); E.draws.alloc(4 * 64);
 }
 
@@ -803,13 +805,17 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   const uint32_t lw = std::max<uint32_t>(1, std::min<uint32_t>(opt.leverage_width, 64));
   const uint32_t tw = std::max<uint32_t>(1, std::min<uint32_t>(opt.target_width, 32));
   const uint32_t KL = lanes_for(lw), KT = lanes_for(tw), K2 = lanes_for(2 * tw);
+  opt.checkpoint("jla_solver_setup");
   E.ensure(std::max({KL, K2, 1u}));
+  // a solve starts with no recorded unit failure (a generation may be solved again)
+  E.reset_status();
   SolveResult res;
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t R = opt.probes;
 
   // ---- full fit (engine.rs:1108-1134)
   {
+    opt.checkpoint("jla_full_fit");
     std::vector<double> wr(E.W, 0.0), fr(E.F, 0.0);
     for (uint64_t c = 0; c < E.C; ++c) {
       wr[P.cell_worker[c]] += P.cell_outcome[c];
@@ -848,6 +854,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     VK_HIP(hipMemsetAsync(E.obs_corr.p, 0, 4 * E.obs_physical * sizeof(double), E.stream));
   }
   for (uint64_t first = 0; first < R; first += lw) {
+    opt.checkpoint("jla_leverage_batch");
     const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
                        E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
@@ -863,13 +870,13 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       if (E.Gm)
         hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.Gm * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Gm, k,
                            E.unit_cell.p, E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p,
-                           E.atoms.p, E.moments.p, E.status.p);
+                           E.atoms.p, E.moments.p, E.sink());
       if (E.Go)  // generic_jla.rs:5540-5892, units [Gm, G)
         hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.Go * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Go,
                            k, first, opt.seed, kTagLeverage, E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm,
                            E.unit_rank.p + E.Gm, E.unit_offset.p + E.Gm, E.row_phys.p, E.obs_physical,
                            E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.obs_m.p, E.obs_corr.p,
-                           E.status.p);
+                           E.sink());
     });
     for (auto &c : rc) {
       res.leverage.push_back(c);
@@ -880,17 +887,18 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   res.seconds_leverage = std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
 
   // ---- leverage adjustment (engine.rs:2269-2410)
+  opt.checkpoint("jla_leverage_adjustment");
   E.adjust.alloc(7 * std::max<uint64_t>(E.Gm, 1));
   E.cellw.alloc(E.C);
   const dev::AdjustParams aprm{(double)R, opt.block_tolerance, opt.rank_tolerance};
   if (E.Gm)
     hipLaunchKernelGGL(dev::k_adjust, grid_for(E.Gm), kBlock, 0, E.stream, E.Gm, E.moments.p, E.unit_cell.p,
-                       E.unit_count.p, E.unit_outcome.p, E.fitted.p, aprm, E.adjust.p, E.status.p);
+                       E.unit_count.p, E.unit_outcome.p, E.fitted.p, aprm, E.adjust.p, E.sink());
   if (E.Go) {  // generic_jla.rs:5893-6017
     E.obs_adjust.alloc(7 * E.Go);
     hipLaunchKernelGGL(dev::k_obs_adjust, grid_for(E.Go), kBlock, 0, E.stream, E.Go, E.obs_m.p, E.obs_corr.p,
                        E.obs_physical, E.row_phys.p, E.unit_count.p + E.Gm, E.unit_cell.p + E.Gm, E.row_outcome.p,
-                       E.fitted.p, aprm, E.obs_adjust.p, E.status.p);
+                       E.fitted.p, aprm, E.obs_adjust.p, E.sink());
   }
   E.check_status("jla_leverage_adjustment");
   // per-unit correction weight contributions, match units then observation units
@@ -1037,6 +1045,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   E.dir.alloc((uint64_t)KT * E.C);
   res.draws.assign(4 * R, 0.0);
   for (uint64_t first = 0; first < R; first += tw) {
+    opt.checkpoint("jla_target_batch");
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
     VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(int32_t), E.stream));
     hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
@@ -1058,9 +1067,9 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
                                                             E.target_total, E.frhs.p});
       const unsigned bw = reduce_grid(E.W * K2c), bf = reduce_grid(E.F * K2c);
       hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bw, kBlock, 0, E.stream, E.W, E.wrhs.p, E.rpartial.p);
-      finish<K2c, 1>(E, E.rpartial.p, bw, dev::FinBalance<K2c>{E.wrhs.p, E.W, 0, E.totals.p, E.status.p});
+      finish<K2c, 1>(E, E.rpartial.p, bw, dev::FinBalance<K2c>{E.wrhs.p, E.W, 0, E.totals.p, E.sink()});
       hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bf, kBlock, 0, E.stream, E.F, E.frhs.p, E.rpartial.p);
-      finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.status.p});
+      finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.sink()});
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
       if (opt.numerical && E.Go)  // observation folds: y e kappa / fold size (generic_jla/numerical.rs:377-389)
         hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.Go), kBlock, 0, E.stream, E.Go, k, first, R,
@@ -1086,6 +1095,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   res.seconds_target = std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
 
   // ---- finalize (engine.rs:3108-3272)
+  opt.checkpoint("jla_finalize");
   Neumaier m[4];
   for (uint64_t t = 0; t < R; ++t)
     for (int j = 0; j < 4; ++j) {
@@ -1205,6 +1215,44 @@ void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double 
   xout = from_renumbered(xn, E.levels[0].perm, K);
 }
 
+void DeviceEngine::solve_batch(const std::vector<double> &wr, const std::vector<double> &fr, uint32_t columns,
+                               double tolerance, double inner_ratio, const SolveOptions &opt,
+                               std::vector<double> &worker, std::vector<double> &firm,
+                               std::vector<ColumnReceipt> &receipts, uint64_t refine[2]) {
+  Impl &E = *impl_;
+  const uint64_t W = E.W, F = E.F;
+  if (columns == 0 || columns > 64) fail(kInvalidInput, "cmg_full_v2", "1 to 64 columns per batch");
+  if (wr.size() != W * columns || fr.size() != F * columns)
+    fail(kInvalidInput, "cmg_full_v2", "direct hybrid batch RHS arrays have incompatible dimensions");
+  const uint32_t K = lanes_for(columns);
+  E.ensure(K);
+  E.gate = E.always();
+  // column-major host blocks -> lane-interleaved device rows (unused lanes zero)
+  std::vector<double> wi(W * K, 0.0), fi(F * K, 0.0);
+  for (uint32_t j = 0; j < columns; ++j) {
+    for (uint64_t w = 0; w < W; ++w) wi[w * K + j] = wr[j * W + w];
+    for (uint64_t f = 0; f < F; ++f) fi[f * K + j] = fr[j * F + f];
+  }
+  VK_HIP(hipMemcpy(E.wrhs.p, wi.data(), wi.size() * sizeof(double), hipMemcpyHostToDevice));
+  VK_HIP(hipMemcpy(E.frhs.p, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice));
+  const uint64_t rounds0 = E.refine_rounds, refined0 = E.refined_columns;
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    solve_columns<KK>(E, columns, tolerance, inner_ratio, opt, receipts);
+  });
+  refine[0] = E.refine_rounds - rounds0;
+  refine[1] = E.refined_columns - refined0;
+  std::vector<double> wo, fo;
+  E.worker.download(wo, W * K, E.stream);
+  E.fsol.download(fo, F * K, E.stream);
+  worker.assign(W * columns, 0.0);
+  firm.assign(F * columns, 0.0);
+  for (uint32_t j = 0; j < columns; ++j) {
+    for (uint64_t w = 0; w < W; ++w) worker[j * W + w] = wo[w * K + j];
+    for (uint64_t f = 0; f < F; ++f) firm[j * F + f] = fo[f * K + j];
+  }
+}
+
 double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
                                            uint64_t &iterations) {
   Impl &E = *impl_;
@@ -1237,7 +1285,7 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
       hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
-                         E.moments.p, E.status.p);
+                         E.moments.p, E.sink());
     });
     for (auto &c : rc) iterations += c.iterations;
     solves += lw;

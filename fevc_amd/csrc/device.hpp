@@ -7,6 +7,7 @@
 #include <memory>
 #include <vector>
 
+#include "common.hpp"
 #include "host_cmg.hpp"
 #include "host_prepare.hpp"
 
@@ -26,6 +27,10 @@ struct SolveOptions {
   int precision = 64;             // 64: FP64 V-cycle; 32: FP32 V-cycle, FP64 Krylov
   bool numerical = false;         // numerical MCSE attachment (replayed leverage probes)
   bool exact = false;             // dense exact estimator instead of JLA probes
+  Interrupt *interrupt = nullptr; // caller's checkpoints (none when null)
+  void checkpoint(const char *phase) const {
+    if (interrupt) interrupt->checkpoint(phase);
+  }
 };
 
 // Receipt of the dense exact estimator (exact_estimator.rs:127-153).
@@ -95,6 +100,13 @@ class DeviceEngine {
   // Solve hybrid systems (n x columns, vertex-major) to a relative tolerance.
   void pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
            std::vector<double> &x, std::vector<ColumnReceipt> &receipts);
+  // One CMG_FULL_V2 batch solve (full_cmg.rs:809-1111) of host right-hand
+  // sides (worker W x columns and firm F x columns, column-major): the Schur
+  // right-hand side, lockstep PCG, extraction, complete-residual gate and the
+  // refinement ladder.  refine: rounds and refined columns of this call.
+  void solve_batch(const std::vector<double> &wr, const std::vector<double> &fr, uint32_t columns,
+                   double tolerance, double inner_ratio, const SolveOptions &opt, std::vector<double> &worker,
+                   std::vector<double> &firm, std::vector<ColumnReceipt> &receipts, uint64_t refine[2]);
   double time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,
                                uint64_t &iterations);
   // Timed search over probes per batch {16, 32, 64} and segment chunk sizes
@@ -132,6 +144,11 @@ class DeviceEngine {
   void smoother_mode(int mode, const std::vector<double> &theta, int precision);
   // Learned weights chosen for this engine, calibrated on the first solve.
   void smoother_default(const std::vector<double> &theta);
+  // Host-forced fallback test: every smoothed level uses the constant weight
+  // omega (omega >= 1 makes the V-cycle indefinite); no calibration, so the
+  // next solve runs with it and falls back to the reference cycle when PCG
+  // reports the loss of positive definiteness.
+  void smoother_force(double omega);
   struct SmootherStatus {
     int mode = 0;
     bool active = false, breakdown = false, shaped = false;

@@ -250,7 +250,9 @@ struct DeviceEngine::Impl {
   // numerical MCSE attachment: unit gradients (5 x G), offsets, fold sums and
   // errors (12 x G), state partials, replay scores (6 per lane)
   DBuf<double> ngrad, noffs, nfolds, npart, nscores;
-  DBuf<int> status;
+  // first per-unit gate failure (dev::StatusSink key; all ones when none)
+  DBuf<unsigned long long> status;
+  uint64_t status_launch = 0;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   double timed_bytes = 0;
@@ -302,11 +304,23 @@ struct DeviceEngine::Impl {
     seg_partial.alloc(std::max<uint64_t>(max_chunks, 1) * lanes);
   }
 
+  // a sink for one launch of a gated kernel; launches are numbered in issue order
+  dev::StatusSink sink() { return {status.p, ++status_launch}; }
+  // clears the failure word (each solve starts without a recorded failure)
+  void reset_status() {
+    VK_HIP(hipMemsetAsync(status.p, 0xff, sizeof(unsigned long long), stream));
+    status_launch = 0;
+  }
   void check_status(const char *phase) {
-    int h = 0;
-    VK_HIP(hipMemcpyAsync(&h, status.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    unsigned long long h = ~0ull;
+    VK_HIP(hipMemcpyAsync(&h, status.p, sizeof h, hipMemcpyDeviceToHost, stream));
     VK_HIP(hipStreamSynchronize(stream));
-    if (h) fail(h, phase, "device gate failed with status " + std::to_string(h));
+    if (h != ~0ull) {
+      const int code = (int)(h & 0xff);
+      const uint64_t unit = (h >> 8) & 0xffffffffull;
+      fail(code, phase, "device gate failed with status " + std::to_string(code) + " at zero-based unit " +
+                            std::to_string(unit));
+    }
   }
 };
 

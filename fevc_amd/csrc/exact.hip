@@ -402,6 +402,7 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   Impl &E = *impl_;
   const Prepared &P = *E.P;
   const auto t0 = std::chrono::steady_clock::now();
+  opt.checkpoint("exact_estimator");
   const uint32_t W = (uint32_t)P.workers, F = (uint32_t)P.firms, C = (uint32_t)P.cells;
   if (P.rows == 0 || W == 0 || F < 2)
     fail(kInvalidInput, "exact_estimator", "the retained exact problem has invalid dimensions");
@@ -611,6 +612,12 @@ SolveResult DeviceEngine::run_exact(const SolveOptions &opt) {
   res.exact.stayer_correction[3] =
       res.exact.stayer_correction[0] + res.exact.stayer_correction[1] + 2.0 * res.exact.stayer_correction[2];
   res.correction[3] = res.correction[0] + res.correction[1] + 2.0 * res.correction[2];
+  if (P.stayers) {
+    // exact_estimator.rs:1154-1158, 2334-2351: the stayer hybrid's correction
+    // is add_components(mover, stayer), component by component and total by
+    // total, so the mover/stayer split sums to it exactly
+    for (int k = 0; k < 4; ++k) res.correction[k] = res.exact.mover_correction[k] + res.exact.stayer_correction[k];
+  }
   for (int k = 0; k < 4; ++k) {
     if (!std::isfinite(res.correction[k]))
       fail(kCorrectionNonFinite, "exact_estimator", "exact correction is nonfinite");

@@ -640,6 +640,20 @@ struct OpFirmResidual : FirmCellOp<K, OpFirmResidual<K>> {
   }
 };
 
+// Failure report of the per-unit estimator gates.  The reference stops at the
+// first failing unit of its serial loops, batches in probe order
+// (engine.rs:1190-1228, 2269-2410); here every failing thread offers the key
+// launch << 40 | unit << 8 | code and the smallest key wins, so the reported
+// code is that of the first failing unit of the first failing launch,
+// independent of thread scheduling.  The host numbers launches in issue order.
+struct StatusSink {
+  unsigned long long *word;
+  uint64_t launch;
+};
+__device__ inline void report(StatusSink s, uint64_t unit, int code) {
+  atomicMin(s.word, (unsigned long long)((s.launch << 40) | ((unit & 0xffffffffull) << 8) | (uint64_t)(code & 0xff)));
+}
+
 // FiveMoments accumulation per deletion unit (engine.rs:1190-1228, 354-374):
 // SoA state m[10][units] = 5 x (sum, correction), one compensated addition
 // per batch of probes.
@@ -654,7 +668,7 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
                           const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ cell_worker,
                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
                           const double *__restrict__ firm, const int32_t *__restrict__ atoms,
-                          double *__restrict__ m, int *__restrict__ status) {
+                          double *__restrict__ m, StatusSink status) {
   // TPR threads per unit, LP lanes each (packed loads); the batch's five sums
   // are combined in-thread, then by a fixed xor butterfly over the unit's
   // threads, and added to the unit's running compensated sums by its first
@@ -673,7 +687,7 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
       if (j < (int)kcols) {
         const double proj = root * (worker[w * K + j] + firm[f * K + j]);
         const double res = (double)atoms[g * K + j] / root - proj;
-        if (!isfinite(proj) || !isfinite(res)) atomicCAS(status, 0, 73);
+        if (!isfinite(proj) || !isfinite(res)) report(status, g, 73);
         const double ps = proj * proj, rs = res * res;
         v[0] += ps;
         v[1] += rs;
@@ -706,12 +720,12 @@ struct AdjustParams {
 static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, const uint32_t *__restrict__ unit_cell,
                          const uint64_t *__restrict__ unit_count, const double *__restrict__ unit_outcome,
                          const double *__restrict__ fitted_cell, AdjustParams prm,
-                         double *__restrict__ out /* 7 x units */, int *__restrict__ status) {
+                         double *__restrict__ out /* 7 x units */, StatusSink status) {
   VK_GRID_LOOP(g, units) {
     const double pf = m[0 * units + g] + m[1 * units + g];
     const double mf = m[2 * units + g] + m[3 * units + g];
     const double total = (pf + mf) / prm.probes;
-    if (!isfinite(total) || total <= prm.block_tolerance) { atomicCAS(status, 0, 74); continue; }
+    if (!isfinite(total) || total <= prm.block_tolerance) { report(status, g, 74); continue; }
     const double proj = (pf / prm.probes) / total;
     const double res = (mf / prm.probes) / total;
     const double p2 = (m[4 * units + g] + m[5 * units + g]) / prm.probes;
@@ -719,18 +733,18 @@ static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, co
     const double mixed = (m[8 * units + g] + m[9 * units + g]) / prm.probes;
     const double bias = (res * p2 - proj * m2 + (res - proj) * mixed) / prm.probes;
     double var = (res * res * p2 + proj * proj * m2 - 2.0 * proj * res * mixed) / prm.probes;
-    if (!isfinite(proj) || !isfinite(res) || !isfinite(bias) || !isfinite(var)) { atomicCAS(status, 0, 73); continue; }
-    if (var < -100.0 * prm.rank_tolerance) { atomicCAS(status, 0, 73); continue; }
+    if (!isfinite(proj) || !isfinite(res) || !isfinite(bias) || !isfinite(var)) { report(status, g, 73); continue; }
+    if (var < -100.0 * prm.rank_tolerance) { report(status, g, 73); continue; }
     var = fmax(var, 0.0);
-    if (1.0 - proj <= prm.block_tolerance) { atomicCAS(status, 0, 75); continue; }
+    if (1.0 - proj <= prm.block_tolerance) { report(status, g, 75); continue; }
     const double recip = 1.0 / res;
     const double rr = fabs(res * recip - 1.0);
-    if (!isfinite(recip) || !isfinite(rr) || rr > fmax(100.0 * prm.rank_tolerance, 1e-10)) { atomicCAS(status, 0, 76); continue; }
+    if (!isfinite(recip) || !isfinite(rr) || rr > fmax(100.0 * prm.rank_tolerance, 1e-10)) { report(status, g, 76); continue; }
     const uint32_t c = unit_cell[g];
     const double resid = unit_outcome[g] - (double)unit_count[g] * fitted_cell[c];
     const double mult = recip + bias * (recip * recip) - var * (recip * recip * recip);
     const double deleted = resid * mult;
-    if (!isfinite(resid) || !isfinite(mult) || !isfinite(deleted)) { atomicCAS(status, 0, 71); continue; }
+    if (!isfinite(resid) || !isfinite(mult) || !isfinite(deleted)) { report(status, g, 71); continue; }
     out[0 * units + g] = proj;
     out[1 * units + g] = res;
     out[2 * units + g] = bias;
@@ -760,7 +774,7 @@ __global__ void k_obs_moments(uint64_t rows, uint32_t kcols, uint64_t first_prob
                               const uint64_t *__restrict__ row_phys, uint64_t physical,
                               const uint32_t *__restrict__ cell_worker, const uint32_t *__restrict__ cell_firm,
                               const double *__restrict__ worker, const double *__restrict__ firm,
-                              double *__restrict__ m, double *__restrict__ corr, int *__restrict__ status) {
+                              double *__restrict__ m, double *__restrict__ corr, StatusSink status) {
   constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
   const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
   VK_GRID_LOOP(t, rows * TPR) {
@@ -775,7 +789,7 @@ __global__ void k_obs_moments(uint64_t rows, uint32_t kcols, uint64_t first_prob
       p[q] = 0.0;
       if (j < (int)kcols) {
         p[q] = worker[w * K + j] + firm[f * K + j];
-        if (!isfinite(p[q])) atomicCAS(status, 0, 73);
+        if (!isfinite(p[q])) report(status, g, 73);
         const double p2 = p[q] * p[q];
         v[0] += p2;
         v[1] += p2 * p2;
@@ -851,7 +865,7 @@ static __global__ void k_obs_adjust(uint64_t rows, const double *__restrict__ m,
                                     uint64_t physical, const uint64_t *__restrict__ row_phys,
                                     const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ unit_cell,
                                     const double *__restrict__ row_outcome, const double *__restrict__ fitted_cell,
-                                    AdjustParams prm, double *__restrict__ out, int *__restrict__ status) {
+                                    AdjustParams prm, double *__restrict__ out, StatusSink status) {
   VK_GRID_LOOP(g, rows) {
     const double pf = m[0 * rows + g] + m[1 * rows + g];
     const double ps = m[2 * rows + g] + m[3 * rows + g];
@@ -881,11 +895,11 @@ static __global__ void k_obs_adjust(uint64_t rows, const double *__restrict__ m,
       neumaier_add(bs, bc, bias);
       neumaier_add(vs, vc, var);
     }
-    if (fail) { atomicCAS(status, 0, fail); continue; }
+    if (fail) { report(status, g, fail); continue; }
     const double f = (double)unit_count[g];
     const double y = row_outcome[g], e = y - fitted_cell[unit_cell[g]];
     const double deleted = e * ((is + ic) / f);
-    if (!isfinite(deleted)) { atomicCAS(status, 0, 71); continue; }
+    if (!isfinite(deleted)) { report(status, g, 71); continue; }
     out[0 * rows + g] = (ls + lc) / f;
     out[1 * rows + g] = (rs + rc) / f;
     out[2 * rows + g] = (bs + bc) / f;
@@ -1257,7 +1271,7 @@ struct FinBalance {
   uint64_t rows;
   int parity;
   const double *__restrict__ totals;
-  int *__restrict__ status;
+  StatusSink status;
   __device__ void reduced(int l, const double (&tot)[1]) const {
     if ((l & 1) != parity) return;
     const double ref = totals[K2 / 2 + l / 2];
@@ -1265,7 +1279,7 @@ struct FinBalance {
     const double original = v[(rows - 1) * K2 + l];
     const double gate = 4096.0 * 2.220446049250313e-16;
     if ((ref == 0.0 && last != original) || (ref > 0.0 && fabs(last - original) > gate * ref))
-      atomicCAS(status, 0, 77);
+      report(status, (uint64_t)l, 77);
     v[(rows - 1) * K2 + l] = last;
   }
 };

@@ -81,6 +81,18 @@ static __global__ void __launch_bounds__(kBlock) k_mpnn_out(uint32_t n, const do
   }
 }
 
+// Constant smoother weight omega on every node (host-forced fallback tests):
+// the inverse diagonals omega / d from the plain 0.5 / d.
+static __global__ void __launch_bounds__(kBlock) k_constant_omega(uint32_t n, double omega,
+                                                           const double *__restrict__ inv_plain,
+                                                           double *__restrict__ inv64, float *__restrict__ inv32) {
+  VK_GRID_LOOP(i, (uint64_t)n) {
+    const double v = 2.0 * omega * inv_plain[i];
+    inv64[i] = v;
+    inv32[i] = (float)v;
+  }
+}
+
 // Output layer backward: dz = g_omega * 0.95 s (1 - s), dh2 = dz a.
 static __global__ void __launch_bounds__(kBlock) k_mpnn_out_back(uint32_t n, const double *__restrict__ omega,
                                                           const double *__restrict__ g_omega,

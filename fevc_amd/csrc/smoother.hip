@@ -553,6 +553,28 @@ void DeviceEngine::smoother_default(const std::vector<double> &theta) {
   S.plain_iterations = S.learned_iterations = 0;
 }
 
+void DeviceEngine::smoother_force(double omega) {
+  Impl &E = *impl_;
+  SmootherState &S = E.sm;
+  if (!std::isfinite(omega) || omega <= 0.0) fail(kInvalidInput, "smoother", "forced omega must be positive");
+  if (!has_smoothed_levels(E)) fail(kInvalidInput, "smoother", "the hierarchy has no smoothed level");
+  E.ensure(M::H);
+  ensure_features(E);
+  for (SmootherLevel &Z : S.L) {
+    DevLevel &D = E.levels[Z.level];
+    hipLaunchKernelGGL(dev::k_constant_omega, grid_for(Z.n), kBlock, 0, E.stream, Z.n, omega, D.inv.p, D.linv.p,
+                       D.linv32.p);
+  }
+  VK_HIP(hipStreamSynchronize(E.stream));
+  S.mode = 1;
+  S.active = true;
+  S.shaped = false;
+  S.calibrated = true;
+  S.breakdown = false;
+  S.plain_iterations = S.learned_iterations = 0;
+  S.calibration.clear();
+}
+
 DeviceEngine::SmootherStatus DeviceEngine::smoother_status() const {
   const SmootherState &S = impl_->sm;
   SmootherStatus s;

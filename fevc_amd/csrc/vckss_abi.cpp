@@ -129,8 +129,8 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   const VckssEngineColumnsV1 &c = columns->v2.v1;
   if (c.rows != o.v3.v2.rows) fail(kInvalidInput, "engine_prepare", "column rows differ from the request");
   const uint64_t rows = c.rows;
-  if (req.interrupt_poll && req.interrupt_poll(req.interrupt_context) == VCKSS_INTERRUPT_USER_BREAK)
-    fail(kUserBreak, "engine_prepare", "user break");
+  Interrupt interrupt(req.interrupt_poll, req.interrupt_context, req.checkpoint_interval);
+  interrupt.checkpoint("engine_prepare");
   auto worker = integer_column(c.worker, rows, "worker identifier", kInvalidIdentifier, true);
   auto firm = integer_column(c.firm, rows, "firm identifier", kInvalidIdentifier, true);
   auto deletion = integer_column(c.deletion, rows, "deletion identifier", kInvalidIdentifier, true);
@@ -138,9 +138,9 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   if (!c.outcome || !c.target_weight) fail(kInvalidInput, "engine_ingest", "outcome or target column is null");
   Columns cols{rows, worker.data(), firm.data(), deletion.data(), c.outcome, frequency.data(), c.target_weight};
   ctx.P = prepare(cols, nullptr, o.v3.deletion_mode == VCKSS_DELETION_OBSERVATION);
-  if (req.interrupt_poll && req.interrupt_poll(req.interrupt_context) == VCKSS_INTERRUPT_USER_BREAK)
-    fail(kUserBreak, "engine_prepare", "user break");
+  interrupt.checkpoint("engine_prepare_graph");
   ctx.HS = build_hybrid(ctx.P);
+  interrupt.checkpoint("cmg_full_v2_hierarchy");
   CmgOptions copt;  // full_cmg.rs:467-476: max_levels 128
   ctx.H = build_hierarchy(ctx.HS.graph, copt);
 }
@@ -300,18 +300,28 @@ DeviceEngine &device_of(Context &ctx) {
   return *ctx.dev;
 }
 
-void run_options(Context &ctx, const SolveOptions &o, VckssInterruptPollV1 poll, void *poll_ctx);
+void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt);
 
-void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, VckssInterruptPollV1 poll, void *poll_ctx) {
-  run_options(ctx, solve_options(q, ctx), poll, poll_ctx);
+void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, Interrupt &interrupt) {
+  run_options(ctx, solve_options(q, ctx), interrupt);
 }
 
-void run_options(Context &ctx, const SolveOptions &o, VckssInterruptPollV1 poll, void *poll_ctx) {
-  if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK) fail(kUserBreak, "engine_solve", "user break");
+// context.rs:150-205: a solve needs a prepared generation; a failed solve
+// leaves the generation Failed and later solves are refused.  Solving a
+// Solved generation again (another seed or tolerance on the same prepared
+// problem) is kept as an extension of the reference lifecycle.
+void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt) {
+  if (ctx.state != kPrepared && ctx.state != kSolved)
+    fail(kContextPoisoned, "context_solve",
+         "generation " + std::to_string(ctx.generation) + " is in state " +
+             (ctx.state == kFailed ? "Failed" : ctx.state == kSolving ? "Solving" : "Poisoned") +
+             "; state Prepared is required");
+  o.interrupt = &interrupt;
   DeviceEngine &dev = device_of(ctx);
   ctx.state = kSolving;
   ctx.has_result = false;
   try {
+    o.checkpoint("engine_solve");
     ctx.result = dev.run(o);
   } catch (...) {
     ctx.state = kFailed;
@@ -376,8 +386,9 @@ VckssEngineSolveRequestV5 default_solve_v5() {
 namespace vk {
 void record_unsupported(const char *name) {
   g_last_error = std::string(name) +
-                 ": not supported by the MI355X engine (it implements the compressed, no-control, "
-                 "match-deletion JLA path on route CMG_FULL_V2)";
+                 ": not supported by the MI355X engine (it implements the no-control leave-out path: "
+                 "match or observation deletion, movers or the stayer hybrid, JLA on route CMG_FULL_V2 "
+                 "and the dense exact estimator)";
 }
 const char *thread_last_error() { return g_last_error.c_str(); }
 }  // namespace vk
@@ -388,7 +399,7 @@ uint32_t vckss_rust_abi_version(void) { return VCKSS_RUST_ABI_VERSION_V1; }
 const char *vckss_rust_backend_version(void) { return "vckss-mi355x 0.1.0 (gfx950, CMG_FULL_V2)"; }
 const char *vckss_rust_capabilities_json(void) {
   return "{\"backend\":\"vckss-mi355x\",\"device\":\"gfx950\",\"routes\":[\"CMG_FULL_V2\"],"
-         "\"deletion\":[\"match\"],\"rng\":[\"VCKSS-COUNTER-V1\"],\"controls\":0,"
+         "\"deletion\":[\"match\",\"observation\"],\"stayers\":[\"movers\",\"all\"],\"rng\":[\"VCKSS-COUNTER-V1\"],\"controls\":0,"
          "\"precision\":[64,32],\"max_batch_rhs\":64}";
 }
 const char *vckss_rust_last_error(void) { return g_last_error.c_str(); }
@@ -416,7 +427,7 @@ int32_t vckss_rust_engine_default_prepare_request_interrupt_v3(VckssEnginePrepar
     d.options.v3.v2.struct_size = sizeof(VckssEnginePrepareRequestV4);
     d.options.v3.v2.memory_limit_bytes = 4ULL << 30;
     d.options.v3.deletion_mode = VCKSS_DELETION_MATCH;
-    d.checkpoint_interval = 4096;
+    d.checkpoint_interval = 0;  // no callback (ffi_engine.rs:520-540)
     std::memcpy(output, &d, sizeof d);
   });
 }
@@ -428,7 +439,7 @@ int32_t vckss_rust_engine_default_solve_request_interrupt_v5(VckssEngineSolveReq
     VckssEngineSolveRequestInterruptV5 d;
     std::memset(&d, 0, sizeof d);
     d.options = default_solve_v5();
-    d.checkpoint_interval = 4096;
+    d.checkpoint_interval = 0;  // no callback (ffi_engine.rs:520-540)
     std::memcpy(output, &d, sizeof d);
   });
 }
@@ -474,8 +485,7 @@ int32_t vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1(
 }
 
 static void augment_stayers_value(uint64_t generation, const VckssStayerAugmentationRequestV1 &r,
-                                  const VckssStayerAugmentationColumnsV1 *columns, VckssInterruptPollV1 poll,
-                                  void *poll_ctx) {
+                                  const VckssStayerAugmentationColumnsV1 *columns, Interrupt &interrupt) {
   if (r.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_abi", "unsupported ABI version");
   if (r.struct_size < sizeof(VckssStayerAugmentationRequestV1) || r.reserved != 0)
     fail(kAbiMismatch, "engine_abi", "stayer augmentation request is short or reserves nonzero fields");
@@ -512,8 +522,7 @@ static void augment_stayers_value(uint64_t generation, const VckssStayerAugmenta
   if (ctx.state != kPrepared)
     fail(kContextPoisoned, "context_augment", "generation " + std::to_string(generation) +
                                                   " is not in the Prepared state");
-  if (poll && poll(poll_ctx) == VCKSS_INTERRUPT_USER_BREAK)
-    fail(kUserBreak, "engine_stayer_augmentation", "user break");
+  interrupt.checkpoint("engine_stayer_augmentation");
   StayerColumns sc{rows, firm.data(), worker.data(), frequency.data(), outcome.data(), target.data()};
   try {
     StayerReceipt rec;
@@ -542,7 +551,8 @@ int32_t vckss_rust_engine_augment_stayers_v1(uint64_t generation, const VckssSta
     if (!request) fail(kInvalidInput, "engine_stayer_augmentation", "request pointer is null");
     VckssStayerAugmentationRequestV1 r;
     std::memcpy(&r, request, sizeof r);
-    augment_stayers_value(generation, r, columns, nullptr, nullptr);
+    Interrupt none;
+    augment_stayers_value(generation, r, columns, none);
   });
 }
 
@@ -555,7 +565,8 @@ int32_t vckss_rust_engine_augment_stayers_interrupt_v1(uint64_t generation,
     VckssStayerAugmentationRequestInterruptV1 r;
     std::memcpy(&r, request, sizeof r);
     if (r.reserved != 0) fail(kAbiMismatch, "engine_abi", "reserved interrupt fields must be zero");
-    augment_stayers_value(generation, r.options, columns, r.interrupt_poll, r.interrupt_context);
+    Interrupt interrupt(r.interrupt_poll, r.interrupt_context, r.checkpoint_interval);
+    augment_stayers_value(generation, r.options, columns, interrupt);
   });
 }
 
@@ -602,6 +613,14 @@ int32_t vckss_rust_engine_stayer_hybrid_result_v1(uint64_t generation, VckssStay
     if (!r.has_exact || !ctx.P.stayers)
       fail(kUnsupported, "engine_stayer_hybrid_result", "the solved generation has no stayer-hybrid result");
     const ExactReceipt &x = r.exact;
+    auto identity = [](const double (&v)[4]) {
+      const double scale = std::max({std::fabs(v[0]), std::fabs(v[1]), std::fabs(2.0 * v[2]), std::fabs(v[3]), 1.0});
+      return std::fabs(v[3] - (v[0] + v[1] + 2.0 * v[2])) / scale;
+    };
+    // jla.rs:232-269 verify_accounting(1e-10) on the five vectors, then the
+    // source identity mover + stayer = total
+    for (const double(*v)[4] : {&r.plugin, &r.correction, &r.corrected, &x.mover_correction, &x.stayer_correction})
+      if (!(identity(*v) <= 1e-10)) fail(kAccounting, "engine_stayer_hybrid_result", "accounting identity failed");
     double source = 0.0;
     for (int k = 0; k < 4; ++k)
       source = std::max(source, std::fabs(x.mover_correction[k] + x.stayer_correction[k] - r.correction[k]));
@@ -613,7 +632,6 @@ int32_t vckss_rust_engine_stayer_hybrid_result_v1(uint64_t generation, VckssStay
       c.worker = v[0]; c.firm = v[1]; c.covariance = v[2]; c.total = v[3];
       return c;
     };
-    auto identity = [](const double (&v)[4]) { return std::fabs(v[3] - (v[0] + v[1] + 2.0 * v[2])); };
     VckssStayerHybridResultV1 o;
     std::memset(&o, 0, sizeof o);
     o.struct_size = sizeof o;
@@ -635,7 +653,9 @@ int32_t vckss_rust_engine_stayer_hybrid_result_v1(uint64_t generation, VckssStay
     o.deletion_rank_gap = x.deletion_rank_gap;
     o.firm_zero_sum_residual = x.firm_zero_sum_residual;
     o.topology_checksum = ctx.P.topology_checksum;
-    o.accounting_residual = std::max({identity(r.plugin), identity(r.correction), identity(r.corrected)});
+    // ffi_engine.rs:8691-8693: the receipt reports the unscaled identity residuals
+    auto raw = [](const double (&v)[4]) { return std::fabs(v[3] - v[0] - v[1] - 2.0 * v[2]); };
+    o.accounting_residual = std::max({raw(r.plugin), raw(r.correction), raw(r.corrected), source});
     std::memcpy(output, &o, sizeof o);
   });
 }
@@ -671,7 +691,8 @@ int32_t vckss_rust_engine_solve_v5(uint64_t generation, const VckssEngineSolveRe
     if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
     VckssEngineSolveRequestV5 q;
     std::memcpy(&q, request, sizeof q);
-    run_solve(live(generation), q, nullptr, nullptr);
+    Interrupt none;
+    run_solve(live(generation), q, none);
   });
 }
 
@@ -682,7 +703,8 @@ int32_t vckss_rust_engine_solve_interrupt_v5(uint64_t generation, const VckssEng
     VckssEngineSolveRequestInterruptV5 q;
     std::memcpy(&q, request, sizeof q);
     if (q.reserved) fail(kAbiMismatch, "engine_solve", "reserved fields must be zero");
-    run_solve(live(generation), q.options, q.interrupt_poll, q.interrupt_context);
+    Interrupt interrupt(q.interrupt_poll, q.interrupt_context, q.checkpoint_interval);
+    run_solve(live(generation), q.options, interrupt);
   });
 }
 
@@ -734,8 +756,7 @@ int32_t vckss_rust_engine_numerical_preflight_v2(const VckssNumericalRequestV2 *
   });
 }
 
-static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &r, VckssInterruptPollV1 poll,
-                            void *poll_ctx) {
+static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &r, Interrupt &interrupt) {
   Context &ctx = live(generation);
   VckssEngineSolveRequestV4 point = r.point;
   // numerical_api.rs:157-161 admits the compressed engine for the attachment
@@ -744,7 +765,7 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
   o.numerical = true;
-  run_options(ctx, o, poll, poll_ctx);
+  run_options(ctx, o, interrupt);
 }
 
 int32_t vckss_rust_engine_solve_numerical_v1(uint64_t generation, const VckssNumericalRequestV1 *request) {
@@ -754,7 +775,8 @@ int32_t vckss_rust_engine_solve_numerical_v1(uint64_t generation, const VckssNum
     VckssNumericalRequestV1 r;
     std::memcpy(&r, request, sizeof r);
     validate_numerical_v1(r, 1);
-    solve_numerical(generation, r, nullptr, nullptr);
+    Interrupt none;
+    solve_numerical(generation, r, none);
   });
 }
 
@@ -767,8 +789,8 @@ int32_t vckss_rust_engine_solve_numerical_interrupt_v1(uint64_t generation, cons
     VckssNumericalRequestV1 r;
     std::memcpy(&r, request, sizeof r);
     validate_numerical_v1(r, 1);
-    (void)interval;
-    solve_numerical(generation, r, callback, context);
+    Interrupt interrupt(callback, context, interval);
+    solve_numerical(generation, r, interrupt);
   });
 }
 
@@ -781,8 +803,8 @@ int32_t vckss_rust_engine_solve_numerical_interrupt_v2(uint64_t generation, cons
     VckssNumericalRequestV2 r;
     std::memcpy(&r, request, sizeof r);
     validate_numerical_v2(r);
-    (void)interval;
-    solve_numerical(generation, r.v1, callback, context);
+    Interrupt interrupt(callback, context, interval);
+    solve_numerical(generation, r.v1, interrupt);
   });
 }
 
@@ -1036,6 +1058,43 @@ int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolera
   });
 }
 
+int32_t vckss_mi355x_solve_columns_v1(uint64_t generation, uint32_t columns, double tolerance, double inner_ratio,
+                                      uint32_t precision, uint32_t maximum_iterations, uint32_t replacement_interval,
+                                      const double *worker_rhs, const double *firm_rhs, double *worker_out,
+                                      double *firm_out, double *receipts, uint64_t *refine) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!worker_rhs || !firm_rhs || !worker_out || !firm_out || !receipts || !refine)
+      fail(kInvalidInput, "cmg_full_v2", "null input or output pointer");
+    if (!std::isfinite(tolerance) || tolerance < 1e-15 || tolerance > 1e-4 || !(inner_ratio > 0.0) ||
+        inner_ratio > 1.0)
+      fail(kInvalidInput, "cmg_full_v2", "tolerance must lie in [1e-15, 1e-4] and the inner ratio in (0, 1]");
+    if (precision != 32 && precision != 64) fail(kInvalidInput, "cmg_full_v2", "precision must be 32 or 64");
+    if (maximum_iterations == 0 || replacement_interval == 0)
+      fail(kInvalidInput, "pcg_options", "iteration limit and replacement interval must be positive");
+    const uint64_t W = ctx.P.workers, F = ctx.P.firms;
+    std::vector<double> wr(worker_rhs, worker_rhs + W * (uint64_t)columns),
+        fr(firm_rhs, firm_rhs + F * (uint64_t)columns), wo, fo;
+    SolveOptions o;
+    o.precision = (int)precision;
+    o.max_iterations = maximum_iterations;
+    o.recompute_interval = replacement_interval;
+    std::vector<ColumnReceipt> rc;
+    uint64_t rf[2] = {0, 0};
+    device_of(ctx).solve_batch(wr, fr, columns, tolerance, inner_ratio, o, wo, fo, rc, rf);
+    std::memcpy(worker_out, wo.data(), wo.size() * sizeof(double));
+    std::memcpy(firm_out, fo.data(), fo.size() * sizeof(double));
+    for (uint32_t j = 0; j < columns; ++j) {
+      const double v[5] = {(double)rc[j].iterations, (double)rc[j].restarts, rc[j].reduced, rc[j].complete,
+                           rc[j].rhs_norm};
+      std::memcpy(receipts + 5 * (uint64_t)j, v, sizeof v);
+    }
+    refine[0] = rf[0];
+    refine[1] = rf[1];
+  });
+}
+
 int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint32_t precision, double tolerance,
                                       uint32_t batches, double *seconds, uint64_t *solves, uint64_t *iterations,
                                       double *kernel_ms, double *kernel_bytes, uint64_t *kernel_launches) {
@@ -1107,6 +1166,11 @@ int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const doubl
     }
     device_of(ctx).smoother_mode((int)mode, theta, (int)ctx.precision);
   });
+}
+
+int32_t vckss_mi355x_smoother_force_v1(uint64_t generation, double omega) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] { device_of(live(generation)).smoother_force(omega); });
 }
 
 int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity) {

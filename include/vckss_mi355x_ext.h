@@ -58,6 +58,21 @@ int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolera
                             uint32_t precision, const double *rhs, double *x,
                             uint32_t *iterations);
 
+/* One CMG_FULL_V2 batch solve (full_cmg.rs:809-1111,
+ * solve_batch_with_options_and_interrupt) of 1..64 host columns: worker_rhs
+ * (workers x columns) and firm_rhs (firms x columns), column-major, in the
+ * prepared problem's dense worker and firm order.  The columns are solved in
+ * lockstep to tolerance * inner_ratio, extracted, gated at
+ * max(10 tolerance, 1e-11) on the reduced and complete residuals, and
+ * refined through the 0.1 / 0.01 / 0.001 ladder.  Outputs: worker_out and
+ * firm_out (same layout; firm part mean-normalised), receipts[5 x columns]
+ * (iterations, restarts, reduced, complete residual, complete rhs norm) and
+ * refine[2] (refinement rounds, refined columns). */
+int32_t vckss_mi355x_solve_columns_v1(uint64_t generation, uint32_t columns, double tolerance, double inner_ratio,
+                                      uint32_t precision, uint32_t maximum_iterations, uint32_t replacement_interval,
+                                      const double *worker_rhs, const double *firm_rhs, double *worker_out,
+                                      double *firm_out, double *receipts, uint64_t *refine);
+
 /* Timed leverage batches (atoms, rhs, solve, extraction, moments) with the
  * dominant kernel's average duration measured by HIP events. */
 int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint32_t precision,
@@ -105,6 +120,12 @@ int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const doubl
  * cycle shape in use, then the calibration iterations of the reference cycle,
  * the learned weights, the tuned shape and both (UINT32_MAX if not tried). */
 int32_t vckss_mi355x_smoother_status_v1(uint64_t generation, uint32_t *out, uint32_t capacity);
+/* Host-forced fallback check (no reference counterpart): every smoothed level
+ * uses the constant smoother weight omega > 0 (inverse diagonal omega / d)
+ * with no calibration, so the next solve runs with it; omega >= 1 makes the
+ * V-cycle indefinite, PCG reports status 60 or 61 and the solve is redone
+ * with the reference cycle (the fallback a solve with learned weights takes). */
+int32_t vckss_mi355x_smoother_force_v1(uint64_t generation, double omega);
 /* HIP-graph capture of the PCG iteration (no reference counterpart): enable
  * 1 captures the iteration body and the preconditioning sequence once per
  * configuration (cached by a fingerprint of every pointer, count and flag

@@ -431,8 +431,8 @@ static int run_batch(batch_job *job, uint32_t threads) {
     return ORC_OK;
 }
 
-/* full_cmg.rs:1375-1387 (euclidean_norm, scaled) */
-static double euclid(const double *v, uint64_t n) {
+/* full_cmg.rs:1392-1402 (euclidean_norm, scaled) */
+double orc_euclidean_norm(const double *v, uint64_t n) {
     double scale = 0.0;
     for (uint64_t i = 0; i < n; ++i) scale = fmax(scale, fabs(v[i]));
     if (scale == 0.0 || !isfinite(scale)) return scale;
@@ -441,10 +441,41 @@ static double euclid(const double *v, uint64_t n) {
     return scale * sqrt(s);
 }
 
-/* full_cmg.rs:809-1111: k columns of (worker_rhs, firm_rhs) -> solutions */
-static int solve_batch(const solver_t *s, const double *wr, const double *fr, uint64_t k,
-                       double effective_tol, double inner_ratio, const orc_engine_options *eo,
-                       column_t *cols) {
+/* full_cmg.rs:1375-1390 (refinement_pcg_options): the round's PCG stops at
+ * the absolute residual factor * gate * (smallest failing rhs norm); when
+ * that target is not a positive finite number, at the relative tolerance
+ * pcg_tolerance * factor instead. */
+void orc_refinement_options(double pcg_tolerance, double factor, double gate, double smallest_norm,
+                            double *relative, double *absolute) {
+    double target = factor * gate * smallest_norm;
+    if (isfinite(target) && target > 0.0) {
+        *relative = 0.0;
+        *absolute = target;
+    } else {
+        *relative = pcg_tolerance * factor;
+        *absolute = 0.0;
+    }
+}
+
+/* full_cmg.rs:1445-1470 (failing_columns): ascending columns whose complete
+ * or reduced residual is nonfinite or above the tolerance. */
+uint64_t orc_failing_columns(const double *complete, const double *reduced, uint64_t k, double tolerance,
+                             uint64_t *which) {
+    uint64_t count = 0;
+    for (uint64_t c = 0; c < k; ++c)
+        if (!isfinite(complete[c]) || complete[c] > tolerance || !isfinite(reduced[c]) || reduced[c] > tolerance)
+            which[count++] = c;
+    return count;
+}
+
+static const double kRefinementFactors[3] = {0.1, 0.01, 0.001}; /* full_cmg.rs:37 */
+double orc_refinement_factor(uint32_t round) { return round < 3 ? kRefinementFactors[round] : 0.0; }
+
+/* full_cmg.rs:809-1111: k columns of (worker_rhs, firm_rhs) -> solutions.
+ * rounds/refined (optional) count the refinement rounds and refined columns. */
+static int solve_batch_counted(const solver_t *s, const double *wr, const double *fr, uint64_t k,
+                               double effective_tol, double inner_ratio, const orc_engine_options *eo,
+                               column_t *cols, uint64_t *rounds, uint64_t *refined) {
     const orc_problem *p = s->p;
     uint64_t n = s->n, F = p->firms, W = p->workers;
     double gate = fmax(10.0 * effective_tol, 1e-11);
@@ -465,31 +496,28 @@ static int solve_batch(const solver_t *s, const double *wr, const double *fr, ui
     job.opt.recompute_interval = eo->recompute_interval;
     job.cols = cols; job.rc = rcs;
     int rc = run_batch(&job, eo->threads);
-    static const double factors[3] = {0.1, 0.01, 0.001};
+    double *complete = orc_alloc(k, 8), *reduced = orc_alloc(k, 8);
     for (int round = 0; rc == ORC_OK && round < 3; ++round) {
-        uint64_t failing = 0;
-        for (uint64_t c = 0; c < k; ++c)
-            if (!isfinite(cols[c].complete) || cols[c].complete > gate ||
-                !isfinite(cols[c].reduced) || cols[c].reduced > gate)
-                which[failing++] = c;
+        for (uint64_t c = 0; c < k; ++c) {
+            complete[c] = cols[c].complete;
+            reduced[c] = cols[c].reduced;
+        }
+        uint64_t failing = orc_failing_columns(complete, reduced, k, gate, which);
         if (!failing) break;
+        if (rounds) ++*rounds;
+        if (refined) *refined += failing;
         /* full_cmg.rs:970-1088: warm start from the lifted firm solution */
         double *rrhs = orc_alloc(failing * n, 8), *guess = orc_alloc(failing * n, 8);
         double smallest = INFINITY;
         for (uint64_t i = 0; i < failing; ++i) {
             memcpy(rrhs + i * n, rhs + which[i] * n, n * 8);
             lift_firm(s, cols[which[i]].firm, guess + i * n);
-            smallest = fmin(smallest, euclid(rrhs + i * n, n));
+            smallest = fmin(smallest, orc_euclidean_norm(rrhs + i * n, n));
         }
         batch_job rj = job;
         rj.rhs = rrhs; rj.guess = guess; rj.count = failing;
-        double target = factors[round] * gate * smallest;
-        if (isfinite(target) && target > 0.0) {
-            rj.opt.relative_tolerance = 0.0;
-            rj.opt.absolute_tolerance = target;
-        } else {
-            rj.opt.relative_tolerance = effective_tol * inner_ratio * factors[round];
-        }
+        orc_refinement_options(effective_tol * inner_ratio, kRefinementFactors[round], gate, smallest,
+                               &rj.opt.relative_tolerance, &rj.opt.absolute_tolerance);
         rc = run_batch(&rj, eo->threads);
         free(rrhs);
         free(guess);
@@ -506,7 +534,15 @@ static int solve_batch(const solver_t *s, const double *wr, const double *fr, ui
     free(rhs);
     free(which);
     free(rcs);
+    free(complete);
+    free(reduced);
     return rc;
+}
+
+static int solve_batch(const solver_t *s, const double *wr, const double *fr, uint64_t k,
+                       double effective_tol, double inner_ratio, const orc_engine_options *eo,
+                       column_t *cols) {
+    return solve_batch_counted(s, wr, fr, k, effective_tol, inner_ratio, eo, cols, NULL, NULL);
 }
 
 /* ---------------- estimator (engine.rs) ---------------- */
@@ -1409,6 +1445,53 @@ int orc_time_probe_solves(const orc_problem *p, orc_engine_options opt, uint32_t
     *iterations = it;
     free_cols(cols, count);
     free(atoms); free(wr); free(fr);
+    solver_free(&s);
+    return rc;
+}
+
+/* Harmonic lift of a firm vector onto the hybrid vertices
+ * (cmg_impl.rs.in:459-500); lifted has the hybrid dimension. */
+int orc_hybrid_lift(const orc_problem *p, const double *firm, double *lifted, uint64_t capacity) {
+    solver_t s = {0};
+    s.p = p;
+    int rc = build_hybrid(&s);
+    if (!rc && capacity < s.n) rc = orc_fail(ORC_INVALID_INPUT, "cmg_graph", "lift capacity too small");
+    if (!rc) lift_firm(&s, firm, lifted);
+    orc_graph_free(s.graph);
+    free(s.aux_ptr); free(s.aux_nbr); free(s.aux_w);
+    return rc;
+}
+
+/* One CMG_FULL_V2 batch solve (full_cmg.rs:809-1111) of k columns of
+ * (worker_rhs, firm_rhs), column-major: solutions, complete and reduced
+ * residuals, PCG iterations per column, refinement rounds and refined
+ * columns. */
+int orc_solve_columns(const orc_problem *p, const double *wr, const double *fr, uint64_t k, double tolerance,
+                      double inner_ratio, uint32_t max_iterations, uint32_t recompute_interval, uint32_t threads,
+                      double *worker_out, double *firm_out, double *complete, double *reduced,
+                      uint64_t *iterations, uint64_t *refine_rounds, uint64_t *refined_columns) {
+    solver_t s;
+    int rc = solver_init(&s, p);
+    if (rc) {
+        solver_free(&s);
+        return rc;
+    }
+    orc_engine_options opt = {0};
+    opt.threads = threads;
+    opt.max_iterations = max_iterations;
+    opt.recompute_interval = recompute_interval;
+    column_t *cols = alloc_cols(p, k);
+    *refine_rounds = *refined_columns = 0;
+    rc = solve_batch_counted(&s, wr, fr, k, tolerance, inner_ratio, &opt, cols, refine_rounds, refined_columns);
+    if (!rc)
+        for (uint64_t c = 0; c < k; ++c) {
+            memcpy(worker_out + c * p->workers, cols[c].worker, p->workers * 8);
+            memcpy(firm_out + c * p->firms, cols[c].firm, p->firms * 8);
+            complete[c] = cols[c].complete;
+            reduced[c] = cols[c].reduced;
+            iterations[c] = cols[c].iterations;
+        }
+    free_cols(cols, k);
     solver_free(&s);
     return rc;
 }

@@ -191,6 +191,16 @@ def lib():
         L.orc_rademacher_fill.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                           C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         L.orc_set_empty_sum_negative.argtypes = [C.c_int]
+        L.orc_refinement_options.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)] * 2
+        L.orc_refinement_factor.restype = C.c_double
+        L.orc_refinement_factor.argtypes = [C.c_uint32]
+        L.orc_failing_columns.restype = C.c_uint64
+        L.orc_failing_columns.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_void_p]
+        L.orc_euclidean_norm.restype = C.c_double
+        L.orc_euclidean_norm.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_hybrid_lift.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        L.orc_solve_columns.argtypes = ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_double,
+                                         C.c_uint32, C.c_uint32, C.c_uint32] + [C.c_void_p] * 7)
         _LIB = L
     return _LIB
 
@@ -225,6 +235,33 @@ def rademacher_fill(seed, tag, first_probe, columns, entity, trials):
     _check(lib().orc_rademacher_fill(seed, tag, first_probe, columns, entity.ctypes.data,
                                      trials.ctypes.data, len(entity), out.ctypes.data))
     return out.reshape(columns, len(entity))
+
+
+# ---------------- CMG_FULL_V2 batch-solve pieces ----------------
+
+def refinement_options(pcg_tolerance, factor, gate, smallest_norm):
+    """(relative, absolute) PCG stop of one refinement round (full_cmg.rs:1375-1390)."""
+    rel, ab = C.c_double(), C.c_double()
+    lib().orc_refinement_options(pcg_tolerance, factor, gate, smallest_norm, C.byref(rel), C.byref(ab))
+    return rel.value, ab.value
+
+
+def refinement_factors():
+    return [float(lib().orc_refinement_factor(r)) for r in range(4)]
+
+
+def failing_columns(complete, reduced, tolerance):
+    """Columns the complete-residual gate sends to refinement (full_cmg.rs:1445-1470)."""
+    c = np.ascontiguousarray(complete, np.float64)
+    r = np.ascontiguousarray(reduced, np.float64)
+    out = np.zeros(len(c), np.uint64)
+    n = lib().orc_failing_columns(c.ctypes.data, r.ctypes.data, len(c), tolerance, out.ctypes.data)
+    return [int(x) for x in out[:n]]
+
+
+def euclidean_norm(v):
+    v = np.ascontiguousarray(v, np.float64)
+    return float(lib().orc_euclidean_norm(v.ctypes.data, len(v)))
 
 
 # ---------------- CMG ----------------
@@ -399,6 +436,33 @@ class Problem:
         g = Graph._borrow(h.value)
         g._owned = True
         return g, int(aux.value)
+
+    def hybrid_lift(self, firm):
+        """Harmonic lift of a firm vector onto the hybrid vertices (cmg_impl.rs.in:459-500)."""
+        g, aux = self.hybrid_graph()
+        n = g.n
+        f = np.ascontiguousarray(firm, np.float64)
+        out = np.zeros(n)
+        _check(lib().orc_hybrid_lift(self._h, f.ctypes.data, out.ctypes.data, n))
+        return out
+
+    def solve_columns(self, worker_rhs, firm_rhs, tolerance=1e-10, inner_ratio=1.0, threads=1,
+                      max_iterations=10000, recompute_interval=100):
+        """One CMG_FULL_V2 batch solve (full_cmg.rs:809-1111) of the columns of
+        worker_rhs (workers x k) and firm_rhs (firms x k)."""
+        wr = np.ascontiguousarray(np.asarray(worker_rhs, np.float64).T)
+        fr = np.ascontiguousarray(np.asarray(firm_rhs, np.float64).T)
+        k = wr.shape[0]
+        wo, fo = np.zeros_like(wr), np.zeros_like(fr)
+        comp, red = np.zeros(k), np.zeros(k)
+        it = np.zeros(k, np.uint64)
+        rounds, refined = C.c_uint64(), C.c_uint64()
+        _check(lib().orc_solve_columns(self._h, wr.ctypes.data, fr.ctypes.data, k, tolerance, inner_ratio,
+                                       max_iterations, recompute_interval, threads, wo.ctypes.data,
+                                       fo.ctypes.data, comp.ctypes.data, red.ctypes.data, it.ctypes.data,
+                                       C.byref(rounds), C.byref(refined)))
+        return wo.T, fo.T, {"complete": comp, "reduced": red, "iterations": it.astype(int)}, \
+            {"rounds": int(rounds.value), "refined": int(refined.value)}
 
     def run(self, seed=8675309, probes=200, leverage_width=8, target_width=8, threads=1,
             fit_tolerance=1e-10, probe_tolerance=1e-6, max_iterations=10000,

@@ -19,11 +19,16 @@
  *   - Numerical MCSE: finite differences of the linearised functions, the
  *     covariance-policy cases and fold enumeration of
  *     rust/crates/vckss-core/tests/numerical_mc.rs, eigendecomposition.
- * Parity unpinned: the CMG_FULL_V2 refinement ladder and the harmonic lift
- * of auxiliary vertices (full_cmg.rs:970-1088, cmg_impl.rs.in:459-500).  No
- * reference vector makes a complete-residual gate fail, so these are
- * restated from the source and checked only through the invariant that
- * every returned column meets the gate.
+ *   - CMG_FULL_V2 refinement ladder (full_cmg.rs:970-1088, 1375-1470): the
+ *     reference's own known answers for refinement_pcg_options,
+ *     failing_columns, euclidean_norm and the frozen factor schedule
+ *     (full_cmg.rs:1985-2006, 2120-2139); its grid-and-hub fixture
+ *     (full_cmg.rs:2008-2118), which must refine and then certify, against
+ *     the closed-form solution (0, psi) of y = F psi.
+ *   - Harmonic lift (cmg_impl.rs.in:459-500): brute-force weighted means of
+ *     the auxiliary stars, and the closed form that the hybrid Laplacian on
+ *     the lifted vector vanishes on auxiliary rows and equals the dense firm
+ *     Schur complement on firm rows.
  */
 #ifndef ORC_H
 #define ORC_H
@@ -240,6 +245,25 @@ int orc_cross_covariance(const double *a, const double *b, uint64_t n, double ou
 void orc_numerical_finalize(const double cond[9], const double lev[9], double raw[9], double usable[9],
                             double mcse[4], uint32_t *status, double *psd_adjustment);
 
+/* ---- CMG_FULL_V2 batch solve pieces (full_cmg.rs, cmg_impl.rs.in) ------ */
+/* refinement_pcg_options (full_cmg.rs:1375-1390) */
+void orc_refinement_options(double pcg_tolerance, double factor, double gate, double smallest_norm,
+                            double *relative, double *absolute);
+/* REFINEMENT_FACTORS[round] (full_cmg.rs:37), 0 past the third round */
+double orc_refinement_factor(uint32_t round);
+/* failing_columns (full_cmg.rs:1445-1470); returns the count, indices in which */
+uint64_t orc_failing_columns(const double *complete, const double *reduced, uint64_t k, double tolerance,
+                             uint64_t *which);
+/* euclidean_norm, scaled (full_cmg.rs:1392-1402) */
+double orc_euclidean_norm(const double *v, uint64_t n);
+/* lift_firm_into (cmg_impl.rs.in:459-500): firm values, then every auxiliary
+ * vertex the weighted mean of its firm neighbours */
+int orc_hybrid_lift(const orc_problem *p, const double *firm, double *lifted, uint64_t capacity);
+/* solve_batch_with_options_and_interrupt (full_cmg.rs:809-1111) on k columns */
+int orc_solve_columns(const orc_problem *p, const double *wr, const double *fr, uint64_t k, double tolerance,
+                      double inner_ratio, uint32_t max_iterations, uint32_t recompute_interval, uint32_t threads,
+                      double *worker_out, double *firm_out, double *complete, double *reduced,
+                      uint64_t *iterations, uint64_t *refine_rounds, uint64_t *refined_columns);
 /* Timed probe solves on an existing problem: solves `count` leverage probes
  * (first probe `first`) with `threads` host threads; returns seconds. */
 int orc_time_probe_solves(const orc_problem *p, orc_engine_options opt,
