@@ -247,6 +247,7 @@ SIGNATURES = [
     ("vckss_mi355x_atoms_v1", C.c_int32,
      [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint64]),
     ("vckss_mi355x_spmv_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_spmv_noalloc_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("vckss_mi355x_vcycle_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("vckss_mi355x_pcg_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -577,6 +578,10 @@ class Engine:
         _check(library().vckss_mi355x_atoms_v1(self.generation, seed, domain, first_probe, columns,
                                                out.ctypes.data, out.size))
         return out.reshape(columns, n)
+
+    def level_vertices(self):
+        """Vertex count of every CMG level of the prepared hierarchy."""
+        return [int(v) for v in self.array(28, self.dims["levels"])]
 
     def spmv(self, level, x, columns):
         x = _f64(x)

@@ -10,6 +10,7 @@
 // a contiguous segmented sum, prolongation labels are monotone); the level-0
 // renumbering is undone at the firm/auxiliary boundary of the hybrid system.
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 
 #include "device_impl.hpp"
@@ -700,6 +701,7 @@ void relayout(Impl &E, uint32_t chunk) {
       D.aggs_g.upload(chunk_segments(Y.agg_ptr, chunk, kSegmentGroup));
     }
     D.perm = Y.perm;
+    D.dperm.upload(Y.perm);
     D.lab_host = Y.lab;
     E.max_chunks = std::max<uint64_t>({E.max_chunks, Y.rows.chunks(), Y.aggs.chunks()});
     E.max_multi = std::max({E.max_multi, Y.rows.multi(), Y.aggs.multi()});
@@ -1251,6 +1253,68 @@ void DeviceEngine::solve_batch(const std::vector<double> &wr, const std::vector<
     for (uint64_t w = 0; w < W; ++w) worker[j * W + w] = wo[w * K + j];
     for (uint64_t f = 0; f < F; ++f) firm[j * F + f] = fo[f * K + j];
   }
+}
+
+// ------------------------------------------------ no-allocation level product
+// Device scratch of vckss_mi355x_spmv_noalloc_v1: the renumbered input and
+// output blocks and the chunk partials of rows split over several chunks.
+// Every GPU holds its own copy for the life of the process; all calls on one
+// GPU share it (it keeps the last call's contents, and calls that overlap on
+// one GPU race).
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_address[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once per
+// device.  The array's name is a host-side stand-in: kernels and copies take
+// this address, never the name.
+static double *scratch_address() {
+  int device = 0;
+  VK_HIP(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxDevices) fail(kResourceLimit, "spmv_noalloc", "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_address[device]) {
+    double *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+      fail(kResourceLimit, "spmv_noalloc",
+           std::string("hipGetSymbolAddress of the no-allocation scratch failed: ") + hipGetErrorString(rc));
+    g_scratch_address[device] = ptr;
+  }
+  return g_scratch_address[device];
+}
+
+void DeviceEngine::spmv_noalloc(uint32_t level, uint32_t columns, const double *x, double *y) {
+  Impl &E = *impl_;
+  if (level >= E.levels.size()) fail(kInvalidInput, "spmv_noalloc", "level out of range");
+  if (!x || !y) fail(kInvalidInput, "spmv_noalloc", "null device pointer");
+  const uint32_t K = lanes_for(columns);
+  if (K != columns) fail(kInvalidInput, "spmv_noalloc", "columns must be a power of two in [1, 64]");
+  DevLevel &L = E.levels[level];
+  const uint64_t m = (uint64_t)L.n * K, parts = (uint64_t)std::max<uint32_t>(L.rows.count, 1) * K;
+  const uint64_t need = 2 * m + parts;
+  if (need > kNoAllocScratchDoubles)
+    fail(kResourceLimit, "spmv_noalloc",
+         "level " + std::to_string(level) + " with " + std::to_string(columns) + " columns needs " +
+             std::to_string(need) + " scratch doubles; the no-allocation scratch holds " +
+             std::to_string(kNoAllocScratchDoubles));
+  double *base = scratch_address();
+  double *xr = base, *yr = base + m, *part = base + 2 * m;
+  E.gate = E.always();
+  hipLaunchKernelGGL(dev::k_permute_lanes, grid_for(m), kBlock, 0, E.stream, (uint64_t)L.n, K, L.dperm.p, x, xr,
+                     1);
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    const dev::SegScratch sc{part, nullptr, E.gate};
+    segments_in<double, KK, 0>(E, L.rows, dev::OpProduct<double, KK>{{L.col.p, L.val.p}, xr, yr}, sc);
+  });
+  hipLaunchKernelGGL(dev::k_permute_lanes, grid_for(m), kBlock, 0, E.stream, (uint64_t)L.n, K, L.dperm.p, yr, y,
+                     0);
+  VK_HIP(hipStreamSynchronize(E.stream));
 }
 
 double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t batches, uint64_t &solves,

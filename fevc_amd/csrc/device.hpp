@@ -13,6 +13,10 @@
 
 namespace vk {
 
+// Size of the no-allocation scratch (doubles, 32 MiB): 2 n columns + chunks
+// x columns of the level must fit.
+constexpr uint64_t kNoAllocScratchDoubles = 4ull << 20;
+
 struct SolveOptions {
   uint64_t seed = 8675309;
   uint32_t probes = 200;
@@ -97,6 +101,10 @@ class DeviceEngine {
              std::vector<int64_t> &out);
   void vcycle(const std::vector<double> &rhs, uint32_t columns, int precision, std::vector<double> &out);
   void spmv(uint32_t level, const std::vector<double> &x, uint32_t columns, std::vector<double> &y);
+  // The same product on caller-owned device memory (n x columns, original
+  // vertex order, lanes interleaved) with no device allocation: the working
+  // blocks live in a fixed device scratch of kNoAllocScratchDoubles doubles.
+  void spmv_noalloc(uint32_t level, uint32_t columns, const double *x, double *y);
   // Solve hybrid systems (n x columns, vertex-major) to a relative tolerance.
   void pcg(const std::vector<double> &rhs, uint32_t columns, double tolerance, int precision,
            std::vector<double> &x, std::vector<ColumnReceipt> &receipts);

@@ -126,6 +126,7 @@ struct DevLevel {
   DevChunks rows, aggs;
   DevChunks rows_g, aggs_g;  // the same with runs of short segments grouped (kSegmentGroup)
   std::vector<uint32_t> perm;  // host copy: renumbered -> original vertex
+  DBuf<uint32_t> dperm;        // the same on the device
   std::vector<uint32_t> lab_host;  // host copy of lab (renumbered coarse vertex)
   // Cycle shape: the next level the tuned cycle visits after this one; when it
   // skips levels, the composite aggregation to it (clab, caggs).
@@ -394,18 +395,24 @@ void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, cons
 // Launch a segment operation over `ch` with operator `op`: pass 1 over all
 // chunks, pass 2 for multi-chunk segments, then the final reduction.
 template <typename T, int K, int NV, class Op>
-void segments(Impl &E, const DevChunks &ch, const Op &op, hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
+void segments_in(Impl &E, const DevChunks &ch, const Op &op, const dev::SegScratch &sc,
+                 hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
   constexpr int G = dev::Shape<K>::G;
   const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
   const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
-  const dev::SegScratch sc = E.scratch();
   if (before) VK_HIP(hipEventRecord(before, E.stream));
   hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
   if (after) VK_HIP(hipEventRecord(after, E.stream));
   if (ch.n_multi)
     hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
                        NV > 0 ? grid1 : 0u);
-  if constexpr (NV > 0) finish<K, NV>(E, E.bpartial.p, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
+  if constexpr (NV > 0) finish<K, NV>(E, sc.bpartial, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
+}
+
+// The same with the engine's own scratch (chunk partials and block partials).
+template <typename T, int K, int NV, class Op>
+void segments(Impl &E, const DevChunks &ch, const Op &op, hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
+  segments_in<T, K, NV, Op>(E, ch, op, E.scratch(), before, after);
 }
 
 }  // namespace gpu

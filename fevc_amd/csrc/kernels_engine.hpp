@@ -640,6 +640,20 @@ struct OpFirmResidual : FirmCellOp<K, OpFirmResidual<K>> {
   }
 };
 
+// Lane blocks between original and renumbered vertex order:
+// to_renumbered (forward 1): out[p K + j] = in[perm[p] K + j];
+// from_renumbered (forward 0): out[perm[p] K + j] = in[p K + j].
+static __global__ void k_permute_lanes(uint64_t n, uint32_t K, const uint32_t *__restrict__ perm,
+                                       const double *__restrict__ in, double *__restrict__ out, int forward) {
+  VK_GRID_LOOP(t, n * K) {
+    const uint64_t p = t / K, j = t % K, o = (uint64_t)perm[p] * K + j;
+    if (forward)
+      out[t] = in[o];
+    else
+      out[o] = in[t];
+  }
+}
+
 // Failure report of the per-unit estimator gates.  The reference stops at the
 // first failing unit of its serial loops, batches in probe order
 // (engine.rs:1190-1228, 2269-2410); here every failing thread offers the key

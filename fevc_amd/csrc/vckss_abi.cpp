@@ -998,6 +998,9 @@ int32_t vckss_mi355x_array_v1(uint64_t generation, uint32_t which, double *out, 
         case 25: tmp.assign(P.unit_cell.begin(), P.unit_cell.end()); break;
         case 26: tmp.assign(P.unit_offset.begin(), P.unit_offset.end()); tmp.resize(P.units, 0.0); break;
         case 27: tmp.assign(P.cell_units.begin(), P.cell_units.end()); break;
+        case 28:
+          for (const auto &L : ctx.H.levels) tmp.push_back((double)L.graph.n);
+          break;
         default: fail(kInvalidInput, "array", "unknown array");
       }
     }
@@ -1030,6 +1033,19 @@ int32_t vckss_mi355x_spmv_v1(uint64_t generation, uint32_t level, uint32_t colum
     std::vector<double> in(x, x + m), outv;
     device_of(ctx).spmv(level, in, columns, outv);
     std::memcpy(y, outv.data(), m * sizeof(double));
+  });
+}
+
+int32_t vckss_mi355x_spmv_noalloc_v1(uint64_t generation, uint32_t level, uint32_t columns, const double *x,
+                                     double *y) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    // no device engine is created here: that would allocate
+    if (!ctx.dev)
+      fail(kInvalidInput, "spmv_noalloc",
+           "the generation has no device engine; prepare it with vckss_rust_engine_prepare_interrupt_v4");
+    ctx.dev->spmv_noalloc(level, columns, x, y);
   });
 }
 

@@ -49,6 +49,22 @@ int32_t vckss_mi355x_atoms_v1(uint64_t generation, uint64_t seed, uint32_t domai
 int32_t vckss_mi355x_spmv_v1(uint64_t generation, uint32_t level, uint32_t columns,
                              const double *x, double *y);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP-graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The level product y = A x of vckss_mi355x_spmv_v1 on
+ * caller-owned DEVICE memory: x and y hold n x columns doubles (original
+ * vertex order, lanes interleaved, columns a power of two up to 64).  The
+ * call allocates nothing: its working blocks (renumbered x and y and the
+ * chunk partials of split rows, 2 n columns + chunks x columns doubles) live
+ * in a fixed 32 MiB device scratch (4,194,304 doubles) of the library, one
+ * per GPU for the life of the process; larger levels return 40.  The
+ * generation must have been prepared with its device engine
+ * (vckss_rust_engine_prepare_interrupt_v4).  x must be ready before the call
+ * (synchronize the stream that wrote it); y is ready when it returns.  All
+ * calls on one GPU share the scratch: calls that overlap on one GPU race. */
+int32_t vckss_mi355x_spmv_noalloc_v1(uint64_t generation, uint32_t level, uint32_t columns, const double *x,
+                                     double *y);
+
 /* One V-cycle z = M r on the finest hybrid level. */
 int32_t vckss_mi355x_vcycle_v1(uint64_t generation, uint32_t columns, uint32_t precision,
                                const double *r, double *z);
@@ -213,7 +229,8 @@ int32_t vckss_mi355x_chunk_table_v1(const uint32_t *ptr, uint64_t segments, uint
  * the prepared problem (which 20..): 20 unit_rank 21 unit_count
  * 22 stratum_rank 23 stratum_count 24 cell_weight 25 unit_cell
  * 26 unit copy offset (observation deletion; zero under match deletion)
- * 27 the units of each cell, cells ascending (the device's gather list). */
+ * 27 the units of each cell, cells ascending (the device's gather list)
+ * 28 the vertex count of every CMG level. */
 int32_t vckss_mi355x_prepare_host_v1(const VckssEnginePrepareRequestInterruptV3 *request,
                                      const VckssEngineColumnsV3 *columns, uint64_t *output_handle);
 
