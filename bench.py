@@ -19,10 +19,12 @@ single-RHS FP64 with the errors between them, and the oracle (the CPU
 restatement of the reference engine) on the host's cores on a bounded
 sample.
 
-The roofline entry is the finest-level prolong + smooth pass of the V-cycle
-(k_segments<..., OpProlongSmooth<..., true>>), timed with HIP events on the
-engine stream; `traffic` comes from the committed rocprofv3 FETCH_SIZE /
-WRITE_SIZE summary of that kernel for this configuration when present.
+The roofline entry is the dominant kernel of the timed region, the finest
+A p of the Krylov iteration (k_segments<double, 64, 1, OpCurvature<64>>,
+pass 1), timed with HIP events on the engine stream; `traffic` comes from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel for this
+configuration when present.  The finest prolong + smooth pass of the V-cycle
+(the V-cycle SpMV) is reported beside it the same way.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -47,15 +49,15 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(width, precision):
-    """HBM bytes per launch of the roofline kernel from the committed counter summary."""
-    path = os.path.join(ROOT, "profiles", "finest_prolong_smooth_traffic.json")
+def measured_traffic(key, name="ap_traffic.json"):
+    """HBM bytes per launch of a kernel from a committed counter summary."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             data = json.load(fh)
     except (OSError, ValueError):
         return None
-    entry = data.get(f"w{width}_fp{precision}")
+    entry = data.get(key)
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
@@ -159,16 +161,24 @@ def main():
     log(f"warmup: {args.warmup} batch(es) of {args.width}, fp{args.precision} V-cycle")
     eng.time_leverage(args.width, args.precision, args.tolerance, max(args.warmup, 1))
     log(f"timed: {args.steps} batch(es)")
+    c0 = eng.counters()
     t0 = time.perf_counter()
     tm = eng.time_leverage(args.width, args.precision, args.tolerance, args.steps)
     wall = time.perf_counter() - t0
+    c1 = eng.counters()
+    refinement = {"rounds": c1["refine_rounds"] - c0["refine_rounds"],
+                  "refined_columns": c1["refined_columns"] - c0["refined_columns"],
+                  "batches": c1["batches"] - c0["batches"]}
     solves = tm["solves"]
     value = solves / wall
     iters = tm["iterations"] / max(solves, 1)
     kernel_s = tm["kernel_ms"] / 1e3
     achieved = tm["kernel_bytes"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
-    log(f"batched: {value:.1f} solves/s, {iters:.1f} it/solve, finest prolong-smooth "
-        f"{tm['kernel_ms'] * 1e3:.1f} us x {tm['kernel_launches']} -> {achieved:.0f} GB/s")
+    ps = eng.kernel_timing(1)
+    ps_gbs = ps["bytes"] / (ps["ms"] / 1e3) / 1e9 if ps["ms"] > 0 else 0.0
+    log(f"batched: {value:.1f} solves/s, {iters:.1f} it/solve, refinement {refinement}; finest A p "
+        f"{tm['kernel_ms'] * 1e3:.1f} us x {tm['kernel_launches']} -> {achieved:.0f} GB/s; finest prolong-smooth "
+        f"{ps['ms'] * 1e3:.1f} us x {ps['launches']} -> {ps_gbs:.0f} GB/s")
 
     plain_batched = None
     if args.learned and args.plain_steps > 0:
@@ -334,7 +344,7 @@ def main():
                "runs_seconds": [round(s_, 3) for s_ in secs], **host}
         log(f"cpu baseline: {cpu['value']:.2f} solves/s (median)")
 
-    traffic = measured_traffic(args.width, args.precision)
+    traffic = measured_traffic(f"w{args.width}_fp{args.precision}")
     observation = None
     if args.observation:
         # one subsample of the many-subsample observation-deletion workload at its
@@ -420,9 +430,17 @@ def main():
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_segments<OpProlongSmooth, finest level> pass 1",
-                     "kernel_us": round(tm["kernel_ms"] * 1e3, 2),
+                     "kernel": "k_segments<double, %d, 1, OpCurvature<%d>> pass 1 (finest A p)"
+                               % (args.width, args.width),
+                     "kernel_us": round(tm["kernel_ms"] * 1e3, 2), "launches": tm["kernel_launches"],
                      "algorithmic_bytes": tm["kernel_bytes"]},
+        "vcycle_spmv": {"kernel": "finest prolong + smooth pass 1 (FP%d V-cycle)" % args.precision,
+                        "achieved": round(ps_gbs, 1), "frac": round(ps_gbs / HBM_PEAK_GBS, 4),
+                        "kernel_us": round(ps["ms"] * 1e3, 2), "launches": ps["launches"],
+                        "algorithmic_bytes": ps["bytes"],
+                        "traffic": measured_traffic(f"w{args.width}_fp{args.precision}",
+                                                    "finest_prolong_smooth_traffic.json")},
+        "refinement": refinement,
         "cpu_baseline": cpu,
         "autotune": tuning,
         "learned_smoother": smoother,

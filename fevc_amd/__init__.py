@@ -255,6 +255,9 @@ SIGNATURES = [
      [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.POINTER(C.c_double),
       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double),
       C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_first_pass_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("vckss_mi355x_kernel_timing_v1", C.c_int32,
+     [C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_solve_columns_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -604,6 +607,22 @@ class Engine:
         _check(library().vckss_mi355x_pcg_v1(self.generation, columns, tolerance, precision,
                                              rhs.ctypes.data, x.ctypes.data, it.ctypes.data))
         return x, it
+
+    def kernel_timing(self, which):
+        """HIP-event timing of a kernel in the last time_leverage run: 0 the
+        finest A p (dominant), 1 the finest prolong + smooth."""
+        ms, nb, n = C.c_double(), C.c_double(), C.c_uint64()
+        _check(library().vckss_mi355x_kernel_timing_v1(self.generation, which, C.byref(ms), C.byref(nb),
+                                                       C.byref(n)))
+        return {"ms": ms.value, "bytes": nb.value, "launches": int(n.value)}
+
+    def first_pass(self, mode=None):
+        """Set the first-pass stop (0: the reference's; 1: gate-aware, the
+        default) and return the previous mode; with None only query it."""
+        prev = C.c_uint32()
+        _check(library().vckss_mi355x_first_pass_v1(self.generation, 2 if mode is None else int(mode),
+                                                    C.byref(prev)))
+        return int(prev.value)
 
     def solve_columns(self, worker_rhs, firm_rhs, tolerance=1e-10, inner_ratio=1.0, precision=64,
                       maximum_iterations=10000, replacement_interval=100):

@@ -102,23 +102,17 @@ struct Cycle {
                                                             x0 ? inv(Lc) : nullptr, x0 ? vp(c.x.p) : nullptr});
       level(nx, E.levels[nx - 1].repeat, nullptr);
       if (l == 0) {
-        // the finest prolong-smooth pass is the kernel the roofline reports:
-        // HIP events around its pass-1 launch when timing
-        hipEvent_t a = nullptr, e = nullptr;
-        if (E.timing) {
-          VK_HIP(hipEventCreate(&a));
-          VK_HIP(hipEventCreate(&e));
-          E.events.push_back({a, e});
-          // algorithmic bytes: CSR once (values, columns, chunk descriptors),
-          // inverse diagonal and labels, r and x read, y written, e read
-          const double s = sizeof(T);
-          E.timed_bytes = (double)L.nnz * (s + 4) + 16.0 * rows.count + (s + 4) * L.n +
-                          K * s * (3.0 * L.n + Lc.n);
-        }
+        // the finest prolong-smooth pass, timed with HIP events in a timed
+        // run; algorithmic bytes: CSR once (values, columns, chunk
+        // descriptors), inverse diagonal and labels, r and x read, y
+        // written, the coarse correction read
+        const double s = sizeof(T);
+        const auto ev = E.timed_pair(1, (double)L.nnz * (s + 4) + 16.0 * rows.count + (s + 4) * L.n +
+                                            K * s * (3.0 * L.n + Lc.n));
         segments<VT, KV, 0>(E, rows,
                             dev::OpProlongSmooth<VT, KV, true>{{pcol, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p),
                                                                inv(L), lab, vp(v.y.p)},
-                            a, e);
+                            ev.first, ev.second);
       } else {
         segments<VT, KV, 0>(E, rows,
                             dev::OpProlongSmooth<VT, KV>{{pcol, val(L)}, vp(v.r.p), vp(v.x.p), vp(c.x.p), inv(L),
@@ -208,6 +202,7 @@ uint64_t pcg_fingerprint(const Impl &E, const double *b, const dev::PcgParams &p
   f.add(prm.n);
   f.real(prm.rel);
   f.real(prm.abs);
+  f.real(prm.cap);
   f.real(prm.bound);
   f.add((uint64_t)(int64_t)prm.max_iterations);
   f.add((uint64_t)(int64_t)prm.interval);
@@ -275,12 +270,12 @@ hipGraphExec_t capture(Impl &E, F &&enqueue) {
 // kPollLag iterations; iterations queued after convergence return at once.
 template <int K>
 void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
-               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out) {
+               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0) {
   const uint32_t n = E.n0;
   const uint64_t m0 = (uint64_t)n * K;
   const DevLevel &L0 = E.levels[0];
   const DevChunks &rows0 = grouped<K>() ? L0.rows_g : L0.rows;
-  dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval};
+  dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval, cap};
   int *flags = E.flags.p;
   const int *active = flags + dev::kFlagActive;
   VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
@@ -323,7 +318,12 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   };
   // iteration body: A p, update, centring, fresh-residual gate, limit
   auto body = [&]() {
-    segments<double, K, 1>(E, rows0, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p});
+    // A p, the dominant kernel of the iteration, timed with HIP events in a
+    // timed run; algorithmic bytes: CSR once (values, columns, chunk
+    // descriptors), p read once, A p written once
+    const auto ev = E.timed_pair(0, 12.0 * (double)L0.nnz + 16.0 * rows0.count + 2.0 * 8.0 * (double)m0);
+    segments<double, K, 1>(E, rows0, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p}, ev.first,
+                           ev.second);
     hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
                        E.rpartial.p, E.st.p, active);
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
@@ -444,8 +444,17 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   for (uint32_t j = 0; j < columns; ++j) use[j] = 1;
   std::vector<dev::ColState> st;
   E.batches++;
+  // Gate-aware first pass: the reference's operator-scaled stop
+  // rel (||b|| + 2 maxdeg ||x||) can accept residuals far above the complete
+  // gate when a heavy firm dominates the degree bound, and the batch is then
+  // refined from a warm start with a second extraction.  Capping the stop at
+  // 0.1 gate ||b|| per lane (the first refinement round's own target) makes
+  // the first pass meet the gate in one extraction; the gate and the ladder
+  // below are unchanged and still decide.  E.first_pass_cap = false keeps the
+  // reference's stop.
+  const double cap = E.first_pass_cap ? 0.1 * gate : 0.0;
   pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
-               opt.precision == 32, use, st);
+               opt.precision == 32, use, st, cap);
   receipts.assign(columns, ColumnReceipt{});
   std::vector<double> first_norm(K, 0.0);
   for (uint32_t j = 0; j < columns; ++j) {
@@ -792,7 +801,7 @@ DeviceEngine::~DeviceEngine() {
   if (impl_ && impl_->stream) {
     (void)hipStreamSynchronize(impl_->stream);
     impl_->drop_graphs();
-    for (auto &e : impl_->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    impl_->drop_timing();
     for (auto &ev : impl_->poll_events)
       if (ev) (void)hipEventDestroy(ev);
     if (impl_->host_flags) (void)hipHostFree(impl_->host_flags);
@@ -1328,8 +1337,7 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   E.fitted.alloc(E.C);
   VK_HIP(hipMemsetAsync(E.moments.p, 0, 10 * E.G * sizeof(double), E.stream));
   VK_HIP(hipStreamSynchronize(E.stream));
-  for (auto &e : E.events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-  E.events.clear();
+  E.drop_timing();
   E.timing = true;
   solves = 0;
   iterations = 0;
@@ -1394,6 +1402,9 @@ DeviceEngine::Tuning DeviceEngine::autotune(const SolveOptions &base, uint32_t b
 
 void DeviceEngine::set_chunk(uint32_t chunk) { relayout(*impl_, chunk); }
 
+void DeviceEngine::set_first_pass_cap(bool enable) { impl_->first_pass_cap = enable; }
+bool DeviceEngine::first_pass_cap() const { return impl_->first_pass_cap; }
+
 void DeviceEngine::set_graphs(bool enable) {
   Impl &E = *impl_;
   E.use_graphs = enable;
@@ -1436,16 +1447,18 @@ double DeviceEngine::stream_copy(uint64_t bytes) {
   return ms;
 }
 
-DeviceEngine::KernelTiming DeviceEngine::dominant_kernel_timing() const {
+DeviceEngine::KernelTiming DeviceEngine::kernel_timing(int which) const {
   const Impl &E = *impl_;
   KernelTiming t;
+  if (which < 0 || which > 1) fail(kInvalidInput, "kernel_timing", "timed kernel must be 0 (A p) or 1");
+  const auto &events = E.timed[which].events;
   // Launches queued after every lane converged return at the device gate
   // without touching the level (a few microseconds); only launches that did
   // the pass (longer than a quarter of the longest) are averaged.
-  std::vector<float> ms(E.events.size(), 0.f);
+  std::vector<float> ms(events.size(), 0.f);
   float longest = 0.f;
-  for (size_t k = 0; k < E.events.size(); ++k) {
-    if (hipEventElapsedTime(&ms[k], E.events[k].first, E.events[k].second) != hipSuccess) ms[k] = 0.f;
+  for (size_t k = 0; k < events.size(); ++k) {
+    if (hipEventElapsedTime(&ms[k], events[k].first, events[k].second) != hipSuccess) ms[k] = 0.f;
     longest = std::max(longest, ms[k]);
   }
   double total = 0.0;
@@ -1455,7 +1468,7 @@ DeviceEngine::KernelTiming DeviceEngine::dominant_kernel_timing() const {
       t.launches++;
     }
   t.ms_per_launch = t.launches ? total / (double)t.launches : 0.0;
-  t.bytes_per_launch = E.timed_bytes;
+  t.bytes_per_launch = E.timed[which].bytes;
   return t;
 }
 

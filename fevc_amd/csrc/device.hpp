@@ -128,6 +128,12 @@ class DeviceEngine {
   };
   Tuning autotune(const SolveOptions &base, uint32_t batches);
   void set_chunk(uint32_t chunk);
+  // First PCG pass of every batch: true caps its stop at 0.1 x the complete
+  // gate x ||b|| per lane (default); false keeps the reference's
+  // operator-scaled stop alone, so gate failures go through the refinement
+  // ladder as in full_cmg.rs:970-1088.
+  void set_first_pass_cap(bool enable);
+  bool first_pass_cap() const;
   // HIP-graph capture of the PCG iteration (two graphs per configuration,
   // cached); returns the captures and graph launches so far.
   void set_graphs(bool enable);
@@ -138,10 +144,12 @@ class DeviceEngine {
   // V-cycles applied, solve batches, refinement rounds, refined columns,
   // PCG iterations launched (including ones queued after convergence).
   void counters(uint64_t out[6]) const;
-  // Average duration of the dominant kernel over the launches of a timed run
-  // that did work (launches returning at the convergence gate excluded).
+  // Average duration of a timed kernel over the launches of the last timed
+  // run that did work (launches returning at the convergence gate excluded):
+  // 0 the finest A p of the Krylov iteration (the dominant kernel), 1 the
+  // finest prolong + smooth pass of the V-cycle.
   struct KernelTiming { double ms_per_launch = 0; uint64_t launches = 0; double bytes_per_launch = 0; };
-  KernelTiming dominant_kernel_timing() const;
+  KernelTiming kernel_timing(int which) const;
 
   // Tuned preconditioner (no counterpart in the reference).  mode 0: the
   // reference cycle; mode 1: the network's per-node smoother weights (theta)

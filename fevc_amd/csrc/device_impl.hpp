@@ -251,12 +251,39 @@ struct DeviceEngine::Impl {
   // numerical MCSE attachment: unit gradients (5 x G), offsets, fold sums and
   // errors (12 x G), state partials, replay scores (6 per lane)
   DBuf<double> ngrad, noffs, nfolds, npart, nscores;
+  // cap the first PCG pass of a batch at 0.1 gate ||b|| (see solve_columns)
+  bool first_pass_cap = true;
   // first per-unit gate failure (dev::StatusSink key; all ones when none)
   DBuf<unsigned long long> status;
   uint64_t status_launch = 0;
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-  double timed_bytes = 0;
+  // HIP events around the launches of the timed kernels in a timed run:
+  // [0] the finest-level A p of the Krylov iteration (pass 1 of
+  // OpCurvature), [1] the finest prolong + smooth pass of the V-cycle; with
+  // their algorithmic bytes per launch
+  struct TimedKernel {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    double bytes = 0;
+  };
+  TimedKernel timed[2];
+  std::pair<hipEvent_t, hipEvent_t> timed_pair(int which, double bytes) {
+    if (!timing) return {nullptr, nullptr};
+    hipEvent_t a = nullptr, e = nullptr;
+    VK_HIP(hipEventCreate(&a));
+    VK_HIP(hipEventCreate(&e));
+    timed[which].events.push_back({a, e});
+    timed[which].bytes = bytes;
+    return {a, e};
+  }
+  void drop_timing() {
+    for (auto &t : timed) {
+      for (auto &e : t.events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+      t.events.clear();
+    }
+  }
   uint64_t vcycles = 0;
   // diagnostics: solve batches, refinement rounds, refined columns, PCG iterations launched
   uint64_t batches = 0, refine_rounds = 0, refined_columns = 0, iterations_launched = 0;

@@ -22,7 +22,16 @@ struct PcgParams {
   uint32_t n;
   double rel, abs, bound;
   int32_t max_iterations, interval;
+  // 0, or a cap c on the stop: ||r|| <= min(abs + rel (||b|| + bound ||x||), c ||b||)
+  double cap;
 };
+
+// The certified stop of pcg.rs:2428-2436, abs + rel (||b|| + 2 maxdeg ||x||),
+// optionally capped relative to ||b|| (the gate-aware first pass).
+__device__ inline double stop_tolerance(const PcgParams &prm, double rhs_norm, double xnorm) {
+  const double t = prm.abs + prm.rel * (rhs_norm + prm.bound * xnorm);
+  return prm.cap > 0.0 ? fmin(t, prm.cap * rhs_norm) : t;
+}
 
 // Device flags of the lockstep PCG: [0] some lane is active (the gate of
 // every iteration kernel), [1] some active lane needs a fresh residual,
@@ -85,7 +94,7 @@ __global__ void k_pcg_start(ColState *__restrict__ st, PcgParams prm) {
   if (j >= K) return;
   ColState &s = st[j];
   s.init_norm = s.rhs_norm;
-  s.tol = prm.abs + prm.rel * s.rhs_norm;
+  s.tol = stop_tolerance(prm, s.rhs_norm, 0.0);
   s.mean_r = 0.0;
   if (s.active && s.init_norm <= s.tol) {
     s.active = 0;
@@ -117,7 +126,7 @@ struct OpWarm : CsrOp<double, K, OpWarm<K>> {
   __device__ void reduced(int j, const double (&tot)[NV]) const {
     ColState &s = st[j];
     s.init_norm = sqrt(tot[0]);
-    s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * sqrt(tot[1]));
+    s.tol = stop_tolerance(prm, s.rhs_norm, sqrt(tot[1]));
     s.mean_r = tot[2] / (double)prm.n;
     if (s.active && s.init_norm <= s.tol) {
       s.active = 0;
@@ -282,7 +291,7 @@ struct FinUpdate {
     s.xnorm = sqrt(fmax(tot[1] - n * s.mean_x * s.mean_x, 0.0));
     s.mean_r = tot[2] / n;
     s.rnorm = sqrt(tot[3]);
-    s.tol = prm.abs + prm.rel * (s.rhs_norm + prm.bound * s.xnorm);
+    s.tol = stop_tolerance(prm, s.rhs_norm, s.xnorm);
     s.need_fresh = (s.rnorm <= s.tol) || (s.iters % prm.interval == 0);
   }
 };

@@ -1074,6 +1074,30 @@ int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolera
   });
 }
 
+int32_t vckss_mi355x_kernel_timing_v1(uint64_t generation, uint32_t which, double *milliseconds, double *bytes,
+                                      uint64_t *launches) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!milliseconds || !bytes || !launches) fail(kInvalidInput, "kernel_timing", "null output pointer");
+    const auto t = device_of(ctx).kernel_timing((int)which);
+    *milliseconds = t.ms_per_launch;
+    *bytes = t.bytes_per_launch;
+    *launches = t.launches;
+  });
+}
+
+int32_t vckss_mi355x_first_pass_v1(uint64_t generation, uint32_t mode, uint32_t *previous) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (mode > 2) fail(kInvalidInput, "first_pass", "mode must be 0 (reference stop), 1 (gate-aware) or 2 (query)");
+    DeviceEngine &d = device_of(ctx);
+    if (previous) *previous = d.first_pass_cap() ? 1u : 0u;
+    if (mode < 2) d.set_first_pass_cap(mode == 1);
+  });
+}
+
 int32_t vckss_mi355x_solve_columns_v1(uint64_t generation, uint32_t columns, double tolerance, double inner_ratio,
                                       uint32_t precision, uint32_t maximum_iterations, uint32_t replacement_interval,
                                       const double *worker_rhs, const double *firm_rhs, double *worker_out,
@@ -1123,7 +1147,7 @@ int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint3
     o.probe_tolerance = tolerance;
     DeviceEngine &d = device_of(ctx);
     *seconds = d.time_leverage_batches(o, batches, *solves, *iterations);
-    auto t = d.dominant_kernel_timing();
+    auto t = d.kernel_timing(0);
     *kernel_ms = t.ms_per_launch;
     *kernel_bytes = t.bytes_per_launch;
     *kernel_launches = t.launches;

@@ -74,6 +74,26 @@ int32_t vckss_mi355x_pcg_v1(uint64_t generation, uint32_t columns, double tolera
                             uint32_t precision, const double *rhs, double *x,
                             uint32_t *iterations);
 
+/* Average duration, algorithmic bytes per launch and working launches of a
+ * timed kernel of the last vckss_mi355x_time_leverage_v1 run, from HIP events
+ * on the engine stream (launches returning at the convergence gate excluded):
+ * which 0 the finest A p of the Krylov iteration, 1 the finest prolong +
+ * smooth pass of the V-cycle. */
+int32_t vckss_mi355x_kernel_timing_v1(uint64_t generation, uint32_t which, double *milliseconds, double *bytes,
+                                      uint64_t *launches);
+
+/* Stop of the first PCG pass of every batch (no reference counterpart).
+ * mode 1 (the default): the reference's certified stop
+ * abs + rel (||b|| + 2 maxdeg ||x||) capped per column at
+ * 0.1 x gate x ||b||, gate = max(10 tolerance, 1e-11): a heavy firm that
+ * dominates the degree bound no longer lets the first pass stop above the
+ * complete-residual gate, so batches meet the gate without a refinement
+ * round and second extraction.  mode 0: the reference's stop alone (gate
+ * failures are refined through the ladder of full_cmg.rs:970-1088).  mode 2
+ * only queries.  previous (optional) receives the mode before the call.  The
+ * gate and the refinement ladder apply in both modes. */
+int32_t vckss_mi355x_first_pass_v1(uint64_t generation, uint32_t mode, uint32_t *previous);
+
 /* One CMG_FULL_V2 batch solve (full_cmg.rs:809-1111,
  * solve_batch_with_options_and_interrupt) of 1..64 host columns: worker_rhs
  * (workers x columns) and firm_rhs (firms x columns), column-major, in the
@@ -90,7 +110,8 @@ int32_t vckss_mi355x_solve_columns_v1(uint64_t generation, uint32_t columns, dou
                                       double *firm_out, double *receipts, uint64_t *refine);
 
 /* Timed leverage batches (atoms, rhs, solve, extraction, moments) with the
- * dominant kernel's average duration measured by HIP events. */
+ * dominant kernel's (the finest A p of the Krylov iteration) average duration
+ * measured by HIP events on the engine stream. */
 int32_t vckss_mi355x_time_leverage_v1(uint64_t generation, uint32_t width, uint32_t precision,
                                       double tolerance, uint32_t batches, double *seconds,
                                       uint64_t *solves, uint64_t *iterations,

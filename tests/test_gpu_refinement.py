@@ -53,6 +53,7 @@ def test_refinement_ladder_matches_closed_form_and_oracle(grid, precision):
         assert e.dims["workers"] == p.workers and e.dims["firms"] == p.firms
         e.configure(precision, 32, 16)
         e.smoother(0)
+        e.first_pass(0)  # the reference's operator-scaled stop, so the ladder runs
         print(f"grid fixture: device batch solve, fp{precision} V-cycle", flush=True)
         gw, gf, rec, ref = e.solve_columns(W, Fm, tolerance=tol, precision=precision)
     finally:
@@ -88,6 +89,7 @@ def test_estimator_refinement_counters_and_parity(grid, oracle):
     try:
         e.configure(64, 8, 4)
         e.smoother(0)
+        e.first_pass(0)
         before = e.counters()
         got = e.solve(probes=8, tolerance=1e-9, fit_tolerance=1e-10)
         after = e.counters()
@@ -102,3 +104,29 @@ def test_estimator_refinement_counters_and_parity(grid, oracle):
         g, w = np.asarray(got[key]), np.asarray(want[key])
         assert np.all(np.abs(g - w) <= 1e-6 * np.maximum(np.abs(w), 1e-3)), (key, g, w)
     assert np.max(arrays["leverage_complete_residual"]) <= 1e-8
+
+
+def test_gate_aware_first_pass_skips_refinement(grid, oracle):
+    """The default first pass caps its stop at 0.1 gate ||b||: on the hub
+    panel the grid columns then meet the gate without a refinement round, and
+    the solutions agree with the reference ladder's within the gate."""
+    cols, p, psi, W, Fm = grid
+    e = F.Engine(*cols)
+    try:
+        e.configure(64, 32, 16)
+        e.smoother(0)
+        assert e.first_pass() == 1
+        gw, gf, rec, ref = e.solve_columns(W, Fm, tolerance=1e-10)
+        e.first_pass(0)
+        rw, rf, rrec, rref = e.solve_columns(W, Fm, tolerance=1e-10)
+    finally:
+        e.release()
+    print(f"gate-aware first pass: refinement {ref}, iterations {rec['iterations']}; "
+          f"reference stop: refinement {rref}, iterations {rrec['iterations']}")
+    assert ref["rounds"] == 0 and rref["rounds"] >= 1
+    assert np.all(rec["complete"] <= 1e-9) and np.all(rec["reduced"] <= 1e-9)
+    m = psi.sum() / p.firms
+    assert np.max(np.abs(gf[:, 0] - (psi - m))) <= 1e-8 * np.abs(psi).max()
+    for j in (0, 2):
+        s = max(np.abs(rf[:, j]).max(), 1.0)
+        assert np.max(np.abs(gf[:, j] - rf[:, j])) <= 1e-8 * s
