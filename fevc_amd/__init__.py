@@ -223,6 +223,26 @@ SIGNATURES = [
     ("vckss_rust_engine_prepare_interrupt_v4", C.c_int32,
      [C.POINTER(PrepareRequestInterruptV3), C.POINTER(ColumnsV3), C.POINTER(C.c_uint64), C.c_uint32]),
     ("vckss_rust_engine_retained_mask_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint64]),
+    ("vckss_rust_backend_capabilities_v1", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_backend_request_capability_v1", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_backend_request_capability_v2", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_backend_request_capability_v3", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_admit_prepare_v4", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_preparation_receipt_v4", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_memory_forecast_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_performance_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_detailed_receipt_v6", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_detailed_receipt_v7", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_full_cmg_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_rhs_receipts_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint64]),
+    ("vckss_rust_engine_default_solve_request_v6", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_default_solve_request_interrupt_v6", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_solve_v6", C.c_int32, [C.c_uint64, C.c_void_p]),
+    ("vckss_rust_engine_solve_interrupt_v6", C.c_int32, [C.c_uint64, C.c_void_p]),
+    ("vckss_rust_engine_default_solve_request_v7", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_default_solve_request_interrupt_v7", C.c_int32, [C.c_void_p, C.c_uint32]),
+    ("vckss_rust_engine_solve_v7", C.c_int32, [C.c_uint64, C.c_void_p]),
+    ("vckss_rust_engine_solve_interrupt_v7", C.c_int32, [C.c_uint64, C.c_void_p]),
     ("vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1", C.c_int32,
      [C.POINTER(StayerRequestInterruptV1), C.c_uint32]),
     ("vckss_rust_engine_augment_stayers_v1", C.c_int32,
@@ -336,6 +356,11 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def host_memory_bytes() -> int:
+    """Physical memory of this host in bytes."""
+    return os.sysconf("SC_PAGE_SIZE") * os.sysconf("SC_PHYS_PAGES")
+
+
 def default_prepare_request(rows: int) -> PrepareRequestInterruptV3:
     req = PrepareRequestInterruptV3()
     _check(library().vckss_rust_engine_default_prepare_request_interrupt_v3(C.byref(req), C.sizeof(req)))
@@ -377,9 +402,11 @@ class Engine:
     """
 
     def __init__(self, worker, firm, deletion, outcome, frequency, target, *, upload=True,
-                 poll=None, deletion_mode="match"):
+                 poll=None, deletion_mode="match", memory_limit=None):
         """deletion_mode "match" (the default) or "observation" (the dense
-        exact route, and the JLA route without numerical MCSE)."""
+        exact route, and the JLA route without numerical MCSE).
+        memory_limit is the whole-command host byte limit of the admission
+        (default: the host's physical memory)."""
         lib = library()
         self.deletion_mode = {"match": 1, "observation": 2}[deletion_mode]
         self.stayers_mode = 1  # movers; 2 (all) after augment_stayers
@@ -388,6 +415,8 @@ class Engine:
         rows = len(cols[0])
         req = default_prepare_request(rows)
         req.options.cleanup_abandoned = 1
+        req.options.caller_copy_bytes = rows * 6 * 8
+        req.options.memory_limit_bytes = memory_limit or host_memory_bytes()
         req.options.deletion_mode = self.deletion_mode
         self._poll = Poll(poll) if poll else None
         if self._poll:
@@ -420,6 +449,7 @@ class Engine:
         _check(library().vckss_rust_engine_default_stayer_augmentation_request_interrupt_v1(C.byref(r),
                                                                                              C.sizeof(r)))
         r.options.rows = rows
+        r.options.caller_copy_bytes = rows * 5 * 8
         c = StayerColumnsV1()
         c.struct_size, c.rows = C.sizeof(c), rows
         if rows:
@@ -732,6 +762,32 @@ class Engine:
         _check(library().vckss_mi355x_stream_copy_v1(self.generation, nbytes, C.byref(ms)))
         return ms.value
 
+    # ---- receipts of the reference plugin ABI
+    def preparation_receipt(self):
+        return _out(library().vckss_rust_engine_preparation_receipt_v4, "VckssEnginePreparationReceiptV4",
+                    self.generation)
+
+    def memory_forecast(self):
+        return _out(library().vckss_rust_engine_memory_forecast_v1, "VckssMemoryForecastV1", self.generation)
+
+    def performance_receipt(self):
+        return _out(library().vckss_rust_engine_performance_receipt_v1, "VckssEnginePerformanceReceiptV1",
+                    self.generation)
+
+    def detailed_receipt(self, version=7):
+        return _out(getattr(library(), f"vckss_rust_engine_detailed_receipt_v{version}"),
+                    f"VckssEngineDetailedReceiptV{version}", self.generation)
+
+    def full_cmg_receipt(self):
+        return _out(library().vckss_rust_engine_full_cmg_receipt_v1, "VckssFullCmgReceiptV1", self.generation)
+
+    def rhs_receipts(self, rows):
+        from . import abi_types
+
+        out = (abi_types.STRUCTS["VckssEngineRhsReceiptV1"] * rows)()
+        _check(library().vckss_rust_engine_rhs_receipts_v1(self.generation, out, rows))
+        return list(out)
+
     def release(self):
         if self.generation:
             _check(library().vckss_rust_engine_release_v1(self.generation))
@@ -742,6 +798,36 @@ class Engine:
             self.release()
         except Exception:
             pass
+
+
+def _out(fn, struct, *lead):
+    """Call an entry point that writes one header struct; return it."""
+    from . import abi_types
+
+    o = abi_types.new(struct)
+    _check(fn(*lead, C.byref(o), C.sizeof(o)))
+    return o
+
+
+def backend_capabilities():
+    return _out(library().vckss_rust_backend_capabilities_v1, "VckssBackendCapabilitiesV1")
+
+
+def request_capability(request):
+    """Request-capability receipt for a V1, V2 or V3 request structure."""
+    version = {"VckssBackendRequestCapabilityRequestV1": 1, "VckssBackendRequestCapabilityRequestV2": 2,
+               "VckssBackendRequestCapabilityRequestV3": 3}[type(request).__name__]
+    fn = getattr(library(), f"vckss_rust_backend_request_capability_v{version}")
+    from . import abi_types
+
+    o = abi_types.new(f"VckssBackendRequestCapabilityReceiptV{version}")
+    _check(fn(C.byref(request), C.byref(o), C.sizeof(o)))
+    return o
+
+
+def admit_prepare(request, probeorder_supplied=0):
+    """Admission of a VckssEnginePrepareRequestV4 before its data (0 when admitted)."""
+    return library().vckss_rust_engine_admit_prepare_v4(C.byref(request), probeorder_supplied)
 
 
 def chunk_table(ptr, chunk=64, group=1):

@@ -724,6 +724,11 @@ void relayout(Impl &E, uint32_t chunk) {
 
 }  // namespace
 
+void device_memory(uint64_t &live, uint64_t &peak) {
+  live = DeviceBytes::live.load();
+  peak = DeviceBytes::peak.load();
+}
+
 bool device_available() {
   int devices = 0;
   return hipGetDeviceCount(&devices) == hipSuccess && devices > 0;

@@ -87,6 +87,9 @@ struct SolveResult {
 
 bool device_available();
 
+// Live and peak bytes of the engine's device buffers in this process.
+void device_memory(uint64_t &live, uint64_t &peak);
+
 class DeviceEngine {
  public:
   DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h);

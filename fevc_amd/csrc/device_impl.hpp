@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <chrono>
 #include <cstdio>
@@ -32,6 +33,19 @@ constexpr uint32_t kChunkEntries = 64;  // default CSR entries or aggregate memb
 constexpr int kPollSlots = 8;
 constexpr int kPollLag = 2;
 
+// Live and peak bytes of the engine's device buffers (all DBufs of the
+// process), for the memory receipts.
+struct DeviceBytes {
+  static inline std::atomic<uint64_t> live{0}, peak{0};
+  static void add(uint64_t b) {
+    const uint64_t now = live.fetch_add(b) + b;
+    uint64_t prev = peak.load();
+    while (now > prev && !peak.compare_exchange_weak(prev, now)) {
+    }
+  }
+  static void sub(uint64_t b) { live.fetch_sub(b); }
+};
+
 template <typename T>
 struct DBuf {
   T *p = nullptr;
@@ -41,13 +55,26 @@ struct DBuf {
   DBuf &operator=(const DBuf &) = delete;
   DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
   DBuf &operator=(DBuf &&o) noexcept { std::swap(p, o.p); std::swap(n, o.n); return *this; }
-  ~DBuf() { if (p) (void)hipFree(p); }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      DeviceBytes::sub(std::max<size_t>(n, 1) * sizeof(T));
+    }
+    p = nullptr;
+    n = 0;
+  }
   void alloc(size_t count) {
     if (p && count <= n) return;
-    if (p) VK_HIP(hipFree(p));
+    if (p) {
+      VK_HIP(hipFree(p));
+      DeviceBytes::sub(std::max<size_t>(n, 1) * sizeof(T));
+    }
     p = nullptr;
-    n = count;
+    n = 0;
     VK_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = count;
+    DeviceBytes::add(std::max<size_t>(count, 1) * sizeof(T));
   }
   template <typename V>
   void upload(const std::vector<V> &v) {

@@ -15,6 +15,7 @@
 #include "host_prepare.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <numeric>
 
 #include "common.hpp"
@@ -456,6 +457,14 @@ void observation_plan(Prepared &P, std::vector<uint32_t> &order) {
 }  // namespace
 
 Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
+  using clock = std::chrono::steady_clock;
+  auto mark = clock::now();
+  auto lap = [&mark]() {
+    const auto now = clock::now();
+    const uint64_t ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(now - mark).count();
+    mark = now;
+    return ns;
+  };
   const uint64_t n = cols.rows;
   if (n == 0) fail(kInvalidInput, "ingest", "input sample is empty");
   Prepared P;
@@ -512,6 +521,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   }
   P.input_topology_checksum = fnv_checksum(n, cw.data(), cf.data(), cd.data(), cols.frequency);
   P.observation = observation;
+  P.canonicalize_ns = lap();
   if (mask) {
     P.mask.assign(mask, mask + n);
   } else if (observation) {
@@ -520,6 +530,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
     select(g, P.mask, P.selection);
   }
 
+  P.graph_ns = lap();
   // ---- compression (problem.rs:206-606)
   std::vector<uint64_t> kept;
   for (uint64_t r = 0; r < n; ++r)
@@ -632,6 +643,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   P.topology_checksum = fnv_checksum(R, P.row_worker.data(), P.row_firm.data(), P.row_deletion.data(),
                                      P.frequency.data());
 
+  P.compress_ns = lap();
   // ---- JLA plan (jla.rs)
   P.per_copy.resize(R);
   for (uint64_t i = 0; i < R; ++i) {
@@ -729,6 +741,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
   for (uint64_t s = 0; s < P.strata; ++s) P.cell_stratum_ptr[P.stratum_cell[s] + 1]++;
   for (uint64_t c = 0; c < C; ++c) P.cell_stratum_ptr[c + 1] += P.cell_stratum_ptr[c];
   if (!P.observation) P.match_units = P.units;
+  P.plan_ns = lap();
   return P;
 }
 

@@ -33,6 +33,9 @@ struct Prepared {
   uint64_t input_topology_checksum = 0;
   std::vector<uint8_t> mask;
   SelectionReceipt selection;
+  // wall time of the preparation phases (nanoseconds): validation and
+  // canonical ids, graph selection, compression, JLA plan
+  uint64_t canonicalize_ns = 0, graph_ns = 0, compress_ns = 0, plan_ns = 0;
   bool observation = false;  // observation deletion (else match deletion)
   bool stayers = false;      // a mover problem augmented with eligible stayers
   // units [0, match_units) are match deletion units; the rest are observation

@@ -11,6 +11,7 @@
 //   * requests are copied and size-checked before use; outputs carry a
 //     capacity that is checked before any write.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -42,9 +43,22 @@ struct Context {
   bool has_result = false;
   SolveResult result;
   StayerReceipt stayer_receipt;
-  uint64_t stayer_copy_bytes = 0;
+  uint64_t stayer_copy_bytes = 0, stayer_forecast = 0;
   uint32_t precision = 64, leverage_width = 32, target_width = 16;
+  // receipt inputs: the prepare request's memory fields and phase times
+  // (nanoseconds), the last solve's request, route and timing
+  uint64_t memory_limit_bytes = 0, caller_copy_bytes = 0, preparation_forecast = 0;
+  uint64_t ingest_ns = 0, hierarchy_ns = 0, hybrid_ns = 0, upload_ns = 0, stayer_ns = 0, solve_ns = 0;
+  VckssEngineSolveRequestV5 request{};
+  uint32_t engine_requested = 0, engine_selected = 0, algorithm_selected = 0;
+  bool full_cmg = false;
+  uint64_t counters_before[6] = {}, counters_after[6] = {};
 };
+
+uint64_t elapsed_ns(std::chrono::steady_clock::time_point t0) {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0)
+      .count();
+}
 
 std::mutex g_mutex;
 std::unique_ptr<Context> g_ctx;
@@ -105,27 +119,73 @@ std::vector<uint64_t> integer_column(const double *p, uint64_t rows, const char 
   return out;
 }
 
+// Admission of a V4 prepare request before its data
+// (ffi_engine.rs:3041-3056, 7392-7445): ABI, size, reserved and boolean
+// fields, deletion mode, the implicit-match preconditions, then the
+// whole-command memory admission of session.rs:202-290 under the legacy
+// budget: the caller's copy must be exactly 8 bytes per row and numeric
+// column (six base columns, the controls, the probe-order key), and that
+// copy plus the preparation charge (768 bytes a row, 32 a row and control,
+// 16 a row for a probe-order key or an implicit match key, 4096 fixed) must
+// fit the declared limit.  The engine's own restrictions come last:
+// controls, probe-order keys and implicit match are not run here.  Returns
+// the preparation peak forecast.
+uint64_t admit_prepare(const VckssEnginePrepareRequestV4 &o, uint32_t probeorder_supplied) {
+  if (probeorder_supplied > 1) fail(kInvalidInput, "engine_prepare", "probeorder_supplied must be zero or one");
+  if (o.v3.v2.abi_version != VCKSS_RUST_ABI_VERSION_V1)
+    fail(kAbiMismatch, "engine_prepare", "unsupported ABI version");
+  if (o.v3.v2.struct_size < sizeof(VckssEnginePrepareRequestV4))
+    fail(kAbiMismatch, "engine_prepare", "V4 prepare request reports a short structure size");
+  if (o.v3.v2.reserved || o.v3.reserved_2 || o.reserved_3 || o.implicit_match > 1)
+    fail(kAbiMismatch, "engine_prepare", "reserved or boolean V4 preparation fields are invalid");
+  if (o.v3.v2.cleanup_abandoned > 1) fail(kInvalidInput, "engine_prepare", "cleanup_abandoned must be zero or one");
+  if (o.v3.deletion_mode != VCKSS_DELETION_MATCH && o.v3.deletion_mode != VCKSS_DELETION_OBSERVATION)
+    fail(kInvalidInput, "engine_prepare", "unknown deletion mode");
+  if (o.implicit_match && (o.v3.deletion_mode != VCKSS_DELETION_MATCH || o.v3.controls_count != 0 ||
+                           probeorder_supplied != 1))
+    fail(kUnsupported, "engine_prepare",
+         "implicit-match preparation requires match deletion, no controls, and an explicit probe order");
+  const uint64_t rows = o.v3.v2.rows, limit = o.v3.v2.memory_limit_bytes;
+  if (rows == 0 || limit == 0)
+    fail(kInvalidInput, "engine_memory", "row count and whole-command memory limit must be positive");
+  const unsigned __int128 columns = 6u + (unsigned __int128)o.v3.controls_count + probeorder_supplied;
+  const unsigned __int128 copy = (unsigned __int128)rows * columns * 8u;
+  if (copy != o.v3.v2.caller_copy_bytes)
+    fail(kInvalidInput, "engine_memory",
+         "declared caller copy is " + std::to_string(o.v3.v2.caller_copy_bytes) + " bytes; " +
+             std::to_string((uint64_t)columns) + " columns require " + std::to_string((uint64_t)copy) + " bytes");
+  const unsigned __int128 charge = (unsigned __int128)rows * 768u +
+                                   (unsigned __int128)rows * o.v3.controls_count * 32u +
+                                   (unsigned __int128)rows * 16u * (probeorder_supplied + o.implicit_match) + 4096u;
+  const unsigned __int128 peak = copy + charge;
+  if (peak > UINT64_MAX) fail(kResourceLimit, "engine_memory", "preparation byte forecast overflow");
+  if (peak > limit)
+    fail(kResourceLimit, "engine_memory",
+         "simultaneous caller and engine preparation forecast " + std::to_string((uint64_t)peak) +
+             " bytes exceeds the declared limit " + std::to_string(limit) + " bytes");
+  if (o.v3.controls_count > 32) fail(kInvalidInput, "engine_prepare", "at most 32 controls");
+  if (o.v3.controls_count != 0)
+    fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
+  if (probeorder_supplied)
+    fail(kUnsupported, "engine_prepare", "probe-order keys are not supported by the MI355X engine");
+  if (o.implicit_match)
+    fail(kUnsupported, "engine_prepare", "the certified implicit-match route is not supported by the MI355X engine");
+  return (uint64_t)peak;
+}
+
 void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *request,
                   const VckssEngineColumnsV3 *columns) {
   if (!request || !columns) fail(kInvalidInput, "engine_prepare", "request or columns pointer is null");
   VckssEnginePrepareRequestInterruptV3 req;
   std::memcpy(&req, request, sizeof req);
   const auto &o = req.options;
-  if (o.v3.v2.abi_version != VCKSS_RUST_ABI_VERSION_V1)
-    fail(kAbiMismatch, "engine_prepare", "unsupported ABI version");
-  if (o.v3.v2.struct_size < sizeof(VckssEnginePrepareRequestV4))
-    fail(kAbiMismatch, "engine_prepare", "prepare request reports a short structure size");
-  if (o.v3.v2.reserved || o.v3.reserved_2 || o.reserved_3 || req.reserved)
-    fail(kAbiMismatch, "engine_prepare", "reserved fields must be zero");
-  if (o.v3.deletion_mode != VCKSS_DELETION_MATCH && o.v3.deletion_mode != VCKSS_DELETION_OBSERVATION)
-    fail(kInvalidInput, "engine_prepare", "unknown deletion mode");
-  if (o.v3.controls_count != 0 || columns->v2.controls_count != 0)
+  if (req.reserved) fail(kAbiMismatch, "engine_prepare", "reserved fields must be zero");
+  ctx.preparation_forecast = admit_prepare(o, columns->probeorder_supplied ? 1u : 0u);
+  if (columns->v2.controls_count != 0)
     fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
-  if (columns->probeorder_supplied)
-    fail(kUnsupported, "engine_prepare", "probe-order keys are not supported by the MI355X engine");
-  if (o.implicit_match > 1) fail(kInvalidInput, "engine_prepare", "implicit_match must be 0 or 1");
-  if (o.implicit_match)
-    fail(kUnsupported, "engine_prepare", "the certified implicit-match route is not supported by the MI355X engine");
+  ctx.memory_limit_bytes = o.v3.v2.memory_limit_bytes;
+  ctx.caller_copy_bytes = o.v3.v2.caller_copy_bytes;
+  const auto t_ingest = std::chrono::steady_clock::now();
   const VckssEngineColumnsV1 &c = columns->v2.v1;
   if (c.rows != o.v3.v2.rows) fail(kInvalidInput, "engine_prepare", "column rows differ from the request");
   const uint64_t rows = c.rows;
@@ -137,12 +197,17 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   auto frequency = integer_column(c.frequency, rows, "frequency weight", kInvalidWeight, false);
   if (!c.outcome || !c.target_weight) fail(kInvalidInput, "engine_ingest", "outcome or target column is null");
   Columns cols{rows, worker.data(), firm.data(), deletion.data(), c.outcome, frequency.data(), c.target_weight};
+  ctx.ingest_ns = elapsed_ns(t_ingest);
   ctx.P = prepare(cols, nullptr, o.v3.deletion_mode == VCKSS_DELETION_OBSERVATION);
   interrupt.checkpoint("engine_prepare_graph");
+  auto t = std::chrono::steady_clock::now();
   ctx.HS = build_hybrid(ctx.P);
+  ctx.hybrid_ns = elapsed_ns(t);
   interrupt.checkpoint("cmg_full_v2_hierarchy");
   CmgOptions copt;  // full_cmg.rs:467-476: max_levels 128
+  t = std::chrono::steady_clock::now();
   ctx.H = build_hierarchy(ctx.HS.graph, copt);
+  ctx.hierarchy_ns = elapsed_ns(t);
 }
 
 // Options of a point request on the CMG_FULL_V2 path (the V4 prefix shared
@@ -302,8 +367,24 @@ DeviceEngine &device_of(Context &ctx) {
 
 void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt);
 
+// What the receipts report about a solve: the request, the engine the path
+// resolves to (the compressed engine for match deletion without stayers, the
+// generic engine for observation deletion and the stayer hybrid, none for the
+// dense exact estimator; engine_plan.rs:243-290) and the algorithm.
+void record_request(Context &ctx, const VckssEngineSolveRequestV5 &q, const SolveOptions &o) {
+  ctx.request = q;
+  ctx.engine_requested = q.v4.v3.engine;
+  ctx.algorithm_selected = o.exact ? VCKSS_ALGORITHM_EXACT : VCKSS_ALGORITHM_JLA;
+  ctx.engine_selected = o.exact ? VCKSS_ENGINE_NOT_APPLICABLE
+                                : ((ctx.P.observation || ctx.P.stayers) ? VCKSS_ENGINE_GENERIC
+                                                                        : VCKSS_ENGINE_COMPRESSED);
+  ctx.full_cmg = !o.exact;
+}
+
 void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, Interrupt &interrupt) {
-  run_options(ctx, solve_options(q, ctx), interrupt);
+  const SolveOptions o = solve_options(q, ctx);
+  record_request(ctx, q, o);
+  run_options(ctx, o, interrupt);
 }
 
 // context.rs:150-205: a solve needs a prepared generation; a failed solve
@@ -320,6 +401,8 @@ void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt) {
   DeviceEngine &dev = device_of(ctx);
   ctx.state = kSolving;
   ctx.has_result = false;
+  const auto t = std::chrono::steady_clock::now();
+  dev.counters(ctx.counters_before);
   try {
     o.checkpoint("engine_solve");
     ctx.result = dev.run(o);
@@ -327,6 +410,8 @@ void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt) {
     ctx.state = kFailed;
     throw;
   }
+  dev.counters(ctx.counters_after);
+  ctx.solve_ns = elapsed_ns(t);
   ctx.has_result = true;
   ctx.state = kSolved;
 }
@@ -381,6 +466,529 @@ VckssEngineSolveRequestV5 default_solve_v5() {
   return q;
 }
 
+// ---------------------------------------------------- capability receipts
+// FNV-1a identity of a request (ffi_engine.rs:9114-9138, 9269-9302,
+// 9492-9530): a domain string, then every field's little-endian bytes.
+struct RequestHash {
+  uint64_t h = 0xcbf29ce484222325ULL;
+  void byte(uint8_t b) {
+    h ^= b;
+    h *= 0x00000100000001b3ULL;
+  }
+  void text(const char *s) {
+    for (; *s; ++s) byte((uint8_t)*s);
+  }
+  void u32(uint32_t v) {
+    for (int k = 0; k < 4; ++k) byte((uint8_t)(v >> (8 * k)));
+  }
+  void u64(uint64_t v) {
+    for (int k = 0; k < 8; ++k) byte((uint8_t)(v >> (8 * k)));
+  }
+};
+
+uint64_t signature_v1(const VckssBackendRequestCapabilityRequestV1 &r, const char *domain) {
+  RequestHash f;
+  f.text(domain);
+  for (uint32_t v : {r.request_schema, r.algorithm, r.deletion_mode, r.nuisance_mode, r.solver_route, r.rng_contract,
+                     r.controls_count, r.frequency_use})
+    f.u32(v);
+  return f.h;
+}
+uint64_t signature_v2_fields(RequestHash &f, const VckssBackendRequestCapabilityRequestV2 &r) {
+  for (uint32_t v : {r.v1.request_schema, r.v1.algorithm, r.v1.deletion_mode, r.v1.nuisance_mode, r.v1.solver_route,
+                     r.v1.rng_contract, r.v1.controls_count, r.v1.frequency_use, r.engine, r.batch_mode,
+                     r.stayers_mode, r.target_weight_mode, r.deletion_unit_source, r.probeorder_supplied,
+                     r.wallseconds_supplied})
+    f.u32(v);
+  return f.h;
+}
+uint64_t signature_v2(const VckssBackendRequestCapabilityRequestV2 &r) {
+  RequestHash f;
+  f.text("VCKSS-REQUEST-CAPABILITY-V2");
+  signature_v2_fields(f, r);
+  f.u64(r.physical_limit);
+  return f.h;
+}
+uint64_t signature_v3(const VckssBackendRequestCapabilityRequestV3 &r) {
+  RequestHash f;
+  f.text("VCKSS-REQUEST-CAPABILITY-V3");
+  signature_v2_fields(f, r.v2);
+  for (uint32_t v : {r.leverage_batch_mode, r.target_batch_mode, r.allow_automatic_cmg_setup_fallback}) f.u32(v);
+  f.u64(r.v2.physical_limit);
+  f.u64(bits_of(r.wallseconds));
+  return f.h;
+}
+
+constexpr uint64_t kMaxExactInteger = 1ULL << 53;
+
+bool known(uint32_t v, std::initializer_list<uint32_t> set) {
+  for (uint32_t s : set)
+    if (v == s) return true;
+  return false;
+}
+
+// The reference's schema-independent unknown-value checks
+// (ffi_engine.rs:9040-9075, 9140-9200, 9304-9400).
+uint32_t unknown_v1_fields(const VckssBackendRequestCapabilityRequestV1 &v) {
+  if (!known(v.algorithm, {VCKSS_ALGORITHM_AUTO, VCKSS_ALGORITHM_EXACT, VCKSS_ALGORITHM_JLA}))
+    return VCKSS_REQUEST_REASON_UNKNOWN_ALGORITHM;
+  if (!known(v.deletion_mode, {VCKSS_DELETION_MATCH, VCKSS_DELETION_OBSERVATION}))
+    return VCKSS_REQUEST_REASON_UNKNOWN_DELETION;
+  if (!known(v.nuisance_mode, {VCKSS_NUISANCE_JOINT, VCKSS_NUISANCE_FIXED_OFFSET}))
+    return VCKSS_REQUEST_REASON_UNKNOWN_NUISANCE;
+  if (!known(v.solver_route, {VCKSS_ROUTE_AUTO, VCKSS_ROUTE_EXACT, VCKSS_ROUTE_DIAGONAL_PCG, VCKSS_ROUTE_CMG_PCG}))
+    return VCKSS_REQUEST_REASON_UNKNOWN_SOLVER_ROUTE;
+  if (!known(v.rng_contract, {VCKSS_RNG_NONE, VCKSS_RNG_COUNTER_V1})) return VCKSS_REQUEST_REASON_UNKNOWN_RNG_CONTRACT;
+  if (!known(v.frequency_use, {VCKSS_REQUEST_FREQUENCY_UNIT, VCKSS_REQUEST_FREQUENCY_LITERAL}))
+    return VCKSS_REQUEST_REASON_UNKNOWN_FREQUENCY_USE;
+  return VCKSS_REQUEST_REASON_SUPPORTED;
+}
+
+// What this engine adds to a request the reference would support: it runs
+// no controls, no fixed-offset nuisance, JLA on the automatic or CMG route
+// only (CMG_FULL_V2; no dense-exact or diagonal-PCG probe solves), no
+// probe-order keys, and chooses its own batch widths.
+uint32_t engine_restriction(uint32_t algorithm, uint32_t nuisance, uint32_t route, uint32_t controls,
+                            uint32_t probeorder, bool explicit_batches) {
+  if (controls != 0)
+    return algorithm == VCKSS_ALGORITHM_JLA ? VCKSS_REQUEST_REASON_JLA_CONTROLS : VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
+  if (nuisance != VCKSS_NUISANCE_JOINT) return VCKSS_REQUEST_REASON_JLA_NUISANCE;
+  if (algorithm != VCKSS_ALGORITHM_EXACT && route != VCKSS_ROUTE_AUTO && route != VCKSS_ROUTE_CMG_PCG)
+    return VCKSS_REQUEST_REASON_JLA_GENERIC_SOLVER_ROUTE;
+  if (probeorder != 0) return VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+  if (explicit_batches) return VCKSS_REQUEST_REASON_BATCH_MODE_UNSUPPORTED;
+  return VCKSS_REQUEST_REASON_SUPPORTED;
+}
+
+// request_capability_classification (ffi_engine.rs:9040-9112) then the
+// engine's restrictions; the profile is EXACT_V1 or JLA_COUNTER_V1.
+std::pair<uint32_t, uint32_t> classify_v1(const VckssBackendRequestCapabilityRequestV1 &v) {
+  uint32_t reason = v.request_schema != VCKSS_REQUEST_CAPABILITY_SCHEMA_V1 ? VCKSS_REQUEST_REASON_UNKNOWN_SCHEMA
+                                                                           : unknown_v1_fields(v);
+  if (reason == VCKSS_REQUEST_REASON_SUPPORTED) {
+    if (v.algorithm == VCKSS_ALGORITHM_AUTO) reason = VCKSS_REQUEST_REASON_ALGORITHM_AUTO_UNRESOLVED;
+    else if (v.algorithm == VCKSS_ALGORITHM_EXACT && v.controls_count > 32) reason = VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
+    else if (v.algorithm == VCKSS_ALGORITHM_EXACT && v.rng_contract != VCKSS_RNG_NONE) reason = VCKSS_REQUEST_REASON_EXACT_RNG;
+    else if (v.algorithm == VCKSS_ALGORITHM_EXACT && !known(v.solver_route, {VCKSS_ROUTE_AUTO, VCKSS_ROUTE_EXACT}))
+      reason = VCKSS_REQUEST_REASON_EXACT_SOLVER_ROUTE;
+    else if (v.algorithm == VCKSS_ALGORITHM_JLA && v.deletion_mode != VCKSS_DELETION_MATCH)
+      reason = VCKSS_REQUEST_REASON_JLA_DELETION;
+    else if (v.algorithm == VCKSS_ALGORITHM_JLA && v.nuisance_mode != VCKSS_NUISANCE_JOINT)
+      reason = VCKSS_REQUEST_REASON_JLA_NUISANCE;
+    else if (v.algorithm == VCKSS_ALGORITHM_JLA && v.controls_count != 0) reason = VCKSS_REQUEST_REASON_JLA_CONTROLS;
+    else if (v.algorithm == VCKSS_ALGORITHM_JLA && v.rng_contract != VCKSS_RNG_COUNTER_V1)
+      reason = VCKSS_REQUEST_REASON_JLA_RNG;
+    else
+      reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, 0, false);
+  }
+  const uint32_t profile = reason != VCKSS_REQUEST_REASON_SUPPORTED ? VCKSS_REQUEST_PROFILE_NONE
+                           : v.algorithm == VCKSS_ALGORITHM_EXACT  ? VCKSS_REQUEST_PROFILE_EXACT_V1
+                                                                   : VCKSS_REQUEST_PROFILE_JLA_COUNTER_V1;
+  return {reason, profile};
+}
+
+// request_capability_classification_v2 (ffi_engine.rs:9140-9267), then the
+// engine's restrictions.  The reference's generic engine runs JLA on the
+// diagonal route only; this engine's generic JLA (observation deletion)
+// runs on CMG_FULL_V2, so the generic branch takes the CMG route here.
+std::pair<uint32_t, uint32_t> classify_v2(const VckssBackendRequestCapabilityRequestV2 &r) {
+  const auto &v = r.v1;
+  uint32_t reason = v.request_schema != VCKSS_REQUEST_CAPABILITY_SCHEMA_V2 ? VCKSS_REQUEST_REASON_UNKNOWN_SCHEMA
+                                                                           : unknown_v1_fields(v);
+  if (reason != VCKSS_REQUEST_REASON_SUPPORTED) {
+  } else if (!known(r.engine, {VCKSS_ENGINE_AUTO_OR_UNSPECIFIED, VCKSS_ENGINE_COMPRESSED, VCKSS_ENGINE_GENERIC})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_ENGINE;
+  } else if (!known(r.batch_mode, {VCKSS_BATCH_MODE_AUTO, VCKSS_BATCH_MODE_EXPLICIT})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_BATCH_MODE;
+  } else if (!known(r.stayers_mode, {VCKSS_STAYERS_MOVERS, VCKSS_STAYERS_ALL})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_STAYERS_MODE;
+  } else if (!known(r.target_weight_mode,
+                    {VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT, VCKSS_TARGET_WEIGHT_STORED_ROW_EXPLICIT})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_TARGET_WEIGHT_MODE;
+  } else if (!known(r.deletion_unit_source, {VCKSS_DELETION_SOURCE_CELL_DEFAULT, VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT,
+                                             VCKSS_DELETION_SOURCE_OBSERVATION_ROW})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_DELETION_UNIT_SOURCE;
+  } else if (r.probeorder_supplied > 1) {
+    reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+  } else if (r.wallseconds_supplied > 1) {
+    reason = VCKSS_REQUEST_REASON_WALLSECONDS_UNSUPPORTED;
+  } else if (r.physical_limit == 0 || r.physical_limit > kMaxExactInteger) {
+    reason = VCKSS_REQUEST_REASON_PHYSICAL_LIMIT;
+  } else if (v.algorithm == VCKSS_ALGORITHM_AUTO) {
+    reason = VCKSS_REQUEST_REASON_ALGORITHM_AUTO_UNRESOLVED;
+  } else if (v.controls_count > 32) {
+    reason = VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
+  } else if (v.algorithm == VCKSS_ALGORITHM_EXACT) {
+    if (r.engine == VCKSS_ENGINE_COMPRESSED) reason = VCKSS_REQUEST_REASON_EXACT_ENGINE;
+    else if (v.rng_contract != VCKSS_RNG_NONE) reason = VCKSS_REQUEST_REASON_EXACT_RNG;
+    else if (!known(v.solver_route, {VCKSS_ROUTE_AUTO, VCKSS_ROUTE_EXACT})) reason = VCKSS_REQUEST_REASON_EXACT_SOLVER_ROUTE;
+  } else if (r.engine == VCKSS_ENGINE_GENERIC) {
+    const bool source_ok = (v.deletion_mode == VCKSS_DELETION_MATCH &&
+                            known(r.deletion_unit_source,
+                                  {VCKSS_DELETION_SOURCE_CELL_DEFAULT, VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT})) ||
+                           (v.deletion_mode == VCKSS_DELETION_OBSERVATION &&
+                            r.deletion_unit_source == VCKSS_DELETION_SOURCE_OBSERVATION_ROW);
+    if (v.rng_contract != VCKSS_RNG_COUNTER_V1) reason = VCKSS_REQUEST_REASON_JLA_RNG;
+    else if (!known(v.solver_route, {VCKSS_ROUTE_AUTO, VCKSS_ROUTE_CMG_PCG}))
+      reason = VCKSS_REQUEST_REASON_JLA_GENERIC_SOLVER_ROUTE;
+    else if (r.probeorder_supplied != 0) reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+    else if (r.wallseconds_supplied != 0) reason = VCKSS_REQUEST_REASON_WALLSECONDS_UNSUPPORTED;
+    else if (!source_ok) reason = VCKSS_REQUEST_REASON_DELETION_UNIT_SOURCE_MISMATCH;
+  } else {
+    VckssBackendRequestCapabilityRequestV1 legacy = v;
+    legacy.request_schema = VCKSS_REQUEST_CAPABILITY_SCHEMA_V1;
+    const uint32_t lr = classify_v1(legacy).first;
+    reason = (r.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED && lr != VCKSS_REQUEST_REASON_SUPPORTED)
+                 ? VCKSS_REQUEST_REASON_JLA_ENGINE_AUTO_UNRESOLVED
+                 : lr;
+  }
+  if (reason == VCKSS_REQUEST_REASON_SUPPORTED)
+    reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, r.probeorder_supplied,
+                                r.batch_mode != VCKSS_BATCH_MODE_AUTO);
+  const uint32_t profile = reason != VCKSS_REQUEST_REASON_SUPPORTED     ? VCKSS_REQUEST_PROFILE_NONE
+                           : v.algorithm == VCKSS_ALGORITHM_EXACT      ? VCKSS_REQUEST_PROFILE_EXACT_V1
+                           : r.engine == VCKSS_ENGINE_GENERIC          ? VCKSS_REQUEST_PROFILE_JLA_GENERIC_COUNTER_V1
+                                                                       : VCKSS_REQUEST_PROFILE_JLA_COUNTER_V1;
+  return {reason, profile};
+}
+
+// request_capability_classification_v3 (ffi_engine.rs:9304-9490), then the
+// engine's restrictions; a supported request has the planned profile.
+std::pair<uint32_t, uint32_t> classify_v3(const VckssBackendRequestCapabilityRequestV3 &r) {
+  const auto &v = r.v2.v1;
+  const auto &q = r.v2;
+  const bool phases_known = known(r.leverage_batch_mode, {VCKSS_BATCH_MODE_AUTO, VCKSS_BATCH_MODE_EXPLICIT}) &&
+                            known(r.target_batch_mode, {VCKSS_BATCH_MODE_AUTO, VCKSS_BATCH_MODE_EXPLICIT});
+  const uint32_t summary =
+      (r.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO && r.target_batch_mode == VCKSS_BATCH_MODE_AUTO)
+          ? VCKSS_BATCH_MODE_AUTO
+      : (r.leverage_batch_mode == VCKSS_BATCH_MODE_EXPLICIT && r.target_batch_mode == VCKSS_BATCH_MODE_EXPLICIT)
+          ? VCKSS_BATCH_MODE_EXPLICIT
+          : VCKSS_BATCH_MODE_INDEPENDENT;
+  const bool source_ok =
+      (v.deletion_mode == VCKSS_DELETION_MATCH &&
+       known(q.deletion_unit_source, {VCKSS_DELETION_SOURCE_CELL_DEFAULT, VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT})) ||
+      (v.deletion_mode == VCKSS_DELETION_OBSERVATION && q.deletion_unit_source == VCKSS_DELETION_SOURCE_OBSERVATION_ROW);
+  const bool all = q.stayers_mode == VCKSS_STAYERS_ALL;
+  uint32_t reason = v.request_schema != VCKSS_REQUEST_CAPABILITY_SCHEMA_V3 ? VCKSS_REQUEST_REASON_UNKNOWN_SCHEMA
+                                                                           : unknown_v1_fields(v);
+  if (reason != VCKSS_REQUEST_REASON_SUPPORTED) {
+  } else if (!known(q.engine, {VCKSS_ENGINE_AUTO_OR_UNSPECIFIED, VCKSS_ENGINE_COMPRESSED, VCKSS_ENGINE_GENERIC})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_ENGINE;
+  } else if (!phases_known) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_BATCH_MODE;
+  } else if (!known(q.batch_mode, {VCKSS_BATCH_MODE_AUTO, VCKSS_BATCH_MODE_EXPLICIT, VCKSS_BATCH_MODE_INDEPENDENT}) ||
+             q.batch_mode != summary) {
+    reason = VCKSS_REQUEST_REASON_BATCH_SUMMARY_MISMATCH;
+  } else if (!known(q.stayers_mode, {VCKSS_STAYERS_MOVERS, VCKSS_STAYERS_ALL})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_STAYERS_MODE;
+  } else if (!known(q.target_weight_mode,
+                    {VCKSS_TARGET_WEIGHT_FREQUENCY_DEFAULT, VCKSS_TARGET_WEIGHT_STORED_ROW_EXPLICIT})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_TARGET_WEIGHT_MODE;
+  } else if (!known(q.deletion_unit_source, {VCKSS_DELETION_SOURCE_CELL_DEFAULT, VCKSS_DELETION_SOURCE_MATCH_ID_EXPLICIT,
+                                             VCKSS_DELETION_SOURCE_OBSERVATION_ROW})) {
+    reason = VCKSS_REQUEST_REASON_UNKNOWN_DELETION_UNIT_SOURCE;
+  } else if (!source_ok) {
+    reason = VCKSS_REQUEST_REASON_DELETION_UNIT_SOURCE_MISMATCH;
+  } else if (q.probeorder_supplied > 1) {
+    reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+  } else if (q.wallseconds_supplied > 1 || (q.wallseconds_supplied == 0 && r.wallseconds != 0.0) ||
+             (q.wallseconds_supplied == 1 && (!std::isfinite(r.wallseconds) || r.wallseconds <= 0.0))) {
+    reason = VCKSS_REQUEST_REASON_WALLSECONDS_VALUE;
+  } else if (r.allow_automatic_cmg_setup_fallback > 1 ||
+             (r.allow_automatic_cmg_setup_fallback == 1 && v.solver_route != VCKSS_ROUTE_AUTO)) {
+    reason = VCKSS_REQUEST_REASON_FALLBACK_ROUTE_MISMATCH;
+  } else if (q.physical_limit == 0 || q.physical_limit > kMaxExactInteger) {
+    reason = VCKSS_REQUEST_REASON_PHYSICAL_LIMIT;
+  } else if (v.controls_count > 32) {
+    reason = VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
+  } else if (v.algorithm == VCKSS_ALGORITHM_EXACT) {
+    if (q.engine == VCKSS_ENGINE_COMPRESSED) reason = VCKSS_REQUEST_REASON_EXACT_ENGINE;
+    else if (all && v.deletion_mode != VCKSS_DELETION_MATCH) reason = VCKSS_REQUEST_REASON_STAYERS_MODE_UNSUPPORTED;
+    else if (all && q.probeorder_supplied != 0) reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+    else if (all && q.wallseconds_supplied != 0) reason = VCKSS_REQUEST_REASON_WALLSECONDS_UNSUPPORTED;
+    else if (v.rng_contract != VCKSS_RNG_NONE) reason = VCKSS_REQUEST_REASON_EXACT_RNG;
+    else if (!known(v.solver_route, {VCKSS_ROUTE_AUTO, VCKSS_ROUTE_EXACT})) reason = VCKSS_REQUEST_REASON_EXACT_SOLVER_ROUTE;
+  } else if (v.algorithm == VCKSS_ALGORITHM_AUTO) {
+    if (all && v.deletion_mode != VCKSS_DELETION_MATCH) reason = VCKSS_REQUEST_REASON_STAYERS_MODE_UNSUPPORTED;
+    else if (all && q.engine == VCKSS_ENGINE_COMPRESSED) reason = VCKSS_REQUEST_REASON_COMPRESSED_SCIENTIFIC_INELIGIBILITY;
+    else if (all && q.probeorder_supplied != 0) reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+    else if (all && q.wallseconds_supplied != 0) reason = VCKSS_REQUEST_REASON_WALLSECONDS_UNSUPPORTED;
+    else if (v.rng_contract != VCKSS_RNG_COUNTER_V1) reason = VCKSS_REQUEST_REASON_AUTO_RNG;
+    else if (v.solver_route != VCKSS_ROUTE_AUTO) reason = VCKSS_REQUEST_REASON_EXACT_SOLVER_ROUTE;
+    else if (q.engine == VCKSS_ENGINE_COMPRESSED) reason = VCKSS_REQUEST_REASON_AUTO_ENGINE_COMPRESSED;
+  } else if (all && v.deletion_mode != VCKSS_DELETION_MATCH) {
+    reason = VCKSS_REQUEST_REASON_STAYERS_MODE_UNSUPPORTED;
+  } else if (all && q.engine == VCKSS_ENGINE_COMPRESSED) {
+    reason = VCKSS_REQUEST_REASON_COMPRESSED_SCIENTIFIC_INELIGIBILITY;
+  } else if (all && q.probeorder_supplied != 0) {
+    reason = VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
+  } else if (all && q.wallseconds_supplied != 0) {
+    reason = VCKSS_REQUEST_REASON_WALLSECONDS_UNSUPPORTED;
+  } else if (v.rng_contract != VCKSS_RNG_COUNTER_V1) {
+    reason = VCKSS_REQUEST_REASON_JLA_RNG;
+  } else if (v.solver_route == VCKSS_ROUTE_EXACT) {
+    reason = VCKSS_REQUEST_REASON_JLA_GENERIC_SOLVER_ROUTE;
+  } else if (q.engine == VCKSS_ENGINE_COMPRESSED &&
+             (v.deletion_mode != VCKSS_DELETION_MATCH || v.controls_count != 0)) {
+    reason = VCKSS_REQUEST_REASON_COMPRESSED_SCIENTIFIC_INELIGIBILITY;
+  }
+  if (reason == VCKSS_REQUEST_REASON_SUPPORTED)
+    reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, q.probeorder_supplied,
+                                r.leverage_batch_mode != VCKSS_BATCH_MODE_AUTO ||
+                                    r.target_batch_mode != VCKSS_BATCH_MODE_AUTO);
+  return {reason, reason == VCKSS_REQUEST_REASON_SUPPORTED ? VCKSS_REQUEST_PROFILE_PLANNED_V1 : VCKSS_REQUEST_PROFILE_NONE};
+}
+
+// ------------------------------------------------------------- receipts
+// Byte model of a prepared generation (host arrays, hybrid graph, CMG
+// hierarchy with its device layout) and of one solve batch's vectors, for
+// the memory, preparation and full-CMG receipts.
+struct MemoryModel {
+  uint64_t host_problem = 0, graph = 0, hierarchy = 0, plan = 0, workspace = 0, device_live = 0, device_peak = 0;
+};
+
+MemoryModel memory_model(const Context &ctx) {
+  MemoryModel m;
+  const Prepared &P = ctx.P;
+  m.host_problem = P.rows * (4 * 4 + 8 * 3 + 8) + P.cells * (4 * 2 + 8 * 3 + 8 * 2) + P.units * (4 + 8 * 4 + 8 * 2) +
+                   P.strata * (4 + 8 * 4) + P.input_rows;
+  m.graph = (uint64_t)ctx.HS.graph.u.size() * 16 + (uint64_t)ctx.HS.graph.n * 8;
+  for (const Level &L : ctx.H.levels) {
+    const uint64_t nnz = 2 * (uint64_t)L.graph.u.size() + L.graph.n;
+    m.hierarchy += nnz * (4 + 8 + 4) + (uint64_t)L.graph.n * (8 + 4 + 4 + 4);
+    m.plan += (nnz / 16 + L.graph.n / 16 + 2) * 16 * 2;
+  }
+  m.hierarchy += (uint64_t)ctx.H.terminal_inverse.size() * 8;
+  const uint64_t K = 64, n0 = ctx.H.levels.empty() ? 0 : ctx.H.levels[0].graph.n;
+  m.workspace = K * 8 * (12 * n0 + 6 * (P.workers + P.firms)) + K * 4 * P.units;
+  device_memory(m.device_live, m.device_peak);
+  return m;
+}
+
+// One row per solved right-hand side: the fit, then every leverage probe,
+// then the worker and firm side of every target probe (ffi_engine.rs:8865-8879,
+// 9005-9025).
+std::vector<VckssEngineRhsReceiptV1> rhs_rows(const Context &ctx) {
+  std::vector<VckssEngineRhsReceiptV1> rows;
+  if (!ctx.has_result || !ctx.full_cmg) return rows;
+  const SolveResult &r = ctx.result;
+  auto row = [&](uint32_t phase, uint32_t side, int64_t probe, const ColumnReceipt &c) {
+    VckssEngineRhsReceiptV1 v;
+    std::memset(&v, 0, sizeof v);
+    v.phase = phase;
+    v.side = side;
+    v.probe = probe;
+    v.route = VCKSS_ROUTE_CMG_PCG;
+    v.iterations = c.iterations;
+    v.zero_rhs = c.rhs_norm == 0.0 ? 1u : 0u;
+    v.reduced_residual = c.reduced;
+    v.complete_residual = c.complete;
+    rows.push_back(v);
+  };
+  row(1, 0, -1, r.fit);
+  for (size_t t = 0; t < r.leverage.size(); ++t) row(2, 0, (int64_t)t, r.leverage[t]);
+  for (size_t c = 0; c < r.target.size(); ++c) row(3, 1 + (uint32_t)(c % 2), (int64_t)(c / 2), r.target[c]);
+  return rows;
+}
+
+const Context &solved(uint64_t generation) {
+  const Context &ctx = live(generation);
+  if (!ctx.has_result) fail(kStaleContext, "engine_receipt", "no solved result is available");
+  return ctx;
+}
+
+VckssEngineDetailedReceiptV6 detailed_v6(uint64_t generation, const Context &ctx) {
+  const SolveResult &r = ctx.result;
+  const auto &q = ctx.request;
+  const auto &v1 = q.v4.v3.v2.v1;
+  const MemoryModel m = memory_model(ctx);
+  VckssEngineDetailedReceiptV6 o;
+  std::memset(&o, 0, sizeof o);
+  VckssEngineDetailedReceiptV2 &d = o.v5.v4.v3.v2;
+  d.struct_size = sizeof(VckssEngineDetailedReceiptV6);
+  d.generation = generation;
+  d.seed = v1.seed;
+  d.probes_requested = v1.probes;
+  d.leverage_probes_accepted = (uint32_t)r.leverage.size();
+  d.target_probes_accepted = (uint32_t)(r.target.size() / 2);
+  d.solver_requested = v1.solver_route;
+  d.solver_selected = ctx.full_cmg ? VCKSS_ROUTE_CMG_PCG : VCKSS_ROUTE_EXACT;
+  d.solver_dimension = ctx.full_cmg ? ctx.HS.graph.n : ctx.P.workers + ctx.P.firms - 1;
+  d.leverage_batch_width = ctx.full_cmg ? ctx.leverage_width : 0;
+  d.target_batch_width = ctx.full_cmg ? ctx.target_width : 0;
+  d.rank_tolerance = v1.rank_tolerance;
+  d.block_tolerance = v1.block_tolerance;
+  const double probe_tol = q.tolerance_supplied ? v1.pcg_tolerance : 1e-6;
+  d.full_residual_tolerance = std::max(10.0 * std::max(v1.pcg_tolerance, probe_tol), 1e-11);
+  d.full_fit_route = d.solver_selected;
+  d.full_fit_iterations = r.fit.iterations;
+  d.full_fit_reduced_residual = r.fit.reduced;
+  d.full_fit_complete_residual = r.fit.complete;
+  d.full_fit_zero_rhs = r.fit.rhs_norm == 0.0 && ctx.full_cmg;
+  d.leverage_rhs_count = r.leverage.size();
+  d.target_rhs_count = r.target.size();
+  double max_red = r.fit.reduced, max_comp = r.max_complete_residual;
+  for (const auto *set : {&r.leverage, &r.target})
+    for (const ColumnReceipt &c : *set) max_red = std::max(max_red, c.reduced);
+  d.max_reduced_residual = max_red;
+  d.max_complete_residual = max_comp;
+  double max_lev = r.has_exact ? r.exact.max_leverage : 0.0, max_recip = 0.0;
+  for (double p : r.projection_share) max_lev = std::max(max_lev, p);
+  for (double m2 : r.residual_share)
+    if (m2 > 0.0) max_recip = std::max(max_recip, 1.0 / m2);
+  d.max_leverage = max_lev;
+  d.max_reciprocal_residual = max_recip;
+  d.accounting_residual = r.accounting_residual;
+  d.topology_checksum = ctx.P.topology_checksum;
+  d.cmg_levels = ctx.H.levels.size();
+  if (!ctx.H.levels.empty()) {
+    d.cmg_fine_vertices = ctx.H.levels[0].graph.n;
+    d.cmg_fine_edges = ctx.H.levels[0].graph.u.size();
+    d.cmg_terminal_vertices = ctx.H.levels.back().graph.n;
+    double e = 0.0, n = 0.0;
+    for (const Level &L : ctx.H.levels) {
+      e += (double)L.graph.u.size();
+      n += (double)L.graph.n;
+    }
+    d.cmg_edge_complexity = e / std::max<double>(1.0, (double)d.cmg_fine_edges);
+    d.cmg_vertex_complexity = n / std::max<double>(1.0, (double)d.cmg_fine_vertices);
+  }
+  d.cmg_structural_bytes = m.graph + m.hierarchy;
+  d.cmg_workspace_bytes = m.workspace;
+  d.cmg_dense_factor_bytes = (uint64_t)ctx.H.terminal_inverse.size() * 8;
+  d.full_fit_weighted_rss = r.weighted_rss;
+  d.memory_limit_bytes = ctx.memory_limit_bytes;
+  d.caller_copy_bytes = ctx.caller_copy_bytes;
+  d.preparation_peak_forecast_bytes = ctx.preparation_forecast;
+  d.prepared_resident_bytes = m.host_problem + m.graph + m.hierarchy + m.plan;
+  d.solver_setup_forecast_bytes = m.graph + m.hierarchy + m.plan;
+  d.leverage_phase_forecast_bytes = m.workspace;
+  d.target_phase_forecast_bytes = m.workspace;
+  d.result_forecast_bytes = (ctx.P.cells * 2 + ctx.P.units * 6) * 8 + r.draws.size() * 8;
+  d.solve_peak_forecast_bytes = std::max(m.device_peak, d.solver_setup_forecast_bytes + m.workspace);
+  d.command_peak_forecast_bytes = d.prepared_resident_bytes + d.solve_peak_forecast_bytes + d.result_forecast_bytes;
+  VckssEngineDetailedReceiptV3 &d3 = o.v5.v4.v3;
+  d3.rng_contract = v1.rng_contract;
+  const uint64_t rows = rhs_rows(ctx).size();
+  d3.rhs_receipt_rows = rows;
+  d3.caller_result_copy_bytes = rows * (sizeof(VckssEngineRhsReceiptV1) + 8 * sizeof(double));
+  VckssEngineDetailedReceiptV4 &d4 = o.v5.v4;
+  d4.algorithm_requested = q.v4.v3.v2.algorithm;
+  d4.algorithm_selected = ctx.algorithm_selected;
+  d4.deletion_mode = v1.deletion_mode;
+  d4.nuisance_mode = q.v4.v3.v2.nuisance_mode;
+  d4.parameters = d4.full_parameters = d4.correction_parameters = ctx.P.workers + ctx.P.firms - 1;
+  if (r.has_exact) {
+    d4.information_rcond = r.exact.information_rcond;
+    d4.inverse_relres = r.exact.inverse_relres;
+  }
+  VckssEngineDetailedReceiptV5 &d5 = o.v5;
+  d5.working_fit_complete_residual = r.fit.complete;
+  if (r.has_exact) {
+    d5.deletion_rank_gap = r.exact.deletion_rank_gap;
+    d5.firm_zero_sum_residual = r.exact.firm_zero_sum_residual;
+  }
+  d5.fit_peak_forecast_bytes = d.solver_setup_forecast_bytes + m.workspace;
+  d5.correction_peak_forecast_bytes = d.solve_peak_forecast_bytes;
+  d5.actual_accounting_residual = r.accounting_residual;
+  o.engine_requested = ctx.engine_requested;
+  o.engine_selected = ctx.engine_selected;
+  o.controls_count = 0;
+  o.rhs_receipt_schema = rows ? 1u : 0u;
+  o.full_joint_fit_complete_residual = r.fit.complete;
+  o.generic_working_fit_complete_residual = r.fit.complete;
+  o.capability_schema = q.v4.v3.capability_schema;
+  o.capability_profile = q.v4.v3.capability_profile;
+  o.batch_mode = q.v4.v3.batch_mode;
+  o.stayers_mode = q.v4.v3.stayers_mode;
+  o.target_weight_mode = q.v4.v3.target_weight_mode;
+  o.deletion_unit_source = q.v4.v3.deletion_unit_source;
+  o.probeorder_supplied = q.v4.v3.probeorder_supplied;
+  o.wallseconds_supplied = q.v4.v3.wallseconds_supplied;
+  o.frequency_use = q.v4.v3.frequency_use;
+  o.physical_limit = q.v4.v3.physical_limit;
+  o.request_signature = q.v4.v3.request_signature;
+  return o;
+}
+
+VckssExecutionPlanReceiptV1 execution_plan(uint64_t generation, const Context &ctx) {
+  const SolveResult &r = ctx.result;
+  const auto &q = ctx.request;
+  const auto &v1 = q.v4.v3.v2.v1;
+  const MemoryModel m = memory_model(ctx);
+  VckssExecutionPlanReceiptV1 e;
+  std::memset(&e, 0, sizeof e);
+  e.struct_size = sizeof e;
+  e.schema_version = 1;
+  e.generation = generation;
+  e.request_signature = q.v4.v3.request_signature;
+  auto &res = e.resolution;
+  res.algorithm_schema = res.engine_schema = 1;
+  res.algorithm_requested = q.v4.v3.v2.algorithm;
+  res.algorithm_selected = ctx.algorithm_selected;
+  res.engine_requested = ctx.engine_requested;
+  res.engine_selected = ctx.engine_selected;
+  res.compressed_eligibility = (!ctx.P.observation && !ctx.P.stayers) ? 1u : 0u;
+  res.resolved_before_rng = 1;
+  res.identified_complexity = ctx.P.workers + ctx.P.firms - 1;
+  res.exact_limit = q.v4.v3.v2.exact_estimator_limit;
+  auto &sol = e.solver;
+  sol.schema_version = 1;
+  sol.requested_route = v1.solver_route;
+  sol.selected_route = ctx.full_cmg ? VCKSS_ROUTE_CMG_PCG : VCKSS_ROUTE_EXACT;
+  sol.full_setup_complete = 1;
+  sol.plan_frozen_before_rng = 1;
+  sol.threads_requested = sol.threads_used = std::max<uint32_t>(q.threads, 1);
+  sol.planned_rhs = 1 + r.leverage.size() + r.target.size();
+  sol.full_solver_dimension = ctx.full_cmg ? ctx.HS.graph.n : ctx.P.workers + ctx.P.firms - 1;
+  auto &b = e.batch;
+  b.schema_version = 1;
+  b.deterministic = 1;
+  b.width_invariance_required = 1;
+  b.whole_command_admitted = 1;
+  b.applicability = ctx.full_cmg ? VCKSS_PLAN_APPLICABILITY_COMPRESSED : VCKSS_PLAN_APPLICABILITY_EXACT;
+  b.non_batched_peak_bytes = m.graph + m.hierarchy + m.plan;
+  b.selected_command_peak_bytes = b.non_batched_peak_bytes + m.workspace;
+  for (int side = 0; side < 2; ++side) {
+    auto &ph = side ? b.target : b.leverage;
+    ph.request_mode = side ? q.v4.target_batch_mode : q.v4.leverage_batch_mode;
+    ph.selection_reason = ctx.full_cmg ? VCKSS_BATCH_SELECTION_AUTO : VCKSS_BATCH_SELECTION_NOT_APPLICABLE;
+    ph.applicability = b.applicability;
+    ph.selected_width = ctx.full_cmg ? (side ? ctx.target_width : ctx.leverage_width) : 0;
+    ph.probe_width_cap = v1.probes;
+    ph.declared_threads = sol.threads_requested;
+    ph.route_width_cap = ph.effective_width_cap = side ? 32 : 64;
+    ph.hard_memory_bytes = ctx.memory_limit_bytes;
+    ph.selected_forecast_bytes = m.workspace;
+  }
+  e.wall.schema_version = 1;
+  e.wall.requested_applicable = q.v4.v3.wallseconds_supplied;
+  e.wall.requested_seconds = q.v4.wallseconds;
+  auto &c = e.counter;
+  c.schema_version = 1;
+  c.rng_contract = v1.rng_contract;
+  c.completed = 1;
+  c.leverage.planned_logical_atoms = c.leverage.actual_logical_atoms = ctx.P.units * (uint64_t)r.leverage.size();
+  c.target.planned_logical_atoms = c.target.actual_logical_atoms = ctx.P.strata * (uint64_t)(r.target.size() / 2);
+  c.total.planned_logical_atoms = c.total.actual_logical_atoms =
+      c.leverage.actual_logical_atoms + c.target.actual_logical_atoms;
+  auto &mem = e.memory;
+  mem.schema_version = 1;
+  mem.applicability = 1;
+  mem.hard_limit_bytes = ctx.memory_limit_bytes;
+  mem.prepared_persistent_bytes = m.host_problem;
+  mem.setup_peak_bytes = m.graph + m.hierarchy + m.plan;
+  mem.leverage_peak_bytes = mem.target_peak_bytes = mem.setup_peak_bytes + m.workspace;
+  mem.fit_peak_bytes = mem.leverage_peak_bytes;
+  mem.command_peak_bytes = m.host_problem + mem.leverage_peak_bytes;
+  mem.non_batched_peak_bytes = mem.setup_peak_bytes;
+  mem.shared_cmg_persistent_bytes = m.hierarchy;
+  mem.cmg_preconditioner_workspace_bytes = m.workspace;
+  mem.cmg_hybrid_graph_bytes = m.graph;
+  return e;
+}
+
 }  // namespace
 
 namespace vk {
@@ -424,8 +1032,7 @@ int32_t vckss_rust_engine_default_prepare_request_interrupt_v3(VckssEnginePrepar
     VckssEnginePrepareRequestInterruptV3 d;
     std::memset(&d, 0, sizeof d);
     d.options.v3.v2.abi_version = VCKSS_RUST_ABI_VERSION_V1;
-    d.options.v3.v2.struct_size = sizeof(VckssEnginePrepareRequestV4);
-    d.options.v3.v2.memory_limit_bytes = 4ULL << 30;
+    d.options.v3.v2.struct_size = sizeof(VckssEnginePrepareRequestInterruptV3);  // ffi_engine.rs:1023-1036
     d.options.v3.deletion_mode = VCKSS_DELETION_MATCH;
     d.checkpoint_interval = 0;  // no callback (ffi_engine.rs:520-540)
     std::memcpy(output, &d, sizeof d);
@@ -460,7 +1067,9 @@ static int32_t prepare_common(const VckssEnginePrepareRequestInterruptV3 *reques
     }
     std::unique_ptr<Context> ctx(new Context);
     prepare_into(*ctx, request, columns);
+    const auto t = std::chrono::steady_clock::now();
     if (upload) ctx->dev.reset(new DeviceEngine(ctx->P, ctx->HS, ctx->H));
+    ctx->upload_ns = elapsed_ns(t);
     ctx->generation = g_next_generation++;
     ctx->state = kPrepared;
     *output_handle = ctx->generation;
@@ -522,10 +1131,29 @@ static void augment_stayers_value(uint64_t generation, const VckssStayerAugmenta
   if (ctx.state != kPrepared)
     fail(kContextPoisoned, "context_augment", "generation " + std::to_string(generation) +
                                                   " is not in the Prepared state");
+  // session.rs:309-372 under the legacy budget: the stayer copy is exactly 8
+  // bytes per row and column (five columns), and the prepared generation's
+  // resident bytes plus that copy plus 768 bytes a combined row and 4096
+  // fixed must fit the limit the generation was prepared under.
+  if (r.caller_copy_bytes != rows * 5u * 8u)
+    fail(kInvalidInput, "stayer_augmentation_memory",
+         "declared stayer caller copy is " + std::to_string(r.caller_copy_bytes) + " bytes; 5 columns require " +
+             std::to_string(rows * 40u) + " bytes");
+  {
+    const MemoryModel m = memory_model(ctx);
+    const uint64_t resident = m.host_problem + m.graph + m.hierarchy;
+    const uint64_t peak = resident + r.caller_copy_bytes + (ctx.P.input_rows + rows) * 768u + 4096u;
+    if (peak > ctx.memory_limit_bytes)
+      fail(kResourceLimit, "stayer_augmentation_memory",
+           "stayer augmentation forecast " + std::to_string(peak) + " bytes exceeds the declared limit " +
+               std::to_string(ctx.memory_limit_bytes) + " bytes");
+    ctx.stayer_forecast = peak;
+  }
   interrupt.checkpoint("engine_stayer_augmentation");
   StayerColumns sc{rows, firm.data(), worker.data(), frequency.data(), outcome.data(), target.data()};
   try {
     StayerReceipt rec;
+    const auto t = std::chrono::steady_clock::now();
     Prepared P = augment_stayers(ctx.P, sc, rec);
     HybridSystem HS = build_hybrid(P);
     CmgOptions copt;  // full_cmg.rs:467-476: max_levels 128
@@ -538,6 +1166,7 @@ static void augment_stayers_value(uint64_t generation, const VckssStayerAugmenta
     if (upload) ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
     ctx.stayer_receipt = rec;
     ctx.stayer_copy_bytes = r.caller_copy_bytes;
+    ctx.stayer_ns = elapsed_ns(t);
   } catch (...) {
     ctx.state = kFailed;
     throw;
@@ -765,6 +1394,13 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
   o.numerical = true;
+  VckssEngineSolveRequestV5 q;
+  std::memset(&q, 0, sizeof q);
+  q.v4 = r.point;
+  q.threads = r.threads;
+  q.tolerance_supplied = r.tolerance_supplied;
+  q.full_cmg_v2 = 1;
+  record_request(ctx, q, o);
   run_options(ctx, o, interrupt);
 }
 
@@ -1441,6 +2077,519 @@ int32_t vckss_mi355x_chunk_table_v1(const uint32_t *ptr, uint64_t segments, uint
 int32_t vckss_mi355x_hierarchy_free_v1(uint64_t handle) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] { g_hierarchies.erase(handle); });
+}
+
+// ---- backend and request capabilities (ffi_engine.rs:2501-2720, vckss_rust.h:1647-1665)
+int32_t vckss_rust_backend_capabilities_v1(VckssBackendCapabilitiesV1 *output, uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssBackendCapabilitiesV1>(output, output_capacity_bytes, "backend capabilities");
+    VckssBackendCapabilitiesV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+    // what this library implements: match and observation graphs, the dense
+    // exact estimator, batched PCG on the CMG hybrid graph with Counter-V1
+    // probes (CMG_FULL_V2 with its point routing and model receipt), movers
+    // and the stayer hybrid
+    o.core_ready_flags = VCKSS_CORE_MATCH_GRAPH_READY | VCKSS_CORE_EXACT_READY | VCKSS_CORE_BATCHED_PCG_READY |
+                         VCKSS_CORE_CMG_GRAPH_READY | VCKSS_CORE_COUNTER_RNG_READY | VCKSS_CORE_JLA_PLAN_READY |
+                         VCKSS_CORE_FULL_CMG_V2_READY | VCKSS_CORE_FULL_CMG_POINT_ROUTING_V1_READY |
+                         VCKSS_CORE_FULL_CMG_MODEL_V1_READY | VCKSS_CORE_DELETION_UNIT_MOVERS_V1_READY;
+    o.support_flags = VCKSS_SUPPORT_EXACT | VCKSS_SUPPORT_JLA | VCKSS_SUPPORT_MATCH_DELETION |
+                      VCKSS_SUPPORT_OBSERVATION_DELETION | VCKSS_SUPPORT_CMG;
+    o.deterministic_parallelism = 1;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+int32_t vckss_rust_backend_request_capability_v1(const VckssBackendRequestCapabilityRequestV1 *request,
+                                                 VckssBackendRequestCapabilityReceiptV1 *output,
+                                                 uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssBackendRequestCapabilityReceiptV1>(output, output_capacity_bytes,
+                                                             "request capability receipt");
+    if (!request) fail(kInvalidInput, "engine_abi", "request capability pointer is null");
+    VckssBackendRequestCapabilityRequestV1 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_abi", "unsupported ABI version");
+    if (r.reserved != 0) fail(kAbiMismatch, "engine_abi", "reserved request capability fields must be zero");
+    const auto c = classify_v1(r);
+    VckssBackendRequestCapabilityReceiptV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+    o.request_schema = r.request_schema;
+    o.supported = c.first == VCKSS_REQUEST_REASON_SUPPORTED;
+    o.reason_code = c.first;
+    o.profile_code = c.second;
+    o.algorithm = r.algorithm; o.deletion_mode = r.deletion_mode; o.nuisance_mode = r.nuisance_mode;
+    o.solver_route = r.solver_route; o.rng_contract = r.rng_contract; o.controls_count = r.controls_count;
+    o.frequency_use = r.frequency_use;
+    o.request_signature = signature_v1(r, "VCKSS-REQUEST-CAPABILITY-V1");
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+static VckssBackendRequestCapabilityReceiptV2 capability_receipt_v2(const VckssBackendRequestCapabilityRequestV2 &r,
+                                                                    std::pair<uint32_t, uint32_t> c, uint64_t sig,
+                                                                    uint32_t size) {
+  VckssBackendRequestCapabilityReceiptV2 o;
+  std::memset(&o, 0, sizeof o);
+  o.v1.struct_size = size;
+  o.v1.abi_version = VCKSS_RUST_ABI_VERSION_V1;
+  o.v1.request_schema = r.v1.request_schema;
+  o.v1.supported = c.first == VCKSS_REQUEST_REASON_SUPPORTED;
+  o.v1.reason_code = c.first;
+  o.v1.profile_code = c.second;
+  o.v1.algorithm = r.v1.algorithm; o.v1.deletion_mode = r.v1.deletion_mode; o.v1.nuisance_mode = r.v1.nuisance_mode;
+  o.v1.solver_route = r.v1.solver_route; o.v1.rng_contract = r.v1.rng_contract;
+  o.v1.controls_count = r.v1.controls_count; o.v1.frequency_use = r.v1.frequency_use;
+  o.v1.request_signature = sig;
+  o.engine = r.engine; o.batch_mode = r.batch_mode; o.stayers_mode = r.stayers_mode;
+  o.target_weight_mode = r.target_weight_mode; o.deletion_unit_source = r.deletion_unit_source;
+  o.probeorder_supplied = r.probeorder_supplied; o.wallseconds_supplied = r.wallseconds_supplied;
+  o.physical_limit = r.physical_limit;
+  return o;
+}
+
+int32_t vckss_rust_backend_request_capability_v2(const VckssBackendRequestCapabilityRequestV2 *request,
+                                                 VckssBackendRequestCapabilityReceiptV2 *output,
+                                                 uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssBackendRequestCapabilityReceiptV2>(output, output_capacity_bytes,
+                                                             "V2 request capability receipt");
+    if (!request) fail(kInvalidInput, "engine_abi", "request capability pointer is null");
+    VckssBackendRequestCapabilityRequestV2 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_abi", "unsupported ABI version");
+    if (r.v1.struct_size < sizeof r) fail(kAbiMismatch, "engine_abi", "V2 request capability reports a short structure size");
+    if (r.v1.reserved != 0 || r.reserved_2 != 0)
+      fail(kAbiMismatch, "engine_abi", "reserved V2 request capability fields must be zero");
+    const auto o = capability_receipt_v2(r, classify_v2(r), signature_v2(r), sizeof(VckssBackendRequestCapabilityReceiptV2));
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+int32_t vckss_rust_backend_request_capability_v3(const VckssBackendRequestCapabilityRequestV3 *request,
+                                                 VckssBackendRequestCapabilityReceiptV3 *output,
+                                                 uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssBackendRequestCapabilityReceiptV3>(output, output_capacity_bytes,
+                                                             "V3 request capability receipt");
+    if (!request) fail(kInvalidInput, "engine_abi", "request capability pointer is null");
+    VckssBackendRequestCapabilityRequestV3 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.v2.v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_abi", "unsupported ABI version");
+    if (r.v2.v1.struct_size < sizeof r)
+      fail(kAbiMismatch, "engine_abi", "V3 request capability reports a short structure size");
+    if (r.v2.v1.reserved != 0 || r.v2.reserved_2 != 0 || r.reserved_3 != 0 || r.reserved_4 != 0)
+      fail(kAbiMismatch, "engine_abi", "reserved V3 request capability fields must be zero");
+    VckssBackendRequestCapabilityReceiptV3 o;
+    std::memset(&o, 0, sizeof o);
+    o.v2 = capability_receipt_v2(r.v2, classify_v3(r), signature_v3(r), sizeof(VckssBackendRequestCapabilityReceiptV3));
+    o.leverage_batch_mode = r.leverage_batch_mode;
+    o.target_batch_mode = r.target_batch_mode;
+    o.allow_automatic_cmg_setup_fallback = r.allow_automatic_cmg_setup_fallback;
+    o.wallseconds = r.wallseconds;
+    // ffi_engine.rs:2657-2671: what is resolved only with the data
+    o.algorithm_resolution_deferred = r.v2.v1.algorithm == VCKSS_ALGORITHM_AUTO;
+    o.engine_resolution_deferred = r.v2.v1.algorithm == VCKSS_ALGORITHM_AUTO ||
+                                   (r.v2.v1.algorithm == VCKSS_ALGORITHM_JLA &&
+                                    r.v2.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED);
+    o.route_resolution_deferred = r.v2.v1.algorithm != VCKSS_ALGORITHM_EXACT && r.v2.v1.solver_route == VCKSS_ROUTE_AUTO;
+    o.leverage_batch_resolution_deferred = r.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO;
+    o.target_batch_resolution_deferred = r.target_batch_mode == VCKSS_BATCH_MODE_AUTO;
+    o.wall_advisory_only = 1;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- admission and preparation receipt (vckss_rust.h:1765-1769, 1923-1927)
+int32_t vckss_rust_engine_admit_prepare_v4(const VckssEnginePrepareRequestV4 *request, uint32_t probeorder_supplied) {
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_prepare", "request pointer is null");
+    VckssEnginePrepareRequestV4 r;
+    std::memcpy(&r, request, sizeof r);
+    (void)admit_prepare(r, probeorder_supplied);
+  });
+}
+
+int32_t vckss_rust_engine_preparation_receipt_v4(uint64_t generation, VckssEnginePreparationReceiptV4 *output,
+                                                 uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssEnginePreparationReceiptV4>(output, output_capacity_bytes, "V4 preparation receipt");
+    const Context &ctx = live(generation);
+    const Prepared &P = ctx.P;
+    const MemoryModel m = memory_model(ctx);
+    VckssEnginePreparationReceiptV4 o;
+    std::memset(&o, 0, sizeof o);
+    VckssEnginePreparationReceiptV2 &d = o.v3.v2;
+    d.struct_size = sizeof o;
+    d.generation = generation;
+    d.input_rows = P.input_rows;
+    d.retained_rows = P.rows;
+    d.workers = P.workers;
+    d.firms = P.firms;
+    d.cells = P.cells;
+    d.deletion_units = P.units;
+    d.target_strata = P.strata;
+    d.memory_limit_bytes = ctx.memory_limit_bytes;
+    d.caller_copy_bytes = ctx.caller_copy_bytes;
+    d.preparation_peak_forecast_bytes = ctx.preparation_forecast;
+    d.prepared_resident_bytes = m.host_problem + m.graph + m.hierarchy + m.plan;
+    const SelectionReceipt &g = P.selection;
+    d.graph_input_rows = g.input_rows;
+    d.graph_retained_rows = g.retained_rows;
+    d.graph_input_physical_mass = g.input_physical_mass;
+    d.graph_retained_physical_mass = g.retained_physical_mass;
+    d.graph_initial_components = g.initial_components;
+    d.graph_maximum_components = g.maximum_components;
+    d.graph_initial_component_rows = g.initial_component_rows;
+    d.graph_mover_input_rows = g.mover_input_rows;
+    d.graph_initial_deletion_edges = g.initial_deletion_edges;
+    d.graph_retained_deletion_edges = g.retained_deletion_edges;
+    d.graph_insufficient_workers_removed = g.insufficient_workers_removed;
+    d.graph_articulation_workers_removed = g.articulation_workers_removed;
+    d.graph_bridge_units_removed = g.bridge_units_removed;
+    d.graph_bridge_rows_removed = g.bridge_rows_removed;
+    d.graph_degree_iterations = g.degree_iterations;
+    d.graph_articulation_iterations = g.articulation_iterations;
+    d.graph_bridge_iterations = g.bridge_iterations;
+    d.graph_fixed_point_iterations = g.fixed_point_iterations;
+    o.v3.target_weight_sum = P.target_total;
+    o.controls_count = 0;
+    o.deletion_mode = P.observation ? VCKSS_DELETION_OBSERVATION : VCKSS_DELETION_MATCH;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- memory forecast (vckss_rust.h:2322-2332): the generation's host
+// arrays and hierarchy plus the peak of the engine's device buffers, with a
+// 20% admission allowance (expected <= admission, reserve = the difference)
+int32_t vckss_rust_engine_memory_forecast_v1(uint64_t generation, VckssMemoryForecastV1 *output,
+                                             uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssMemoryForecastV1>(output, output_capacity_bytes, "memory forecast");
+    const Context &ctx = live(generation);
+    const MemoryModel m = memory_model(ctx);
+    VckssMemoryForecastV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    // memory_api.rs:215-287: a prepared generation reports its preparation
+    // forecast; a solved one the larger of that and the solve's peak, where
+    // the expected peak counts the device high-water mark measured by the
+    // allocator and the admission peak the modelled hierarchy, plan and
+    // K = 64 workspace.
+    const uint64_t host = m.host_problem + m.graph + m.hierarchy;
+    const uint64_t prep = std::max(ctx.preparation_forecast, ctx.stayer_forecast);
+    uint64_t expected = prep, admission = prep;
+    if (ctx.has_result) {
+      expected = std::max(expected, host + m.device_peak);
+      admission = std::max(expected, host + m.hierarchy + m.plan + m.workspace);
+    }
+    o.expected_peak_bytes = expected;
+    o.admission_peak_bytes = admission;
+    o.conditional_reserve_bytes = admission - expected;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- performance receipt (vckss_rust.h:1441-1456, 2116-2120)
+int32_t vckss_rust_engine_performance_receipt_v1(uint64_t generation, VckssEnginePerformanceReceiptV1 *output,
+                                                 uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssEnginePerformanceReceiptV1>(output, output_capacity_bytes, "performance receipt");
+    const Context &ctx = solved(generation);
+    VckssEnginePerformanceReceiptV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    o.applicability_flags = 3;  // preparation phases and the solve are timed
+    o.algorithm_selected = ctx.algorithm_selected;
+    o.engine_selected = ctx.engine_selected;
+    o.ingest_ns = ctx.ingest_ns;
+    o.canonicalize_ns = ctx.P.canonicalize_ns;
+    o.graph_ns = ctx.P.graph_ns;
+    o.compress_ns = ctx.P.compress_ns;
+    o.plan_ns = ctx.P.plan_ns + ctx.hybrid_ns + ctx.hierarchy_ns + ctx.upload_ns;
+    o.stayer_augmentation_ns = ctx.stayer_ns;
+    o.solve_ns = ctx.solve_ns;
+    o.native_total_ns = o.ingest_ns + o.canonicalize_ns + o.graph_ns + o.compress_ns + o.plan_ns +
+                        o.stayer_augmentation_ns + o.solve_ns;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- detailed receipts V6 and V7 (vckss_rust.h:2101-2115)
+int32_t vckss_rust_engine_detailed_receipt_v6(uint64_t generation, VckssEngineDetailedReceiptV6 *output,
+                                              uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssEngineDetailedReceiptV6>(output, output_capacity_bytes, "V6 detailed receipt");
+    const Context &ctx = solved(generation);
+    const VckssEngineDetailedReceiptV6 o = detailed_v6(generation, ctx);
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+int32_t vckss_rust_engine_detailed_receipt_v7(uint64_t generation, VckssEngineDetailedReceiptV7 *output,
+                                              uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssEngineDetailedReceiptV7>(output, output_capacity_bytes, "V7 detailed receipt");
+    const Context &ctx = solved(generation);
+    VckssEngineDetailedReceiptV7 o;
+    std::memset(&o, 0, sizeof o);
+    o.v6 = detailed_v6(generation, ctx);
+    o.v6.v5.v4.v3.v2.struct_size = sizeof o;
+    o.execution = execution_plan(generation, ctx);
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- RHS receipts (vckss_rust.h:2126-2130): fit, leverage, target rows
+int32_t vckss_rust_engine_rhs_receipts_v1(uint64_t generation, VckssEngineRhsReceiptV1 *output,
+                                          uint64_t output_capacity_rows) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    const Context &ctx = solved(generation);
+    const std::vector<VckssEngineRhsReceiptV1> rows = rhs_rows(ctx);
+    if (rows.empty()) fail(kUnsupported, "engine_rhs_receipts", "the solved route keeps no RHS receipts");
+    if (!output) fail(kInvalidInput, "engine_rhs_receipts", "output pointer is null");
+    if (output_capacity_rows < rows.size())
+      fail(kAbiMismatch, "engine_rhs_receipts", "output capacity is smaller than the RHS receipt rows");
+    std::memcpy(output, rows.data(), rows.size() * sizeof(VckssEngineRhsReceiptV1));
+  });
+}
+
+// ---- full-CMG receipt (vckss_rust.h:1458-1509, 2121-2125; ffi_engine.rs:8598-8689)
+int32_t vckss_rust_engine_full_cmg_receipt_v1(uint64_t generation, VckssFullCmgReceiptV1 *output,
+                                              uint32_t output_capacity_bytes) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    require_capacity<VckssFullCmgReceiptV1>(output, output_capacity_bytes, "full-CMG receipt");
+    const Context &ctx = solved(generation);
+    if (!ctx.full_cmg) fail(kInvalidInput, "engine_full_cmg_receipt", "the solved generation did not use CMG_FULL_V2");
+    const SolveResult &r = ctx.result;
+    const auto &v1 = ctx.request.v4.v3.v2.v1;
+    const MemoryModel m = memory_model(ctx);
+    VckssFullCmgReceiptV1 o;
+    std::memset(&o, 0, sizeof o);
+    o.struct_size = sizeof o;
+    o.schema_version = 1;
+    o.generation = generation;
+    // the CMG hierarchy is that of the vendored cmg crate at this revision,
+    // reproduced bit for bit; identity 2 as the reference reports it
+    o.backend_identity = 2;
+    o.platform_os = 2;    // Linux
+    o.platform_arch = 2;  // x86_64 host
+    static const char kCommit[] = "e5613e81fb3b03c78de98658b372646fa802aabb";
+    std::memcpy(o.cmg_source_commit, kCommit, 40);
+    o.threads_requested = o.threads_used = std::max<uint32_t>(ctx.request.threads, 1);
+    o.maximum_concurrency = 1;  // one lockstep batch on the device at a time
+    o.vertices = ctx.HS.graph.n;
+    o.edges = ctx.HS.graph.u.size();
+    o.hierarchy_levels = ctx.H.levels.size();
+    o.terminal_vertices = ctx.H.levels.empty() ? 0 : ctx.H.levels.back().graph.n;
+    o.graph_copy_bytes = m.graph;
+    o.hierarchy_bytes = m.hierarchy;
+    o.plan_bytes = m.plan;
+    o.workspace_bytes_each = o.workspace_pool_bytes = m.workspace;
+    o.fit_effective_tolerance = v1.pcg_tolerance;
+    o.probe_effective_tolerance = ctx.request.tolerance_supplied ? v1.pcg_tolerance : 1e-6;
+    o.fit_initial_inner_tolerance = 0.01 * o.fit_effective_tolerance;  // full_cmg.rs:399-412
+    o.probe_initial_inner_tolerance = o.probe_effective_tolerance;
+    const uint64_t *a = ctx.counters_before, *b = ctx.counters_after;
+    o.refinement_attempts = b[2] - a[2];
+    o.refined_columns = b[3] - a[3];
+    o.batch_calls = b[1] - a[1];
+    o.across_rhs_batches = o.batch_calls;  // every batch is a lockstep multi-RHS solve
+    o.batch_strategy_mask = o.across_rhs_batches > 0 ? 4u : 0u;  // ffi_engine.rs:8607-8609
+    o.rhs_count = 1 + r.leverage.size() + r.target.size();
+    uint64_t it = r.fit.iterations;
+    double red = r.fit.reduced;
+    for (const auto *set : {&r.leverage, &r.target})
+      for (const ColumnReceipt &c : *set) {
+        it += c.iterations;
+        red = std::max(red, c.reduced);
+      }
+    o.total_iterations = it;
+    o.total_operator_applications = it + o.rhs_count;
+    o.total_preconditioner_applications = b[0] - a[0];
+    o.maximum_reduced_residual = red;
+    o.maximum_complete_residual = r.max_complete_residual;
+    o.graph_ns = ctx.hybrid_ns;
+    o.hierarchy_plan_ns = ctx.hierarchy_ns + ctx.upload_ns;
+    o.solve_ns = ctx.solve_ns;
+    o.preparation_peak_bytes = std::max(ctx.preparation_forecast, ctx.stayer_forecast);
+    o.prepared_persistent_bytes = m.host_problem;
+    o.actual_retained_bytes = m.graph + m.hierarchy + m.plan + m.workspace;
+    o.allocator_allowance_bytes = o.actual_retained_bytes / 5;
+    o.non_cmg_command_peak_bytes = m.host_problem + (ctx.P.cells * 2 + ctx.P.units * 6) * 8;
+    o.admitted_peak_bytes = std::max(o.preparation_peak_bytes, o.non_cmg_command_peak_bytes + o.actual_retained_bytes +
+                                                                   o.allocator_allowance_bytes);
+    o.pre_rng_forecast_bytes = o.admitted_peak_bytes;
+    // a leverage batch holds one column a probe, a target batch two
+    o.maximum_batch_rhs = std::max<uint64_t>(ctx.leverage_width, 2 * (uint64_t)ctx.target_width);
+    o.workspace_count = 1;
+    std::memcpy(output, &o, sizeof o);
+  });
+}
+
+// ---- V6 and V7 solve requests (vckss_rust.h:1584-1598;
+// generic_execution_api.rs:20-160, 285-520): the generic engine's executor
+// selection.  DIRECT_ATTACHMENTS on the CMG route with automatic batches is
+// this engine's CMG_FULL_V2 path; the diagonal queue is refused (11).
+static VckssEngineSolveRequestV6 default_solve_v6(uint32_t size) {
+  VckssEngineSolveRequestV6 r;
+  std::memset(&r, 0, sizeof r);
+  r.v4 = default_solve_v5().v4;
+  r.v4.v3.v2.v1.struct_size = size;
+  r.v4.v3.v2.algorithm = VCKSS_ALGORITHM_JLA;
+  r.v4.v3.engine = VCKSS_ENGINE_GENERIC;
+  r.v4.v3.v2.v1.solver_route = VCKSS_ROUTE_DIAGONAL_PCG;
+  r.v4.v3.v2.v1.allow_automatic_cmg_setup_fallback = 0;
+  r.execution_mode = VCKSS_GENERIC_EXECUTION_DIAGONAL_QUEUE;
+  r.threads = 1;
+  return r;
+}
+
+int32_t vckss_rust_engine_default_solve_request_v6(VckssEngineSolveRequestV6 *output, uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEngineSolveRequestV6>(output, output_capacity_bytes, "V6 solve default");
+    const VckssEngineSolveRequestV6 d = default_solve_v6(sizeof(VckssEngineSolveRequestV6));
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+int32_t vckss_rust_engine_default_solve_request_interrupt_v6(VckssEngineSolveRequestInterruptV6 *output,
+                                                             uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEngineSolveRequestInterruptV6>(output, output_capacity_bytes, "V6 interrupt solve default");
+    VckssEngineSolveRequestInterruptV6 d;
+    std::memset(&d, 0, sizeof d);
+    d.options = default_solve_v6(sizeof(VckssEngineSolveRequestInterruptV6));
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+int32_t vckss_rust_engine_default_solve_request_v7(VckssEngineSolveRequestV7 *output, uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEngineSolveRequestV7>(output, output_capacity_bytes, "V7 solve default");
+    VckssEngineSolveRequestV7 d;
+    std::memset(&d, 0, sizeof d);
+    d.v6 = default_solve_v6(sizeof(VckssEngineSolveRequestV7));
+    d.component_batch_mode = 1;
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+int32_t vckss_rust_engine_default_solve_request_interrupt_v7(VckssEngineSolveRequestInterruptV7 *output,
+                                                             uint32_t output_capacity_bytes) {
+  return guarded([&] {
+    require_capacity<VckssEngineSolveRequestInterruptV7>(output, output_capacity_bytes, "V7 interrupt solve default");
+    VckssEngineSolveRequestInterruptV7 d;
+    std::memset(&d, 0, sizeof d);
+    d.options.v6 = default_solve_v6(sizeof(VckssEngineSolveRequestInterruptV7));
+    d.options.component_batch_mode = 1;
+    std::memcpy(output, &d, sizeof d);
+  });
+}
+
+// generic_execution_api.rs:329-372 (validate) for this engine's executor
+static VckssEngineSolveRequestV5 v6_as_v5(const VckssEngineSolveRequestV6 &r, uint32_t minimum_size) {
+  const auto &v1 = r.v4.v3.v2.v1;
+  if (v1.abi_version != VCKSS_RUST_ABI_VERSION_V1) fail(kAbiMismatch, "engine_solve", "unsupported ABI version");
+  if (v1.struct_size < minimum_size || r.reserved_6 != 0)
+    fail(kAbiMismatch, "engine_solve", "invalid V6 solve structure size or reserved field");
+  if (r.threads == 0) fail(kInvalidInput, "generic_execution", "V6 requires positive permitted threads");
+  if (r.v4.v3.v2.algorithm != VCKSS_ALGORITHM_JLA || r.v4.v3.engine != VCKSS_ENGINE_GENERIC ||
+      v1.rng_contract != VCKSS_RNG_COUNTER_V1 || v1.allow_automatic_cmg_setup_fallback != 0)
+    fail(kUnsupported, "generic_execution", "V6 requires explicit generic JLA, Counter-V1 and no fallback");
+  const bool direct = r.execution_mode == VCKSS_GENERIC_EXECUTION_DIRECT_ATTACHMENTS &&
+                      v1.solver_route == VCKSS_ROUTE_CMG_PCG && r.v4.v3.batch_mode == VCKSS_BATCH_MODE_AUTO &&
+                      r.v4.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO &&
+                      r.v4.target_batch_mode == VCKSS_BATCH_MODE_AUTO && v1.leverage_batch_width == 0 &&
+                      v1.target_batch_width == 0;
+  if (!direct)
+    fail(kUnsupported, "generic_execution",
+         "the MI355X engine runs the generic engine's direct CMG executor only (CMG route, automatic batches)");
+  VckssEngineSolveRequestV5 q;
+  std::memset(&q, 0, sizeof q);
+  q.v4 = r.v4;
+  q.v4.v3.v2.v1.struct_size = sizeof(VckssEngineSolveRequestV5);
+  q.threads = r.threads;
+  q.tolerance_supplied = 1;  // the executor plans the probes at the request tolerance
+  q.full_cmg_v2 = 1;
+  return q;
+}
+
+static void solve_v6_common(uint64_t generation, const VckssEngineSolveRequestV5 &q, Interrupt &interrupt) {
+  Context &ctx = live(generation);
+  const SolveOptions o = point_options(q.v4, q.tolerance_supplied, ctx);
+  record_request(ctx, q, o);
+  run_options(ctx, o, interrupt);
+}
+
+int32_t vckss_rust_engine_solve_v6(uint64_t generation, const VckssEngineSolveRequestV6 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestV6 r;
+    std::memcpy(&r, request, sizeof r);
+    Interrupt none;
+    solve_v6_common(generation, v6_as_v5(r, sizeof(VckssEngineSolveRequestV6)), none);
+  });
+}
+
+int32_t vckss_rust_engine_solve_interrupt_v6(uint64_t generation, const VckssEngineSolveRequestInterruptV6 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestInterruptV6 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.reserved) fail(kAbiMismatch, "engine_solve", "reserved interrupt solve fields must be zero");
+    const VckssEngineSolveRequestV5 q = v6_as_v5(r.options, sizeof(VckssEngineSolveRequestV6));
+    Interrupt interrupt(r.interrupt_poll, r.interrupt_context, r.checkpoint_interval);
+    solve_v6_common(generation, q, interrupt);
+  });
+}
+
+// generic_execution_api.rs:473-500 (validate_v7)
+static VckssEngineSolveRequestV5 v7_as_v5(const VckssEngineSolveRequestV7 &r) {
+  if (r.v6.v4.v3.v2.v1.struct_size < sizeof(VckssEngineSolveRequestV7) || r.component_batch_mode != 1 ||
+      r.reserved_7 != 0)
+    fail(kAbiMismatch, "engine_solve", "invalid V7 solve structure or component batch mode");
+  return v6_as_v5(r.v6, sizeof(VckssEngineSolveRequestV7));
+}
+
+int32_t vckss_rust_engine_solve_v7(uint64_t generation, const VckssEngineSolveRequestV7 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestV7 r;
+    std::memcpy(&r, request, sizeof r);
+    Interrupt none;
+    solve_v6_common(generation, v7_as_v5(r), none);
+  });
+}
+
+int32_t vckss_rust_engine_solve_interrupt_v7(uint64_t generation, const VckssEngineSolveRequestInterruptV7 *request) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!request) fail(kInvalidInput, "engine_solve", "request pointer is null");
+    VckssEngineSolveRequestInterruptV7 r;
+    std::memcpy(&r, request, sizeof r);
+    if (r.reserved) fail(kAbiMismatch, "engine_solve", "reserved interrupt solve fields must be zero");
+    const VckssEngineSolveRequestV5 q = v7_as_v5(r.options);
+    Interrupt interrupt(r.interrupt_poll, r.interrupt_context, r.checkpoint_interval);
+    solve_v6_common(generation, q, interrupt);
+  });
 }
 
 }  // extern "C"

@@ -26,15 +26,10 @@ const char *thread_last_error();
 
 extern "C" {
 
-VK_REFUSE(vckss_rust_backend_capabilities_v1)
-VK_REFUSE(vckss_rust_backend_request_capability_v1)
-VK_REFUSE(vckss_rust_backend_request_capability_v2)
-VK_REFUSE(vckss_rust_backend_request_capability_v3)
 VK_REFUSE(vckss_rust_engine_admit_prepare_memory_v1)
 VK_REFUSE(vckss_rust_engine_admit_prepare_probe_order_v1)
 VK_REFUSE(vckss_rust_engine_admit_prepare_v2)
 VK_REFUSE(vckss_rust_engine_admit_prepare_v3)
-VK_REFUSE(vckss_rust_engine_admit_prepare_v4)
 VK_REFUSE(vckss_rust_engine_augment_component_inference_interrupt_v1)
 VK_REFUSE(vckss_rust_engine_augment_component_inference_interrupt_v2)
 VK_REFUSE(vckss_rust_engine_augment_component_inference_interrupt_v3)
@@ -64,37 +59,28 @@ VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v1)
 VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v2)
 VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v3)
 VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v4)
-VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v6)
-VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v7)
 VK_REFUSE(vckss_rust_engine_default_solve_request_interrupt_v8)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v1)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v2)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v3)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v4)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v5)
-VK_REFUSE(vckss_rust_engine_default_solve_request_v6)
 VK_REFUSE(vckss_rust_engine_default_solve_request_v8)
 VK_REFUSE(vckss_rust_engine_detailed_receipt_v1)
 VK_REFUSE(vckss_rust_engine_detailed_receipt_v2)
 VK_REFUSE(vckss_rust_engine_detailed_receipt_v3)
 VK_REFUSE(vckss_rust_engine_detailed_receipt_v4)
 VK_REFUSE(vckss_rust_engine_detailed_receipt_v5)
-VK_REFUSE(vckss_rust_engine_detailed_receipt_v6)
-VK_REFUSE(vckss_rust_engine_detailed_receipt_v7)
 VK_REFUSE(vckss_rust_engine_exact_execution_receipt_v1)
 VK_REFUSE(vckss_rust_engine_execution_plan_receipt_v1)
 VK_REFUSE(vckss_rust_engine_full_cmg_model_receipt_v1)
-VK_REFUSE(vckss_rust_engine_full_cmg_receipt_v1)
 VK_REFUSE(vckss_rust_engine_generic_execution_receipt_v1)
 VK_REFUSE(vckss_rust_engine_generic_execution_receipt_v2)
 VK_REFUSE(vckss_rust_engine_leaveout_residual_diagnostic_v1)
-VK_REFUSE(vckss_rust_engine_memory_forecast_v1)
 VK_REFUSE(vckss_rust_engine_memory_policy_v1)
-VK_REFUSE(vckss_rust_engine_performance_receipt_v1)
 VK_REFUSE(vckss_rust_engine_preparation_receipt_v1)
 VK_REFUSE(vckss_rust_engine_preparation_receipt_v2)
 VK_REFUSE(vckss_rust_engine_preparation_receipt_v3)
-VK_REFUSE(vckss_rust_engine_preparation_receipt_v4)
 VK_REFUSE(vckss_rust_engine_prepare_interrupt_v1)
 VK_REFUSE(vckss_rust_engine_prepare_interrupt_v2)
 VK_REFUSE(vckss_rust_engine_prepare_interrupt_v3)
@@ -107,7 +93,6 @@ VK_REFUSE(vckss_rust_engine_projection_result_receipt_v1)
 VK_REFUSE(vckss_rust_engine_projection_result_v1)
 VK_REFUSE(vckss_rust_engine_repeat_measurement_diagnostic_v1)
 VK_REFUSE(vckss_rust_engine_repeat_measurement_leading_v1)
-VK_REFUSE(vckss_rust_engine_rhs_receipts_v1)
 VK_REFUSE(vckss_rust_engine_rhs_receipts_v2)
 VK_REFUSE(vckss_rust_engine_solve_exact_execution_interrupt_v1)
 VK_REFUSE(vckss_rust_engine_solve_exact_execution_v1)
@@ -118,8 +103,6 @@ VK_REFUSE(vckss_rust_engine_solve_interrupt_v1)
 VK_REFUSE(vckss_rust_engine_solve_interrupt_v2)
 VK_REFUSE(vckss_rust_engine_solve_interrupt_v3)
 VK_REFUSE(vckss_rust_engine_solve_interrupt_v4)
-VK_REFUSE(vckss_rust_engine_solve_interrupt_v6)
-VK_REFUSE(vckss_rust_engine_solve_interrupt_v7)
 VK_REFUSE(vckss_rust_engine_solve_interrupt_v8)
 VK_REFUSE(vckss_rust_engine_solve_numerical_generic_legacy_v2)
 VK_REFUSE(vckss_rust_engine_solve_numerical_legacy_v2)
@@ -127,7 +110,6 @@ VK_REFUSE(vckss_rust_engine_solve_v1)
 VK_REFUSE(vckss_rust_engine_solve_v2)
 VK_REFUSE(vckss_rust_engine_solve_v3)
 VK_REFUSE(vckss_rust_engine_solve_v4)
-VK_REFUSE(vckss_rust_engine_solve_v6)
 VK_REFUSE(vckss_rust_engine_solve_v8)
 VK_REFUSE(vckss_rust_report_call_v1)
 VK_REFUSE(vckss_rust_session_clear_abandoned_v1)
