@@ -1237,28 +1237,21 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
             ssum g2 = {0, 0}, g4 = {0, 0}, off = {0, 0};
             for (uint64_t q = obs.phys[i]; q < obs.phys[i + 1]; ++q) {
                 double c1 = ss_fin(obs.first[q]) / r, c3 = ss_fin(obs.third[q]) / r;
-                double u[5] = {a2, 1.0 + a2 - 2.0 * c1, a4, 1.0 + 6.0 * a2 + a4 - 4.0 * c1 - 4.0 * c3,
-                               a2 + a4 - 2.0 * c3};
+                const double x[4] = {a2, a4, c1, c3};
+                double u[5], fd[20], pb[4], value;
+                const int st = orc_copy_pullback(x, 0.0, (uint32_t)R, u, fd, &value, pb);
                 num->minimum_constrained = fmin(num->minimum_constrained, u[0] + u[1]);
-                double fd[20], gr[5];
-                if (!orc_finite_derivative(u, (uint32_t)R, fd)) {
+                if (st == 6 && !orc_finite_derivative(u, (uint32_t)R, fd)) {
                     ns.status = 6;
                     continue;
                 }
                 double ell = fd[1];
                 num->minimum_residual_margin = fmin(num->minimum_residual_margin, ell);
                 num->maximum_sensitivity_ratio = fmax(num->maximum_sensitivity_ratio, sqrt(fmax(fd[3], 0.0)) / ell);
-                if (fd[19] != 0.0) {
-                    ns.status = 5;
+                if (st != 0) {
+                    ns.status = st;
                     continue;
                 }
-                if (!orc_fd_observation(fd, 0.0, gr)) {
-                    ns.status = 6;
-                    continue;
-                }
-                /* numerical_mc.rs:137-145 (copy_pullback) */
-                double pb[4] = {gr[0] + gr[1] + 6.0 * gr[3] + gr[4], gr[2] + gr[3] + gr[4],
-                                -2.0 * gr[1] - 4.0 * gr[3], -4.0 * gr[3] - 2.0 * gr[4]};
                 ss_add(&g2, pb[0]);
                 ss_add(&g4, pb[1]);
                 ss_add(&off, pb[2] * c1);

@@ -2,9 +2,9 @@
  * CPU oracle: local point-probe linearisation of the numerical MCSE
  * (rust/crates/vckss-core/src/numerical_mc.rs).  TEST INFRASTRUCTURE ONLY.
  *
- * Pins (tests/test_oracle_numerical.py): the derivative fields and the
- * observation gradient against central finite differences of the functions
- * they differentiate; the covariance policy cases of
+ * Pins (tests/test_oracle_numerical.py): the derivative fields, the
+ * observation gradient and the copy pullback against central finite
+ * differences of the functions they differentiate; the covariance policy cases of
  * rust/crates/vckss-core/tests/numerical_mc.rs:7-50 (scaled covariance,
  * signed stage, exact zero, clipped adjustment); the fold estimator's exact
  * unbiasedness by enumeration (tests/numerical_mc.rs:52-107); finalize
@@ -58,6 +58,34 @@ int orc_fd_observation(const double fd[20], double control, double grad[5]) {
         if (!isfinite(grad[j])) return 0;
     }
     return 1;
+}
+
+/* One copy of an observation unit (generic_jla/numerical.rs:213-256): the
+ * moment map u(a2, a4, c1, c3) of its row's mean projection square and
+ * fourth power and the copy's first and third correlations, the linearised
+ * correction value 1/l + bias/l^2 - variance/l^3 with l = m - control, and
+ * its pullback to (a2, a4, c1, c3) (numerical_mc.rs:137-145, copy_pullback).
+ * fd receives the derivative fields.  Returns 0, 5 (nonsmooth adjustment)
+ * or 6 (nonfinite derivative). */
+int orc_copy_pullback(const double x[4], double control, uint32_t r, double u[5], double fd[20], double *value,
+                      double pb[4]) {
+    const double a2 = x[0], a4 = x[1], c1 = x[2], c3 = x[3];
+    u[0] = a2;
+    u[1] = 1.0 + a2 - 2.0 * c1;
+    u[2] = a4;
+    u[3] = 1.0 + 6.0 * a2 + a4 - 4.0 * c1 - 4.0 * c3;
+    u[4] = a2 + a4 - 2.0 * c3;
+    if (!orc_finite_derivative(u, r, fd)) return 6;
+    if (fd[19] != 0.0) return 5;
+    double g[5];
+    if (!orc_fd_observation(fd, control, g)) return 6;
+    const double ell = fd[1] - control;
+    *value = 1.0 / ell + fd[2] / (ell * ell) - fd[3] / (ell * ell * ell);
+    pb[0] = g[0] + g[1] + 6.0 * g[3] + g[4];
+    pb[1] = g[2] + g[3] + g[4];
+    pb[2] = -2.0 * g[1] - 4.0 * g[3];
+    pb[3] = -4.0 * g[3] - 2.0 * g[4];
+    return 0;
 }
 
 static void means3(const double *x, uint64_t n, double out[3]) {

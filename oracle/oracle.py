@@ -108,6 +108,15 @@ def fd_observation(fd, control=0.0):
     return g
 
 
+def copy_pullback(x, r, control=0.0):
+    """One observation copy (generic_jla/numerical.rs:213-256): (status,
+    value, pullback) for x = (a2, a4, c1, c3)."""
+    u, fd, pb, v = np.zeros(5), np.zeros(20), np.zeros(4), C.c_double()
+    st = lib().orc_copy_pullback(np.ascontiguousarray(x, np.float64).ctypes.data, control, r, u.ctypes.data,
+                                 fd.ctypes.data, C.byref(v), pb.ctypes.data)
+    return st, v.value, pb
+
+
 def cross_covariance(a, b):
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
@@ -180,6 +189,8 @@ def lib():
                                                C.POINTER(NumericalResult)]
         L.orc_finite_derivative.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_fd_observation.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.orc_copy_pullback.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
         L.orc_cross_covariance.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         L.orc_numerical_finalize.argtypes = [C.c_void_p] * 5 + [C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_time_probe_solves.argtypes = [C.c_void_p, EngineOptions, C.c_uint32, C.c_uint32,

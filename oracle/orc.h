@@ -241,6 +241,8 @@ int orc_engine_run_numerical(const orc_problem *p, orc_engine_options opt, orc_e
 /* numerical_mc.rs pieces, exposed for the pins. */
 int orc_finite_derivative(const double u[5], uint32_t r, double out[20]);
 int orc_fd_observation(const double fd[20], double control, double grad[5]);
+int orc_copy_pullback(const double x[4], double control, uint32_t r, double u[5], double fd[20], double *value,
+                      double pb[4]);
 int orc_cross_covariance(const double *a, const double *b, uint64_t n, double out[9]);
 void orc_numerical_finalize(const double cond[9], const double lev[9], double raw[9], double usable[9],
                             double mcse[4], uint32_t *status, double *psd_adjustment);

@@ -102,8 +102,47 @@ def test_observation_physical_split_preserves_results(oracle):
 # so every fold's scores average to zero (the offsets are the linearisation
 # at the probe means); the result does not depend on the replay batch width;
 # the reported MCSE is of the size of the estimator's spread across seeds.
-# The copy pullback coefficients (numerical_mc.rs:137-145) have no
-# reference vector: parity unpinned beyond these invariants.
+# The copy pullback coefficients (numerical_mc.rs:137-145) are pinned by
+# central differences of the copy's correction value below.
+def _copy_points(seed, n):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        a2 = rng.uniform(0.05, 0.4)
+        a4 = a2 * a2 * rng.uniform(1.0, 3.0)
+        yield np.array([a2, a4, a2 * rng.uniform(0.6, 1.0), a4 * rng.uniform(0.6, 1.0)])
+
+
+def _central(oracle, x, r, h=1e-5):
+    out = np.zeros(4)
+    for j in range(4):
+        e = np.zeros(4)
+        e[j] = h * max(abs(x[j]), 1e-3)
+        hi = oracle.copy_pullback(x + e, r)
+        lo = oracle.copy_pullback(x - e, r)
+        assert hi[0] == 0 and lo[0] == 0
+        out[j] = (hi[1] - lo[1]) / (2 * e[j])
+    return out
+
+
+def test_copy_pullback_central_differences(oracle):
+    """The pullback of one copy's linearised correction to the row moments
+    (a2, a4) and the copy correlations (c1, c3) equals central differences of
+    the correction value; a transposed pair or a changed coefficient fails."""
+    checked = 0
+    caught = {"swap": 0, "sign": 0, "scale": 0}
+    for x in _copy_points(11, 40):
+        st, _, pb = oracle.copy_pullback(x, 64)
+        if st != 0:
+            continue
+        fd = _central(oracle, x, 64)
+        tol = 1e-6 * np.max(np.abs(fd))
+        assert np.all(np.abs(pb - fd) <= tol), (x, pb, fd)
+        checked += 1
+        for name, bad in (("swap", pb[[0, 1, 3, 2]]), ("sign", pb * [1, 1, -1, 1]),
+                          ("scale", pb * [1, 1, 1, 1.5])):
+            caught[name] += bool(np.any(np.abs(bad - fd) > tol))
+    assert checked >= 30
+    assert all(v >= checked // 2 for v in caught.values()), caught
 def test_observation_numerical_scores_are_centred(oracle):
     cols = _panel(3)
     p = oracle.Problem(*cols, observation=True)
