@@ -20,8 +20,9 @@ restatement of the reference engine) on the host's cores on a bounded
 sample.
 
 The roofline entry is the dominant kernel of the timed region, the finest
-A p of the Krylov iteration (k_segments<double, 64, 1, OpCurvature<64>>,
-pass 1), timed with HIP events on the engine stream; `traffic` comes from the
+direction and A p of the Krylov iteration (k_segments<double, 64, 1,
+OpDirectionCurvature<64, float>>, pass 1), timed with HIP events on the
+engine stream; `traffic` comes from the
 committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel for this
 configuration when present.  The finest prolong + smooth pass of the V-cycle
 (the V-cycle SpMV) is reported beside it the same way.
@@ -430,8 +431,9 @@ def main():
                    "iterations_per_solve": round(iters, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_segments<double, %d, 1, OpCurvature<%d>> pass 1 (finest A p)"
-                               % (args.width, args.width),
+                     "kernel": "k_segments<double, %d, 1, OpDirectionCurvature<%d, %s>> pass 1 "
+                               "(finest direction and A p)"
+                               % (args.width, args.width, "float" if args.precision == 32 else "double"),
                      "kernel_us": round(tm["kernel_ms"] * 1e3, 2), "launches": tm["kernel_launches"],
                      "algorithmic_bytes": tm["kernel_bytes"]},
         "vcycle_spmv": {"kernel": "finest prolong + smooth pass 1 (FP%d V-cycle)" % args.precision,

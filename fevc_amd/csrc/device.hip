@@ -297,33 +297,42 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     segments<double, K, 4>(E, rows0, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
   }
   E.gate = active;
+  // The preconditioner's output of the latest enqueued cycle: the cycle
+  // alternates its level-0 buffers, so the body reads the address the last
+  // precondition step produced (with graphs, the one captured in `pre`).
+  CycleOut zcur{};
   auto precondition = [&]() {
     hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags,
                        (const int *)nullptr);
-    const CycleOut z = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
-    if (z.z32) {
-      hipLaunchKernelGGL((dev::k_pcg_rho<K, float>), rg, kBlock, 0, E.stream, n, E.rc.p, z.z32, E.rpartial.p, active);
+    zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
+    if (zcur.z32) {
+      hipLaunchKernelGGL((dev::k_pcg_rho<K, float>), rg, kBlock, 0, E.stream, n, E.rc.p, zcur.z32, E.rpartial.p,
+                         active);
     } else {
-      hipLaunchKernelGGL((dev::k_pcg_rho<K, double>), rg, kBlock, 0, E.stream, n, E.rc.p, z.z64, E.rpartial.p,
+      hipLaunchKernelGGL((dev::k_pcg_rho<K, double>), rg, kBlock, 0, E.stream, n, E.rc.p, zcur.z64, E.rpartial.p,
                          active);
     }
     finish<K, 3>(E, E.rpartial.p, rg, dev::FinRho{E.st.p, n}, active);
-    if (z.z32) {
-      hipLaunchKernelGGL((dev::k_pcg_direction<K, float>), grid_for(m0), kBlock, 0, E.stream, n, z.z32, E.p.p, E.st.p,
-                         active);
-    } else {
-      hipLaunchKernelGGL((dev::k_pcg_direction<K, double>), grid_for(m0), kBlock, 0, E.stream, n, z.z64, E.p.p,
-                         E.st.p, active);
-    }
   };
-  // iteration body: A p, update, centring, fresh-residual gate, limit
+  // iteration body: direction and A p, update, centring, fresh-residual
+  // gate, limit
   auto body = [&]() {
-    // A p, the dominant kernel of the iteration, timed with HIP events in a
-    // timed run; algorithmic bytes: CSR once (values, columns, chunk
-    // descriptors), p read once, A p written once
-    const auto ev = E.timed_pair(0, 12.0 * (double)L0.nnz + 16.0 * rows0.count + 2.0 * 8.0 * (double)m0);
-    segments<double, K, 1>(E, rows0, dev::OpCurvature<K>{{L0.col.p, L0.val.p}, E.p.p, E.ap.p, E.st.p}, ev.first,
-                           ev.second);
+    // direction and A p, the dominant kernel of the iteration, timed with HIP
+    // events in a timed run; algorithmic bytes: CSR once (values, columns,
+    // chunk descriptors), z read once in the cycle's precision, p and A p
+    // each read and written once
+    const double zb = zcur.z32 ? 4.0 : 8.0;
+    const auto ev = E.timed_pair(0, 12.0 * (double)L0.nnz + 16.0 * rows0.count + (zb + 4.0 * 8.0) * (double)m0);
+    if (zcur.z32)
+      segments<double, K, 1>(E, rows0,
+                             dev::OpDirectionCurvature<K, float>{{L0.col.p, L0.val.p}, zcur.z32, E.p.p, E.ap.p,
+                                                                 E.st.p},
+                             ev.first, ev.second);
+    else
+      segments<double, K, 1>(E, rows0,
+                             dev::OpDirectionCurvature<K, double>{{L0.col.p, L0.val.p}, zcur.z64, E.p.p, E.ap.p,
+                                                                  E.st.p},
+                             ev.first, ev.second);
     hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
                        E.rpartial.p, E.st.p, active);
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
@@ -347,10 +356,10 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
         E.drop_graphs();
       }
       Impl::PcgGraphs g;
-      g.body = capture(E, body);
       const uint64_t v = E.vcycles;
-      g.pre = capture(E, precondition);
+      g.pre = capture(E, precondition);  // first: the body reads the cycle output it writes
       E.vcycles = v;
+      g.body = capture(E, body);
       found = E.pcg_graphs.emplace(key, g).first;
     }
     graphs = &found->second;

@@ -285,8 +285,9 @@ struct DeviceEngine::Impl {
   uint64_t status_launch = 0;
   bool timing = false;
   // HIP events around the launches of the timed kernels in a timed run:
-  // [0] the finest-level A p of the Krylov iteration (pass 1 of
-  // OpCurvature), [1] the finest prolong + smooth pass of the V-cycle; with
+  // [0] the finest-level direction and A p of the Krylov iteration (pass 1
+  // of OpDirectionCurvature), [1] the finest prolong + smooth pass of the
+  // V-cycle; with
   // their algorithmic bytes per launch
   struct TimedKernel {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -337,11 +337,37 @@ __global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, S
   if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial, blocks1 + blockIdx.x);
 }
 
+// Value of lane `src` of a wavefront.  With `uniform` (the index is the same
+// on every lane) a scalar read-lane, else a cross-lane permute.
+template <bool uniform>
+__device__ inline uint32_t lane_value(uint32_t v, uint32_t src) {
+  if constexpr (uniform) return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+  else return (uint32_t)__shfl((int)v, (int)src, 64);
+}
+template <bool uniform>
+__device__ inline float lane_value(float v, uint32_t src) {
+  return __uint_as_float(lane_value<uniform>(__float_as_uint(v), src));
+}
+template <bool uniform>
+__device__ inline double lane_value(double v, uint32_t src) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const uint32_t lo = lane_value<uniform>((uint32_t)b, src), hi = lane_value<uniform>((uint32_t)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Row chunk of y = A x with the neighbour values supplied by D:
 // sum over entries of w_ik (x_i - x_k), the reference's per-edge form
 // (vendor/cmg/src/graph.rs:664-705).
 // An operator may hoist per-lane values out of the walk with lane(j); the
 // result is handed to self/nbr (NoLane when nothing is hoisted).
+//
+// When a wavefront owns the chunk (Shape<K>::G == 64) the walk is staged:
+// the wavefront loads up to 64 entries' columns and weights with one
+// coalesced load, then every slice issues kStage neighbour gathers at once
+// (columns and weights handed across lanes) before it accumulates them.
+// Each slice still sums its entries s, s + V, ... in increasing order, so the
+// result is the plain walk's bit for bit; only the number of gathers in
+// flight changes (a latency-bound walk of dependent loads otherwise).
 struct NoLane {};
 
 template <typename T, int K, class D>
@@ -355,6 +381,52 @@ struct CsrOp {
     const auto ln = op.lane(j);
     const T xi = op.self(d.x, j, ln);
     T acc = T(0);
+    if constexpr (Shape<K>::G == 64 && STEP == Shape<K>::V) {
+      constexpr int V = Shape<K>::V;
+      constexpr int kStage = V == 1 ? 16 : 8;
+      const uint32_t lane = threadIdx.x & 63u;
+      for (uint32_t base = d.y; base < d.z; base += 64) {
+        const uint32_t cnt = d.z - base < 64 ? d.z - base : 64;  // wave-uniform
+        uint32_t mc = 0;
+        scalar_t<T> mv = scalar_t<T>(0);
+        if (lane < cnt) {
+          mc = col[base + lane];
+          mv = val[base + lane];
+        }
+        const uint32_t rounds = (cnt + V - 1) / V;
+        for (uint32_t r = 0; r < rounds; r += kStage) {
+          T g[kStage];
+          scalar_t<T> w[kStage];
+          // With one slice the rounds are wave-uniform: every gather is
+          // issued unconditionally so none waits on a branch, a round past
+          // the end re-reading the column of the block's first entry (a
+          // valid index of the gathered vector, whose space may differ from
+          // the rows', as for precomposed coarse columns).  With several
+          // slices validity differs across lanes and the gathers are masked.
+#pragma unroll
+          for (int u = 0; u < kStage; ++u) {
+            const uint32_t idx = (r + u) * V + (uint32_t)s;
+            const bool ok = r + u < rounds && idx < cnt;
+            const uint32_t src = ok ? idx : 0u;
+            w[u] = lane_value<V == 1>(mv, src);
+            const uint32_t c = lane_value<V == 1>(mc, src);
+            if constexpr (V == 1) g[u] = op.nbr(c, j, ln);
+            else g[u] = ok ? op.nbr(c, j, ln) : xi;
+          }
+#pragma unroll
+          for (int u = 0; u < kStage; ++u) {
+            const uint32_t idx = (r + u) * V + (uint32_t)s;
+            if constexpr (V == 1) {
+              const T next = acc + w[u] * (xi - g[u]);
+              acc = r + u < rounds ? next : acc;
+            } else if (r + u < rounds && idx < cnt) {
+              acc += w[u] * (xi - g[u]);
+            }
+          }
+        }
+      }
+      return acc;
+    }
     if constexpr (std::is_same_v<T, double>) {
 #pragma unroll VK_SEG_UNROLL_F64
       for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += val[k] * (xi - op.nbr(col[k], j, ln));

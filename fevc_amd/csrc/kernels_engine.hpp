@@ -206,33 +206,32 @@ struct FinRho {
   }
 };
 
-// p = z_c (first or after a residual replacement) or z_c + beta p.
+// Direction and curvature in one pass (pcg.rs:1640-1690): p = z_c (first, or
+// after a residual replacement) or z_c + beta p, A p, and the curvature
+// p'Ap; alpha = rho / p'Ap.  A p is formed as A z + beta (A p)_prev: the
+// Laplacian's rows sum to zero, so A z_c = A z, and each edge term
+// w (z_i - z_c) takes z in the precision the cycle produced it (an FP32
+// cycle output differs exactly in FP64, and is gathered at 4 bytes a lane).
 template <int K, typename Z>
-__global__ void k_pcg_direction(uint32_t n, const Z *__restrict__ z, double *__restrict__ p,
-                                const ColState *__restrict__ st, const int *__restrict__ gate) {
-  VK_GATE(gate);
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const int j = t % K;
-    const ColState &s = st[j];
-    if (!s.active) continue;
-    const double zc = (double)z[t] - s.mean_z;
-    p[t] = (s.first || s.restarted) ? zc : zc + s.beta * p[t];
-  }
-}
-
-// Ap = A p and the curvature p'Ap; alpha = rho / p'Ap (pcg.rs:1665-1690).
-template <int K>
-struct OpCurvature : CsrOp<double, K, OpCurvature<K>> {
-  const double *__restrict__ p;
+struct OpDirectionCurvature : CsrOp<double, K, OpDirectionCurvature<K, Z>> {
+  const Z *__restrict__ z;
+  double *__restrict__ p;
   double *__restrict__ ap;
   ColState *__restrict__ st;
-  __device__ double self(uint32_t i, int j, NoLane = {}) const { return p[(uint64_t)i * K + j]; }
-  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return p[(uint64_t)c * K + j]; }
+  __device__ double self(uint32_t i, int j, NoLane = {}) const { return (double)z[(uint64_t)i * K + j]; }
+  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return (double)z[(uint64_t)c * K + j]; }
   template <int NA>
-  __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
+  __device__ void finish(uint32_t i, int j, double az, double (&acc)[NA]) const {
+    const ColState &s = st[j];
+    if (!s.active) return;
     const uint64_t q = (uint64_t)i * K + j;
-    ap[q] = ax;
-    acc[0] += p[q] * ax;
+    const double zc = (double)z[q] - s.mean_z;
+    const bool fresh = s.first || s.restarted;
+    const double pn = fresh ? zc : zc + s.beta * p[q];
+    const double apn = fresh ? az : az + s.beta * ap[q];
+    p[q] = pn;
+    ap[q] = apn;
+    acc[0] += pn * apn;
   }
   template <int NV>
   __device__ void reduced(int j, const double (&tot)[NV]) const {
