@@ -307,6 +307,8 @@ SIGNATURES = [
     ("vckss_mi355x_smoother_params_v1", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vckss_mi355x_smoother_omega_v1", C.c_int32,
      [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("vckss_mi355x_smoother_infer_v1", C.c_int32,
+     [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_smoother_gradient_v1", C.c_int32,
      [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.c_void_p]),
     ("vckss_mi355x_smoother_train_v1", C.c_int32,
@@ -712,14 +714,15 @@ class Engine:
                     shape=[int(x) for x in shape],
                     calibration=[None if x == 0xFFFFFFFF else int(x) for x in out[6:10]])
 
-    def smoother_omega(self, params):
+    def smoother_omega(self, params, bf16=False):
+        """omega of every smoothed level: the FP64 training forward, or with
+        bf16 the solve path's BF16 inference."""
+        fn = library().vckss_mi355x_smoother_infer_v1 if bf16 else library().vckss_mi355x_smoother_omega_v1
         p = np.ascontiguousarray(params, np.float64)
         written = C.c_uint64()
-        _check(library().vckss_mi355x_smoother_omega_v1(self.generation, p.ctypes.data, len(p), None, 0,
-                                                        C.byref(written)))
+        _check(fn(self.generation, p.ctypes.data, len(p), None, 0, C.byref(written)))
         out = np.zeros(written.value)
-        _check(library().vckss_mi355x_smoother_omega_v1(self.generation, p.ctypes.data, len(p), out.ctypes.data,
-                                                        len(out), C.byref(written)))
+        _check(fn(self.generation, p.ctypes.data, len(p), out.ctypes.data, len(out), C.byref(written)))
         return out
 
     def smoother_gradient(self, params, errors):

@@ -16,6 +16,10 @@ namespace vk {
 // Size of the no-allocation scratch (doubles, 32 MiB): 2 n columns + chunks
 // x columns of the level must fit.
 constexpr uint64_t kNoAllocScratchDoubles = 4ull << 20;
+// Parameters of the learned smoother network (8 features, an input layer and
+// three message-passing layers of width 64, a scalar output).
+constexpr int kSmootherLayers = 3;
+constexpr uint32_t kSmootherParams = 64 * 8 + 64 + kSmootherLayers * (2 * 64 * 64 + 64) + 64 + 1;
 
 struct SolveOptions {
   uint64_t seed = 8675309;
@@ -178,7 +182,9 @@ class DeviceEngine {
   SmootherStatus smoother_status() const;
   // omega of every node of every smoothed level (levels in order, original
   // vertex order within a level).
-  void smoother_omega(const std::vector<double> &theta, std::vector<double> &omega);
+  // omega of every smoothed level for parameters theta: the FP64 training
+  // forward, or with `bf16` the solve path's BF16 inference
+  void smoother_omega(const std::vector<double> &theta, std::vector<double> &omega, bool bf16 = false);
   // Training loss and gradient for 32 given error columns (n x 32, original
   // vertex order).
   double smoother_gradient(const std::vector<double> &theta, const std::vector<double> &errors,

@@ -198,7 +198,13 @@ inline std::vector<double> from_renumbered(const std::vector<double> &in, const 
 // training V-cycle vectors (32 error columns).
 struct SmootherLevel {
   uint32_t level = 0, n = 0;
-  DBuf<double> X, rs, h0, h1, h2, m1, m2, omega, gomega;
+  // activations of the input layer and the message-passing layers (h[0..L])
+  // and the messages of each message-passing layer (m[t - 1] for layer t)
+  DBuf<double> X, rs, h[kSmootherLayers + 1], m[kSmootherLayers], omega, gomega;
+  // the solve path's BF16 inference: activations (two alternating), messages,
+  // and the FP32 last layer
+  DBuf<uint16_t> hb[2], mb;
+  DBuf<float> h32;
 };
 struct TrainLevel {
   DBuf<double> braw, b, s, t1, t2, t3, y, ly, mean;
@@ -207,6 +213,7 @@ struct SmootherState {
   std::vector<SmootherLevel> L;
   std::vector<TrainLevel> T;
   DBuf<double> theta, grad, am, av, partial, dz, dh, dpre, dms, dtmp;
+  DBuf<uint16_t> wcat;  // BF16 layer weights of the inference path
   DBuf<double> err, ae, u, au, lam, q, uau;
   bool features = false, train_buffers = false;
   int mode = 0;             // 0 plain cycle, 1 learned weights with fallback

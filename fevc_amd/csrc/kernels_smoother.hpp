@@ -1,6 +1,7 @@
-// Learned CMG smoother weights: the message-passing network that predicts a
-// per-node Jacobi weight omega_i in (0, 0.95) for every smoothed level, its
-// backward pass, and the training V-cycle with its exact adjoint.
+// Learned CMG smoother weights: the message-passing network (an input layer
+// and three message-passing layers of width 64) that predicts a per-node
+// Jacobi weight omega_i in (0, 0.95) for every smoothed level, its backward
+// pass, and the training V-cycle with its exact adjoint.
 //
 // No counterpart in the reference: the reference cycle smooths with the fixed
 // inverse diagonal 0.5 / d_i (vendor/cmg/src/preconditioner/cycle.rs:136-311).
@@ -8,10 +9,11 @@
 // keeps 2 D / omega - A positive definite, so the symmetric V-cycle remains a
 // valid PCG preconditioner.
 //
-// Layouts: node-major rows of H = 32 hidden channels, h[i * 32 + c]; the 8
+// Layouts: node-major rows of H = 64 hidden channels, h[i * 64 + c]; the 8
 // input features X[i * 8 + f]; training vectors interleave 32 error columns,
-// v[i * 32 + lane].  All arithmetic is FP64 (the network is tiny next to a
-// solve; FP64 keeps the gradient comparable with the CPU oracle).
+// v[i * 32 + lane].  Training runs in FP64 (its gradient is compared with the
+// CPU oracle); the solve path's inference runs in BF16 with FP32 accumulation
+// on the matrix cores (k_mpnn_layer_bf16).
 #pragma once
 
 #include "kernels.hpp"
@@ -21,14 +23,18 @@ namespace dev {
 namespace mpnn {
 
 constexpr int F = 8;
-constexpr int H = 32;
+constexpr int H = 64;
+constexpr int LAYERS = 3;  // message-passing layers after the input layer
 constexpr double kOmegaMax = 0.95;
-// flat parameter layout: W0[H][F], b0[H], U1[H][H], V1[H][H], c1[H],
-// U2[H][H], V2[H][H], c2[H], a[H], a0
-constexpr int kW0 = 0, kB0 = kW0 + H * F, kU1 = kB0 + H, kV1 = kU1 + H * H, kC1 = kV1 + H * H;
-constexpr int kU2 = kC1 + H, kV2 = kU2 + H * H, kC2 = kV2 + H * H, kA = kC2 + H, kA0 = kA + H;
+// flat parameter layout: W0[H][F], b0[H], then per layer t = 1..LAYERS
+// U_t[H][H], V_t[H][H], c_t[H]; then a[H], a0
+constexpr int kW0 = 0, kB0 = kW0 + H * F, kLayer0 = kB0 + H, kLayerSize = 2 * H * H + H;
+constexpr int kU(int t) { return kLayer0 + (t - 1) * kLayerSize; }
+constexpr int kV(int t) { return kU(t) + H * H; }
+constexpr int kC(int t) { return kV(t) + H * H; }
+constexpr int kA = kLayer0 + LAYERS * kLayerSize, kA0 = kA + H;
 constexpr int P = kA0 + 1;
-constexpr int kNodesPerBlock = kBlock / H;  // 8
+constexpr int kNodesPerBlock = kBlock / H;  // 4
 
 }  // namespace mpnn
 
@@ -63,17 +69,153 @@ static __global__ void __launch_bounds__(kBlock) k_mpnn_layer(uint32_t n, const 
   }
 }
 
-// omega = 0.95 sigmoid(a . h2 + a0); the smoother inverse diagonal is
+// omega = 0.95 sigmoid(a . h_L + a0); the smoother inverse diagonal is
 // omega / d = 2 omega inv_plain (inv_plain = 0.5 / d).
-static __global__ void __launch_bounds__(kBlock) k_mpnn_out(uint32_t n, const double *__restrict__ h2,
+static __global__ void __launch_bounds__(kBlock) k_mpnn_out(uint32_t n, const double *__restrict__ hl,
                                                      const double *__restrict__ th,
                                                      const double *__restrict__ inv_plain, double *__restrict__ omega,
                                                      double *__restrict__ inv64, float *__restrict__ inv32) {
   using namespace mpnn;
   VK_GRID_LOOP(i, (uint64_t)n) {
     double z = th[kA0];
-    for (int c = 0; c < H; ++c) z += th[kA + c] * h2[i * H + c];
+    for (int c = 0; c < H; ++c) z += th[kA + c] * hl[i * H + c];
     const double w = kOmegaMax / (1.0 + exp(-z));
+    if (omega) omega[i] = w;
+    const double v = 2.0 * w * inv_plain[i];
+    if (inv64) inv64[i] = v;
+    if (inv32) inv32[i] = (float)v;
+  }
+}
+
+// ------------------------------------------------ BF16 inference (solve path)
+// The solve path evaluates the network with BF16 activations and weights and
+// FP32 accumulation: the input layer and messages on the vector units, the
+// 64 x 128 layer products on the matrix cores (v_mfma_f32_16x16x16_bf16).
+// BF16 values are the upper half of an FP32, rounded to nearest even.
+__device__ inline uint16_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Layer weights of every message-passing layer as BF16 rows
+// wcat[t - 1][c][k] = U_t[c][k] (k < H) and V_t[c][k - H] (k >= H).
+static __global__ void __launch_bounds__(kBlock) k_mpnn_weights_bf16(const double *__restrict__ th,
+                                                                     uint16_t *__restrict__ wcat) {
+  using namespace mpnn;
+  VK_GRID_LOOP(e, (uint64_t)LAYERS * H * 2 * H) {
+    const int t = (int)(e / (H * 2 * H)) + 1, c = (int)(e / (2 * H) % H), k = (int)(e % (2 * H));
+    const double w = k < H ? th[kU(t) + c * H + k] : th[kV(t) + c * H + (k - H)];
+    wcat[e] = bf16_bits((float)w);
+  }
+}
+
+// h0 = relu(W0 x + b0) in FP32, stored as BF16.
+static __global__ void __launch_bounds__(kBlock) k_mpnn_in_bf16(uint32_t n, const double *__restrict__ X,
+                                                                const double *__restrict__ th,
+                                                                uint16_t *__restrict__ h0) {
+  using namespace mpnn;
+  VK_GRID_LOOP(t, (uint64_t)n * H) {
+    const uint64_t i = t / H;
+    const int c = t % H;
+    float s = (float)th[kB0 + c];
+#pragma unroll
+    for (int f = 0; f < F; ++f) s += (float)th[kW0 + c * F + f] * (float)X[i * F + f];
+    h0[t] = bf16_bits(s > 0.0f ? s : 0.0f);
+  }
+}
+
+// Messages m_i = rs_i sum_k w_ik h_k over the level's chunked CSR rows (64
+// channels per row, FP32 sums in entry order), stored as BF16.
+template <int K>
+struct OpMessageBf16 {
+  const uint32_t *__restrict__ col;
+  const double *__restrict__ val;
+  const uint16_t *__restrict__ h;
+  const double *__restrict__ rs;
+  uint16_t *__restrict__ m;
+  template <int STEP = Shape<K>::V>
+  __device__ float partial(const uint4 &d, int j, int s) const {
+    float acc = 0.0f;
+    for (uint32_t k = d.y + s; k < d.z; k += STEP) acc += (float)val[k] * bf16_value(h[(uint64_t)col[k] * K + j]);
+    return acc;
+  }
+  __device__ bool enabled() const { return true; }
+  template <int NA>
+  __device__ void finish(uint32_t i, int j, float sum, double (&)[NA]) const {
+    m[(uint64_t)i * K + j] = bf16_bits((float)rs[i] * sum);
+  }
+  template <int NV>
+  __device__ void reduced(int, const double (&)[NV]) const {}
+};
+
+// One message-passing layer on the matrix cores: out = relu([h m] W_t^T + c_t)
+// for a 16-node tile per wavefront (four tiles per block).  The MFMA operand
+// layouts of v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + i]
+// and B[4 (l / 16) + i][l % 16], i = 0..3, and receives D[4 (l / 16) + i][l % 16].
+// The layer output is BF16 (out16) or, for the last layer, FP32 (out32).
+typedef short mfma_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
+static __global__ void __launch_bounds__(kBlock) k_mpnn_layer_bf16(uint32_t n, const uint16_t *__restrict__ h,
+                                                                   const uint16_t *__restrict__ m,
+                                                                   const uint16_t *__restrict__ wcat,
+                                                                   const double *__restrict__ th, int c_off,
+                                                                   uint16_t *__restrict__ out16,
+                                                                   float *__restrict__ out32) {
+  using namespace mpnn;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lo = lane & 15, quad = lane >> 4;
+  const uint64_t tiles = ((uint64_t)n + 15) / 16;
+  for (uint64_t tile = (uint64_t)blockIdx.x * (kBlock / 64) + wave; tile < tiles;
+       tile += (uint64_t)gridDim.x * (kBlock / 64)) {
+    const uint64_t row = tile * 16 + lo;
+    mfma_f32x4 acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = mfma_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2 * H / 16; ++kb) {
+      const int k = kb * 16 + quad * 4;  // the lane's four K indices, k .. k + 3
+      mfma_bf16x4 a = {0, 0, 0, 0};
+      if (row < n) {
+        const uint16_t *src = k < H ? h + row * H + k : m + row * H + (k - H);
+        a = *reinterpret_cast<const mfma_bf16x4 *>(src);
+      }
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const mfma_bf16x4 b = *reinterpret_cast<const mfma_bf16x4 *>(wcat + (uint64_t)(cb * 16 + lo) * 2 * H + k);
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[cb], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int c = cb * 16 + lo;
+      const float bias = (float)th[c_off + c];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t node = tile * 16 + quad * 4 + i;
+        if (node >= n) continue;
+        float v = acc[cb][i] + bias;
+        v = v > 0.0f ? v : 0.0f;
+        if (out32) out32[node * H + c] = v;
+        else out16[node * H + c] = bf16_bits(v);
+      }
+    }
+  }
+}
+
+// omega = 0.95 sigmoid(a . h_L + a0) from the FP32 last layer (FP32 sums).
+static __global__ void __launch_bounds__(kBlock) k_mpnn_out_f32(uint32_t n, const float *__restrict__ hl,
+                                                                const double *__restrict__ th,
+                                                                const double *__restrict__ inv_plain,
+                                                                double *__restrict__ omega, double *__restrict__ inv64,
+                                                                float *__restrict__ inv32) {
+  using namespace mpnn;
+  VK_GRID_LOOP(i, (uint64_t)n) {
+    float z = (float)th[kA0];
+    for (int c = 0; c < H; ++c) z += (float)th[kA + c] * hl[i * H + c];
+    const double w = kOmegaMax / (1.0 + exp(-(double)z));
     if (omega) omega[i] = w;
     const double v = 2.0 * w * inv_plain[i];
     if (inv64) inv64[i] = v;
@@ -93,23 +235,23 @@ static __global__ void __launch_bounds__(kBlock) k_constant_omega(uint32_t n, do
   }
 }
 
-// Output layer backward: dz = g_omega * 0.95 s (1 - s), dh2 = dz a.
+// Output layer backward: dz = g_omega * 0.95 s (1 - s), dh_L = dz a.
 static __global__ void __launch_bounds__(kBlock) k_mpnn_out_back(uint32_t n, const double *__restrict__ omega,
                                                           const double *__restrict__ g_omega,
                                                           const double *__restrict__ th, double *__restrict__ dz,
-                                                          double *__restrict__ dh2) {
+                                                          double *__restrict__ dhl) {
   using namespace mpnn;
   VK_GRID_LOOP(t, (uint64_t)n * H) {
     const uint64_t i = t / H;
     const int c = t % H;
     const double s = omega[i] / kOmegaMax;
     const double d = g_omega[i] * kOmegaMax * s * (1.0 - s);
-    dh2[t] = d * th[kA + c];
+    dhl[t] = d * th[kA + c];
     if (c == 0) dz[i] = d;
   }
 }
 
-// Hidden layer backward for nodes in groups of 8 per block:
+// Hidden layer backward for nodes in groups of kNodesPerBlock per block:
 //   dpre = dh * [h > 0], dhp = U^T dpre, dms = (V^T dpre) / s_i
 // (dms is then scattered through the adjacency: dhp += W dms).
 static __global__ void __launch_bounds__(kBlock) k_mpnn_layer_back(uint32_t n, const double *__restrict__ dh,

@@ -19,6 +19,8 @@ namespace vk {
 namespace {
 
 namespace M = dev::mpnn;
+static_assert(M::P == (int)kSmootherParams && M::LAYERS == kSmootherLayers,
+              "network layout and the declared parameter count differ");
 constexpr int KT = 32;  // training columns
 constexpr unsigned kOuterBlocks = 512;
 
@@ -80,6 +82,9 @@ void level_features(const Level &L, const std::vector<uint32_t> &perm, uint32_t 
 
 void ensure_features(Impl &E) {
   SmootherState &S = E.sm;
+  // the network's segment passes run H lanes: the segment scratch must hold
+  // H lanes per chunk whatever width the solves use
+  E.ensure(M::H);
   if (S.features) return;
   const Hierarchy &h = *E.H;
   S.L.clear();
@@ -95,7 +100,11 @@ void ensure_features(Impl &E) {
     Z.X.upload(X);
     Z.rs.upload(rs);
     const size_t m = (size_t)Z.n * M::H;
-    for (auto *v : {&Z.h0, &Z.h1, &Z.h2, &Z.m1, &Z.m2}) v->alloc(m);
+    for (auto &v : Z.h) v.alloc(m);
+    for (auto &v : Z.m) v.alloc(m);
+    for (auto &v : Z.hb) v.alloc(m);
+    Z.mb.alloc(m);
+    Z.h32.alloc(m);
     Z.omega.alloc(Z.n);
     Z.gomega.alloc(Z.n);
     E.levels[l].linv.alloc(Z.n);
@@ -105,6 +114,7 @@ void ensure_features(Impl &E) {
   const size_t m = std::max<size_t>(maxn, 1) * M::H;
   for (auto *v : {&S.dh, &S.dpre, &S.dms, &S.dtmp}) v->alloc(m);
   S.dz.alloc(std::max<size_t>(maxn, 1));
+  S.wcat.alloc((size_t)M::LAYERS * M::H * 2 * M::H);
   S.theta.alloc(M::P);
   S.grad.alloc(M::P);
   S.am.alloc(M::P);
@@ -126,15 +136,41 @@ void mpnn_forward(Impl &E) {
   for (SmootherLevel &Z : S.L) {
     DevLevel &D = E.levels[Z.level];
     const uint64_t m = (uint64_t)Z.n * M::H;
-    hipLaunchKernelGGL(dev::k_mpnn_in, grid_for(m), kBlock, 0, E.stream, Z.n, Z.X.p, th, Z.h0.p);
-    adjacency(E, D, Z.h0.p, Z.rs.p, Z.m1.p, false);
-    hipLaunchKernelGGL(dev::k_mpnn_layer, grid_for(m), kBlock, 0, E.stream, Z.n, Z.h0.p, Z.m1.p, th, M::kU1, M::kV1,
-                       M::kC1, Z.h1.p);
-    adjacency(E, D, Z.h1.p, Z.rs.p, Z.m2.p, false);
-    hipLaunchKernelGGL(dev::k_mpnn_layer, grid_for(m), kBlock, 0, E.stream, Z.n, Z.h1.p, Z.m2.p, th, M::kU2, M::kV2,
-                       M::kC2, Z.h2.p);
-    hipLaunchKernelGGL(dev::k_mpnn_out, grid_for(Z.n), kBlock, 0, E.stream, Z.n, Z.h2.p, th, D.inv.p, Z.omega.p,
-                       D.linv.p, D.linv32.p);
+    hipLaunchKernelGGL(dev::k_mpnn_in, grid_for(m), kBlock, 0, E.stream, Z.n, Z.X.p, th, Z.h[0].p);
+    for (int t = 1; t <= M::LAYERS; ++t) {
+      adjacency(E, D, Z.h[t - 1].p, Z.rs.p, Z.m[t - 1].p, false);
+      hipLaunchKernelGGL(dev::k_mpnn_layer, grid_for(m), kBlock, 0, E.stream, Z.n, Z.h[t - 1].p, Z.m[t - 1].p, th,
+                         M::kU(t), M::kV(t), M::kC(t), Z.h[t].p);
+    }
+    hipLaunchKernelGGL(dev::k_mpnn_out, grid_for(Z.n), kBlock, 0, E.stream, Z.n, Z.h[M::LAYERS].p, th, D.inv.p,
+                       Z.omega.p, D.linv.p, D.linv32.p);
+  }
+}
+
+// The solve path's inference of the same network: BF16 weights and
+// activations, FP32 accumulation, the layer products on the matrix cores.
+void mpnn_infer(Impl &E) {
+  SmootherState &S = E.sm;
+  const double *th = S.theta.p;
+  const uint64_t wc = (uint64_t)M::LAYERS * M::H * 2 * M::H;
+  hipLaunchKernelGGL(dev::k_mpnn_weights_bf16, grid_for(wc), kBlock, 0, E.stream, th, S.wcat.p);
+  for (SmootherLevel &Z : S.L) {
+    DevLevel &D = E.levels[Z.level];
+    const uint64_t m = (uint64_t)Z.n * M::H;
+    hipLaunchKernelGGL(dev::k_mpnn_in_bf16, grid_for(m), kBlock, 0, E.stream, Z.n, Z.X.p, th, Z.hb[0].p);
+    const uint64_t tiles = ((uint64_t)Z.n + 15) / 16;
+    const unsigned blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 16384));
+    int cur = 0;
+    for (int t = 1; t <= M::LAYERS; ++t) {
+      segments<float, M::H, 0>(E, D.rows, dev::OpMessageBf16<M::H>{D.col.p, D.val.p, Z.hb[cur].p, Z.rs.p, Z.mb.p});
+      const bool last = t == M::LAYERS;
+      hipLaunchKernelGGL(dev::k_mpnn_layer_bf16, blocks, kBlock, 0, E.stream, Z.n, Z.hb[cur].p, Z.mb.p,
+                         S.wcat.p + (uint64_t)(t - 1) * M::H * 2 * M::H, th, M::kC(t),
+                         last ? nullptr : Z.hb[1 - cur].p, last ? Z.h32.p : nullptr);
+      cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(dev::k_mpnn_out_f32, grid_for(Z.n), kBlock, 0, E.stream, Z.n, Z.h32.p, th, D.inv.p,
+                       Z.omega.p, D.linv.p, D.linv32.p);
   }
 }
 
@@ -158,21 +194,19 @@ void mpnn_backward(Impl &E, SmootherLevel &Z) {
                                                                                    M::kNodesPerBlock));
   hipLaunchKernelGGL(dev::k_mpnn_out_back, grid_for(m), kBlock, 0, E.stream, n, Z.omega.p, Z.gomega.p, th, S.dz.p,
                      S.dh.p);
-  outer<1, M::H, true>(E, n, S.dz.p, Z.h2.p, g + M::kA);
-  // layer 2: dh (= dh2) -> dtmp (= dh1)
-  hipLaunchKernelGGL(dev::k_mpnn_layer_back, lb, kBlock, 0, E.stream, n, S.dh.p, Z.h2.p, th, M::kU2, M::kV2, Z.rs.p,
-                     S.dpre.p, S.dtmp.p, S.dms.p);
-  outer<M::H, M::H, false>(E, n, S.dpre.p, Z.h1.p, g + M::kU2);
-  outer<M::H, M::H, true>(E, n, S.dpre.p, Z.m2.p, g + M::kV2);
-  adjacency(E, D, S.dms.p, nullptr, S.dtmp.p, true);
-  // layer 1: dtmp (= dh1) -> dh (= dh0)
-  hipLaunchKernelGGL(dev::k_mpnn_layer_back, lb, kBlock, 0, E.stream, n, S.dtmp.p, Z.h1.p, th, M::kU1, M::kV1,
-                     Z.rs.p, S.dpre.p, S.dh.p, S.dms.p);
-  outer<M::H, M::H, false>(E, n, S.dpre.p, Z.h0.p, g + M::kU1);
-  outer<M::H, M::H, true>(E, n, S.dpre.p, Z.m1.p, g + M::kV1);
-  adjacency(E, D, S.dms.p, nullptr, S.dh.p, true);
+  outer<1, M::H, true>(E, n, S.dz.p, Z.h[M::LAYERS].p, g + M::kA);
+  // layers L..1: dh (= dh_t) -> dtmp (= dh_{t-1}), the buffers then swap
+  double *dh = S.dh.p, *dnext = S.dtmp.p;
+  for (int t = M::LAYERS; t >= 1; --t) {
+    hipLaunchKernelGGL(dev::k_mpnn_layer_back, lb, kBlock, 0, E.stream, n, dh, Z.h[t].p, th, M::kU(t), M::kV(t),
+                       Z.rs.p, S.dpre.p, dnext, S.dms.p);
+    outer<M::H, M::H, false>(E, n, S.dpre.p, Z.h[t - 1].p, g + M::kU(t));
+    outer<M::H, M::H, true>(E, n, S.dpre.p, Z.m[t - 1].p, g + M::kV(t));
+    adjacency(E, D, S.dms.p, nullptr, dnext, true);
+    std::swap(dh, dnext);
+  }
   // input layer
-  hipLaunchKernelGGL(dev::k_relu_back, grid_for(m), kBlock, 0, E.stream, m, S.dh.p, Z.h0.p, S.dpre.p);
+  hipLaunchKernelGGL(dev::k_relu_back, grid_for(m), kBlock, 0, E.stream, m, dh, Z.h[0].p, S.dpre.p);
   outer<M::H, M::F, true>(E, n, S.dpre.p, Z.X.p, g + M::kW0);
 }
 
@@ -351,14 +385,15 @@ double iteration_work(const Impl &E, const std::vector<uint32_t> &shape);
 
 }  // namespace
 
-void DeviceEngine::smoother_omega(const std::vector<double> &theta, std::vector<double> &omega) {
+void DeviceEngine::smoother_omega(const std::vector<double> &theta, std::vector<double> &omega, bool bf16) {
   Impl &E = *impl_;
   check_params(theta);
   E.gate = E.always();
   E.ensure(M::H);
   ensure_features(E);
   upload_theta(E, theta);
-  mpnn_forward(E);
+  if (bf16) mpnn_infer(E);
+  else mpnn_forward(E);
   omega.clear();
   for (SmootherLevel &Z : E.sm.L) {
     std::vector<double> w;
@@ -368,7 +403,7 @@ void DeviceEngine::smoother_omega(const std::vector<double> &theta, std::vector<
   }
   if (E.sm.active) {  // restore the inverse diagonals of the weights in use
     upload_theta(E, E.sm.params);
-    mpnn_forward(E);
+    mpnn_infer(E);
     VK_HIP(hipStreamSynchronize(E.stream));
   }
 }
@@ -390,7 +425,7 @@ double DeviceEngine::smoother_gradient(const std::vector<double> &theta, const s
   E.sm.grad.download(grad, M::P, E.stream);
   if (keep.saved) {
     upload_theta(E, E.sm.params);
-    mpnn_forward(E);
+    mpnn_infer(E);
     VK_HIP(hipStreamSynchronize(E.stream));
   }
   return L;
@@ -431,7 +466,7 @@ void DeviceEngine::smoother_train(std::vector<double> &state, uint64_t step0, ui
   std::copy(vv.begin(), vv.end(), state.begin() + 2 * M::P);
   if (keep.saved) {
     upload_theta(E, S.params);
-    mpnn_forward(E);
+    mpnn_infer(E);
     VK_HIP(hipStreamSynchronize(E.stream));
   }
 }
@@ -491,7 +526,7 @@ void DeviceEngine::smoother_mode(int mode, const std::vector<double> &theta, int
   E.ensure(M::H);
   ensure_features(E);
   upload_theta(E, theta);
-  mpnn_forward(E);
+  mpnn_infer(E);
   VK_HIP(hipStreamSynchronize(E.stream));
   // Calibration: one leverage batch of 16 probes to rel.res 1e-8 with the
   // reference cycle, then with the learned weights and/or the tuned shape.

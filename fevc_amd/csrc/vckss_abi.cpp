@@ -360,7 +360,7 @@ DeviceEngine &device_of(Context &ctx) {
   if (!ctx.dev) {
     ctx.dev.reset(new DeviceEngine(ctx.P, ctx.HS, ctx.H));
     // the built-in trained smoother weights, used where they help
-    if (kBuiltinSmootherCount) ctx.dev->smoother_default(builtin_smoother());
+    if (kBuiltinSmootherCount == kSmootherParams) ctx.dev->smoother_default(builtin_smoother());
   }
   return *ctx.dev;
 }
@@ -1836,7 +1836,8 @@ int32_t vckss_mi355x_smoother_v1(uint64_t generation, uint32_t mode, const doubl
       if (params) {
         theta.assign(params, params + count);
       } else {
-        if (!kBuiltinSmootherCount) fail(kUnsupported, "smoother", "the library carries no trained smoother weights");
+        if (kBuiltinSmootherCount != kSmootherParams)
+          fail(kUnsupported, "smoother", "the library carries no trained smoother weights for this network");
         theta = builtin_smoother();
       }
     }
@@ -1914,28 +1915,39 @@ int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint3
 int32_t vckss_mi355x_smoother_params_v1(double *out, uint32_t capacity, uint32_t *count) {
   return guarded([&] {
     if (!count) fail(kInvalidInput, "smoother_params", "count is required");
-    *count = (uint32_t)kBuiltinSmootherCount;
-    if (out) {
+    // parameters of another network layout are not the library's weights
+    *count = kBuiltinSmootherCount == kSmootherParams ? (uint32_t)kBuiltinSmootherCount : 0u;
+    if (out && *count) {
       if (capacity < kBuiltinSmootherCount) fail(kAbiMismatch, "smoother_params", "capacity too small");
       std::memcpy(out, kBuiltinSmoother, kBuiltinSmootherCount * sizeof(double));
     }
   });
 }
 
-int32_t vckss_mi355x_smoother_omega_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
-                                       uint64_t capacity, uint64_t *written) {
+static int32_t smoother_omega_entry(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                    uint64_t capacity, uint64_t *written, bool bf16) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {
     Context &ctx = live(generation);
     if (!params || !written) fail(kInvalidInput, "smoother_omega", "params and written are required");
     std::vector<double> w;
-    device_of(ctx).smoother_omega(std::vector<double>(params, params + count), w);
+    device_of(ctx).smoother_omega(std::vector<double>(params, params + count), w, bf16);
     *written = w.size();
     if (omega) {
       if (capacity < w.size()) fail(kAbiMismatch, "smoother_omega", "capacity too small");
       std::memcpy(omega, w.data(), w.size() * sizeof(double));
     }
   });
+}
+
+int32_t vckss_mi355x_smoother_omega_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                       uint64_t capacity, uint64_t *written) {
+  return smoother_omega_entry(generation, params, count, omega, capacity, written, false);
+}
+
+int32_t vckss_mi355x_smoother_infer_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                       uint64_t capacity, uint64_t *written) {
+  return smoother_omega_entry(generation, params, count, omega, capacity, written, true);
 }
 
 int32_t vckss_mi355x_smoother_gradient_v1(uint64_t generation, const double *params, uint32_t count,

@@ -2,8 +2,8 @@
 fevc_amd/smoother_params.json (built into the library by fevc_amd/build.py).
 
 Training data are generated by the method itself: worker-firm panels of the
-project's generator (50k-600k workers, T = 5, mobility 0.05-0.3, Pareto firm
-sizes), their hybrid Laplacians and CMG hierarchies built by the engine, and
+project's generator (200 panels of 50k-1M workers, T = 5, mobility
+0.05-0.3, Pareto firm sizes), their hybrid Laplacians and CMG hierarchies built by the engine, and
 error vectors that the plain V-cycle leaves behind.  Every step is one Adam
 step of the device network on one hierarchy
 (vckss_mi355x_smoother_train_v1).  Held-out panels then report the PCG
@@ -30,10 +30,13 @@ import fevc_amd as F  # noqa: E402
 from fevc_amd import synth  # noqa: E402
 
 COLS = ("worker", "firm", "deletion", "outcome", "frequency", "target")
-FEATURES, HIDDEN, OMEGA_MAX = 8, 32, 0.95
-# parameter layout of the device network: W0[H][F], b0, U1, V1, c1, U2, V2, c2, a, a0
-SIZES = (("W0", HIDDEN * FEATURES), ("b0", HIDDEN), ("U1", HIDDEN * HIDDEN), ("V1", HIDDEN * HIDDEN),
-         ("c1", HIDDEN), ("U2", HIDDEN * HIDDEN), ("V2", HIDDEN * HIDDEN), ("c2", HIDDEN), ("a", HIDDEN), ("a0", 1))
+FEATURES, HIDDEN, LAYERS, OMEGA_MAX = 8, 64, 3, 0.95
+# parameter layout of the device network: W0[H][F], b0, then U_t, V_t, c_t of
+# each message-passing layer, then a, a0
+SIZES = ((("W0", HIDDEN * FEATURES), ("b0", HIDDEN))
+         + tuple(x for t in range(1, LAYERS + 1)
+                 for x in ((f"U{t}", HIDDEN * HIDDEN), (f"V{t}", HIDDEN * HIDDEN), (f"c{t}", HIDDEN)))
+         + (("a", HIDDEN), ("a0", 1)))
 COUNT = sum(s for _, s in SIZES)
 
 
@@ -44,7 +47,7 @@ def initial_params(seed: int, scale: float = 0.1) -> np.ndarray:
     theta = np.zeros(COUNT)
     o = 0
     for name, size in SIZES:
-        if name in ("W0", "U1", "V1", "U2", "V2"):
+        if name == "W0" or name[0] in "UV":
             fan_in = FEATURES if name == "W0" else HIDDEN
             theta[o:o + size] = rng.standard_normal(size) * scale / math.sqrt(fan_in)
         o += size
@@ -57,7 +60,7 @@ def training_panels(count: int, seed: int):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        workers = int(math.exp(rng.uniform(math.log(50_000), math.log(600_000))))
+        workers = int(math.exp(rng.uniform(math.log(50_000), math.log(1_000_000))))
         out.append(synth.PanelSpec(workers=workers, firms=max(workers // 20, 50), periods=5,
                                    mobility=float(rng.uniform(0.05, 0.3)),
                                    pareto_shape=float(rng.uniform(1.05, 1.3)), size_cap=5000.0,
@@ -85,9 +88,12 @@ def log(msg):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--panels", type=int, default=24)
-    ap.add_argument("--epochs", type=int, default=2)
-    ap.add_argument("--steps", type=int, default=20, help="Adam steps per panel visit")
+    ap.add_argument("--panels", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="Adam steps per panel visit")
+    ap.add_argument("--count", type=int, default=0,
+                    help="panels of this run (0: to the end); with --state the run resumes and saves there")
+    ap.add_argument("--state", default=None, help="training checkpoint (.npz) to resume from and save into")
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--passes", type=int, default=2, help="plain-cycle error steps per training batch")
     ap.add_argument("--seed", type=int, default=1)
@@ -100,23 +106,34 @@ def main():
     step = 0
     total = args.epochs * args.panels * args.steps
     history = []
-    for epoch in range(args.epochs):
-        for spec in panels:
-            e = engine(spec)
-            try:
-                lr = args.lr * (0.1 + 0.9 * (1.0 - step / total))
-                losses = e.smoother_train(state, step, args.steps, lr,
-                                          seed=(args.seed << 20) ^ (epoch << 10) ^ spec.seed,
-                                          passes=args.passes)
-                dims = e.dims
-            finally:
-                e.release()
-            step += args.steps
-            history.append(dict(epoch=epoch, panel=spec.name, workers=spec.workers,
-                                hybrid_vertices=dims["hybrid_vertices"], levels=dims["levels"],
-                                first_loss=float(losses[0]), last_loss=float(losses[-1])))
-            log(f"epoch {epoch} {spec.name}: n={dims['hybrid_vertices']} levels={dims['levels']} "
-                f"loss {losses[0]:.4f} -> {losses[-1]:.4f} (lr {lr:.2e})")
+    done = 0  # panel visits already trained
+    if args.state and os.path.exists(args.state):
+        ck = np.load(args.state, allow_pickle=False)
+        state, step, done = ck["state"], int(ck["step"]), int(ck["done"])
+        history = json.loads(str(ck["history"]))
+        log(f"resumed at visit {done} (step {step})")
+    visits = [(epoch, spec) for epoch in range(args.epochs) for spec in panels]
+    stop = len(visits) if args.count == 0 else min(len(visits), done + args.count)
+    for epoch, spec in visits[done:stop]:
+        e = engine(spec)
+        try:
+            lr = args.lr * (0.1 + 0.9 * (1.0 - step / total))
+            losses = e.smoother_train(state, step, args.steps, lr,
+                                      seed=(args.seed << 20) ^ (epoch << 10) ^ spec.seed,
+                                      passes=args.passes)
+            dims = e.dims
+        finally:
+            e.release()
+        step += args.steps
+        history.append(dict(epoch=epoch, panel=spec.name, workers=spec.workers,
+                            hybrid_vertices=dims["hybrid_vertices"], levels=dims["levels"],
+                            first_loss=float(losses[0]), last_loss=float(losses[-1])))
+        log(f"epoch {epoch} {spec.name}: n={dims['hybrid_vertices']} levels={dims['levels']} "
+            f"loss {losses[0]:.4f} -> {losses[-1]:.4f} (lr {lr:.2e})")
+    if stop < len(visits):
+        np.savez(args.state, state=state, step=step, done=stop, history=json.dumps(history))
+        log(f"saved the training state after visit {stop} of {len(visits)} to {args.state}")
+        return
     params = state[:COUNT].copy()
     evaluation = []
     for spec in held_out_panels():

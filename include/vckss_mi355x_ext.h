@@ -188,6 +188,10 @@ int32_t vckss_mi355x_smoother_params_v1(double *out, uint32_t capacity, uint32_t
  * levels in order, original vertex order within a level. */
 int32_t vckss_mi355x_smoother_omega_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
                                        uint64_t capacity, uint64_t *written);
+/* The same omega from the solve path's inference: BF16 weights and
+ * activations with FP32 accumulation, layer products on the matrix cores. */
+int32_t vckss_mi355x_smoother_infer_v1(uint64_t generation, const double *params, uint32_t count, double *omega,
+                                       uint64_t capacity, uint64_t *written);
 /* Training loss L = mean_j |e_j - M A e_j|_A^2 / |e_j|_A^2 over 32 error
  * columns (rows x 32, vertex-major, original hybrid-vertex order) and dL/dparams. */
 int32_t vckss_mi355x_smoother_gradient_v1(uint64_t generation, const double *params, uint32_t count,

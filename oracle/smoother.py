@@ -13,7 +13,9 @@ symmetric V-cycle stays positive definite.
 
 Everything here is the definition the device kernels implement:
   features(level)       8 per-node inputs
-  forward(params, ...)  the network: omega per node of every smoothed level
+  forward(params, ...)  the network (input layer, three message-passing
+                        layers of width 64): omega per node of every
+                        smoothed level
   cycle_forward/adjoint the training V-cycle (one visit per level) and its
                         exact adjoint, giving dL/domega
   loss_and_grad         L = mean_k |e_k - M A e_k|_A^2 / |e_k|_A^2 and dL/dparams
@@ -28,18 +30,21 @@ from __future__ import annotations
 import numpy as np
 
 F = 8          # input features per node
-H = 32         # hidden width
+H = 64         # hidden width
 OMEGA_MAX = 0.95
-LAYERS = 2
+LAYERS = 3     # message-passing layers after the input layer
 
-# flat parameter layout
+# flat parameter layout: W0, b0, then U_t, V_t, c_t per layer, then a, a0
 OFF = {}
 _o = 0
-for _name, _size in (("W0", H * F), ("b0", H), ("U1", H * H), ("V1", H * H), ("c1", H),
-                     ("U2", H * H), ("V2", H * H), ("c2", H), ("a", H), ("a0", 1)):
+for _name, _size in ((("W0", H * F), ("b0", H))
+                     + tuple(x for t in range(1, LAYERS + 1)
+                             for x in ((f"U{t}", H * H), (f"V{t}", H * H), (f"c{t}", H)))
+                     + (("a", H), ("a0", 1))):
     OFF[_name] = (_o, _size)
     _o += _size
 P = _o
+MATRICES = tuple(f"{k}{t}" for t in range(1, LAYERS + 1) for k in ("U", "V"))
 
 
 def unpack(theta):
@@ -47,7 +52,7 @@ def unpack(theta):
     for k, (o, s) in OFF.items():
         out[k] = theta[o:o + s]
     out["W0"] = out["W0"].reshape(H, F)
-    for k in ("U1", "V1", "U2", "V2"):
+    for k in MATRICES:
         out[k] = out[k].reshape(H, H)
     return out
 
@@ -57,7 +62,7 @@ def initial_params(seed=1, scale=0.1):
     (the plain cycle) at the start of training."""
     rng = np.random.default_rng(seed)
     theta = np.zeros(P)
-    for k in ("W0", "U1", "V1", "U2", "V2"):
+    for k in ("W0",) + MATRICES:
         o, s = OFF[k]
         fan_in = F if k == "W0" else H
         theta[o:o + s] = rng.standard_normal(s) * scale / np.sqrt(fan_in)
@@ -186,7 +191,7 @@ def forward(theta, X, lv, rs):
     h0 = np.maximum(X @ p["W0"].T + p["b0"], 0.0)
     hs, ms = [h0], []
     h = h0
-    for t in (1, 2):
+    for t in range(1, LAYERS + 1):
         m = rs[:, None] * (Wm @ h)
         h = np.maximum(h @ p[f"U{t}"].T + m @ p[f"V{t}"].T + p[f"c{t}"], 0.0)
         ms.append(m)
@@ -196,6 +201,39 @@ def forward(theta, X, lv, rs):
     return OMEGA_MAX * sig, dict(hs=hs, ms=ms, sig=sig)
 
 
+def bf16(x):
+    """Round FP32 values to BF16 (nearest even), returned as FP32."""
+    a = np.ascontiguousarray(x, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) << 16
+    return u.view(np.float32)
+
+
+def forward_bf16(theta, X, lv, rs):
+    """The solve path's inference of the same network: BF16 weights and
+    activations (rounded where the device stores them), FP32 arithmetic;
+    omega as FP64 of 0.95 sigmoid(z)."""
+    p = unpack(theta)
+    Wm = lv.get("_W")
+    if Wm is None:
+        Wm = lv["_W"] = adjacency(lv)
+    W32 = Wm.astype(np.float32)
+    X32 = np.asarray(X, np.float32)
+    s = np.tile(p["b0"].astype(np.float32), (X.shape[0], 1))
+    for f in range(F):
+        s = s + p["W0"][:, f].astype(np.float32)[None, :] * X32[:, f:f + 1]
+    h = bf16(np.maximum(s, np.float32(0)))
+    for t in range(1, LAYERS + 1):
+        m = bf16(np.asarray(rs, np.float32)[:, None] * (W32 @ h))
+        U, V = bf16(p[f"U{t}"].astype(np.float32)), bf16(p[f"V{t}"].astype(np.float32))
+        pre = h @ U.T + m @ V.T + p[f"c{t}"].astype(np.float32)
+        h = np.maximum(pre, np.float32(0))
+        if t < LAYERS:
+            h = bf16(h)
+    z = h @ p["a"].astype(np.float32) + np.float32(p["a0"][0])
+    return OMEGA_MAX / (1.0 + np.exp(-z.astype(np.float64)))
+
+
 def backward(theta, X, lv, rs, cache, g_omega):
     """dL/dtheta from dL/domega for one level."""
     p = unpack(theta)
@@ -203,10 +241,10 @@ def backward(theta, X, lv, rs, cache, g_omega):
     grad = {k: np.zeros_like(v) for k, v in p.items()}
     hs, ms, sig = cache["hs"], cache["ms"], cache["sig"]
     dz = g_omega * OMEGA_MAX * sig * (1.0 - sig)
-    grad["a"] = hs[2].T @ dz
+    grad["a"] = hs[LAYERS].T @ dz
     grad["a0"] = np.array([dz.sum()])
     dh = np.outer(dz, p["a"])
-    for t in (2, 1):
+    for t in range(LAYERS, 0, -1):
         dpre = dh * (hs[t] > 0)
         grad[f"U{t}"] = dpre.T @ hs[t - 1]
         grad[f"V{t}"] = dpre.T @ ms[t - 1]

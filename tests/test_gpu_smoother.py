@@ -67,6 +67,36 @@ def test_omega_matches_oracle(oracle, spec):
 
 
 @pytest.mark.parametrize("spec", [ONE_LEVEL, DEEP], ids=["one-level", "deep"])
+def test_bf16_inference_matches_oracle(oracle, spec):
+    """The solve path's BF16 inference (matrix-core layer products, FP32
+    accumulation) against the oracle's BF16 emulation of the same network:
+    activations round to BF16 at the same places, so the omegas agree up to
+    the accumulation order's rare one-ulp BF16 flips; against the FP64
+    network they agree to the BF16 model error."""
+    n, levels, feats = oracle_side(spec)
+    theta = _theta(3)
+    emu = np.concatenate([forward_bf16(l, theta, levels, feats) for l in sorted(feats)])
+    full = np.concatenate([om for _, (om, _) in sorted(S.omegas(theta, levels, feats).items())])
+    e = F.Engine(*columns(spec))
+    try:
+        got = e.smoother_omega(theta, bf16=True)
+    finally:
+        e.release()
+    assert got.shape == emu.shape
+    d = np.abs(got - emu)
+    print(f"{spec.name}: BF16 omega vs emulation max {d.max():.2e} mean {d.mean():.2e}; "
+          f"vs FP64 max {np.abs(got - full).max():.2e}")
+    assert d.max() <= 5e-3 and d.mean() <= 2e-4
+    assert np.max(np.abs(got - full)) <= 3e-2
+    assert np.all((got > 0) & (got < S.OMEGA_MAX))
+
+
+def forward_bf16(l, theta, levels, feats):
+    X, rs = feats[l]
+    return S.forward_bf16(theta, X, levels[l], rs)
+
+
+@pytest.mark.parametrize("spec", [ONE_LEVEL, DEEP], ids=["one-level", "deep"])
 def test_gradient_matches_oracle(oracle, spec):
     n, levels, feats = oracle_side(spec)
     theta = _theta(5)
