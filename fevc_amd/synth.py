@@ -120,6 +120,26 @@ def generate(spec: PanelSpec):
         target=np.ones(W * T, np.float64), alpha=alpha, psi=psi)
 
 
+def year_controls(spec: PanelSpec, data, years: int = 11, effect: float = 0.1):
+    """Year-dummy controls for a panel of `generate` (rows are worker-major,
+    row = worker * T + period): worker w enters in calendar year s_w, uniform
+    in 0 .. years - T, and is observed in years s_w .. s_w + T - 1.  Returns
+    the dummies of years 1 .. years - 1 (year 0 the baseline; 10 controls with
+    the default 11 years), rows x (years - 1), and the outcome plus a year
+    effect of standard deviation `effect`."""
+    W, T = spec.workers, spec.periods
+    if years < T + 1:
+        raise ValueError("need at least T + 1 calendar years")
+    rng = np.random.default_rng(spec.seed + 0x5EED)
+    start = rng.integers(0, years - T + 1, W)
+    year = (start[:, None] + np.arange(T)[None, :]).reshape(-1)
+    z = np.zeros((W * T, years - 1))
+    has = year > 0
+    z[np.flatnonzero(has), year[has] - 1] = 1.0
+    gamma = np.concatenate([[0.0], rng.normal(0.0, effect, years - 1)])
+    return z, np.asarray(data["outcome"], np.float64) + gamma[year]
+
+
 def eligible_stayers(worker, firm, outcome, frequency, target, mover_mask):
     """Eligible stayer rows of a panel for the stayer hybrid, in the form the
     augmentation entry point takes (stayer_hybrid.rs:23-34): workers outside
