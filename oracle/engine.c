@@ -552,6 +552,10 @@ struct orc_engine_result {
     uint64_t cells, units, probes;
     double *fitted_cell, *projection_share, *residual_share, *finite_bias, *finite_variance,
         *residual_mass, *deleted_mass, *cell_weight, *draws, *lev_iters;
+    /* optional: every target probe's two FE solutions (worker side, firm
+     * side), probe-major: target_worker[(2 t + side) W + w] */
+    double *target_worker, *target_firm;
+    uint64_t workers, firms;
 };
 
 typedef struct {
@@ -710,16 +714,20 @@ static int copy_sign(uint64_t seed, uint64_t probe, uint64_t entity, uint64_t co
 static uint64_t observation_row(const orc_problem *p, uint64_t g);
 
 static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
-                      orc_numerical_result *num);
+                      orc_numerical_result *num, int keep_targets);
 
 int orc_engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out) {
-    return engine_run(p, opt, out, NULL);
+    return engine_run(p, opt, out, NULL, 0);
+}
+
+int orc_engine_run_targets(const orc_problem *p, orc_engine_options opt, orc_engine_result **out) {
+    return engine_run(p, opt, out, NULL, 1);
 }
 
 int orc_engine_run_numerical(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
                              orc_numerical_result *num) {
     memset(num, 0, sizeof *num);
-    return engine_run(p, opt, out, num);
+    return engine_run(p, opt, out, num, 0);
 }
 
 /* engine/numerical.rs:160-330 (replay): the leverage probes are solved again
@@ -1024,7 +1032,7 @@ static int observation_adjustment(const orc_problem *p, const obs_state *o, cons
 }
 
 static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_result **out,
-                      orc_numerical_result *num) {
+                      orc_numerical_result *num, int keep_targets) {
     if (opt.probes < 2)
         return orc_fail(ORC_INVALID_INPUT, "jla_validate", "JLA probe count must be at least two");
     num_state ns = {NULL, NULL, NULL, 0};
@@ -1038,6 +1046,11 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
     uint64_t C = p->cells, G = p->units, W = p->workers, F = p->firms, S = p->strata;
     uint64_t R = opt.probes;
     res->cells = C; res->units = G; res->probes = R;
+    res->workers = W; res->firms = F;
+    if (keep_targets) {
+        res->target_worker = orc_alloc(2 * R * W, 8);
+        res->target_firm = orc_alloc(2 * R * F, 8);
+    }
     double t0 = now_seconds();
     /* full fit: engine.rs:1108-1134 */
     uint64_t maxw = opt.leverage_width > opt.target_width ? opt.leverage_width : opt.target_width;
@@ -1284,6 +1297,13 @@ static int engine_run(const orc_problem *p, orc_engine_options opt, orc_engine_r
         for (uint64_t c = 0; c < k; ++c) {
             const column_t *a = &cols[2 * c], *b = &cols[2 * c + 1];
             res->sum.target_iterations += a->iterations + b->iterations;
+            if (keep_targets) {
+                uint64_t t = first + c;
+                memcpy(res->target_worker + 2 * t * W, a->worker, W * 8);
+                memcpy(res->target_worker + (2 * t + 1) * W, b->worker, W * 8);
+                memcpy(res->target_firm + 2 * t * F, a->firm, F * 8);
+                memcpy(res->target_firm + (2 * t + 1) * F, b->firm, F * 8);
+            }
             max_complete = fmax(max_complete, fmax(a->complete, b->complete));
             /* engine.rs:3003-3046 (contract_target_solutions) */
             ssum ws = {0, 0}, fs = {0, 0}, cv = {0, 0};
@@ -1387,6 +1407,7 @@ void orc_engine_result_free(orc_engine_result *r) {
     free(r->fitted_cell); free(r->projection_share); free(r->residual_share);
     free(r->finite_bias); free(r->finite_variance); free(r->residual_mass);
     free(r->deleted_mass); free(r->cell_weight); free(r->draws); free(r->lev_iters);
+    free(r->target_worker); free(r->target_firm);
     free(r);
 }
 
@@ -1406,8 +1427,11 @@ int orc_engine_result_array(const orc_engine_result *r, int which, double *dst, 
     case 7: src = r->cell_weight; n = r->cells; break;
     case 8: src = r->draws; n = 4 * r->probes; break;
     case 9: src = r->lev_iters; n = r->probes; break;
+    case 10: src = r->target_worker; n = 2 * r->probes * r->workers; break;
+    case 11: src = r->target_firm; n = 2 * r->probes * r->firms; break;
     default: return orc_fail(ORC_INVALID_INPUT, "engine", "unknown array");
     }
+    if (!src) return orc_fail(ORC_INVALID_INPUT, "engine", "array not recorded by this run");
     if (cap < n) return orc_fail(ORC_INVALID_INPUT, "engine", "capacity too small");
     memcpy(dst, src, n * 8);
     return ORC_OK;

@@ -182,6 +182,7 @@ def lib():
         L.orc_problem_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_hybrid_graph.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
         L.orc_engine_run.argtypes = [C.c_void_p, EngineOptions, C.POINTER(C.c_void_p)]
+        L.orc_engine_run_targets.argtypes = [C.c_void_p, EngineOptions, C.POINTER(C.c_void_p)]
         L.orc_engine_result_free.argtypes = [C.c_void_p]
         L.orc_engine_result_summary.argtypes = [C.c_void_p, C.POINTER(EngineSummary)]
         L.orc_engine_result_array.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]
@@ -383,7 +384,9 @@ class Problem:
               "stratum_cell": (6, np.uint32, 8), "stratum_count": (7, np.uint64, 8),
               "stratum_rank": (8, np.uint64, 8), "stratum_per_copy": (9, np.float64, 8),
               "cell_outcome_sum": (10, np.float64, 5), "cell_target_sum": (11, np.float64, 5),
-              "unit_outcome_sum": (12, np.float64, 6), "unit_offset": (13, np.uint64, 6)}
+              "unit_outcome_sum": (12, np.float64, 6), "unit_offset": (13, np.uint64, 6),
+              "retained": (14, np.uint64, 1), "row_cell": (15, np.uint32, 1),
+              "unit_ptr": (16, np.uint64, 9), "unit_rows": (17, np.uint64, 1)}
 
     def __init__(self, worker, firm, deletion, outcome, frequency, target, mask=None, observation=False):
         cols = [np.ascontiguousarray(worker, np.uint64), np.ascontiguousarray(firm, np.uint64),
@@ -430,7 +433,7 @@ class Problem:
     def array(self, name):
         which, dtype, dim = self.ARRAYS[name]
         n = int([self.input_rows, self.rows, self.physical, self.workers, self.firms, self.cells,
-                 self.units, self.exact_strata, self.strata][dim])
+                 self.units, self.exact_strata, self.strata, self.units + 1][dim])
         out = np.zeros(n, dtype)
         _check(lib().orc_problem_array(self._h, which, out.ctypes.data))
         return out
@@ -477,7 +480,11 @@ class Problem:
 
     def run(self, seed=8675309, probes=200, leverage_width=8, target_width=8, threads=1,
             fit_tolerance=1e-10, probe_tolerance=1e-6, max_iterations=10000,
-            recompute_interval=100, rank_tolerance=1e-10, block_tolerance=1e-10, numerical=False):
+            recompute_interval=100, rank_tolerance=1e-10, block_tolerance=1e-10, numerical=False,
+            targets=False):
+        """The compressed JLA engine.  With `targets`, the result also holds
+        every target probe's two FE solutions: target_worker (2R x workers) and
+        target_firm (2R x firms), row 2t the worker-side solve of probe t."""
         opt = EngineOptions(seed, probes, leverage_width, target_width, threads, fit_tolerance,
                             probe_tolerance, max_iterations, recompute_interval, rank_tolerance,
                             block_tolerance)
@@ -485,6 +492,8 @@ class Problem:
         num = NumericalResult() if numerical else None
         if numerical:
             _check(lib().orc_engine_run_numerical(self._h, opt, C.byref(h), C.byref(num)))
+        elif targets:
+            _check(lib().orc_engine_run_targets(self._h, opt, C.byref(h)))
         else:
             _check(lib().orc_engine_run(self._h, opt, C.byref(h)))
         try:
@@ -506,6 +515,11 @@ class Problem:
                 res[name] = a.reshape(probes, 4) if name == "target_draws" else a
             if numerical:
                 res["numerical"] = num.as_dict()
+            if targets:
+                for i, (name, n) in enumerate((("target_worker", self.workers), ("target_firm", self.firms))):
+                    a = np.zeros(2 * probes * n)
+                    _check(lib().orc_engine_result_array(h.value, 10 + i, a.ctypes.data, a.size))
+                    res[name] = a.reshape(2 * probes, n)
             return res
         finally:
             lib().orc_engine_result_free(h.value)

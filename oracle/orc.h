@@ -221,6 +221,11 @@ void orc_engine_result_summary(const orc_engine_result *r,
  *        9 leverage_iterations(per probe, as double) */
 int orc_engine_result_array(const orc_engine_result *r, int which, double *dst,
                             uint64_t capacity);
+/* As orc_engine_run, and it keeps every target probe's two FE solutions:
+ * arrays 10 (2 R x workers) and 11 (2 R x firms), probe-major, worker-side
+ * solve first. */
+int orc_engine_run_targets(const orc_problem *p, orc_engine_options opt,
+                           orc_engine_result **out);
 
 /* ---- numerical MCSE attachment (engine/numerical.rs, numerical_mc.rs) --- */
 typedef struct {

@@ -1023,6 +1023,10 @@ int orc_problem_array(const orc_problem *p, int which, void *dst) {
         if (p->unit_offset) memcpy(dst, p->unit_offset, p->units * 8);
         else memset(dst, 0, p->units * 8);
         break;
+    case 14: memcpy(dst, p->retained, p->rows * 8); break;
+    case 15: memcpy(dst, p->row_cell, p->rows * 4); break;
+    case 16: memcpy(dst, p->unit_ptr, (p->units + 1) * 8); break;
+    case 17: memcpy(dst, p->unit_rows, p->rows * 8); break;
     default: return orc_fail(ORC_INVALID_INPUT, "problem", "unknown array %d", which);
     }
     return ORC_OK;
