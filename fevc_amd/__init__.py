@@ -301,6 +301,7 @@ SIGNATURES = [
     ("vckss_mi355x_smoother_status_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_smoother_force_v1", C.c_int32, [C.c_uint64, C.c_double]),
     ("vckss_mi355x_exact_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_controls_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_graphs_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -404,20 +405,29 @@ class Engine:
     """
 
     def __init__(self, worker, firm, deletion, outcome, frequency, target, *, upload=True,
-                 poll=None, deletion_mode="match", memory_limit=None):
+                 poll=None, deletion_mode="match", memory_limit=None, controls=None):
         """deletion_mode "match" (the default) or "observation" (the dense
         exact route, and the JLA route without numerical MCSE).
         memory_limit is the whole-command host byte limit of the admission
-        (default: the host's physical memory)."""
+        (default: the host's physical memory).  controls: rows x Q (Q <= 32)
+        control columns; solves then take nuisance "joint" (the default) or
+        "fixed_offset" on the JLA route under match deletion."""
         lib = library()
         self.deletion_mode = {"match": 1, "observation": 2}[deletion_mode]
         self.stayers_mode = 1  # movers; 2 (all) after augment_stayers
         cols = [_f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency), _f64(target)]
         self._cols = cols
         rows = len(cols[0])
+        self.controls = 0
+        self._control_cols = []
+        if controls is not None:
+            z = np.asarray(controls, np.float64).reshape(rows, -1)
+            self._control_cols = [np.ascontiguousarray(z[:, j]) for j in range(z.shape[1])]
+            self.controls = len(self._control_cols)
         req = default_prepare_request(rows)
         req.options.cleanup_abandoned = 1
-        req.options.caller_copy_bytes = rows * 6 * 8
+        req.options.caller_copy_bytes = rows * (6 + self.controls) * 8
+        req.options.controls_count = self.controls
         req.options.memory_limit_bytes = memory_limit or host_memory_bytes()
         req.options.deletion_mode = self.deletion_mode
         self._poll = Poll(poll) if poll else None
@@ -428,6 +438,10 @@ class Engine:
         c.struct_size = C.sizeof(c)
         c.rows = rows
         c.worker, c.firm, c.deletion, c.outcome, c.frequency, c.target_weight = [a.ctypes.data for a in cols]
+        if self.controls:
+            self._control_ptrs = (C.c_void_p * self.controls)(*[a.ctypes.data for a in self._control_cols])
+            c.controls = C.cast(self._control_ptrs, C.c_void_p)
+            c.controls_count = self.controls
         h = C.c_uint64()
         if upload:
             _check(lib.vckss_rust_engine_prepare_interrupt_v4(C.byref(req), C.byref(c), C.byref(h), 8))
@@ -489,12 +503,14 @@ class Engine:
 
     def solve(self, seed=8675309, probes=200, tolerance=None, fit_tolerance=1e-10,
               maximum_iterations=10000, residual_replacement_interval=100, poll=None, checkpoint_interval=1,
-              **overrides):
+              nuisance="joint", **overrides):
         """poll: optional callable returning 0 (continue) or 1 (user break),
-        called at every `checkpoint_interval`-th checkpoint of the solve."""
+        called at every `checkpoint_interval`-th checkpoint of the solve.
+        nuisance: "joint" or "fixed_offset" (how prepared controls enter)."""
         req = default_solve_request()
         o = req.options
         o.algorithm = 2  # JLA
+        o.nuisance_mode = {"joint": 1, "fixed_offset": 2}[nuisance]  # VCKSS_NUISANCE_*
         o.full_cmg_v2 = 1
         o.threads = 1
         o.deletion_mode = self.deletion_mode
@@ -588,6 +604,14 @@ class Engine:
         _check(library().vckss_rust_engine_result_v1(self.generation, C.byref(r), C.sizeof(r)))
         return dict(plugin=r.plugin.as_list(), correction=r.correction.as_list(),
                     corrected=r.corrected.as_list(), numerical_mcse=r.numerical_mcse.as_list())
+
+    def control_fit(self):
+        """The last solve's control fit: q, fixed offset, the generalised
+        reciprocal condition of the control information, the coefficients."""
+        out = np.zeros(3 + max(self.controls, 1))
+        _check(library().vckss_mi355x_controls_v1(self.generation, out.ctypes.data, out.size))
+        q = int(out[0])
+        return dict(controls=q, fixed_offset=bool(out[1]), rcond=float(out[2]), beta=out[3:3 + q].copy())
 
     def array(self, which: int, size: int):
         out = np.zeros(size, np.float64)

@@ -14,6 +14,7 @@
 #include <numeric>
 
 #include "device_impl.hpp"
+#include "kernels_controls.hpp"
 #include "numerical.hpp"
 
 namespace vk {
@@ -566,6 +567,314 @@ double component_identity(const double c[4]) {
   return std::fabs(c[3] - implied) / scale;
 }
 
+// ------------------------------------------------------------------ controls
+namespace {
+
+// Symmetric eigendecomposition of an n x n matrix (row-major) by cyclic
+// Jacobi: eigenvalues ascending in `lam`, eigenvectors as columns of `vec`.
+void sym_eigen(std::vector<double> a, uint32_t n, std::vector<double> &lam, std::vector<double> &vec) {
+  vec.assign((size_t)n * n, 0.0);
+  for (uint32_t i = 0; i < n; ++i) vec[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      diag += a[(size_t)i * n + i] * a[(size_t)i * n + i];
+      for (uint32_t j = i + 1; j < n; ++j) off += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+    }
+    if (!(off > 1e-32 * diag)) break;
+    for (uint32_t p = 0; p < n; ++p)
+      for (uint32_t r = p + 1; r < n; ++r) {
+        const double apr = a[(size_t)p * n + r];
+        if (apr == 0.0) continue;
+        const double theta = (a[(size_t)r * n + r] - a[(size_t)p * n + p]) / (2.0 * apr);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (uint32_t k = 0; k < n; ++k) {
+          const double akp = a[(size_t)k * n + p], akr = a[(size_t)k * n + r];
+          a[(size_t)k * n + p] = c * akp - sn * akr;
+          a[(size_t)k * n + r] = sn * akp + c * akr;
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+          const double apk = a[(size_t)p * n + k], ark = a[(size_t)r * n + k];
+          a[(size_t)p * n + k] = c * apk - sn * ark;
+          a[(size_t)r * n + k] = sn * apk + c * ark;
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+          const double vkp = vec[(size_t)k * n + p], vkr = vec[(size_t)k * n + r];
+          vec[(size_t)k * n + p] = c * vkp - sn * vkr;
+          vec[(size_t)k * n + r] = sn * vkp + c * vkr;
+        }
+      }
+  }
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  std::vector<double> v2((size_t)n * n);
+  lam.resize(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    lam[k] = a[(size_t)order[k] * n + order[k]];
+    for (uint32_t i = 0; i < n; ++i) v2[(size_t)i * n + k] = vec[(size_t)i * n + order[k]];
+  }
+  vec.swap(v2);
+}
+
+// Eigenvalues of D^-1/2 m D^-1/2 (diagonal scaling); false when a diagonal
+// entry is not positive and finite.
+bool scaled_spectrum(const std::vector<double> &m, uint32_t n, std::vector<double> &lam) {
+  std::vector<double> d(n), a(m);
+  for (uint32_t i = 0; i < n; ++i) {
+    d[i] = m[(size_t)i * n + i];
+    if (!(std::isfinite(d[i]) && d[i] > 0.0)) return false;
+    d[i] = 1.0 / std::sqrt(d[i]);
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = 0; j < n; ++j) a[(size_t)i * n + j] *= d[i] * d[j];
+  std::vector<double> vec;
+  sym_eigen(a, n, lam, vec);
+  return true;
+}
+
+// Inverse of a symmetric positive definite matrix from its eigendecomposition
+// after diagonal scaling.
+std::vector<double> spd_inverse(const std::vector<double> &m, uint32_t n) {
+  std::vector<double> d(n), a(m), lam, vec, out((size_t)n * n, 0.0);
+  for (uint32_t i = 0; i < n; ++i) d[i] = 1.0 / std::sqrt(m[(size_t)i * n + i]);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = 0; j < n; ++j) a[(size_t)i * n + j] *= d[i] * d[j];
+  sym_eigen(a, n, lam, vec);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (uint32_t k = 0; k < n; ++k) s += vec[(size_t)i * n + k] * vec[(size_t)j * n + k] / lam[k];
+      out[(size_t)i * n + j] = d[i] * s * d[j];
+    }
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j) {
+      const double v = 0.5 * (out[(size_t)i * n + j] + out[(size_t)j * n + i]);
+      out[(size_t)i * n + j] = out[(size_t)j * n + i] = v;
+    }
+  return out;
+}
+
+// Lower Cholesky factor of an SPD matrix (row-major); false if not positive.
+bool cholesky(const std::vector<double> &m, uint32_t n, std::vector<double> &l) {
+  l.assign((size_t)n * n, 0.0);
+  for (uint32_t j = 0; j < n; ++j) {
+    double s = m[(size_t)j * n + j];
+    for (uint32_t k = 0; k < j; ++k) s -= l[(size_t)j * n + k] * l[(size_t)j * n + k];
+    if (!(s > 0.0) || !std::isfinite(s)) return false;
+    l[(size_t)j * n + j] = std::sqrt(s);
+    for (uint32_t i = j + 1; i < n; ++i) {
+      double t = m[(size_t)i * n + j];
+      for (uint32_t k = 0; k < j; ++k) t -= l[(size_t)i * n + k] * l[(size_t)j * n + k];
+      l[(size_t)i * n + j] = t / l[(size_t)j * n + j];
+    }
+  }
+  return true;
+}
+
+// Weighted Gram reduction of rows x m (row-major) on the device; returns the
+// upper triangle (mode 0) or the cross block (mode 1) summed over blocks.
+std::vector<double> device_gram(Impl &E, uint64_t rows, uint32_t m, int mode, const double *x, const double *w) {
+  const uint32_t outs = mode == 1 ? (m / 2) * (m / 2) : m * (m + 1) / 2;
+  const uint64_t tiles = (rows + dev::kGramTile - 1) / dev::kGramTile;
+  const unsigned blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(tiles, 1024));
+  E.ctl_part.alloc((size_t)blocks * outs);
+  E.ctl_out.alloc(std::max<uint32_t>(outs, 1));
+  hipLaunchKernelGGL(dev::k_ctl_gram, blocks, kBlock, 0, E.stream, rows, m, mode, x, w, E.ctl_part.p);
+  hipLaunchKernelGGL(dev::k_ctl_finish, grid_for(outs), kBlock, 0, E.stream, blocks, outs, E.ctl_part.p, E.ctl_out.p);
+  std::vector<double> out;
+  E.ctl_out.download(out, outs, E.stream);
+  return out;
+}
+
+// Expand an upper triangle (row by row, i <= j) of an m x m matrix.
+std::vector<double> from_upper(const std::vector<double> &u, uint32_t m) {
+  std::vector<double> a((size_t)m * m);
+  size_t o = 0;
+  for (uint32_t i = 0; i < m; ++i)
+    for (uint32_t j = i; j < m; ++j, ++o) a[(size_t)i * m + j] = a[(size_t)j * m + i] = u[o];
+  return a;
+}
+
+void ensure_controls(Impl &E) {
+  if (E.ctl_ready) return;
+  const Prepared &P = *E.P;
+  const uint32_t q = P.controls;
+  if (q == 0 || q > (uint32_t)dev::kMaxControls) fail(kInvariant, "controls", "no controls to upload");
+  E.ctl_z.upload(P.control);
+  std::vector<double> f(P.frequency.begin(), P.frequency.end());
+  E.ctl_f.upload(f);
+  E.ctl_y.upload(P.outcome);
+  E.ctl_row_cell.upload(P.row_cell);
+  E.ctl_unit_ptr.upload(P.unit_ptr);
+  E.ctl_unit_rows.upload(P.unit_rows);
+  E.ctl_cellz.upload(P.cell_control);
+  uint64_t widest = 0;
+  for (uint64_t g = 0; g < P.units; ++g) widest = std::max<uint64_t>(widest, P.unit_ptr[g + 1] - P.unit_ptr[g]);
+  E.ctl_block = (uint32_t)std::min<uint64_t>(widest, q + 1);
+  E.ctl_gram0 = from_upper(device_gram(E, P.rows, q, 0, E.ctl_z.p, E.ctl_f.p), q);
+  E.ctl_ready = true;
+}
+
+// The control fit; leaves the FE residuals (zry) and the joint residual (e)
+// on the device and returns the joint two-way coefficients.
+ControlFit fit_controls(Impl &E, const SolveOptions &opt, std::vector<double> &alpha, std::vector<double> &psi) {
+  const Prepared &P = *E.P;
+  const uint32_t q = P.controls, cols = q + 1, K = lanes_for(cols);
+  ensure_controls(E);
+  E.ensure(K);
+  ControlFit cf;
+  cf.q = q;
+  // model_solver.rs:1799-1870: the raw control Gram must be nonsingular
+  std::vector<double> lam;
+  if (!scaled_spectrum(E.ctl_gram0, q, lam) || !(lam[q - 1] > 0.0) || !(lam[0] > 1e-12 * lam[q - 1]))
+    fail(kSingularInformation, "model_rank", "original control Gram is singular or numerically unresolved");
+  // the Q + 1 right-hand sides: frequency-weighted control and outcome sums
+  std::vector<double> wi(E.W * K, 0.0), fi(E.F * K, 0.0);
+  for (uint64_t c = 0; c < E.C; ++c) {
+    double *a = &wi[(size_t)P.cell_worker[c] * K], *b = &fi[(size_t)P.cell_firm[c] * K];
+    for (uint32_t j = 0; j < q; ++j) {
+      a[j] += P.cell_control[c * q + j];
+      b[j] += P.cell_control[c * q + j];
+    }
+    a[q] += P.cell_outcome[c];
+    b[q] += P.cell_outcome[c];
+  }
+  VK_HIP(hipMemcpy(E.wrhs.p, wi.data(), wi.size() * sizeof(double), hipMemcpyHostToDevice));
+  VK_HIP(hipMemcpy(E.frhs.p, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice));
+  E.gate = E.always();
+  E.ctl_zry.alloc(P.rows * cols);
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    solve_columns<KK>(E, cols, opt.fit_tolerance, 0.01, opt, cf.receipts);
+    hipLaunchKernelGGL((dev::k_ctl_residualize<KK>), grid_for(P.rows), kBlock, 0, E.stream, P.rows, q,
+                       E.ctl_row_cell.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.ctl_z.p, E.ctl_y.p,
+                       E.ctl_zry.p);
+  });
+  const std::vector<double> full = from_upper(device_gram(E, P.rows, cols, 0, E.ctl_zry.p, E.ctl_f.p), cols);
+  cf.S.assign((size_t)q * q, 0.0);
+  cf.b.assign(q, 0.0);
+  for (uint32_t i = 0; i < q; ++i) {
+    cf.b[i] = full[(size_t)i * cols + q];
+    for (uint32_t j = 0; j < q; ++j) cf.S[(size_t)i * q + j] = full[(size_t)i * cols + j];
+  }
+  // generalised eigenvalues of S against the raw Gram (model_solver.rs:1818-1870)
+  std::vector<double> l0, g0inv = spd_inverse(E.ctl_gram0, q);
+  if (!cholesky(g0inv, q, l0))
+    fail(kSingularInformation, "model_rank", "original-control whitening failed");
+  std::vector<double> gen((size_t)q * q, 0.0), tmp((size_t)q * q, 0.0), vec;
+  for (uint32_t i = 0; i < q; ++i)
+    for (uint32_t j = 0; j < q; ++j)
+      for (uint32_t k = 0; k < q; ++k) tmp[(size_t)i * q + j] += cf.S[(size_t)i * q + k] * l0[(size_t)k * q + j];
+  for (uint32_t i = 0; i < q; ++i)
+    for (uint32_t j = 0; j < q; ++j)
+      for (uint32_t k = 0; k < q; ++k) gen[(size_t)i * q + j] += l0[(size_t)k * q + i] * tmp[(size_t)k * q + j];
+  for (uint32_t i = 0; i < q; ++i)
+    for (uint32_t j = i + 1; j < q; ++j)
+      gen[(size_t)i * q + j] = gen[(size_t)j * q + i] = 0.5 * (gen[(size_t)i * q + j] + gen[(size_t)j * q + i]);
+  sym_eigen(gen, q, lam, vec);
+  cf.generalized_rcond = lam[q - 1] > 0.0 ? lam[0] / lam[q - 1] : 0.0;
+  if (!(lam[q - 1] > 0.0) || !std::isfinite(cf.generalized_rcond) ||
+      cf.generalized_rcond <= std::max(opt.rank_tolerance, 1e-12))
+    fail(kSingularInformation, "model_rank",
+         "FE-residualized generalized control information is singular or numerically unresolved");
+  // control_geometry (generic_jla.rs:4407-4536): the scaled Schur complement
+  if (!scaled_spectrum(cf.S, q, lam) || !(lam[q - 1] > 0.0) || lam[0] <= opt.rank_tolerance * lam[q - 1])
+    fail(kSingularInformation, "generic_jla_control_schur", "the control Schur complement is singular");
+  cf.s_inv = spd_inverse(cf.S, q);
+  cf.beta.assign(q, 0.0);
+  for (uint32_t i = 0; i < q; ++i)
+    for (uint32_t j = 0; j < q; ++j) cf.beta[i] += cf.s_inv[(size_t)i * q + j] * cf.b[j];
+  for (double v : cf.beta)
+    if (!std::isfinite(v)) fail(kNonfiniteCorrected, "generic_jla_control_schur", "control coefficients are nonfinite");
+  // joint residual, weighted RSS and the joint two-way coefficients
+  E.ctl_small.alloc(2 * (size_t)q * q + 64);
+  VK_HIP(hipMemcpy(E.ctl_small.p, cf.beta.data(), q * sizeof(double), hipMemcpyHostToDevice));
+  E.ctl_e.alloc(P.rows);
+  const unsigned rb = reduce_grid(P.rows);
+  E.ctl_part.alloc(std::max<size_t>(rb, E.W + E.F));
+  hipLaunchKernelGGL(dev::k_ctl_resid, rb, kBlock, 0, E.stream, P.rows, q, E.ctl_zry.p, E.ctl_small.p, E.ctl_f.p,
+                     E.ctl_e.p, E.ctl_part.p);
+  std::vector<double> part;
+  E.ctl_part.download(part, rb, E.stream);
+  Neumaier rss;
+  for (double v : part) rss.add(v);
+  cf.rss = rss.value();
+  E.ctl_out.alloc(E.W + E.F);
+  with_k(K, [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    hipLaunchKernelGGL((dev::k_ctl_combine<KK>), grid_for(E.W), kBlock, 0, E.stream, E.W, q, E.worker.p,
+                       E.ctl_small.p, E.ctl_out.p);
+    hipLaunchKernelGGL((dev::k_ctl_combine<KK>), grid_for(E.F), kBlock, 0, E.stream, E.F, q, E.fsol.p,
+                       E.ctl_small.p, E.ctl_out.p + E.W);
+  });
+  std::vector<double> both;
+  E.ctl_out.download(both, E.W + E.F, E.stream);
+  alpha.assign(both.begin(), both.begin() + E.W);
+  psi.assign(both.begin() + E.W, both.end());
+  E.ctl_s_inv = cf.s_inv;
+  return cf;
+}
+
+// The control terms of one target batch's draws (k probes, columns 2l and
+// 2l + 1): worker += b_w' H_w + b_w' M2 b_w, firm likewise, total with
+// b_w + b_f and H_w + H_f, covariance = (total - worker - firm) / 2.
+void control_draws(Impl &E, uint32_t q, const std::vector<double> &m2, uint32_t k2, uint32_t k,
+                   std::vector<double> &d) {
+  const uint32_t m = 2 * q;
+  std::vector<double> out;
+  E.ctl_out.download(out, (size_t)m * k2, E.stream);
+  const std::vector<double> &si = E.ctl_s_inv;
+  auto term = [&](const std::vector<double> &beta, const std::vector<double> &h) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < q; ++i) {
+      s += beta[i] * h[i];
+      for (uint32_t j = 0; j < q; ++j) s += beta[i] * m2[(size_t)i * q + j] * beta[j];
+    }
+    return s;
+  };
+  std::vector<double> bw(q), bf(q), hw(q), hf(q), bt(q), ht(q);
+  for (uint32_t l = 0; l < k; ++l) {
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t col = 2 * l + side;
+      std::vector<double> &beta = side ? bf : bw, &h = side ? hf : hw;
+      for (uint32_t i = 0; i < q; ++i) {
+        double v = 0.0;
+        for (uint32_t j = 0; j < q; ++j) v -= si[(size_t)i * q + j] * out[(size_t)j * k2 + col];
+        beta[i] = v;
+        h[i] = out[(size_t)(q + i) * k2 + col];
+      }
+    }
+    for (uint32_t i = 0; i < q; ++i) {
+      bt[i] = bw[i] + bf[i];
+      ht[i] = hw[i] + hf[i];
+    }
+    double *dd = &d[4 * l];
+    const double worker = dd[0] + term(bw, hw), firm = dd[1] + term(bf, hf), total = dd[3] + term(bt, ht);
+    dd[0] = worker;
+    dd[1] = firm;
+    dd[2] = 0.5 * (total - worker - firm);
+    dd[3] = total;
+  }
+}
+
+// Launch the match-block kernel with the smallest per-thread block bound.
+void launch_blocks(Impl &E, uint32_t q, const SolveOptions &opt) {
+  const dev::BlockParams bp{(double)opt.probes, opt.block_tolerance, opt.rank_tolerance};
+  auto go = [&](auto nn) {
+    constexpr int N = decltype(nn)::value;
+    hipLaunchKernelGGL((dev::k_ctl_blocks<N>), grid_for(E.Gm), kBlock, 0, E.stream, E.Gm, q, E.ctl_unit_ptr.p,
+                       E.ctl_unit_rows.p, E.adjust.p, E.ctl_f.p, E.ctl_y.p, E.ctl_e.p, E.ctl_zry.p, E.ctl_u.p, bp,
+                       E.contrib.p, E.ctl_uab.p, E.ctl_umass.p, E.sink());
+  };
+  if (E.ctl_block <= 8) go(std::integral_constant<int, 8>{});
+  else if (E.ctl_block <= 16) go(std::integral_constant<int, 16>{});
+  else go(std::integral_constant<int, dev::kMaxControls + 1>{});
+}
+
+}  // namespace
+
 namespace {
 
 // Host permutation of K-lane vectors between original and renumbered order.
@@ -838,8 +1147,42 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   auto t0 = std::chrono::steady_clock::now();
   const uint64_t R = opt.probes;
 
+  const bool joint = opt.joint_controls;
+  if (joint && (P.controls == 0 || E.Go || P.stayers))
+    fail(kInvariant, "generic_jla", "joint controls need a match-deletion problem with controls");
+  if (joint && opt.numerical)
+    fail(kUnsupported, "numerical_mc", "the numerical attachment with controls is not supported by the MI355X engine");
+  const uint32_t q = joint ? P.controls : 0;
+  // ---- joint fit with the controls (generic_jla.rs:1930-2000): Q + 1
+  // two-way solves, the control Schur complement and beta; the plug-in uses
+  // the joint two-way coefficients
+  if (joint) {
+    opt.checkpoint("generic_jla_full_fit");
+    std::vector<double> alpha, psi;
+    ControlFit cf = fit_controls(E, opt, alpha, psi);
+    res.controls = q;
+    res.control_beta = cf.beta;
+    res.control_rcond = cf.generalized_rcond;
+    res.control_fit.assign(cf.receipts.begin(), cf.receipts.begin() + q);
+    res.fit = cf.receipts[q];
+    res.max_complete_residual = 0.0;
+    for (const auto &c : cf.receipts) res.max_complete_residual = std::max(res.max_complete_residual, c.complete);
+    res.fitted_cell.resize(E.C);
+    for (uint64_t c = 0; c < E.C; ++c) res.fitted_cell[c] = alpha[P.cell_worker[c]] + psi[P.cell_firm[c]];
+    res.weighted_rss = cf.rss;
+    plugin_components(P, alpha, psi, res.plugin);
+    E.fitted.upload(res.fitted_cell);
+    // the control rows of the block factors: sqrt(f) Phi' zr, Phi Phi' = S^-1
+    std::vector<double> phi;
+    if (!cholesky(cf.s_inv, q, phi))
+      fail(kInverseResidual, "generic_jla_control_factor", "control inverse square root failed");
+    VK_HIP(hipMemcpy(E.ctl_small.p + q, phi.data(), (size_t)q * q * sizeof(double), hipMemcpyHostToDevice));
+    E.ctl_u.alloc(P.rows * q);
+    hipLaunchKernelGGL(dev::k_ctl_whiten, grid_for(P.rows), kBlock, 0, E.stream, P.rows, q, E.ctl_zry.p, E.ctl_f.p,
+                       E.ctl_small.p + q, E.ctl_u.p);
+  }
   // ---- full fit (engine.rs:1108-1134)
-  {
+  if (!joint) {
     opt.checkpoint("jla_full_fit");
     std::vector<double> wr(E.W, 0.0), fr(E.F, 0.0);
     for (uint64_t c = 0; c < E.C; ++c) {
@@ -934,6 +1277,22 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   if (E.Go)
     VK_HIP(hipMemcpyAsync(E.contrib.p + E.Gm, E.obs_adjust.p + 6 * E.Go, E.Go * sizeof(double),
                           hipMemcpyDeviceToDevice, E.stream));
+  std::vector<double> m2;  // sum over units of a_g b_g' (q x q)
+  if (joint) {
+    // match blocks with the exact control part (generic_jla.rs:6053-6273):
+    // per unit contrib = A B, (a, b) and (sum f e, B); per cell [Zc | hc]
+    E.ctl_uab.alloc(E.Gm * 2 * q);
+    E.ctl_umass.alloc(2 * E.Gm);
+    launch_blocks(E, q, opt);
+    E.check_status("generic_jla_match_adjustment");
+    E.ctl_cellmat.alloc(E.C * 2 * q);
+    hipLaunchKernelGGL(dev::k_ctl_cells, grid_for(E.C), kBlock, 0, E.stream, E.C, q, E.cell_unit_ptr.p,
+                       E.cell_units.p, E.unit_outcome.p, E.ctl_umass.p, E.ctl_uab.p, E.ctl_cellz.p, E.ctl_cellmat.p);
+    const std::vector<double> cross = device_gram(E, E.Gm, 2 * q, 1, E.ctl_uab.p, nullptr);
+    m2.assign((size_t)q * q, 0.0);
+    for (uint32_t i = 0; i < q; ++i)
+      for (uint32_t j = 0; j < q; ++j) m2[(size_t)i * q + j] = 0.5 * (cross[(size_t)i * q + j] + cross[(size_t)j * q + i]);
+  }
   hipLaunchKernelGGL(dev::k_cell_weight, grid_for(E.C), kBlock, 0, E.stream, E.C, E.cell_unit_ptr.p,
                      E.cell_units.p, E.contrib.p, E.cellw.p);
   {
@@ -947,6 +1306,14 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     slice(0, res.projection_share); slice(1, res.residual_share); slice(2, res.finite_bias);
     slice(3, res.finite_variance); slice(4, res.residual_mass); slice(5, res.deleted_mass);
     E.cellw.download(res.cell_weight, E.C, E.stream);
+    if (joint) {
+      std::vector<double> um;
+      E.ctl_umass.download(um, 2 * E.Gm, E.stream);
+      for (uint64_t g = 0; g < E.Gm; ++g) {
+        res.residual_mass[g] = um[2 * g];
+        res.deleted_mass[g] = um[2 * g + 1];
+      }
+    }
   }
 
   // ---- numerical attachment state (engine/numerical.rs:62-122)
@@ -1069,6 +1436,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   auto tt = std::chrono::steady_clock::now();
   E.dir.alloc((uint64_t)KT * E.C);
   res.draws.assign(4 * R, 0.0);
+  uint32_t ctl_k2 = 0;
   for (uint64_t first = 0; first < R; first += tw) {
     opt.checkpoint("jla_target_batch");
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
@@ -1107,9 +1475,32 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       hipLaunchKernelGGL((dev::k_contract<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_worker.p, E.cell_firm.p,
                          E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p);
       finish<KTc, 3>(E, E.rpartial.p, cg, dev::FinDraws{E.draws.p});
+      if (joint) {
+        // statistical_batches.rs:259-346 with the joint solve's control
+        // coefficients: per column s, Y_s = sum_c P_cs Zc_c and H_s = sum_c
+        // P_cs hc_c (the host adds beta_s' H_s + beta_s' M2 beta_s with
+        // beta_s = -S^-1 Y_s)
+        const uint32_t m = 2 * q;
+        const unsigned tb = (unsigned)std::max<uint64_t>(
+            1, std::min<uint64_t>(512, (E.C + kBlock / K2c - 1) / (kBlock / K2c)));
+        E.ctl_part.alloc((size_t)tb * m * K2c);
+        E.ctl_out.alloc((size_t)m * K2c);
+        auto go = [&](auto qm) {
+          constexpr int QM = decltype(qm)::value;
+          hipLaunchKernelGGL((dev::k_ctl_target<K2c, QM>), tb, kBlock, 0, E.stream, E.C, m, E.cell_worker.p,
+                             E.cell_firm.p, E.worker.p, E.fsol.p, E.ctl_cellmat.p, E.ctl_part.p);
+        };
+        if (m <= 16) go(std::integral_constant<int, 16>{});
+        else if (m <= 32) go(std::integral_constant<int, 32>{});
+        else go(std::integral_constant<int, 64>{});
+        hipLaunchKernelGGL(dev::k_ctl_finish, grid_for((uint64_t)m * K2c), kBlock, 0, E.stream, tb, m * K2c,
+                           E.ctl_part.p, E.ctl_out.p);
+        ctl_k2 = K2c;
+      }
     });
     std::vector<double> d;
     E.draws.download(d, 4 * k, E.stream);
+    if (joint) control_draws(E, q, m2, ctl_k2, k, d);
     std::copy(d.begin(), d.end(), res.draws.begin() + 4 * first);
     for (auto &c : rc) {
       res.target.push_back(c);
@@ -1238,6 +1629,24 @@ void DeviceEngine::pcg(const std::vector<double> &rhs, uint32_t columns, double 
   std::vector<double> xn;
   E.x.download(xn, m0, E.stream);
   xout = from_renumbered(xn, E.levels[0].perm, K);
+}
+
+ControlFit DeviceEngine::control_fit(const SolveOptions &opt) {
+  Impl &E = *impl_;
+  if (E.P->controls == 0) fail(kInvariant, "controls", "the prepared problem has no controls");
+  std::vector<double> alpha, psi;
+  return fit_controls(E, opt, alpha, psi);
+}
+
+void DeviceEngine::refresh_outcome() {
+  Impl &E = *impl_;
+  const Prepared &p = *E.P;
+  E.unit_outcome.upload(p.unit_outcome);
+  if (E.Go) {
+    const uint64_t first_row = p.rows - E.Go;
+    std::vector<double> y(p.outcome.begin() + first_row, p.outcome.end());
+    E.row_outcome.upload(y);
+  }
 }
 
 void DeviceEngine::solve_batch(const std::vector<double> &wr, const std::vector<double> &fr, uint32_t columns,

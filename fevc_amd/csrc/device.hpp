@@ -35,6 +35,8 @@ struct SolveOptions {
   int precision = 64;             // 64: FP64 V-cycle; 32: FP32 V-cycle, FP64 Krylov
   bool numerical = false;         // numerical MCSE attachment (replayed leverage probes)
   bool exact = false;             // dense exact estimator instead of JLA probes
+  bool joint_controls = false;    // the prepared controls enter jointly (joint nuisance)
+  bool fixed_offset = false;      // the prepared controls enter as a fixed offset
   Interrupt *interrupt = nullptr; // caller's checkpoints (none when null)
   void checkpoint(const char *phase) const {
     if (interrupt) interrupt->checkpoint(phase);
@@ -87,6 +89,24 @@ struct SolveResult {
   NumericalResult numerical;
   bool has_exact = false;
   ExactReceipt exact;
+  // controls: the fit receipts of the control columns (then, under the fixed
+  // offset, the outcome column of the joint fit), the control coefficients,
+  // and the generalised reciprocal condition of the control information
+  uint32_t controls = 0;
+  bool fixed_offset = false;
+  std::vector<ColumnReceipt> control_fit;
+  std::vector<double> control_beta;
+  double control_rcond = 0.0;
+};
+
+// The joint fit with the controls by Frisch-Waugh-Lovell: beta = S^-1 b from
+// the control Schur complement S = Zr' F Zr and b = Zr' F yr of the FE
+// residuals of the controls and the outcome (generic_jla.rs:1930-2000).
+struct ControlFit {
+  uint32_t q = 0;
+  std::vector<double> beta, S, s_inv, b;
+  std::vector<ColumnReceipt> receipts;  // the q control columns, then the outcome column
+  double rss = 0.0, generalized_rcond = 0.0;
 };
 
 bool device_available();
@@ -99,6 +119,12 @@ class DeviceEngine {
   DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h);
   ~DeviceEngine();
   SolveResult run(const SolveOptions &opt);
+  // The joint fit with the prepared controls (Q + 1 two-way solves); refuses
+  // a singular control block with status 82.
+  ControlFit control_fit(const SolveOptions &opt);
+  // Upload the unit and row outcome sums again after the prepared outcome
+  // changed (the fixed-offset working outcome).
+  void refresh_outcome();
   // Dense exact estimator (match deletion, joint nuisance, no controls):
   // exact leverages and target bilinear forms from the inverse information.
   SolveResult run_exact(const SolveOptions &opt);

@@ -287,6 +287,19 @@ struct DeviceEngine::Impl {
   DBuf<double> ngrad, noffs, nfolds, npart, nscores;
   // cap the first PCG pass of a batch at 0.1 gate ||b|| (see solve_columns)
   bool first_pass_cap = true;
+  // controls, uploaded at the first solve that uses them: rows x q controls,
+  // frequency, outcome, the row -> cell map and the unit row lists; per solve
+  // the FE residuals of the controls and the outcome (rows x (q + 1)), the
+  // joint residual, the whitened control rows, per unit (a, b) (units x 2q)
+  // and (sum f e, B), per cell [Zc | hc] (cells x 2q)
+  bool ctl_ready = false;
+  uint32_t ctl_block = 0;           // largest block dimension min(rows of a unit, 1 + q)
+  std::vector<double> ctl_gram0;    // sum f z z' (q x q)
+  std::vector<double> ctl_s_inv, ctl_m2;
+  DBuf<double> ctl_z, ctl_f, ctl_y, ctl_zry, ctl_e, ctl_u, ctl_uab, ctl_umass, ctl_cellmat, ctl_cellz;
+  DBuf<double> ctl_small, ctl_part, ctl_out;
+  DBuf<uint32_t> ctl_row_cell, ctl_unit_rows;
+  DBuf<uint64_t> ctl_unit_ptr;
   // first per-unit gate failure (dev::StatusSink key; all ones when none)
   DBuf<unsigned long long> status;
   uint64_t status_launch = 0;

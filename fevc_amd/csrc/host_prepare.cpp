@@ -537,6 +537,7 @@ Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation) {
     if (P.mask[r]) kept.push_back(r);
   const uint64_t R = kept.size();
   if (!R) fail(kGraphEmpty, "compression", "no rows remain after graph selection");
+  P.row_input = kept;
   auto relabel = [&](const std::vector<uint32_t> &ids, uint64_t levels, uint64_t &count) {
     std::vector<uint8_t> present(levels, 0);
     for (uint64_t r : kept) present[ids[r]] = 1;
@@ -839,6 +840,46 @@ Prepared augment_stayers(const Prepared &M, const StayerColumns &s, StayerReceip
   if (!std::isfinite(rec.stayer_target_mass) || rec.stayer_target_mass < 0.0)
     fail(kInvalidTargetWeight, "stayer_augmentation", "stayer target mass is not finite and nonnegative");
   return P;
+}
+
+void attach_controls(Prepared &P, const double *const *columns, uint32_t q) {
+  if (q > 32) fail(kInvalidInput, "engine_prepare", "at most 32 controls");
+  if (q && !columns) fail(kInvalidInput, "engine_ingest", "control-pointer array is null");
+  if (!q && columns) fail(kInvalidInput, "engine_ingest", "zero controls require a null control-pointer array");
+  for (uint32_t j = 0; j < q; ++j) {
+    if (!columns[j]) fail(kInvalidInput, "engine_ingest", "control column " + std::to_string(j) + " is null");
+    for (uint64_t r = 0; r < P.input_rows; ++r)
+      if (!std::isfinite(columns[j][r]))
+        fail(kInvalidInput, "engine_ingest", "control column " + std::to_string(j) + " has a nonfinite value");
+  }
+  P.controls = q;
+  P.control.assign(P.rows * q, 0.0);
+  for (uint64_t i = 0; i < P.rows; ++i)
+    for (uint32_t j = 0; j < q; ++j) P.control[i * q + j] = columns[j][P.row_input[i]];
+  std::vector<Kahan> sums((size_t)P.cells * q);
+  for (uint64_t i = 0; i < P.rows; ++i) {
+    const double f = static_cast<double>(P.frequency[i]);
+    for (uint32_t j = 0; j < q; ++j) sums[(size_t)P.row_cell[i] * q + j].add(f * P.control[i * q + j]);
+  }
+  P.cell_control.resize(sums.size());
+  for (size_t k = 0; k < sums.size(); ++k) P.cell_control[k] = sums[k].sum;
+}
+
+void set_outcome(Prepared &P, const std::vector<double> &y) {
+  if (y.size() != P.rows) fail(kInvariant, "working_outcome", "working outcome has the wrong length");
+  P.outcome = y;
+  // cell sums in (cell, ascending row) order, as the compression forms them
+  std::vector<Kahan> cell(P.cells);
+  for (uint64_t i = 0; i < P.rows; ++i) cell[P.row_cell[i]].add(static_cast<double>(P.frequency[i]) * y[i]);
+  for (uint64_t c = 0; c < P.cells; ++c) P.cell_outcome[c] = cell[c].sum;
+  for (uint64_t u = 0; u < P.units; ++u) {
+    Neumaier os;
+    for (uint64_t k = P.unit_ptr[u]; k < P.unit_ptr[u + 1]; ++k) {
+      const uint32_t row = P.unit_rows[k];
+      os.add(static_cast<double>(P.frequency[row]) * y[row]);
+    }
+    P.unit_outcome[u] = os.value();
+  }
 }
 
 }  // namespace vk

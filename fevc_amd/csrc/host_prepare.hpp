@@ -69,6 +69,13 @@ struct Prepared {
   std::vector<uint64_t> cell_unit_ptr;  // units of a cell, ascending
   std::vector<uint32_t> cell_units;
   std::vector<uint64_t> cell_stratum_ptr;  // strata of a cell are contiguous
+
+  std::vector<uint64_t> row_input;  // input row of each stored row (ascending)
+  // Controls (problem.rs:647-674 keeps them column-major per stored row):
+  // stored row x controls, row-major, and per cell the frequency-weighted
+  // sums over its rows (Kahan, rows ascending, as the cell outcome sums).
+  uint32_t controls = 0;
+  std::vector<double> control, cell_control;
 };
 
 // Validate, canonicalize, select (unless `mask` is given), compress, plan.
@@ -77,6 +84,14 @@ struct Prepared {
 // instead of the match-deletion one; the exact estimator then deletes every
 // physical observation on its own.
 Prepared prepare(const Columns &cols, const uint8_t *mask, bool observation = false);
+
+// Attach `q` (<= 32) control columns of the input rows to a prepared problem:
+// null pointers and nonfinite values are refused (status 20).
+void attach_controls(Prepared &P, const double *const *columns, uint32_t q);
+
+// Replace the stored-row outcome by `y`: the outcome, cell and unit sums are
+// formed as prepare forms them; graph, cells and plan ranks are unchanged.
+void set_outcome(Prepared &P, const std::vector<double> &y);
 
 // Eligible stayer rows: 1-based dense retained-mover firm and stayer worker
 // indices (stayer_hybrid.rs:23-34).

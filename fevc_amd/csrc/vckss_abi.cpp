@@ -128,8 +128,8 @@ std::vector<uint64_t> integer_column(const double *p, uint64_t rows, const char 
 // copy plus the preparation charge (768 bytes a row, 32 a row and control,
 // 16 a row for a probe-order key or an implicit match key, 4096 fixed) must
 // fit the declared limit.  The engine's own restrictions come last:
-// controls, probe-order keys and implicit match are not run here.  Returns
-// the preparation peak forecast.
+// probe-order keys and implicit match are not run here.  Returns the
+// preparation peak forecast.
 uint64_t admit_prepare(const VckssEnginePrepareRequestV4 &o, uint32_t probeorder_supplied) {
   if (probeorder_supplied > 1) fail(kInvalidInput, "engine_prepare", "probeorder_supplied must be zero or one");
   if (o.v3.v2.abi_version != VCKSS_RUST_ABI_VERSION_V1)
@@ -164,8 +164,6 @@ uint64_t admit_prepare(const VckssEnginePrepareRequestV4 &o, uint32_t probeorder
          "simultaneous caller and engine preparation forecast " + std::to_string((uint64_t)peak) +
              " bytes exceeds the declared limit " + std::to_string(limit) + " bytes");
   if (o.v3.controls_count > 32) fail(kInvalidInput, "engine_prepare", "at most 32 controls");
-  if (o.v3.controls_count != 0)
-    fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
   if (probeorder_supplied)
     fail(kUnsupported, "engine_prepare", "probe-order keys are not supported by the MI355X engine");
   if (o.implicit_match)
@@ -181,8 +179,9 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   const auto &o = req.options;
   if (req.reserved) fail(kAbiMismatch, "engine_prepare", "reserved fields must be zero");
   ctx.preparation_forecast = admit_prepare(o, columns->probeorder_supplied ? 1u : 0u);
-  if (columns->v2.controls_count != 0)
-    fail(kUnsupported, "engine_prepare", "the MI355X engine supports the no-control route only");
+  // ffi_engine.rs:3564-3569: the request and the descriptor declare the same controls
+  if (columns->v2.controls_count != o.v3.controls_count)
+    fail(kInvalidInput, "engine_prepare", "request and descriptor control counts disagree");
   ctx.memory_limit_bytes = o.v3.v2.memory_limit_bytes;
   ctx.caller_copy_bytes = o.v3.v2.caller_copy_bytes;
   const auto t_ingest = std::chrono::steady_clock::now();
@@ -199,6 +198,7 @@ void prepare_into(Context &ctx, const VckssEnginePrepareRequestInterruptV3 *requ
   Columns cols{rows, worker.data(), firm.data(), deletion.data(), c.outcome, frequency.data(), c.target_weight};
   ctx.ingest_ns = elapsed_ns(t_ingest);
   ctx.P = prepare(cols, nullptr, o.v3.deletion_mode == VCKSS_DELETION_OBSERVATION);
+  attach_controls(ctx.P, columns->v2.controls, columns->v2.controls_count);
   interrupt.checkpoint("engine_prepare_graph");
   auto t = std::chrono::steady_clock::now();
   ctx.HS = build_hybrid(ctx.P);
@@ -241,7 +241,8 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
       (v2.algorithm == VCKSS_ALGORITHM_JLA || v2.algorithm == VCKSS_ALGORITHM_AUTO ||
        v2.algorithm == VCKSS_ALGORITHM_EXACT) &&
       (v3.engine == VCKSS_ENGINE_AUTO_OR_UNSPECIFIED || v3.engine == VCKSS_ENGINE_GENERIC) &&
-      deletion_ok && v2.nuisance_mode == VCKSS_NUISANCE_JOINT &&
+      deletion_ok &&
+      (v2.nuisance_mode == VCKSS_NUISANCE_JOINT || v2.nuisance_mode == VCKSS_NUISANCE_FIXED_OFFSET) &&
       v1.rng_contract == VCKSS_RNG_COUNTER_V1 &&
       (v1.solver_route == VCKSS_ROUTE_AUTO || v1.solver_route == VCKSS_ROUTE_CMG_PCG) &&
       v3.batch_mode == VCKSS_BATCH_MODE_AUTO && v4.leverage_batch_mode == VCKSS_BATCH_MODE_AUTO &&
@@ -257,9 +258,20 @@ SolveOptions point_options(const VckssEngineSolveRequestV4 &v4, uint32_t toleran
   if (!eligible)
     fail(kUnsupported, "cmg_full_v2",
          "the MI355X engine requires auto/exact/JLA, auto/generic engine, auto/CMG solver, match or observation "
-         "deletion, joint nuisance, movers, automatic batches and no probe order");
+         "deletion, joint or fixed-offset nuisance, movers, automatic batches and no probe order");
   SolveOptions o;
   o.exact = exact;
+  // controls (generic_jla.rs:1930-2140): the JLA route under match deletion,
+  // joint or fixed-offset nuisance
+  if (prepared && ctx.P.controls) {
+    if (exact)
+      fail(kUnsupported, "exact_estimator", "the MI355X engine's exact route does not take controls");
+    if (ctx.P.observation || ctx.P.stayers)
+      fail(kUnsupported, "generic_jla",
+           "the MI355X engine takes controls under match deletion without stayers only");
+    o.joint_controls = v2.nuisance_mode == VCKSS_NUISANCE_JOINT;
+    o.fixed_offset = v2.nuisance_mode == VCKSS_NUISANCE_FIXED_OFFSET;
+  }
   if (exact) {
     // ffi_engine.rs:5733-5751: dimension and block-size limits of the exact route
     if (prepared && parameters > v2.exact_estimator_limit)
@@ -376,8 +388,8 @@ void record_request(Context &ctx, const VckssEngineSolveRequestV5 &q, const Solv
   ctx.engine_requested = q.v4.v3.engine;
   ctx.algorithm_selected = o.exact ? VCKSS_ALGORITHM_EXACT : VCKSS_ALGORITHM_JLA;
   ctx.engine_selected = o.exact ? VCKSS_ENGINE_NOT_APPLICABLE
-                                : ((ctx.P.observation || ctx.P.stayers) ? VCKSS_ENGINE_GENERIC
-                                                                        : VCKSS_ENGINE_COMPRESSED);
+                                : ((ctx.P.observation || ctx.P.stayers || ctx.P.controls) ? VCKSS_ENGINE_GENERIC
+                                                                                          : VCKSS_ENGINE_COMPRESSED);
   ctx.full_cmg = !o.exact;
 }
 
@@ -385,6 +397,49 @@ void run_solve(Context &ctx, const VckssEngineSolveRequestV5 &q, Interrupt &inte
   const SolveOptions o = solve_options(q, ctx);
   record_request(ctx, q, o);
   run_options(ctx, o, interrupt);
+}
+
+// The fixed-offset nuisance with controls (generic_jla.rs:2012-2045): the
+// joint fit's control coefficients give the working outcome y - Z beta (a
+// compensated sum per row), with which the two-way route runs (its own
+// working fit and the leverage and target probes, whose plan keeps the
+// prepared ranks); the prepared outcome is restored afterwards.
+SolveResult run_fixed_offset(Context &ctx, DeviceEngine &dev, const SolveOptions &o) {
+  Prepared &P = ctx.P;
+  const uint32_t q = P.controls;
+  ControlFit cf = dev.control_fit(o);
+  std::vector<double> working(P.rows);
+  for (uint64_t r = 0; r < P.rows; ++r) {
+    Neumaier v;
+    v.add(P.outcome[r]);
+    for (uint32_t j = 0; j < q; ++j) v.add(-P.control[r * q + j] * cf.beta[j]);
+    working[r] = v.value();
+    if (!std::isfinite(working[r])) fail(kInvalidInput, "generic_jla_fixedoffset", "working outcome is nonfinite");
+  }
+  const std::vector<double> outcome = P.outcome, cell_outcome = P.cell_outcome, unit_outcome = P.unit_outcome;
+  auto restore = [&] {
+    P.outcome = outcome;
+    P.cell_outcome = cell_outcome;
+    P.unit_outcome = unit_outcome;
+    dev.refresh_outcome();
+  };
+  set_outcome(P, working);
+  dev.refresh_outcome();
+  SolveResult r;
+  try {
+    r = dev.run(o);
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+  r.controls = q;
+  r.fixed_offset = true;
+  r.control_fit = cf.receipts;
+  r.control_beta = cf.beta;
+  r.control_rcond = cf.generalized_rcond;
+  for (const auto &c : cf.receipts) r.max_complete_residual = std::max(r.max_complete_residual, c.complete);
+  return r;
 }
 
 // context.rs:150-205: a solve needs a prepared generation; a failed solve
@@ -405,7 +460,8 @@ void run_options(Context &ctx, SolveOptions o, Interrupt &interrupt) {
   dev.counters(ctx.counters_before);
   try {
     o.checkpoint("engine_solve");
-    ctx.result = dev.run(o);
+    if (o.fixed_offset) ctx.result = run_fixed_offset(ctx, dev, o);
+    else ctx.result = dev.run(o);
   } catch (...) {
     ctx.state = kFailed;
     throw;
@@ -544,15 +600,15 @@ uint32_t unknown_v1_fields(const VckssBackendRequestCapabilityRequestV1 &v) {
   return VCKSS_REQUEST_REASON_SUPPORTED;
 }
 
-// What this engine adds to a request the reference would support: it runs
-// no controls, no fixed-offset nuisance, JLA on the automatic or CMG route
-// only (CMG_FULL_V2; no dense-exact or diagonal-PCG probe solves), no
-// probe-order keys, and chooses its own batch widths.
-uint32_t engine_restriction(uint32_t algorithm, uint32_t nuisance, uint32_t route, uint32_t controls,
+// What this engine adds to a request the reference would support: controls
+// on the JLA route under match deletion only (the exact route takes none),
+// JLA on the automatic or CMG route only (CMG_FULL_V2; no dense-exact or
+// diagonal-PCG probe solves), no probe-order keys, and it chooses its own
+// batch widths.
+uint32_t engine_restriction(uint32_t algorithm, uint32_t deletion, uint32_t route, uint32_t controls,
                             uint32_t probeorder, bool explicit_batches) {
-  if (controls != 0)
-    return algorithm == VCKSS_ALGORITHM_JLA ? VCKSS_REQUEST_REASON_JLA_CONTROLS : VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
-  if (nuisance != VCKSS_NUISANCE_JOINT) return VCKSS_REQUEST_REASON_JLA_NUISANCE;
+  if (controls != 0 && algorithm == VCKSS_ALGORITHM_EXACT) return VCKSS_REQUEST_REASON_CONTROLS_LIMIT;
+  if (controls != 0 && deletion != VCKSS_DELETION_MATCH) return VCKSS_REQUEST_REASON_JLA_CONTROLS;
   if (algorithm != VCKSS_ALGORITHM_EXACT && route != VCKSS_ROUTE_AUTO && route != VCKSS_ROUTE_CMG_PCG)
     return VCKSS_REQUEST_REASON_JLA_GENERIC_SOLVER_ROUTE;
   if (probeorder != 0) return VCKSS_REQUEST_REASON_PROBEORDER_UNSUPPORTED;
@@ -579,7 +635,7 @@ std::pair<uint32_t, uint32_t> classify_v1(const VckssBackendRequestCapabilityReq
     else if (v.algorithm == VCKSS_ALGORITHM_JLA && v.rng_contract != VCKSS_RNG_COUNTER_V1)
       reason = VCKSS_REQUEST_REASON_JLA_RNG;
     else
-      reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, 0, false);
+      reason = engine_restriction(v.algorithm, v.deletion_mode, v.solver_route, v.controls_count, 0, false);
   }
   const uint32_t profile = reason != VCKSS_REQUEST_REASON_SUPPORTED ? VCKSS_REQUEST_PROFILE_NONE
                            : v.algorithm == VCKSS_ALGORITHM_EXACT  ? VCKSS_REQUEST_PROFILE_EXACT_V1
@@ -643,7 +699,7 @@ std::pair<uint32_t, uint32_t> classify_v2(const VckssBackendRequestCapabilityReq
                  : lr;
   }
   if (reason == VCKSS_REQUEST_REASON_SUPPORTED)
-    reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, r.probeorder_supplied,
+    reason = engine_restriction(v.algorithm, v.deletion_mode, v.solver_route, v.controls_count, r.probeorder_supplied,
                                 r.batch_mode != VCKSS_BATCH_MODE_AUTO);
   const uint32_t profile = reason != VCKSS_REQUEST_REASON_SUPPORTED     ? VCKSS_REQUEST_PROFILE_NONE
                            : v.algorithm == VCKSS_ALGORITHM_EXACT      ? VCKSS_REQUEST_PROFILE_EXACT_V1
@@ -734,7 +790,7 @@ std::pair<uint32_t, uint32_t> classify_v3(const VckssBackendRequestCapabilityReq
     reason = VCKSS_REQUEST_REASON_COMPRESSED_SCIENTIFIC_INELIGIBILITY;
   }
   if (reason == VCKSS_REQUEST_REASON_SUPPORTED)
-    reason = engine_restriction(v.algorithm, v.nuisance_mode, v.solver_route, v.controls_count, q.probeorder_supplied,
+    reason = engine_restriction(v.algorithm, v.deletion_mode, v.solver_route, v.controls_count, q.probeorder_supplied,
                                 r.leverage_batch_mode != VCKSS_BATCH_MODE_AUTO ||
                                     r.target_batch_mode != VCKSS_BATCH_MODE_AUTO);
   return {reason, reason == VCKSS_REQUEST_REASON_SUPPORTED ? VCKSS_REQUEST_PROFILE_PLANNED_V1 : VCKSS_REQUEST_PROFILE_NONE};
@@ -786,7 +842,15 @@ std::vector<VckssEngineRhsReceiptV1> rhs_rows(const Context &ctx) {
     v.complete_residual = c.complete;
     rows.push_back(v);
   };
-  row(1, 0, -1, r.fit);
+  // ffi_engine.rs:8880-8890 phases: control projections (5), the full joint
+  // fit (1), the fixed-offset working fit (4), leverage (2), target (3)
+  for (uint32_t j = 0; j < r.controls && j < r.control_fit.size(); ++j) row(5, 0, (int64_t)j, r.control_fit[j]);
+  if (r.fixed_offset && r.control_fit.size() > r.controls) {
+    row(1, 0, -1, r.control_fit[r.controls]);
+    row(4, 0, -1, r.fit);
+  } else {
+    row(1, 0, -1, r.fit);
+  }
   for (size_t t = 0; t < r.leverage.size(); ++t) row(2, 0, (int64_t)t, r.leverage[t]);
   for (size_t c = 0; c < r.target.size(); ++c) row(3, 1 + (uint32_t)(c % 2), (int64_t)(c / 2), r.target[c]);
   return rows;
@@ -894,9 +958,12 @@ VckssEngineDetailedReceiptV6 detailed_v6(uint64_t generation, const Context &ctx
   d5.actual_accounting_residual = r.accounting_residual;
   o.engine_requested = ctx.engine_requested;
   o.engine_selected = ctx.engine_selected;
-  o.controls_count = 0;
+  o.controls_count = r.controls;
   o.rhs_receipt_schema = rows ? 1u : 0u;
-  o.full_joint_fit_complete_residual = r.fit.complete;
+  // the full joint fit: its outcome column (joint nuisance), or the outcome
+  // column of the control fit before the working fit (fixed offset)
+  o.full_joint_fit_complete_residual =
+      r.fixed_offset && r.control_fit.size() > r.controls ? r.control_fit[r.controls].complete : r.fit.complete;
   o.generic_working_fit_complete_residual = r.fit.complete;
   o.capability_schema = q.v4.v3.capability_schema;
   o.capability_profile = q.v4.v3.capability_profile;
@@ -940,7 +1007,7 @@ VckssExecutionPlanReceiptV1 execution_plan(uint64_t generation, const Context &c
   sol.full_setup_complete = 1;
   sol.plan_frozen_before_rng = 1;
   sol.threads_requested = sol.threads_used = std::max<uint32_t>(q.threads, 1);
-  sol.planned_rhs = 1 + r.leverage.size() + r.target.size();
+  sol.planned_rhs = 1 + r.control_fit.size() + r.leverage.size() + r.target.size();
   sol.full_solver_dimension = ctx.full_cmg ? ctx.HS.graph.n : ctx.P.workers + ctx.P.firms - 1;
   auto &b = e.batch;
   b.schema_version = 1;
@@ -1007,7 +1074,7 @@ uint32_t vckss_rust_abi_version(void) { return VCKSS_RUST_ABI_VERSION_V1; }
 const char *vckss_rust_backend_version(void) { return "vckss-mi355x 0.1.0 (gfx950, CMG_FULL_V2)"; }
 const char *vckss_rust_capabilities_json(void) {
   return "{\"backend\":\"vckss-mi355x\",\"device\":\"gfx950\",\"routes\":[\"CMG_FULL_V2\"],"
-         "\"deletion\":[\"match\",\"observation\"],\"stayers\":[\"movers\",\"all\"],\"rng\":[\"VCKSS-COUNTER-V1\"],\"controls\":0,"
+         "\"deletion\":[\"match\",\"observation\"],\"stayers\":[\"movers\",\"all\"],\"rng\":[\"VCKSS-COUNTER-V1\"],\"controls\":32,"
          "\"precision\":[64,32],\"max_batch_rhs\":64}";
 }
 const char *vckss_rust_last_error(void) { return g_last_error.c_str(); }
@@ -1128,6 +1195,9 @@ static void augment_stayers_value(uint64_t generation, const VckssStayerAugmenta
     target.assign(c.target_weight, c.target_weight + rows);
   }
   Context &ctx = live(generation);
+  if (ctx.P.controls)
+    fail(kUnsupported, "engine_stayer_augmentation",
+         "the MI355X engine does not augment a generation prepared with controls");
   if (ctx.state != kPrepared)
     fail(kContextPoisoned, "context_augment", "generation " + std::to_string(generation) +
                                                   " is not in the Prepared state");
@@ -1393,6 +1463,8 @@ static void solve_numerical(uint64_t generation, const VckssNumericalRequestV1 &
   SolveOptions o = point_options(point, r.tolerance_supplied, ctx);
   if (o.exact)
     fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE to the JLA route only");
+  if (ctx.P.controls)
+    fail(kUnsupported, "numerical_solve", "the MI355X engine attaches numerical MCSE without controls only");
   o.numerical = true;
   VckssEngineSolveRequestV5 q;
   std::memset(&q, 0, sizeof q);
@@ -1893,6 +1965,21 @@ int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t
   });
 }
 
+int32_t vckss_mi355x_controls_v1(uint64_t generation, double *out, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Context &ctx = live(generation);
+    if (!ctx.has_result) fail(kStaleContext, "controls_receipt", "no solved result is available");
+    const SolveResult &r = ctx.result;
+    if (!r.controls) fail(kUnsupported, "controls_receipt", "the result has no controls");
+    if (!out || capacity < 3 + r.controls) fail(kAbiMismatch, "controls_receipt", "capacity must be at least 3 + q");
+    out[0] = r.controls;
+    out[1] = r.fixed_offset ? 1.0 : 0.0;
+    out[2] = r.control_rcond;
+    for (uint32_t j = 0; j < r.controls; ++j) out[3 + j] = r.control_beta[j];
+  });
+}
+
 int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint32_t capacity, uint32_t *count) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {
@@ -2270,7 +2357,7 @@ int32_t vckss_rust_engine_preparation_receipt_v4(uint64_t generation, VckssEngin
     d.graph_bridge_iterations = g.bridge_iterations;
     d.graph_fixed_point_iterations = g.fixed_point_iterations;
     o.v3.target_weight_sum = P.target_total;
-    o.controls_count = 0;
+    o.controls_count = P.controls;
     o.deletion_mode = P.observation ? VCKSS_DELETION_OBSERVATION : VCKSS_DELETION_MATCH;
     std::memcpy(output, &o, sizeof o);
   });
@@ -2422,10 +2509,10 @@ int32_t vckss_rust_engine_full_cmg_receipt_v1(uint64_t generation, VckssFullCmgR
     o.batch_calls = b[1] - a[1];
     o.across_rhs_batches = o.batch_calls;  // every batch is a lockstep multi-RHS solve
     o.batch_strategy_mask = o.across_rhs_batches > 0 ? 4u : 0u;  // ffi_engine.rs:8607-8609
-    o.rhs_count = 1 + r.leverage.size() + r.target.size();
+    o.rhs_count = 1 + r.control_fit.size() + r.leverage.size() + r.target.size();
     uint64_t it = r.fit.iterations;
     double red = r.fit.reduced;
-    for (const auto *set : {&r.leverage, &r.target})
+    for (const auto *set : {&r.control_fit, &r.leverage, &r.target})
       for (const ColumnReceipt &c : *set) {
         it += c.iterations;
         red = std::max(red, c.reduced);

@@ -180,6 +180,13 @@ int32_t vckss_mi355x_graphs_v1(uint64_t generation, uint32_t enable, uint64_t *c
  * factorisation and fit, seconds for the cell pass.  11 if the result is a
  * JLA estimate. */
 int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t capacity);
+/* The control fit of the last solve with controls (no counterpart in the
+ * reference's C ABI; its generic-engine receipt holds the same quantities):
+ * out[0] the control count q, out[1] 1 under the fixed-offset nuisance else
+ * 0, out[2] the generalised reciprocal condition of the residualised control
+ * information, out[3 .. 3 + q) the control coefficients.  capacity >= 3 + q;
+ * 11 if the result has no controls. */
+int32_t vckss_mi355x_controls_v1(uint64_t generation, double *out, uint32_t capacity);
 /* Levels the cycle visits (all levels unless the tuned shape is in use). */
 int32_t vckss_mi355x_cycle_shape_v1(uint64_t generation, uint32_t *levels, uint32_t capacity, uint32_t *count);
 /* Built-in trained parameters (count = 0 when the library carries none). */

@@ -88,9 +88,8 @@ def test_v1_reason_codes_and_engine_restrictions():
         (dict(jla, nuisance_mode=K["VCKSS_NUISANCE_FIXED_OFFSET"]), R["JLA_NUISANCE"]),
         (dict(jla, controls_count=1), R["JLA_CONTROLS"]),
         (dict(jla, rng_contract=K["VCKSS_RNG_NONE"]), R["JLA_RNG"]),
-        # this engine: no controls, joint nuisance, JLA on the automatic or CMG route
+        # this engine: no controls on the exact route, JLA on the automatic or CMG route
         (dict(controls_count=32), R["CONTROLS_LIMIT"]),
-        (dict(nuisance_mode=K["VCKSS_NUISANCE_FIXED_OFFSET"]), R["JLA_NUISANCE"]),
         (dict(jla, solver_route=K["VCKSS_ROUTE_DIAGONAL_PCG"]), R["JLA_GENERIC_SOLVER_ROUTE"]),
         (dict(jla, solver_route=K["VCKSS_ROUTE_EXACT"]), R["JLA_GENERIC_SOLVER_ROUTE"]),
     ]
@@ -100,8 +99,10 @@ def test_v1_reason_codes_and_engine_restrictions():
     for deletion in (K["VCKSS_DELETION_MATCH"], K["VCKSS_DELETION_OBSERVATION"]):
         for route in (K["VCKSS_ROUTE_AUTO"], K["VCKSS_ROUTE_EXACT"]):
             for freq in (0, 1):
-                o = receipt_v1(v1_request(deletion_mode=deletion, solver_route=route, frequency_use=freq))
-                assert (o.supported, o.profile_code) == (1, K["VCKSS_REQUEST_PROFILE_EXACT_V1"])
+                for nuisance in (K["VCKSS_NUISANCE_JOINT"], K["VCKSS_NUISANCE_FIXED_OFFSET"]):
+                    o = receipt_v1(v1_request(deletion_mode=deletion, solver_route=route, frequency_use=freq,
+                                              nuisance_mode=nuisance))
+                    assert (o.supported, o.profile_code) == (1, K["VCKSS_REQUEST_PROFILE_EXACT_V1"])
     for route in (K["VCKSS_ROUTE_AUTO"], K["VCKSS_ROUTE_CMG_PCG"]):
         o = receipt_v1(v1_request(**jla, solver_route=route))
         assert (o.supported, o.profile_code) == (1, K["VCKSS_REQUEST_PROFILE_JLA_COUNTER_V1"])
@@ -147,7 +148,8 @@ def test_v2_signatures_distinct_and_reasons():
                                     controls_count=controls, frequency_use=freq, target_weight_mode=tw,
                                     physical_limit=limit))
                                 assert o.v1.struct_size == 104
-                                ok = nuisance == 1 and controls == 0
+                                # controls run on the match route only
+                                ok = controls == 0 or deletion == K["VCKSS_DELETION_MATCH"]
                                 assert o.v1.supported == int(ok), (deletion, source, nuisance, controls)
                                 if ok:
                                     assert o.v1.profile_code == K["VCKSS_REQUEST_PROFILE_JLA_GENERIC_COUNTER_V1"]
@@ -245,10 +247,16 @@ def test_prepare_admission():
     assert F.admit_prepare(prepare_v4(10), probeorder_supplied=2) == 20
     # implicit match needs match deletion, no controls and a probe order (11)
     assert F.admit_prepare(prepare_v4(10, implicit_match=1)) == 11
-    # the engine's restrictions, after a passing admission (11)
+    # controls: 8 bytes a row and control in the caller's copy, 32 in the
+    # preparation charge (session.rs:202-290)
     r = prepare_v4(10, controls_count=2, caller_copy_bytes=10 * 8 * 8,
                    memory_limit_bytes=10 * (64 + 768 + 64) + 4096)
-    assert F.admit_prepare(r) == 11
+    assert F.admit_prepare(r) == 0
+    r.v3.v2.memory_limit_bytes -= 1
+    assert F.admit_prepare(r) == 40
+    assert F.admit_prepare(prepare_v4(10, controls_count=33, caller_copy_bytes=10 * 39 * 8,
+                                      memory_limit_bytes=1 << 30)) == 20
+    # the engine's restrictions, after a passing admission (11)
     r = prepare_v4(10, caller_copy_bytes=10 * 7 * 8, memory_limit_bytes=10 * (56 + 768 + 16) + 4096)
     assert F.admit_prepare(r, probeorder_supplied=1) == 11
     assert F.admit_prepare(prepare_v4(10, deletion_mode=K["VCKSS_DELETION_OBSERVATION"])) == 0
