@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=16,
                     help="single-RHS FP64 solves timed for the comparison (0 skips)")
     ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
+    ap.add_argument("--controls", type=int, default=10,
+                    help="year-dummy controls of the controls estimate on the same panel (0 skips)")
     ap.add_argument("--compare-plain", type=int, default=1,
                     help="also run the KSS estimate on the plain single-RHS FP64 path and report the error")
     ap.add_argument("--exact", type=int, default=1, help="time the dense exact estimator on the small panel")
@@ -127,6 +129,11 @@ def main():
     log(f"generate {spec.name}")
     data = synth.generate(spec)
     cols = [data[k] for k in ("worker", "firm", "deletion", "outcome", "frequency", "target")]
+    year_z = year_y = None
+    if args.controls > 0:
+        # the same panel with `controls` year dummies and a year effect in the outcome
+        year_z, year_y = synth.year_controls(spec, data, years=max(args.controls, spec.periods) + 1)
+        year_z = year_z[:, :args.controls]
     del data
     t0 = time.perf_counter()
     eng = F.Engine(*cols)
@@ -242,6 +249,45 @@ def main():
             kss["max_abs_leverage_error_vs_plain_fp64"] = lev
             log(f"plain FP64 KSS: {pw:.2f}s; component rel. error {rel}; leverage abs. error {lev}")
     eng.release()
+
+    controls = None
+    if year_z is not None:
+        # the same graph with year-dummy controls: the joint nuisance (the
+        # command's default: Q projection solves, the joint fit, 3R probe
+        # solves) and the fixed offset (one more working fit)
+        ccols = list(cols)
+        ccols[3] = year_y
+        probes = args.kss_probes or 256
+        t0 = time.perf_counter()
+        ce = F.Engine(*ccols, controls=year_z)
+        cprep = time.perf_counter() - t0
+        try:
+            ce.configure(args.precision, min(args.width, 64), min(max(args.width // 2, 1), 32))
+            ce.smoother(1 if args.learned else 0)
+            # device set-up (a few-probe estimate can meet a block whose
+            # estimated joint leverage reaches one, so the warm-up uses all)
+            ce.solve(probes=probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
+            q = year_z.shape[1]
+            controls = {"controls": q, "kind": "year dummies", "prepare_seconds": round(cprep, 1),
+                        "probes": probes}
+            for nuisance in ("joint", "fixed_offset"):
+                t0 = time.perf_counter()
+                r = ce.solve(probes=probes, tolerance=args.tolerance, fit_tolerance=args.tolerance,
+                             nuisance=nuisance)
+                w = time.perf_counter() - t0
+                cf = ce.control_fit()
+                rhs = q + 1 + (nuisance == "fixed_offset") + 3 * probes
+                controls[nuisance] = {"wall_seconds": round(w, 3), "rhs_solves": rhs,
+                                      "rhs_solves_per_s": round(rhs / w, 1), "corrected": r["corrected"],
+                                      "control_rcond": cf["rcond"],
+                                      "max_abs_year_coefficient": float(np.max(np.abs(cf["beta"])))}
+                log(f"controls ({q} year dummies, {nuisance}): KSS {w:.2f}s, {rhs / w:.0f} RHS solves/s, "
+                    f"corrected {r['corrected']}")
+            if kss:
+                controls["joint_wall_vs_no_controls"] = round(controls["joint"]["wall_seconds"] /
+                                                              kss["wall_seconds"], 3)
+        finally:
+            ce.release()
 
     exact = None
     if args.exact:
@@ -456,6 +502,7 @@ def main():
         "plain_fp64_single_rhs": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,
+        "controls_estimate": controls,
         "exact_estimate": exact,
         "observation_estimate": observation,
         "learned_eval_10M": learned_eval,

@@ -1285,7 +1285,9 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     E.ctl_umass.alloc(2 * E.Gm);
     launch_blocks(E, q, opt);
     E.check_status("generic_jla_match_adjustment");
-    E.ctl_cellmat.alloc(E.C * 2 * q);
+    // 64 doubles of zero padding: the target kernel reads rows 32 or 64 wide
+    E.ctl_cellmat.alloc(E.C * 2 * q + 64);
+    VK_HIP(hipMemsetAsync(E.ctl_cellmat.p + E.C * 2 * q, 0, 64 * sizeof(double), E.stream));
     hipLaunchKernelGGL(dev::k_ctl_cells, grid_for(E.C), kBlock, 0, E.stream, E.C, q, E.cell_unit_ptr.p,
                        E.cell_units.p, E.unit_outcome.p, E.ctl_umass.p, E.ctl_uab.p, E.ctl_cellz.p, E.ctl_cellmat.p);
     const std::vector<double> cross = device_gram(E, E.Gm, 2 * q, 1, E.ctl_uab.p, nullptr);
@@ -1482,7 +1484,7 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
         // beta_s = -S^-1 Y_s)
         const uint32_t m = 2 * q;
         const unsigned tb = (unsigned)std::max<uint64_t>(
-            1, std::min<uint64_t>(512, (E.C + kBlock / K2c - 1) / (kBlock / K2c)));
+            1, std::min<uint64_t>(4096, (E.C + kBlock / K2c - 1) / (kBlock / K2c)));
         E.ctl_part.alloc((size_t)tb * m * K2c);
         E.ctl_out.alloc((size_t)m * K2c);
         auto go = [&](auto qm) {
@@ -1493,8 +1495,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
         if (m <= 16) go(std::integral_constant<int, 16>{});
         else if (m <= 32) go(std::integral_constant<int, 32>{});
         else go(std::integral_constant<int, 64>{});
-        hipLaunchKernelGGL(dev::k_ctl_finish, grid_for((uint64_t)m * K2c), kBlock, 0, E.stream, tb, m * K2c,
-                           E.ctl_part.p, E.ctl_out.p);
+        hipLaunchKernelGGL(dev::k_ctl_finish_wave, grid_for((uint64_t)m * K2c * 64), kBlock, 0, E.stream, tb,
+                           m * K2c, E.ctl_part.p, E.ctl_out.p);
         ctl_k2 = K2c;
       }
     });

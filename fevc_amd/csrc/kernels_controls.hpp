@@ -142,12 +142,17 @@ __global__ void k_ctl_combine(uint64_t n, uint32_t q, const double *__restrict__
 // columns of a row of the block factor L.
 __global__ void k_ctl_whiten(uint64_t rows, uint32_t q, const double *__restrict__ zry, const double *__restrict__ f,
                              const double *__restrict__ phi, double *__restrict__ u) {
+  __shared__ double sphi[kMaxControls * kMaxControls];
+  for (uint32_t k = threadIdx.x; k < q * q; k += kBlock) sphi[k] = phi[k];
+  __syncthreads();
   VK_GRID_LOOP(r, rows) {
     const double *z = zry + r * (q + 1);
+    double zr[kMaxControls];
+    for (uint32_t i = 0; i < q; ++i) zr[i] = z[i];
     const double root = sqrt(f[r]);
     for (uint32_t j = 0; j < q; ++j) {
       double v = 0.0;
-      for (uint32_t i = 0; i < q; ++i) v += z[i] * phi[i * q + j];
+      for (uint32_t i = 0; i < q; ++i) v += zr[i] * sphi[i * q + j];
       u[r * q + j] = root * v;
     }
   }
@@ -361,8 +366,14 @@ __global__ void k_ctl_cells(uint64_t cells, uint32_t q, const uint64_t *__restri
 }
 
 // Per target column j (K2 lanes): sum over cells of P_cj [Zc | hc]_c with
-// P_cj = worker[w K2 + j] + firm[f K2 + j].  Threads of a block take the
-// columns of 256 / K2 cells at a time; per block the partial is (m x K2).
+// P_cj = worker[w K2 + j] + firm[f K2 + j] (w, f the cell's worker and firm).
+// With K2 = 64 a wavefront takes one cell at a time, its lanes the columns:
+// the cell's indices and its [Zc | hc] row are wave-uniform (scalar loads)
+// and only the two solution rows are vector loads; two cells per step keep
+// both cells' loads in flight.  With fewer columns a wavefront takes 64 / K2
+// cells.  Rows are read QM wide (cellmat carries QM doubles of padding at
+// its end; columns m .. QM - 1 are never stored).  Per block the partial is
+// (m x K2).
 template <int K2, int QM>
 __global__ void __launch_bounds__(256) k_ctl_target(uint64_t cells, uint32_t m, const uint32_t *__restrict__ cell_worker,
                                                     const uint32_t *__restrict__ cell_firm,
@@ -370,29 +381,71 @@ __global__ void __launch_bounds__(256) k_ctl_target(uint64_t cells, uint32_t m, 
                                                     const double *__restrict__ cellmat, double *__restrict__ partial) {
   constexpr int G = kBlock / K2;
   __shared__ double red[kBlock];
-  const int j = threadIdx.x % K2, grp = threadIdx.x / K2;
+  const int j = threadIdx.x % K2;
   double acc[QM];
 #pragma unroll
   for (int i = 0; i < QM; ++i) acc[i] = 0.0;
-  for (uint64_t c = (uint64_t)blockIdx.x * G + grp; c < cells; c += (uint64_t)gridDim.x * G) {
-    const double p = worker[(uint64_t)cell_worker[c] * K2 + j] + firm[(uint64_t)cell_firm[c] * K2 + j];
-    const double *row = cellmat + c * m;
+  if constexpr (K2 == 64) {
+    const uint64_t wave = (uint64_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    const uint64_t stride = (uint64_t)gridDim.x * G;
+    uint64_t c = (uint64_t)blockIdx.x * G + wave;
+    for (; c + stride < cells; c += 2 * stride) {
+      const uint64_t d = c + stride;
+      const double p = worker[(uint64_t)cell_worker[c] * K2 + j] + firm[(uint64_t)cell_firm[c] * K2 + j];
+      const double p2 = worker[(uint64_t)cell_worker[d] * K2 + j] + firm[(uint64_t)cell_firm[d] * K2 + j];
+      const double *row = cellmat + c * m, *row2 = cellmat + d * m;
 #pragma unroll
-    for (int i = 0; i < QM; ++i)
-      if (i < (int)m) acc[i] += p * row[i];
+      for (int i = 0; i < QM; ++i) acc[i] += p * row[i] + p2 * row2[i];
+    }
+    if (c < cells) {
+      const double p = worker[(uint64_t)cell_worker[c] * K2 + j] + firm[(uint64_t)cell_firm[c] * K2 + j];
+      const double *row = cellmat + c * m;
+#pragma unroll
+      for (int i = 0; i < QM; ++i)
+        acc[i] += p * row[i];
+    }
+  } else {
+    const int grp = threadIdx.x / K2;
+    for (uint64_t c = (uint64_t)blockIdx.x * G + grp; c < cells; c += (uint64_t)gridDim.x * G) {
+      const double p = worker[(uint64_t)cell_worker[c] * K2 + j] + firm[(uint64_t)cell_firm[c] * K2 + j];
+      const double *row = cellmat + c * m;
+#pragma unroll
+      for (int i = 0; i < QM; ++i)
+        acc[i] += p * row[i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < QM; ++i) {
-    if (i >= (int)m) break;
-    __syncthreads();
-    red[threadIdx.x] = acc[i];
-    __syncthreads();
-    if (threadIdx.x < K2) {
-      double s = 0.0;
-      for (int k = 0; k < G; ++k) s += red[k * K2 + threadIdx.x];
-      partial[((uint64_t)blockIdx.x * m + i) * K2 + threadIdx.x] = s;
+    if (i < (int)m) {  // m is uniform: every thread takes the same barriers
+      __syncthreads();
+      red[threadIdx.x] = acc[i];
+      __syncthreads();
+      if (threadIdx.x < K2) {
+        double s = 0.0;
+        for (int k = 0; k < G; ++k) s += red[k * K2 + threadIdx.x];
+        partial[((uint64_t)blockIdx.x * m + i) * K2 + threadIdx.x] = s;
+      }
     }
   }
+}
+
+// Column sums of a blocks x n partial table, one wavefront per column: each
+// lane a compensated sum over every 64th block, then the lanes' sums.
+__global__ void k_ctl_finish_wave(uint32_t blocks, uint32_t n, const double *__restrict__ partial,
+                                  double *__restrict__ out) {
+  const uint64_t o = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (o >= n) return;
+  double s = 0.0, c = 0.0;
+  for (uint32_t b = lane; b < blocks; b += 64) {
+    const double v = partial[(uint64_t)b * n + o];
+    const double t = s + v;
+    c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  s += c;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[o] = s;
 }
 
 }  // namespace dev
