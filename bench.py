@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=16,
                     help="single-RHS FP64 solves timed for the comparison (0 skips)")
     ap.add_argument("--kss-probes", type=int, default=256, help="full KSS estimate probes (0 skips)")
+    ap.add_argument("--subsamples", type=int, default=64,
+                    help="uneven-size subsamples of the sample-batch estimate (0 skips)")
     ap.add_argument("--controls", type=int, default=10,
                     help="year-dummy controls of the controls estimate on the same panel (0 skips)")
     ap.add_argument("--compare-plain", type=int, default=1,
@@ -283,11 +285,72 @@ def main():
                                       "max_abs_year_coefficient": float(np.max(np.abs(cf["beta"])))}
                 log(f"controls ({q} year dummies, {nuisance}): KSS {w:.2f}s, {rhs / w:.0f} RHS solves/s, "
                     f"corrected {r['corrected']}")
+                if nuisance == "joint":
+                    joint_arrays = ce.arrays(probes)
+                    joint_result = r
             if kss:
                 controls["joint_wall_vs_no_controls"] = round(controls["joint"]["wall_seconds"] /
                                                               kss["wall_seconds"], 3)
+            if args.compare_plain:
+                # the joint estimate on the plain single-RHS FP64 path, identical draws:
+                # per-component and per-leverage error of the batched mixed-precision run
+                ce.configure(64, 1, 1)
+                ce.smoother(0)
+                t0 = time.perf_counter()
+                ref = ce.solve(probes=probes, tolerance=args.tolerance, fit_tolerance=args.tolerance)
+                pw = time.perf_counter() - t0
+                ref_arr = ce.arrays(probes)
+                controls["joint"]["plain_fp64_wall_seconds"] = round(pw, 3)
+                controls["joint"]["max_rel_error_vs_plain_fp64"] = {
+                    k: float(np.max(np.abs(np.asarray(joint_result[k]) - np.asarray(ref[k])) /
+                                    np.maximum(np.abs(np.asarray(ref[k])), 1e-300)))
+                    for k in ("plugin", "correction", "corrected")}
+                controls["joint"]["max_abs_leverage_error_vs_plain_fp64"] = {
+                    k: float(np.max(np.abs(joint_arrays[k] - ref_arr[k])))
+                    for k in ("projection_share", "residual_share")}
+                log(f"controls, plain FP64 joint KSS: {pw:.2f}s; errors "
+                    f"{controls['joint']['max_rel_error_vs_plain_fp64']}")
         finally:
             ce.release()
+
+    batch = None
+    if args.subsamples > 0:
+        # many independent region-year subsamples of uneven size (log-normal
+        # workers, median 40 000, clipped to 5 000-500 000; firms = workers /
+        # 20; T = 4; mobility U(0.05, 0.25)), observation deletion, 128 probes
+        # each: concurrently on 4 streams against one sample at a time
+        sc, sspecs = synth.subsamples(count=args.subsamples, scale=args.scale)
+        t0 = time.perf_counter()
+        sb = F.SampleBatch(sc["sample"], *[sc[k] for k in ("worker", "firm", "deletion", "outcome", "frequency",
+                                                             "target")],
+                           deletion_mode="observation", threads=cpu_threads(0))
+        bprep = time.perf_counter() - t0
+        try:
+            sb.solve(probes=16, tolerance=args.tolerance, fit_tolerance=args.tolerance, precision=args.precision,
+                     streams=4)  # warm: device buffers, code objects
+            runs = {}
+            for streams in (1, 4):
+                t0 = time.perf_counter()
+                res = sb.solve(probes=128, tolerance=args.tolerance, fit_tolerance=args.tolerance,
+                               precision=args.precision, streams=streams)
+                runs[streams] = (time.perf_counter() - t0, res)
+            w1, r1 = runs[1]
+            w4, r4 = runs[4]
+            same = all(np.array_equal(a["corrected"], b["corrected"]) for a, b in zip(r1, r4))
+            rows = [int(r["rows"]) for r in r4]
+            batch = {"subsamples": len(r4), "deletion": "observation", "probes": 128,
+                     "workers_median": int(np.median([s.workers for s in sspecs])),
+                     "workers_range": [min(s.workers for s in sspecs), max(s.workers for s in sspecs)],
+                     "stored_rows_total": int(sum(rows)), "prepare_seconds": round(bprep, 2),
+                     "one_at_a_time_seconds": round(w1, 3), "concurrent_4_streams_seconds": round(w4, 3),
+                     "speedup": round(w1 / w4, 2),
+                     "probe_solves_per_s": round(3 * 128 * len(r4) / w4, 1),
+                     "results_identical": bool(same),
+                     "largest_sample_seconds": round(max(r["seconds"] for r in r4), 3)}
+            log(f"subsample batch ({len(r4)} samples, {sum(rows)} rows): one at a time {w1:.2f}s, "
+                f"4 streams {w4:.2f}s ({w1 / w4:.2f}x), identical {same}")
+        finally:
+            sb.release()
 
     exact = None
     if args.exact:
@@ -503,6 +566,7 @@ def main():
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "kss_estimate": kss,
         "controls_estimate": controls,
+        "subsample_batch": batch,
         "exact_estimate": exact,
         "observation_estimate": observation,
         "learned_eval_10M": learned_eval,

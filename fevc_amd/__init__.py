@@ -302,6 +302,12 @@ SIGNATURES = [
     ("vckss_mi355x_smoother_force_v1", C.c_int32, [C.c_uint64, C.c_double]),
     ("vckss_mi355x_exact_receipt_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_controls_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_batch_prepare_v1", C.c_int32, [C.c_uint64] + [C.c_void_p] * 7 +
+     [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("vckss_mi355x_batch_solve_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double,
+                                                C.c_uint32, C.c_uint32]),
+    ("vckss_mi355x_batch_result_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
+    ("vckss_mi355x_batch_release_v1", C.c_int32, [C.c_uint64]),
     ("vckss_mi355x_graphs_v1", C.c_int32,
      [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("vckss_mi355x_cycle_shape_v1", C.c_int32, [C.c_uint64, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -957,3 +963,44 @@ def build_hierarchy(n, u, v, w, direct_threshold=700, max_levels=128):
         return levels, int(fnnz.value)
     finally:
         lib.vckss_mi355x_hierarchy_free_v1(h.value)
+
+
+class SampleBatch:
+    """Independent samples (rows tagged by `sample`) prepared together and
+    estimated concurrently on the GPU (vckss_mi355x_batch_*_v1)."""
+
+    def __init__(self, sample, worker, firm, deletion, outcome, frequency, target, *,
+                 deletion_mode="match", threads=8):
+        cols = [_f64(sample), _f64(worker), _f64(firm), _f64(deletion), _f64(outcome), _f64(frequency),
+                _f64(target)]
+        self._cols = cols
+        h, n = C.c_uint64(), C.c_uint32()
+        _check(library().vckss_mi355x_batch_prepare_v1(
+            len(cols[0]), *[c.ctypes.data for c in cols], {"match": 1, "observation": 2}[deletion_mode],
+            threads, C.byref(h), C.byref(n)))
+        self.handle, self.samples = h.value, n.value
+
+    def solve(self, seed=8675309, probes=128, tolerance=1e-8, fit_tolerance=1e-8, precision=32, streams=4):
+        _check(library().vckss_mi355x_batch_solve_v1(self.handle, seed, probes, tolerance, fit_tolerance,
+                                                     precision, streams))
+        return [self.result(i) for i in range(self.samples)]
+
+    def result(self, i):
+        out = np.zeros(24)
+        _check(library().vckss_mi355x_batch_result_v1(self.handle, i, out.ctypes.data, out.size))
+        keys = ("sample", "rows", "workers", "firms", "units", "seconds", "iterations", "max_complete_residual")
+        r = dict(plugin=out[0:4].copy(), correction=out[4:8].copy(), corrected=out[8:12].copy(),
+                 mcse=out[12:16].copy())
+        r.update({k: float(v) for k, v in zip(keys, out[16:24])})
+        return r
+
+    def release(self):
+        if getattr(self, "handle", 0):
+            library().vckss_mi355x_batch_release_v1(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -1057,7 +1057,10 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   int devices = 0;
   if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
     fail(kUnsupported, "device", "no HIP device is available for the MI355X engine");
-  VK_HIP(hipStreamCreate(&E.stream));
+  // non-blocking: engines of independent samples run concurrently, each on
+  // its own stream (the host copies to the device happen while the stream is
+  // idle, after its last synchronisation)
+  VK_HIP(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
   E.P = &p;
   E.HS = &hs;
   E.W = p.workers; E.F = p.firms; E.C = p.cells; E.G = p.units; E.S = p.strata;

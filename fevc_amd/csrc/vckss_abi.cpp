@@ -17,7 +17,10 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <string>
+#include <thread>
+#include <atomic>
 
 #include "../../include/vckss_mi355x.h"
 #include "../../include/vckss_mi355x_ext.h"
@@ -1068,6 +1071,65 @@ void record_unsupported(const char *name) {
 const char *thread_last_error() { return g_last_error.c_str(); }
 }  // namespace vk
 
+// ------------------------------------------------------------ sample batches
+// Independent samples (e.g. region-year subsamples) prepared together and
+// estimated concurrently: each sample is its own prepared problem, hierarchy
+// and device engine, and up to `streams` of them run at once, each driven by
+// a host thread on its own non-blocking HIP stream.  No counterpart in the
+// reference, whose caller runs one sample per command.
+struct BatchSample {
+  uint64_t id = 0;
+  Prepared P;
+  HybridSystem HS;
+  Hierarchy H;
+  std::unique_ptr<DeviceEngine> dev;
+  SolveResult result;
+  bool solved = false;
+  int32_t status = 0;
+  std::string error;
+  double seconds = 0.0;
+};
+struct Batch {
+  bool observation = false;
+  std::vector<std::unique_ptr<BatchSample>> samples;
+};
+std::map<uint64_t, std::unique_ptr<Batch>> g_batches;
+uint64_t g_next_batch = 1;
+
+Batch &live_batch(uint64_t handle) {
+  auto it = g_batches.find(handle);
+  if (it == g_batches.end()) fail(kStaleContext, "batch", "batch handle " + std::to_string(handle) + " is not live");
+  return *it->second;
+}
+
+// Run f(i) for i in [0, n) on `threads` host threads, in the given order;
+// the first failure (lowest i) is rethrown after every thread has joined.
+template <typename F>
+void parallel_for(const std::vector<size_t> &order, uint32_t threads, F &&f) {
+  const size_t n = order.size();
+  std::vector<std::exception_ptr> errors(n);
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t k = next++; k < n; k = next++) {
+      try {
+        f(order[k]);
+      } catch (...) {
+        errors[order[k]] = std::current_exception();
+      }
+    }
+  };
+  const uint32_t t = std::max<uint32_t>(1, std::min<uint32_t>(threads, (uint32_t)n));
+  if (t == 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (uint32_t i = 0; i < t; ++i) pool.emplace_back(work);
+    for (auto &th : pool) th.join();
+  }
+  for (auto &e : errors)
+    if (e) std::rethrow_exception(e);
+}
+
 extern "C" {
 
 uint32_t vckss_rust_abi_version(void) { return VCKSS_RUST_ABI_VERSION_V1; }
@@ -1963,6 +2025,143 @@ int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t
                           x.seconds_cells};
     std::memcpy(out, v, sizeof v);
   });
+}
+
+int32_t vckss_mi355x_batch_prepare_v1(uint64_t rows, const double *sample, const double *worker,
+                                      const double *firm, const double *deletion, const double *outcome,
+                                      const double *frequency, const double *target, uint32_t deletion_mode,
+                                      uint32_t threads, uint64_t *handle, uint32_t *samples) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    if (!handle || !samples) fail(kInvalidInput, "batch_prepare", "handle and sample-count outputs are required");
+    if (rows == 0) fail(kInvalidInput, "batch_prepare", "row count must be positive");
+    if (deletion_mode != VCKSS_DELETION_MATCH && deletion_mode != VCKSS_DELETION_OBSERVATION)
+      fail(kInvalidInput, "batch_prepare", "unknown deletion mode");
+    if (!outcome || !target) fail(kInvalidInput, "engine_ingest", "outcome or target column is null");
+    auto sid = integer_column(sample, rows, "sample identifier", kInvalidIdentifier, true);
+    auto w = integer_column(worker, rows, "worker identifier", kInvalidIdentifier, true);
+    auto f = integer_column(firm, rows, "firm identifier", kInvalidIdentifier, true);
+    auto d = integer_column(deletion, rows, "deletion identifier", kInvalidIdentifier, true);
+    auto fr = integer_column(frequency, rows, "frequency weight", kInvalidWeight, false);
+    // rows of each sample, in input order; samples ascend by identifier
+    std::vector<uint64_t> order(rows);
+    std::iota(order.begin(), order.end(), 0ull);
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return sid[a] < sid[b]; });
+    auto batch = std::make_unique<Batch>();
+    batch->observation = deletion_mode == VCKSS_DELETION_OBSERVATION;
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    for (uint64_t i = 0; i < rows;) {
+      uint64_t j = i;
+      while (j < rows && sid[order[j]] == sid[order[i]]) ++j;
+      ranges.push_back({i, j});
+      auto bs = std::make_unique<BatchSample>();
+      bs->id = sid[order[i]];
+      batch->samples.push_back(std::move(bs));
+      i = j;
+    }
+    std::vector<size_t> idx(ranges.size());
+    std::iota(idx.begin(), idx.end(), (size_t)0);
+    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      return ranges[a].second - ranges[a].first > ranges[b].second - ranges[b].first;
+    });
+    parallel_for(idx, std::max<uint32_t>(threads, 1), [&](size_t k) {
+      const uint64_t b = ranges[k].first, e = ranges[k].second, n = e - b;
+      std::vector<uint64_t> cw(n), cf(n), cd(n), cfr(n);
+      std::vector<double> y(n), t(n);
+      for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t r = order[b + i];
+        cw[i] = w[r]; cf[i] = f[r]; cd[i] = d[r]; cfr[i] = fr[r];
+        y[i] = outcome[r]; t[i] = target[r];
+      }
+      Columns cols{n, cw.data(), cf.data(), cd.data(), y.data(), cfr.data(), t.data()};
+      BatchSample &bs = *batch->samples[k];
+      try {
+        bs.P = prepare(cols, nullptr, batch->observation);
+        bs.HS = build_hybrid(bs.P);
+        CmgOptions copt;
+        bs.H = build_hierarchy(bs.HS.graph, copt);
+      } catch (const Error &err) {
+        fail(err.code, "batch_prepare", "sample " + std::to_string(k) + ": " + err.what());
+      }
+    });
+    for (auto &bs : batch->samples) bs->dev.reset(new DeviceEngine(bs->P, bs->HS, bs->H));
+    *samples = (uint32_t)batch->samples.size();
+    *handle = g_next_batch++;
+    g_batches[*handle] = std::move(batch);
+  });
+}
+
+int32_t vckss_mi355x_batch_solve_v1(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
+                                    double fit_tolerance, uint32_t precision, uint32_t streams) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Batch &B = live_batch(handle);
+    if (probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
+    if (precision != 32 && precision != 64) fail(kInvalidInput, "batch_solve", "precision must be 32 or 64");
+    for (double t : {tolerance, fit_tolerance})
+      if (!std::isfinite(t) || t < 1e-15 || t > 1e-4) fail(kInvalidInput, "cmg_full_v2", "tolerance must lie in [1e-15, 1e-4]");
+    SolveOptions o;
+    o.seed = seed;
+    o.probes = probes;
+    o.probe_tolerance = tolerance;
+    o.fit_tolerance = fit_tolerance;
+    o.precision = (int)precision;
+    o.leverage_width = std::min<uint32_t>(64, probes);
+    o.target_width = std::min<uint32_t>(32, probes);
+    std::vector<size_t> idx(B.samples.size());
+    std::iota(idx.begin(), idx.end(), (size_t)0);
+    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return B.samples[a]->P.rows > B.samples[b]->P.rows; });
+    for (auto &bs : B.samples) {
+      bs->solved = false;
+      bs->status = 0;
+      bs->error.clear();
+    }
+    parallel_for(idx, std::max<uint32_t>(streams, 1), [&](size_t k) {
+      BatchSample &bs = *B.samples[k];
+      const auto t0 = std::chrono::steady_clock::now();
+      try {
+        bs.result = bs.dev->run(o);
+        bs.solved = true;
+      } catch (const Error &err) {
+        bs.status = err.code;
+        bs.error = err.what();
+      }
+      bs.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+    for (size_t k = 0; k < B.samples.size(); ++k)
+      if (B.samples[k]->status)
+        fail(B.samples[k]->status, "batch_solve", "sample " + std::to_string(k) + ": " + B.samples[k]->error);
+  });
+}
+
+int32_t vckss_mi355x_batch_result_v1(uint64_t handle, uint32_t sample, double *out, uint32_t capacity) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] {
+    Batch &B = live_batch(handle);
+    if (sample >= B.samples.size()) fail(kInvalidInput, "batch_result", "sample index out of range");
+    if (!out || capacity < 24) fail(kAbiMismatch, "batch_result", "capacity must be at least 24");
+    const BatchSample &bs = *B.samples[sample];
+    if (!bs.solved) fail(bs.status ? bs.status : kStaleContext, "batch_result", "the sample has no solved result");
+    const SolveResult &r = bs.result;
+    for (int j = 0; j < 4; ++j) {
+      out[j] = r.plugin[j];
+      out[4 + j] = r.correction[j];
+      out[8 + j] = r.corrected[j];
+      out[12 + j] = r.mcse[j];
+    }
+    uint64_t it = 0;
+    for (const auto *set : {&r.leverage, &r.target})
+      for (const ColumnReceipt &c : *set) it += c.iterations;
+    // identifiers are held order-encoded ((i64) ^ 2^63), as the column ids are
+    const double extra[8] = {(double)(int64_t)(bs.id ^ (1ULL << 63)), (double)bs.P.rows, (double)bs.P.workers, (double)bs.P.firms,
+                             (double)bs.P.units, bs.seconds, (double)it, r.max_complete_residual};
+    std::memcpy(out + 16, extra, sizeof extra);
+  });
+}
+
+int32_t vckss_mi355x_batch_release_v1(uint64_t handle) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return guarded([&] { g_batches.erase(handle); });
 }
 
 int32_t vckss_mi355x_controls_v1(uint64_t generation, double *out, uint32_t capacity) {

@@ -171,3 +171,28 @@ def eligible_stayers(worker, firm, outcome, frequency, target, mover_mask):
     worker_index = np.concatenate([np.full(r.size, k + 1.0) for k, r in enumerate(rows)])
     return (firm_index, worker_index, np.asarray(outcome, float)[rows_all], frequency[rows_all],
             np.asarray(target, float)[rows_all])
+
+
+def subsamples(count: int = 64, median: int = 40_000, low: int = 5_000, high: int = 500_000, periods: int = 4,
+               sigma: float = 1.0, seed: int = 11, scale: float = 1.0):
+    """Many independent region-year subsamples of uneven size: worker counts
+    log-normal with the given median and log-sd `sigma`, clipped to [low,
+    high] (all three times `scale`), firms = workers / 20, T = `periods`,
+    mobility U(0.05, 0.25) per subsample, the outcome DGP of `generate` with
+    uniform firm sizes.  Returns (columns dict with a `sample` column, list
+    of the subsamples' PanelSpecs); subsample k's rows carry sample id k."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(np.log(median) + sigma * rng.standard_normal(count)), low, high) * scale
+    mobility = rng.uniform(0.05, 0.25, count)
+    specs, parts = [], []
+    for k in range(count):
+        w = max(50, int(sizes[k]))
+        spec = PanelSpec(workers=w, firms=max(10, w // 20), periods=periods, mobility=float(mobility[k]),
+                         seed=seed * 1000 + k, name=f"subsample-{k}")
+        d = generate(spec)
+        d = {c: np.asarray(d[c]) for c in ("worker", "firm", "deletion", "outcome", "frequency", "target")}
+        d["sample"] = np.full(d["worker"].size, k, np.uint64)
+        specs.append(spec)
+        parts.append(d)
+    cols = {c: np.concatenate([p[c] for p in parts]) for c in parts[0]}
+    return cols, specs

@@ -180,6 +180,31 @@ int32_t vckss_mi355x_graphs_v1(uint64_t generation, uint32_t enable, uint64_t *c
  * factorisation and fit, seconds for the cell pass.  11 if the result is a
  * JLA estimate. */
 int32_t vckss_mi355x_exact_receipt_v1(uint64_t generation, double *out, uint32_t capacity);
+/* Sample batches (no counterpart in the reference, whose caller runs one
+ * sample per command; for callers with many independent samples, such as
+ * region-year subsamples).  Rows carry a sample identifier (any exact binary64
+ * integer); each sample is prepared as its own problem (the graph selection,
+ * compression, plan and CMG hierarchy of vckss_rust_engine_prepare_v1, match
+ * or observation deletion), on `threads` host threads.  *samples receives the
+ * sample count; samples are numbered by ascending identifier.  A sample that
+ * fails preparation fails the batch with its status. */
+int32_t vckss_mi355x_batch_prepare_v1(uint64_t rows, const double *sample, const double *worker,
+                                      const double *firm, const double *deletion, const double *outcome,
+                                      const double *frequency, const double *target, uint32_t deletion_mode,
+                                      uint32_t threads, uint64_t *handle, uint32_t *samples);
+/* The JLA estimate of every sample (Counter-V1 seed, probes, probe and fit
+ * tolerances, V-cycle precision 32 or 64; batches of up to 64 leverage and
+ * 32 target probes), up to `streams` samples at once, each on its own HIP
+ * stream; largest samples first.  Returns the status of the lowest-numbered
+ * failing sample, after all have run. */
+int32_t vckss_mi355x_batch_solve_v1(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
+                                    double fit_tolerance, uint32_t precision, uint32_t streams);
+/* out[0..16): plug-in, correction, corrected, MCSE (worker, firm, covariance,
+ * total each); out[16..24): sample identifier, stored rows, workers, firms,
+ * deletion units, solve seconds, PCG iterations, largest complete residual.
+ * capacity >= 24. */
+int32_t vckss_mi355x_batch_result_v1(uint64_t handle, uint32_t sample, double *out, uint32_t capacity);
+int32_t vckss_mi355x_batch_release_v1(uint64_t handle);
 /* The control fit of the last solve with controls (no counterpart in the
  * reference's C ABI; its generic-engine receipt holds the same quantities):
  * out[0] the control count q, out[1] 1 under the fixed-offset nuisance else
