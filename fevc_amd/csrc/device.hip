@@ -58,7 +58,7 @@ struct Cycle {
     const uint64_t work = (uint64_t)L.n * K;
     if (L.terminal) {
       if (L.terminal == kDirect)
-        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, E.ginv.p,
+        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(dev::terminal_threads<K>(L.n)), kBlock, 0, E.stream, L.n, E.ginv.p,
                            braw, mean, b, v.x.p, E.gate);
       else
         hipLaunchKernelGGL((dev::k_terminal_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L), braw,

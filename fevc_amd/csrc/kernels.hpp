@@ -585,21 +585,92 @@ struct OpAggregate {
 };
 
 // Direct terminal: b = braw - mean, x = G b with G the grounded inverse
-// (ldl.rs:892-997).
+// (ldl.rs:892-997).  G is dense (n <= the direct threshold, 700 by default),
+// so this is a small dense product of an n x n matrix with n x K lanes.  A
+// block owns a tile of Terminal<K>::kRows rows for all K lanes and walks the
+// inner index in chunks of kChunk: the chunk's centred right-hand side
+// (kChunk x K) and the tile's slice of G (rows x kChunk) are loaded with
+// coalesced reads into registers one chunk ahead, staged through LDS, and
+// every thread accumulates its outputs over the chunk.  Each output is still
+// the serial sum over k = 0 .. n - 1 in increasing order, so the result is
+// that of a plain row-by-row dot product bit for bit.
+template <int K>
+struct Terminal {
+  static constexpr int kGroups = kBlock / K;            // row groups of K lanes
+  static constexpr int kPerThread = K >= 32 ? 2 : 1;    // rows per thread
+  static constexpr int kRows = kGroups * kPerThread;    // rows per tile
+  static constexpr int kChunk = kRows >= 128 ? 16 : 64; // inner indices per stage
+  static constexpr int kLoadB = (kChunk * K + kBlock - 1) / kBlock;
+  static constexpr int kLoadG = (kRows * kChunk + kBlock - 1) / kBlock;
+};
+template <int K>
+inline uint64_t terminal_threads(uint32_t n) {
+  return (uint64_t)((n + Terminal<K>::kRows - 1) / Terminal<K>::kRows) * kBlock;
+}
+
 template <typename T, int K>
-__global__ void k_terminal_direct(uint32_t n, const double *__restrict__ g, const T *__restrict__ braw,
-                                  const double *__restrict__ mean, T *__restrict__ b, T *__restrict__ x,
-                                  const int *__restrict__ gate) {
+__global__ void __launch_bounds__(kBlock) k_terminal_direct(uint32_t n, const double *__restrict__ g,
+                                                            const T *__restrict__ braw,
+                                                            const double *__restrict__ mean, T *__restrict__ b,
+                                                            T *__restrict__ x, const int *__restrict__ gate) {
   if (gate && *gate == 0) return;
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = t / K;
-    const int j = t % K;
+  using TT = Terminal<K>;
+  constexpr int R = TT::kRows, KC = TT::kChunk, NB = TT::kLoadB, NG = TT::kLoadG, RG = TT::kGroups;
+  __shared__ double bt[KC][K];
+  __shared__ double gt[R][KC + 1];
+  const int tid = threadIdx.x, j = tid % K, row = tid / K;
+  for (uint32_t i0 = blockIdx.x * (uint32_t)R; i0 < n; i0 += gridDim.x * (uint32_t)R) {
+    double rb[NB], rg[NG];
+    auto load = [&](uint32_t k0) {
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {
+        const int idx = tid + e * kBlock, kk = idx / K, jj = idx % K;
+        rb[e] = 0.0;
+        if (idx < KC * K && k0 + kk < n)
+          rb[e] = (double)braw[(uint64_t)(k0 + kk) * K + jj] - (mean ? mean[jj] : 0.0);
+      }
+#pragma unroll
+      for (int e = 0; e < NG; ++e) {
+        const int idx = tid + e * kBlock, rr = idx / KC, kk = idx % KC;
+        rg[e] = 0.0;
+        if (idx < R * KC && i0 + rr < n && k0 + kk < n) rg[e] = g[(uint64_t)(i0 + rr) * n + k0 + kk];
+      }
+    };
+    double s[TT::kPerThread];
+#pragma unroll
+    for (int q = 0; q < TT::kPerThread; ++q) s[q] = 0.0;
+    load(0);
+    for (uint32_t k0 = 0; k0 < n; k0 += KC) {
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {
+        const int idx = tid + e * kBlock;
+        if (idx < KC * K) bt[idx / K][idx % K] = rb[e];
+      }
+#pragma unroll
+      for (int e = 0; e < NG; ++e) {
+        const int idx = tid + e * kBlock;
+        if (idx < R * KC) gt[idx / KC][idx % KC] = rg[e];
+      }
+      __syncthreads();
+      if (k0 + KC < n) load(k0 + KC);  // the next chunk's loads fly during this chunk's products
+      const uint32_t kc = n - k0 < (uint32_t)KC ? n - k0 : (uint32_t)KC;
+      for (uint32_t kk = 0; kk < kc; ++kk) {
+        const double bv = bt[kk][j];
+#pragma unroll
+        for (int q = 0; q < TT::kPerThread; ++q) s[q] += gt[row + RG * q][kk] * bv;
+      }
+      __syncthreads();
+    }
     const double mu = mean ? mean[j] : 0.0;
-    double s = 0.0;
-    for (uint32_t k = 0; k < n; ++k) s += g[(uint64_t)i * n + k] * ((double)braw[(uint64_t)k * K + j] - mu);
-    x[t] = (T)s;
-    b[t] = (T)((double)braw[t] - mu);
+#pragma unroll
+    for (int q = 0; q < TT::kPerThread; ++q) {
+      const uint32_t i = i0 + row + RG * q;
+      if (i < n) {
+        const uint64_t t = (uint64_t)i * K + j;
+        x[t] = (T)s[q];
+        b[t] = (T)((double)braw[t] - mu);
+      }
+    }
   }
 }
 

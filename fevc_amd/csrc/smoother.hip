@@ -256,7 +256,7 @@ const double *train_forward(Impl &E, size_t l, const double *braw, bool learned)
   const double *mean = l ? T.mean.p : nullptr;
   if (D.terminal) {
     if (D.terminal == kDirect)
-      hipLaunchKernelGGL((dev::k_terminal_direct<double, KT>), grid_for(m), kBlock, 0, E.stream, n, E.ginv.p, braw,
+      hipLaunchKernelGGL((dev::k_terminal_direct<double, KT>), grid_for(dev::terminal_threads<KT>(n)), kBlock, 0, E.stream, n, E.ginv.p, braw,
                          mean, T.b.p, T.y.p, (const int *)nullptr);
     else
       hipLaunchKernelGGL((dev::k_terminal_scale<double, KT>), grid_for(m), kBlock, 0, E.stream, n, D.inv.p, braw,
