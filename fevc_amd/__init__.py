@@ -306,6 +306,8 @@ SIGNATURES = [
      [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("vckss_mi355x_batch_solve_v1", C.c_int32, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double,
                                                 C.c_uint32, C.c_uint32]),
+    ("vckss_mi355x_batch_solve_v2", C.c_int32, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double,
+                                                 C.c_uint32, C.c_uint32, C.c_uint32]),
     ("vckss_mi355x_batch_result_v1", C.c_int32, [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("vckss_mi355x_batch_release_v1", C.c_int32, [C.c_uint64]),
     ("vckss_mi355x_graphs_v1", C.c_int32,
@@ -980,9 +982,13 @@ class SampleBatch:
             threads, C.byref(h), C.byref(n)))
         self.handle, self.samples = h.value, n.value
 
-    def solve(self, seed=8675309, probes=128, tolerance=1e-8, fit_tolerance=1e-8, precision=32, streams=4):
-        _check(library().vckss_mi355x_batch_solve_v1(self.handle, seed, probes, tolerance, fit_tolerance,
-                                                     precision, streams))
+    def solve(self, seed=8675309, probes=128, tolerance=1e-8, fit_tolerance=1e-8, precision=32, streams=4,
+              lockstep=False):
+        """Every sample's JLA estimate: up to `streams` samples at once, each on
+        its own stream, or with `lockstep` up to `streams` samples whose PCG
+        passes run as single launches over all of them."""
+        _check(library().vckss_mi355x_batch_solve_v2(self.handle, seed, probes, tolerance, fit_tolerance,
+                                                     precision, streams, 1 if lockstep else 0))
         return [self.result(i) for i in range(self.samples)]
 
     def result(self, i):

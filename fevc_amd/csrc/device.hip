@@ -10,6 +10,8 @@
 // a contiguous segmented sum, prolongation labels are monotone); the level-0
 // renumbering is undone at the firm/auxiliary boundary of the hybrid system.
 #include <algorithm>
+#include <condition_variable>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 
@@ -58,11 +60,11 @@ struct Cycle {
     const uint64_t work = (uint64_t)L.n * K;
     if (L.terminal) {
       if (L.terminal == kDirect)
-        hipLaunchKernelGGL((dev::k_terminal_direct<T, K>), grid_for(dev::terminal_threads<K>(L.n)), kBlock, 0, E.stream, L.n, E.ginv.p,
-                           braw, mean, b, v.x.p, E.gate);
+        run(E, dev::k_terminal_direct<T, K>, grid_for(dev::terminal_threads<K>(L.n)),
+            dev::TerminalDirect<T, K>{L.n, E.ginv.p, braw, mean, b, v.x.p, E.gate});
       else
-        hipLaunchKernelGGL((dev::k_terminal_scale<T, K>), grid_for(work), kBlock, 0, E.stream, L.n, inv(L), braw,
-                           mean, b, v.x.p, E.gate);
+        run(E, dev::k_terminal_scale<T, K>, grid_for(work),
+            dev::TerminalScale<T, K>{L.n, inv(L), braw, mean, b, v.x.p, E.gate});
       return;
     }
     // next level visited: l + 1, or with the tuned shape a deeper level
@@ -81,6 +83,7 @@ struct Cycle {
     if (kPacked && mean) fail(kInvariant, "vcycle", "the packed FP32 cycle takes uncentred right-hand sides");
     const T *bb = kPacked ? braw : b;
     for (uint32_t it = 0; it < iterations; ++it) {
+      const Nest visit(E);  // the passes of this iteration, numbered apart from other iterations'
       if (it == 0) {
         if constexpr (kPacked)
           segments<VT, KV, 0>(E, rows,
@@ -101,7 +104,10 @@ struct Cycle {
       const bool x0 = kPacked && !Lc.terminal;  // the child's first iterate, written with its braw
       segments<VT, KV, 0>(E, aggs, dev::OpAggregate<VT, KV>{vp(v.r.p), vp(c.braw.p), nullptr, Lc.n,
                                                             x0 ? inv(Lc) : nullptr, x0 ? vp(c.x.p) : nullptr});
-      level(nx, E.levels[nx - 1].repeat, nullptr);
+      {
+        const Nest child(E);
+        level(nx, E.levels[nx - 1].repeat, nullptr);
+      }
       if (l == 0) {
         // the finest prolong-smooth pass, timed with HIP events in a timed
         // run; algorithmic bytes: CSR once (values, columns, chunk
@@ -136,15 +142,20 @@ CycleOut apply_cycle_raw(Impl &E, const double *r, const double *mean_r, double 
   E.vcycles++;
   if (!fp32) {
     Cycle<double, K> cyc{E, E.lv64, r, rc};
+    const Nest cycle(E);
     cyc.level(0, 1, mean_r);
     return {E.lv64[0].x.p, nullptr};
   }
   const uint64_t m0 = E.n0 * K;
   Cycle<float, K> cyc{E, E.lv32, E.lv32[0].braw.p, E.lv32[0].b.p};
   const bool x0 = Cycle<float, K>::kPacked;  // level 0's first iterate, written with its braw
-  hipLaunchKernelGGL((dev::k_centre_split<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, r, mean_r, rc,
-                     E.lv32[0].braw.p, x0 ? cyc.inv(E.levels[0]) : nullptr, x0 ? E.lv32[0].x.p : nullptr, E.gate);
-  cyc.level(0, 1, nullptr);
+  run(E, dev::k_centre_split<K>, grid_for(m0),
+      dev::CentreSplit<K>{E.n0, r, mean_r, rc, E.lv32[0].braw.p, x0 ? cyc.inv(E.levels[0]) : nullptr,
+                          x0 ? E.lv32[0].x.p : nullptr, E.gate});
+  {
+    const Nest cycle(E);
+    cyc.level(0, 1, nullptr);
+  }
   return {nullptr, E.lv32[0].x.p};
 }
 // As above with z always in FP64 (returns its device address).
@@ -265,59 +276,68 @@ hipGraphExec_t capture(Impl &E, F &&enqueue) {
   return x;
 }
 
-// Lockstep certified PCG (pcg.rs:1537-1909) on the finest hybrid Laplacian
-// (renumbered).  b: n0 x K right-hand sides.  With `warm`, E.x holds the
-// initial guesses.  The host polls the any-lane-active flag with a lag of
-// kPollLag iterations; iterations queued after convergence return at once.
+// One lockstep certified PCG call (pcg.rs:1537-1909) for K lanes on the
+// finest hybrid Laplacian (renumbered): the setup passes, the
+// preconditioning step (lane means, V-cycle, rho) and the iteration body
+// (direction and A p, update, centring, fresh-residual gate, limit).  b:
+// n0 x K right-hand sides; E.use holds the lanes that take part; with a warm
+// start E.x holds the initial guesses.
 template <int K>
-void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
-               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0) {
-  const uint32_t n = E.n0;
-  const uint64_t m0 = (uint64_t)n * K;
-  const DevLevel &L0 = E.levels[0];
-  const DevChunks &rows0 = grouped<K>() ? L0.rows_g : L0.rows;
-  dev::PcgParams prm{n, rel, abs_tol, E.bound, (int32_t)max_it, (int32_t)interval, cap};
-  int *flags = E.flags.p;
-  const int *active = flags + dev::kFlagActive;
-  VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
-  const unsigned rg = reduce_grid(m0);
-  hipLaunchKernelGGL((dev::k_pcg_stats<K>), rg, kBlock, 0, E.stream, n, b, E.rpartial.p);
-  finish<K, 3>(E, E.rpartial.p, rg, dev::FinStats{E.st.p, E.use.p, n});
-  E.gate = E.always();
-  if (!warm) {
-    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.x.p,
-                       E.st.p);
-    hipLaunchKernelGGL((dev::k_pcg_start<K>), 1, 64, 0, E.stream, E.st.p, prm);
-  } else {
-    hipLaunchKernelGGL((dev::k_pcg_project<K>), grid_for(m0), kBlock, 0, E.stream, n, b, E.proj.p, E.r.p, E.ap.p,
-                       E.st.p);
-    hipLaunchKernelGGL((dev::k_lane_sum<K>), rg, kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.rpartial.p);
-    finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
-    hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p,
-                       E.lane_a.p, nullptr);
-    segments<double, K, 4>(E, rows0, dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
-  }
-  E.gate = active;
-  // The preconditioner's output of the latest enqueued cycle: the cycle
+struct PcgCall {
+  Impl &E;
+  const double *b;
+  dev::PcgParams prm;
+  bool fp32;
+  const DevLevel &L0;
+  const DevChunks &rows0;
+  unsigned rg;
+  int *flags;
+  const int *active;
+  // the preconditioner's output of the latest enqueued cycle: the cycle
   // alternates its level-0 buffers, so the body reads the address the last
-  // precondition step produced (with graphs, the one captured in `pre`).
+  // precondition step produced
   CycleOut zcur{};
-  auto precondition = [&]() {
-    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags,
-                       (const int *)nullptr);
-    zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
-    if (zcur.z32) {
-      hipLaunchKernelGGL((dev::k_pcg_rho<K, float>), rg, kBlock, 0, E.stream, n, E.rc.p, zcur.z32, E.rpartial.p,
-                         active);
+
+  PcgCall(Impl &e, const double *rhs, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool f32,
+          double cap)
+      : E(e), b(rhs), prm{(uint32_t)e.n0, rel, abs_tol, e.bound, (int32_t)max_it, (int32_t)interval, cap}, fp32(f32),
+        L0(e.levels[0]), rows0(grouped<K>() ? e.levels[0].rows_g : e.levels[0].rows),
+        rg(reduce_grid((uint64_t)e.n0 * K)), flags(e.flags.p), active(e.flags.p + dev::kFlagActive) {}
+
+  void setup(bool warm) {
+    const uint32_t n = E.n0;
+    const uint64_t m0 = (uint64_t)n * K;
+    run(E, dev::k_pcg_stats<K>, rg, dev::PcgStats<K>{n, b, E.rpartial.p});
+    finish<K, 3>(E, E.rpartial.p, rg, dev::FinStats{E.st.p, E.use.p, n});
+    E.gate = E.always();
+    if (!warm) {
+      run(E, dev::k_pcg_project<K>, grid_for(m0), dev::PcgProject<K>{n, b, E.proj.p, E.r.p, E.x.p, E.st.p});
+      run(E, dev::k_pcg_start<K>, 1, dev::PcgStart<K>{E.st.p, prm});
     } else {
-      hipLaunchKernelGGL((dev::k_pcg_rho<K, double>), rg, kBlock, 0, E.stream, n, E.rc.p, zcur.z64, E.rpartial.p,
-                         active);
+      run(E, dev::k_pcg_project<K>, grid_for(m0), dev::PcgProject<K>{n, b, E.proj.p, E.r.p, E.ap.p, E.st.p});
+      run(E, dev::k_lane_sum<K>, rg, dev::LaneSum<K>{(uint64_t)n, E.x.p, E.rpartial.p});
+      finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
+      run(E, dev::k_center_cols<K>, grid_for(m0), dev::CenterCols<K>{(uint64_t)n, E.x.p, E.lane_a.p, nullptr});
+      segments<double, K, 4>(E, rows0,
+                             dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
     }
+    E.gate = active;
+  }
+
+  void precondition() {
+    const uint32_t n = E.n0;
+    run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, nullptr});
+    zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
+    if (zcur.z32)
+      run(E, dev::k_pcg_rho<K, float>, rg, dev::PcgRho<K, float>{n, E.rc.p, zcur.z32, E.rpartial.p, active});
+    else
+      run(E, dev::k_pcg_rho<K, double>, rg, dev::PcgRho<K, double>{n, E.rc.p, zcur.z64, E.rpartial.p, active});
     finish<K, 3>(E, E.rpartial.p, rg, dev::FinRho{E.st.p, n}, active);
-  };
-  // iteration body: direction and A p, update, centring, fresh-residual
-  // gate, limit
-  auto body = [&]() {
+  }
+
+  void body() {
+    const uint32_t n = E.n0;
+    const uint64_t m0 = (uint64_t)n * K;
     // direction and A p, the dominant kernel of the iteration, timed with HIP
     // events in a timed run; algorithmic bytes: CSR once (values, columns,
     // chunk descriptors), z read once in the cycle's precision, p and A p
@@ -334,16 +354,71 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
                              dev::OpDirectionCurvature<K, double>{{L0.col.p, L0.val.p}, zcur.z64, E.p.p, E.ap.p,
                                                                   E.st.p},
                              ev.first, ev.second);
-    hipLaunchKernelGGL((dev::k_pcg_update<K>), rg, kBlock, 0, E.stream, n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p,
-                       E.rpartial.p, E.st.p, active);
+    run(E, dev::k_pcg_update<K>, rg,
+        dev::PcgUpdate<K>{n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p, E.rpartial.p, E.st.p, active});
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
-    hipLaunchKernelGGL((dev::k_state_means<K>), 1, 64, 0, E.stream, E.st.p, E.lane_a.p, E.lane_b.p, flags, active);
-    hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(m0), kBlock, 0, E.stream, (uint64_t)n, E.x.p, E.lane_a.p,
-                       active);
+    run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, active});
+    run(E, dev::k_center_cols<K>, grid_for(m0), dev::CenterCols<K>{(uint64_t)n, E.x.p, E.lane_a.p, active});
     segments<double, K, 3>(E, rows0,
                            dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags});
-    hipLaunchKernelGGL((dev::k_pcg_limit<K>), 1, 64, 0, E.stream, E.st.p, prm, flags);
-  };
+    run(E, dev::k_pcg_limit<K>, 1, dev::PcgLimit<K>{E.st.p, prm, flags});
+  }
+};
+
+// The calls of a sample batch in lockstep: record this call's passes (the
+// setup, then two preconditioning steps and bodies, one for each parity of
+// the cycle's alternating buffers: two cycles swap every level's buffers an
+// even number of times) and run them with the other samples' waiting calls.
+template <int K>
+void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vector<dev::ColState> &out) {
+  PcgProgram prog;
+  Recorder rec;
+  const uint64_t cycles = E.vcycles;
+  E.rec = &rec;
+  try {
+    c.setup(warm);
+    prog.setup = rec.take();
+    for (int parity = 0; parity < 2; ++parity) {
+      c.precondition();
+      prog.pre[parity] = rec.take();
+      c.body();
+      prog.body[parity] = rec.take();
+    }
+  } catch (...) {
+    E.rec = nullptr;
+    throw;
+  }
+  E.rec = nullptr;
+  E.vcycles = cycles;
+  prog.stream = E.stream;
+  prog.active = c.active;
+  prog.st = E.st.p;
+  prog.lanes = K;
+  prog.max_iterations = max_it;
+  lockstep_submit(*E.lockstep, prog);
+  E.vcycles += prog.cycles;
+  E.iterations_launched += prog.iterations;
+  E.gate = E.always();
+  out = prog.out;
+}
+
+// Lockstep certified PCG on the finest hybrid Laplacian: with `warm`, E.x
+// holds the initial guesses.  Alone, the host polls the any-lane-active flag
+// with a lag of kPollLag iterations and iterations queued after convergence
+// return at once; in a lockstep sample batch the call joins the others'.
+template <int K>
+void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
+               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0) {
+  VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
+  PcgCall<K> c(E, b, rel, abs_tol, max_it, interval, fp32, cap);
+  if (E.lockstep) {
+    pcg_lockstep<K>(E, c, warm, max_it, out);
+    return;
+  }
+  c.setup(warm);
+  const dev::PcgParams &prm = c.prm;
+  auto precondition = [&]() { c.precondition(); };
+  auto body = [&]() { c.body(); };
   // With graphs, each iteration launches the two captured sequences; the flag
   // copy and the host poll stay between them, as without graphs.  Timed runs
   // keep individual launches (their per-launch events).
@@ -374,6 +449,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
       precondition();
     }
   };
+  const int *active = c.active;
   precondition_step();
   for (uint32_t it = 1; it <= max_it; ++it) {
     E.iterations_launched++;
@@ -408,17 +484,14 @@ void extract(Impl &E) {
   const uint64_t F = E.F, W = E.W;
   hipLaunchKernelGGL((dev::k_gather_rows<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.pos0.p, E.x.p, E.fsol.p);
   const unsigned fg = reduce_grid(F * K), wg = reduce_grid(W * dev::Pack<K>::TPR);
-  hipLaunchKernelGGL((dev::k_lane_sum<K>), fg, kBlock, 0, E.stream, F, E.fsol.p, E.rpartial.p);
+  run(E, dev::k_lane_sum<K>, fg, dev::LaneSum<K>{(uint64_t)(F), E.fsol.p, E.rpartial.p});
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(E.n0 * K), kBlock, 0, E.stream, E.n0, E.x.p, E.lane_a.p,
-                     nullptr);
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.fsol.p, E.lane_a.p,
-                     nullptr);
+  run(E, dev::k_center_cols<K>, grid_for(E.n0 * K), dev::CenterCols<K>{(uint64_t)(E.n0), E.x.p, E.lane_a.p, nullptr});
+  run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.fsol.p, E.lane_a.p, nullptr});
   VK_HIP(hipMemcpyAsync(E.red.p, E.fsol.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
-  hipLaunchKernelGGL((dev::k_lane_sum<K>), fg, kBlock, 0, E.stream, F, E.red.p, E.rpartial.p);
+  run(E, dev::k_lane_sum<K>, fg, dev::LaneSum<K>{(uint64_t)(F), E.red.p, E.rpartial.p});
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.red.p, E.lane_a.p,
-                     nullptr);
+  run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.red.p, E.lane_a.p, nullptr});
   hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * dev::Pack<K>::TPR), kBlock, 0, E.stream, W, E.worker_ptr.p,
                      E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
   segments<double, K, 1>(E, E.firm_chunks,
@@ -447,7 +520,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   segments<double, K, 1>(E, E.firm_chunks,
                          dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.wdiag.p, E.wrhs.p,
                                             E.frhs.p, E.sch.p, E.lane_a.p, (uint32_t)F});
-  hipLaunchKernelGGL((dev::k_center_cols<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.sch.p, E.lane_a.p, nullptr);
+  run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.sch.p, E.lane_a.p, nullptr});
   hipLaunchKernelGGL((dev::k_hybrid_rhs<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, (uint32_t)F, E.perm0.p,
                      E.sch.p, E.b0.p);
   std::vector<uint8_t> use(K, 0);
@@ -1051,6 +1124,229 @@ bool device_available() {
   int devices = 0;
   return hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
 }
+
+// ------------------------------------------------------------ lockstep batches
+// The PCG calls of a sample batch that wait at the same time run together on
+// one stream: the recorded passes of all calls are merged in key order, and
+// passes with the same key and functor type become one k_multi launch over
+// their samples (each sample keeps its own grid, buffers, reductions and
+// convergence gate, so its results are those of running alone, bit for bit).
+// The host polls the OR of the samples' any-lane-active flags with the same
+// lag as a single engine.
+class Lockstep {
+ public:
+  Lockstep() {
+    VK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    VK_HIP(hipHostMalloc((void **)&host_flags_, kPollSlots * sizeof(int), hipHostMallocDefault));
+    for (auto &e : poll_) VK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    VK_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    any_.alloc(1);
+  }
+  ~Lockstep() {
+    (void)hipStreamSynchronize(stream_);
+    for (auto &e : poll_) (void)hipEventDestroy(e);
+    for (auto &e : ready_) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(done_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (host_flags_) (void)hipHostFree(host_flags_);
+    (void)hipStreamDestroy(stream_);
+  }
+  void join() {
+    std::lock_guard<std::mutex> lk(m_);
+    ++running_;
+  }
+  void leave() {
+    std::unique_lock<std::mutex> lk(m_);
+    --running_;
+    lead(lk);
+  }
+  void submit(PcgProgram &p) {
+    std::unique_lock<std::mutex> lk(m_);
+    waiting_.push_back(&p);
+    lead(lk);
+    cv_.wait(lk, [&] { return p.done; });
+    if (p.error) std::rethrow_exception(p.error);
+  }
+
+ private:
+  struct Launch {
+    MultiLaunch fn;
+    uint32_t count, blocks;
+    size_t fs, ends;
+  };
+  using Plan = std::vector<Launch>;
+
+  // Runs the waiting calls once every joined thread waits (the caller's
+  // thread leads; the others sleep until their call is done).
+  void lead(std::unique_lock<std::mutex> &lk) {
+    while (!busy_ && !waiting_.empty() && waiting_.size() >= running_) {
+      busy_ = true;
+      std::vector<PcgProgram *> batch;
+      batch.swap(waiting_);
+      lk.unlock();
+      std::exception_ptr err;
+      try {
+        execute(batch);
+      } catch (...) {
+        err = std::current_exception();
+      }
+      lk.lock();
+      busy_ = false;
+      for (PcgProgram *p : batch) {
+        if (err && !p->error) p->error = err;
+        p->done = true;
+      }
+      cv_.notify_all();
+    }
+  }
+
+  static void align(std::vector<uint8_t> &t, size_t a) { t.resize((t.size() + a - 1) / a * a, 0); }
+
+  // Merge one recorded segment of every call into launches; their functor
+  // arrays and block ends are appended to `table`.
+  static Plan merge(const std::vector<const std::vector<Record> *> &segs, std::vector<uint8_t> &table) {
+    struct Item {
+      const Record *r;
+      uint32_t sample;
+    };
+    std::vector<Item> items;
+    for (uint32_t p = 0; p < segs.size(); ++p)
+      for (const Record &r : *segs[p]) items.push_back({&r, p});
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+      if (a.r->key != b.r->key) return a.r->key < b.r->key;
+      return reinterpret_cast<uintptr_t>(a.r->launch) < reinterpret_cast<uintptr_t>(b.r->launch);
+    });
+    Plan plan;
+    for (size_t i = 0; i < items.size();) {
+      size_t j = i;
+      while (j < items.size() && items[j].r->key == items[i].r->key && items[j].r->launch == items[i].r->launch) ++j;
+      for (size_t c = i; c < j; c += dev::kMultiMax) {
+        const size_t e = std::min(j, c + (size_t)dev::kMultiMax);
+        Launch L{items[c].r->launch, (uint32_t)(e - c), 0, 0, 0};
+        align(table, 256);
+        L.fs = table.size();
+        for (size_t k = c; k < e; ++k) {
+          const Record &r = *items[k].r;
+          if (r.threads != items[c].r->threads || r.size != items[c].r->size)
+            fail(kInvariant, "lockstep", "merged passes differ in functor layout");
+          table.insert(table.end(), r.bytes.begin(), r.bytes.end());
+        }
+        align(table, 16);
+        L.ends = table.size();
+        uint32_t end = 0;
+        for (size_t k = c; k < e; ++k) {
+          end += items[k].r->grid;
+          const uint8_t *b = reinterpret_cast<const uint8_t *>(&end);
+          table.insert(table.end(), b, b + sizeof end);
+        }
+        L.blocks = end;
+        plan.push_back(L);
+      }
+      i = j;
+    }
+    return plan;
+  }
+
+  void launch(const Plan &plan) {
+    for (const Launch &L : plan)
+      L.fn(table_.p + L.fs, reinterpret_cast<const uint32_t *>(table_.p + L.ends), L.count, L.blocks, stream_);
+  }
+
+  void execute(std::vector<PcgProgram *> &batch) {
+    const uint32_t P = (uint32_t)batch.size();
+    std::vector<uint8_t> table;
+    auto segments = [&](auto pick) {
+      std::vector<const std::vector<Record> *> v;
+      for (PcgProgram *p : batch) v.push_back(&pick(*p));
+      return merge(v, table);
+    };
+    const Plan setup = segments([](PcgProgram &p) -> const std::vector<Record> & { return p.setup; });
+    Plan pre[2], body[2];
+    for (int q = 0; q < 2; ++q) {
+      pre[q] = segments([q](PcgProgram &p) -> const std::vector<Record> & { return p.pre[q]; });
+      body[q] = segments([q](PcgProgram &p) -> const std::vector<Record> & { return p.body[q]; });
+    }
+    align(table, 16);
+    const size_t flags_at = table.size();
+    for (PcgProgram *p : batch) {
+      const uint8_t *b = reinterpret_cast<const uint8_t *>(&p->active);
+      table.insert(table.end(), b, b + sizeof p->active);
+    }
+    if (table.size() > pinned_bytes_) {
+      if (pinned_) VK_HIP(hipHostFree(pinned_));
+      pinned_ = nullptr;
+      pinned_bytes_ = 0;
+      VK_HIP(hipHostMalloc((void **)&pinned_, table.size(), hipHostMallocDefault));
+      pinned_bytes_ = table.size();
+    }
+    std::memcpy(pinned_, table.data(), table.size());
+    table_.alloc(table.size());
+    // after the work each engine queued before its call
+    while (ready_.size() < P) {
+      hipEvent_t e = nullptr;
+      VK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ready_.push_back(e);
+    }
+    for (uint32_t i = 0; i < P; ++i) {
+      VK_HIP(hipEventRecord(ready_[i], batch[i]->stream));
+      VK_HIP(hipStreamWaitEvent(stream_, ready_[i], 0));
+    }
+    VK_HIP(hipMemcpyAsync(table_.p, pinned_, table.size(), hipMemcpyHostToDevice, stream_));
+    const int *const *flags = reinterpret_cast<const int *const *>(table_.p + flags_at);
+    uint32_t max_it = 0;
+    for (PcgProgram *p : batch) max_it = std::max(max_it, p->max_iterations);
+    launch(setup);
+    launch(pre[0]);
+    uint64_t cycles = 1, iterations = 0;
+    for (uint32_t it = 1; it <= max_it; ++it) {
+      ++iterations;
+      launch(body[(it - 1) % 2]);
+      hipLaunchKernelGGL(dev::k_any_active, 1, kBlock, 0, stream_, flags, P, any_.p);
+      const int slot = it % kPollSlots;
+      VK_HIP(hipMemcpyAsync(host_flags_ + slot, any_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+      VK_HIP(hipEventRecord(poll_[slot], stream_));
+      if (it > (uint32_t)kPollLag) {
+        const int seen = (it - kPollLag) % kPollSlots;
+        VK_HIP(hipEventSynchronize(poll_[seen]));
+        if (host_flags_[seen] == 0) break;
+      }
+      launch(pre[it % 2]);
+      ++cycles;
+    }
+    for (PcgProgram *p : batch) {
+      p->out.assign(p->lanes, dev::ColState{});
+      VK_HIP(hipMemcpyAsync(p->out.data(), p->st, p->lanes * sizeof(dev::ColState), hipMemcpyDeviceToHost,
+                            stream_));
+      p->iterations = iterations;
+      p->cycles = cycles;
+    }
+    VK_HIP(hipEventRecord(done_, stream_));
+    for (PcgProgram *p : batch) VK_HIP(hipStreamWaitEvent(p->stream, done_, 0));
+    VK_HIP(hipStreamSynchronize(stream_));
+  }
+
+  std::mutex m_;
+  std::condition_variable cv_;
+  uint32_t running_ = 0;
+  std::vector<PcgProgram *> waiting_;
+  bool busy_ = false;
+  hipStream_t stream_ = nullptr;
+  int *host_flags_ = nullptr;
+  hipEvent_t poll_[kPollSlots] = {};
+  hipEvent_t done_ = nullptr;
+  std::vector<hipEvent_t> ready_;
+  uint8_t *pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
+  DBuf<uint8_t> table_;
+  DBuf<int> any_;
+};
+
+std::shared_ptr<Lockstep> make_lockstep() { return std::make_shared<Lockstep>(); }
+void lockstep_join(Lockstep &l) { l.join(); }
+void lockstep_leave(Lockstep &l) { l.leave(); }
+namespace gpu {
+void lockstep_submit(Lockstep &l, PcgProgram &prog) { l.submit(prog); }
+}  // namespace gpu
 
 DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hierarchy &h) : impl_(new Impl) {
   Impl &E = *impl_;
@@ -1831,6 +2127,7 @@ DeviceEngine::Tuning DeviceEngine::autotune(const SolveOptions &base, uint32_t b
 void DeviceEngine::set_chunk(uint32_t chunk) { relayout(*impl_, chunk); }
 
 void DeviceEngine::set_first_pass_cap(bool enable) { impl_->first_pass_cap = enable; }
+void DeviceEngine::set_lockstep(Lockstep *l) { impl_->lockstep = l; }
 bool DeviceEngine::first_pass_cap() const { return impl_->first_pass_cap; }
 
 void DeviceEngine::set_graphs(bool enable) {

@@ -111,6 +111,16 @@ struct ControlFit {
 
 bool device_available();
 
+// A sample batch whose engines run their PCG calls in lockstep: the calls
+// waiting at the same time run together, each pass one launch over all their
+// samples, every sample with its own convergence gate.  Each sample thread
+// joins before its work and leaves after it (also on failure); the batch runs
+// the waiting calls whenever every joined thread is waiting.
+class Lockstep;
+std::shared_ptr<Lockstep> make_lockstep();
+void lockstep_join(Lockstep &l);
+void lockstep_leave(Lockstep &l);
+
 // Live and peak bytes of the engine's device buffers in this process.
 void device_memory(uint64_t &live, uint64_t &peak);
 
@@ -167,6 +177,8 @@ class DeviceEngine {
   // ladder as in full_cmg.rs:970-1088.
   void set_first_pass_cap(bool enable);
   bool first_pass_cap() const;
+  // Run this engine's PCG calls in the lockstep batch `l` (null: alone).
+  void set_lockstep(Lockstep *l);
   // HIP-graph capture of the PCG iteration (two graphs per configuration,
   // cached); returns the captures and graph launches so far.
   void set_graphs(bool enable);

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <map>
 #include <chrono>
+#include <exception>
 #include <cstdio>
 #include <string>
 #include <type_traits>
@@ -193,6 +194,76 @@ inline std::vector<double> from_renumbered(const std::vector<double> &in, const 
   return out;
 }
 
+// Recorded passes for multi-sample launches.  While an engine records (its
+// `rec` is set), every pass it would launch is kept instead: the functor's
+// bytes, its grid, and the launcher of k_multi for the functor's type.  The
+// key is the pass's position in a nested numbering (a segment operation, a
+// V-cycle visit of a level and each of its iterations open a nested level),
+// so the passes of several samples whose cycles differ in depth or repeats
+// still line up: equal keys and types across samples become one k_multi
+// launch, and merging in key order keeps every sample's own order.
+using MultiLaunch = void (*)(const void *table, const uint32_t *ends, uint32_t count, uint32_t blocks,
+                             hipStream_t stream);
+template <class F>
+void multi_launch(const void *table, const uint32_t *ends, uint32_t count, uint32_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((dev::k_multi<F>), blocks, F::kThreads, 0, stream, static_cast<const F *>(table), ends, count);
+}
+struct Record {
+  std::vector<uint32_t> key;
+  MultiLaunch launch = nullptr;  // also the functor type's identity
+  uint32_t grid = 0, threads = 0, size = 0, align = 0;
+  std::vector<uint8_t> bytes;
+};
+struct Recorder {
+  std::vector<Record> records;
+  std::vector<uint32_t> path{0};
+  template <class F>
+  void add(const F &f, uint64_t grid) {
+    static_assert(std::is_trivially_copyable_v<F>, "recorded functors are copied as bytes");
+    Record r;
+    r.key = path;
+    ++path.back();
+    r.launch = &multi_launch<F>;
+    r.grid = static_cast<uint32_t>(grid);
+    r.threads = F::kThreads;
+    r.size = sizeof(F);
+    r.align = alignof(F);
+    r.bytes.assign(reinterpret_cast<const uint8_t *>(&f), reinterpret_cast<const uint8_t *>(&f) + sizeof(F));
+    records.push_back(std::move(r));
+  }
+  void enter() { path.push_back(0); }
+  void leave() {
+    path.pop_back();
+    ++path.back();
+  }
+  void at(uint32_t i) { path.back() = i; }
+  std::vector<Record> take() {
+    std::vector<Record> out = std::move(records);
+    records.clear();
+    path.assign(1, 0);
+    return out;
+  }
+};
+
+// One engine's PCG call in a lockstep sample batch: its recorded setup, and
+// the preconditioning step and iteration body of both cycle parities; the
+// driver runs the calls waiting together (iteration it runs body[(it - 1) %
+// 2], then pre[it % 2]) until no sample has an active lane, and returns each
+// call's lane states and launch counts.
+struct PcgProgram {
+  std::vector<Record> setup, pre[2], body[2];
+  hipStream_t stream = nullptr;    // the engine's stream (ordered before and after)
+  const int *active = nullptr;     // the engine's any-lane-active flag
+  const dev::ColState *st = nullptr;
+  uint32_t lanes = 0, max_iterations = 0;
+  std::vector<dev::ColState> out;
+  uint64_t iterations = 0, cycles = 0;
+  std::exception_ptr error;
+  bool done = false;
+};
+// Blocks until the batch has run `prog` (rethrows its failure).
+void lockstep_submit(Lockstep &l, PcgProgram &prog);
+
 // Learned smoother state (kernels_smoother.hpp): the network's inputs and
 // activations for every smoothed level, parameters with Adam moments, and the
 // training V-cycle vectors (32 error columns).
@@ -315,7 +386,7 @@ struct DeviceEngine::Impl {
   };
   TimedKernel timed[2];
   std::pair<hipEvent_t, hipEvent_t> timed_pair(int which, double bytes) {
-    if (!timing) return {nullptr, nullptr};
+    if (!timing || rec) return {nullptr, nullptr};
     hipEvent_t a = nullptr, e = nullptr;
     VK_HIP(hipEventCreate(&a));
     VK_HIP(hipEventCreate(&e));
@@ -353,6 +424,11 @@ struct DeviceEngine::Impl {
     }
     pcg_graphs.clear();
   }
+
+  // recording of passes for a multi-sample launch (null: launch directly),
+  // and the sample batch whose PCG calls run in lockstep (null: alone)
+  Recorder *rec = nullptr;
+  Lockstep *lockstep = nullptr;
 
   const int *always() const { return flags.p + dev::kFlagOne; }
 
@@ -461,27 +537,59 @@ double component_identity(const double c[4]);
 
 namespace gpu {
 
+// Launch functor f of kernel `kernel` on `grid` blocks on the engine's stream,
+// or record it while the engine records.
+template <class F>
+void run(Impl &E, void (*kernel)(F), uint64_t grid, const F &f) {
+  if (E.rec) {
+    E.rec->add(f, grid);
+    return;
+  }
+  hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(F::kThreads), 0, E.stream, f);
+}
+// A nested level of the recorded numbering (no-op when not recording).
+struct Nest {
+  Recorder *r;
+  explicit Nest(Impl &E) : r(E.rec) {
+    if (r) r->enter();
+  }
+  ~Nest() {
+    if (r) r->leave();
+  }
+  void at(uint32_t i) const {
+    if (r) r->at(i);
+  }
+};
+
 // Final per-lane reduction of `items` block partials (one block per lane).
 template <int K, int NV, class Fin>
 void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, const int *gate = nullptr) {
-  hipLaunchKernelGGL((dev::k_finish<K, NV, Fin>), K, kBlock, 0, E.stream, partial, items, fin, gate);
+  run(E, dev::k_finish<K, NV, Fin>, K, dev::Finish<K, NV, Fin>{partial, items, fin, gate});
 }
 
 // Launch a segment operation over `ch` with operator `op`: pass 1 over all
-// chunks, pass 2 for multi-chunk segments, then the final reduction.
+// chunks, pass 2 for multi-chunk segments, then the final reduction (in a
+// recording, positions 0, 1 and 2 of a nested level whether or not pass 2
+// runs, so samples with and without multi-chunk segments line up).
 template <typename T, int K, int NV, class Op>
 void segments_in(Impl &E, const DevChunks &ch, const Op &op, const dev::SegScratch &sc,
                  hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
   constexpr int G = dev::Shape<K>::G;
   const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
   const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
+  const Nest nest(E);
   if (before) VK_HIP(hipEventRecord(before, E.stream));
-  hipLaunchKernelGGL((dev::k_segments<T, K, NV, Op>), grid1, kBlock, 0, E.stream, ch.view(), op, sc);
+  run(E, dev::k_segments<T, K, NV, Op>, grid1, dev::SegPass<T, K, NV, Op>{ch.view(), op, sc});
   if (after) VK_HIP(hipEventRecord(after, E.stream));
-  if (ch.n_multi)
-    hipLaunchKernelGGL((dev::k_segments_combine<T, K, NV, Op>), grid2, kBlock, 0, E.stream, ch.view(), op, sc,
-                       NV > 0 ? grid1 : 0u);
-  if constexpr (NV > 0) finish<K, NV>(E, sc.bpartial, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
+  if (ch.n_multi) {
+    nest.at(1);
+    run(E, dev::k_segments_combine<T, K, NV, Op>, grid2,
+        dev::SegCombine<T, K, NV, Op>{ch.view(), op, sc, NV > 0 ? grid1 : 0u});
+  }
+  if constexpr (NV > 0) {
+    nest.at(2);
+    finish<K, NV>(E, sc.bpartial, grid1 + (ch.n_multi ? grid2 : 0u), op, E.gate);
+  }
 }
 
 // The same with the engine's own scratch (chunk partials and block partials).

@@ -53,6 +53,36 @@ constexpr uint32_t kPartialStride = 2 * kMaxReduceBlocks;
 constexpr uint32_t kSingle = 0xffffffffu;
 constexpr uint32_t kGroupBase = 0xffffff00u;
 
+// Block coordinates of a pass: the hardware grid of a plain launch, or one
+// sample's share of a multi-sample launch (k_multi).  Kernels that can run
+// inside a multi-sample launch are written as functors over a Grid; the
+// plain launch is a named kernel that calls the functor with Grid::hw().
+struct Grid {
+  uint32_t block, blocks;
+  __device__ static Grid hw() { return {blockIdx.x, gridDim.x}; }
+};
+#define VK_GRID_STRIDE(g, t, count)                                                     \
+  for (uint64_t t = (g).block * (uint64_t)blockDim.x + threadIdx.x; t < (count);       \
+       t += (uint64_t)(g).blocks * blockDim.x)
+
+// One launch over several independent samples' instances of one pass: sample
+// s runs its own functor fs[s] on blocks [ends[s - 1], ends[s]) (ends[-1] =
+// 0), as that functor would run alone on a grid of ends[s] - ends[s - 1]
+// blocks.  At most 64 samples per launch: one wavefront reads the block ends
+// and the sample is the number of ends at or below the block (wave-uniform).
+constexpr uint32_t kMultiMax = 64;
+template <class F>
+__global__ void __launch_bounds__(kBlock) k_multi(const F *__restrict__ fs, const uint32_t *__restrict__ ends,
+                                               uint32_t count) {
+  const uint32_t b = blockIdx.x, lane = threadIdx.x & 63u;
+  const uint32_t e = lane < count ? ends[lane] : 0xffffffffu;
+  const uint32_t s = (uint32_t)__popcll(__ballot(e <= b));
+  const uint32_t begin = s ? (uint32_t)__builtin_amdgcn_readlane((int)e, (int)s - 1) : 0u;
+  const uint32_t end = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)s);
+  const F f = fs[s];
+  f(Grid{b - begin, end - begin});
+}
+
 // Threads per chunk: V slices of K lanes, G = K * V <= 64 (one wavefront).
 template <int K>
 struct Shape {
@@ -171,47 +201,57 @@ __device__ inline void block_partial(const double (&val)[NV], double *__restrict
   block_partial<K, NV>(val, out, blockIdx.x);
 }
 
-// Lane blockIdx.x of `items` block partials: thread t adds items t, t + kBlock,
+// Lane g.block of `items` block partials: thread t adds items t, t + kBlock,
 // ... in order, then a fixed LDS tree; thread 0 calls fin.reduced.
 template <int K, int NV, class Fin>
-__global__ void __launch_bounds__(kBlock) k_finish(const double *__restrict__ partial, uint32_t items, Fin fin,
-                                                   const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
-  if (!fin.enabled()) return;
-  __shared__ double lds[NV][kBlock];
-  const int j = blockIdx.x, tid = threadIdx.x;
+struct Finish {
+  static constexpr int kThreads = kBlock;
+  const double *__restrict__ partial;
+  uint32_t items;
+  Fin fin;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    if (gate && *gate == 0) return;
+    if (!fin.enabled()) return;
+    __shared__ double lds[NV][kBlock];
+    const int j = g.block, tid = threadIdx.x;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    double s = 0.0;
-    const double *row = partial + ((uint64_t)v * K + j) * kPartialStride;
-    uint32_t b = tid;
-    for (; b + 3 * kBlock < items; b += 4 * kBlock) {
-      const double a0 = row[b];
-      const double a1 = row[b + kBlock];
-      const double a2 = row[b + 2 * kBlock];
-      const double a3 = row[b + 3 * kBlock];
-      s += a0;
-      s += a1;
-      s += a2;
-      s += a3;
-    }
-    for (; b < items; b += kBlock) s += row[b];
-    lds[v][tid] = s;
-  }
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (tid < w) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w];
+    for (int v = 0; v < NV; ++v) {
+      double s = 0.0;
+      const double *row = partial + ((uint64_t)v * K + j) * kPartialStride;
+      uint32_t b = tid;
+      for (; b + 3 * kBlock < items; b += 4 * kBlock) {
+        const double a0 = row[b];
+        const double a1 = row[b + kBlock];
+        const double a2 = row[b + 2 * kBlock];
+        const double a3 = row[b + 3 * kBlock];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+      }
+      for (; b < items; b += kBlock) s += row[b];
+      lds[v][tid] = s;
     }
     __syncthreads();
-  }
-  if (tid == 0) {
-    double tot[NV];
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (tid < w) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) tot[v] = lds[v][0];
-    fin.reduced(j, tot);
+        for (int v = 0; v < NV; ++v) lds[v][tid] += lds[v][tid + w];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      double tot[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) tot[v] = lds[v][0];
+      fin.reduced(j, tot);
+    }
   }
+};
+template <int K, int NV, class Fin>
+__global__ void __launch_bounds__(kBlock) k_finish(Finish<K, NV, Fin> f) {
+  f(Grid::hw());
 }
 
 // ---------------------------------------------------------------- segments
@@ -255,46 +295,56 @@ __device__ inline T slice_sum(T v) {
 // device-scope fence is needed).  With NV > 0 every block stores its
 // reduction partial (no ticket).
 template <typename T, int K, int NV, class Op>
-__global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratch sc) {
-  if (sc.gate && *sc.gate == 0) return;
-  if (!op.enabled()) return;
-  constexpr int G = Shape<K>::G, NA = NV > 0 ? NV : 1;
-  const int j = threadIdx.x % K, s = (threadIdx.x % G) / K;
-  double acc[NA];
+struct SegPass {
+  static constexpr int kThreads = kBlock;
+  Chunks ch;
+  Op op;
+  SegScratch sc;
+  __device__ void operator()(Grid g) const {
+    if (sc.gate && *sc.gate == 0) return;
+    if (!op.enabled()) return;
+    constexpr int G = Shape<K>::G, NA = NV > 0 ? NV : 1;
+    const int j = threadIdx.x % K, s = (threadIdx.x % G) / K;
+    double acc[NA];
 #pragma unroll
-  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
-  T *partial = reinterpret_cast<T *>(sc.partial);
-  const uint64_t total = (uint64_t)ch.count * G;
-  for (uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x; t < total;
-       t += (uint64_t)gridDim.x * kBlock) {
-    uint32_t u = (uint32_t)(t / G);
-    if constexpr (G == 64) u = __builtin_amdgcn_readfirstlane(u);  // one chunk per wavefront
-    uint4 d = ch.desc[u];
-    if constexpr (G == 64) {
-      // wave-uniform chunk: the CSR walk below then uses scalar loads
-      d.x = __builtin_amdgcn_readfirstlane(d.x);
-      d.y = __builtin_amdgcn_readfirstlane(d.y);
-      d.z = __builtin_amdgcn_readfirstlane(d.z);
-      d.w = __builtin_amdgcn_readfirstlane(d.w);
-    }
-    if (d.w > kGroupBase && d.w != kSingle) {
-      // a group of short segments: slice s finishes segments s, s + V, ...
-      // of the group, each walked by its own K threads
-      for (uint32_t q = s; q < d.w - kGroupBase; q += Shape<K>::V) {
-        const uint32_t seg = d.x + q;
-        const uint4 e = make_uint4(seg, ch.ptr[seg], ch.ptr[seg + 1], kSingle);
-        op.finish(seg, j, op.template partial<1>(e, j, 0), acc);
+    for (int v = 0; v < NA; ++v) acc[v] = 0.0;
+    T *partial = reinterpret_cast<T *>(sc.partial);
+    const uint64_t total = (uint64_t)ch.count * G;
+    for (uint64_t t = g.block * (uint64_t)kBlock + threadIdx.x; t < total;
+         t += (uint64_t)g.blocks * kBlock) {
+      uint32_t u = (uint32_t)(t / G);
+      if constexpr (G == 64) u = __builtin_amdgcn_readfirstlane(u);  // one chunk per wavefront
+      uint4 d = ch.desc[u];
+      if constexpr (G == 64) {
+        // wave-uniform chunk: the CSR walk below then uses scalar loads
+        d.x = __builtin_amdgcn_readfirstlane(d.x);
+        d.y = __builtin_amdgcn_readfirstlane(d.y);
+        d.z = __builtin_amdgcn_readfirstlane(d.z);
+        d.w = __builtin_amdgcn_readfirstlane(d.w);
       }
-      continue;
+      if (d.w > kGroupBase && d.w != kSingle) {
+        // a group of short segments: slice s finishes segments s, s + V, ...
+        // of the group, each walked by its own K threads
+        for (uint32_t q = s; q < d.w - kGroupBase; q += Shape<K>::V) {
+          const uint32_t seg = d.x + q;
+          const uint4 e = make_uint4(seg, ch.ptr[seg], ch.ptr[seg + 1], kSingle);
+          op.finish(seg, j, op.template partial<1>(e, j, 0), acc);
+        }
+        continue;
+      }
+      const T part = slice_sum<T, K>(op.partial(d, j, s));
+      if (s != 0) continue;
+      if (d.w == kSingle)
+        op.finish(d.x, j, part, acc);
+      else
+        partial[(uint64_t)u * K + j] = part;
     }
-    const T part = slice_sum<T, K>(op.partial(d, j, s));
-    if (s != 0) continue;
-    if (d.w == kSingle)
-      op.finish(d.x, j, part, acc);
-    else
-      partial[(uint64_t)u * K + j] = part;
+    if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial, g.block);
   }
-  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial);
+};
+template <typename T, int K, int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_segments(SegPass<T, K, NV, Op> f) {
+  f(Grid::hw());
 }
 
 // Pass 2: every multi-chunk segment is summed by one block (kBlock / K
@@ -302,39 +352,50 @@ __global__ void __launch_bounds__(kBlock) k_segments(Chunks ch, Op op, SegScratc
 // each block stores its reduction partial after the `blocks1` pass-1
 // partials, and k_finish over both passes runs Op::reduced.
 template <typename T, int K, int NV, class Op>
-__global__ void __launch_bounds__(kBlock) k_segments_combine(Chunks ch, Op op, SegScratch sc, uint32_t blocks1) {
-  if (sc.gate && *sc.gate == 0) return;
-  if (!op.enabled()) return;
-  constexpr int S = kBlock / K, NA = NV > 0 ? NV : 1;
-  __shared__ T tree[kBlock];
-  const int j = threadIdx.x % K, s = threadIdx.x / K;
-  double acc[NA];
+struct SegCombine {
+  static constexpr int kThreads = kBlock;
+  Chunks ch;
+  Op op;
+  SegScratch sc;
+  uint32_t blocks1;
+  __device__ void operator()(Grid g) const {
+    if (sc.gate && *sc.gate == 0) return;
+    if (!op.enabled()) return;
+    constexpr int S = kBlock / K, NA = NV > 0 ? NV : 1;
+    __shared__ T tree[kBlock];
+    const int j = threadIdx.x % K, s = threadIdx.x / K;
+    double acc[NA];
 #pragma unroll
-  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
-  const T *partial = reinterpret_cast<const T *>(sc.partial);
-  for (uint32_t m = blockIdx.x; m < ch.n_multi; m += gridDim.x) {
-    const uint32_t first = ch.mrange[2 * m], end = ch.mrange[2 * m + 1];
-    T sum = T(0);
-    uint32_t q = first + s;
-    for (; q + 3 * S < end; q += 4 * S) {
-      const T a0 = partial[(uint64_t)q * K + j], a1 = partial[(uint64_t)(q + S) * K + j];
-      const T a2 = partial[(uint64_t)(q + 2 * S) * K + j], a3 = partial[(uint64_t)(q + 3 * S) * K + j];
-      sum += a0;
-      sum += a1;
-      sum += a2;
-      sum += a3;
-    }
-    for (; q < end; q += S) sum += partial[(uint64_t)q * K + j];
-    tree[threadIdx.x] = sum;
-    __syncthreads();
-    for (int w = S / 2; w > 0; w >>= 1) {
-      if (s < w) tree[threadIdx.x] += tree[threadIdx.x + w * K];
+    for (int v = 0; v < NA; ++v) acc[v] = 0.0;
+    const T *partial = reinterpret_cast<const T *>(sc.partial);
+    for (uint32_t m = g.block; m < ch.n_multi; m += g.blocks) {
+      const uint32_t first = ch.mrange[2 * m], end = ch.mrange[2 * m + 1];
+      T sum = T(0);
+      uint32_t q = first + s;
+      for (; q + 3 * S < end; q += 4 * S) {
+        const T a0 = partial[(uint64_t)q * K + j], a1 = partial[(uint64_t)(q + S) * K + j];
+        const T a2 = partial[(uint64_t)(q + 2 * S) * K + j], a3 = partial[(uint64_t)(q + 3 * S) * K + j];
+        sum += a0;
+        sum += a1;
+        sum += a2;
+        sum += a3;
+      }
+      for (; q < end; q += S) sum += partial[(uint64_t)q * K + j];
+      tree[threadIdx.x] = sum;
+      __syncthreads();
+      for (int w = S / 2; w > 0; w >>= 1) {
+        if (s < w) tree[threadIdx.x] += tree[threadIdx.x + w * K];
+        __syncthreads();
+      }
+      if (s == 0) op.finish(ch.desc[first].x, j, tree[j], acc);
       __syncthreads();
     }
-    if (s == 0) op.finish(ch.desc[first].x, j, tree[j], acc);
-    __syncthreads();
+    if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial, blocks1 + g.block);
   }
-  if constexpr (NV > 0) block_partial<K, NA>(acc, sc.bpartial, blocks1 + blockIdx.x);
+};
+template <typename T, int K, int NV, class Op>
+__global__ void __launch_bounds__(kBlock) k_segments_combine(SegCombine<T, K, NV, Op> f) {
+  f(Grid::hw());
 }
 
 // Value of lane `src` of a wavefront.  With `uniform` (the index is the same
@@ -609,85 +670,104 @@ inline uint64_t terminal_threads(uint32_t n) {
 }
 
 template <typename T, int K>
-__global__ void __launch_bounds__(kBlock) k_terminal_direct(uint32_t n, const double *__restrict__ g,
-                                                            const T *__restrict__ braw,
-                                                            const double *__restrict__ mean, T *__restrict__ b,
-                                                            T *__restrict__ x, const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
-  using TT = Terminal<K>;
-  constexpr int R = TT::kRows, KC = TT::kChunk, NB = TT::kLoadB, NG = TT::kLoadG, RG = TT::kGroups;
-  __shared__ double bt[KC][K];
-  __shared__ double gt[R][KC + 1];
-  const int tid = threadIdx.x, j = tid % K, row = tid / K;
-  for (uint32_t i0 = blockIdx.x * (uint32_t)R; i0 < n; i0 += gridDim.x * (uint32_t)R) {
-    double rb[NB], rg[NG];
-    auto load = [&](uint32_t k0) {
+struct TerminalDirect {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const double *__restrict__ ginv;
+  const T *__restrict__ braw;
+  const double *__restrict__ mean;
+  T *__restrict__ b, *__restrict__ x;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    if (gate && *gate == 0) return;
+    using TT = Terminal<K>;
+    constexpr int R = TT::kRows, KC = TT::kChunk, NB = TT::kLoadB, NG = TT::kLoadG, RG = TT::kGroups;
+    __shared__ double bt[KC][K];
+    __shared__ double gt[R][KC + 1];
+    const int tid = threadIdx.x, j = tid % K, row = tid / K;
+    for (uint32_t i0 = g.block * (uint32_t)R; i0 < n; i0 += g.blocks * (uint32_t)R) {
+      double rb[NB], rg[NG];
+      auto load = [&](uint32_t k0) {
 #pragma unroll
-      for (int e = 0; e < NB; ++e) {
-        const int idx = tid + e * kBlock, kk = idx / K, jj = idx % K;
-        rb[e] = 0.0;
-        if (idx < KC * K && k0 + kk < n)
-          rb[e] = (double)braw[(uint64_t)(k0 + kk) * K + jj] - (mean ? mean[jj] : 0.0);
+        for (int e = 0; e < NB; ++e) {
+          const int idx = tid + e * kBlock, kk = idx / K, jj = idx % K;
+          rb[e] = 0.0;
+          if (idx < KC * K && k0 + kk < n)
+            rb[e] = (double)braw[(uint64_t)(k0 + kk) * K + jj] - (mean ? mean[jj] : 0.0);
+        }
+#pragma unroll
+        for (int e = 0; e < NG; ++e) {
+          const int idx = tid + e * kBlock, rr = idx / KC, kk = idx % KC;
+          rg[e] = 0.0;
+          if (idx < R * KC && i0 + rr < n && k0 + kk < n) rg[e] = ginv[(uint64_t)(i0 + rr) * n + k0 + kk];
+        }
+      };
+      double s[TT::kPerThread];
+#pragma unroll
+      for (int q = 0; q < TT::kPerThread; ++q) s[q] = 0.0;
+      load(0);
+      for (uint32_t k0 = 0; k0 < n; k0 += KC) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+          const int idx = tid + e * kBlock;
+          if (idx < KC * K) bt[idx / K][idx % K] = rb[e];
+        }
+#pragma unroll
+        for (int e = 0; e < NG; ++e) {
+          const int idx = tid + e * kBlock;
+          if (idx < R * KC) gt[idx / KC][idx % KC] = rg[e];
+        }
+        __syncthreads();
+        if (k0 + KC < n) load(k0 + KC);  // the next chunk's loads fly during this chunk's products
+        const uint32_t kc = n - k0 < (uint32_t)KC ? n - k0 : (uint32_t)KC;
+        for (uint32_t kk = 0; kk < kc; ++kk) {
+          const double bv = bt[kk][j];
+#pragma unroll
+          for (int q = 0; q < TT::kPerThread; ++q) s[q] += gt[row + RG * q][kk] * bv;
+        }
+        __syncthreads();
       }
+      const double mu = mean ? mean[j] : 0.0;
 #pragma unroll
-      for (int e = 0; e < NG; ++e) {
-        const int idx = tid + e * kBlock, rr = idx / KC, kk = idx % KC;
-        rg[e] = 0.0;
-        if (idx < R * KC && i0 + rr < n && k0 + kk < n) rg[e] = g[(uint64_t)(i0 + rr) * n + k0 + kk];
-      }
-    };
-    double s[TT::kPerThread];
-#pragma unroll
-    for (int q = 0; q < TT::kPerThread; ++q) s[q] = 0.0;
-    load(0);
-    for (uint32_t k0 = 0; k0 < n; k0 += KC) {
-#pragma unroll
-      for (int e = 0; e < NB; ++e) {
-        const int idx = tid + e * kBlock;
-        if (idx < KC * K) bt[idx / K][idx % K] = rb[e];
-      }
-#pragma unroll
-      for (int e = 0; e < NG; ++e) {
-        const int idx = tid + e * kBlock;
-        if (idx < R * KC) gt[idx / KC][idx % KC] = rg[e];
-      }
-      __syncthreads();
-      if (k0 + KC < n) load(k0 + KC);  // the next chunk's loads fly during this chunk's products
-      const uint32_t kc = n - k0 < (uint32_t)KC ? n - k0 : (uint32_t)KC;
-      for (uint32_t kk = 0; kk < kc; ++kk) {
-        const double bv = bt[kk][j];
-#pragma unroll
-        for (int q = 0; q < TT::kPerThread; ++q) s[q] += gt[row + RG * q][kk] * bv;
-      }
-      __syncthreads();
-    }
-    const double mu = mean ? mean[j] : 0.0;
-#pragma unroll
-    for (int q = 0; q < TT::kPerThread; ++q) {
-      const uint32_t i = i0 + row + RG * q;
-      if (i < n) {
-        const uint64_t t = (uint64_t)i * K + j;
-        x[t] = (T)s[q];
-        b[t] = (T)((double)braw[t] - mu);
+      for (int q = 0; q < TT::kPerThread; ++q) {
+        const uint32_t i = i0 + row + RG * q;
+        if (i < n) {
+          const uint64_t t = (uint64_t)i * K + j;
+          x[t] = (T)s[q];
+          b[t] = (T)((double)braw[t] - mu);
+        }
       }
     }
   }
+};
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock) k_terminal_direct(TerminalDirect<T, K> f) {
+  f(Grid::hw());
 }
 
 // Iterative terminal: b = braw - mean, x = D^-1 b  (cycle.rs:342-379)
 template <typename T, int K>
-__global__ void k_terminal_scale(uint32_t n, const T *__restrict__ inv, const T *__restrict__ braw,
-                                 const double *__restrict__ mean, T *__restrict__ b, T *__restrict__ x,
-                                 const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < (uint64_t)n * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = t / K;
-    const int j = t % K;
-    const T bi = mean ? braw[t] - (T)mean[j] : braw[t];
-    b[t] = bi;
-    x[t] = inv[i] * bi;
+struct TerminalScale {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const T *__restrict__ inv, *__restrict__ braw;
+  const double *__restrict__ mean;
+  T *__restrict__ b, *__restrict__ x;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    if (gate && *gate == 0) return;
+    VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
+      const uint32_t i = t / K;
+      const int j = t % K;
+      const T bi = mean ? braw[t] - (T)mean[j] : braw[t];
+      b[t] = bi;
+      x[t] = inv[i] * bi;
+    }
   }
+};
+template <typename T, int K>
+__global__ void k_terminal_scale(TerminalScale<T, K> f) {
+  f(Grid::hw());
 }
 
 // Plain stream copy (HBM traffic calibration of the profiler counters).
@@ -701,17 +781,28 @@ static __global__ void k_stream_copy(uint64_t count, const double4 *__restrict__
 // rc = r - mean (FP64, kept for the Krylov recurrences) and its FP32 copy for
 // the V-cycle; with `inv`, also the cycle's first iterate x0 = D^-1 rf.
 template <int K>
-__global__ void k_centre_split(uint64_t rows, const double *__restrict__ r, const double *__restrict__ mean,
-                               double *__restrict__ rc, float *__restrict__ rf, const float *__restrict__ inv,
-                               float *__restrict__ x0, const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < rows * K;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const double v = mean ? r[t] - mean[t % K] : r[t];
-    rc[t] = v;
-    rf[t] = (float)v;
-    if (inv) x0[t] = inv[t / K] * (float)v;
+struct CentreSplit {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const double *__restrict__ r, *__restrict__ mean;
+  double *__restrict__ rc;
+  float *__restrict__ rf;
+  const float *__restrict__ inv;
+  float *__restrict__ x0;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    if (gate && *gate == 0) return;
+    VK_GRID_STRIDE(g, t, rows * K) {
+      const double v = mean ? r[t] - mean[t % K] : r[t];
+      rc[t] = v;
+      rf[t] = (float)v;
+      if (inv) x0[t] = inv[t / K] * (float)v;
+    }
   }
+};
+template <int K>
+__global__ void k_centre_split(CentreSplit<K> f) {
+  f(Grid::hw());
 }
 template <int K>
 __global__ void k_to_double(uint64_t count, const float *__restrict__ a, double *__restrict__ b,

@@ -43,15 +43,25 @@ enum { kFlagActive = 0, kFlagFresh = 1, kFlagOne = 2 };
 
 // ---- setup: compatibility check, norms (components.rs:669-744, pcg.rs:1596-1660)
 template <int K>
-__global__ void k_pcg_stats(uint32_t n, const double *__restrict__ b, double *__restrict__ partial) {
-  double acc[3] = {0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const double v = b[t];
-    acc[0] += v;
-    acc[1] += fabs(v);
-    acc[2] += v * v;
+struct PcgStats {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const double *__restrict__ b;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    double acc[3] = {0.0, 0.0, 0.0};
+    VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
+      const double v = b[t];
+      acc[0] += v;
+      acc[1] += fabs(v);
+      acc[2] += v * v;
+    }
+    block_partial<K, 3>(acc, partial, g.block);
   }
-  block_partial<K, 3>(acc, partial);
+};
+template <int K>
+__global__ void k_pcg_stats(PcgStats<K> f) {
+  f(Grid::hw());
 }
 struct FinStats {
   __device__ bool enabled() const { return true; }
@@ -75,31 +85,49 @@ struct FinStats {
 // proj = b - mean, r = proj, x = 0 (cold start; a warm start passes a
 // scratch buffer as `x`).
 template <int K>
-__global__ void k_pcg_project(uint32_t n, const double *__restrict__ b, double *__restrict__ proj,
-                              double *__restrict__ r, double *__restrict__ x,
-                              const ColState *__restrict__ st) {
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const int j = t % K;
-    const double p = b[t] - st[j].mean_r;
-    proj[t] = p;
-    r[t] = p;
-    x[t] = 0.0;
+struct PcgProject {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const double *__restrict__ b;
+  double *__restrict__ proj, *__restrict__ r, *__restrict__ x;
+  const ColState *__restrict__ st;
+  __device__ void operator()(Grid g) const {
+    VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
+      const int j = t % K;
+      const double p = b[t] - st[j].mean_r;
+      proj[t] = p;
+      r[t] = p;
+      x[t] = 0.0;
+    }
   }
+};
+template <int K>
+__global__ void k_pcg_project(PcgProject<K> f) {
+  f(Grid::hw());
 }
 
 // Initial tolerance test with a zero start: initial residual = ||b||.
 template <int K>
-__global__ void k_pcg_start(ColState *__restrict__ st, PcgParams prm) {
-  const int j = threadIdx.x;
-  if (j >= K) return;
-  ColState &s = st[j];
-  s.init_norm = s.rhs_norm;
-  s.tol = stop_tolerance(prm, s.rhs_norm, 0.0);
-  s.mean_r = 0.0;
-  if (s.active && s.init_norm <= s.tol) {
-    s.active = 0;
-    s.iters = 0;
+struct PcgStart {
+  static constexpr int kThreads = 64;
+  ColState *__restrict__ st;
+  PcgParams prm;
+  __device__ void operator()(Grid) const {
+    const int j = threadIdx.x;
+    if (j >= K) return;
+    ColState &s = st[j];
+    s.init_norm = s.rhs_norm;
+    s.tol = stop_tolerance(prm, s.rhs_norm, 0.0);
+    s.mean_r = 0.0;
+    if (s.active && s.init_norm <= s.tol) {
+      s.active = 0;
+      s.iters = 0;
+    }
   }
+};
+template <int K>
+__global__ void k_pcg_start(PcgStart<K> f) {
+  f(Grid::hw());
 }
 
 // Warm start: r = proj - A x (x centred), initial residual of the original rhs.
@@ -138,32 +166,51 @@ struct OpWarm : CsrOp<double, K, OpWarm<K>> {
 };
 
 template <int K>
-__global__ void k_center_cols(uint64_t rows, double *__restrict__ v, const double *__restrict__ mean,
-                              const int *__restrict__ gate = nullptr) {
-  VK_GATE(gate);
-  VK_GRID_LOOP(t, rows * K) v[t] -= mean[t % K];
+struct CenterCols {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  double *__restrict__ v;
+  const double *__restrict__ mean;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    VK_GATE(gate);
+    VK_GRID_STRIDE(g, t, rows * K) v[t] -= mean[t % K];
+  }
+};
+template <int K>
+__global__ void k_center_cols(CenterCols<K> f) {
+  f(Grid::hw());
 }
 
 // Lane means of the active lanes (zero for finished lanes, whose iterates stay
 // frozen) and the any-lane flags.
 template <int K>
-__global__ void k_state_means(const ColState *__restrict__ st, double *__restrict__ mean_x,
-                              double *__restrict__ mean_r, int *__restrict__ flags,
-                              const int *__restrict__ gate) {
-  VK_GATE(gate);
-  const int j = threadIdx.x;
-  const bool lane = j < K;
-  const bool active = lane && st[j].active;
-  const bool fresh = active && st[j].need_fresh;
-  if (lane) {
-    mean_x[j] = active ? st[j].mean_x : 0.0;
-    mean_r[j] = active ? st[j].mean_r : 0.0;
+struct StateMeans {
+  static constexpr int kThreads = 64;
+  const ColState *__restrict__ st;
+  double *__restrict__ mean_x, *__restrict__ mean_r;
+  int *__restrict__ flags;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid) const {
+    VK_GATE(gate);
+    const int j = threadIdx.x;
+    const bool lane = j < K;
+    const bool active = lane && st[j].active;
+    const bool fresh = active && st[j].need_fresh;
+    if (lane) {
+      mean_x[j] = active ? st[j].mean_x : 0.0;
+      mean_r[j] = active ? st[j].mean_r : 0.0;
+    }
+    const unsigned long long any_active = __ballot(active), any_fresh = __ballot(fresh);
+    if (j == 0) {
+      flags[kFlagActive] = any_active != 0ULL;
+      flags[kFlagFresh] = any_fresh != 0ULL;
+    }
   }
-  const unsigned long long any_active = __ballot(active), any_fresh = __ballot(fresh);
-  if (j == 0) {
-    flags[kFlagActive] = any_active != 0ULL;
-    flags[kFlagFresh] = any_fresh != 0ULL;
-  }
+};
+template <int K>
+__global__ void k_state_means(StateMeans<K> f) {
+  f(Grid::hw());
 }
 
 // Zero the means of lanes that are not part of the current solve.
@@ -175,17 +222,28 @@ __global__ void k_mask_means(double *__restrict__ mean, const uint8_t *__restric
 
 // rho = r.z - mean_z * sum r with r already centred (pcg.rs:1912-1947).
 template <int K, typename Z>
-__global__ void k_pcg_rho(uint32_t n, const double *__restrict__ r, const Z *__restrict__ z,
-                          double *__restrict__ partial, const int *__restrict__ gate) {
-  VK_GATE(gate);
-  double acc[3] = {0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const double zz = (double)z[t], rr = r[t];
-    acc[0] += zz;
-    acc[1] += rr * zz;
-    acc[2] += rr;
+struct PcgRho {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const double *__restrict__ r;
+  const Z *__restrict__ z;
+  double *__restrict__ partial;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    VK_GATE(gate);
+    double acc[3] = {0.0, 0.0, 0.0};
+    VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
+      const double zz = (double)z[t], rr = r[t];
+      acc[0] += zz;
+      acc[1] += rr * zz;
+      acc[2] += rr;
+    }
+    block_partial<K, 3>(acc, partial, g.block);
   }
-  block_partial<K, 3>(acc, partial);
+};
+template <int K, typename Z>
+__global__ void k_pcg_rho(PcgRho<K, Z> f) {
+  f(Grid::hw());
 }
 struct FinRho {
   __device__ bool enabled() const { return true; }
@@ -252,28 +310,39 @@ struct OpDirectionCurvature : CsrOp<double, K, OpDirectionCurvature<K, Z>> {
 // norms of the centred x and of r; candidate and scheduled residual
 // replacement flags (pcg.rs:1692-1760).
 template <int K>
-__global__ void k_pcg_update(uint32_t n, const double *__restrict__ p, const double *__restrict__ ap,
-                             double *__restrict__ x, const double *__restrict__ rc, double *__restrict__ r,
-                             double *__restrict__ partial, const ColState *__restrict__ st,
-                             const int *__restrict__ gate) {
-  VK_GATE(gate);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)n * K) {
-    const int j = t % K;
-    const ColState &s = st[j];
-    double xv = x[t], rv = rc[t];
-    if (s.active) {
-      xv += s.alpha * p[t];
-      rv -= s.alpha * ap[t];
-      x[t] = xv;
+struct PcgUpdate {
+  static constexpr int kThreads = kBlock;
+  uint32_t n;
+  const double *__restrict__ p, *__restrict__ ap;
+  double *__restrict__ x;
+  const double *__restrict__ rc;
+  double *__restrict__ r, *__restrict__ partial;
+  const ColState *__restrict__ st;
+  const int *__restrict__ gate;
+  __device__ void operator()(Grid g) const {
+    VK_GATE(gate);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
+      const int j = t % K;
+      const ColState &s = st[j];
+      double xv = x[t], rv = rc[t];
+      if (s.active) {
+        xv += s.alpha * p[t];
+        rv -= s.alpha * ap[t];
+        x[t] = xv;
+      }
+      r[t] = rv;
+      acc[0] += xv;
+      acc[1] += xv * xv;
+      acc[2] += rv;
+      acc[3] += rv * rv;
     }
-    r[t] = rv;
-    acc[0] += xv;
-    acc[1] += xv * xv;
-    acc[2] += rv;
-    acc[3] += rv * rv;
+    block_partial<K, 4>(acc, partial, g.block);
   }
-  block_partial<K, 4>(acc, partial);
+};
+template <int K>
+__global__ void k_pcg_update(PcgUpdate<K> f) {
+  f(Grid::hw());
 }
 struct FinUpdate {
   __device__ bool enabled() const { return true; }
@@ -347,29 +416,58 @@ struct OpFresh : CsrOp<double, K, OpFresh<K>> {
 
 // Iteration limit and the active flag for the next iteration.
 template <int K>
-__global__ void k_pcg_limit(ColState *__restrict__ st, PcgParams prm, int *__restrict__ flags) {
-  const int j = threadIdx.x;
-  bool active = false;
-  if (j < K) {
-    ColState &s = st[j];
-    if (s.active && !s.need_fresh && s.iters >= prm.max_iterations) {
-      s.status = 63;
-      s.active = 0;
+struct PcgLimit {
+  static constexpr int kThreads = 64;
+  ColState *__restrict__ st;
+  PcgParams prm;
+  int *__restrict__ flags;
+  __device__ void operator()(Grid) const {
+    const int j = threadIdx.x;
+    bool active = false;
+    if (j < K) {
+      ColState &s = st[j];
+      if (s.active && !s.need_fresh && s.iters >= prm.max_iterations) {
+        s.status = 63;
+        s.active = 0;
+      }
+      active = s.active != 0;
     }
-    active = s.active != 0;
+    const unsigned long long any = __ballot(active);
+    if (j == 0) flags[kFlagActive] = any != 0ULL;
   }
-  const unsigned long long any = __ballot(active);
-  if (j == 0) flags[kFlagActive] = any != 0ULL;
+};
+template <int K>
+__global__ void k_pcg_limit(PcgLimit<K> f) {
+  f(Grid::hw());
+}
+
+// Some lane of some sample active: the OR of the samples' any-lane-active
+// flags (one block), the poll of a lockstep sample batch.
+static __global__ void k_any_active(const int *const *__restrict__ flags, uint32_t count, int *__restrict__ out) {
+  int any = 0;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) any |= *flags[i] != 0;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) *out = any;
 }
 
 // ---------------------------------------------------------------- estimator passes
 
 // Column means over `rows` rows of a K-lane matrix.
 template <int K>
-__global__ void k_lane_sum(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial) {
-  double acc[1] = {0.0};
-  VK_GRID_LOOP(t, rows * K) acc[0] += v[t];
-  block_partial<K, 1>(acc, partial);
+struct LaneSum {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const double *__restrict__ v;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    double acc[1] = {0.0};
+    VK_GRID_STRIDE(g, t, rows * K) acc[0] += v[t];
+    block_partial<K, 1>(acc, partial, g.block);
+  }
+};
+template <int K>
+__global__ void k_lane_sum(LaneSum<K> f) {
+  f(Grid::hw());
 }
 struct FinMean {
   __device__ bool enabled() const { return true; }

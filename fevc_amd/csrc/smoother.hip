@@ -237,7 +237,7 @@ const double *train_w(Impl &E, size_t l, bool learned) {
 
 void lane_mean(Impl &E, uint32_t n, const double *v, double *mean) {
   const unsigned g = reduce_grid((uint64_t)n * KT);
-  hipLaunchKernelGGL((dev::k_lane_sum<KT>), g, kBlock, 0, E.stream, (uint64_t)n, v, E.rpartial.p);
+  run(E, dev::k_lane_sum<KT>, g, dev::LaneSum<KT>{(uint64_t)n, v, E.rpartial.p});
   finish<KT, 1>(E, E.rpartial.p, g, dev::FinMean{mean, nullptr, (uint64_t)n});
 }
 
@@ -256,11 +256,11 @@ const double *train_forward(Impl &E, size_t l, const double *braw, bool learned)
   const double *mean = l ? T.mean.p : nullptr;
   if (D.terminal) {
     if (D.terminal == kDirect)
-      hipLaunchKernelGGL((dev::k_terminal_direct<double, KT>), grid_for(dev::terminal_threads<KT>(n)), kBlock, 0, E.stream, n, E.ginv.p, braw,
-                         mean, T.b.p, T.y.p, (const int *)nullptr);
+      run(E, dev::k_terminal_direct<double, KT>, grid_for(dev::terminal_threads<KT>(n)),
+          dev::TerminalDirect<double, KT>{n, E.ginv.p, braw, mean, T.b.p, T.y.p, nullptr});
     else
-      hipLaunchKernelGGL((dev::k_terminal_scale<double, KT>), grid_for(m), kBlock, 0, E.stream, n, D.inv.p, braw,
-                         mean, T.b.p, T.y.p, (const int *)nullptr);
+      run(E, dev::k_terminal_scale<double, KT>, grid_for(m),
+          dev::TerminalScale<double, KT>{n, D.inv.p, braw, mean, T.b.p, T.y.p, nullptr});
     return T.y.p;
   }
   const double *w = train_w(E, l, learned);

@@ -2093,9 +2093,15 @@ int32_t vckss_mi355x_batch_prepare_v1(uint64_t rows, const double *sample, const
 
 int32_t vckss_mi355x_batch_solve_v1(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
                                     double fit_tolerance, uint32_t precision, uint32_t streams) {
+  return vckss_mi355x_batch_solve_v2(handle, seed, probes, tolerance, fit_tolerance, precision, streams, 0);
+}
+
+int32_t vckss_mi355x_batch_solve_v2(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
+                                    double fit_tolerance, uint32_t precision, uint32_t streams, uint32_t mode) {
   std::lock_guard<std::mutex> lock(g_mutex);
   return guarded([&] {
     Batch &B = live_batch(handle);
+    if (mode > 1) fail(kInvalidInput, "batch_solve", "mode must be 0 (streams) or 1 (lockstep)");
     if (probes < 2) fail(kInvalidInput, "jla_validate", "JLA probe count must be at least two");
     if (precision != 32 && precision != 64) fail(kInvalidInput, "batch_solve", "precision must be 32 or 64");
     for (double t : {tolerance, fit_tolerance})
@@ -2116,18 +2122,39 @@ int32_t vckss_mi355x_batch_solve_v1(uint64_t handle, uint64_t seed, uint32_t pro
       bs->status = 0;
       bs->error.clear();
     }
-    parallel_for(idx, std::max<uint32_t>(streams, 1), [&](size_t k) {
-      BatchSample &bs = *B.samples[k];
-      const auto t0 = std::chrono::steady_clock::now();
-      try {
-        bs.result = bs.dev->run(o);
-        bs.solved = true;
-      } catch (const Error &err) {
-        bs.status = err.code;
-        bs.error = err.what();
+    // lockstep: the sample threads' PCG calls meet in one batch (a thread
+    // joins before its sample and leaves after it, also on failure, so the
+    // others never wait for it)
+    std::shared_ptr<Lockstep> ls = mode == 1 ? make_lockstep() : nullptr;
+    struct Member {
+      Lockstep *l;
+      explicit Member(Lockstep *x) : l(x) {
+        if (l) lockstep_join(*l);
       }
-      bs.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    });
+      ~Member() {
+        if (l) lockstep_leave(*l);
+      }
+    };
+    for (auto &bs : B.samples) bs->dev->set_lockstep(ls.get());
+    try {
+      parallel_for(idx, std::max<uint32_t>(streams, 1), [&](size_t k) {
+        BatchSample &bs = *B.samples[k];
+        const Member member(ls.get());
+        const auto t0 = std::chrono::steady_clock::now();
+        try {
+          bs.result = bs.dev->run(o);
+          bs.solved = true;
+        } catch (const Error &err) {
+          bs.status = err.code;
+          bs.error = err.what();
+        }
+        bs.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      });
+    } catch (...) {
+      for (auto &bs : B.samples) bs->dev->set_lockstep(nullptr);
+      throw;
+    }
+    for (auto &bs : B.samples) bs->dev->set_lockstep(nullptr);
     for (size_t k = 0; k < B.samples.size(); ++k)
       if (B.samples[k]->status)
         fail(B.samples[k]->status, "batch_solve", "sample " + std::to_string(k) + ": " + B.samples[k]->error);

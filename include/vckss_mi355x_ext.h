@@ -199,6 +199,14 @@ int32_t vckss_mi355x_batch_prepare_v1(uint64_t rows, const double *sample, const
  * failing sample, after all have run. */
 int32_t vckss_mi355x_batch_solve_v1(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
                                     double fit_tolerance, uint32_t precision, uint32_t streams);
+/* As v1, with `mode`: 0 runs up to `streams` samples at once, each alone on
+ * its own stream (v1); 1 runs up to `streams` samples in lockstep: the PCG
+ * calls of the samples waiting together run as one, each pass of the
+ * lockstep certified PCG and its V-cycle a single launch over all of those
+ * samples with per-sample block ranges, convergence gates and lane states.
+ * Each sample's estimate is the same in both modes, bit for bit. */
+int32_t vckss_mi355x_batch_solve_v2(uint64_t handle, uint64_t seed, uint32_t probes, double tolerance,
+                                    double fit_tolerance, uint32_t precision, uint32_t streams, uint32_t mode);
 /* out[0..16): plug-in, correction, corrected, MCSE (worker, firm, covariance,
  * total each); out[16..24): sample identifier, stored rows, workers, firms,
  * deletion units, solve seconds, PCG iterations, largest complete residual.

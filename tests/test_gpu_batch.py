@@ -1,13 +1,17 @@
-"""Uneven-size sample batches on the GPU (vckss_mi355x_batch_*_v1): several
-independent subsamples of different sizes prepared together and estimated
-concurrently, each on its own HIP stream.
+"""Uneven-size sample batches on the GPU (vckss_mi355x_batch_*_v1/_v2):
+several independent subsamples of different sizes prepared together and
+estimated concurrently, each on its own HIP stream, or in lockstep (the PCG
+calls of all samples run as one: each pass a single launch over every
+sample, with per-sample block ranges, convergence gates and lane states).
 
 Bars: every subsample's components within 1e-8 relative of the oracle's
 estimate of that subsample alone (observation deletion, the workload's
-mode, and match deletion); the concurrent run bit-identical to one sample at
-a time (each sample's engine is deterministic whatever runs beside it); the
-sample order follows the identifiers; a sample that fails preparation fails
-the batch with its status.
+mode, and match deletion); the concurrent and the lockstep runs
+bit-identical to one sample at a time (each sample's passes are the ones it
+runs alone, with its own grid and reduction order), with the FP64 and the
+FP32 V-cycle, with every sample in one lockstep group and with fewer
+lockstep threads than samples; the sample order follows the identifiers; a
+sample that fails preparation fails the batch with its status.
 """
 import functools
 
@@ -67,3 +71,40 @@ def test_batch_sample_failure_reports_status():
     with pytest.raises(F.EngineError) as err:
         F.SampleBatch(c["sample"], *[c[k] for k in COLS], deletion_mode="match")
     assert err.value.code != 0 and "sample 6" in str(err.value)
+
+
+@pytest.mark.parametrize("mode,precision,streams", [("observation", 64, 6), ("observation", 32, 6),
+                                                    ("match", 32, 6), ("observation", 32, 4)])
+def test_lockstep_matches_one_sample_at_a_time(mode, precision, streams):
+    cols, specs, ids = workload()
+    probes = 32
+    b = F.SampleBatch(cols["sample"], *[cols[k] for k in COLS], deletion_mode=mode, threads=4)
+    try:
+        print(f"lockstep batch: {b.samples} samples, mode {mode}, precision {precision}, {streams} threads",
+              flush=True)
+        alone = b.solve(probes=probes, tolerance=1e-10, fit_tolerance=1e-10, precision=precision, streams=1)
+        print("lockstep batch: one at a time done", flush=True)
+        together = b.solve(probes=probes, tolerance=1e-10, fit_tolerance=1e-10, precision=precision,
+                           streams=streams, lockstep=True)
+        print("lockstep batch: lockstep done", flush=True)
+        again = b.solve(probes=probes, tolerance=1e-10, fit_tolerance=1e-10, precision=precision,
+                        streams=streams, lockstep=True)
+    finally:
+        b.release()
+    for k, sid in enumerate(np.sort(ids)):
+        assert together[k]["sample"] == sid
+        for key in ("plugin", "correction", "corrected", "mcse"):
+            assert np.array_equal(together[k][key], alone[k][key]), (k, key, together[k][key], alone[k][key])
+            assert np.array_equal(again[k][key], alone[k][key]), (k, key)
+        for key in ("iterations", "max_complete_residual", "units"):
+            assert together[k][key] == alone[k][key], (k, key)
+
+
+def test_lockstep_mode_argument():
+    cols, specs, ids = workload()
+    b = F.SampleBatch(cols["sample"], *[cols[k] for k in COLS], deletion_mode="match", threads=4)
+    try:
+        code = F.library().vckss_mi355x_batch_solve_v2(b.handle, 1, 8, 1e-8, 1e-8, 32, 4, 2)
+        assert code == 20
+    finally:
+        b.release()
