@@ -318,7 +318,8 @@ def main():
         # many independent region-year subsamples of uneven size (log-normal
         # workers, median 40 000, clipped to 5 000-500 000; firms = workers /
         # 20; T = 4; mobility U(0.05, 0.25)), observation deletion, 128 probes
-        # each: concurrently on 4, 8 and 16 streams against one sample at a time
+        # each: in lockstep (every PCG pass one launch over all 64 samples) and
+        # concurrently on 4, 8 and 16 streams, against one sample at a time
         sc, sspecs = synth.subsamples(count=args.subsamples, scale=args.scale)
         t0 = time.perf_counter()
         sb = F.SampleBatch(sc["sample"], *[sc[k] for k in ("worker", "firm", "deletion", "outcome", "frequency",
@@ -326,33 +327,44 @@ def main():
                            deletion_mode="observation", threads=cpu_threads(0))
         bprep = time.perf_counter() - t0
         try:
-            sb.solve(probes=16, tolerance=args.tolerance, fit_tolerance=args.tolerance, precision=args.precision,
-                     streams=4)  # warm: device buffers, code objects
+            n = int(sb.samples)
+            for lockstep in (False, True):  # warm: device buffers, code objects
+                sb.solve(probes=16, tolerance=args.tolerance, fit_tolerance=args.tolerance,
+                         precision=args.precision, streams=n if lockstep else 4, lockstep=lockstep)
             runs = {}
             for streams in (1, 4, 8, 16):
                 t0 = time.perf_counter()
                 res = sb.solve(probes=128, tolerance=args.tolerance, fit_tolerance=args.tolerance,
                                precision=args.precision, streams=streams)
                 runs[streams] = (time.perf_counter() - t0, res)
+            lock = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                res = sb.solve(probes=128, tolerance=args.tolerance, fit_tolerance=args.tolerance,
+                               precision=args.precision, streams=n, lockstep=True)
+                lock.append((time.perf_counter() - t0, res))
+            wl, rl = min(lock, key=lambda x: x[0])
             w1, r1 = runs[1]
             best = min((s for s in runs if s > 1), key=lambda s: runs[s][0])
-            w4, r4 = runs[best]
-            same = all(np.array_equal(a["corrected"], b["corrected"])
-                       for s in runs for a, b in zip(r1, runs[s][1]))
-            rows = [int(r["rows"]) for r in r4]
-            batch = {"subsamples": len(r4), "deletion": "observation", "probes": 128,
+            ws, rs = runs[best]
+            same = all(np.array_equal(a[k], b[k]) for other in [runs[s][1] for s in runs] + [r for _, r in lock]
+                       for a, b in zip(r1, other) for k in ("plugin", "correction", "corrected", "mcse"))
+            rows = [int(r["rows"]) for r in rl]
+            batch = {"subsamples": len(rl), "deletion": "observation", "probes": 128,
                      "workers_median": int(np.median([s.workers for s in sspecs])),
                      "workers_range": [min(s.workers for s in sspecs), max(s.workers for s in sspecs)],
                      "stored_rows_total": int(sum(rows)), "prepare_seconds": round(bprep, 2),
+                     "lockstep_seconds": round(wl, 3),
+                     "lockstep_runs_seconds": [round(w, 3) for w, _ in lock],
                      "one_at_a_time_seconds": round(w1, 3),
                      "concurrent_seconds": {str(s): round(runs[s][0], 3) for s in runs if s > 1},
-                     "streams": best, "speedup": round(w1 / w4, 2),
-                     "probe_solves_per_s": round(3 * 128 * len(r4) / w4, 1),
-                     "results_identical": bool(same),
-                     "largest_sample_seconds": round(max(r["seconds"] for r in r4), 3)}
-            log(f"subsample batch ({len(r4)} samples, {sum(rows)} rows): one at a time {w1:.2f}s, "
-                + ", ".join(f"{s} streams {runs[s][0]:.2f}s" for s in runs if s > 1)
-                + f" (best {w1 / w4:.2f}x), identical {same}")
+                     "best_streams": best, "speedup_lockstep_vs_one_at_a_time": round(w1 / wl, 2),
+                     "speedup_lockstep_vs_best_streams": round(ws / wl, 2),
+                     "probe_solves_per_s": round(3 * 128 * len(rl) / wl, 1),
+                     "results_identical": bool(same)}
+            log(f"subsample batch ({len(rl)} samples, {sum(rows)} rows): lockstep {wl:.3f}s, one at a time "
+                f"{w1:.2f}s, " + ", ".join(f"{s} streams {runs[s][0]:.2f}s" for s in runs if s > 1)
+                + f" (lockstep {w1 / wl:.2f}x vs one at a time, {ws / wl:.2f}x vs best streams), identical {same}")
         finally:
             sb.release()
 

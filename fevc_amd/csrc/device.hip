@@ -327,11 +327,15 @@ struct PcgCall {
   void precondition() {
     const uint32_t n = E.n0;
     run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, nullptr});
-    zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, E.rc.p, fp32);
+    // the FP32 cycle centres r into its own FP32 input, so no FP64 copy is
+    // stored; the FP64 cycle uses it as its level-0 right-hand side
+    zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, fp32 ? nullptr : E.rc.p, fp32);
     if (zcur.z32)
-      run(E, dev::k_pcg_rho<K, float>, rg, dev::PcgRho<K, float>{n, E.rc.p, zcur.z32, E.rpartial.p, active});
+      run(E, dev::k_pcg_rho<K, float>, rg,
+          dev::PcgRho<K, float>{n, E.r.p, E.lane_b.p, zcur.z32, E.rpartial.p, active});
     else
-      run(E, dev::k_pcg_rho<K, double>, rg, dev::PcgRho<K, double>{n, E.rc.p, zcur.z64, E.rpartial.p, active});
+      run(E, dev::k_pcg_rho<K, double>, rg,
+          dev::PcgRho<K, double>{n, E.r.p, E.lane_b.p, zcur.z64, E.rpartial.p, active});
     finish<K, 3>(E, E.rpartial.p, rg, dev::FinRho{E.st.p, n}, active);
   }
 
@@ -354,14 +358,22 @@ struct PcgCall {
                              dev::OpDirectionCurvature<K, double>{{L0.col.p, L0.val.p}, zcur.z64, E.p.p, E.ap.p,
                                                                   E.st.p},
                              ev.first, ev.second);
+    // x is centred inside the next update (its mean after this one is kept
+    // in the lane state) and on the fly by the fresh residual
     run(E, dev::k_pcg_update<K>, rg,
-        dev::PcgUpdate<K>{n, E.p.p, E.ap.p, E.x.p, E.rc.p, E.r.p, E.rpartial.p, E.st.p, active});
+        dev::PcgUpdate<K>{n, E.p.p, E.ap.p, E.x.p, E.r.p, E.rpartial.p, E.st.p, E.lane_a.p, E.lane_b.p, active});
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
     run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, active});
-    run(E, dev::k_center_cols<K>, grid_for(m0), dev::CenterCols<K>{(uint64_t)n, E.x.p, E.lane_a.p, active});
     segments<double, K, 3>(E, rows0,
-                           dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags});
+                           dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags,
+                                           E.lane_a.p});
     run(E, dev::k_pcg_limit<K>, 1, dev::PcgLimit<K>{E.st.p, prm, flags});
+  }
+
+  // after the iterations: every lane's last update leaves x uncentred by its
+  // mean, kept in the lane state
+  void center() {
+    run(E, dev::k_center_state<K>, grid_for((uint64_t)E.n0 * K), dev::CenterState<K>{E.n0, E.x.p, E.st.p});
   }
 };
 
@@ -396,6 +408,7 @@ void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vecto
   prog.lanes = K;
   prog.max_iterations = max_it;
   lockstep_submit(*E.lockstep, prog);
+  c.center();  // on the engine's stream, which the batch has released
   E.vcycles += prog.cycles;
   E.iterations_launched += prog.iterations;
   E.gate = E.always();
@@ -470,6 +483,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     precondition_step();
   }
   E.gate = E.always();
+  c.center();
   std::vector<dev::ColState> host(K);
   VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
   VK_HIP(hipStreamSynchronize(E.stream));

@@ -792,9 +792,11 @@ struct CentreSplit {
   const int *__restrict__ gate;
   __device__ void operator()(Grid g) const {
     if (gate && *gate == 0) return;
+    static_assert(kBlock % K == 0, "a thread keeps its lane across the grid stride");
+    const double m = mean ? mean[threadIdx.x % K] : 0.0;
     VK_GRID_STRIDE(g, t, rows * K) {
-      const double v = mean ? r[t] - mean[t % K] : r[t];
-      rc[t] = v;
+      const double v = mean ? r[t] - m : r[t];
+      if (rc) rc[t] = v;
       rf[t] = (float)v;
       if (inv) x0[t] = inv[t / K] * (float)v;
     }

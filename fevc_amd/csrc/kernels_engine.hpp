@@ -174,7 +174,9 @@ struct CenterCols {
   const int *__restrict__ gate;
   __device__ void operator()(Grid g) const {
     VK_GATE(gate);
-    VK_GRID_STRIDE(g, t, rows * K) v[t] -= mean[t % K];
+    static_assert(kBlock % K == 0, "a thread keeps its lane across the grid stride");
+    const double m = mean[threadIdx.x % K];
+    VK_GRID_STRIDE(g, t, rows * K) v[t] -= m;
   }
 };
 template <int K>
@@ -225,15 +227,17 @@ template <int K, typename Z>
 struct PcgRho {
   static constexpr int kThreads = kBlock;
   uint32_t n;
-  const double *__restrict__ r;
+  const double *__restrict__ r, *__restrict__ mean_r;  // r centred on the fly (the V-cycle's input)
   const Z *__restrict__ z;
   double *__restrict__ partial;
   const int *__restrict__ gate;
   __device__ void operator()(Grid g) const {
     VK_GATE(gate);
+    static_assert(kBlock % K == 0, "a thread keeps its lane across the grid stride");
     double acc[3] = {0.0, 0.0, 0.0};
+    const double mr = mean_r[threadIdx.x % K];
     VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
-      const double zz = (double)z[t], rr = r[t];
+      const double zz = (double)z[t], rr = r[t] - mr;
       acc[0] += zz;
       acc[1] += rr * zz;
       acc[2] += rr;
@@ -315,20 +319,24 @@ struct PcgUpdate {
   uint32_t n;
   const double *__restrict__ p, *__restrict__ ap;
   double *__restrict__ x;
-  const double *__restrict__ rc;
   double *__restrict__ r, *__restrict__ partial;
   const ColState *__restrict__ st;
+  // per lane, 0 for inactive lanes: x's mean after the previous update (its
+  // centring, folded in here) and the residual mean the V-cycle removed
+  const double *__restrict__ mean_x, *__restrict__ mean_r;
   const int *__restrict__ gate;
   __device__ void operator()(Grid g) const {
     VK_GATE(gate);
+    static_assert(kBlock % K == 0, "a thread keeps its lane across the grid stride");
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int j = (int)(threadIdx.x % K);
+    const bool active = st[j].active != 0;
+    const double alpha = st[j].alpha, mx = mean_x[j], mr = mean_r[j];
     VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
-      const int j = t % K;
-      const ColState &s = st[j];
-      double xv = x[t], rv = rc[t];
-      if (s.active) {
-        xv += s.alpha * p[t];
-        rv -= s.alpha * ap[t];
+      double xv = x[t] - mx, rv = r[t] - mr;
+      if (active) {
+        xv += alpha * p[t];
+        rv -= alpha * ap[t];
         x[t] = xv;
       }
       r[t] = rv;
@@ -364,8 +372,9 @@ struct FinUpdate {
   }
 };
 
-// Fresh residual for flagged lanes: r = proj - A x (x already centred), the
-// certificate against the unprojected rhs (pcg.rs:1757-1817, 2209-2236).
+// Fresh residual for flagged lanes: r = proj - A x with x centred by the
+// lane's pending mean, the certificate against the unprojected rhs
+// (pcg.rs:1757-1817, 2209-2236).
 template <int K>
 struct OpFresh : CsrOp<double, K, OpFresh<K>> {
   const double *__restrict__ b, *__restrict__ proj, *__restrict__ x;
@@ -373,9 +382,11 @@ struct OpFresh : CsrOp<double, K, OpFresh<K>> {
   ColState *__restrict__ st;
   PcgParams prm;
   const int *__restrict__ flags;
+  const double *__restrict__ mean_x;  // per lane: x's mean after the latest update (0 for inactive lanes)
   __device__ bool enabled() const { return flags[kFlagFresh] != 0; }
-  __device__ double self(uint32_t i, int j, NoLane = {}) const { return x[(uint64_t)i * K + j]; }
-  __device__ double nbr(uint32_t c, int j, NoLane = {}) const { return x[(uint64_t)c * K + j]; }
+  __device__ double lane(int j) const { return mean_x[j]; }
+  __device__ double self(uint32_t i, int j, double m) const { return x[(uint64_t)i * K + j] - m; }
+  __device__ double nbr(uint32_t c, int j, double m) const { return x[(uint64_t)c * K + j] - m; }
   template <int NA>
   __device__ void finish(uint32_t i, int j, double ax, double (&acc)[NA]) const {
     if (!(st[j].active && st[j].need_fresh)) return;
@@ -438,6 +449,25 @@ struct PcgLimit {
 };
 template <int K>
 __global__ void k_pcg_limit(PcgLimit<K> f) {
+  f(Grid::hw());
+}
+
+// x -= the lane's pending mean (the centring of the last update of every
+// lane, which the update of the next iteration would have folded in).
+template <int K>
+struct CenterState {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  double *__restrict__ x;
+  const ColState *__restrict__ st;
+  __device__ void operator()(Grid g) const {
+    static_assert(kBlock % K == 0, "a thread keeps its lane across the grid stride");
+    const double m = st[threadIdx.x % K].mean_x;
+    VK_GRID_STRIDE(g, t, rows * K) x[t] -= m;
+  }
+};
+template <int K>
+__global__ void k_center_state(CenterState<K> f) {
   f(Grid::hw());
 }
 
