@@ -312,21 +312,24 @@ struct PcgCall {
     E.gate = E.always();
     if (!warm) {
       run(E, dev::k_pcg_project<K>, grid_for(m0), dev::PcgProject<K>{n, b, E.proj.p, E.r.p, E.x.p, E.st.p});
-      run(E, dev::k_pcg_start<K>, 1, dev::PcgStart<K>{E.st.p, prm});
+      run(E, dev::k_pcg_start<K>, 1, dev::PcgStart<K>{E.st.p, prm, pub()});
     } else {
       run(E, dev::k_pcg_project<K>, grid_for(m0), dev::PcgProject<K>{n, b, E.proj.p, E.r.p, E.ap.p, E.st.p});
       run(E, dev::k_lane_sum<K>, rg, dev::LaneSum<K>{(uint64_t)n, E.x.p, E.rpartial.p});
       finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
       run(E, dev::k_center_cols<K>, grid_for(m0), dev::CenterCols<K>{(uint64_t)n, E.x.p, E.lane_a.p, nullptr});
       segments<double, K, 4>(E, rows0,
-                             dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm});
+                             dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, pub()});
+      run(E, dev::k_pcg_flags<K>, 1, dev::PcgFlags<K>{E.st.p, flags});
     }
     E.gate = active;
   }
 
+  // the lane means and flags the state-changing passes write
+  dev::LanePublish pub() const { return {E.lane_a.p, E.lane_b.p, flags}; }
+
   void precondition() {
     const uint32_t n = E.n0;
-    run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, nullptr});
     // the FP32 cycle centres r into its own FP32 input, so no FP64 copy is
     // stored; the FP64 cycle uses it as its level-0 right-hand side
     zcur = apply_cycle_raw<K>(E, E.r.p, E.lane_b.p, fp32 ? nullptr : E.rc.p, fp32);
@@ -362,12 +365,11 @@ struct PcgCall {
     // in the lane state) and on the fly by the fresh residual
     run(E, dev::k_pcg_update<K>, rg,
         dev::PcgUpdate<K>{n, E.p.p, E.ap.p, E.x.p, E.r.p, E.rpartial.p, E.st.p, E.lane_a.p, E.lane_b.p, active});
-    finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm}, active);
-    run(E, dev::k_state_means<K>, 1, dev::StateMeans<K>{E.st.p, E.lane_a.p, E.lane_b.p, flags, active});
+    finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm, pub()}, active);
     segments<double, K, 3>(E, rows0,
                            dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags,
-                                           E.lane_a.p});
-    run(E, dev::k_pcg_limit<K>, 1, dev::PcgLimit<K>{E.st.p, prm, flags});
+                                           pub()});
+    run(E, dev::k_pcg_limit<K>, 1, dev::PcgLimit<K>{E.st.p, prm, pub()});
   }
 
   // after the iterations: every lane's last update leaves x uncentred by its
@@ -463,6 +465,10 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     }
   };
   const int *active = c.active;
+  // A long iteration covers the host's enqueue of the next one, so the poll
+  // can look one iteration back (one queued no-op iteration after the last);
+  // short ones keep two in flight.
+  const uint32_t lag = (uint64_t)E.n0 * K >= kLongIteration ? 1u : (uint32_t)kPollLag;
   precondition_step();
   for (uint32_t it = 1; it <= max_it; ++it) {
     E.iterations_launched++;
@@ -475,8 +481,8 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
     const int slot = it % kPollSlots;
     VK_HIP(hipMemcpyAsync(E.host_flags + slot, active, sizeof(int), hipMemcpyDeviceToHost, E.stream));
     VK_HIP(hipEventRecord(E.poll_events[slot], E.stream));
-    if (it > (uint32_t)kPollLag) {
-      const int seen = (it - kPollLag) % kPollSlots;
+    if (it > lag) {
+      const int seen = (it - lag) % kPollSlots;
       VK_HIP(hipEventSynchronize(E.poll_events[seen]));
       if (E.host_flags[seen] == 0) break;
     }

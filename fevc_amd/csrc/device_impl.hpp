@@ -33,6 +33,9 @@ using dev::kBlock;
 constexpr uint32_t kChunkEntries = 64;  // default CSR entries or aggregate members per chunk
 constexpr int kPollSlots = 8;
 constexpr int kPollLag = 2;
+// finest rows x lanes from which one PCG iteration outlasts the host's enqueue
+// of the next (then the poll lags one iteration)
+constexpr uint64_t kLongIteration = 1ull << 20;
 
 // Live and peak bytes of the engine's device buffers (all DBufs of the
 // process), for the memory receipts.

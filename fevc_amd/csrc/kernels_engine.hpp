@@ -38,6 +38,20 @@ __device__ inline double stop_tolerance(const PcgParams &prm, double rhs_norm, d
 // [2] constant 1 (always-open gate).
 enum { kFlagActive = 0, kFlagFresh = 1, kFlagOne = 2 };
 
+// The lane means the iteration kernels subtract (x's mean after its latest
+// update, folded into the next update; the residual mean the V-cycle
+// removes), zero for inactive lanes, and the flags.  Every pass that changes
+// a lane's state (start, warm start, update, fresh residual, limit) writes
+// them for that lane, so no separate pass recomputes them.
+struct LanePublish {
+  double *__restrict__ mean_x, *__restrict__ mean_r;
+  int *__restrict__ flags;
+  __device__ void put(int j, const ColState &s) const {
+    mean_x[j] = s.active ? s.mean_x : 0.0;
+    mean_r[j] = s.active ? s.mean_r : 0.0;
+  }
+};
+
 #define VK_GATE(gate) \
   if ((gate) && *(gate) == 0) return
 
@@ -112,19 +126,49 @@ struct PcgStart {
   static constexpr int kThreads = 64;
   ColState *__restrict__ st;
   PcgParams prm;
+  LanePublish pub;
   __device__ void operator()(Grid) const {
     const int j = threadIdx.x;
-    if (j >= K) return;
-    ColState &s = st[j];
-    s.init_norm = s.rhs_norm;
-    s.tol = stop_tolerance(prm, s.rhs_norm, 0.0);
-    s.mean_r = 0.0;
-    if (s.active && s.init_norm <= s.tol) {
-      s.active = 0;
-      s.iters = 0;
+    bool active = false;
+    if (j < K) {
+      ColState &s = st[j];
+      s.init_norm = s.rhs_norm;
+      s.tol = stop_tolerance(prm, s.rhs_norm, 0.0);
+      s.mean_r = 0.0;
+      if (s.active && s.init_norm <= s.tol) {
+        s.active = 0;
+        s.iters = 0;
+      }
+      pub.put(j, s);
+      active = s.active != 0;
+    }
+    const unsigned long long any = __ballot(active);
+    if (j == 0) {
+      pub.flags[kFlagActive] = any != 0ULL;
+      pub.flags[kFlagFresh] = 0;
     }
   }
 };
+
+// The flags after a warm start (whose lanes published their means).
+template <int K>
+struct PcgFlags {
+  static constexpr int kThreads = 64;
+  const ColState *__restrict__ st;
+  int *__restrict__ flags;
+  __device__ void operator()(Grid) const {
+    const int j = threadIdx.x;
+    const unsigned long long any = __ballot(j < K && st[j].active);
+    if (j == 0) {
+      flags[kFlagActive] = any != 0ULL;
+      flags[kFlagFresh] = 0;
+    }
+  }
+};
+template <int K>
+__global__ void k_pcg_flags(PcgFlags<K> f) {
+  f(Grid::hw());
+}
 template <int K>
 __global__ void k_pcg_start(PcgStart<K> f) {
   f(Grid::hw());
@@ -150,6 +194,7 @@ struct OpWarm : CsrOp<double, K, OpWarm<K>> {
     acc[2] += rr;
     acc[3] += rr * rr;
   }
+  LanePublish pub;
   template <int NV>
   __device__ void reduced(int j, const double (&tot)[NV]) const {
     ColState &s = st[j];
@@ -162,6 +207,7 @@ struct OpWarm : CsrOp<double, K, OpWarm<K>> {
       s.status = 64;
       s.active = 0;
     }
+    pub.put(j, s);
   }
 };
 
@@ -181,37 +227,6 @@ struct CenterCols {
 };
 template <int K>
 __global__ void k_center_cols(CenterCols<K> f) {
-  f(Grid::hw());
-}
-
-// Lane means of the active lanes (zero for finished lanes, whose iterates stay
-// frozen) and the any-lane flags.
-template <int K>
-struct StateMeans {
-  static constexpr int kThreads = 64;
-  const ColState *__restrict__ st;
-  double *__restrict__ mean_x, *__restrict__ mean_r;
-  int *__restrict__ flags;
-  const int *__restrict__ gate;
-  __device__ void operator()(Grid) const {
-    VK_GATE(gate);
-    const int j = threadIdx.x;
-    const bool lane = j < K;
-    const bool active = lane && st[j].active;
-    const bool fresh = active && st[j].need_fresh;
-    if (lane) {
-      mean_x[j] = active ? st[j].mean_x : 0.0;
-      mean_r[j] = active ? st[j].mean_r : 0.0;
-    }
-    const unsigned long long any_active = __ballot(active), any_fresh = __ballot(fresh);
-    if (j == 0) {
-      flags[kFlagActive] = any_active != 0ULL;
-      flags[kFlagFresh] = any_fresh != 0ULL;
-    }
-  }
-};
-template <int K>
-__global__ void k_state_means(StateMeans<K> f) {
   f(Grid::hw());
 }
 
@@ -356,19 +371,23 @@ struct FinUpdate {
   __device__ bool enabled() const { return true; }
   ColState *__restrict__ st;
   PcgParams prm;
+  LanePublish pub;
   __device__ void reduced(int j, const double (&tot)[4]) const {
     ColState &s = st[j];
     s.restarted = 0;
     s.need_fresh = 0;
-    if (!s.active) return;
-    const double n = (double)prm.n;
-    s.iters += 1;
-    s.mean_x = tot[0] / n;
-    s.xnorm = sqrt(fmax(tot[1] - n * s.mean_x * s.mean_x, 0.0));
-    s.mean_r = tot[2] / n;
-    s.rnorm = sqrt(tot[3]);
-    s.tol = stop_tolerance(prm, s.rhs_norm, s.xnorm);
-    s.need_fresh = (s.rnorm <= s.tol) || (s.iters % prm.interval == 0);
+    if (s.active) {
+      const double n = (double)prm.n;
+      s.iters += 1;
+      s.mean_x = tot[0] / n;
+      s.xnorm = sqrt(fmax(tot[1] - n * s.mean_x * s.mean_x, 0.0));
+      s.mean_r = tot[2] / n;
+      s.rnorm = sqrt(tot[3]);
+      s.tol = stop_tolerance(prm, s.rhs_norm, s.xnorm);
+      s.need_fresh = (s.rnorm <= s.tol) || (s.iters % prm.interval == 0);
+      if (s.need_fresh) pub.flags[kFlagFresh] = 1;
+    }
+    pub.put(j, s);
   }
 };
 
@@ -382,9 +401,9 @@ struct OpFresh : CsrOp<double, K, OpFresh<K>> {
   ColState *__restrict__ st;
   PcgParams prm;
   const int *__restrict__ flags;
-  const double *__restrict__ mean_x;  // per lane: x's mean after the latest update (0 for inactive lanes)
+  LanePublish pub;  // its mean_x: x's mean after the latest update (0 for inactive lanes)
   __device__ bool enabled() const { return flags[kFlagFresh] != 0; }
-  __device__ double lane(int j) const { return mean_x[j]; }
+  __device__ double lane(int j) const { return pub.mean_x[j]; }
   __device__ double self(uint32_t i, int j, double m) const { return x[(uint64_t)i * K + j] - m; }
   __device__ double nbr(uint32_t c, int j, double m) const { return x[(uint64_t)c * K + j] - m; }
   template <int NA>
@@ -407,21 +426,16 @@ struct OpFresh : CsrOp<double, K, OpFresh<K>> {
     s.fresh = sqrt(tot[0]);
     s.orig = sqrt(tot[1]);
     s.mean_r = tot[2] / (double)prm.n;
-    if (s.fresh <= s.tol) {
-      if (s.orig <= s.tol) {
-        s.active = 0;
-        return;
-      }
-      if (s.iters >= prm.max_iterations) {
-        s.status = 64;
-        s.active = 0;
-        return;
-      }
-    }
-    if (s.iters >= prm.max_iterations) {
+    if (s.fresh <= s.tol && s.orig <= s.tol) {
+      s.active = 0;
+    } else if (s.fresh <= s.tol && s.iters >= prm.max_iterations) {
+      s.status = 64;
+      s.active = 0;
+    } else if (s.iters >= prm.max_iterations) {
       s.status = 63;
       s.active = 0;
     }
+    pub.put(j, s);
   }
 };
 
@@ -431,7 +445,7 @@ struct PcgLimit {
   static constexpr int kThreads = 64;
   ColState *__restrict__ st;
   PcgParams prm;
-  int *__restrict__ flags;
+  LanePublish pub;
   __device__ void operator()(Grid) const {
     const int j = threadIdx.x;
     bool active = false;
@@ -441,10 +455,14 @@ struct PcgLimit {
         s.status = 63;
         s.active = 0;
       }
+      pub.put(j, s);
       active = s.active != 0;
     }
     const unsigned long long any = __ballot(active);
-    if (j == 0) flags[kFlagActive] = any != 0ULL;
+    if (j == 0) {
+      pub.flags[kFlagActive] = any != 0ULL;
+      pub.flags[kFlagFresh] = 0;
+    }
   }
 };
 template <int K>
