@@ -2084,7 +2084,12 @@ int32_t vckss_mi355x_batch_prepare_v1(uint64_t rows, const double *sample, const
         fail(err.code, "batch_prepare", "sample " + std::to_string(k) + ": " + err.what());
       }
     });
-    for (auto &bs : batch->samples) bs->dev.reset(new DeviceEngine(bs->P, bs->HS, bs->H));
+    for (auto &bs : batch->samples) {
+      bs->dev.reset(new DeviceEngine(bs->P, bs->HS, bs->H));
+      // as for a single problem: the built-in trained smoother weights and
+      // cycle shape, kept for a sample only where its calibration says they help
+      if (kBuiltinSmootherCount == kSmootherParams) bs->dev->smoother_default(builtin_smoother());
+    }
     *samples = (uint32_t)batch->samples.size();
     *handle = g_next_batch++;
     g_batches[*handle] = std::move(batch);
