@@ -423,7 +423,8 @@ void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vecto
 // return at once; in a lockstep sample batch the call joins the others'.
 template <int K>
 void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
-               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0) {
+               bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0,
+               bool states = true) {
   VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
   PcgCall<K> c(E, b, rel, abs_tol, max_it, interval, fp32, cap);
   if (E.lockstep) {
@@ -490,10 +491,27 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   }
   E.gate = E.always();
   c.center();
+  if (!states) return;  // the caller reads the lane states later (read_batch)
   std::vector<dev::ColState> host(K);
   VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
   VK_HIP(hipStreamSynchronize(E.stream));
   out = host;
+}
+
+// The lane states of the last PCG call and the gate figures of the last
+// extraction, with one synchronisation.
+template <int K>
+void read_batch(Impl &E, std::vector<dev::ColState> &st, std::vector<double> &red, std::vector<double> &comp,
+                std::vector<double> &rn) {
+  VK_HIP(hipMemcpyAsync(E.host_states, E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
+  VK_HIP(hipMemcpyAsync(E.host_gates, E.reduced_res.p, K * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  VK_HIP(hipMemcpyAsync(E.host_gates + 64, E.complete_res.p, K * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  VK_HIP(hipMemcpyAsync(E.host_gates + 128, E.rhs_norm.p, K * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  VK_HIP(hipStreamSynchronize(E.stream));
+  st.assign(E.host_states, E.host_states + K);
+  red.assign(E.host_gates, E.host_gates + K);
+  comp.assign(E.host_gates + 64, E.host_gates + 64 + K);
+  rn.assign(E.host_gates + 128, E.host_gates + 128 + K);
 }
 
 // full_cmg.rs:1158-1340 extraction for K lanes: normalise the firm mean, the
@@ -556,8 +574,14 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   // below are unchanged and still decide.  E.first_pass_cap = false keeps the
   // reference's stop.
   const double cap = E.first_pass_cap ? 0.1 * gate : 0.0;
+  // the extraction is queued behind the PCG call before the host reads the
+  // lane states, so the batch synchronises once (a failed lane's extraction is
+  // discarded with the batch)
   pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
-               opt.precision == 32, use, st, cap);
+               opt.precision == 32, use, st, cap, false);
+  extract<K>(E);
+  std::vector<double> red, comp, rn;
+  read_batch<K>(E, st, red, comp, rn);
   receipts.assign(columns, ColumnReceipt{});
   std::vector<double> first_norm(K, 0.0);
   for (uint32_t j = 0; j < columns; ++j) {
@@ -566,14 +590,6 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
     receipts[j].iterations = st[j].iters;
     receipts[j].restarts = st[j].restarts;
   }
-  extract<K>(E);
-  std::vector<double> red, comp, rn;
-  auto read_gate = [&]() {
-    E.reduced_res.download(red, K, E.stream);
-    E.complete_res.download(comp, K, E.stream);
-    E.rhs_norm.download(rn, K, E.stream);
-  };
-  read_gate();
   static const double factors[3] = {0.1, 0.01, 0.001};
   for (int round = 0; round < 3; ++round) {
     std::vector<uint8_t> fail_lane(K, 0);
@@ -602,15 +618,15 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
     E.refine_rounds++;
     for (uint32_t j = 0; j < columns; ++j) E.refined_columns += fail_lane[j];
     pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true, opt.precision == 32,
-                 fail_lane, st);
+                 fail_lane, st, 0.0, false);
+    extract<K>(E);
+    read_batch<K>(E, st, red, comp, rn);
     for (uint32_t j = 0; j < columns; ++j)
       if (fail_lane[j]) {
         if (st[j].status) fail(st[j].status, "cmg_full_v2_refinement", "refinement PCG failed");
         receipts[j].iterations += st[j].iters;
         receipts[j].restarts += st[j].restarts;
       }
-    extract<K>(E);
-    read_gate();
   }
   for (uint32_t j = 0; j < columns; ++j) {
     if (!std::isfinite(comp[j]) || comp[j] > gate || !std::isfinite(red[j]) || red[j] > gate)
@@ -1430,6 +1446,8 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   VK_HIP(hipMemcpy(E.flags.p, init_flags, sizeof init_flags, hipMemcpyHostToDevice));
   E.gate = E.always();
   VK_HIP(hipHostMalloc((void **)&E.host_flags, kPollSlots * sizeof(int), hipHostMallocDefault));
+  VK_HIP(hipHostMalloc((void **)&E.host_states, 64 * sizeof(dev::ColState), hipHostMallocDefault));
+  VK_HIP(hipHostMalloc((void **)&E.host_gates, 3 * 64 * sizeof(double), hipHostMallocDefault));
   for (auto &ev : E.poll_events) VK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   E.st.alloc(64); E.use.alloc(64);
   E.lane_a.alloc(64); E.lane_b.alloc(64); E.lane_c.alloc(64);
@@ -1447,6 +1465,8 @@ DeviceEngine::~DeviceEngine() {
     for (auto &ev : impl_->poll_events)
       if (ev) (void)hipEventDestroy(ev);
     if (impl_->host_flags) (void)hipHostFree(impl_->host_flags);
+    if (impl_->host_states) (void)hipHostFree(impl_->host_states);
+    if (impl_->host_gates) (void)hipHostFree(impl_->host_gates);
     (void)hipStreamDestroy(impl_->stream);
   }
 }

@@ -345,6 +345,11 @@ struct DeviceEngine::Impl {
   DBuf<int> flags;
   const int *gate = nullptr;
   int *host_flags = nullptr;
+  // pinned host copies of a batch's lane states and gate figures (reduced
+  // residual, complete residual, right-hand side norm per lane), read with one
+  // synchronisation after the extraction
+  dev::ColState *host_states = nullptr;
+  double *host_gates = nullptr;
   hipEvent_t poll_events[kPollSlots] = {};
   uint32_t cap = 0;
   std::vector<LevelVecs<double>> lv64;
