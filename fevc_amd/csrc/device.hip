@@ -1023,10 +1023,11 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
         constexpr int K = decltype(kk)::value;
         hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W,
                            E.worker_ptr.p, (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p,
-                           E.G, k, E.wrhs.p);
+                           E.G, k, E.wrhs.p,
+                         E.cell_unit.p);
         segments<double, K, 0>(E, E.firm_chunks,
                                dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G,
-                                                   k, E.frhs.p});
+                                                   k, E.frhs.p, E.firm_units.p});
         num.attempted_replay_rhs += k;
         try {
           solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
@@ -1414,6 +1415,17 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.cell_worker.upload(p.cell_worker); E.cell_firm.upload(p.cell_firm); E.cell_weight.upload(p.cell_weight);
   E.wdiag.upload(wd); E.fdiag.upload(fd);
   E.cell_unit_ptr.upload(p.cell_unit_ptr); E.cell_units.upload(p.cell_units);
+  {
+    bool single = true;
+    for (uint64_t c = 0; c < E.C && single; ++c) single = p.cell_unit_ptr[c + 1] - p.cell_unit_ptr[c] == 1;
+    if (single && E.C) {
+      std::vector<uint32_t> fu(E.C);
+      for (uint64_t k = 0; k < E.C; ++k) fu[k] = p.cell_units[p.cell_unit_ptr[p.firm_cells[k]]];
+      E.firm_units.upload(fu);
+      for (uint64_t c = 0; c < E.C; ++c) fu[c] = p.cell_units[p.cell_unit_ptr[c]];
+      E.cell_unit.upload(fu);
+    }
+  }
   E.unit_cell.upload(p.unit_cell); E.unit_count.upload(p.unit_count); E.unit_rank.upload(p.unit_rank);
   E.unit_outcome.upload(p.unit_outcome);
   E.Gm = p.match_units;
@@ -1569,10 +1581,11 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
-                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p);
+                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p,
+                         E.cell_unit.p);
       segments<double, K, 0>(E, E.firm_chunks,
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
-                                                 E.frhs.p});
+                                                 E.frhs.p, E.firm_units.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
       if (E.Gm)
         hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.Gm * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Gm, k,
@@ -2114,10 +2127,11 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
-                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p);
+                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p,
+                         E.cell_unit.p);
       segments<double, K, 0>(E, E.firm_chunks,
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
-                                                 E.frhs.p});
+                                                 E.frhs.p, E.firm_units.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
       hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
                          E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,

@@ -320,6 +320,9 @@ struct DeviceEngine::Impl {
   double target_total = 0, bound = 0;
   DBuf<uint64_t> worker_ptr, firm_ptr, cell_unit_ptr, cell_stratum_ptr, aux_ptr;
   DBuf<uint32_t> firm_cells, cell_worker, cell_firm, cell_units, unit_cell, aux_nbr;
+  // when every cell is one unit (match deletion), else empty: the unit of
+  // each cell in firm_cells order
+  DBuf<uint32_t> firm_units, cell_unit;  // ... and of each cell in cell order
   DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
   DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;
   // observation units (null without them): per unit the first copy in its

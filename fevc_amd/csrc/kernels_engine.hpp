@@ -561,11 +561,14 @@ __device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV]
 
 // Exact-integer leverage right-hand sides (engine.rs:2562-2619): per worker
 // and per firm, the sum of the atoms of the units of its cells.
+// With `cunit` (every cell one unit: the unit of each cell) the unit is read
+// directly instead of through the cell's unit range.
 template <int K>
 __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
                            const uint32_t *__restrict__ cells, const uint64_t *__restrict__ cu_ptr,
                            const uint32_t *__restrict__ cu, const int32_t *__restrict__ atoms,
-                           uint64_t units, uint32_t kcols, double *__restrict__ out) {
+                           uint64_t units, uint32_t kcols, double *__restrict__ out,
+                           const uint32_t *__restrict__ cunit = nullptr) {
   constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
   VK_GRID_LOOP(t, rows * TPR) {
     const uint64_t i = t / TPR;
@@ -575,6 +578,12 @@ __global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
     for (int q = 0; q < LP; ++q) s[q] = 0.0;
     for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
       const uint32_t c = cells ? cells[k] : (uint32_t)k;
+      if (cunit) {
+        const int32_t *a = atoms + (uint64_t)cunit[c] * K + j0;
+#pragma unroll
+        for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
+        continue;
+      }
       for (uint64_t u = cu_ptr[c]; u < cu_ptr[c + 1]; ++u) {
         const int32_t *a = atoms + (uint64_t)cu[u] * K + j0;
 #pragma unroll
@@ -716,8 +725,37 @@ struct OpFirmUnits {
   uint64_t units;
   uint32_t kcols;
   double *__restrict__ out;
+  // when every cell is one unit: the unit of each firm-ordered cell, so a
+  // wavefront that owns a chunk loads its units with one coalesced load and
+  // keeps 16 atom-row gathers in flight (the same integer sums in the same
+  // order as the cell-by-cell walk)
+  const uint32_t *__restrict__ funits = nullptr;
   template <int STEP = Shape<K>::V>
   __device__ double partial(const uint4 &d, int j, int s) const {
+    if constexpr (Shape<K>::G == 64 && Shape<K>::V == 1 && STEP == 1) {
+      if (funits) {
+        constexpr int kStage = 16;
+        const uint32_t lane = threadIdx.x & 63u;
+        double acc = 0.0;
+        for (uint32_t base = d.y; base < d.z; base += 64) {
+          const uint32_t cnt = d.z - base < 64 ? d.z - base : 64;  // wave-uniform
+          const uint32_t mu = lane < cnt ? funits[base + lane] : 0u;
+          for (uint32_t r = 0; r < cnt; r += kStage) {
+            int32_t g[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+              const uint32_t idx = r + u < cnt ? r + u : 0u;
+              const uint32_t unit = (uint32_t)__builtin_amdgcn_readlane((int)mu, (int)idx);
+              g[u] = atoms[(uint64_t)unit * K + j];
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u)
+              if (r + u < cnt) acc += (double)g[u];
+          }
+        }
+        return j < (int)kcols ? acc : 0.0;
+      }
+    }
     double acc = 0.0;
     if (j >= (int)kcols) return acc;
     for (uint32_t k = d.y + s; k < d.z; k += STEP) {
