@@ -533,7 +533,7 @@ void extract(Impl &E) {
   hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * dev::Pack<K>::TPR), kBlock, 0, E.stream, W, E.worker_ptr.p,
                      E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
   segments<double, K, 1>(E, E.firm_chunks,
-                         dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.red.p,
+                         dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.fdiag.p, E.red.p,
                                               E.wsum.p, E.sout.p, E.lane_b.p, (uint32_t)F});
   hipLaunchKernelGGL((dev::k_reduced_residual<K>), fg, kBlock, 0, E.stream, (uint32_t)F, E.sch.p, E.sout.p,
                      E.lane_b.p, E.rpartial.p);
@@ -542,7 +542,7 @@ void extract(Impl &E) {
                      E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p, E.worker.p, E.rpartial.p);
   finish<K, 2>(E, E.rpartial.p, wg, dev::FinSums<K>{E.sums.p});
   segments<double, K, 2>(E, E.firm_chunks,
-                         dev::OpFirmResidual<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.fdiag.p, E.frhs.p,
+                         dev::OpFirmResidual<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.fdiag.p, E.frhs.p,
                                                 E.fsol.p, E.worker.p, E.sums.p, E.complete_res.p, E.rhs_norm.p});
 }
 
@@ -556,7 +556,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   const uint64_t m0 = E.n0 * K;
   const double gate = std::max(10.0 * effective_tol, 1e-11);
   segments<double, K, 1>(E, E.firm_chunks,
-                         dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p}, E.wdiag.p, E.wrhs.p,
+                         dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.wdiag.p, E.wrhs.p,
                                             E.frhs.p, E.sch.p, E.lane_a.p, (uint32_t)F});
   run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.sch.p, E.lane_a.p, nullptr});
   hipLaunchKernelGGL((dev::k_hybrid_rhs<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, (uint32_t)F, E.perm0.p,
@@ -1415,6 +1415,16 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
   E.cell_worker.upload(p.cell_worker); E.cell_firm.upload(p.cell_firm); E.cell_weight.upload(p.cell_weight);
   E.wdiag.upload(wd); E.fdiag.upload(fd);
   E.cell_unit_ptr.upload(p.cell_unit_ptr); E.cell_units.upload(p.cell_units);
+  {
+    std::vector<uint32_t> fw(E.C);
+    std::vector<double> fwt(E.C);
+    for (uint64_t k = 0; k < E.C; ++k) {
+      fw[k] = p.cell_worker[p.firm_cells[k]];
+      fwt[k] = p.cell_weight[p.firm_cells[k]];
+    }
+    E.firm_worker.upload(fw);
+    E.firm_weight.upload(fwt);
+  }
   {
     bool single = true;
     for (uint64_t c = 0; c < E.C && single; ++c) single = p.cell_unit_ptr[c + 1] - p.cell_unit_ptr[c] == 1;

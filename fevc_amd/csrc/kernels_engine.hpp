@@ -698,9 +698,43 @@ template <int K, class D>
 struct FirmCellOp {
   const uint32_t *__restrict__ fcells, *__restrict__ cell_worker;
   const double *__restrict__ cell_weight;
+  // the worker and weight of each cell in fcells order: a wavefront that
+  // owns a chunk loads them with one coalesced load and keeps 16 worker-row
+  // gathers in flight, summing in the cell-by-cell walk's order
+  const uint32_t *__restrict__ fworker = nullptr;
+  const double *__restrict__ fweight = nullptr;
   template <int STEP = Shape<K>::V>
   __device__ double partial(const uint4 &d, int j, int s) const {
     const D &op = static_cast<const D &>(*this);
+    if constexpr (Shape<K>::G == 64 && Shape<K>::V == 1 && STEP == 1) {
+      if (fworker) {
+        constexpr int kStage = 16;
+        const uint32_t lane = threadIdx.x & 63u;
+        double acc = 0.0;
+        for (uint32_t base = d.y; base < d.z; base += 64) {
+          const uint32_t cnt = d.z - base < 64 ? d.z - base : 64;  // wave-uniform
+          uint32_t mw = 0;
+          double mv = 0.0;
+          if (lane < cnt) {
+            mw = fworker[base + lane];
+            mv = fweight[base + lane];
+          }
+          for (uint32_t r = 0; r < cnt; r += kStage) {
+            double g[kStage], wt[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+              const uint32_t idx = r + u < cnt ? r + u : 0u;
+              wt[u] = lane_value<true>(mv, idx);
+              g[u] = op.term(lane_value<true>(mw, idx), j);
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u)
+              if (r + u < cnt) acc += wt[u] * g[u];
+          }
+        }
+        return acc;
+      }
+    }
     double acc = 0.0;
 #pragma unroll 4
     for (uint32_t k = d.y + s; k < d.z; k += STEP) {
