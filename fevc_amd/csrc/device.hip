@@ -1424,6 +1424,13 @@ DeviceEngine::DeviceEngine(const Prepared &p, const HybridSystem &hs, const Hier
     }
     E.firm_worker.upload(fw);
     E.firm_weight.upload(fwt);
+    std::vector<uint32_t> uw(p.units), uf(p.units);
+    for (uint64_t g = 0; g < p.units; ++g) {
+      uw[g] = p.cell_worker[p.unit_cell[g]];
+      uf[g] = p.cell_firm[p.unit_cell[g]];
+    }
+    E.unit_worker.upload(uw);
+    E.unit_firm.upload(uf);
   }
   {
     bool single = true;
@@ -1599,8 +1606,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
       if (E.Gm)
         hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.Gm * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Gm, k,
-                           E.unit_cell.p, E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p,
-                           E.atoms.p, E.moments.p, E.sink());
+                           (const uint32_t *)nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p,
+                           E.fsol.p, E.atoms.p, E.moments.p, E.sink());
       if (E.Go)  // generic_jla.rs:5540-5892, units [Gm, G)
         hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.Go * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Go,
                            k, first, opt.seed, kTagLeverage, E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm,
@@ -2143,9 +2150,9 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
                                                  E.frhs.p, E.firm_units.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw, E.unit_cell.p,
-                         E.unit_count.p, E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.atoms.p,
-                         E.moments.p, E.sink());
+      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw,
+                         (const uint32_t *)nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p,
+                         E.fsol.p, E.atoms.p, E.moments.p, E.sink());
     });
     for (auto &c : rc) iterations += c.iterations;
     solves += lw;

@@ -323,8 +323,9 @@ struct DeviceEngine::Impl {
   // when every cell is one unit (match deletion), else empty: the unit of
   // each cell in firm_cells order
   DBuf<uint32_t> firm_units, cell_unit;  // ... and of each cell in cell order
-  // the worker and the weight of each cell in firm_cells order
-  DBuf<uint32_t> firm_worker;
+  // the worker and the weight of each cell in firm_cells order; the worker
+  // and the firm of each unit
+  DBuf<uint32_t> firm_worker, unit_worker, unit_firm;
   DBuf<double> firm_weight;
   DBuf<double> cell_weight, wdiag, fdiag, stratum_per_copy, cell_target, unit_outcome, aux_w;
   DBuf<uint64_t> unit_rank, unit_count, stratum_rank, stratum_count;

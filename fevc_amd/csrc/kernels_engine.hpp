@@ -903,6 +903,8 @@ __device__ inline void neumaier_add(double &s, double &c, double v) {
   s = next;
 }
 
+// With unit_cell null, cell_worker and cell_firm are indexed by unit (the
+// unit's worker and firm, precomputed for match units).
 template <int K>
 __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__restrict__ unit_cell,
                           const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ cell_worker,
@@ -912,42 +914,61 @@ __global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__rest
   // TPR threads per unit, LP lanes each (packed loads); the batch's five sums
   // are combined in-thread, then by a fixed xor butterfly over the unit's
   // threads, and added to the unit's running compensated sums by its first
-  // thread, batches in probe order.
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
-  VK_GRID_LOOP(t, units * TPR) {
-    const uint64_t g = t / TPR;
-    const int j0 = (int)(t % TPR) * LP;
-    const uint32_t c = unit_cell[g];
-    const uint64_t w = cell_worker[c], f = cell_firm[c];
-    const double root = sqrt((double)unit_count[g]);
-    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // thread, batches in probe order.  A thread takes two units per step, their
+  // loads issued together.
+  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR, U = 2;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, total = units * TPR;
+  for (uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t0 < total; t0 += U * stride) {
+    const int j0 = (int)(t0 % TPR) * LP;  // the same for both units (stride is a multiple of TPR)
+    uint64_t g[U];
+    bool on[U];
+    double wv[U][LP], fv[U][LP], av[U][LP], root[U];
 #pragma unroll
-    for (int q = 0; q < LP; ++q) {
-      const int j = j0 + q;
-      if (j < (int)kcols) {
-        const double proj = root * (worker[w * K + j] + firm[f * K + j]);
-        const double res = (double)atoms[g * K + j] / root - proj;
-        if (!isfinite(proj) || !isfinite(res)) report(status, g, 73);
-        const double ps = proj * proj, rs = res * res;
-        v[0] += ps;
-        v[1] += rs;
-        v[2] += ps * ps;
-        v[3] += rs * rs;
-        v[4] += ps * rs;
+    for (int q = 0; q < U; ++q) {
+      const uint64_t t = t0 + q * stride;
+      on[q] = t < total;
+      g[q] = on[q] ? t / TPR : t0 / TPR;
+      const uint64_t c = unit_cell ? unit_cell[g[q]] : g[q];
+      const uint64_t w = cell_worker[c], f = cell_firm[c];
+      root[q] = sqrt((double)unit_count[g[q]]);
+#pragma unroll
+      for (int l = 0; l < LP; ++l) {
+        const int j = j0 + l;
+        wv[q][l] = worker[w * K + j];
+        fv[q][l] = firm[f * K + j];
+        av[q][l] = (double)atoms[g[q] * K + j];
       }
     }
 #pragma unroll
-    for (int off = 1; off < TPR; off <<= 1) {
+    for (int q = 0; q < U; ++q) {
+      double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < 5; ++q) v[q] += __shfl_xor(v[q], off);
-    }
-    if (j0 == 0) {
+      for (int l = 0; l < LP; ++l) {
+        if (j0 + l < (int)kcols) {
+          const double proj = root[q] * (wv[q][l] + fv[q][l]);
+          const double res = av[q][l] / root[q] - proj;
+          if (on[q] && (!isfinite(proj) || !isfinite(res))) report(status, g[q], 73);
+          const double ps = proj * proj, rs = res * res;
+          v[0] += ps;
+          v[1] += rs;
+          v[2] += ps * ps;
+          v[3] += rs * rs;
+          v[4] += ps * rs;
+        }
+      }
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        double s = m[(uint64_t)(2 * q) * units + g], e = m[(uint64_t)(2 * q + 1) * units + g];
-        neumaier_add(s, e, v[q]);
-        m[(uint64_t)(2 * q) * units + g] = s;
-        m[(uint64_t)(2 * q + 1) * units + g] = e;
+      for (int off = 1; off < TPR; off <<= 1) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) v[r] += __shfl_xor(v[r], off);
+      }
+      if (j0 == 0 && on[q]) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+          double s = m[(uint64_t)(2 * r) * units + g[q]], e = m[(uint64_t)(2 * r + 1) * units + g[q]];
+          neumaier_add(s, e, v[r]);
+          m[(uint64_t)(2 * r) * units + g[q]] = s;
+          m[(uint64_t)(2 * r + 1) * units + g[q]] = e;
+        }
       }
     }
   }
