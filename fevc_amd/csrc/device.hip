@@ -302,7 +302,8 @@ struct PcgCall {
           double cap)
       : E(e), b(rhs), prm{(uint32_t)e.n0, rel, abs_tol, e.bound, (int32_t)max_it, (int32_t)interval, cap}, fp32(f32),
         L0(e.levels[0]), rows0(grouped<K>() ? e.levels[0].rows_g : e.levels[0].rows),
-        rg(reduce_grid((uint64_t)e.n0 * K)), flags(e.flags.p), active(e.flags.p + dev::kFlagActive) {}
+        rg(reduce_grid((uint64_t)e.n0 * K, dev::kKrylovReduceBlocks)), flags(e.flags.p),
+        active(e.flags.p + dev::kFlagActive) {}
 
   void setup(bool warm) {
     const uint32_t n = E.n0;
@@ -319,7 +320,8 @@ struct PcgCall {
       finish<K, 1>(E, E.rpartial.p, rg, dev::FinMean{E.lane_a.p, E.use.p, n});
       run(E, dev::k_center_cols<K>, grid_for(m0), dev::CenterCols<K>{(uint64_t)n, E.x.p, E.lane_a.p, nullptr});
       segments<double, K, 4>(E, rows0,
-                             dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, pub()});
+                             dev::OpWarm<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, pub()},
+                             nullptr, nullptr, dev::kKrylovReduceBlocks);
       run(E, dev::k_pcg_flags<K>, 1, dev::PcgFlags<K>{E.st.p, flags});
     }
     E.gate = active;
@@ -355,12 +357,12 @@ struct PcgCall {
       segments<double, K, 1>(E, rows0,
                              dev::OpDirectionCurvature<K, float>{{L0.col.p, L0.val.p}, zcur.z32, E.p.p, E.ap.p,
                                                                  E.st.p},
-                             ev.first, ev.second);
+                             ev.first, ev.second, dev::kKrylovReduceBlocks);
     else
       segments<double, K, 1>(E, rows0,
                              dev::OpDirectionCurvature<K, double>{{L0.col.p, L0.val.p}, zcur.z64, E.p.p, E.ap.p,
                                                                   E.st.p},
-                             ev.first, ev.second);
+                             ev.first, ev.second, dev::kKrylovReduceBlocks);
     // x is centred inside the next update (its mean after this one is kept
     // in the lane state) and on the fly by the fresh residual
     run(E, dev::k_pcg_update<K>, rg,
@@ -368,7 +370,8 @@ struct PcgCall {
     finish<K, 4>(E, E.rpartial.p, rg, dev::FinUpdate{E.st.p, prm, pub()}, active);
     segments<double, K, 3>(E, rows0,
                            dev::OpFresh<K>{{L0.col.p, L0.val.p}, b, E.proj.p, E.x.p, E.r.p, E.st.p, prm, flags,
-                                           pub()});
+                                           pub()},
+                           nullptr, nullptr, dev::kKrylovReduceBlocks);
     run(E, dev::k_pcg_limit<K>, 1, dev::PcgLimit<K>{E.st.p, prm, pub()});
   }
 

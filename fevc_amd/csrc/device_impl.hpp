@@ -107,7 +107,7 @@ inline unsigned grid_for(uint64_t threads, unsigned cap = VK_GRID_CAP) {
   g = std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
   return static_cast<unsigned>((g + 7) / 8 * 8);
 }
-inline unsigned reduce_grid(uint64_t threads) { return grid_for(threads, dev::kMaxReduceBlocks); }
+inline unsigned reduce_grid(uint64_t threads, unsigned cap = dev::kMaxReduceBlocks) { return grid_for(threads, cap); }
 
 template <typename F>
 inline void with_k(uint32_t k, F &&f) {
@@ -588,9 +588,9 @@ void finish(Impl &E, const double *partial, unsigned items, const Fin &fin, cons
 // runs, so samples with and without multi-chunk segments line up).
 template <typename T, int K, int NV, class Op>
 void segments_in(Impl &E, const DevChunks &ch, const Op &op, const dev::SegScratch &sc,
-                 hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
+                 hipEvent_t before = nullptr, hipEvent_t after = nullptr, unsigned cap = dev::kMaxReduceBlocks) {
   constexpr int G = dev::Shape<K>::G;
-  const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G) : grid_for((uint64_t)ch.count * G);
+  const unsigned grid1 = NV > 0 ? reduce_grid((uint64_t)ch.count * G, cap) : grid_for((uint64_t)ch.count * G);
   const unsigned grid2 = std::max(1u, std::min<unsigned>(ch.n_multi, dev::kMaxReduceBlocks));
   const Nest nest(E);
   if (before) VK_HIP(hipEventRecord(before, E.stream));
@@ -609,8 +609,9 @@ void segments_in(Impl &E, const DevChunks &ch, const Op &op, const dev::SegScrat
 
 // The same with the engine's own scratch (chunk partials and block partials).
 template <typename T, int K, int NV, class Op>
-void segments(Impl &E, const DevChunks &ch, const Op &op, hipEvent_t before = nullptr, hipEvent_t after = nullptr) {
-  segments_in<T, K, NV, Op>(E, ch, op, E.scratch(), before, after);
+void segments(Impl &E, const DevChunks &ch, const Op &op, hipEvent_t before = nullptr, hipEvent_t after = nullptr,
+              unsigned cap = dev::kMaxReduceBlocks) {
+  segments_in<T, K, NV, Op>(E, ch, op, E.scratch(), before, after, cap);
 }
 
 }  // namespace gpu

@@ -45,6 +45,10 @@ namespace dev {
 
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 4096;
+// block cap of the reductions of the PCG iteration (update, rho, direction
+// and A p, fresh residual): fewer, longer blocks measured faster there (the
+// finishes read fewer partials), while the per-batch passes keep the larger cap
+constexpr int kKrylovReduceBlocks = 2048;
 constexpr int kMaxReduceValues = 6;
 // Block partials: value v of lane j from block slot b sits at
 // [(v * K + j) * kPartialStride + b], so the final per-lane sum reads
