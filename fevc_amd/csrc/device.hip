@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 
@@ -387,12 +388,14 @@ struct PcgCall {
 // the cycle's alternating buffers: two cycles swap every level's buffers an
 // even number of times) and run them with the other samples' waiting calls.
 template <int K>
-void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vector<dev::ColState> &out) {
+void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vector<dev::ColState> &out,
+                  const std::function<void()> *head, const std::function<void()> *tail) {
   PcgProgram prog;
   Recorder rec;
   const uint64_t cycles = E.vcycles;
   E.rec = &rec;
   try {
+    if (head) (*head)();
     c.setup(warm);
     prog.setup = rec.take();
     for (int parity = 0; parity < 2; ++parity) {
@@ -401,6 +404,10 @@ void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vecto
       c.body();
       prog.body[parity] = rec.take();
     }
+    E.gate = E.always();
+    c.center();
+    if (tail) (*tail)();
+    prog.tail = rec.take();
   } catch (...) {
     E.rec = nullptr;
     throw;
@@ -413,7 +420,6 @@ void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vecto
   prog.lanes = K;
   prog.max_iterations = max_it;
   lockstep_submit(*E.lockstep, prog);
-  c.center();  // on the engine's stream, which the batch has released
   E.vcycles += prog.cycles;
   E.iterations_launched += prog.iterations;
   E.gate = E.always();
@@ -424,16 +430,20 @@ void pcg_lockstep(Impl &E, PcgCall<K> &c, bool warm, uint32_t max_it, std::vecto
 // holds the initial guesses.  Alone, the host polls the any-lane-active flag
 // with a lag of kPollLag iterations and iterations queued after convergence
 // return at once; in a lockstep sample batch the call joins the others'.
+// `head` enqueues passes that produce b before the call; `tail` passes that
+// follow it (after x is centred).
 template <int K>
 void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t max_it, uint32_t interval, bool warm,
                bool fp32, const std::vector<uint8_t> &use, std::vector<dev::ColState> &out, double cap = 0.0,
-               bool states = true) {
+               bool states = true, const std::function<void()> *head = nullptr,
+               const std::function<void()> *tail = nullptr) {
   VK_HIP(hipMemcpyAsync(E.use.p, use.data(), K, hipMemcpyHostToDevice, E.stream));
   PcgCall<K> c(E, b, rel, abs_tol, max_it, interval, fp32, cap);
   if (E.lockstep) {
-    pcg_lockstep<K>(E, c, warm, max_it, out);
+    pcg_lockstep<K>(E, c, warm, max_it, out, head, tail);
     return;
   }
+  if (head) (*head)();
   c.setup(warm);
   const dev::PcgParams &prm = c.prm;
   auto precondition = [&]() { c.precondition(); };
@@ -494,6 +504,7 @@ void pcg_lanes(Impl &E, const double *b, double rel, double abs_tol, uint32_t ma
   }
   E.gate = E.always();
   c.center();
+  if (tail) (*tail)();
   if (!states) return;  // the caller reads the lane states later (read_batch)
   std::vector<dev::ColState> host(K);
   VK_HIP(hipMemcpyAsync(host.data(), E.st.p, K * sizeof(dev::ColState), hipMemcpyDeviceToHost, E.stream));
@@ -523,26 +534,26 @@ void read_batch(Impl &E, std::vector<dev::ColState> &st, std::vector<double> &re
 template <int K>
 void extract(Impl &E) {
   const uint64_t F = E.F, W = E.W;
-  hipLaunchKernelGGL((dev::k_gather_rows<K>), grid_for(F * K), kBlock, 0, E.stream, F, E.pos0.p, E.x.p, E.fsol.p);
+  run(E, dev::k_gather_rows<K>, grid_for(F * K), dev::GatherRows<K>{F, E.pos0.p, E.x.p, E.fsol.p});
   const unsigned fg = reduce_grid(F * K), wg = reduce_grid(W * dev::Pack<K>::TPR);
   run(E, dev::k_lane_sum<K>, fg, dev::LaneSum<K>{(uint64_t)(F), E.fsol.p, E.rpartial.p});
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   run(E, dev::k_center_cols<K>, grid_for(E.n0 * K), dev::CenterCols<K>{(uint64_t)(E.n0), E.x.p, E.lane_a.p, nullptr});
   run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.fsol.p, E.lane_a.p, nullptr});
-  VK_HIP(hipMemcpyAsync(E.red.p, E.fsol.p, F * K * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
+  run(E, dev::k_copy_doubles, grid_for(F * K), dev::CopyDoubles{F * K, E.fsol.p, E.red.p});
   run(E, dev::k_lane_sum<K>, fg, dev::LaneSum<K>{(uint64_t)(F), E.red.p, E.rpartial.p});
   finish<K, 1>(E, E.rpartial.p, fg, dev::FinMean{E.lane_a.p, nullptr, F});
   run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.red.p, E.lane_a.p, nullptr});
-  hipLaunchKernelGGL((dev::k_worker_gather<K>), grid_for(W * dev::Pack<K>::TPR), kBlock, 0, E.stream, W, E.worker_ptr.p,
-                     E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p);
+  run(E, dev::k_worker_gather<K>, grid_for(W * dev::Pack<K>::TPR),
+      dev::WorkerGather<K>{W, E.worker_ptr.p, E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.red.p, E.wsum.p});
   segments<double, K, 1>(E, E.firm_chunks,
                          dev::OpSchurApply<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.fdiag.p, E.red.p,
                                               E.wsum.p, E.sout.p, E.lane_b.p, (uint32_t)F});
-  hipLaunchKernelGGL((dev::k_reduced_residual<K>), fg, kBlock, 0, E.stream, (uint32_t)F, E.sch.p, E.sout.p,
-                     E.lane_b.p, E.rpartial.p);
+  run(E, dev::k_reduced_residual<K>, fg, dev::ReducedResidual<K>{(uint32_t)F, E.sch.p, E.sout.p, E.lane_b.p, E.rpartial.p});
   finish<K, 2>(E, E.rpartial.p, fg, dev::FinRelative{E.reduced_res.p});
-  hipLaunchKernelGGL((dev::k_worker_solution<K>), wg, kBlock, 0, E.stream, W, E.worker_ptr.p, E.cell_firm.p,
-                     E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p, E.worker.p, E.rpartial.p);
+  run(E, dev::k_worker_solution<K>, wg,
+      dev::WorkerSolution<K>{W, E.worker_ptr.p, E.cell_firm.p, E.cell_weight.p, E.wdiag.p, E.wrhs.p, E.fsol.p,
+                             E.worker.p, E.rpartial.p});
   finish<K, 2>(E, E.rpartial.p, wg, dev::FinSums<K>{E.sums.p});
   segments<double, K, 2>(E, E.firm_chunks,
                          dev::OpFirmResidual<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.fdiag.p, E.frhs.p,
@@ -558,12 +569,18 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   const uint64_t F = E.F;
   const uint64_t m0 = E.n0 * K;
   const double gate = std::max(10.0 * effective_tol, 1e-11);
-  segments<double, K, 1>(E, E.firm_chunks,
-                         dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p, E.firm_weight.p}, E.wdiag.p, E.wrhs.p,
-                                            E.frhs.p, E.sch.p, E.lane_a.p, (uint32_t)F});
-  run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.sch.p, E.lane_a.p, nullptr});
-  hipLaunchKernelGGL((dev::k_hybrid_rhs<K>), grid_for(m0), kBlock, 0, E.stream, E.n0, (uint32_t)F, E.perm0.p,
-                     E.sch.p, E.b0.p);
+  // the Schur right-hand side into the hybrid system (before the first PCG
+  // call), and the extraction after every PCG call: in a lockstep sample
+  // batch both join the call's multi-sample launches
+  const std::function<void()> head = [&]() {
+    segments<double, K, 1>(E, E.firm_chunks,
+                           dev::OpSchurRhs<K>{{E.firm_cells.p, E.cell_worker.p, E.cell_weight.p, E.firm_worker.p,
+                                               E.firm_weight.p},
+                                              E.wdiag.p, E.wrhs.p, E.frhs.p, E.sch.p, E.lane_a.p, (uint32_t)F});
+    run(E, dev::k_center_cols<K>, grid_for(F * K), dev::CenterCols<K>{(uint64_t)(F), E.sch.p, E.lane_a.p, nullptr});
+    run(E, dev::k_hybrid_rhs<K>, grid_for(m0), dev::HybridRhs<K>{E.n0, (uint32_t)F, E.perm0.p, E.sch.p, E.b0.p});
+  };
+  const std::function<void()> tail = [&]() { extract<K>(E); };
   std::vector<uint8_t> use(K, 0);
   for (uint32_t j = 0; j < columns; ++j) use[j] = 1;
   std::vector<dev::ColState> st;
@@ -581,8 +598,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
   // lane states, so the batch synchronises once (a failed lane's extraction is
   // discarded with the batch)
   pcg_lanes<K>(E, E.b0.p, effective_tol * inner_ratio, 0.0, opt.max_iterations, opt.recompute_interval, false,
-               opt.precision == 32, use, st, cap, false);
-  extract<K>(E);
+               opt.precision == 32, use, st, cap, false, &head, &tail);
   std::vector<double> red, comp, rn;
   read_batch<K>(E, st, red, comp, rn);
   receipts.assign(columns, ColumnReceipt{});
@@ -621,8 +637,7 @@ void solve_columns(Impl &E, uint32_t columns, double effective_tol, double inner
     E.refine_rounds++;
     for (uint32_t j = 0; j < columns; ++j) E.refined_columns += fail_lane[j];
     pcg_lanes<K>(E, E.b0.p, rel, abs_tol, opt.max_iterations, opt.recompute_interval, true, opt.precision == 32,
-                 fail_lane, st, 0.0, false);
-    extract<K>(E);
+                 fail_lane, st, 0.0, false, nullptr, &tail);
     read_batch<K>(E, st, red, comp, rn);
     for (uint32_t j = 0; j < columns; ++j)
       if (fail_lane[j]) {
@@ -1018,16 +1033,16 @@ void numerical_replay(Impl &E, const SolveOptions &opt, SolveResult &res) {
     for (uint64_t first = 0; first < R; first += lw) {
       opt.checkpoint("numerical_replay_batch");
       const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
-      hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                         E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
+      run(E, dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)),
+          dev::Atoms{E.G, E.unit_rank.p, E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL,
+                     E.atoms.p});
       std::vector<ColumnReceipt> rc;
       bool failed = false;
       with_k(KL, [&](auto kk) {
         constexpr int K = decltype(kk)::value;
-        hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W,
-                           E.worker_ptr.p, (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p,
-                           E.G, k, E.wrhs.p,
-                         E.cell_unit.p);
+        run(E, dev::k_unit_rhs<K>, grid_for(E.W * dev::Pack<K>::TPR),
+            dev::UnitRhs<K>{E.W, E.worker_ptr.p, nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
+                            E.wrhs.p, E.cell_unit.p});
         segments<double, K, 0>(E, E.firm_chunks,
                                dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G,
                                                    k, E.frhs.p, E.firm_units.p});
@@ -1306,6 +1321,7 @@ class Lockstep {
       pre[q] = segments([q](PcgProgram &p) -> const std::vector<Record> & { return p.pre[q]; });
       body[q] = segments([q](PcgProgram &p) -> const std::vector<Record> & { return p.body[q]; });
     }
+    const Plan tail = segments([](PcgProgram &p) -> const std::vector<Record> & { return p.tail; });
     align(table, 16);
     const size_t flags_at = table.size();
     for (PcgProgram *p : batch) {
@@ -1353,10 +1369,12 @@ class Lockstep {
       launch(pre[it % 2]);
       ++cycles;
     }
+    launch(tail);
     for (PcgProgram *p : batch) {
       p->out.assign(p->lanes, dev::ColState{});
-      VK_HIP(hipMemcpyAsync(p->out.data(), p->st, p->lanes * sizeof(dev::ColState), hipMemcpyDeviceToHost,
-                            stream_));
+      if (p->lanes)
+        VK_HIP(hipMemcpyAsync(p->out.data(), p->st, p->lanes * sizeof(dev::ColState), hipMemcpyDeviceToHost,
+                              stream_));
       p->iterations = iterations;
       p->cycles = cycles;
     }
@@ -1595,28 +1613,29 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
   for (uint64_t first = 0; first < R; first += lw) {
     opt.checkpoint("jla_leverage_batch");
     const uint32_t k = (uint32_t)std::min<uint64_t>(lw, R - first);
-    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL, E.atoms.p);
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
-                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k, E.wrhs.p,
-                         E.cell_unit.p);
+      vk::run(E, dev::k_atoms, grid_for(E.G * ((k + 3) / 4 + 1)),
+              dev::Atoms{E.G, E.unit_rank.p, E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, k, KL,
+                         E.atoms.p});
+      vk::run(E, dev::k_unit_rhs<K>, grid_for(E.W * dev::Pack<K>::TPR),
+              dev::UnitRhs<K>{E.W, E.worker_ptr.p, nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
+                              E.wrhs.p, E.cell_unit.p});
       segments<double, K, 0>(E, E.firm_chunks,
-                             dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, k,
-                                                 E.frhs.p, E.firm_units.p});
+                             dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G,
+                                                 k, E.frhs.p, E.firm_units.p});
       solve_columns<K>(E, k, opt.probe_tolerance, 1.0, opt, rc);
       if (E.Gm)
-        hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.Gm * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Gm, k,
-                           (const uint32_t *)nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p,
-                           E.fsol.p, E.atoms.p, E.moments.p, E.sink());
+        vk::run(E, dev::k_moments<K>, grid_for(E.Gm * dev::Pack<K>::TPR),
+                dev::Moments<K>{E.Gm, k, nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p,
+                                E.fsol.p, E.atoms.p, E.moments.p, E.sink()});
       if (E.Go)  // generic_jla.rs:5540-5892, units [Gm, G)
-        hipLaunchKernelGGL((dev::k_obs_moments<K>), grid_for(E.Go * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.Go,
-                           k, first, opt.seed, kTagLeverage, E.unit_cell.p + E.Gm, E.unit_count.p + E.Gm,
-                           E.unit_rank.p + E.Gm, E.unit_offset.p + E.Gm, E.row_phys.p, E.obs_physical,
-                           E.cell_worker.p, E.cell_firm.p, E.worker.p, E.fsol.p, E.obs_m.p, E.obs_corr.p,
-                           E.sink());
+        vk::run(E, dev::k_obs_moments<K>, grid_for(E.Go * dev::Pack<K>::TPR),
+                dev::ObsMoments<K>{E.Go, k, first, opt.seed, kTagLeverage, E.unit_cell.p + E.Gm,
+                                   E.unit_count.p + E.Gm, E.unit_rank.p + E.Gm, E.unit_offset.p + E.Gm,
+                                   E.row_phys.p, E.obs_physical, E.cell_worker.p, E.cell_firm.p, E.worker.p,
+                                   E.fsol.p, E.obs_m.p, E.obs_corr.p, E.sink()});
     });
     for (auto &c : rc) {
       res.leverage.push_back(c);
@@ -1815,27 +1834,28 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
     opt.checkpoint("jla_target_batch");
     const uint32_t k = (uint32_t)std::min<uint64_t>(tw, R - first);
     VK_HIP(hipMemsetAsync(E.atoms.p, 0, (uint64_t)KT * E.S * sizeof(int32_t), E.stream));
-    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)), kBlock, 0, E.stream, E.S,
-                       E.stratum_rank.p, E.stratum_count.p, (const uint64_t *)nullptr, opt.seed, kTagTarget, first, k, KT, E.atoms.p);
+    vk::run(E, dev::k_atoms, grid_for(E.S * ((k + 3) / 4 + 1)),
+            dev::Atoms{E.S, E.stratum_rank.p, E.stratum_count.p, nullptr, opt.seed, kTagTarget, first, k, KT, E.atoms.p});
     std::vector<ColumnReceipt> rc;
     with_k(KT, [&](auto kt) {
       // target batches hold at most 32 probes (64 columns)
       constexpr int KTc = decltype(kt)::value > 32 ? 32 : decltype(kt)::value;
       constexpr int K2c = 2 * KTc;
       const unsigned cg = reduce_grid(E.C * KTc);
-      hipLaunchKernelGGL((dev::k_target_direction<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_stratum_ptr.p,
-                         E.stratum_per_copy.p, E.target_total, E.atoms.p, E.dir.p, E.rpartial.p);
+      vk::run(E, dev::k_target_direction<KTc>, cg,
+              dev::TargetDirection<KTc>{E.C, E.cell_stratum_ptr.p, E.stratum_per_copy.p, E.target_total, E.atoms.p,
+                                        E.dir.p, E.rpartial.p});
       finish<KTc, 2>(E, E.rpartial.p, cg, dev::FinTotals<KTc>{E.totals.p});
-      hipLaunchKernelGGL((dev::k_target_scores<KTc>), grid_for(E.W * K2c), kBlock, 0, E.stream, E.W,
-                         E.worker_ptr.p, (const uint32_t *)nullptr, E.dir.p, E.cell_target.p, E.target_total,
-                         E.totals.p, 0, E.wrhs.p);
+      vk::run(E, dev::k_target_scores<KTc>, grid_for(E.W * K2c),
+              dev::TargetScores<KTc>{E.W, E.worker_ptr.p, nullptr, E.dir.p, E.cell_target.p, E.target_total,
+                                     E.totals.p, 0, E.wrhs.p});
       segments<double, K2c, 0>(E, E.firm_chunks,
                                dev::OpFirmTargetScores<KTc>{E.firm_cells.p, E.dir.p, E.cell_target.p, E.totals.p,
                                                             E.target_total, E.frhs.p});
       const unsigned bw = reduce_grid(E.W * K2c), bf = reduce_grid(E.F * K2c);
-      hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bw, kBlock, 0, E.stream, E.W, E.wrhs.p, E.rpartial.p);
+      vk::run(E, dev::k_balance_sum<K2c>, bw, dev::BalanceSum<K2c>{E.W, E.wrhs.p, E.rpartial.p});
       finish<K2c, 1>(E, E.rpartial.p, bw, dev::FinBalance<K2c>{E.wrhs.p, E.W, 0, E.totals.p, E.sink()});
-      hipLaunchKernelGGL((dev::k_balance_sum<K2c>), bf, kBlock, 0, E.stream, E.F, E.frhs.p, E.rpartial.p);
+      vk::run(E, dev::k_balance_sum<K2c>, bf, dev::BalanceSum<K2c>{E.F, E.frhs.p, E.rpartial.p});
       finish<K2c, 1>(E, E.rpartial.p, bf, dev::FinBalance<K2c>{E.frhs.p, E.F, 1, E.totals.p, E.sink()});
       solve_columns<K2c>(E, 2 * k, opt.probe_tolerance, 1.0, opt, rc);
       if (opt.numerical && E.Go)  // observation folds: y e kappa / fold size (generic_jla/numerical.rs:377-389)
@@ -1846,8 +1866,8 @@ SolveResult DeviceEngine::run_once(const SolveOptions &opt) {
         hipLaunchKernelGGL((dev::k_numerical_folds<K2c>), grid_for(E.Gm), kBlock, 0, E.stream, E.Gm, k, first, R,
                            E.unit_cell.p, E.cell_worker.p, E.cell_firm.p, E.unit_outcome.p, E.adjust.p + 4 * E.Gm,
                            E.worker.p, E.fsol.p, E.nfolds.p);
-      hipLaunchKernelGGL((dev::k_contract<KTc>), cg, kBlock, 0, E.stream, E.C, E.cell_worker.p, E.cell_firm.p,
-                         E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p);
+      vk::run(E, dev::k_contract<KTc>, cg,
+              dev::Contract<KTc>{E.C, E.cell_worker.p, E.cell_firm.p, E.cellw.p, E.worker.p, E.fsol.p, E.rpartial.p});
       finish<KTc, 3>(E, E.rpartial.p, cg, dev::FinDraws{E.draws.p});
       if (joint) {
         // statistical_batches.rs:259-346 with the joint solve's control
@@ -1920,10 +1940,9 @@ void DeviceEngine::atoms(uint64_t seed, uint64_t tag, uint64_t first_probe, uint
   const uint64_t n = target ? E.S : E.G;
   DBuf<int32_t> buf;
   buf.alloc(n * columns);
-  hipLaunchKernelGGL(dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)), kBlock, 0, E.stream, n,
-                     target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p,
-                     target ? (const uint64_t *)nullptr : E.unit_offset.p, seed,
-                     tag, first_probe, columns, columns, buf.p);
+  vk::run(E, dev::k_atoms, grid_for(n * ((columns + 3) / 4 + 1)),
+          dev::Atoms{n, target ? E.stratum_rank.p : E.unit_rank.p, target ? E.stratum_count.p : E.unit_count.p,
+                     target ? nullptr : E.unit_offset.p, seed, tag, first_probe, columns, columns, buf.p});
   std::vector<int32_t> h;
   buf.download(h, n * columns, E.stream);
   out.resize(h.size());
@@ -2141,21 +2160,22 @@ double DeviceEngine::time_leverage_batches(const SolveOptions &opt, uint32_t bat
   auto t0 = std::chrono::steady_clock::now();
   for (uint32_t b = 0; b < batches; ++b) {
     const uint64_t first = (uint64_t)b * lw;
-    hipLaunchKernelGGL(dev::k_atoms, grid_for(E.G * ((lw + 3) / 4 + 1)), kBlock, 0, E.stream, E.G, E.unit_rank.p,
-                       E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, lw, KL, E.atoms.p);
+    vk::run(E, dev::k_atoms, grid_for(E.G * ((lw + 3) / 4 + 1)),
+            dev::Atoms{E.G, E.unit_rank.p, E.unit_count.p, E.unit_offset.p, opt.seed, kTagLeverage, first, lw, KL,
+                       E.atoms.p});
     std::vector<ColumnReceipt> rc;
     with_k(KL, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
-      hipLaunchKernelGGL((dev::k_unit_rhs<K>), grid_for(E.W * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.W, E.worker_ptr.p,
-                         (const uint32_t *)nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw, E.wrhs.p,
-                         E.cell_unit.p);
+      vk::run(E, dev::k_unit_rhs<K>, grid_for(E.W * dev::Pack<K>::TPR),
+              dev::UnitRhs<K>{E.W, E.worker_ptr.p, nullptr, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
+                              E.wrhs.p, E.cell_unit.p});
       segments<double, K, 0>(E, E.firm_chunks,
                              dev::OpFirmUnits<K>{E.firm_cells.p, E.cell_unit_ptr.p, E.cell_units.p, E.atoms.p, E.G, lw,
                                                  E.frhs.p, E.firm_units.p});
       solve_columns<K>(E, lw, opt.probe_tolerance, 1.0, opt, rc);
-      hipLaunchKernelGGL((dev::k_moments<K>), grid_for(E.G * dev::Pack<K>::TPR), kBlock, 0, E.stream, E.G, lw,
-                         (const uint32_t *)nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p,
-                         E.fsol.p, E.atoms.p, E.moments.p, E.sink());
+      vk::run(E, dev::k_moments<K>, grid_for(E.G * dev::Pack<K>::TPR),
+              dev::Moments<K>{E.G, lw, nullptr, E.unit_count.p, E.unit_worker.p, E.unit_firm.p, E.worker.p, E.fsol.p,
+                              E.atoms.p, E.moments.p, E.sink()});
     });
     for (auto &c : rc) iterations += c.iterations;
     solves += lw;

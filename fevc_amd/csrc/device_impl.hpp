@@ -248,13 +248,15 @@ struct Recorder {
   }
 };
 
-// One engine's PCG call in a lockstep sample batch: its recorded setup, and
-// the preconditioning step and iteration body of both cycle parities; the
-// driver runs the calls waiting together (iteration it runs body[(it - 1) %
-// 2], then pre[it % 2]) until no sample has an active lane, and returns each
-// call's lane states and launch counts.
+// One engine's PCG call in a lockstep sample batch: its recorded setup (with
+// the passes that produce its right-hand sides), the preconditioning step and
+// iteration body of both cycle parities, and the tail (the centring of x and
+// the passes that read the solution); the driver runs the calls waiting
+// together (iteration it runs body[(it - 1) % 2], then pre[it % 2]) until no
+// sample has an active lane, then the tails, and returns each call's lane
+// states and launch counts.
 struct PcgProgram {
-  std::vector<Record> setup, pre[2], body[2];
+  std::vector<Record> setup, pre[2], body[2], tail;
   hipStream_t stream = nullptr;    // the engine's stream (ordered before and after)
   const int *active = nullptr;     // the engine's any-lane-active flag
   const dev::ColState *st = nullptr;

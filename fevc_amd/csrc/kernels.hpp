@@ -144,33 +144,46 @@ __device__ inline void philox(uint64_t k0, uint64_t k1, uint64_t c[4]) {
 // entity, copy j being bit j % 64 of Counter-V1 word j / 64 (rng.rs:121-179;
 // generic_jla.rs:10321-10329 with a copy offset); offset may be null (all
 // zero).  Callers guarantee trials < 2^31.
-static __global__ void k_atoms(uint64_t n, const uint64_t *__restrict__ entity,
-                        const uint64_t *__restrict__ trials, const uint64_t *__restrict__ offset, uint64_t seed,
-                        uint64_t tag, uint64_t first_probe, uint32_t columns, uint32_t ld,
-                        int32_t *__restrict__ out) {
-  const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
-  const uint64_t blocks = last_block - first_block + 1;
-  const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n * blocks;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t row = t / blocks, pb = first_block + t % blocks;
-    const uint64_t tr = trials[row], lo = offset ? offset[row] : 0, hi = lo + tr;
-    uint64_t pos[4] = {0, 0, 0, 0};
-    for (uint64_t w = lo / 64; w * 64 < hi; ++w) {
-      const uint64_t b0 = lo > w * 64 ? lo - w * 64 : 0, b1 = hi - w * 64 < 64 ? hi - w * 64 : 64;
-      const uint64_t mask = (b1 == 64 ? ~0ULL : ((1ULL << b1) - 1)) & ~((1ULL << b0) - 1);
-      uint64_t c[4] = {entity[row], pb, w, tag};
-      philox(k0, k1, c);
-#pragma unroll
-      for (int l = 0; l < 4; ++l) pos[l] += __popcll(c[l] & mask);
-    }
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const uint64_t probe = pb * 4 + l;
-      if (probe < first_probe || probe >= first_probe + columns) continue;
-      out[row * ld + (probe - first_probe)] = (int32_t)((int64_t)(2 * pos[l]) - (int64_t)tr);
+struct Atoms {
+  static constexpr int kThreads = kBlock;
+  uint64_t n;
+  const uint64_t *__restrict__ entity;
+  const uint64_t *__restrict__ trials;
+  const uint64_t *__restrict__ offset;
+  uint64_t seed;
+  uint64_t tag;
+  uint64_t first_probe;
+  uint32_t columns;
+  uint32_t ld;
+  int32_t *__restrict__ out;
+  __device__ void operator()(Grid g) const {
+    const uint64_t first_block = first_probe / 4, last_block = (first_probe + columns - 1) / 4;
+    const uint64_t blocks = last_block - first_block + 1;
+    const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
+    for (uint64_t t = g.block * (uint64_t)blockDim.x + threadIdx.x; t < n * blocks;
+         t += (uint64_t)g.blocks * blockDim.x) {
+      const uint64_t row = t / blocks, pb = first_block + t % blocks;
+      const uint64_t tr = trials[row], lo = offset ? offset[row] : 0, hi = lo + tr;
+      uint64_t pos[4] = {0, 0, 0, 0};
+      for (uint64_t w = lo / 64; w * 64 < hi; ++w) {
+        const uint64_t b0 = lo > w * 64 ? lo - w * 64 : 0, b1 = hi - w * 64 < 64 ? hi - w * 64 : 64;
+        const uint64_t mask = (b1 == 64 ? ~0ULL : ((1ULL << b1) - 1)) & ~((1ULL << b0) - 1);
+        uint64_t c[4] = {entity[row], pb, w, tag};
+        philox(k0, k1, c);
+  #pragma unroll
+        for (int l = 0; l < 4; ++l) pos[l] += __popcll(c[l] & mask);
+      }
+  #pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const uint64_t probe = pb * 4 + l;
+        if (probe < first_probe || probe >= first_probe + columns) continue;
+        out[row * ld + (probe - first_probe)] = (int32_t)((int64_t)(2 * pos[l]) - (int64_t)tr);
+      }
     }
   }
+};
+static __global__ void k_atoms(Atoms f) {
+  f(Grid::hw());
 }
 
 // ---------------------------------------------------------------- reductions
@@ -829,11 +842,19 @@ static __global__ void k_compose(uint64_t count, const uint32_t *__restrict__ a,
 
 // Row permutation of K-lane vectors: out[p] = in[perm[p]] (gather).
 template <int K>
-__global__ void k_gather_rows(uint64_t rows, const uint32_t *__restrict__ idx, const double *__restrict__ in,
-                              double *__restrict__ out) {
-  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < rows * K;
-       t += (uint64_t)gridDim.x * blockDim.x)
-    out[t] = in[(uint64_t)idx[t / K] * K + t % K];
+struct GatherRows {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const uint32_t *__restrict__ idx;
+  const double *__restrict__ in;
+  double *__restrict__ out;
+  __device__ void operator()(Grid g) const {
+    VK_GRID_STRIDE(g, t, rows * K) out[t] = in[(uint64_t)idx[t / K] * K + t % K];
+  }
+};
+template <int K>
+__global__ void k_gather_rows(GatherRows<K> f) {
+  f(Grid::hw());
 }
 
 }  // namespace dev

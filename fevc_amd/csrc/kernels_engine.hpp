@@ -538,7 +538,8 @@ struct Pack {
 
 // block_partial for packed threads: thread t holds lanes (t % TPR) * LP + q.
 template <int K, int NV>
-__device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV], double *__restrict__ out) {
+__device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV], double *__restrict__ out,
+                                             uint32_t slot) {
   constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR, R = kBlock / TPR;
   __shared__ double lds[NV][R * K];
   const int tid = threadIdx.x, row = tid / TPR, j0 = (tid % TPR) * LP;
@@ -555,7 +556,7 @@ __device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV]
   }
   if (tid < K) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[((uint64_t)v * K + tid) * kPartialStride + blockIdx.x] = lds[v][tid];
+    for (int v = 0; v < NV; ++v) out[((uint64_t)v * K + tid) * kPartialStride + slot] = lds[v][tid];
   }
 }
 
@@ -564,75 +565,105 @@ __device__ inline void block_partial_packed(const double (&val)[Pack<K>::LP][NV]
 // With `cunit` (every cell one unit: the unit of each cell) the unit is read
 // directly instead of through the cell's unit range.
 template <int K>
-__global__ void k_unit_rhs(uint64_t rows, const uint64_t *__restrict__ ptr,
-                           const uint32_t *__restrict__ cells, const uint64_t *__restrict__ cu_ptr,
-                           const uint32_t *__restrict__ cu, const int32_t *__restrict__ atoms,
-                           uint64_t units, uint32_t kcols, double *__restrict__ out,
-                           const uint32_t *__restrict__ cunit = nullptr) {
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
-  VK_GRID_LOOP(t, rows * TPR) {
-    const uint64_t i = t / TPR;
-    const int j0 = (int)(t % TPR) * LP;
-    double s[LP];
-#pragma unroll
-    for (int q = 0; q < LP; ++q) s[q] = 0.0;
-    for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-      const uint32_t c = cells ? cells[k] : (uint32_t)k;
-      if (cunit) {
-        const int32_t *a = atoms + (uint64_t)cunit[c] * K + j0;
-#pragma unroll
-        for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
-        continue;
+struct UnitRhs {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const uint64_t *__restrict__ ptr;
+  const uint32_t *__restrict__ cells;
+  const uint64_t *__restrict__ cu_ptr;
+  const uint32_t *__restrict__ cu;
+  const int32_t *__restrict__ atoms;
+  uint64_t units;
+  uint32_t kcols;
+  double *__restrict__ out;
+  const uint32_t *__restrict__ cunit;
+  __device__ void operator()(Grid g) const {
+    constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+    VK_GRID_STRIDE(g, t, rows * TPR) {
+      const uint64_t i = t / TPR;
+      const int j0 = (int)(t % TPR) * LP;
+      double s[LP];
+  #pragma unroll
+      for (int q = 0; q < LP; ++q) s[q] = 0.0;
+      for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const uint32_t c = cells ? cells[k] : (uint32_t)k;
+        if (cunit) {
+          const int32_t *a = atoms + (uint64_t)cunit[c] * K + j0;
+  #pragma unroll
+          for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
+          continue;
+        }
+        for (uint64_t u = cu_ptr[c]; u < cu_ptr[c + 1]; ++u) {
+          const int32_t *a = atoms + (uint64_t)cu[u] * K + j0;
+  #pragma unroll
+          for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
+        }
       }
-      for (uint64_t u = cu_ptr[c]; u < cu_ptr[c + 1]; ++u) {
-        const int32_t *a = atoms + (uint64_t)cu[u] * K + j0;
-#pragma unroll
-        for (int q = 0; q < LP; ++q) s[q] += (double)a[q];
-      }
+  #pragma unroll
+      for (int q = 0; q < LP; ++q) out[i * K + j0 + q] = j0 + q < (int)kcols ? s[q] : 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < LP; ++q) out[i * K + j0 + q] = j0 + q < (int)kcols ? s[q] : 0.0;
   }
+};
+template <int K>
+__global__ void k_unit_rhs(UnitRhs<K> f) {
+  f(Grid::hw());
 }
 
 // wsum[w] = sum_c w_c v[f_c] / d_w  (operator.rs:210-262 first half)
 template <int K>
-__global__ void k_worker_gather(uint64_t workers, const uint64_t *__restrict__ wptr,
-                                const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
-                                const double *__restrict__ wdiag, const double *__restrict__ v,
-                                double *__restrict__ out) {
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
-  VK_GRID_LOOP(t, workers * TPR) {
-    const uint64_t w = t / TPR;
-    const int j0 = (int)(t % TPR) * LP;
-    double s[LP];
+struct WorkerGather {
+  static constexpr int kThreads = kBlock;
+  uint64_t workers;
+  const uint64_t *__restrict__ wptr;
+  const uint32_t *__restrict__ cell_firm;
+  const double *__restrict__ cell_weight, *__restrict__ wdiag, *__restrict__ v;
+  double *__restrict__ out;
+  __device__ void operator()(Grid g) const {
+    constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+    VK_GRID_STRIDE(g, t, workers * TPR) {
+      const uint64_t w = t / TPR;
+      const int j0 = (int)(t % TPR) * LP;
+      double s[LP];
 #pragma unroll
-    for (int q = 0; q < LP; ++q) s[q] = 0.0;
-    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
-      const double cw = cell_weight[c];
-      const double *f = v + (uint64_t)cell_firm[c] * K + j0;
+      for (int q = 0; q < LP; ++q) s[q] = 0.0;
+      for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
+        const double cw = cell_weight[c];
+        const double *f = v + (uint64_t)cell_firm[c] * K + j0;
 #pragma unroll
-      for (int q = 0; q < LP; ++q) s[q] += cw * f[q];
+        for (int q = 0; q < LP; ++q) s[q] += cw * f[q];
+      }
+      const double d = wdiag[w];
+#pragma unroll
+      for (int q = 0; q < LP; ++q) out[w * K + j0 + q] = s[q] / d;
     }
-    const double d = wdiag[w];
-#pragma unroll
-    for (int q = 0; q < LP; ++q) out[w * K + j0 + q] = s[q] / d;
   }
+};
+template <int K>
+__global__ void k_worker_gather(WorkerGather<K> f) {
+  f(Grid::hw());
 }
 
 // reduced relative residual ||rhs_F - (out - mean)|| / ||rhs_F||  (full_cmg.rs:1425-1443)
 template <int K>
-__global__ void k_reduced_residual(uint32_t firms, const double *__restrict__ rhs,
-                                   const double *__restrict__ out, const double *__restrict__ mean,
-                                   double *__restrict__ partial) {
-  double acc[2] = {0.0, 0.0};
-  VK_GRID_LOOP(t, (uint64_t)firms * K) {
-    const int j = t % K;
-    const double a = rhs[t] - (out[t] - mean[j]);
-    acc[0] += a * a;
-    acc[1] += rhs[t] * rhs[t];
+struct ReducedResidual {
+  static constexpr int kThreads = kBlock;
+  uint32_t firms;
+  const double *__restrict__ rhs, *__restrict__ out, *__restrict__ mean;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    double acc[2] = {0.0, 0.0};
+    VK_GRID_STRIDE(g, t, (uint64_t)firms * K) {
+      const int j = t % K;
+      const double a = rhs[t] - (out[t] - mean[j]);
+      acc[0] += a * a;
+      acc[1] += rhs[t] * rhs[t];
+    }
+    block_partial<K, 2>(acc, partial, g.block);
   }
-  block_partial<K, 2>(acc, partial);
+};
+template <int K>
+__global__ void k_reduced_residual(ReducedResidual<K> f) {
+  f(Grid::hw());
 }
 struct FinRelative {
   __device__ bool enabled() const { return true; }
@@ -646,38 +677,46 @@ struct FinRelative {
 // worker block of the complete residual (operator.rs:386-458),
 // r_w = (wrhs - sum_c w_c firm[f_c]) - d_w worker with the computed worker.
 template <int K>
-__global__ void k_worker_solution(uint64_t workers, const uint64_t *__restrict__ wptr,
-                                  const uint32_t *__restrict__ cell_firm, const double *__restrict__ cell_weight,
-                                  const double *__restrict__ wdiag, const double *__restrict__ wrhs,
-                                  const double *__restrict__ firm, double *__restrict__ worker,
-                                  double *__restrict__ partial) {
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
-  double acc[LP][2];
+struct WorkerSolution {
+  static constexpr int kThreads = kBlock;
+  uint64_t workers;
+  const uint64_t *__restrict__ wptr;
+  const uint32_t *__restrict__ cell_firm;
+  const double *__restrict__ cell_weight, *__restrict__ wdiag, *__restrict__ wrhs, *__restrict__ firm;
+  double *__restrict__ worker, *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+    double acc[LP][2];
 #pragma unroll
-  for (int q = 0; q < LP; ++q) acc[q][0] = acc[q][1] = 0.0;
-  VK_GRID_LOOP(t, workers * TPR) {
-    const uint64_t w = t / TPR;
-    const int j0 = (int)(t % TPR) * LP;
-    double b[LP], s[LP];
+    for (int q = 0; q < LP; ++q) acc[q][0] = acc[q][1] = 0.0;
+    VK_GRID_STRIDE(g, t, workers * TPR) {
+      const uint64_t w = t / TPR;
+      const int j0 = (int)(t % TPR) * LP;
+      double b[LP], s[LP];
 #pragma unroll
-    for (int q = 0; q < LP; ++q) s[q] = b[q] = wrhs[w * K + j0 + q];
-    for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
-      const double cw = cell_weight[c];
-      const double *f = firm + (uint64_t)cell_firm[c] * K + j0;
+      for (int q = 0; q < LP; ++q) s[q] = b[q] = wrhs[w * K + j0 + q];
+      for (uint64_t c = wptr[w]; c < wptr[w + 1]; ++c) {
+        const double cw = cell_weight[c];
+        const double *f = firm + (uint64_t)cell_firm[c] * K + j0;
 #pragma unroll
-      for (int q = 0; q < LP; ++q) s[q] -= cw * f[q];
+        for (int q = 0; q < LP; ++q) s[q] -= cw * f[q];
+      }
+      const double d = wdiag[w];
+#pragma unroll
+      for (int q = 0; q < LP; ++q) {
+        const double x = s[q] / d;
+        worker[w * K + j0 + q] = x;
+        const double rw = s[q] - d * x;
+        acc[q][0] += rw * rw;
+        acc[q][1] += b[q] * b[q];
+      }
     }
-    const double d = wdiag[w];
-#pragma unroll
-    for (int q = 0; q < LP; ++q) {
-      const double x = s[q] / d;
-      worker[w * K + j0 + q] = x;
-      const double rw = s[q] - d * x;
-      acc[q][0] += rw * rw;
-      acc[q][1] += b[q] * b[q];
-    }
+    block_partial_packed<K, 2>(acc, partial, g.block);
   }
-  block_partial_packed<K, 2>(acc, partial);
+};
+template <int K>
+__global__ void k_worker_solution(WorkerSolution<K> f) {
+  f(Grid::hw());
 }
 template <int K>
 struct FinSums {
@@ -906,72 +945,86 @@ __device__ inline void neumaier_add(double &s, double &c, double v) {
 // With unit_cell null, cell_worker and cell_firm are indexed by unit (the
 // unit's worker and firm, precomputed for match units).
 template <int K>
-__global__ void k_moments(uint64_t units, uint32_t kcols, const uint32_t *__restrict__ unit_cell,
-                          const uint64_t *__restrict__ unit_count, const uint32_t *__restrict__ cell_worker,
-                          const uint32_t *__restrict__ cell_firm, const double *__restrict__ worker,
-                          const double *__restrict__ firm, const int32_t *__restrict__ atoms,
-                          double *__restrict__ m, StatusSink status) {
-  // TPR threads per unit, LP lanes each (packed loads); the batch's five sums
-  // are combined in-thread, then by a fixed xor butterfly over the unit's
-  // threads, and added to the unit's running compensated sums by its first
-  // thread, batches in probe order.  A thread takes two units per step, their
-  // loads issued together.
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR, U = 2;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, total = units * TPR;
-  for (uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t0 < total; t0 += U * stride) {
-    const int j0 = (int)(t0 % TPR) * LP;  // the same for both units (stride is a multiple of TPR)
-    uint64_t g[U];
-    bool on[U];
-    double wv[U][LP], fv[U][LP], av[U][LP], root[U];
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const uint64_t t = t0 + q * stride;
-      on[q] = t < total;
-      g[q] = on[q] ? t / TPR : t0 / TPR;
-      const uint64_t c = unit_cell ? unit_cell[g[q]] : g[q];
-      const uint64_t w = cell_worker[c], f = cell_firm[c];
-      root[q] = sqrt((double)unit_count[g[q]]);
-#pragma unroll
-      for (int l = 0; l < LP; ++l) {
-        const int j = j0 + l;
-        wv[q][l] = worker[w * K + j];
-        fv[q][l] = firm[f * K + j];
-        av[q][l] = (double)atoms[g[q] * K + j];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-      double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int l = 0; l < LP; ++l) {
-        if (j0 + l < (int)kcols) {
-          const double proj = root[q] * (wv[q][l] + fv[q][l]);
-          const double res = av[q][l] / root[q] - proj;
-          if (on[q] && (!isfinite(proj) || !isfinite(res))) report(status, g[q], 73);
-          const double ps = proj * proj, rs = res * res;
-          v[0] += ps;
-          v[1] += rs;
-          v[2] += ps * ps;
-          v[3] += rs * rs;
-          v[4] += ps * rs;
+struct Moments {
+  static constexpr int kThreads = kBlock;
+  uint64_t units;
+  uint32_t kcols;
+  const uint32_t *__restrict__ unit_cell;
+  const uint64_t *__restrict__ unit_count;
+  const uint32_t *__restrict__ cell_worker;
+  const uint32_t *__restrict__ cell_firm;
+  const double *__restrict__ worker;
+  const double *__restrict__ firm;
+  const int32_t *__restrict__ atoms;
+  double *__restrict__ m;
+  StatusSink status;
+  __device__ void operator()(Grid grid) const {
+    // TPR threads per unit, LP lanes each (packed loads); the batch's five sums
+    // are combined in-thread, then by a fixed xor butterfly over the unit's
+    // threads, and added to the unit's running compensated sums by its first
+    // thread, batches in probe order.  A thread takes two units per step, their
+    // loads issued together.
+    constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR, U = 2;
+    const uint64_t stride = (uint64_t)grid.blocks * blockDim.x, total = units * TPR;
+    for (uint64_t t0 = grid.block * (uint64_t)blockDim.x + threadIdx.x; t0 < total; t0 += U * stride) {
+      const int j0 = (int)(t0 % TPR) * LP;  // the same for both units (stride is a multiple of TPR)
+      uint64_t g[U];
+      bool on[U];
+      double wv[U][LP], fv[U][LP], av[U][LP], root[U];
+  #pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const uint64_t t = t0 + q * stride;
+        on[q] = t < total;
+        g[q] = on[q] ? t / TPR : t0 / TPR;
+        const uint64_t c = unit_cell ? unit_cell[g[q]] : g[q];
+        const uint64_t w = cell_worker[c], f = cell_firm[c];
+        root[q] = sqrt((double)unit_count[g[q]]);
+  #pragma unroll
+        for (int l = 0; l < LP; ++l) {
+          const int j = j0 + l;
+          wv[q][l] = worker[w * K + j];
+          fv[q][l] = firm[f * K + j];
+          av[q][l] = (double)atoms[g[q] * K + j];
         }
       }
-#pragma unroll
-      for (int off = 1; off < TPR; off <<= 1) {
-#pragma unroll
-        for (int r = 0; r < 5; ++r) v[r] += __shfl_xor(v[r], off);
-      }
-      if (j0 == 0 && on[q]) {
-#pragma unroll
-        for (int r = 0; r < 5; ++r) {
-          double s = m[(uint64_t)(2 * r) * units + g[q]], e = m[(uint64_t)(2 * r + 1) * units + g[q]];
-          neumaier_add(s, e, v[r]);
-          m[(uint64_t)(2 * r) * units + g[q]] = s;
-          m[(uint64_t)(2 * r + 1) * units + g[q]] = e;
+  #pragma unroll
+      for (int q = 0; q < U; ++q) {
+        double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  #pragma unroll
+        for (int l = 0; l < LP; ++l) {
+          if (j0 + l < (int)kcols) {
+            const double proj = root[q] * (wv[q][l] + fv[q][l]);
+            const double res = av[q][l] / root[q] - proj;
+            if (on[q] && (!isfinite(proj) || !isfinite(res))) report(status, g[q], 73);
+            const double ps = proj * proj, rs = res * res;
+            v[0] += ps;
+            v[1] += rs;
+            v[2] += ps * ps;
+            v[3] += rs * rs;
+            v[4] += ps * rs;
+          }
+        }
+  #pragma unroll
+        for (int off = 1; off < TPR; off <<= 1) {
+  #pragma unroll
+          for (int r = 0; r < 5; ++r) v[r] += __shfl_xor(v[r], off);
+        }
+        if (j0 == 0 && on[q]) {
+  #pragma unroll
+          for (int r = 0; r < 5; ++r) {
+            double s = m[(uint64_t)(2 * r) * units + g[q]], e = m[(uint64_t)(2 * r + 1) * units + g[q]];
+            neumaier_add(s, e, v[r]);
+            m[(uint64_t)(2 * r) * units + g[q]] = s;
+            m[(uint64_t)(2 * r + 1) * units + g[q]] = e;
+          }
         }
       }
     }
   }
+};
+template <int K>
+__global__ void k_moments(Moments<K> f) {
+  f(Grid::hw());
 }
 
 // Leverage adjustment per unit (engine.rs:2269-2410), outcome mean 0.
@@ -1029,91 +1082,110 @@ static __global__ void k_adjust(uint64_t units, const double *__restrict__ m, co
 // correction); corr: 4 x physical copies (s p sum, correction, s p^3 sum,
 // correction); one compensated addition per batch, batches in probe order.
 template <int K>
-__global__ void k_obs_moments(uint64_t rows, uint32_t kcols, uint64_t first_probe, uint64_t seed, uint64_t tag,
-                              const uint32_t *__restrict__ unit_cell, const uint64_t *__restrict__ unit_count,
-                              const uint64_t *__restrict__ unit_rank, const uint64_t *__restrict__ unit_offset,
-                              const uint64_t *__restrict__ row_phys, uint64_t physical,
-                              const uint32_t *__restrict__ cell_worker, const uint32_t *__restrict__ cell_firm,
-                              const double *__restrict__ worker, const double *__restrict__ firm,
-                              double *__restrict__ m, double *__restrict__ corr, StatusSink status) {
-  constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
-  const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
-  VK_GRID_LOOP(t, rows * TPR) {
-    const uint64_t g = t / TPR;
-    const int j0 = (int)(t % TPR) * LP;
-    const uint32_t c = unit_cell[g];
-    const uint64_t w = cell_worker[c], f = cell_firm[c];
-    double p[LP], v[2] = {0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < LP; ++q) {
-      const int j = j0 + q;
-      p[q] = 0.0;
-      if (j < (int)kcols) {
-        p[q] = worker[w * K + j] + firm[f * K + j];
-        if (!isfinite(p[q])) report(status, g, 73);
-        const double p2 = p[q] * p[q];
-        v[0] += p2;
-        v[1] += p2 * p2;
-      }
-    }
-#pragma unroll
-    for (int off = 1; off < TPR; off <<= 1) {
-      v[0] += __shfl_xor(v[0], off);
-      v[1] += __shfl_xor(v[1], off);
-    }
-    if (j0 == 0) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        double s = m[(uint64_t)(2 * q) * rows + g], e = m[(uint64_t)(2 * q + 1) * rows + g];
-        neumaier_add(s, e, v[q]);
-        m[(uint64_t)(2 * q) * rows + g] = s;
-        m[(uint64_t)(2 * q + 1) * rows + g] = e;
-      }
-    }
-    // every thread of the row runs the same copy loops, so the butterflies
-    // below see the whole group
-    const uint64_t lo = unit_offset[g], hi = lo + unit_count[g], entity = unit_rank[g], base = row_phys[g];
-    for (uint64_t word = lo / 64; word * 64 < hi; ++word) {
-      uint64_t bits[LP], blk[4] = {0, 0, 0, 0}, cached = ~0ULL;
-#pragma unroll
+struct ObsMoments {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  uint32_t kcols;
+  uint64_t first_probe;
+  uint64_t seed;
+  uint64_t tag;
+  const uint32_t *__restrict__ unit_cell;
+  const uint64_t *__restrict__ unit_count;
+  const uint64_t *__restrict__ unit_rank;
+  const uint64_t *__restrict__ unit_offset;
+  const uint64_t *__restrict__ row_phys;
+  uint64_t physical;
+  const uint32_t *__restrict__ cell_worker;
+  const uint32_t *__restrict__ cell_firm;
+  const double *__restrict__ worker;
+  const double *__restrict__ firm;
+  double *__restrict__ m;
+  double *__restrict__ corr;
+  StatusSink status;
+  __device__ void operator()(Grid grid) const {
+    constexpr int LP = Pack<K>::LP, TPR = Pack<K>::TPR;
+    const uint64_t k0 = splitmix64(seed), k1 = splitmix64(seed ^ 0xd1b54a32d192ed03ULL);
+    VK_GRID_STRIDE(grid, t, rows * TPR) {
+      const uint64_t g = t / TPR;
+      const int j0 = (int)(t % TPR) * LP;
+      const uint32_t c = unit_cell[g];
+      const uint64_t w = cell_worker[c], f = cell_firm[c];
+      double p[LP], v[2] = {0.0, 0.0};
+  #pragma unroll
       for (int q = 0; q < LP; ++q) {
-        const uint64_t probe = first_probe + (uint64_t)(j0 + q), pb = probe / 4;
-        if (pb != cached) {
-          blk[0] = entity; blk[1] = pb; blk[2] = word; blk[3] = tag;
-          philox(k0, k1, blk);
-          cached = pb;
+        const int j = j0 + q;
+        p[q] = 0.0;
+        if (j < (int)kcols) {
+          p[q] = worker[w * K + j] + firm[f * K + j];
+          if (!isfinite(p[q])) report(status, g, 73);
+          const double p2 = p[q] * p[q];
+          v[0] += p2;
+          v[1] += p2 * p2;
         }
-        bits[q] = blk[probe % 4];
       }
-      const uint64_t c0 = lo > word * 64 ? lo : word * 64, c1 = hi < word * 64 + 64 ? hi : word * 64 + 64;
-      for (uint64_t copy = c0; copy < c1; ++copy) {
-        double a[2] = {0.0, 0.0};
-#pragma unroll
+  #pragma unroll
+      for (int off = 1; off < TPR; off <<= 1) {
+        v[0] += __shfl_xor(v[0], off);
+        v[1] += __shfl_xor(v[1], off);
+      }
+      if (j0 == 0) {
+  #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          double s = m[(uint64_t)(2 * q) * rows + g], e = m[(uint64_t)(2 * q + 1) * rows + g];
+          neumaier_add(s, e, v[q]);
+          m[(uint64_t)(2 * q) * rows + g] = s;
+          m[(uint64_t)(2 * q + 1) * rows + g] = e;
+        }
+      }
+      // every thread of the row runs the same copy loops, so the butterflies
+      // below see the whole group
+      const uint64_t lo = unit_offset[g], hi = lo + unit_count[g], entity = unit_rank[g], base = row_phys[g];
+      for (uint64_t word = lo / 64; word * 64 < hi; ++word) {
+        uint64_t bits[LP], blk[4] = {0, 0, 0, 0}, cached = ~0ULL;
+  #pragma unroll
         for (int q = 0; q < LP; ++q) {
-          if (j0 + q < (int)kcols) {
-            const double sgn = ((bits[q] >> (copy % 64)) & 1) ? 1.0 : -1.0;
-            a[0] += sgn * p[q];
-            a[1] += sgn * (p[q] * p[q] * p[q]);
+          const uint64_t probe = first_probe + (uint64_t)(j0 + q), pb = probe / 4;
+          if (pb != cached) {
+            blk[0] = entity; blk[1] = pb; blk[2] = word; blk[3] = tag;
+            philox(k0, k1, blk);
+            cached = pb;
           }
+          bits[q] = blk[probe % 4];
         }
-#pragma unroll
-        for (int off = 1; off < TPR; off <<= 1) {
-          a[0] += __shfl_xor(a[0], off);
-          a[1] += __shfl_xor(a[1], off);
-        }
-        if (j0 == 0) {
-          const uint64_t x = base + (copy - lo);
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            double s = corr[(uint64_t)(2 * q) * physical + x], e = corr[(uint64_t)(2 * q + 1) * physical + x];
-            neumaier_add(s, e, a[q]);
-            corr[(uint64_t)(2 * q) * physical + x] = s;
-            corr[(uint64_t)(2 * q + 1) * physical + x] = e;
+        const uint64_t c0 = lo > word * 64 ? lo : word * 64, c1 = hi < word * 64 + 64 ? hi : word * 64 + 64;
+        for (uint64_t copy = c0; copy < c1; ++copy) {
+          double a[2] = {0.0, 0.0};
+  #pragma unroll
+          for (int q = 0; q < LP; ++q) {
+            if (j0 + q < (int)kcols) {
+              const double sgn = ((bits[q] >> (copy % 64)) & 1) ? 1.0 : -1.0;
+              a[0] += sgn * p[q];
+              a[1] += sgn * (p[q] * p[q] * p[q]);
+            }
+          }
+  #pragma unroll
+          for (int off = 1; off < TPR; off <<= 1) {
+            a[0] += __shfl_xor(a[0], off);
+            a[1] += __shfl_xor(a[1], off);
+          }
+          if (j0 == 0) {
+            const uint64_t x = base + (copy - lo);
+  #pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              double s = corr[(uint64_t)(2 * q) * physical + x], e = corr[(uint64_t)(2 * q + 1) * physical + x];
+              neumaier_add(s, e, a[q]);
+              corr[(uint64_t)(2 * q) * physical + x] = s;
+              corr[(uint64_t)(2 * q + 1) * physical + x] = e;
+            }
           }
         }
       }
     }
   }
+};
+template <int K>
+__global__ void k_obs_moments(ObsMoments<K> f) {
+  f(Grid::hw());
 }
 
 // observation_deleted_adjustment (generic_jla.rs:5893-6017) with
@@ -1440,23 +1512,34 @@ static __global__ void k_cell_weight(uint64_t cells, const uint64_t *__restrict_
 // sqrt(per_copy / T) atom (engine.rs:2621-2703), totals for centring.
 // atoms: strata x KT, interleaved.
 template <int KT>
-__global__ void k_target_direction(uint64_t cells, const uint64_t *__restrict__ csp,
-                                   const double *__restrict__ per_copy, double target_total,
-                                   const int32_t *__restrict__ atoms, double *__restrict__ dir,
-                                   double *__restrict__ partial) {
-  double acc[2] = {0.0, 0.0};
-  VK_GRID_LOOP(t, cells * KT) {
-    const uint64_t c = t / KT;
-    const int j = t % KT;
-    double s = 0.0, k = 0.0;
-    for (uint64_t q = csp[c]; q < csp[c + 1]; ++q)
-      neumaier_add(s, k, sqrt(per_copy[q] / target_total) * (double)atoms[q * KT + j]);
-    const double d = s + k;
-    dir[t] = d;
-    acc[0] += d;
-    acc[1] += fabs(d);
+struct TargetDirection {
+  static constexpr int kThreads = kBlock;
+  uint64_t cells;
+  const uint64_t *__restrict__ csp;
+  const double *__restrict__ per_copy;
+  double target_total;
+  const int32_t *__restrict__ atoms;
+  double *__restrict__ dir;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    double acc[2] = {0.0, 0.0};
+    VK_GRID_STRIDE(g, t, cells * KT) {
+      const uint64_t c = t / KT;
+      const int j = t % KT;
+      double s = 0.0, k = 0.0;
+      for (uint64_t q = csp[c]; q < csp[c + 1]; ++q)
+        neumaier_add(s, k, sqrt(per_copy[q] / target_total) * (double)atoms[q * KT + j]);
+      const double d = s + k;
+      dir[t] = d;
+      acc[0] += d;
+      acc[1] += fabs(d);
+    }
+    block_partial<KT, 2>(acc, partial, g.block);
   }
-  block_partial<KT, 2>(acc, partial);
+};
+template <int KT>
+__global__ void k_target_direction(TargetDirection<KT> f) {
+  f(Grid::hw());
 }
 template <int KT>
 struct FinTotals {
@@ -1471,24 +1554,37 @@ struct FinTotals {
 // Centre, then transpose to worker/firm scores in the 2-column layout:
 // column 2j = (score_w, 0), column 2j+1 = (0, score_f)  (engine.rs:2705-2971).
 template <int KT>
-__global__ void k_target_scores(uint64_t rows, const uint64_t *__restrict__ ptr,
-                                const uint32_t *__restrict__ cells, const double *__restrict__ dir,
-                                const double *__restrict__ cell_target, double target_total,
-                                const double *__restrict__ totals, int parity, double *__restrict__ out) {
-  constexpr int K2 = 2 * KT;
-  VK_GRID_LOOP(t, rows * K2) {
-    const uint64_t i = t / K2;
-    const int l = t % K2;
-    const int j = l / 2;
-    double s = 0.0;
-    if ((l & 1) == parity) {
-      for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-        const uint64_t c = cells ? cells[k] : k;
-        s += dir[c * KT + j] - cell_target[c] / target_total * totals[j];
+struct TargetScores {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const uint64_t *__restrict__ ptr;
+  const uint32_t *__restrict__ cells;
+  const double *__restrict__ dir;
+  const double *__restrict__ cell_target;
+  double target_total;
+  const double *__restrict__ totals;
+  int parity;
+  double *__restrict__ out;
+  __device__ void operator()(Grid g) const {
+    constexpr int K2 = 2 * KT;
+    VK_GRID_STRIDE(g, t, rows * K2) {
+      const uint64_t i = t / K2;
+      const int l = t % K2;
+      const int j = l / 2;
+      double s = 0.0;
+      if ((l & 1) == parity) {
+        for (uint64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+          const uint64_t c = cells ? cells[k] : k;
+          s += dir[c * KT + j] - cell_target[c] / target_total * totals[j];
+        }
       }
+      out[t] = s;
     }
-    out[t] = s;
   }
+};
+template <int KT>
+__global__ void k_target_scores(TargetScores<KT> f) {
+  f(Grid::hw());
 }
 
 // Firm scores of the target directions in the 2-column layout: column
@@ -1520,10 +1616,20 @@ struct OpFirmTargetScores {
 
 // Compatibility repair of the last score entry (engine.rs:2842-2881).
 template <int K2>
-__global__ void k_balance_sum(uint64_t rows, const double *__restrict__ v, double *__restrict__ partial) {
-  double acc[1] = {0.0};
-  VK_GRID_LOOP(t, (rows - 1) * K2) acc[0] += v[t];
-  block_partial<K2, 1>(acc, partial);
+struct BalanceSum {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  const double *__restrict__ v;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    double acc[1] = {0.0};
+    VK_GRID_STRIDE(g, t, (rows - 1) * K2) acc[0] += v[t];
+    block_partial<K2, 1>(acc, partial, g.block);
+  }
+};
+template <int K2>
+__global__ void k_balance_sum(BalanceSum<K2> f) {
+  f(Grid::hw());
 }
 template <int K2>
 struct FinBalance {
@@ -1547,24 +1653,35 @@ struct FinBalance {
 
 // Target contraction per probe (engine.rs:3003-3100).
 template <int KT>
-__global__ void k_contract(uint64_t cells, const uint32_t *__restrict__ cell_worker,
-                           const uint32_t *__restrict__ cell_firm, const double *__restrict__ weight,
-                           const double *__restrict__ worker, const double *__restrict__ firm,
-                           double *__restrict__ partial) {
-  constexpr int K2 = 2 * KT;
-  double acc[3] = {0.0, 0.0, 0.0};
-  VK_GRID_LOOP(t, cells * KT) {
-    const uint64_t c = t / KT;
-    const int j = t % KT;
-    const uint64_t w = cell_worker[c], f = cell_firm[c];
-    const double pw = worker[w * K2 + 2 * j] + firm[f * K2 + 2 * j];
-    const double pf = worker[w * K2 + 2 * j + 1] + firm[f * K2 + 2 * j + 1];
-    const double wt = weight[c];
-    acc[0] += wt * pw * pw;
-    acc[1] += wt * pf * pf;
-    acc[2] += wt * pw * pf;
+struct Contract {
+  static constexpr int kThreads = kBlock;
+  uint64_t cells;
+  const uint32_t *__restrict__ cell_worker;
+  const uint32_t *__restrict__ cell_firm;
+  const double *__restrict__ weight;
+  const double *__restrict__ worker;
+  const double *__restrict__ firm;
+  double *__restrict__ partial;
+  __device__ void operator()(Grid g) const {
+    constexpr int K2 = 2 * KT;
+    double acc[3] = {0.0, 0.0, 0.0};
+    VK_GRID_STRIDE(g, t, cells * KT) {
+      const uint64_t c = t / KT;
+      const int j = t % KT;
+      const uint64_t w = cell_worker[c], f = cell_firm[c];
+      const double pw = worker[w * K2 + 2 * j] + firm[f * K2 + 2 * j];
+      const double pf = worker[w * K2 + 2 * j + 1] + firm[f * K2 + 2 * j + 1];
+      const double wt = weight[c];
+      acc[0] += wt * pw * pw;
+      acc[1] += wt * pf * pf;
+      acc[2] += wt * pw * pf;
+    }
+    block_partial<KT, 3>(acc, partial, g.block);
   }
-  block_partial<KT, 3>(acc, partial);
+};
+template <int KT>
+__global__ void k_contract(Contract<KT> f) {
+  f(Grid::hw());
 }
 struct FinDraws {
   __device__ bool enabled() const { return true; }
@@ -1601,13 +1718,37 @@ __global__ void k_lift(uint32_t firms, uint32_t aux, const uint64_t *__restrict_
 // Hybrid right-hand side in the renumbered order: firm entries from the
 // centred Schur right-hand side, auxiliary entries zero (full_cmg.rs:842-895).
 template <int K>
-__global__ void k_hybrid_rhs(uint64_t rows, uint32_t firms, const uint32_t *__restrict__ perm,
-                             const double *__restrict__ sch, double *__restrict__ b) {
-  VK_GRID_LOOP(t, rows * K) {
-    const uint32_t v = perm[t / K];
-    b[t] = v < firms ? sch[(uint64_t)v * K + t % K] : 0.0;
+struct HybridRhs {
+  static constexpr int kThreads = kBlock;
+  uint64_t rows;
+  uint32_t firms;
+  const uint32_t *__restrict__ perm;
+  const double *__restrict__ sch;
+  double *__restrict__ b;
+  __device__ void operator()(Grid g) const {
+    VK_GRID_STRIDE(g, t, rows * K) {
+      const uint32_t v = perm[t / K];
+      b[t] = v < firms ? sch[(uint64_t)v * K + t % K] : 0.0;
+    }
   }
+};
+template <int K>
+__global__ void k_hybrid_rhs(HybridRhs<K> f) {
+  f(Grid::hw());
 }
+
+// out = in over `count` doubles (a device copy that can join a multi-sample
+// launch).
+struct CopyDoubles {
+  static constexpr int kThreads = kBlock;
+  uint64_t count;
+  const double *__restrict__ in;
+  double *__restrict__ out;
+  __device__ void operator()(Grid g) const {
+    VK_GRID_STRIDE(g, t, count) out[t] = in[t];
+  }
+};
+static __global__ void k_copy_doubles(CopyDoubles f) { f(Grid::hw()); }
 
 }  // namespace dev
 }  // namespace vk
