@@ -304,10 +304,12 @@ struct OpDirectionCurvature : CsrOp<double, K, OpDirectionCurvature<K, Z>> {
     const uint64_t q = (uint64_t)i * K + j;
     const double zc = (double)z[q] - s.mean_z;
     const bool fresh = s.first || s.restarted;
-    const double pn = fresh ? zc : zc + s.beta * p[q];
-    const double apn = fresh ? az : az + s.beta * ap[q];
-    p[q] = pn;
-    ap[q] = apn;
+    // p and A p stream through once per iteration: non-temporal accesses
+    // leave the L2 to the neighbour gathers of z
+    const double pn = fresh ? zc : zc + s.beta * __builtin_nontemporal_load(&p[q]);
+    const double apn = fresh ? az : az + s.beta * __builtin_nontemporal_load(&ap[q]);
+    __builtin_nontemporal_store(pn, &p[q]);
+    __builtin_nontemporal_store(apn, &ap[q]);
     acc[0] += pn * apn;
   }
   template <int NV>
