@@ -350,11 +350,12 @@ struct PcgUpdate {
     const bool active = st[j].active != 0;
     const double alpha = st[j].alpha, mx = mean_x[j], mr = mean_r[j];
     VK_GRID_STRIDE(g, t, (uint64_t)n * K) {
-      double xv = x[t] - mx, rv = r[t] - mr;
+      // x, p and A p are next touched an iteration later: non-temporal
+      double xv = __builtin_nontemporal_load(&x[t]) - mx, rv = r[t] - mr;
       if (active) {
-        xv += alpha * p[t];
-        rv -= alpha * ap[t];
-        x[t] = xv;
+        xv += alpha * __builtin_nontemporal_load(&p[t]);
+        rv -= alpha * __builtin_nontemporal_load(&ap[t]);
+        __builtin_nontemporal_store(xv, &x[t]);
       }
       r[t] = rv;
       acc[0] += xv;
