@@ -614,7 +614,17 @@ struct OpProlongSmooth : CsrOp<T, K, OpProlongSmooth<T, K, Finest>> {
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ape, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
-    y[q] = x[q] + self(i, j) + inv[i] * (r[q] - ape);
+    if constexpr (Finest && std::is_same_v<T, float4>) {
+      // the finest x and r end their use here: non-temporal, so the cycle's
+      // output (gathered next by the Krylov passes) keeps the cache
+      typedef float v4 __attribute__((ext_vector_type(4)));
+      const v4 xn = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(x + q));
+      const v4 rn = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(r + q));
+      const T xv{xn.x, xn.y, xn.z, xn.w}, rv{rn.x, rn.y, rn.z, rn.w};
+      y[q] = xv + self(i, j) + inv[i] * (rv - ape);
+    } else {
+      y[q] = x[q] + self(i, j) + inv[i] * (r[q] - ape);
+    }
   }
 };
 
