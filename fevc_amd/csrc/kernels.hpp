@@ -563,7 +563,14 @@ struct OpFirstResidual : CsrOp<T, K, OpFirstResidual<T, K>> {
   template <int NA>
   __device__ void finish(uint32_t i, int j, T ax, double (&)[NA]) const {
     const uint64_t q = (uint64_t)i * K + j;
-    r[q] = braw[q] - ax;
+    if constexpr (std::is_same_v<T, float4>) {
+      // b is read for the last time in the cycle: non-temporal
+      typedef float v4 __attribute__((ext_vector_type(4)));
+      const v4 bn = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(braw + q));
+      r[q] = T{bn.x, bn.y, bn.z, bn.w} - ax;
+    } else {
+      r[q] = braw[q] - ax;
+    }
   }
 };
 
